@@ -1,0 +1,325 @@
+"""CPU oracle for semba-fdtd's field time-stepping path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package; the product in
+``semba_fdtd_amd`` never does.
+
+Two checkers live here:
+
+* ``Oracle`` -- ctypes wrapper of ``fdtd_oracle.c``, a plain-C restatement of
+  the reference's loops and coefficient tables (no FMA contraction).
+* ``Reference`` -- ctypes wrapper of ``_ref/libsemba_ref.so``: the reference's
+  own Fortran modules (calc_constants, borderscpml, electricdispersive,
+  nodalsources) compiled from their sources by ``oracle/Makefile``, driven by
+  ``ref_harness.F90``.  Only built where the reference tree is present.
+
+Both consume a ``semba_fdtd_amd.problem.Problem`` (plain numpy data: media
+tables, int16 media-index arrays in the Alloc layout, cell sizes, CPML
+settings, poles, nodal sources, probes).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle_fdtd.so")
+REF_SO = os.path.join(HERE, "_ref", "libsemba_ref.so")
+PMAX = 8
+
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_i16p = C.POINTER(C.c_int16)
+
+
+class OrcNodal(C.Structure):
+    _fields_ = [("comp", C.c_int32), ("hard", C.c_int32),
+                ("xi", C.c_int32), ("xe", C.c_int32), ("yi", C.c_int32),
+                ("ye", C.c_int32), ("zi", C.c_int32), ("ze", C.c_int32),
+                ("amp", C.c_float), ("deltaevol", C.c_float),
+                ("numus", C.c_int32), ("evol", _f32p)]
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("f", _f32p * 6), ("mi", _i16p * 6),
+                ("g1", _f32p), ("g2", _f32p), ("gm1", _f32p), ("gm2", _f32p),
+                ("nmedia", C.c_int32), ("id", _f32p * 6),
+                ("L", (C.c_int32 * 2) * 3),
+                ("pbe", _f32p * 3), ("pce", _f32p * 3), ("pbm", _f32p * 3),
+                ("pcm", _f32p * 3), ("psi", _f32p * 12),
+                ("ndisp", C.c_int32), ("disp_medium", _i32p),
+                ("disp_npoles", _i32p), ("disp_kappa", _f64p),
+                ("disp_beta", _f64p), ("disp_g3", _f64p),
+                ("eprev", _f32p * 3), ("jcur", _f64p * 3),
+                ("dt", C.c_double), ("tiempo", _f64p),
+                ("nnodal", C.c_int32), ("nodal", C.POINTER(OrcNodal)),
+                ("medkind", _i32p)]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def load_oracle():
+    if not os.path.exists(ORACLE_SO):
+        raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+    lib = C.CDLL(ORACLE_SO)
+    lib.orc_dt.restype = C.c_double
+    lib.orc_dt.argtypes = [C.c_float] * 5 + [C.c_float, C.c_int32, C.c_double]
+    lib.orc_g_tables.argtypes = [C.c_int32, _f32p, _f32p, _f32p, _f32p, _i32p,
+                                 C.c_double, C.c_float, C.c_float,
+                                 _f32p, _f32p, _f32p, _f32p]
+    lib.orc_spacing.argtypes = [C.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p]
+    lib.orc_cpml_axis.argtypes = ([C.c_int32] * 3 + [C.c_float] * 7 +
+                                  [C.c_double, C.c_float, C.c_float] + [_f32p] * 8)
+    lib.orc_ade_tables.argtypes = [C.c_int32, _f64p, _f64p, C.c_float, C.c_float,
+                                   C.c_double, _f64p, _f64p, _f64p, _f32p, _f32p]
+    lib.orc_time_vector.argtypes = [C.c_int32, C.c_double, _f64p]
+    lib.orc_evolucion.restype = C.c_float
+    lib.orc_evolucion.argtypes = [C.c_float, C.POINTER(OrcNodal)]
+    for fn in ("orc_yee_e", "orc_yee_h", "orc_cpml_e", "orc_cpml_h", "orc_ade_e"):
+        getattr(lib, fn).argtypes = [C.POINTER(OrcGrid)]
+    for fn in ("orc_nodal_e", "orc_nodal_h", "orc_step"):
+        getattr(lib, fn).argtypes = [C.POINTER(OrcGrid), C.c_int32]
+    lib.orc_run.argtypes = [C.POINTER(OrcGrid), C.c_int32, C.c_int32, C.c_int32,
+                            _i32p, _i32p, _i32p, _i32p, _f32p]
+    return lib
+
+
+class Tables:
+    """Coefficient tables of one problem (as the reference builds them)."""
+
+    def __init__(self, g1, g2, gm1, gm2, ide, idh, pbe, pce, pbm, pcm,
+                 kappa, beta, g3, disp_media, disp_npoles):
+        self.g1, self.g2, self.gm1, self.gm2 = g1, g2, gm1, gm2
+        self.ide, self.idh = ide, idh          # lists of 3 arrays (x,y,z)
+        self.pbe, self.pce, self.pbm, self.pcm = pbe, pce, pbm, pcm
+        self.kappa, self.beta, self.g3 = kappa, beta, g3
+        self.disp_media, self.disp_npoles = disp_media, disp_npoles
+
+
+def oracle_tables(lib, pb) -> Tables:
+    """Build every coefficient table with the oracle's restatement."""
+    nm = pb.nmedia
+    f = lambda n: np.zeros(n, np.float32)
+    g1, g2, gm1, gm2 = f(nm + 1), f(nm + 1), f(nm + 1), f(nm + 1)
+    lib.orc_g_tables(nm, _p(pb.epr, _f32p), _p(pb.sigma, _f32p), _p(pb.mur, _f32p),
+                     _p(pb.sigmam, _f32p), _p(pb.kind, _i32p), pb.dt, pb.eps0, pb.mu0,
+                     _p(g1, _f32p), _p(g2, _f32p), _p(gm1, _f32p), _p(gm2, _f32p))
+    ide, idh, pbe, pce, pbm, pcm = [], [], [], [], [], []
+    for a, n in enumerate((pb.nx, pb.ny, pb.nz)):
+        de, dh, Ie, Ih = f(n + 3), f(n + 3), f(n + 3), f(n + 3)
+        lib.orc_spacing(n, _p(pb.D[a], _f32p), _p(de, _f32p), _p(dh, _f32p),
+                        _p(Ie, _f32p), _p(Ih, _f32p))
+        b1, c1, b2, c2 = f(n + 3), f(n + 3), f(n + 3), f(n + 3)
+        L = pb.cpml_layers[a]
+        if L[0] or L[1]:
+            lib.orc_cpml_axis(n, int(L[0]), int(L[1]), float(pb.cpml_order[a][0]),
+                              float(pb.cpml_order[a][1]), float(pb.cpml_refl[a][0]),
+                              float(pb.cpml_refl[a][1]), pb.alphamax, pb.alphaorden,
+                              pb.kappamax, pb.dt, pb.eps0, pb.mu0,
+                              _p(de, _f32p), _p(dh, _f32p), _p(b1, _f32p), _p(c1, _f32p),
+                              _p(b2, _f32p), _p(c2, _f32p), _p(Ie, _f32p), _p(Ih, _f32p))
+        else:
+            b1[:] = 1.0
+            b2[:] = 1.0
+        ide.append(Ie); idh.append(Ih)
+        pbe.append(b1); pce.append(c1); pbm.append(b2); pcm.append(c2)
+    disp = [m for m in range(1, nm + 1) if pb.kind[m] & 4]
+    nd = len(disp)
+    kappa = np.zeros((max(nd, 1), PMAX, 2)); beta = np.zeros_like(kappa); g3 = np.zeros_like(kappa)
+    for d, m in enumerate(disp):
+        a1, a2 = np.zeros(1, np.float32), np.zeros(1, np.float32)
+        np_ = int(pb.npoles[m])
+        cc = np.ascontiguousarray(pb.pole_c[m]); aa = np.ascontiguousarray(pb.pole_a[m])
+        kk = np.zeros((PMAX, 2)); bb = np.zeros((PMAX, 2)); gg = np.zeros((PMAX, 2))
+        lib.orc_ade_tables(np_, _p(cc, _f64p), _p(aa, _f64p), float(pb.eps11[m]),
+                           float(pb.sig11[m]), pb.dt, _p(kk, _f64p), _p(bb, _f64p),
+                           _p(gg, _f64p), _p(a1, _f32p), _p(a2, _f32p))
+        kappa[d], beta[d], g3[d] = kk, bb, gg
+        g1[m], g2[m] = a1[0], a2[0]
+    return Tables(g1, g2, gm1, gm2, ide, idh, pbe, pce, pbm, pcm, kappa, beta, g3,
+                  np.array(disp, np.int32), np.array([pb.npoles[m] for m in disp], np.int32))
+
+
+class Oracle:
+    """Runs a Problem through the C restatement."""
+
+    def __init__(self, pb, tables: Tables | None = None):
+        self.lib = load_oracle()
+        self.pb = pb
+        self.t = tables if tables is not None else oracle_tables(self.lib, pb)
+        t = self.t
+        shape = pb.alloc_shape
+        self.fields = np.zeros((6,) + shape, np.float32)
+        self.mi = np.ascontiguousarray(pb.mi, dtype=np.int16)
+        self.psi = np.zeros((12,) + shape, np.float32) if pb.has_cpml else None
+        nd = len(t.disp_media)
+        self.eprev = np.zeros((3,) + shape, np.float32) if nd else None
+        self.jcur = np.zeros((3,) + shape + (PMAX, 2)) if nd else None
+        self.tiempo = np.zeros(pb.nsteps_max + 3)
+        self.lib.orc_time_vector(pb.nsteps_max + 2, pb.dt, _p(self.tiempo, _f64p))
+        self.medkind = np.ascontiguousarray(pb.kind, np.int32)
+        self._keep = []
+        g = OrcGrid()
+        g.nx, g.ny, g.nz = pb.nx, pb.ny, pb.nz
+        for c in range(6):
+            g.f[c] = self.fields[c].ctypes.data_as(_f32p)
+            g.mi[c] = self.mi[c].ctypes.data_as(_i16p)
+        g.g1, g.g2, g.gm1, g.gm2 = (_p(t.g1, _f32p), _p(t.g2, _f32p), _p(t.gm1, _f32p),
+                                    _p(t.gm2, _f32p))
+        g.nmedia = pb.nmedia
+        ids = t.ide + t.idh
+        for c in range(6):
+            g.id[c] = ids[c].ctypes.data_as(_f32p)
+        for a in range(3):
+            for s in range(2):
+                g.L[a][s] = int(pb.cpml_layers[a][s])
+            g.pbe[a] = _p(t.pbe[a], _f32p); g.pce[a] = _p(t.pce[a], _f32p)
+            g.pbm[a] = _p(t.pbm[a], _f32p); g.pcm[a] = _p(t.pcm[a], _f32p)
+        if self.psi is not None:
+            for s in range(12):
+                g.psi[s] = self.psi[s].ctypes.data_as(_f32p)
+        g.ndisp = nd
+        if nd:
+            self._dm = np.ascontiguousarray(t.disp_media, np.int32)
+            self._dn = np.ascontiguousarray(t.disp_npoles, np.int32)
+            self._k = np.ascontiguousarray(t.kappa); self._b = np.ascontiguousarray(t.beta)
+            self._g = np.ascontiguousarray(t.g3)
+            g.disp_medium = _p(self._dm, _i32p); g.disp_npoles = _p(self._dn, _i32p)
+            g.disp_kappa = _p(self._k, _f64p); g.disp_beta = _p(self._b, _f64p)
+            g.disp_g3 = _p(self._g, _f64p)
+            for c in range(3):
+                g.eprev[c] = self.eprev[c].ctypes.data_as(_f32p)
+                g.jcur[c] = self.jcur[c].ctypes.data_as(_f64p)
+        g.dt = pb.dt
+        g.tiempo = _p(self.tiempo, _f64p)
+        ns = len(pb.nodal)
+        self._nod = (OrcNodal * max(ns, 1))()
+        for s, src in enumerate(pb.nodal):
+            ev = np.ascontiguousarray(src.evol, np.float32)
+            self._keep.append(ev)
+            o = self._nod[s]
+            o.comp, o.hard = src.comp, int(src.hard)
+            o.xi, o.xe, o.yi, o.ye, o.zi, o.ze = src.box
+            o.amp, o.deltaevol, o.numus = src.amp, src.deltaevol, len(ev) - 1
+            o.evol = _p(ev, _f32p)
+        g.nnodal = ns
+        g.nodal = C.cast(self._nod, C.POINTER(OrcNodal))
+        g.medkind = _p(self.medkind, _i32p)
+        self.g = g
+        self.n = 0
+
+    def step(self):
+        self.lib.orc_step(C.byref(self.g), self.n)
+        self.n += 1
+
+    def run(self, nsteps, probes=None):
+        probes = self.pb.probes if probes is None else probes
+        npb = len(probes)
+        pc = np.array([p[0] for p in probes] or [0], np.int32)
+        pi = np.array([p[1] for p in probes] or [0], np.int32)
+        pj = np.array([p[2] for p in probes] or [0], np.int32)
+        pk = np.array([p[3] for p in probes] or [0], np.int32)
+        out = np.zeros((nsteps, max(npb, 1)), np.float32)
+        self.lib.orc_run(C.byref(self.g), self.n, nsteps, npb, _p(pc, _i32p), _p(pi, _i32p),
+                         _p(pj, _i32p), _p(pk, _i32p), _p(out, _f32p))
+        self.n += nsteps
+        return out[:, :npb]
+
+
+def load_reference():
+    if not os.path.exists(REF_SO):
+        return None
+    lib = C.CDLL(REF_SO)
+    lib.ref_init.argtypes = ([C.c_int32] * 4 + [_f32p] * 4 + [_i32p, _i32p, _f64p, _f64p,
+                             _f32p, _f32p, _i32p, _f32p, _f32p] + [C.c_float] * 3 +
+                             [C.c_double, _f32p, _f32p, _f32p, _i16p, C.c_float, C.c_float])
+    lib.ref_nodal.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _f32p, _f32p, _i32p, _f32p,
+                              C.c_int32]
+    lib.ref_step.argtypes = [C.c_int32]
+    lib.ref_get_field.argtypes = [C.c_int32, _f32p]
+    lib.ref_get_tables.argtypes = [_f32p] * 10
+    return lib
+
+
+class Reference:
+    """Runs a Problem through the reference's own Fortran modules.
+
+    The Fortran modules keep their state in module variables, so one
+    Reference can exist per process; tests run it in a subprocess."""
+
+    def __init__(self, pb):
+        lib = load_reference()
+        if lib is None:
+            raise RuntimeError("reference harness not built (make -C oracle ref)")
+        self.lib, self.pb = lib, pb
+        nm = pb.nmedia
+        pc = np.zeros((nm + 1, PMAX, 2)); pa = np.zeros((nm + 1, PMAX, 2))
+        for m in range(nm + 1):
+            if pb.kind[m] & 4:
+                pc[m, :pb.npoles[m]] = pb.pole_c[m][:pb.npoles[m]]
+                pa[m, :pb.npoles[m]] = pb.pole_a[m][:pb.npoles[m]]
+        lay = np.ascontiguousarray(np.array(pb.cpml_layers, np.int32))
+        orden = np.ascontiguousarray(np.array(pb.cpml_order, np.float32))
+        refl = np.ascontiguousarray(np.array(pb.cpml_refl, np.float32))
+        self._keep = [pc, pa, lay, orden, refl]
+        mi = np.ascontiguousarray(pb.mi, np.int16)
+        self._keep.append(mi)
+        kind = np.ascontiguousarray(pb.kind, np.int32)
+        npol = np.ascontiguousarray(pb.npoles, np.int32)
+        self._keep += [kind, npol]
+        lib.ref_init(pb.nx, pb.ny, pb.nz, nm, _p(pb.epr, _f32p), _p(pb.sigma, _f32p),
+                     _p(pb.mur, _f32p), _p(pb.sigmam, _f32p), _p(kind, _i32p),
+                     _p(npol, _i32p), _p(pc, _f64p), _p(pa, _f64p),
+                     _p(pb.eps11, _f32p), _p(pb.sig11, _f32p), _p(lay, _i32p),
+                     _p(orden, _f32p), _p(refl, _f32p), pb.alphamax, pb.alphaorden,
+                     pb.kappamax, pb.dt, _p(pb.D[0], _f32p), _p(pb.D[1], _f32p),
+                     _p(pb.D[2], _f32p), _p(mi, _i16p), pb.eps0, pb.mu0)
+        ns = len(pb.nodal)
+        if ns:
+            comp = np.array([s.comp for s in pb.nodal], np.int32)
+            hard = np.array([int(s.hard) for s in pb.nodal], np.int32)
+            box = np.ascontiguousarray(np.array([s.box for s in pb.nodal], np.int32))
+            amp = np.array([s.amp for s in pb.nodal], np.float32)
+            de = np.array([s.deltaevol for s in pb.nodal], np.float32)
+            nu = np.array([len(s.evol) - 1 for s in pb.nodal], np.int32)
+            smp = np.ascontiguousarray(np.concatenate([np.asarray(s.evol, np.float32)
+                                                       for s in pb.nodal]))
+            self._keep += [comp, hard, box, amp, de, nu, smp]
+            lib.ref_nodal(ns, _p(comp, _i32p), _p(hard, _i32p), _p(box, _i32p), _p(amp, _f32p),
+                          _p(de, _f32p), _p(nu, _i32p), _p(smp, _f32p), pb.nsteps_max)
+        self.n = 0
+
+    def tables(self):
+        pb = self.pb
+        nm = pb.nmedia
+        f = lambda n: np.zeros(n, np.float32)
+        out = [f(nm + 1) for _ in range(4)] + [f(pb.nx + 3), f(pb.ny + 3), f(pb.nz + 3)] * 2
+        self.lib.ref_get_tables(*[_p(a, _f32p) for a in out])
+        return out
+
+    def step(self):
+        self.lib.ref_step(self.n)
+        self.n += 1
+
+    def field(self, c):
+        out = np.zeros(self.pb.alloc_shape, np.float32)
+        self.lib.ref_get_field(c, _p(out, _f32p))
+        return out
+
+    def run(self, nsteps, probes=None):
+        probes = self.pb.probes if probes is None else probes
+        out = np.zeros((nsteps, len(probes)), np.float32)
+        cache = {}
+        for s in range(nsteps):
+            self.step()
+            cache.clear()
+            for p, (c, i, j, k) in enumerate(probes):
+                if c not in cache:
+                    cache[c] = self.field(c)
+                out[s, p] = cache[c][k + 1, j + 1, i + 1]
+        return out

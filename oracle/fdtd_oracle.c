@@ -1,0 +1,572 @@
+/*
+ * fdtd_oracle.c -- CPU restatement of semba-fdtd's field time-stepping path
+ * (Yee E/H curl updates, CPML, electric pole-residue ADE, nodal sources,
+ * coefficient tables).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  Build it
+ * with -ffp-contract=off and no fast-math so that every expression is
+ * evaluated in the reference's order with one rounding per operation; the
+ * reference's default x86-64 build has no FMA either.
+ *
+ * Every function names the reference lines it restates (paths relative to the
+ * reference root).  Precision follows the reference kinds: fields and tables
+ * real(4) (-DCompileWithReal4, fdetypes.F90:140-162), time and dt real(8)
+ * (RKIND_tiempo), dispersive currents complex(8) (CKIND).
+ *
+ * Parity pins (see tests/): tables and a full step loop are compared
+ * bit-for-bit against the reference's own Fortran modules built from source
+ * (oracle/_ref), and against closed forms (cavity eigenfrequencies, Fresnel /
+ * CPML reflection, discrete Debye/Lorentz permittivity).
+ */
+#include "fdtd_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define AX(g) ((size_t)(g)->nx + 3)
+#define AY(g) ((size_t)(g)->ny + 3)
+#define IDX(g, i, j, k) \
+  ((((size_t)((k) + 1)) * AY(g) + (size_t)((j) + 1)) * AX(g) + (size_t)((i) + 1))
+
+/* media kind bits: PEC, PMC, electric dispersive */
+#define ORC_KIND_PEC 1
+#define ORC_KIND_PMC 2
+#define ORC_KIND_EDISP 4
+
+/* --------------------------------------------------------------------------
+ * dt: semba_fdtd.F90:1211-1250.  dtlay is formed in real(4) from the minimum
+ * cell sizes; with -cfl the step is dtlay*cfl (real(4) product stored in the
+ * real(8) sgg%dt), otherwise the input dt is clipped to heurCFL*dtlay
+ * (heurCFL = 0.8, fdetypes.F90:186).  cluz = 1/sqrt(eps0*mu0) in real(4)
+ * (semba_fdtd.F90:154-156).
+ * ------------------------------------------------------------------------ */
+double orc_dt(float dxmin, float dymin, float dzmin, float eps0, float mu0,
+              float cfl, int32_t forcecfl, double dt_in) {
+  float cluz = 1.0f / sqrtf(eps0 * mu0);
+  float ix = 1.0f / dxmin, iy = 1.0f / dymin, iz = 1.0f / dzmin;
+  float dtlay = 1.0f / (cluz * sqrtf(ix * ix + iy * iy + iz * iz));
+  float dt = dtlay;
+  if (forcecfl) {
+    float p = dt * cfl;
+    return (double)p;
+  }
+  if (dt_in > (double)(dt * 0.8f)) return (double)(dt * 0.8f);
+  return dt_in;
+}
+
+/* --------------------------------------------------------------------------
+ * Per-medium update coefficients: calc_constants.F90:36-148.
+ * kind: 0 lossy dielectric (:132-144), ORC_KIND_PEC (or r == 0) all zero
+ * (:62-64), ORC_KIND_EDISP: G1/G2 placeholders 0 (overwritten by the ADE
+ * init), GM from mu/sigmam (:109-118).  Expressions mix real(4) with the
+ * real(8) dt exactly as written there.
+ * ------------------------------------------------------------------------ */
+static void orc_lossy(double dt, float sig, float eps, float *g1, float *g2) {
+  /* G1=(1 - S*dt/(2*eps))/(1 + S*dt/(2*eps)); G2=dt/eps/(1 + S*dt/(2*eps)) */
+  double a = (double)sig * dt / (double)(2.0f * eps);
+  double x1 = (1.0 - a) / (1.0 + a);
+  double x2 = dt / (double)eps / (1.0 + a);
+  *g1 = (float)x1;
+  *g2 = (float)x2;
+  if (*g1 < 0.0f) { /* exponential time stepping, :135-138 */
+    *g1 = (float)exp(-(double)sig * dt / (double)eps);
+    *g2 = (1.0f - *g1) / sig;
+  }
+}
+
+void orc_g_tables(int32_t nmedia, const float *epr, const float *sigma,
+                  const float *mur, const float *sigmam, const int32_t *kind,
+                  double dt, float eps0, float mu0,
+                  float *g1, float *g2, float *gm1, float *gm2) {
+  for (int32_t r = 0; r <= nmedia; ++r) {
+    float eps = eps0 * epr[r];
+    float mu = mu0 * mur[r];
+    if ((kind[r] & ORC_KIND_PEC) || r == 0) {
+      g1[r] = g2[r] = gm1[r] = gm2[r] = 0.0f;
+    } else if (kind[r] & ORC_KIND_EDISP) {
+      g1[r] = 0.0f;
+      g2[r] = 0.0f;
+      orc_lossy(dt, sigmam[r], mu, &gm1[r], &gm2[r]);
+    } else {
+      orc_lossy(dt, sigma[r], eps, &g1[r], &g2[r]);
+      orc_lossy(dt, sigmam[r], mu, &gm1[r], &gm2[r]);
+    }
+  }
+}
+
+/* --------------------------------------------------------------------------
+ * Spacings: timestepping.F90:314-370.  dxe(i) = DX(i) (primal, H side),
+ * dxh(i) = (DX(i) + DX(i-1))/2 (dual, E side), inverses in real(4), over the
+ * Alloc range [-1, n+1].  DX is indexed [-2, n+1] (DX[i+2]).
+ * ------------------------------------------------------------------------ */
+void orc_spacing(int32_t n, const float *DX, float *de, float *dh, float *Ide,
+                 float *Idh) {
+  for (int32_t i = -1; i <= n + 1; ++i) {
+    de[i + 1] = DX[i + 2];
+    dh[i + 1] = (DX[i + 2] + DX[i + 1]) / 2.0f;
+    Ide[i + 1] = 1.0f / de[i + 1];
+    Idh[i + 1] = 1.0f / dh[i + 1];
+  }
+}
+
+/* --------------------------------------------------------------------------
+ * CPML profiles along one axis.
+ * Depths: borderscpml.F90:228-300 (integer depth ce for E, half-cell cm for
+ * H).  sigma_max / alpha / kappa: calc_cpmlconstants, borderscpml.F90:
+ * 1465-1733, including the 5/10-layer Gedney optimum (:1494-1497) and the
+ * absent-side value 1e29 (:1504).  b = exp(-(s/k + a)*dt/eps0) in real(8)
+ * stored real(4); c = s*(b-1)/(s + k*a)/k / delta; Id = 1/(k*delta).
+ * E uses i <= SIN_lo-1 / i >= SIN_hi+1 (:1545,1557); H uses i <= SIN_lo-1 /
+ * i >= SIN_hi (:1627,1643), with SIN_lo = Llo, SIN_hi = n - Lhi in total
+ * coordinates.  Arrays have length n+3 (index i+1).
+ * ------------------------------------------------------------------------ */
+static float orc_sigmax(int32_t L, float orden, float R, float del, float eps0,
+                        float mu0) {
+  if (L == 0) return 1.0e29f;
+  if (L == 10 || L == 5)
+    return 0.8f * (orden + 1.0f) / (sqrtf(mu0 / eps0) * del);
+  return -((logf(R) * (orden + 1.0f)) /
+           (2.0f * sqrtf(mu0 / eps0) * (float)L * del));
+}
+
+void orc_cpml_axis(int32_t n, int32_t Llo, int32_t Lhi, float orden_lo,
+                   float orden_hi, float R_lo, float R_hi, float alphamaxpar,
+                   float alphaorden, float kappamaxpar, double dt, float eps0,
+                   float mu0, const float *de, const float *dh, float *pbe,
+                   float *pce, float *pbm, float *pcm, float *Ide, float *Idh) {
+  const int32_t SI = Llo, SE = n - Lhi;
+  /* del: dxe at Alloc(iEx) XI / XE = -1 / n (:1480-1485) */
+  float smax[2], amax[2], kmax[2];
+  float dlo = de[-1 + 1], dhi = de[n + 1];
+  smax[0] = orc_sigmax(Llo, orden_lo, R_lo, dlo, eps0, mu0);
+  smax[1] = orc_sigmax(Lhi, orden_hi, R_hi, dhi, eps0, mu0);
+  amax[0] = Llo ? alphamaxpar * smax[0] : 0.0f;
+  amax[1] = Lhi ? alphamaxpar * smax[1] : 0.0f;
+  kmax[0] = Llo ? kappamaxpar : 1.0f;
+  kmax[1] = Lhi ? kappamaxpar : 1.0f;
+  const float ord[2] = {orden_lo, orden_hi};
+  const int32_t Ls[2] = {Llo, Lhi};
+
+  for (int32_t i = -1; i <= n + 1; ++i) {
+    pbe[i + 1] = 1.0f; pce[i + 1] = 0.0f;
+    pbm[i + 1] = 1.0f; pcm[i + 1] = 0.0f;
+  }
+  /* electric: i over Alloc(iEx) = [-1, n] */
+  for (int32_t i = -1; i <= n; ++i) {
+    int side = -1;
+    float ce = 0.0f, Ice = 0.0f;
+    if (i <= SI - 1) {
+      side = 0;
+      ce = 1.0f * (float)(SI - i) / (float)Ls[0];
+      Ice = 1.0f * (float)(Ls[0] - (SI - i)) / (float)Ls[0];
+    } else if (i >= SE + 1) {
+      side = 1;
+      ce = 1.0f * (float)(i - SE) / (float)Ls[1];
+      Ice = 1.0f * (float)(Ls[1] - (i - SE)) / (float)Ls[1];
+    }
+    if (side < 0) continue;
+    float s, k;
+    if (ord[side] == 0.0f) {
+      s = smax[side];
+      k = 1.0f + (kmax[side] - 1.0f);
+    } else {
+      s = smax[side] * powf(ce, ord[side]);
+      k = 1.0f + (kmax[side] - 1.0f) * powf(ce, ord[side]);
+    }
+    float a = amax[side] * powf(Ice, alphaorden);
+    float b = (float)exp((double)(-(s / k + a)) * dt / (double)eps0);
+    pbe[i + 1] = b;
+    pce[i + 1] = (s * (b - 1.0f) / (s + k * a) / k) / dh[i + 1];
+    Idh[i + 1] = 1.0f / (k * dh[i + 1]);
+  }
+  /* magnetic: i over Alloc(iHx) = [-1, n+1] */
+  for (int32_t i = -1; i <= n + 1; ++i) {
+    int side = -1;
+    float cm = 0.0f, Icm = 0.0f;
+    if (i <= SI - 1) {
+      side = 0;
+      cm = 1.0f * ((float)SI - ((float)i + 0.5f)) / (float)Ls[0];
+      Icm = 1.0f * ((float)Ls[0] - ((float)SI - ((float)i + 0.5f))) / (float)Ls[0];
+    } else if (i >= SE) {
+      side = 1;
+      cm = 1.0f * ((float)(i - SE) + 0.5f) / (float)Ls[1];
+      Icm = 1.0f * ((float)Ls[1] - ((float)(i - SE) + 0.5f)) / (float)Ls[1];
+    }
+    if (side < 0) continue;
+    float s, k;
+    if (ord[side] == 0.0f) {
+      s = smax[side];
+      k = 1.0f + (kmax[side] - 1.0f);
+    } else {
+      s = smax[side] * powf(cm, ord[side]);
+      k = 1.0f + (kmax[side] - 1.0f) * powf(cm, ord[side]);
+    }
+    float a = amax[side] * powf(Icm, alphaorden);
+    float b = (float)exp((double)(-(s / k + a)) * dt / (double)eps0);
+    pbm[i + 1] = b;
+    pcm[i + 1] = (s * (b - 1.0f) / (s + k * a) / k) / de[i + 1];
+    Ide[i + 1] = 1.0f / (k * de[i + 1]);
+  }
+}
+
+/* --------------------------------------------------------------------------
+ * Pole-residue ADE coefficients: electricdispersive.F90:98-146.
+ * kappa_p = (1 + a_p dt/2)/(1 - a_p dt/2), beta_p = c_p dt/(1 - a_p dt/2)
+ * (complex(8)); tempo = sum Re(beta_p) accumulated in real(4);
+ * G1 = (2 eps + tempo - s dt)/(2 eps + tempo + s dt),
+ * G2 = 2 dt/(2 eps + tempo + s dt)   (real(8) expressions stored real(4));
+ * G3_p = G2/2 * (1 + kappa_p) with G2/2 in real(4).
+ * Complex division uses Smith's range-reduced form (gfortran's
+ * -fcx-fortran-rules lowering).  Arrays are [p][2] (re, im).
+ * ------------------------------------------------------------------------ */
+static void cdiv(double a, double b, double c, double d, double *re,
+                 double *im) {
+  if (fabs(c) >= fabs(d)) {
+    double r = d / c, den = c + d * r;
+    *re = (a + b * r) / den;
+    *im = (b - a * r) / den;
+  } else {
+    double r = c / d, den = c * r + d;
+    *re = (a * r + b) / den;
+    *im = (b * r - a) / den;
+  }
+}
+
+void orc_ade_tables(int32_t npoles, const double *c, const double *a,
+                    float eps11, float sigma11, double dt, double *kappa,
+                    double *beta, double *g3, float *g1_out, float *g2_out) {
+  for (int32_t p = 0; p < npoles; ++p) {
+    double ar = a[2 * p], ai = a[2 * p + 1];
+    /* a*dt/2 : complex * real(8), then / real(4) 2.0 */
+    double hr = ar * dt / 2.0, hi = ai * dt / 2.0;
+    double nr = 1.0 + hr, ni = hi;
+    double dr = 1.0 - hr, di = -hi;
+    cdiv(nr, ni, dr, di, &kappa[2 * p], &kappa[2 * p + 1]);
+    double cr = c[2 * p] * dt, ci = c[2 * p + 1] * dt;
+    cdiv(cr, ci, dr, di, &beta[2 * p], &beta[2 * p + 1]);
+  }
+  float tempo = 0.0f;
+  for (int32_t p = 0; p < npoles; ++p) tempo = (float)((double)tempo + beta[2 * p]);
+  double sdt = (double)sigma11 * dt;
+  float e2 = 2.0f * eps11 + tempo;
+  float g1 = (float)(((double)e2 - sdt) / ((double)e2 + sdt));
+  float g2 = (float)(2.0 * dt / ((double)e2 + sdt));
+  *g1_out = g1;
+  *g2_out = g2;
+  float h = g2 / 2.0f;
+  for (int32_t p = 0; p < npoles; ++p) {
+    double kr = 1.0 + kappa[2 * p], ki = kappa[2 * p + 1];
+    g3[2 * p] = (double)h * kr;
+    g3[2 * p + 1] = (double)h * ki;
+  }
+}
+
+/* crea_timevector, timestepping.F90:3230-3240: tiempo(0)=0, tiempo(i) =
+ * tiempo(i-1)+dt accumulated in real(8). */
+void orc_time_vector(int32_t nlast, double dt, double *tiempo) {
+  tiempo[0] = 0.0;
+  for (int32_t i = 1; i <= nlast; ++i) tiempo[i] = tiempo[i - 1] + dt;
+}
+
+/* --------------------------------------------------------------------------
+ * Yee E update: Advance_Ex/Ey/Ez, timestepping.F90:2165-2285.
+ * Sweeps (total coordinates): Ex x[0,nx-1] y[0,ny] z[0,nz]; Ey x[0,nx]
+ * y[0,ny-1] z[0,nz]; Ez x[0,nx] y[0,ny] z[0,nz-1].
+ * ------------------------------------------------------------------------ */
+void orc_yee_e(orc_grid *g) {
+  float *Ex = g->f[0], *Ey = g->f[1], *Ez = g->f[2];
+  const float *Hx = g->f[3], *Hy = g->f[4], *Hz = g->f[5];
+  const float *Idxh = g->id[3], *Idyh = g->id[4], *Idzh = g->id[5];
+  const size_t sx = 1, sy = AX(g), sz = AX(g) * AY(g);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k <= g->nz; ++k)
+    for (int j = 0; j <= g->ny; ++j)
+      for (int i = 0; i <= g->nx - 1; ++i) {
+        size_t p = IDX(g, i, j, k);
+        int m = g->mi[0][p];
+        Ex[p] = g->g1[m] * Ex[p] + g->g2[m] * ((Hz[p] - Hz[p - sy]) * Idyh[j + 1] -
+                                              (Hy[p] - Hy[p - sz]) * Idzh[k + 1]);
+      }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k <= g->nz; ++k)
+    for (int j = 0; j <= g->ny - 1; ++j)
+      for (int i = 0; i <= g->nx; ++i) {
+        size_t p = IDX(g, i, j, k);
+        int m = g->mi[1][p];
+        Ey[p] = g->g1[m] * Ey[p] + g->g2[m] * ((Hx[p] - Hx[p - sz]) * Idzh[k + 1] -
+                                              (Hz[p] - Hz[p - sx]) * Idxh[i + 1]);
+      }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k <= g->nz - 1; ++k)
+    for (int j = 0; j <= g->ny; ++j)
+      for (int i = 0; i <= g->nx; ++i) {
+        size_t p = IDX(g, i, j, k);
+        int m = g->mi[2][p];
+        Ez[p] = g->g1[m] * Ez[p] + g->g2[m] * ((Hy[p] - Hy[p - sx]) * Idxh[i + 1] -
+                                              (Hx[p] - Hx[p - sy]) * Idyh[j + 1]);
+      }
+}
+
+/* Yee H update: Advance_Hx/Hy/Hz, timestepping.F90:2294-2409.
+ * Sweeps: Hx x[0,nx] y[0,ny-1] z[0,nz-1]; Hy x[0,nx-1] y[0,ny] z[0,nz-1];
+ * Hz x[0,nx-1] y[0,ny-1] z[0,nz]. */
+void orc_yee_h(orc_grid *g) {
+  const float *Ex = g->f[0], *Ey = g->f[1], *Ez = g->f[2];
+  float *Hx = g->f[3], *Hy = g->f[4], *Hz = g->f[5];
+  const float *Idxe = g->id[0], *Idye = g->id[1], *Idze = g->id[2];
+  const size_t sx = 1, sy = AX(g), sz = AX(g) * AY(g);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k <= g->nz - 1; ++k)
+    for (int j = 0; j <= g->ny - 1; ++j)
+      for (int i = 0; i <= g->nx; ++i) {
+        size_t p = IDX(g, i, j, k);
+        int m = g->mi[3][p];
+        Hx[p] = g->gm1[m] * Hx[p] + g->gm2[m] * ((Ey[p + sz] - Ey[p]) * Idze[k + 1] -
+                                                (Ez[p + sy] - Ez[p]) * Idye[j + 1]);
+      }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k <= g->nz - 1; ++k)
+    for (int j = 0; j <= g->ny; ++j)
+      for (int i = 0; i <= g->nx - 1; ++i) {
+        size_t p = IDX(g, i, j, k);
+        int m = g->mi[4][p];
+        Hy[p] = g->gm1[m] * Hy[p] + g->gm2[m] * ((Ez[p + sx] - Ez[p]) * Idxe[i + 1] -
+                                                (Ex[p + sz] - Ex[p]) * Idze[k + 1]);
+      }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k <= g->nz; ++k)
+    for (int j = 0; j <= g->ny - 1; ++j)
+      for (int i = 0; i <= g->nx - 1; ++i) {
+        size_t p = IDX(g, i, j, k);
+        int m = g->mi[5][p];
+        Hz[p] = g->gm1[m] * Hz[p] + g->gm2[m] * ((Ex[p + sy] - Ex[p]) * Idye[j + 1] -
+                                                (Ey[p + sx] - Ey[p]) * Idxe[i + 1]);
+      }
+}
+
+/* --------------------------------------------------------------------------
+ * CPML corrections.  Region boxes PMLc: borderscpml.F90:149-193 (each slab
+ * spans the full sweep of the other two axes, so edges/corners overlap);
+ * updates AdvanceelectricCPML :609-871 / AdvanceMagneticCPML :907-1171.
+ * Region order left,right (y) ; down,up (z) ; back,front (x).
+ * ------------------------------------------------------------------------ */
+typedef struct { int lo[3], hi[3]; } box3;
+
+/* sweep of component c (0..5) along axis a */
+static int sweep_hi(const orc_grid *g, int c, int a) {
+  int n = a == 0 ? g->nx : (a == 1 ? g->ny : g->nz);
+  int node = (c < 3) ? (c != a) : (c - 3 == a);
+  return node ? n : n - 1;
+}
+
+/* slab box of component c on axis a, side s (0 low, 1 high); 0 if empty */
+static int slab_box(const orc_grid *g, int c, int a, int s, box3 *b) {
+  if (g->L[a][s] == 0) return 0;
+  for (int d = 0; d < 3; ++d) { b->lo[d] = 0; b->hi[d] = sweep_hi(g, c, d); }
+  int n = a == 0 ? g->nx : (a == 1 ? g->ny : g->nz);
+  int node = (c < 3) ? (c != a) : (c - 3 == a);
+  int sin_hi = node ? n - g->L[a][1] : n - g->L[a][1] - 1; /* SINPML end */
+  if (s == 0) {
+    int e = g->L[a][0] - 1; /* SINPML start - 1 */
+    if (e < b->hi[a]) b->hi[a] = e;
+  } else {
+    int st = sin_hi + 1;
+    if (st > b->lo[a]) b->lo[a] = st;
+  }
+  return b->lo[a] <= b->hi[a];
+}
+
+/* one E (or H) psi loop over a slab; sign applies to the field correction.
+ * tgt: field updated; src: field differenced along axis a; forward: H-side
+ * difference src(p+s)-src(p), otherwise src(p)-src(p-s). */
+static void cpml_loop(orc_grid *g, int c, int a, int side, int psislot,
+                      int src, int forward, float sign) {
+  box3 b;
+  if (!slab_box(g, c, a, side, &b)) return;
+  float *F = g->f[c];
+  const float *S = g->f[src];
+  float *psi = g->psi[psislot];
+  const int16_t *mi = g->mi[c];
+  const float *gg = (c < 3) ? g->g2 : g->gm2;
+  const float *pb = (c < 3) ? g->pbe[a] : g->pbm[a];
+  const float *pc = (c < 3) ? g->pce[a] : g->pcm[a];
+  const size_t st[3] = {1, AX(g), AX(g) * AY(g)};
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+    for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+      for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
+        size_t p = IDX(g, i, j, k);
+        int q = a == 0 ? i : (a == 1 ? j : k);
+        float d = forward ? (S[p + st[a]] - S[p]) : (S[p] - S[p - st[a]]);
+        psi[p] = pb[q + 1] * psi[p] + d * pc[q + 1];
+        int m = mi[p];
+        if (sign > 0)
+          F[p] = F[p] + gg[m] * psi[p];
+        else
+          F[p] = F[p] - gg[m] * psi[p];
+      }
+}
+
+void orc_cpml_e(orc_grid *g) {
+  for (int s = 0; s < 2; ++s) { /* left, right */
+    cpml_loop(g, ORC_EX, 1, s, ORC_PSI_EXY, ORC_HZ, 0, +1.0f);
+    cpml_loop(g, ORC_EZ, 1, s, ORC_PSI_EZY, ORC_HX, 0, -1.0f);
+  }
+  for (int s = 0; s < 2; ++s) { /* down, up */
+    cpml_loop(g, ORC_EY, 2, s, ORC_PSI_EYZ, ORC_HX, 0, +1.0f);
+    cpml_loop(g, ORC_EX, 2, s, ORC_PSI_EXZ, ORC_HY, 0, -1.0f);
+  }
+  for (int s = 0; s < 2; ++s) { /* back, front */
+    cpml_loop(g, ORC_EZ, 0, s, ORC_PSI_EZX, ORC_HY, 0, +1.0f);
+    cpml_loop(g, ORC_EY, 0, s, ORC_PSI_EYX, ORC_HZ, 0, -1.0f);
+  }
+}
+
+void orc_cpml_h(orc_grid *g) {
+  for (int s = 0; s < 2; ++s) { /* left, right */
+    cpml_loop(g, ORC_HX, 1, s, ORC_PSI_HXY, ORC_EZ, 1, -1.0f);
+    cpml_loop(g, ORC_HZ, 1, s, ORC_PSI_HZY, ORC_EX, 1, +1.0f);
+  }
+  for (int s = 0; s < 2; ++s) { /* down, up */
+    cpml_loop(g, ORC_HY, 2, s, ORC_PSI_HYZ, ORC_EX, 1, -1.0f);
+    cpml_loop(g, ORC_HX, 2, s, ORC_PSI_HXZ, ORC_EY, 1, +1.0f);
+  }
+  for (int s = 0; s < 2; ++s) { /* back, front */
+    cpml_loop(g, ORC_HZ, 0, s, ORC_PSI_HZX, ORC_EY, 1, -1.0f);
+    cpml_loop(g, ORC_HY, 0, s, ORC_PSI_HYX, ORC_EZ, 1, +1.0f);
+  }
+}
+
+/* --------------------------------------------------------------------------
+ * ADE: AdvanceEDispersiveE, electricdispersive.F90:304-362.  Per dispersive
+ * edge: E <- E - Re(G3_p J_p) one pole at a time (real(8) product, rounded to
+ * real(4) after each pole); then J_p <- kappa_p J_p + beta_p (E - Eprev)/dt
+ * with (E - Eprev) in real(4); then Eprev <- E.  Edges are those of the E
+ * sweeps whose media index equals the dispersive medium (:148-244).
+ * ------------------------------------------------------------------------ */
+void orc_ade_e(orc_grid *g) {
+  for (int d = 0; d < g->ndisp; ++d) {
+    const int med = g->disp_medium[d];
+    const int np = g->disp_npoles[d];
+    const double *K = g->disp_kappa + (size_t)d * ORC_PMAX * 2;
+    const double *B = g->disp_beta + (size_t)d * ORC_PMAX * 2;
+    const double *G = g->disp_g3 + (size_t)d * ORC_PMAX * 2;
+    for (int c = 0; c < 3; ++c) {
+      float *F = g->f[c];
+      float *Ep = g->eprev[c];
+      double *J = g->jcur[c];
+      int xe = sweep_hi(g, c, 0), ye = sweep_hi(g, c, 1), ze = sweep_hi(g, c, 2);
+      for (int k = 0; k <= ze; ++k)
+        for (int j = 0; j <= ye; ++j)
+          for (int i = 0; i <= xe; ++i) {
+            size_t p = IDX(g, i, j, k);
+            if (g->mi[c][p] != med) continue;
+            double *Jp = J + p * ORC_PMAX * 2;
+            float e = F[p];
+            for (int q = 0; q < np; ++q) {
+              double re = G[2 * q] * Jp[2 * q] - G[2 * q + 1] * Jp[2 * q + 1];
+              e = (float)((double)e - re);
+            }
+            F[p] = e;
+            float diff = e - Ep[p];
+            for (int q = 0; q < np; ++q) {
+              double kr = K[2 * q], ki = K[2 * q + 1];
+              double jr = Jp[2 * q], ji = Jp[2 * q + 1];
+              double ar = kr * jr - ki * ji;
+              double ai = kr * ji + ki * jr;
+              double br = B[2 * q] * (double)diff / g->dt;
+              double bi = B[2 * q + 1] * (double)diff / g->dt;
+              Jp[2 * q] = ar + br;
+              Jp[2 * q + 1] = ai + bi;
+            }
+            Ep[p] = e;
+          }
+    }
+  }
+}
+
+/* --------------------------------------------------------------------------
+ * Nodal sources: evolucion (linear interpolation of the excitation table,
+ * nodalsources.F90:317-356) and AdvancenodalE/H (:396-578, :584-777).
+ * E time stamp tiempo(n) cast to real(4) (:426); H stamp tiempo(n)+0.5*dt
+ * (:618).  Hard: F = amp*f(t); soft: F = F - G2*Id_a*Id_b*amp*f(t) with the
+ * reference's factor order.  PEC (E) / PMC (H) media are skipped; the H hard
+ * sources read the Hx media matrix for Hy and Hz as the reference does
+ * (:641,:715).
+ * ------------------------------------------------------------------------ */
+float orc_evolucion(float t, const orc_nodal *s) {
+  float de = s->deltaevol;
+  long long nprev = (long long)(t / de);
+  if (nprev + 1 > s->numus || nprev + 1 <= 0) return 0.0f;
+  return (s->evol[nprev + 1] - s->evol[nprev]) / de * (t - (float)nprev * de) +
+         s->evol[nprev];
+}
+
+static void nodal_apply(orc_grid *g, const orc_nodal *s, float t) {
+  const int c = s->comp;
+  float *F = g->f[c];
+  const int16_t *mi = g->mi[(c >= 4 && s->hard) ? 3 : c];
+  const int skipbit = c < 3 ? ORC_KIND_PEC : ORC_KIND_PMC;
+  const float *gg = c < 3 ? g->g2 : g->gm2;
+  const float *Ix = c < 3 ? g->id[3] : g->id[0];
+  const float *Iy = c < 3 ? g->id[4] : g->id[1];
+  const float *Iz = c < 3 ? g->id[5] : g->id[2];
+  const int cc = c % 3;
+  for (int k = s->zi; k <= s->ze; ++k)
+    for (int j = s->yi; j <= s->ye; ++j)
+      for (int i = s->xi; i <= s->xe; ++i) {
+        size_t p = IDX(g, i, j, k);
+        int m = mi[p];
+        if (g->medkind[m] & skipbit) continue;
+        if (s->hard) {
+          F[p] = s->amp * orc_evolucion(t, s);
+        } else {
+          float ia, ib;
+          if (cc == 0) { ia = Iy[j + 1]; ib = Iz[k + 1]; }
+          else if (cc == 1) { ia = Ix[i + 1]; ib = Iz[k + 1]; }
+          else { ia = Iy[j + 1]; ib = Ix[i + 1]; }
+          F[p] = F[p] - gg[m] * ia * ib * s->amp * orc_evolucion(t, s);
+        }
+      }
+}
+
+void orc_nodal_e(orc_grid *g, int32_t n) {
+  float t = (float)g->tiempo[n];
+  for (int c = 0; c < 3; ++c)
+    for (int hard = 1; hard >= 0; --hard)
+      for (int s = 0; s < g->nnodal; ++s)
+        if (g->nodal[s].comp == c && g->nodal[s].hard == hard)
+          nodal_apply(g, &g->nodal[s], t);
+}
+
+void orc_nodal_h(orc_grid *g, int32_t n) {
+  float t = (float)(g->tiempo[n] + (double)0.5f * g->dt);
+  for (int c = 3; c < 6; ++c)
+    for (int hard = 1; hard >= 0; --hard)
+      for (int s = 0; s < g->nnodal; ++s)
+        if (g->nodal[s].comp == c && g->nodal[s].hard == hard)
+          nodal_apply(g, &g->nodal[s], t);
+}
+
+/* One full step in the order of ciclo_temporal, timestepping.F90:1248-1981:
+ * E Yee -> E CPML -> E ADE -> nodal E -> H Yee -> H CPML -> nodal H. */
+void orc_step(orc_grid *g, int32_t n) {
+  orc_yee_e(g);
+  orc_cpml_e(g);
+  if (g->ndisp) orc_ade_e(g);
+  if (g->nnodal) orc_nodal_e(g, n);
+  orc_yee_h(g);
+  orc_cpml_h(g);
+  if (g->nnodal) orc_nodal_h(g, n);
+}
+
+void orc_run(orc_grid *g, int32_t n0, int32_t nsteps, int32_t nprobe,
+             const int32_t *pc, const int32_t *pi, const int32_t *pj,
+             const int32_t *pk, float *out) {
+  for (int32_t s = 0; s < nsteps; ++s) {
+    orc_step(g, n0 + s);
+    for (int32_t p = 0; p < nprobe; ++p)
+      out[(size_t)s * nprobe + p] = g->f[pc[p]][IDX(g, pi[p], pj[p], pk[p])];
+  }
+}

@@ -1,0 +1,114 @@
+/*
+ * fdtd_oracle.h -- CPU restatement of semba-fdtd's field time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP library is
+ * compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product path never does.
+ *
+ * Layout shared with the HIP library (and with the reference's Alloc box,
+ * semba_fdtd.F90:1285-1300): every field / media-index array covers the cell
+ * coordinates [-1, n+1] on each axis, x fastest:
+ *     idx(i,j,k) = ((k+1)*(ny+3) + (j+1))*(nx+3) + (i+1)
+ * Coordinates are "total" cells: 0 is the outer face of the low CPML layer,
+ * nx/ny/nz count the interior plus both CPML slabs.  1-D spacing tables cover
+ * [-1, n+1] (index i+1); the raw cell-size array DX covers [-2, n+1]
+ * (index i+2), as sgg%dx does (preprocess_geom.F90:5639-5666).
+ */
+#ifndef FDTD_ORACLE_H
+#define FDTD_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_PMAX 8 /* max poles per dispersive medium */
+
+enum { ORC_EX = 0, ORC_EY, ORC_EZ, ORC_HX, ORC_HY, ORC_HZ };
+/* psi slots: E then H; (component, stretched axis) */
+enum {
+  ORC_PSI_EXY = 0, ORC_PSI_EXZ, ORC_PSI_EYX, ORC_PSI_EYZ, ORC_PSI_EZX, ORC_PSI_EZY,
+  ORC_PSI_HXY, ORC_PSI_HXZ, ORC_PSI_HYX, ORC_PSI_HYZ, ORC_PSI_HZX, ORC_PSI_HZY
+};
+
+typedef struct {
+  int32_t comp;        /* ORC_EX..ORC_HZ */
+  int32_t hard;        /* 1: E = amp*f(t) ; 0: soft current E -= G2*Id*Id*amp*f(t) */
+  int32_t xi, xe, yi, ye, zi, ze; /* box, already clipped to the sweep */
+  float amp;
+  float deltaevol;     /* sample spacing of the excitation table */
+  int32_t numus;       /* samples are evol[0..numus] */
+  const float *evol;
+} orc_nodal;
+
+typedef struct {
+  int32_t nx, ny, nz;
+  float *f[6];                 /* Ex Ey Ez Hx Hy Hz, Alloc layout */
+  const int16_t *mi[6];        /* media index per component, Alloc layout */
+  const float *g1, *g2, *gm1, *gm2; /* 0..nmedia */
+  int32_t nmedia;
+  const float *id[6];          /* Idxe Idye Idze Idxh Idyh Idzh, length n+3 */
+  /* CPML */
+  int32_t L[3][2];             /* layers (axis, side) ; 0 = none */
+  const float *pbe[3], *pce[3], *pbm[3], *pcm[3]; /* per axis, length n+3 */
+  float *psi[12];              /* full Alloc-sized arrays, only slab entries used */
+  /* electric pole-residue ADE */
+  int32_t ndisp;
+  const int32_t *disp_medium;  /* [ndisp] media index */
+  const int32_t *disp_npoles;  /* [ndisp] */
+  const double *disp_kappa;    /* [ndisp][ORC_PMAX][2] */
+  const double *disp_beta;     /* [ndisp][ORC_PMAX][2] */
+  const double *disp_g3;       /* [ndisp][ORC_PMAX][2] */
+  float *eprev[3];             /* Alloc-sized, per E component */
+  double *jcur[3];             /* Alloc-sized * ORC_PMAX * 2 per E component */
+  double dt;
+  const double *tiempo;        /* tiempo[n], n = 0..nsteps+1 */
+  int32_t nnodal;
+  const orc_nodal *nodal;
+  const int32_t *medkind;      /* [nmedia+1] bits: 1 PEC, 2 PMC, 4 E-dispersive */
+} orc_grid;
+
+/* ---- coefficient tables ---- */
+double orc_dt(float dxmin, float dymin, float dzmin, float eps0, float mu0,
+              float cfl, int32_t forcecfl, double dt_in);
+void orc_g_tables(int32_t nmedia, const float *epr, const float *sigma,
+                  const float *mur, const float *sigmam, const int32_t *kind,
+                  double dt, float eps0, float mu0,
+                  float *g1, float *g2, float *gm1, float *gm2);
+void orc_spacing(int32_t n, const float *DX, float *de, float *dh,
+                 float *Ide, float *Idh);
+void orc_cpml_axis(int32_t n, int32_t Llo, int32_t Lhi,
+                   float orden_lo, float orden_hi, float R_lo, float R_hi,
+                   float alphamaxpar, float alphaorden, float kappamaxpar,
+                   double dt, float eps0, float mu0,
+                   const float *de, const float *dh,
+                   float *pbe, float *pce, float *pbm, float *pcm,
+                   float *Ide, float *Idh);
+void orc_ade_tables(int32_t npoles, const double *c_re_im, const double *a_re_im,
+                    float eps11, float sigma11, double dt,
+                    double *kappa, double *beta, double *g3,
+                    float *g1_out, float *g2_out);
+void orc_time_vector(int32_t nlast, double dt, double *tiempo);
+
+/* ---- stepping ---- */
+void orc_yee_e(orc_grid *g);
+void orc_yee_h(orc_grid *g);
+void orc_cpml_e(orc_grid *g);
+void orc_cpml_h(orc_grid *g);
+void orc_ade_e(orc_grid *g);
+float orc_evolucion(float t, const orc_nodal *s);
+void orc_nodal_e(orc_grid *g, int32_t n);
+void orc_nodal_h(orc_grid *g, int32_t n);
+/* one full step n (E half, then H half) in the reference order */
+void orc_step(orc_grid *g, int32_t n);
+/* run steps n0..n0+nsteps-1, sampling probes after each step:
+ * probe p reads component pc[p] at (pi,pj,pk) into out[s*nprobe + p] */
+void orc_run(orc_grid *g, int32_t n0, int32_t nsteps, int32_t nprobe,
+             const int32_t *pc, const int32_t *pi, const int32_t *pj,
+             const int32_t *pk, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,312 @@
+"""Problem description: everything semba-fdtd's ``launch_simulation`` receives
+for the field time-stepping path, as plain numpy data.
+
+Coordinates are total cells per axis (CPML slabs included, 0 = outer face of
+the low slab).  Every field and media-index array covers the reference's Alloc
+box [-1, n+1] on each axis (semba_fdtd.F90:1285-1300) and is stored x-fastest
+as ``array[k+1, j+1, i+1]``.  The raw cell sizes ``D[a]`` cover [-2, n+1]
+(``D[a][i+2]``), like ``sgg%dx`` (preprocess_geom.F90:5611-5666).
+
+Media follow the reference conventions: index 0 is PEC
+(preprocess_geom.F90:470-476), index 1 the background, larger indices are
+bodies.  ``kind`` bits: 1 PEC, 2 PMC, 4 electric pole-residue dispersive.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Tuple
+import numpy as np
+
+# fdetypes.F90:219-222 / semba_fdtd.F90:154-155, held in real(4)
+EPS0 = float(np.float32(8.8541878176203898505365630317107502606083701665994498081024171524053950954599821142852891607182008932e-12))
+MU0 = float(np.float32(1.2566370614359172953850573533118011536788677597500423283899778369231265625144835994512139301368468271e-6))
+PMAX = 8
+KIND_PEC, KIND_PMC, KIND_EDISP = 1, 2, 4
+EX, EY, EZ, HX, HY, HZ = range(6)
+
+
+@dataclass
+class NodalSource:
+    """A hard or soft line/point source on one field component
+    (nodalsources.F90:245-300).  ``box`` = (xi, xe, yi, ye, zi, ze), already
+    inside the component's sweep; ``evol`` = excitation samples 0..numus,
+    spaced ``deltaevol`` seconds."""
+    comp: int
+    hard: bool
+    box: Tuple[int, int, int, int, int, int]
+    amp: float
+    evol: np.ndarray
+    deltaevol: float
+
+
+@dataclass
+class Problem:
+    nx: int
+    ny: int
+    nz: int
+    epr: np.ndarray
+    sigma: np.ndarray
+    mur: np.ndarray
+    sigmam: np.ndarray
+    kind: np.ndarray
+    mi: np.ndarray                      # int16 (6, nz+3, ny+3, nx+3)
+    D: List[np.ndarray]                 # float32 cell sizes, [-2, n+1]
+    dt: float
+    npoles: np.ndarray = None
+    pole_c: np.ndarray = None           # (nmedia+1, PMAX, 2) residues c_p
+    pole_a: np.ndarray = None           # (nmedia+1, PMAX, 2) poles a_p
+    eps11: np.ndarray = None            # absolute permittivity of dispersive media
+    sig11: np.ndarray = None
+    cpml_layers: list = field(default_factory=lambda: [[0, 0], [0, 0], [0, 0]])
+    cpml_order: list = field(default_factory=lambda: [[2.0, 2.0]] * 3)
+    cpml_refl: list = field(default_factory=lambda: [[1e-3, 1e-3]] * 3)
+    alphamax: float = 0.0
+    alphaorden: float = 1.0
+    kappamax: float = 1.0
+    nodal: List[NodalSource] = field(default_factory=list)
+    probes: List[Tuple[int, int, int, int]] = field(default_factory=list)
+    nsteps_max: int = 0
+    eps0: float = EPS0
+    mu0: float = MU0
+    name: str = ""
+
+    def __post_init__(self):
+        nm = self.nmedia
+        f32 = lambda a: np.ascontiguousarray(a, np.float32)
+        self.epr, self.sigma = f32(self.epr), f32(self.sigma)
+        self.mur, self.sigmam = f32(self.mur), f32(self.sigmam)
+        self.kind = np.ascontiguousarray(self.kind, np.int32)
+        if self.npoles is None:
+            self.npoles = np.zeros(nm + 1, np.int32)
+        if self.pole_c is None:
+            self.pole_c = np.zeros((nm + 1, PMAX, 2))
+        if self.pole_a is None:
+            self.pole_a = np.zeros((nm + 1, PMAX, 2))
+        if self.eps11 is None:
+            self.eps11 = np.zeros(nm + 1, np.float32)
+        if self.sig11 is None:
+            self.sig11 = np.zeros(nm + 1, np.float32)
+        self.npoles = np.ascontiguousarray(self.npoles, np.int32)
+        self.eps11, self.sig11 = f32(self.eps11), f32(self.sig11)
+        self.D = [f32(d) for d in self.D]
+        self.mi = np.ascontiguousarray(self.mi, np.int16)
+        assert self.mi.shape == (6,) + self.alloc_shape, (self.mi.shape, self.alloc_shape)
+        for a, n in enumerate((self.nx, self.ny, self.nz)):
+            assert self.D[a].shape == (n + 4,)
+
+    @property
+    def nmedia(self) -> int:
+        return len(self.epr) - 1
+
+    @property
+    def alloc_shape(self):
+        return (self.nz + 3, self.ny + 3, self.nx + 3)
+
+    @property
+    def has_cpml(self) -> bool:
+        return any(l for ax in self.cpml_layers for l in ax)
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+
+# --------------------------------------------------------------------------
+# sweeps (preprocess_geom.F90:5563-5591 in total coordinates)
+def sweep_hi(c: int, a: int, n: Tuple[int, int, int]) -> int:
+    """Last swept index of component c (0..5) along axis a."""
+    node = (c != a) if c < 3 else (c - 3 == a)
+    return n[a] if node else n[a] - 1
+
+
+def courant_dt(dmin, cfl=None, dt_in=None, eps0=EPS0, mu0=MU0) -> float:
+    """semba_fdtd.F90:1211-1250 with real(4) arithmetic."""
+    f = np.float32
+    cluz = f(1.0) / np.sqrt(f(eps0) * f(mu0), dtype=np.float32)
+    ix, iy, iz = (f(1.0) / f(d) for d in dmin)
+    dtlay = f(1.0) / (cluz * np.sqrt(ix * ix + iy * iy + iz * iz, dtype=np.float32))
+    if cfl is not None:
+        return float(f(dtlay * f(cfl)))
+    lim = f(dtlay * f(0.8))
+    return float(lim) if (dt_in is None or dt_in > float(lim)) else float(dt_in)
+
+
+def uniform_D(n: int, d: float) -> np.ndarray:
+    return np.full(n + 4, d, np.float32)
+
+
+def graded_D(n: int, d0: float, ratio: float, L: Tuple[int, int]) -> np.ndarray:
+    """Cell sizes growing geometrically by ``ratio`` from the centre of the
+    interior towards both boundaries; the CPML slabs and the padding repeat
+    the edge interior cell (preprocess_geom.F90:5639-5666)."""
+    D = np.empty(n + 4, np.float64)
+    lo, hi = L[0], n - L[1]
+    m = hi - lo
+    c = (m - 1) / 2.0
+    for i in range(lo, hi):
+        D[i + 2] = d0 * ratio ** abs(i - c) if m > 1 else d0
+    D[: lo + 2] = D[lo + 2]
+    D[hi + 2:] = D[hi + 1]
+    return D.astype(np.float32)
+
+
+def new_media_index(nx, ny, nz, background=1) -> np.ndarray:
+    return np.full((6, nz + 3, ny + 3, nx + 3), background, np.int16)
+
+
+def pec_walls(mi: np.ndarray, n, faces=(True,) * 6):
+    """PEC outer walls as media index 0 on tangential E and normal H
+    (healer.F90:1442-1617).  faces = (xlo, xhi, ylo, yhi, zlo, zhi)."""
+    nx, ny, nz = n
+    def plane(c, a, v):
+        sl = [slice(None)] * 3
+        sl[2 - a] = v + 1
+        mi[c][tuple(sl)] = 0
+    for a in range(3):
+        for side in range(2):
+            if not faces[2 * a + side]:
+                continue
+            v = 0 if side == 0 else n[a]
+            for c in range(3):
+                if c != a:
+                    plane(c, a, v)
+            plane(3 + a, a, v)
+
+
+def paint_voxels(mi: np.ndarray, inside: np.ndarray, medium: int, prio: np.ndarray):
+    """CreateVolumeMM (healer.F90:66-234) for a set of voxels of one medium.
+
+    ``inside`` is a bool array (nz, ny, nx) over the cells.  A voxel claims its
+    12 edges when its priority is strictly larger than the edge's current
+    medium (E) and its 6 faces when it is larger or equal (H)."""
+    nz, ny, nx = inside.shape
+    v = np.zeros((nz + 3, ny + 3, nx + 3), bool)
+    v[1:nz + 1, 1:ny + 1, 1:nx + 1] = inside        # v[k+1,j+1,i+1] = voxel(i,j,k)
+    def sh(dk, dj, di):
+        # voxel at (i+di, j+dj, k+dk) for each array position (i, j, k)
+        out = np.zeros_like(v)
+        out[max(0, -dk):v.shape[0] - max(0, dk), max(0, -dj):v.shape[1] - max(0, dj),
+            max(0, -di):v.shape[2] - max(0, di)] = \
+            v[max(0, dk):v.shape[0] - max(0, -dk), max(0, dj):v.shape[1] - max(0, -dj),
+              max(0, di):v.shape[2] - max(0, -di)]
+        return out
+    claim = [
+        sh(0, 0, 0) | sh(0, -1, 0) | sh(-1, 0, 0) | sh(-1, -1, 0),   # Ex(i,j,k): voxels (i, j-1..j, k-1..k)
+        sh(0, 0, 0) | sh(0, 0, -1) | sh(-1, 0, 0) | sh(-1, 0, -1),   # Ey: (i-1..i, j, k-1..k)
+        sh(0, 0, 0) | sh(0, 0, -1) | sh(0, -1, 0) | sh(0, -1, -1),   # Ez: (i-1..i, j-1..j, k)
+        sh(0, 0, 0) | sh(0, 0, -1),                                   # Hx: (i-1..i, j, k)
+        sh(0, 0, 0) | sh(0, -1, 0),                                   # Hy: (i, j-1..j, k)
+        sh(0, 0, 0) | sh(-1, 0, 0),                                   # Hz: (i, j, k-1..k)
+    ]
+    p_new = prio[medium]
+    for c in range(6):
+        cur = prio[mi[c].astype(np.int64)]
+        win = (p_new > cur) if c < 3 else (p_new >= cur)
+        mi[c][claim[c] & win] = medium
+
+
+def sphere_cells(n, centre, radius) -> np.ndarray:
+    """Voxels whose centre lies inside the sphere (cells indexed (k, j, i))."""
+    nx, ny, nz = n
+    k, j, i = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5,
+                          indexing="ij")
+    cx, cy, cz = centre
+    return (i - cx) ** 2 + (j - cy) ** 2 + (k - cz) ** 2 <= radius ** 2
+
+
+def gaussian_table(dt: float, nsamples: int, spread: float, delay: float) -> np.ndarray:
+    """Excitation samples g(n) = exp(-((n - delay)/spread)^2) at spacing dt."""
+    n = np.arange(nsamples + 1, dtype=np.float64)
+    return np.exp(-((n - delay) / spread) ** 2).astype(np.float32)
+
+
+# --------------------------------------------------------------------------
+def cavity_problem(n: int = 64, nsteps: int = 2000, seed: int = 1001, hard: bool = True,
+                   dx: float = 0.01, nprobes: int = 8) -> Problem:
+    """Plain method, CPU-runnable: n^3 vacuum cavity, PEC walls, Hertzian Ez
+    dipole at a seeded interior edge driven by a Gaussian (spread 30 steps,
+    delay 120 steps), dt = 0.9 x Courant, ``nprobes`` seeded Ex/Ey/Ez probes."""
+    rng = np.random.default_rng(seed)
+    mi = new_media_index(n, n, n)
+    pec_walls(mi, (n, n, n))
+    dt = courant_dt((dx, dx, dx), cfl=0.9)
+    s = rng.integers(n // 4, 3 * n // 4, size=3)
+    evol = gaussian_table(dt, nsteps + 4, 30.0, 120.0)
+    src = NodalSource(EZ, hard, (int(s[0]), int(s[0]), int(s[1]), int(s[1]), int(s[2]), int(s[2])),
+                      1.0, evol, float(np.float32(dt)))
+    probes = []
+    for p in range(nprobes):
+        c = int(rng.integers(0, 3))
+        pos = rng.integers(2, n - 2, size=3)
+        probes.append((c, int(pos[0]), int(pos[1]), int(pos[2])))
+    return Problem(n, n, n, epr=[1.0, 1.0], sigma=[0.0, 0.0], mur=[1.0, 1.0],
+                   sigmam=[0.0, 0.0], kind=[KIND_PEC, 0], mi=mi,
+                   D=[uniform_D(n, dx)] * 3, dt=dt, nodal=[src], probes=probes,
+                   nsteps_max=nsteps, name=f"cavity{n}")
+
+
+def lorentz_pair(f0: float, delta: float, deps: float, eps0: float = EPS0):
+    """A complex-conjugate pole pair with resonance f0 [Hz], damping delta
+    [1/s] and static increment deps (relative).  Both poles are listed
+    explicitly, as the reference requires (electricdispersive.F90:26-31).
+    Returns (c, a) arrays of shape (2, 2) = [pole][re, im]."""
+    a = complex(-delta, 2 * np.pi * f0)
+    c = -deps * eps0 * a / 2.0            # -2 Re(c/a) = deps*eps0 at s = 0
+    cs = np.array([[c.real, c.imag], [c.real, -c.imag]])
+    as_ = np.array([[a.real, a.imag], [a.real, -a.imag]])
+    return cs, as_
+
+
+def scatterer_problem(n=(24, 22, 20), L=((4, 5), (3, 4), (5, 3)), seed: int = 7,
+                      nsteps: int = 300, dx: float = 0.01, graded_z: float = 1.0,
+                      order: float = 3.0, refl: float = 1e-6, disp: bool = True,
+                      soft: bool = True, alphamax: float = 0.0, kappamax: float = 1.0,
+                      nprobes: int = 12) -> Problem:
+    """Small open problem exercising every piece of the path: CPML of uneven
+    thickness on all faces, optional geometric z grading, a lossy dielectric
+    sphere, a 2-pole Lorentz blob, a soft Ex and a hard Hy source and probes
+    on all six components."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    mi = new_media_index(nx, ny, nz)
+    epr = [1.0, 1.0, 4.0]
+    sig = [0.0, 0.0, 0.01]
+    kind = [KIND_PEC, 0, 0]
+    prio = np.array([100, 0, 10, 20], np.int64)
+    inner = [(L[a][0], (nx, ny, nz)[a] - L[a][1]) for a in range(3)]
+    ctr = [(lo + hi) / 2.0 for lo, hi in inner]
+    rad = 0.3 * min(hi - lo for lo, hi in inner)
+    paint_voxels(mi, sphere_cells(n, ctr, rad), 2, prio)
+    npoles = [0, 0, 0]
+    pc = np.zeros((4, PMAX, 2)); pa = np.zeros((4, PMAX, 2))
+    eps11 = np.zeros(4, np.float32); sig11 = np.zeros(4, np.float32)
+    if disp:
+        epr.append(2.0); sig.append(0.005); kind.append(KIND_EDISP); npoles.append(2)
+        c, a = lorentz_pair(5e9, 2e9, 1.5)
+        pc[3, :2] = c; pa[3, :2] = a
+        eps11[3] = np.float32(2.0 * EPS0); sig11[3] = 0.005
+        box = np.zeros((nz, ny, nx), bool)
+        x0, y0, z0 = (inner[0][0] + 1, inner[1][0] + 1, inner[2][0] + 1)
+        box[z0:z0 + 4, y0:y0 + 5, x0:x0 + 6] = True
+        paint_voxels(mi, box, 3, prio)
+    nm = len(epr) - 1
+    D = [uniform_D(nx, dx), uniform_D(ny, dx),
+         graded_D(nz, dx, graded_z, L[2]) if graded_z != 1.0 else uniform_D(nz, dx)]
+    dmin = [float(d.min()) for d in D]
+    dt = courant_dt(dmin, cfl=0.9)
+    evol = gaussian_table(dt, nsteps + 4, 12.0, 40.0)
+    s = [int(rng.integers(inner[a][0] + 2, inner[a][1] - 2)) for a in range(3)]
+    srcs = [NodalSource(EX, not soft, (s[0], s[0], s[1], s[1], s[2], s[2]), 3.0, evol,
+                        float(np.float32(dt))),
+            NodalSource(HY, True, (s[0] + 1, s[0] + 1, s[1], s[1], s[2] - 1, s[2] - 1), 0.01,
+                        evol, float(np.float32(dt)))]
+    probes = []
+    for p in range(nprobes):
+        c = p % 6
+        probes.append((c,) + tuple(int(rng.integers(0, sweep_hi(c, a, n) + 1)) for a in range(3)))
+    return Problem(nx, ny, nz, epr=epr, sigma=sig, mur=[1.0] * (nm + 1), sigmam=[0.0] * (nm + 1),
+                   kind=kind, mi=mi, D=D, dt=dt, npoles=npoles, pole_c=pc[:nm + 1],
+                   pole_a=pa[:nm + 1], eps11=eps11[:nm + 1], sig11=sig11[:nm + 1],
+                   cpml_layers=[list(l) for l in L], cpml_order=[[order, order]] * 3,
+                   cpml_refl=[[refl, refl]] * 3, alphamax=alphamax, kappamax=kappamax,
+                   nodal=srcs, probes=probes, nsteps_max=nsteps, name="scatterer")
