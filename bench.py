@@ -1,0 +1,202 @@
+"""Throughput of the gfx950 time-stepping path on one MI355X.
+
+Workload (default): the large heterogeneous single-GPU grid -- 1024^3 total
+cells with a 12-cell CPML on every face, 8 seeded lossy dielectric spheres and
+a 2-pole pole-residue medium in 20% of the volume -- built in HBM by the
+device scene builders.  A step is one full Yee iteration (E half + H half,
+CPML and pole state fused in, probes sampled), replayed from a captured HIP
+graph.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--mode fast|exact]
+                    [--no-cpu] [--profile]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+METRIC = "Mcell-updates/s + %HBM roofline; batched-serve p50/p99 latency vs plain"
+
+
+def psi_counts(pb):
+    """psi values updated per half step (E side, H side): the CPML slab
+    arrays of both passes."""
+    nx, ny, nz = pb.nx, pb.ny, pb.nz
+    Lx, Ly, Lz = (sum(pb.cpml_layers[a]) for a in range(3))
+    e = (nx * Ly * (nz + 1) + nx * (ny + 1) * Lz + Lx * ny * (nz + 1) + (nx + 1) * ny * Lz
+         + Lx * (ny + 1) * nz + (nx + 1) * Ly * nz)
+    h = ((nx + 1) * Ly * nz + (nx + 1) * ny * Lz + Lx * (ny + 1) * nz + nx * (ny + 1) * Lz
+         + Lx * ny * (nz + 1) + nx * Ly * (nz + 1))
+    return e, h
+
+
+def algorithmic_bytes(pb, ndisp_edges, mode, stored_poles):
+    """Compulsory HBM bytes of one E pass and one H pass.
+    Per cell and pass: read the 3 updated components (12 B), the 3 curl
+    components (12 B) and 3 int16 media indices (6 B), write 3 components
+    (12 B) = 42 B.  CPML: 8 B per psi value (read + write).  Poles: per
+    dispersive edge 8 B of Eprev (r+w) + 16 B per stored complex64 state
+    (FAST, conjugate pairs folded) or 32 B per complex128 state (EXACT)."""
+    cells = pb.cells
+    pe, ph = psi_counts(pb)
+    per_state = 16 if mode == "fast" else 32
+    e = 42 * cells + 8 * pe + ndisp_edges * (8 + per_state * stored_poles)
+    h = 42 * cells + 8 * ph
+    return e, h
+
+
+def cpu_baseline(n_sample, steps):
+    """The oracle (C restatement, OpenMP over k/j like the reference loops)
+    on the host cores, on the same heterogeneous workload at n_sample^3."""
+    import oracle
+    from semba_fdtd_amd import problem as P
+    pb = P.heterogeneous_problem(n=n_sample, L=max(4, 12 * n_sample // 1024), nsteps=steps,
+                                 host_media=True)
+    o = oracle.Oracle(pb)
+    o.run(2)
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(pb.cells * steps / dt / 1e6, 2), "unit": "Mcell-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"oracle (fdtd_oracle.c, -O2 OpenMP {threads} threads) on the same "
+                      f"heterogeneous workload at {n_sample}^3 (CPML {pb.cpml_layers[0][0]}, "
+                      f"spheres, 20% 2-pole medium, complex128 poles), {steps} steps, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=192)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(independent grids shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import semba_fdtd_amd as sf
+    from semba_fdtd_amd import problem as P
+
+    total = args.warmup + args.steps + 2 * args.kernel_reps + 4
+    pb = P.heterogeneous_problem(n=args.n, nsteps=total)
+    print(f"[bench] building {pb.name} ({pb.cells / 1e9:.3f} G cells) mode={args.mode}",
+          file=sys.stderr, flush=True)
+    torch.cuda.init()
+    stream = torch.cuda.Stream()
+    t0 = time.perf_counter()
+    s = sf.Solver(pb, mode=args.mode, stream=stream.cuda_stream, probe_buffer=total)
+    s.run(args.warmup)  # builds pole state, captures the graph
+    s.synchronize()
+    setup_s = time.perf_counter() - t0
+    ndisp = sum(int(s.lib.fdtd_dispersive_edges(s.h, c)) for c in range(3))
+    print(f"[bench] setup {setup_s:.1f} s, {s.device_bytes / 2**30:.1f} GiB in HBM, "
+          f"{ndisp} dispersive edges", file=sys.stderr, flush=True)
+
+    # timed region: K whole steps
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        ev0.record(stream)
+        s.run(args.steps)
+        ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    ms_per_step = ms / args.steps
+    value = pb.cells * args.steps / (ms / 1e3) / 1e6
+
+    # per-pass kernel durations on the same stream (E pass: Yee + CPML + poles)
+    def timed(fn, reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+    lib, h = s.lib, s.h
+    n0 = s.n
+    e_ms = timed(lambda: lib.fdtd_advance_e(h, n0), args.kernel_reps)
+    h_ms = timed(lambda: lib.fdtd_advance_h(h, n0), args.kernel_reps)
+    stored = 1 if args.mode == "fast" else 2
+    e_bytes, h_bytes = algorithmic_bytes(pb, ndisp, args.mode, stored)
+    e_gbs = e_bytes / (e_ms / 1e3) / 1e9
+    h_gbs = h_bytes / (h_ms / 1e3) / 1e9
+    step_bytes = e_bytes + h_bytes
+    step_gbs = step_bytes / (ms_per_step / 1e3) / 1e9
+    probes = s.probes()
+    finite = bool(np.isfinite(probes).all())
+    s.close()
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "e_pass_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            t = json.load(f)
+        if t.get("workload") == pb.name and t.get("mode") == args.mode:
+            traffic = t.get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_n, args.cpu_steps)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.mode == "fast" else "f32 (no FMA) + f64 poles",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{pb.name}: {pb.nx}^3 cells incl. 12-cell CPML, 8 lossy spheres, "
+                        f"2-pole medium in 20% of the volume, soft Ex line source, "
+                        f"{len(pb.probes)} probes",
+            "mode": args.mode,
+            "cells": pb.cells,
+            "dispersive_edges": ndisp,
+            "step_hbm_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+            "e_pass_ms": round(e_ms, 4),
+            "h_pass_ms": round(h_ms, 4),
+            "h_pass_gbs": round(h_gbs, 1),
+            "probes_finite": finite,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(e_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(e_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "E pass (Yee E + CPML E + pole-residue ADE), algorithmic "
+                      f"{e_bytes / pb.cells:.1f} B/cell",
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,219 @@
+/*
+ * sembafdtd.h -- C ABI of the MI355X (gfx950) field time-stepping library.
+ *
+ * Drop-in for the hot path of semba-fdtd's time loop (ciclo_temporal,
+ * src_main_pub/timestepping.F90:1248-1981).  The reference calls Fortran
+ * module/internal subroutines on host arrays; this library keeps the fields,
+ * media indices, CPML state and dispersive state resident in HBM and exposes
+ * one entry point per reference call site.  A Fortran driver binds these
+ * through iso_c_binding (semba_fdtd_amd/fortran/sembafdtd_gpu.F90).
+ *
+ * Conventions
+ *  - Coordinates are total cells per axis (CPML slabs included); 0 is the
+ *    outer face of the low slab.  nx, ny, nz are the reference's swept cell
+ *    counts (fullsize, semba_fdtd.F90:1285-1309).
+ *  - Host arrays of fields / media indices use the reference's Alloc box
+ *    [-1, n+1] on every axis, x fastest:
+ *        idx(i,j,k) = ((k+1)*(ny+3) + (j+1))*(nx+3) + (i+1)
+ *    (all six components share the Alloc origin, timestepping.F90:282-290).
+ *  - 1-D tables (inverse spacings, CPML profiles) have length n+3, index i+1.
+ *  - Every function returns 0 on success or a negative FDTD_ERR_* code;
+ *    fdtd_last_error() returns the message.  A handle is bound to one HIP
+ *    stream and is not re-entrant; distinct handles may run concurrently.
+ *  - The library never falls back to the CPU: without a usable gfx950 device
+ *    fdtd_create fails with FDTD_ERR_DEVICE.
+ */
+#ifndef SEMBAFDTD_H
+#define SEMBAFDTD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDTD_ABI_VERSION 1
+
+enum {
+  FDTD_OK = 0,
+  FDTD_ERR_ARG = -1,
+  FDTD_ERR_DEVICE = -2,
+  FDTD_ERR_NOMEM = -3,
+  FDTD_ERR_STATE = -4,
+  FDTD_ERR_HIP = -5
+};
+
+/* Arithmetic mode.
+ * FAST : FMA contraction allowed, conjugate pole pairs folded into one
+ *        complex64 state (fp32), the production path.
+ * EXACT: no FMA contraction, complex(8) pole state updated pole by pole in
+ *        the reference's order; bit-identical to the reference loops. */
+enum { FDTD_MODE_FAST = 0, FDTD_MODE_EXACT = 1 };
+
+/* Field storage in HBM.  FP16 keeps each field as binary16 times a per-array
+ * scale with fp32 arithmetic in registers (CPML psi and pole state stay fp32). */
+enum { FDTD_STORE_FP32 = 0, FDTD_STORE_FP16 = 1 };
+
+/* field components */
+enum { FDTD_EX = 0, FDTD_EY, FDTD_EZ, FDTD_HX, FDTD_HY, FDTD_HZ };
+
+/* media kind bits (per media index) */
+enum { FDTD_KIND_PEC = 1, FDTD_KIND_PMC = 2, FDTD_KIND_EDISP = 4 };
+
+typedef struct fdtd_handle fdtd_handle;
+
+typedef struct {
+  int32_t nx, ny, nz; /* total swept cells per axis, CPML included */
+  int32_t mode;       /* FDTD_MODE_* */
+  int32_t storage;    /* FDTD_STORE_* */
+  int32_t device;     /* HIP device ordinal */
+  void *stream;       /* hipStream_t to run on; NULL = library-owned stream */
+} fdtd_grid_desc;
+
+/* ---- lifetime --------------------------------------------------------- */
+/* Allocates the six fields and media matrices (launch_simulation's field
+ * allocation, timestepping.F90:375-402); fields start at zero. */
+int fdtd_create(const fdtd_grid_desc *desc, fdtd_handle **out);
+/* Frees everything (Destroy_All_exceptSGGMxx, timestepping.F90:3177-3227). */
+int fdtd_destroy(fdtd_handle *h);
+/* NUL-terminated message of the last failure on h (or of fdtd_create when
+ * h is NULL). */
+const char *fdtd_last_error(const fdtd_handle *h);
+int fdtd_abi_version(void);
+
+/* ---- set-up ----------------------------------------------------------- */
+/* Coefficient tables G1/G2/GM1/GM2 [0..nmedia] (calc_G1G2Gm1Gm2,
+ * calc_constants.F90:36-148, after InitEDispersives overrides them) and media
+ * kind bits [0..nmedia]. */
+int fdtd_set_media(fdtd_handle *h, int32_t nmedia, const float *g1,
+                   const float *g2, const float *gm1, const float *gm2,
+                   const int32_t *kind);
+/* Media index matrix of one component (sggMiEx..sggMiHz, host Alloc layout;
+ * semba_fdtd.F90:93, built by read_geomData). */
+int fdtd_set_media_index(fdtd_handle *h, int32_t comp, const int16_t *mi);
+/* Inverse spacings Idxe,Idye,Idze,Idxh,Idyh,Idzh (timestepping.F90:314-370,
+ * already stretched by the CPML, borderscpml.F90:1556-1724). */
+int fdtd_set_spacing(fdtd_handle *h, const float *idxe, const float *idye,
+                     const float *idze, const float *idxh, const float *idyh,
+                     const float *idzh);
+/* CPML (InitCPMLBorders, borderscpml.F90:90-473): layers[a*2+s] for axis
+ * a = x,y,z and side s = low,high (0 = no CPML on that face); profiles
+ * P_be, P_ce (E) and P_bm, P_cm (H) per axis, length n+3.  psi starts at 0. */
+int fdtd_set_cpml(fdtd_handle *h, const int32_t layers[6],
+                  const float *pbe_x, const float *pce_x, const float *pbm_x,
+                  const float *pcm_x, const float *pbe_y, const float *pce_y,
+                  const float *pbm_y, const float *pcm_y, const float *pbe_z,
+                  const float *pce_z, const float *pbm_z, const float *pcm_z);
+/* Electric pole-residue media (InitEDispersives, electricdispersive.F90:
+ * 77-298): for each of ndisp media its index, pole count (<= 8) and the
+ * complex(8) coefficients kappa_p, beta_p, G3_p as [ndisp][8][re,im].
+ * Pole state starts at zero. */
+int fdtd_set_edispersive(fdtd_handle *h, int32_t ndisp, const int32_t *medium,
+                         const int32_t *npoles, const double *kappa,
+                         const double *beta, const double *g3);
+/* Time step and time vector (crea_timevector, timestepping.F90:3230-3240):
+ * tiempo[0..nlast]; NULL builds tiempo(n) = tiempo(n-1) + dt from 0. */
+int fdtd_set_time(fdtd_handle *h, double dt, int32_t nlast, const double *tiempo);
+/* One nodal source (InitnodalSources/createnodal, nodalsources.F90:245-300):
+ * component, hard (1) or soft (0), box {xi,xe,yi,ye,zi,ze} inside the sweep,
+ * amplitude and excitation samples evol[0..numus] spaced deltaevol. */
+int fdtd_add_nodal(fdtd_handle *h, int32_t comp, int32_t hard,
+                   const int32_t box[6], float amp, float deltaevol,
+                   int32_t numus, const float *evol);
+
+/* ---- the hot path ----------------------------------------------------- */
+/* Fused E half step: Advance_Ex/Ey/Ez (timestepping.F90:1286-1297) +
+ * AdvanceelectricCPML (:1381) + AdvanceEDispersiveE (:1416) in one HBM pass,
+ * then AdvanceNodalE at tiempo(n) (:1438-1446). */
+int fdtd_advance_e(fdtd_handle *h, int32_t n);
+/* Fused H half step: Advance_Hx/Hy/Hz (:1497-1507) + AdvanceMagneticCPML
+ * (:1535) in one pass, then AdvanceNodalH at tiempo(n)+dt/2 (:1596-1635). */
+int fdtd_advance_h(fdtd_handle *h, int32_t n);
+/* The same pieces one call site at a time, for drivers that interleave other
+ * host modules between them. */
+int fdtd_advance_e_yee(fdtd_handle *h);        /* Advance_Ex/Ey/Ez */
+int fdtd_advance_e_cpml(fdtd_handle *h);       /* AdvanceelectricCPML */
+int fdtd_advance_e_dispersive(fdtd_handle *h); /* AdvanceEDispersiveE */
+int fdtd_advance_nodal_e(fdtd_handle *h, int32_t n); /* AdvanceNodalE */
+int fdtd_advance_h_yee(fdtd_handle *h);        /* Advance_Hx/Hy/Hz */
+int fdtd_advance_h_cpml(fdtd_handle *h);       /* AdvanceMagneticCPML */
+int fdtd_advance_nodal_h(fdtd_handle *h, int32_t n); /* AdvanceNodalH */
+/* nsteps full steps n0 .. n0+nsteps-1 (E half then H half, probes sampled
+ * after each step as UpdateObservation does, timestepping.F90:1725-1742),
+ * replayed from a captured HIP graph. */
+int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps);
+
+/* ---- observation ------------------------------------------------------ */
+/* Point probes (observation.F90:2620-2672): component and Yee index each;
+ * samples are buffered on the device, capacity = bufsteps samples. */
+int fdtd_set_probes(fdtd_handle *h, int32_t nprobe, const int32_t *comp,
+                    const int32_t *i, const int32_t *j, const int32_t *k,
+                    int32_t bufsteps);
+/* Copies the buffered samples [nsamples][nprobe] to out (capacity
+ * max_samples rows), clears the buffer, returns the count in *nsamples. */
+int fdtd_read_probes(fdtd_handle *h, float *out, int32_t max_samples,
+                     int32_t *nsamples);
+/* Sample probes once now (used by drivers that step with fdtd_advance_*). */
+int fdtd_sample_probes(fdtd_handle *h);
+
+/* ---- field access ----------------------------------------------------- */
+int fdtd_get_field(fdtd_handle *h, int32_t comp, float *host_alloc);
+int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host_alloc);
+int fdtd_synchronize(fdtd_handle *h);
+/* Device bytes held by the handle. */
+int64_t fdtd_device_bytes(const fdtd_handle *h);
+/* Number of dispersive E edges of component comp (0..2) holding pole state
+ * (the node lists of InitEDispersives, electricdispersive.F90:148-244);
+ * valid after the first advance/run call, -1 before. */
+int64_t fdtd_dispersive_edges(const fdtd_handle *h, int32_t comp);
+
+/* ---- host-side coefficient tables (built once, uploaded with the set-up
+ * calls above).  They evaluate the reference's expressions in its kinds. */
+/* dt: semba_fdtd.F90:1211-1250 (forcecfl = the -cfl switch). */
+double fdtd_courant_dt(float dxmin, float dymin, float dzmin, float eps0, float mu0,
+                       float cfl, int32_t forcecfl, double dt_in);
+/* calc_G1G2Gm1Gm2, calc_constants.F90:36-148; arrays [0..nmedia]. */
+int fdtd_build_g_tables(int32_t nmedia, const float *epr, const float *sigma,
+                        const float *mur, const float *sigmam, const int32_t *kind,
+                        double dt, float eps0, float mu0, float *g1, float *g2,
+                        float *gm1, float *gm2);
+/* spacings, timestepping.F90:314-370: D[-2..n+1] -> de, dh, 1/de, 1/dh on
+ * [-1..n+1] (length n+3). */
+int fdtd_build_spacing(int32_t n, const float *D, float *de, float *dh, float *ide,
+                       float *idh);
+/* CPML profiles of one axis, calc_cpmlconstants (borderscpml.F90:1465-1733);
+ * stretches ide/idh in place. */
+int fdtd_build_cpml_axis(int32_t n, int32_t Llo, int32_t Lhi, float orden_lo,
+                         float orden_hi, float R_lo, float R_hi, float alphamaxpar,
+                         float alphaorden, float kappamaxpar, double dt, float eps0,
+                         float mu0, const float *de, const float *dh, float *pbe,
+                         float *pce, float *pbm, float *pcm, float *ide, float *idh);
+/* pole-residue coefficients and the medium's G1/G2 (electricdispersive.F90:
+ * 98-146); c, a, kappa, beta, g3 are [npoles][re,im]. */
+int fdtd_build_ade(int32_t npoles, const double *c, const double *a, float eps11,
+                   float sigma11, double dt, double *kappa, double *beta, double *g3,
+                   float *g1, float *g2);
+/* evolucion, nodalsources.F90:317-356. */
+float fdtd_evolucion(float t, float deltaevol, int32_t numus, const float *evol);
+
+/* ---- device-side scene building for grids too large to build on the host.
+ * Same rules as the reference preprocessing, applied in HBM. */
+int fdtd_media_fill(fdtd_handle *h, int16_t medium);
+/* PEC outer walls (healer.F90:1442-1617): faces bit mask xlo,xhi,ylo,yhi,zlo,zhi. */
+int fdtd_media_pec_walls(fdtd_handle *h, int32_t faces);
+/* Voxelised sphere (cells whose centre is inside) painted with
+ * CreateVolumeMM's priority rule (healer.F90:66-234): an edge takes the medium
+ * when prio[medium] > prio[current], a face when >=. prio has nmedia+1
+ * entries. */
+int fdtd_media_paint_sphere(fdtd_handle *h, float cx, float cy, float cz,
+                            float radius, int16_t medium, int32_t nprio,
+                            const int32_t *prio);
+/* Axis-aligned box of cells [x0,x1)x[y0,y1)x[z0,z1) painted the same way. */
+int fdtd_media_paint_box(fdtd_handle *h, int32_t x0, int32_t x1, int32_t y0,
+                         int32_t y1, int32_t z0, int32_t z1, int16_t medium,
+                         int32_t nprio, const int32_t *prio);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -53,7 +53,7 @@ class OrcGrid(C.Structure):
                 ("ndisp", C.c_int32), ("disp_medium", _i32p),
                 ("disp_npoles", _i32p), ("disp_kappa", _f64p),
                 ("disp_beta", _f64p), ("disp_g3", _f64p),
-                ("eprev", _f32p * 3), ("jcur", _f64p * 3),
+                ("eprev", _f32p * 3), ("jcur", _f64p * 3), ("jstride", C.c_int32),
                 ("dt", C.c_double), ("tiempo", _f64p),
                 ("nnodal", C.c_int32), ("nodal", C.POINTER(OrcNodal)),
                 ("medkind", _i32p)]
@@ -159,7 +159,8 @@ class Oracle:
         self.psi = np.zeros((12,) + shape, np.float32) if pb.has_cpml else None
         nd = len(t.disp_media)
         self.eprev = np.zeros((3,) + shape, np.float32) if nd else None
-        self.jcur = np.zeros((3,) + shape + (PMAX, 2)) if nd else None
+        self.jstride = int(max(t.disp_npoles)) if nd else 1
+        self.jcur = np.zeros((3,) + shape + (self.jstride, 2)) if nd else None
         self.tiempo = np.zeros(pb.nsteps_max + 3)
         self.lib.orc_time_vector(pb.nsteps_max + 2, pb.dt, _p(self.tiempo, _f64p))
         self.medkind = np.ascontiguousarray(pb.kind, np.int32)
@@ -195,6 +196,7 @@ class Oracle:
             for c in range(3):
                 g.eprev[c] = self.eprev[c].ctypes.data_as(_f32p)
                 g.jcur[c] = self.jcur[c].ctypes.data_as(_f64p)
+        g.jstride = self.jstride
         g.dt = pb.dt
         g.tiempo = _p(self.tiempo, _f64p)
         ns = len(pb.nodal)
@@ -298,7 +300,7 @@ class Reference:
         pb = self.pb
         nm = pb.nmedia
         f = lambda n: np.zeros(n, np.float32)
-        out = [f(nm + 1) for _ in range(4)] + [f(pb.nx + 3), f(pb.ny + 3), f(pb.nz + 3)] * 2
+        out = [f(nm + 1) for _ in range(4)] + [f(n + 3) for n in (pb.nx, pb.ny, pb.nz) * 2]
         self.lib.ref_get_tables(*[_p(a, _f32p) for a in out])
         return out
 

@@ -462,7 +462,7 @@ void orc_ade_e(orc_grid *g) {
           for (int i = 0; i <= xe; ++i) {
             size_t p = IDX(g, i, j, k);
             if (g->mi[c][p] != med) continue;
-            double *Jp = J + p * ORC_PMAX * 2;
+            double *Jp = J + p * (size_t)g->jstride * 2;
             float e = F[p];
             for (int q = 0; q < np; ++q) {
               double re = G[2 * q] * Jp[2 * q] - G[2 * q + 1] * Jp[2 * q + 1];

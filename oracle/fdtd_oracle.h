@@ -61,7 +61,8 @@ typedef struct {
   const double *disp_beta;     /* [ndisp][ORC_PMAX][2] */
   const double *disp_g3;       /* [ndisp][ORC_PMAX][2] */
   float *eprev[3];             /* Alloc-sized, per E component */
-  double *jcur[3];             /* Alloc-sized * ORC_PMAX * 2 per E component */
+  double *jcur[3];             /* Alloc-sized * jstride * 2 per E component */
+  int32_t jstride;             /* pole slots per edge (>= max poles) */
   double dt;
   const double *tiempo;        /* tiempo[n], n = 0..nsteps+1 */
   int32_t nnodal;

@@ -1,0 +1,235 @@
+// fdtd_aux.hip -- small gfx950 kernels around the two passes: nodal sources,
+// point probes, dispersive-edge counting for the compact pole state, and the
+// device-side scene builders used for grids too large to voxelise on the host.
+// Built with -ffp-contract=off: the soft-source update keeps the reference's
+// rounding order.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include "fdtd_internal.h"
+
+namespace sfd {
+
+__device__ __forceinline__ long long fidx(const DevGrid &g, int i, int j, int k) {
+  return (long long)(k + 1) * g.pxy + (long long)(j + 1) * g.px + kXOff + i;
+}
+
+// AdvanceNodalE/H (nodalsources.F90:396-777).  cells are in the reference's
+// processing order; values[n*nsrc + s] = evolucion(tiempo) of source s at
+// step n, evaluated on the host in real(4).  Hard: F = amp*f;
+// soft: F = F - ((G2*Ia*Ib)*amp)*f.
+__global__ void k_nodal(DevGrid g, const NodalCell *cells, int ncells,
+                        const float *values, int nsrc, int n, const int *dn,
+                        int serial) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dn) n = *dn;  // graph replay: step index lives on the device
+  const int lo = serial ? 0 : t, hi = serial ? (t == 0 ? ncells : 0) : min(t + 1, ncells);
+  for (int c = lo; c < hi; ++c) {
+    const NodalCell cell = cells[c];
+    if (cell.skip) continue;
+    const float v = values[(long long)n * nsrc + cell.src];
+    float *F = g.f[cell.comp];
+    if (cell.hard)
+      F[cell.p] = cell.amp * v;
+    else
+      F[cell.p] = F[cell.p] - cell.coef * cell.amp * v;
+  }
+}
+
+int launch_nodal(const DevGrid &g, const NodalCell *cells, int ncells,
+                 const float *values, int nsrc, int n, const int *dn, int serial,
+                 void *stream) {
+  if (ncells <= 0) return 0;
+  const int threads = 64;
+  const int blocks = serial ? 1 : (ncells + threads - 1) / threads;
+  hipLaunchKernelGGL(k_nodal, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, g,
+                     cells, ncells, values, nsrc, n, dn, serial);
+  return (int)hipGetLastError();
+}
+
+// UpdateObservation point probes (observation.F90:2620-2672): one row of
+// samples per call; *count advances on the device so captured graphs can
+// replay it.
+__global__ void k_probes(DevGrid g, const ProbeDev *pr, int np, float *buf,
+                         int *count, int cap) {
+  __shared__ int row;
+  if (threadIdx.x == 0) row = *count;
+  __syncthreads();
+  if (row < cap) {
+    for (int t = threadIdx.x; t < np; t += blockDim.x)
+      buf[(long long)row * np + t] = g.f[pr[t].comp][pr[t].p];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && row < cap) *count = row + 1;
+}
+
+int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
+                  int *count, int cap, void *stream) {
+  if (np <= 0) return 0;
+  hipLaunchKernelGGL(k_probes, dim3(1), dim3(256), 0, (hipStream_t)stream, g, pr, np,
+                     buf, count, cap);
+  return (int)hipGetLastError();
+}
+
+// number of dispersive edges of component comp in every 64-wide x segment
+// of every row (rows = (j,k) of the component's sweep).
+__global__ void k_count_disp(DevGrid g, int comp, int ylast, int zlast, int xlast,
+                             int32_t *counts) {
+  const int lane = threadIdx.x;
+  const long long w = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const long long nrows = (long long)(ylast + 1) * (zlast + 1);
+  if (w >= nrows * g.nseg) return;
+  const int seg = (int)(w % g.nseg);
+  const long long row = w / g.nseg;
+  const int j = (int)(row % (ylast + 1)), k = (int)(row / (ylast + 1));
+  const int i = seg * kWave + (lane & (kWave - 1));
+  bool d = false;
+  if (i <= xlast) d = g.dslot[g.mi[comp][fidx(g, i, j, k)]] >= 0;
+  const unsigned long long m = __ballot(d);
+  if ((lane & (kWave - 1)) == 0) counts[w] = __popcll(m);
+}
+
+int launch_count_disp(const DevGrid &g, int comp, int32_t *counts, void *stream) {
+  const int xl = comp == 0 ? g.nx - 1 : g.nx;
+  const int yl = comp == 1 ? g.ny - 1 : g.ny;
+  const int zl = comp == 2 ? g.nz - 1 : g.nz;
+  const long long nw = (long long)(yl + 1) * (zl + 1) * g.nseg;
+  const int wpb = 4;
+  const long long blocks = (nw + wpb - 1) / wpb;
+  hipLaunchKernelGGL(k_count_disp, dim3((unsigned)blocks), dim3(kWave * wpb), 0,
+                     (hipStream_t)stream, g, comp, yl, zl, xl, counts);
+  return (int)hipGetLastError();
+}
+
+__global__ void k_tick(int *dn) { *dn = *dn + 1; }
+
+int launch_tick(int *dn, void *stream) {
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, (hipStream_t)stream, dn);
+  return (int)hipGetLastError();
+}
+
+int exclusive_scan_i32(const int32_t *in, int32_t *out, int n, void *tmp,
+                       size_t *tmp_bytes, void *stream) {
+  return (int)hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, n,
+                                               (hipStream_t)stream);
+}
+
+__global__ void k_fill_i16(int16_t *a, long long n, int16_t v) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x)
+    a[t] = v;
+}
+
+int launch_fill_i16(int16_t *a, long long n, int16_t v, void *stream) {
+  hipLaunchKernelGGL(k_fill_i16, dim3(4096), dim3(256), 0, (hipStream_t)stream, a, n, v);
+  return (int)hipGetLastError();
+}
+
+// PEC outer walls: tangential E and normal H of a face get media 0
+// (healer.F90:1442-1617).  One thread per (u, v) position of a face plane.
+__global__ void k_pec_walls(DevGrid g, int16_t *m0, int16_t *m1, int16_t *m2,
+                            int16_t *m3, int16_t *m4, int16_t *m5, int faces) {
+  int16_t *mi[6] = {m0, m1, m2, m3, m4, m5};
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const int face = blockIdx.y;  // 0..5
+  if (!(faces & (1 << face))) return;
+  const int a = face >> 1;
+  const int v = (face & 1) ? n[a] : 0;
+  const int b = (a + 1) % 3, c = (a + 2) % 3;
+  const long long cnt = (long long)(n[b] + 1) * (n[c] + 1);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+       t += (long long)gridDim.x * blockDim.x) {
+    int crd[3];
+    crd[a] = v;
+    crd[b] = (int)(t % (n[b] + 1));
+    crd[c] = (int)(t / (n[b] + 1));
+    const long long p = fidx(g, crd[0], crd[1], crd[2]);
+    for (int comp = 0; comp < 3; ++comp)
+      if (comp != a) mi[comp][p] = 0;
+    mi[3 + a][p] = 0;
+  }
+}
+
+int launch_pec_walls(const DevGrid &g, int16_t *mi[6], int faces, void *stream) {
+  hipLaunchKernelGGL(k_pec_walls, dim3(1024, 6), dim3(256), 0, (hipStream_t)stream, g,
+                     mi[0], mi[1], mi[2], mi[3], mi[4], mi[5], faces);
+  return (int)hipGetLastError();
+}
+
+// CreateVolumeMM (healer.F90:66-234) for a voxelised shape: a voxel claims
+// its 12 edges (priority strictly larger than the edge's medium) and its 6
+// faces (larger or equal).  shape 0: sphere {cx,cy,cz,r} (voxel centre
+// inside), 1: box {x0,x1,y0,y1,z0,z1}.  par[6..11] = launch box origin/size.
+__device__ __forceinline__ bool voxel_in(const DevGrid &g, int shape, const float *par,
+                                         int i, int j, int k) {
+  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return false;
+  if (shape == 0) {
+    const float dx = (float)i + 0.5f - par[0], dy = (float)j + 0.5f - par[1],
+                dz = (float)k + 0.5f - par[2];
+    return dx * dx + dy * dy + dz * dz <= par[3] * par[3];
+  }
+  return i >= (int)par[0] && i < (int)par[1] && j >= (int)par[2] && j < (int)par[3] &&
+         k >= (int)par[4] && k < (int)par[5];
+}
+
+struct PaintArgs {
+  int16_t *mi[6];
+  float par[12];
+  int shape;
+  int16_t medium;
+  int nprio;
+  int ox, oy, oz, sx, sy, sz;
+};
+
+__global__ void k_paint(DevGrid g, PaintArgs a, const int32_t *prio) {
+  const long long tot = (long long)a.sx * a.sy * a.sz;
+  const int pn = prio[a.medium];
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = a.ox + (int)(t % a.sx);
+    const int j = a.oy + (int)((t / a.sx) % a.sy);
+    const int k = a.oz + (int)(t / ((long long)a.sx * a.sy));
+    if (i > g.nx || j > g.ny || k > g.nz || i < 0 || j < 0 || k < 0) continue;
+    const long long p = fidx(g, i, j, k);
+    // media outside the priority table are never overwritten
+#define PR(c) (a.mi[c][p] < a.nprio ? prio[a.mi[c][p]] : 0x7fffffff)
+#define IN(di, dj, dk) voxel_in(g, a.shape, a.par, i + (di), j + (dj), k + (dk))
+    const bool v000 = IN(0, 0, 0);
+    // Ex(i,j,k): voxels (i, j-1..j, k-1..k)
+    if (i < g.nx && (v000 || IN(0, -1, 0) || IN(0, 0, -1) || IN(0, -1, -1)))
+      if (pn > PR(0)) a.mi[0][p] = a.medium;
+    // Ey: (i-1..i, j, k-1..k)
+    if (j < g.ny && (v000 || IN(-1, 0, 0) || IN(0, 0, -1) || IN(-1, 0, -1)))
+      if (pn > PR(1)) a.mi[1][p] = a.medium;
+    // Ez: (i-1..i, j-1..j, k)
+    if (k < g.nz && (v000 || IN(-1, 0, 0) || IN(0, -1, 0) || IN(-1, -1, 0)))
+      if (pn > PR(2)) a.mi[2][p] = a.medium;
+    // Hx: (i-1..i, j, k)
+    if (j < g.ny && k < g.nz && (v000 || IN(-1, 0, 0)))
+      if (pn >= PR(3)) a.mi[3][p] = a.medium;
+    // Hy: (i, j-1..j, k)
+    if (i < g.nx && k < g.nz && (v000 || IN(0, -1, 0)))
+      if (pn >= PR(4)) a.mi[4][p] = a.medium;
+    // Hz: (i, j, k-1..k)
+    if (i < g.nx && j < g.ny && (v000 || IN(0, 0, -1)))
+      if (pn >= PR(5)) a.mi[5][p] = a.medium;
+#undef IN
+#undef PR
+  }
+}
+
+int launch_paint(const DevGrid &g, int16_t *mi[6], int shape, const float *par,
+                 int16_t medium, const int32_t *prio, int nprio, void *stream) {
+  PaintArgs a;
+  a.nprio = nprio;
+  for (int c = 0; c < 6; ++c) a.mi[c] = mi[c];
+  for (int t = 0; t < 12; ++t) a.par[t] = par[t];
+  a.shape = shape;
+  a.medium = medium;
+  a.ox = (int)par[6]; a.oy = (int)par[7]; a.oz = (int)par[8];
+  a.sx = (int)par[9]; a.sy = (int)par[10]; a.sz = (int)par[11];
+  if (a.sx <= 0 || a.sy <= 0 || a.sz <= 0) return 0;
+  hipLaunchKernelGGL(k_paint, dim3(8192), dim3(256), 0, (hipStream_t)stream, g, a, prio);
+  return (int)hipGetLastError();
+}
+
+}  // namespace sfd

@@ -1,0 +1,880 @@
+// fdtd_host.cpp -- C ABI of the gfx950 field time-stepping library: device
+// memory, set-up uploads, launches and the captured step graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fdtd_internal.h"
+#include "sembafdtd.h"
+
+using namespace sfd;
+
+namespace {
+thread_local std::string g_create_error;
+
+struct Source {
+  int comp, hard;
+  int box[6];
+  float amp, deltaevol;
+  int numus;
+  std::vector<float> evol;
+};
+}  // namespace
+
+struct fdtd_handle {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int mode = FDTD_MODE_FAST;
+  int storage = FDTD_STORE_FP32;
+  DevGrid g{};
+  TileMap tm{};
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+  float *fld[6] = {};
+  int16_t *mi[6] = {};
+  // media
+  int nmedia = -1;
+  int media_max = 0;  // largest media index written so far
+  std::vector<float> g2h, gm2h;
+  std::vector<int32_t> kindh;
+  float *dg[4] = {};
+  // spacing
+  std::vector<float> idh_[6];
+  float *did[6] = {};
+  // cpml
+  float *dprof[12] = {};
+  // dispersive
+  int ndisp = 0;
+  std::vector<int32_t> dmed, dnp;
+  std::vector<double> dk, db, dg3;
+  bool ade_ready = false;
+  std::vector<void *> ade_allocs;
+  // time
+  double dt = 0.0;
+  std::vector<double> tiempo;
+  // nodal
+  std::vector<Source> srcs;
+  bool nodal_ready = false;
+  std::vector<void *> nodal_allocs;
+  NodalCell *dcells[2] = {};
+  int ncells[2] = {0, 0};
+  int serial[2] = {0, 0};
+  float *dvalues = nullptr;
+  int nvalues_steps = 0;
+  // probes
+  ProbeDev *dprobes = nullptr;
+  int nprobe = 0;
+  float *dpbuf = nullptr;
+  int *dpcount = nullptr;
+  int pcap = 0;
+  // graph
+  int *dstep = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool graph_ok = false;
+};
+
+namespace {
+
+int fail(fdtd_handle *h, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+int fail(fdtd_handle *h, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf; else g_create_error = buf;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(h, FDTD_ERR_HIP, "%s: %s (%s:%d)", #call,                    \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                  \
+  } while (0)
+
+int dalloc(fdtd_handle *h, void **p, size_t bytes, std::vector<void *> *list = nullptr) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return fail(h, FDTD_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  (list ? list : &h->allocs)->push_back(*p);
+  h->bytes += (int64_t)bytes;
+  return FDTD_OK;
+}
+
+int dzero(fdtd_handle *h, void *p, size_t bytes) {
+  HIPCHK(h, hipMemsetAsync(p, 0, bytes, h->stream));
+  return FDTD_OK;
+}
+
+template <class T>
+int upload(fdtd_handle *h, T **dst, const T *src, size_t n,
+           std::vector<void *> *list = nullptr) {
+  if (!*dst) {
+    int rc = dalloc(h, (void **)dst, n * sizeof(T), list);
+    if (rc) return rc;
+  }
+  HIPCHK(h, hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FDTD_OK;
+}
+
+size_t field_elems(const fdtd_handle *h) { return (size_t)h->g.pxy * (size_t)(h->g.nz + 3); }
+
+int axis_n(const fdtd_handle *h, int a) { return a == 0 ? h->g.nx : (a == 1 ? h->g.ny : h->g.nz); }
+
+int check_ready(fdtd_handle *h) {
+  if (h->nmedia < 0) return fail(h, FDTD_ERR_STATE, "fdtd_set_media not called");
+  if (h->media_max > h->nmedia)
+    return fail(h, FDTD_ERR_STATE, "media index %d written but only %d media set", h->media_max, h->nmedia);
+  for (int c = 0; c < 6; ++c)
+    if (!h->did[c]) return fail(h, FDTD_ERR_STATE, "fdtd_set_spacing not called");
+  return FDTD_OK;
+}
+
+void invalidate_graph(fdtd_handle *h) {
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  h->graph_ok = false;
+}
+
+// ---- compact pole state --------------------------------------------------
+int build_ade(fdtd_handle *h) {
+  if (h->ade_ready) return FDTD_OK;
+  for (void *p : h->ade_allocs) hipFree(p);
+  h->ade_allocs.clear();
+  h->g.has_ade = 0;
+  if (h->ndisp == 0) { h->ade_ready = true; return FDTD_OK; }
+  const int nm = h->nmedia;
+  std::vector<int32_t> slot(nm + 1, -1);
+  for (int d = 0; d < h->ndisp; ++d) {
+    if (h->dmed[d] < 1 || h->dmed[d] > nm)
+      return fail(h, FDTD_ERR_ARG, "dispersive medium %d out of range", h->dmed[d]);
+    slot[h->dmed[d]] = d;
+  }
+  int32_t *dslot = nullptr;
+  int rc = upload(h, &dslot, slot.data(), slot.size(), &h->ade_allocs);
+  if (rc) return rc;
+  h->g.dslot = dslot;
+  // coefficients
+  std::vector<int32_t> anp(h->ndisp);
+  std::vector<float> cf((size_t)h->ndisp * kPMax * 6, 0.0f);
+  std::vector<double> cd((size_t)h->ndisp * kPMax * 6, 0.0);
+  for (int d = 0; d < h->ndisp; ++d) {
+    const int np = h->dnp[d];
+    const double *K = &h->dk[(size_t)d * kPMax * 2], *B = &h->db[(size_t)d * kPMax * 2],
+                 *G = &h->dg3[(size_t)d * kPMax * 2];
+    for (int q = 0; q < np; ++q) {
+      double *o = &cd[((size_t)d * kPMax + q) * 6];
+      o[0] = K[2 * q]; o[1] = K[2 * q + 1]; o[2] = B[2 * q]; o[3] = B[2 * q + 1];
+      o[4] = G[2 * q]; o[5] = G[2 * q + 1];
+    }
+    if (h->mode == FDTD_MODE_EXACT) { anp[d] = np; continue; }
+    // FAST: fold complex-conjugate pairs (their states stay conjugate, so the
+    // pair contributes 2 Re(G3 J) of one state); beta is pre-divided by dt.
+    std::vector<int> used(np, 0);
+    int m = 0;
+    for (int q = 0; q < np; ++q) {
+      if (used[q]) continue;
+      used[q] = 1;
+      double w = 1.0;
+      for (int r = q + 1; r < np; ++r)
+        if (!used[r] && K[2 * r] == K[2 * q] && K[2 * r + 1] == -K[2 * q + 1] &&
+            B[2 * r] == B[2 * q] && B[2 * r + 1] == -B[2 * q + 1] &&
+            G[2 * r] == G[2 * q] && G[2 * r + 1] == -G[2 * q + 1] && K[2 * q + 1] != 0.0) {
+          used[r] = 1;
+          w = 2.0;
+          break;
+        }
+      float *o = &cf[((size_t)d * kPMax + m) * 6];
+      o[0] = (float)K[2 * q]; o[1] = (float)K[2 * q + 1];
+      o[2] = (float)(B[2 * q] / h->dt); o[3] = (float)(B[2 * q + 1] / h->dt);
+      o[4] = (float)(w * G[2 * q]); o[5] = (float)(w * G[2 * q + 1]);
+      ++m;
+    }
+    anp[d] = m;
+  }
+  int32_t *danp = nullptr;
+  if ((rc = upload(h, &danp, anp.data(), anp.size(), &h->ade_allocs))) return rc;
+  h->g.anp = danp;
+  if (h->mode == FDTD_MODE_EXACT) {
+    double *p = nullptr;
+    if ((rc = upload(h, &p, cd.data(), cd.size(), &h->ade_allocs))) return rc;
+    h->g.aded = p;
+  } else {
+    float *p = nullptr;
+    if ((rc = upload(h, &p, cf.data(), cf.size(), &h->ade_allocs))) return rc;
+    h->g.adef = p;
+  }
+  int maxp = 0;
+  for (int d = 0; d < h->ndisp; ++d) maxp = std::max(maxp, anp[d]);
+  // segment offsets per E component
+  h->g.nseg = (h->g.nx + 1 + kWave - 1) / kWave;
+  for (int c = 0; c < 3; ++c) {
+    const int yl = c == 1 ? h->g.ny - 1 : h->g.ny;
+    const int zl = c == 2 ? h->g.nz - 1 : h->g.nz;
+    const long long nseg_tot = (long long)(yl + 1) * (zl + 1) * h->g.nseg;
+    if (nseg_tot + 1 > 0x7fffffffLL) return fail(h, FDTD_ERR_ARG, "grid too large for segment table");
+    int32_t *counts = nullptr, *off = nullptr;
+    if ((rc = dalloc(h, (void **)&counts, (nseg_tot + 1) * 4, &h->ade_allocs))) return rc;
+    if ((rc = dalloc(h, (void **)&off, (nseg_tot + 1) * 4, &h->ade_allocs))) return rc;
+    if ((rc = dzero(h, counts, (nseg_tot + 1) * 4))) return rc;
+    HIPCHK(h, (hipError_t)launch_count_disp(h->g, c, counts, h->stream));
+    size_t tb = 0;
+    HIPCHK(h, (hipError_t)exclusive_scan_i32(counts, off, (int)nseg_tot + 1, nullptr, &tb, h->stream));
+    void *tmp = nullptr;
+    if ((rc = dalloc(h, &tmp, tb, &h->ade_allocs))) return rc;
+    HIPCHK(h, (hipError_t)exclusive_scan_i32(counts, off, (int)nseg_tot + 1, tmp, &tb, h->stream));
+    int32_t total = 0;
+    HIPCHK(h, hipMemcpyAsync(&total, off + nseg_tot, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->g.segoff[c] = off;
+    h->g.nstate[c] = total;
+    float *ep = nullptr;
+    void *js = nullptr;
+    const size_t jbytes = (size_t)std::max(total, 1) * maxp *
+                          (h->mode == FDTD_MODE_EXACT ? 16 : 8);
+    if ((rc = dalloc(h, (void **)&ep, (size_t)std::max(total, 1) * 4, &h->ade_allocs))) return rc;
+    if ((rc = dalloc(h, &js, jbytes, &h->ade_allocs))) return rc;
+    if ((rc = dzero(h, ep, (size_t)std::max(total, 1) * 4))) return rc;
+    if ((rc = dzero(h, js, jbytes))) return rc;
+    h->g.eprev[c] = ep;
+    h->g.jst[c] = js;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->g.has_ade = 1;
+  h->ade_ready = true;
+  return FDTD_OK;
+}
+
+// ---- nodal sources --------------------------------------------------------
+int build_nodal(fdtd_handle *h) {
+  if (h->nodal_ready) return FDTD_OK;
+  if (h->srcs.empty()) { h->nodal_ready = true; return FDTD_OK; }
+  if (h->tiempo.empty()) return fail(h, FDTD_ERR_STATE, "fdtd_set_time not called");
+  std::vector<NodalCell> cells[2];
+  for (int half = 0; half < 2; ++half) {
+    for (int c = 3 * half; c < 3 * half + 3; ++c)
+      for (int hard = 1; hard >= 0; --hard)
+        for (size_t s = 0; s < h->srcs.size(); ++s) {
+          const Source &S = h->srcs[s];
+          if (S.comp != c || S.hard != hard) continue;
+          for (int k = S.box[4]; k <= S.box[5]; ++k)
+            for (int j = S.box[2]; j <= S.box[3]; ++j)
+              for (int i = S.box[0]; i <= S.box[1]; ++i) {
+                NodalCell nc{};
+                nc.p = (long long)(k + 1) * h->g.pxy + (long long)(j + 1) * h->g.px + kXOff + i;
+                nc.comp = c;
+                nc.hard = hard;
+                nc.src = (int)s;
+                nc.amp = S.amp;
+                // media read by the reference: own matrix, except hard Hy/Hz
+                // which read sggMiHx (nodalsources.F90:641,715)
+                const int mc = (c >= 4 && hard) ? 3 : c;
+                int16_t m = 0;
+                HIPCHK(h, hipMemcpy(&m, h->mi[mc] + nc.p, 2, hipMemcpyDeviceToHost));
+                const int skipbit = c < 3 ? FDTD_KIND_PEC : FDTD_KIND_PMC;
+                nc.skip = (h->kindh[m] & skipbit) ? 1 : 0;
+                if (!hard) {
+                  const float *Ix = c < 3 ? h->idh_[3].data() : h->idh_[0].data();
+                  const float *Iy = c < 3 ? h->idh_[4].data() : h->idh_[1].data();
+                  const float *Iz = c < 3 ? h->idh_[5].data() : h->idh_[2].data();
+                  const float gg = c < 3 ? h->g2h[m] : h->gm2h[m];
+                  float ia, ib;
+                  switch (c % 3) {
+                    case 0: ia = Iy[j + 1]; ib = Iz[k + 1]; break;
+                    case 1: ia = Ix[i + 1]; ib = Iz[k + 1]; break;
+                    default: ia = Iy[j + 1]; ib = Ix[i + 1]; break;
+                  }
+                  nc.coef = gg * ia * ib;
+                }
+                cells[half].push_back(nc);
+              }
+        }
+  }
+  // excitation values per step (real(4) time stamps: E tiempo(n), H
+  // tiempo(n)+0.5*dt; nodalsources.F90:426,618)
+  const int nsteps = (int)h->tiempo.size() - 1;
+  const int ns = (int)h->srcs.size();
+  std::vector<float> vals((size_t)(nsteps + 1) * ns);
+  for (int n = 0; n <= nsteps; ++n)
+    for (int s = 0; s < ns; ++s) {
+      const Source &S = h->srcs[s];
+      const float t = S.comp < 3 ? (float)h->tiempo[n]
+                                 : (float)(h->tiempo[n] + (double)0.5f * h->dt);
+      vals[(size_t)n * ns + s] = fdtd_evolucion(t, S.deltaevol, S.numus, S.evol.data());
+    }
+  int rc;
+  for (void *p : h->nodal_allocs) hipFree(p);
+  h->nodal_allocs.clear();
+  h->dvalues = nullptr;
+  h->dcells[0] = h->dcells[1] = nullptr;
+  if ((rc = dalloc(h, (void **)&h->dvalues, vals.size() * 4, &h->nodal_allocs))) return rc;
+  HIPCHK(h, hipMemcpy(h->dvalues, vals.data(), vals.size() * 4, hipMemcpyHostToDevice));
+  h->nvalues_steps = nsteps + 1;
+  for (int half = 0; half < 2; ++half) {
+    h->ncells[half] = (int)cells[half].size();
+    // the reference applies the cells in order; run them serially when two
+    // entries touch the same field value
+    std::vector<std::pair<long long, int>> key;
+    for (auto &c : cells[half]) key.push_back({c.p, c.comp});
+    std::sort(key.begin(), key.end());
+    h->serial[half] = std::adjacent_find(key.begin(), key.end()) != key.end();
+    h->dcells[half] = nullptr;
+    if (!cells[half].empty()) {
+      if ((rc = dalloc(h, (void **)&h->dcells[half], cells[half].size() * sizeof(NodalCell),
+                       &h->nodal_allocs))) return rc;
+      HIPCHK(h, hipMemcpy(h->dcells[half], cells[half].data(),
+                          cells[half].size() * sizeof(NodalCell), hipMemcpyHostToDevice));
+    }
+  }
+  h->nodal_ready = true;
+  return FDTD_OK;
+}
+
+int prepare(fdtd_handle *h) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if ((rc = build_ade(h))) return rc;
+  if ((rc = build_nodal(h))) return rc;
+  return FDTD_OK;
+}
+
+int run_e(fdtd_handle *h, int yee, int cpml, int ade) {
+  const int c = cpml && h->g.has_cpml;
+  const int a = ade && h->g.has_ade;
+  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_e(h->g, h->tm, yee, c, a, h->stream)
+                                     : fast::launch_e(h->g, h->tm, yee, c, a, h->stream);
+  if (e) return fail(h, FDTD_ERR_HIP, "E pass launch failed: %s", hipGetErrorString((hipError_t)e));
+  return FDTD_OK;
+}
+
+int run_h(fdtd_handle *h, int yee, int cpml) {
+  const int c = cpml && h->g.has_cpml;
+  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_h(h->g, h->tm, yee, c, h->stream)
+                                     : fast::launch_h(h->g, h->tm, yee, c, h->stream);
+  if (e) return fail(h, FDTD_ERR_HIP, "H pass launch failed: %s", hipGetErrorString((hipError_t)e));
+  return FDTD_OK;
+}
+
+int run_nodal(fdtd_handle *h, int half, int n, const int *dn) {
+  if (h->ncells[half] == 0) return FDTD_OK;
+  if (!dn && (n < 0 || n >= h->nvalues_steps))
+    return fail(h, FDTD_ERR_ARG, "step %d outside the time vector (0..%d)", n, h->nvalues_steps - 1);
+  int e = launch_nodal(h->g, h->dcells[half], h->ncells[half], h->dvalues, (int)h->srcs.size(),
+                       n, dn, h->serial[half], h->stream);
+  if (e) return fail(h, FDTD_ERR_HIP, "nodal launch failed");
+  return FDTD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdtd_abi_version(void) { return FDTD_ABI_VERSION; }
+
+const char *fdtd_last_error(const fdtd_handle *h) {
+  return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
+  if (!d || !out) return fail(nullptr, FDTD_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1)
+    return fail(nullptr, FDTD_ERR_ARG, "grid %dx%dx%d must be at least 1 cell per axis", d->nx, d->ny, d->nz);
+  if (d->mode != FDTD_MODE_FAST && d->mode != FDTD_MODE_EXACT)
+    return fail(nullptr, FDTD_ERR_ARG, "unknown mode %d", d->mode);
+  if (d->storage != FDTD_STORE_FP32)
+    return fail(nullptr, FDTD_ERR_ARG, "storage %d not available in this build", d->storage);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, FDTD_ERR_DEVICE, "no HIP device available");
+  if (d->device < 0 || d->device >= ndev)
+    return fail(nullptr, FDTD_ERR_DEVICE, "device %d out of range (%d devices)", d->device, ndev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, d->device) != hipSuccess)
+    return fail(nullptr, FDTD_ERR_DEVICE, "cannot query device %d", d->device);
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(nullptr, FDTD_ERR_DEVICE, "device %d is %s; this library is built for gfx950",
+                d->device, prop.gcnArchName);
+  fdtd_handle *h = new fdtd_handle();
+  h->device = d->device;
+  h->mode = d->mode;
+  h->storage = d->storage;
+  if (hipSetDevice(d->device) != hipSuccess) { delete h; return fail(nullptr, FDTD_ERR_DEVICE, "hipSetDevice failed"); }
+  if (d->stream) {
+    h->stream = (hipStream_t)d->stream;
+  } else {
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete h;
+      return fail(nullptr, FDTD_ERR_HIP, "hipStreamCreate failed");
+    }
+    h->own_stream = true;
+  }
+  DevGrid &g = h->g;
+  g.nx = d->nx; g.ny = d->ny; g.nz = d->nz;
+  g.px = ((long long)kXOff + d->nx + 2 + 31) / 32 * 32;
+  g.pxy = g.px * (long long)(d->ny + 3);
+  const size_t ne = field_elems(h);
+  int rc;
+  for (int c = 0; c < 6; ++c) {
+    if ((rc = dalloc(h, (void **)&h->fld[c], ne * 4)) ||
+        (rc = dalloc(h, (void **)&h->mi[c], ne * 2)) || (rc = dzero(h, h->fld[c], ne * 4)) ||
+        (rc = dzero(h, h->mi[c], ne * 2))) {
+      g_create_error = h->err;
+      fdtd_destroy(h);
+      return rc;
+    }
+    g.f[c] = h->fld[c];
+    g.mi[c] = h->mi[c];
+  }
+  if ((rc = dalloc(h, (void **)&h->dstep, 16)) || (rc = dzero(h, h->dstep, 16))) {
+    g_create_error = h->err;
+    fdtd_destroy(h);
+    return rc;
+  }
+  // tiles: 64 x 4 threads, z chunks of 64 planes (or fewer for thin grids)
+  TileMap &tm = h->tm;
+  tm.tx = (d->nx + 1 + kWave - 1) / kWave;
+  tm.ty = (d->ny + 1 + 3) / 4;
+  tm.kc = std::min(64, d->nz + 1);
+  tm.tz = (d->nz + 1 + tm.kc - 1) / tm.kc;
+  tm.nblk = tm.tx * tm.ty * tm.tz;
+  tm.remap = 1;
+  hipStreamSynchronize(h->stream);
+  *out = h;
+  return FDTD_OK;
+}
+
+int fdtd_destroy(fdtd_handle *h) {
+  if (!h) return FDTD_OK;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  invalidate_graph(h);
+  for (void *p : h->ade_allocs) hipFree(p);
+  for (void *p : h->nodal_allocs) hipFree(p);
+  for (void *p : h->allocs) hipFree(p);
+  if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return FDTD_OK;
+}
+
+int fdtd_set_media(fdtd_handle *h, int32_t nmedia, const float *g1, const float *g2,
+                   const float *gm1, const float *gm2, const int32_t *kind) {
+  if (!h || nmedia < 0 || nmedia > 32767 || !g1 || !g2 || !gm1 || !gm2 || !kind)
+    return fail(h, FDTD_ERR_ARG, "bad media tables");
+  if (h->media_max > nmedia)
+    return fail(h, FDTD_ERR_ARG, "media index %d already in use, %d media given", h->media_max, nmedia);
+  hipSetDevice(h->device);
+  if (h->nmedia != nmedia) {
+    for (int t = 0; t < 4; ++t) h->dg[t] = nullptr;  // old buffers stay in allocs until destroy
+  }
+  const float *src[4] = {g1, g2, gm1, gm2};
+  int rc;
+  for (int t = 0; t < 4; ++t)
+    if ((rc = upload(h, &h->dg[t], src[t], (size_t)nmedia + 1))) return rc;
+  h->g.g1 = h->dg[0]; h->g.g2 = h->dg[1]; h->g.gm1 = h->dg[2]; h->g.gm2 = h->dg[3];
+  h->g2h.assign(g2, g2 + nmedia + 1);
+  h->gm2h.assign(gm2, gm2 + nmedia + 1);
+  h->kindh.assign(kind, kind + nmedia + 1);
+  h->nmedia = nmedia;
+  h->g.nmedia = nmedia;
+  h->ade_ready = false;
+  h->nodal_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_set_media_index(fdtd_handle *h, int32_t comp, const int16_t *mi) {
+  if (!h || comp < 0 || comp > 5 || !mi) return fail(h, FDTD_ERR_ARG, "bad media index arguments");
+  hipSetDevice(h->device);
+  const DevGrid &g = h->g;
+  int16_t mx = 0;
+  const long long tot = (long long)(g.nx + 3) * (g.ny + 3) * (g.nz + 3);
+  for (long long t = 0; t < tot; ++t) {
+    if (mi[t] < 0) return fail(h, FDTD_ERR_ARG, "negative media index at %lld", t);
+    mx = std::max(mx, mi[t]);
+  }
+  if (h->nmedia >= 0 && mx > h->nmedia)
+    return fail(h, FDTD_ERR_ARG, "media index %d outside 0..%d", mx, h->nmedia);
+  h->media_max = std::max(h->media_max, (int)mx);
+  HIPCHK(h, hipMemcpy2DAsync(h->mi[comp] + kXOff - 1, (size_t)g.px * 2, mi, (size_t)(g.nx + 3) * 2,
+                             (size_t)(g.nx + 3) * 2, (size_t)(g.ny + 3) * (g.nz + 3),
+                             hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->ade_ready = false;
+  h->nodal_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_set_spacing(fdtd_handle *h, const float *idxe, const float *idye, const float *idze,
+                     const float *idxh, const float *idyh, const float *idzh) {
+  if (!h || !idxe || !idye || !idze || !idxh || !idyh || !idzh)
+    return fail(h, FDTD_ERR_ARG, "null spacing table");
+  hipSetDevice(h->device);
+  const float *src[6] = {idxe, idye, idze, idxh, idyh, idzh};
+  int rc;
+  for (int c = 0; c < 6; ++c) {
+    const int n = axis_n(h, c % 3) + 3;
+    if ((rc = upload(h, &h->did[c], src[c], (size_t)n))) return rc;
+    h->g.id[c] = h->did[c];
+    h->idh_[c].assign(src[c], src[c] + n);
+  }
+  h->nodal_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_set_cpml(fdtd_handle *h, const int32_t layers[6], const float *pbe_x,
+                  const float *pce_x, const float *pbm_x, const float *pcm_x,
+                  const float *pbe_y, const float *pce_y, const float *pbm_y,
+                  const float *pcm_y, const float *pbe_z, const float *pce_z,
+                  const float *pbm_z, const float *pcm_z) {
+  if (!h || !layers) return fail(h, FDTD_ERR_ARG, "null CPML arguments");
+  hipSetDevice(h->device);
+  DevGrid &g = h->g;
+  const float *prof[12] = {pbe_x, pce_x, pbm_x, pcm_x, pbe_y, pce_y,
+                           pbm_y, pcm_y, pbe_z, pce_z, pbm_z, pcm_z};
+  bool any = false;
+  for (int a = 0; a < 3; ++a) {
+    const int n = axis_n(h, a);
+    if (layers[2 * a] < 0 || layers[2 * a + 1] < 0 || layers[2 * a] + layers[2 * a + 1] > n)
+      return fail(h, FDTD_ERR_ARG, "CPML layers on axis %d exceed the grid", a);
+    g.L[a][0] = layers[2 * a];
+    g.L[a][1] = layers[2 * a + 1];
+    any = any || layers[2 * a] || layers[2 * a + 1];
+    int rc;
+    for (int t = 0; t < 4; ++t) {
+      if (!prof[4 * a + t]) return fail(h, FDTD_ERR_ARG, "null CPML profile");
+      if ((rc = upload(h, &h->dprof[4 * a + t], prof[4 * a + t], (size_t)n + 3))) return rc;
+    }
+    g.pbe[a] = h->dprof[4 * a]; g.pce[a] = h->dprof[4 * a + 1];
+    g.pbm[a] = h->dprof[4 * a + 2]; g.pcm[a] = h->dprof[4 * a + 3];
+  }
+  // psi slab arrays (component sweep on the two other axes, Llo+Lhi along
+  // the stretched one); slot order matches PSI_*
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int Lx = g.L[0][0] + g.L[0][1], Ly = g.L[1][0] + g.L[1][1], Lz = g.L[2][0] + g.L[2][1];
+  const long long len[12] = {
+      (long long)nx * Ly * (nz + 1),        // Exy
+      (long long)nx * (ny + 1) * Lz,        // Exz
+      (long long)Lx * ny * (nz + 1),        // Eyx
+      (long long)(nx + 1) * ny * Lz,        // Eyz
+      (long long)Lx * (ny + 1) * nz,        // Ezx
+      (long long)(nx + 1) * Ly * nz,        // Ezy
+      (long long)(nx + 1) * Ly * nz,        // Hxy
+      (long long)(nx + 1) * ny * Lz,        // Hxz
+      (long long)Lx * (ny + 1) * nz,        // Hyx
+      (long long)nx * (ny + 1) * Lz,        // Hyz
+      (long long)Lx * ny * (nz + 1),        // Hzx
+      (long long)nx * Ly * (nz + 1)};       // Hzy
+  for (int s = 0; s < 12; ++s) {
+    int rc;
+    if (!g.psi[s] && len[s] > 0) {
+      if ((rc = dalloc(h, (void **)&g.psi[s], (size_t)len[s] * 4))) return rc;
+    }
+    if (len[s] > 0 && (rc = dzero(h, g.psi[s], (size_t)len[s] * 4))) return rc;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  g.has_cpml = any ? 1 : 0;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_set_edispersive(fdtd_handle *h, int32_t ndisp, const int32_t *medium,
+                         const int32_t *npoles, const double *kappa, const double *beta,
+                         const double *g3) {
+  if (!h || ndisp < 0 || (ndisp > 0 && (!medium || !npoles || !kappa || !beta || !g3)))
+    return fail(h, FDTD_ERR_ARG, "bad dispersive arguments");
+  for (int d = 0; d < ndisp; ++d)
+    if (npoles[d] < 0 || npoles[d] > kPMax)
+      return fail(h, FDTD_ERR_ARG, "medium %d has %d poles (max %d)", medium[d], npoles[d], kPMax);
+  h->ndisp = ndisp;
+  h->dmed.assign(medium, medium + ndisp);
+  h->dnp.assign(npoles, npoles + ndisp);
+  h->dk.assign(kappa, kappa + (size_t)ndisp * kPMax * 2);
+  h->db.assign(beta, beta + (size_t)ndisp * kPMax * 2);
+  h->dg3.assign(g3, g3 + (size_t)ndisp * kPMax * 2);
+  h->ade_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_set_time(fdtd_handle *h, double dt, int32_t nlast, const double *tiempo) {
+  if (!h || !(dt > 0.0) || nlast < 0) return fail(h, FDTD_ERR_ARG, "bad time arguments");
+  h->dt = dt;
+  h->g.dt = dt;
+  h->tiempo.assign((size_t)nlast + 1, 0.0);
+  if (tiempo) {
+    std::copy(tiempo, tiempo + nlast + 1, h->tiempo.begin());
+  } else {
+    for (int32_t i = 1; i <= nlast; ++i) h->tiempo[i] = h->tiempo[i - 1] + dt;
+  }
+  h->ade_ready = false;  // FAST coefficients pre-divide by dt
+  h->nodal_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_add_nodal(fdtd_handle *h, int32_t comp, int32_t hard, const int32_t box[6], float amp,
+                   float deltaevol, int32_t numus, const float *evol) {
+  if (!h || comp < 0 || comp > 5 || !box || numus < 0 || !evol || !(deltaevol > 0.0f))
+    return fail(h, FDTD_ERR_ARG, "bad nodal source");
+  const int n[3] = {h->g.nx, h->g.ny, h->g.nz};
+  for (int a = 0; a < 3; ++a) {
+    const bool node = comp < 3 ? (comp != a) : (comp - 3 == a);
+    const int hi = node ? n[a] : n[a] - 1;
+    if (box[2 * a] < 0 || box[2 * a + 1] > hi || box[2 * a] > box[2 * a + 1])
+      return fail(h, FDTD_ERR_ARG, "nodal box outside the sweep of component %d", comp);
+  }
+  Source s;
+  s.comp = comp; s.hard = hard ? 1 : 0;
+  std::copy(box, box + 6, s.box);
+  s.amp = amp; s.deltaevol = deltaevol; s.numus = numus;
+  s.evol.assign(evol, evol + numus + 1);
+  h->srcs.push_back(std::move(s));
+  h->nodal_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_advance_e(fdtd_handle *h, int32_t n) {
+  if (!h) return FDTD_ERR_ARG;
+  hipSetDevice(h->device);
+  int rc;
+  if ((rc = prepare(h)) || (rc = run_e(h, 1, 1, 1)) || (rc = run_nodal(h, 0, n, nullptr))) return rc;
+  return FDTD_OK;
+}
+
+int fdtd_advance_h(fdtd_handle *h, int32_t n) {
+  if (!h) return FDTD_ERR_ARG;
+  hipSetDevice(h->device);
+  int rc;
+  if ((rc = prepare(h)) || (rc = run_h(h, 1, 1)) || (rc = run_nodal(h, 1, n, nullptr))) return rc;
+  return FDTD_OK;
+}
+
+#define SFD_GRANULAR(name, body)                                               \
+  int name {                                                                   \
+    if (!h) return FDTD_ERR_ARG;                                               \
+    hipSetDevice(h->device);                                                   \
+    int rc = prepare(h);                                                       \
+    if (rc) return rc;                                                         \
+    return body;                                                               \
+  }
+SFD_GRANULAR(fdtd_advance_e_yee(fdtd_handle *h), run_e(h, 1, 0, 0))
+SFD_GRANULAR(fdtd_advance_e_cpml(fdtd_handle *h), run_e(h, 0, 1, 0))
+SFD_GRANULAR(fdtd_advance_e_dispersive(fdtd_handle *h), run_e(h, 0, 0, 1))
+SFD_GRANULAR(fdtd_advance_nodal_e(fdtd_handle *h, int32_t n), run_nodal(h, 0, n, nullptr))
+SFD_GRANULAR(fdtd_advance_h_yee(fdtd_handle *h), run_h(h, 1, 0))
+SFD_GRANULAR(fdtd_advance_h_cpml(fdtd_handle *h), run_h(h, 0, 1))
+SFD_GRANULAR(fdtd_advance_nodal_h(fdtd_handle *h, int32_t n), run_nodal(h, 1, n, nullptr))
+#undef SFD_GRANULAR
+
+int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
+  if (!h || nsteps < 0) return fail(h, FDTD_ERR_ARG, "bad run arguments");
+  hipSetDevice(h->device);
+  int rc = prepare(h);
+  if (rc) return rc;
+  if (nsteps == 0) return FDTD_OK;
+  if (!h->srcs.empty() && n0 + nsteps > h->nvalues_steps)
+    return fail(h, FDTD_ERR_ARG, "steps %d..%d beyond the time vector", n0, n0 + nsteps - 1);
+  // the step counter lives on the device so one captured step replays for
+  // every n
+  HIPCHK(h, hipMemcpyAsync(h->dstep, &n0, 4, hipMemcpyHostToDevice, h->stream));
+  if (!h->graph_ok) {
+    invalidate_graph(h);
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int r2 = run_e(h, 1, 1, 1);
+    if (!r2) r2 = run_nodal(h, 0, 0, h->dstep);
+    if (!r2) r2 = run_h(h, 1, 1);
+    if (!r2) r2 = run_nodal(h, 1, 0, h->dstep);
+    if (!r2 && h->nprobe)
+      r2 = launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap, h->stream)
+               ? fail(h, FDTD_ERR_HIP, "probe launch failed") : 0;
+    if (!r2) r2 = launch_tick(h->dstep, h->stream) ? fail(h, FDTD_ERR_HIP, "tick failed") : 0;
+    hipGraph_t graph = nullptr;
+    hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+    if (r2) { if (graph) hipGraphDestroy(graph); return r2; }
+    if (ec != hipSuccess) return fail(h, FDTD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+    h->graph = graph;
+    HIPCHK(h, hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0));
+    h->graph_ok = true;
+  }
+  for (int32_t s = 0; s < nsteps; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+  return FDTD_OK;
+}
+
+int fdtd_set_probes(fdtd_handle *h, int32_t nprobe, const int32_t *comp, const int32_t *i,
+                    const int32_t *j, const int32_t *k, int32_t bufsteps) {
+  if (!h || nprobe < 0 || bufsteps < 1 || (nprobe && (!comp || !i || !j || !k)))
+    return fail(h, FDTD_ERR_ARG, "bad probe arguments");
+  hipSetDevice(h->device);
+  std::vector<ProbeDev> pr(nprobe);
+  const int n[3] = {h->g.nx, h->g.ny, h->g.nz};
+  for (int p = 0; p < nprobe; ++p) {
+    const int crd[3] = {i[p], j[p], k[p]};
+    if (comp[p] < 0 || comp[p] > 5) return fail(h, FDTD_ERR_ARG, "probe %d: bad component", p);
+    for (int a = 0; a < 3; ++a)
+      if (crd[a] < -1 || crd[a] > n[a] + 1) return fail(h, FDTD_ERR_ARG, "probe %d outside the grid", p);
+    pr[p].comp = comp[p];
+    pr[p].p = (long long)(k[p] + 1) * h->g.pxy + (long long)(j[p] + 1) * h->g.px + kXOff + i[p];
+  }
+  int rc;
+  h->dprobes = nullptr; h->dpbuf = nullptr; h->dpcount = nullptr;
+  if ((rc = dalloc(h, (void **)&h->dprobes, sizeof(ProbeDev) * std::max(nprobe, 1)))) return rc;
+  if ((rc = dalloc(h, (void **)&h->dpbuf, (size_t)bufsteps * std::max(nprobe, 1) * 4))) return rc;
+  if ((rc = dalloc(h, (void **)&h->dpcount, 16))) return rc;
+  if (nprobe) HIPCHK(h, hipMemcpy(h->dprobes, pr.data(), sizeof(ProbeDev) * nprobe, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(h->dpcount, 0, 16));
+  h->nprobe = nprobe;
+  h->pcap = bufsteps;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_sample_probes(fdtd_handle *h) {
+  if (!h) return FDTD_ERR_ARG;
+  if (!h->nprobe) return FDTD_OK;
+  hipSetDevice(h->device);
+  if (launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap, h->stream))
+    return fail(h, FDTD_ERR_HIP, "probe launch failed");
+  return FDTD_OK;
+}
+
+int fdtd_read_probes(fdtd_handle *h, float *out, int32_t max_samples, int32_t *nsamples) {
+  if (!h || !nsamples || (max_samples > 0 && !out)) return fail(h, FDTD_ERR_ARG, "bad probe read");
+  hipSetDevice(h->device);
+  *nsamples = 0;
+  if (!h->nprobe) return FDTD_OK;
+  int cnt = 0;
+  HIPCHK(h, hipMemcpyAsync(&cnt, h->dpcount, 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (cnt > max_samples) return fail(h, FDTD_ERR_ARG, "%d samples buffered, room for %d", cnt, max_samples);
+  if (cnt) HIPCHK(h, hipMemcpy(out, h->dpbuf, (size_t)cnt * h->nprobe * 4, hipMemcpyDeviceToHost));
+  const int zero = 0;
+  HIPCHK(h, hipMemcpyAsync(h->dpcount, &zero, 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  *nsamples = cnt;
+  return FDTD_OK;
+}
+
+int fdtd_get_field(fdtd_handle *h, int32_t comp, float *host) {
+  if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field read");
+  hipSetDevice(h->device);
+  const DevGrid &g = h->g;
+  HIPCHK(h, hipMemcpy2DAsync(host, (size_t)(g.nx + 3) * 4, h->fld[comp] + kXOff - 1, (size_t)g.px * 4,
+                             (size_t)(g.nx + 3) * 4, (size_t)(g.ny + 3) * (g.nz + 3),
+                             hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FDTD_OK;
+}
+
+int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host) {
+  if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field write");
+  hipSetDevice(h->device);
+  const DevGrid &g = h->g;
+  HIPCHK(h, hipMemcpy2DAsync(h->fld[comp] + kXOff - 1, (size_t)g.px * 4, host, (size_t)(g.nx + 3) * 4,
+                             (size_t)(g.nx + 3) * 4, (size_t)(g.ny + 3) * (g.nz + 3),
+                             hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FDTD_OK;
+}
+
+int fdtd_synchronize(fdtd_handle *h) {
+  if (!h) return FDTD_ERR_ARG;
+  hipSetDevice(h->device);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FDTD_OK;
+}
+
+int64_t fdtd_device_bytes(const fdtd_handle *h) { return h ? h->bytes : 0; }
+
+int64_t fdtd_dispersive_edges(const fdtd_handle *h, int32_t comp) {
+  if (!h || comp < 0 || comp > 2 || !h->ade_ready) return -1;
+  return h->g.has_ade ? (int64_t)h->g.nstate[comp] : 0;
+}
+
+int fdtd_media_fill(fdtd_handle *h, int16_t medium) {
+  if (!h || medium < 0) return fail(h, FDTD_ERR_ARG, "bad medium");
+  h->media_max = medium;  // every previous index is overwritten
+  hipSetDevice(h->device);
+  for (int c = 0; c < 6; ++c)
+    if (launch_fill_i16(h->mi[c], (long long)field_elems(h), medium, h->stream))
+      return fail(h, FDTD_ERR_HIP, "fill launch failed");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->ade_ready = false; h->nodal_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_media_pec_walls(fdtd_handle *h, int32_t faces) {
+  if (!h) return FDTD_ERR_ARG;
+  hipSetDevice(h->device);
+  if (launch_pec_walls(h->g, h->mi, faces, h->stream)) return fail(h, FDTD_ERR_HIP, "pec launch failed");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->ade_ready = false; h->nodal_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+static int paint(fdtd_handle *h, int shape, const float par[12], int16_t medium, int32_t nprio,
+                 const int32_t *prio) {
+  if (!h || !prio || medium < 0 || medium >= nprio) return fail(h, FDTD_ERR_ARG, "bad paint arguments");
+  h->media_max = std::max(h->media_max, (int)medium);
+  hipSetDevice(h->device);
+  int32_t *dprio = nullptr;
+  HIPCHK(h, hipMalloc((void **)&dprio, (size_t)nprio * 4));
+  hipError_t e = hipMemcpy(dprio, prio, (size_t)nprio * 4, hipMemcpyHostToDevice);
+  // media indices beyond the priority table are left untouched by the kernel
+  if (e == hipSuccess && launch_paint(h->g, h->mi, shape, par, medium, dprio, nprio, h->stream))
+    e = hipErrorLaunchFailure;
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(dprio);
+  if (e != hipSuccess) return fail(h, FDTD_ERR_HIP, "paint failed: %s", hipGetErrorString(e));
+  h->ade_ready = false; h->nodal_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_media_paint_sphere(fdtd_handle *h, float cx, float cy, float cz, float r, int16_t medium,
+                            int32_t nprio, const int32_t *prio) {
+  if (!h || !(r > 0.0f)) return fail(h, FDTD_ERR_ARG, "bad sphere");
+  if (h->nmedia >= 0 && nprio < h->nmedia + 1) return fail(h, FDTD_ERR_ARG, "priority table too short");
+  float par[12] = {cx, cy, cz, r, 0, 0};
+  const int x0 = std::max(0, (int)std::floor(cx - r) - 1), y0 = std::max(0, (int)std::floor(cy - r) - 1),
+            z0 = std::max(0, (int)std::floor(cz - r) - 1);
+  const int x1 = std::min(h->g.nx, (int)std::ceil(cx + r) + 1), y1 = std::min(h->g.ny, (int)std::ceil(cy + r) + 1),
+            z1 = std::min(h->g.nz, (int)std::ceil(cz + r) + 1);
+  par[6] = (float)x0; par[7] = (float)y0; par[8] = (float)z0;
+  par[9] = (float)(x1 - x0 + 1); par[10] = (float)(y1 - y0 + 1); par[11] = (float)(z1 - z0 + 1);
+  return paint(h, 0, par, medium, nprio, prio);
+}
+
+int fdtd_media_paint_box(fdtd_handle *h, int32_t x0, int32_t x1, int32_t y0, int32_t y1, int32_t z0,
+                         int32_t z1, int16_t medium, int32_t nprio, const int32_t *prio) {
+  if (!h || x1 <= x0 || y1 <= y0 || z1 <= z0) return fail(h, FDTD_ERR_ARG, "bad box");
+  if (h->nmedia >= 0 && nprio < h->nmedia + 1) return fail(h, FDTD_ERR_ARG, "priority table too short");
+  float par[12] = {(float)x0, (float)x1, (float)y0, (float)y1, (float)z0, (float)z1};
+  const int lx = std::max(0, x0), ly = std::max(0, y0), lz = std::max(0, z0);
+  const int hx = std::min(h->g.nx, x1), hy = std::min(h->g.ny, y1), hz = std::min(h->g.nz, z1);
+  par[6] = (float)lx; par[7] = (float)ly; par[8] = (float)lz;
+  par[9] = (float)(hx - lx + 1); par[10] = (float)(hy - ly + 1); par[11] = (float)(hz - lz + 1);
+  return paint(h, 1, par, medium, nprio, prio);
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// fdtd_internal.h -- device data layout shared by the host code and the
+// gfx950 kernels.
+//
+// HBM layout of one grid
+//   fields      6 arrays (Ex Ey Ez Hx Hy Hz), each (nz+3) planes x (ny+3) rows
+//               x px elements; cell (i,j,k) at ((k+1)*(ny+3)+(j+1))*px + xoff+i.
+//               xoff = 32 puts x = 0 on a 128-byte line boundary in every row
+//               (px is a multiple of 32), the Alloc padding planes/rows/columns
+//               (-1 and n+1) stay zero forever.
+//   media idx   6 int16 arrays with the same geometry as the fields.
+//   tables      G1/G2/GM1/GM2 and kind bits per media index; inverse spacings
+//               (length n+3, index i+1); CPML profiles per axis.
+//   psi         12 compact slab arrays (component, stretched axis): along the
+//               stretched axis only the low and high slab layers are stored
+//               (Llo + Lhi entries), the two other axes span the component's
+//               sweep; x-fastest.
+//   pole state  per E component a compact list of the dispersive edges in
+//               (k, j, i) order.  segoff[c][row*nseg + s] is the index of the
+//               first dispersive edge of the 64-wide x segment s of row
+//               (j, k); a wave finds its lanes' slots with one ballot.
+#pragma once
+#include <stdint.h>
+
+namespace sfd {
+
+constexpr int kPMax = 8;     // poles per dispersive medium
+constexpr int kWave = 64;    // CDNA wavefront
+constexpr int kXOff = 32;    // x = 0 offset inside a row (elements)
+
+enum { PSI_EXY = 0, PSI_EXZ, PSI_EYX, PSI_EYZ, PSI_EZX, PSI_EZY,
+       PSI_HXY, PSI_HXZ, PSI_HYX, PSI_HYZ, PSI_HZX, PSI_HZY };
+
+struct DevGrid {
+  int nx, ny, nz;
+  int nmedia;
+  long long px, pxy;            // row pitch and plane stride in elements
+  float *f[6];
+  const int16_t *mi[6];
+  const float *g1, *g2, *gm1, *gm2;
+  const float *id[6];           // Idxe Idye Idze Idxh Idyh Idzh
+  // CPML
+  int L[3][2];
+  const float *pbe[3], *pce[3], *pbm[3], *pcm[3];
+  float *psi[12];
+  int has_cpml;
+  // electric pole-residue state
+  int has_ade;
+  const int32_t *dslot;         // [nmedia+1] dispersive slot or -1
+  int nseg;                     // 64-wide x segments per row
+  const int32_t *segoff[3];
+  float *eprev[3];
+  void *jst[3];                 // FAST: float2[P][N], EXACT: double2[P][N]
+  long long nstate[3];
+  const int32_t *anp;           // per slot: stored poles
+  const float *adef;            // FAST: per slot per pole {kr,ki,bdtr,bdti,g3r,g3i}
+  const double *aded;           // EXACT: per slot per pole {kr,ki,br,bi,g3r,g3i}
+  double dt;
+};
+
+// blocks of (64 x ty) threads march z in chunks of kc planes.  nblk tiles
+// are numbered x fastest; remap spreads contiguous tile ranges over the 8
+// XCDs so that neighbouring tiles share an L2.
+struct TileMap {
+  int tx, ty, tz;   // tiles per axis
+  int kc;           // planes per z chunk
+  int nblk;
+  int remap;
+};
+
+struct NodalCell {   // one cell touched by a nodal source, in reference order
+  long long p;       // field offset
+  int comp;
+  int hard;
+  int src;           // source index (value table column)
+  int skip;          // media is PEC (E) / PMC (H): leave untouched
+  float coef;        // soft: G2*Ia*Ib (already rounded in reference order)
+  float amp;
+};
+
+struct ProbeDev {
+  long long p;
+  int comp;
+};
+
+// launchers, one set per arithmetic mode
+#define SFD_DECLARE_LAUNCHERS(NS)                                              \
+  namespace NS {                                                             \
+  int launch_e(const DevGrid &g, const TileMap &tm, int yee, int cpml,       \
+               int ade, void *stream);                                       \
+  int launch_h(const DevGrid &g, const TileMap &tm, int yee, int cpml,       \
+               void *stream);                                                \
+  }
+SFD_DECLARE_LAUNCHERS(fast)
+SFD_DECLARE_LAUNCHERS(exact)
+
+// mode-independent helper kernels (fdtd_aux.hip)
+int launch_nodal(const DevGrid &g, const NodalCell *cells, int ncells,
+                 const float *values, int nsrc, int n, const int *dn, int serial,
+                 void *stream);
+int launch_tick(int *dn, void *stream);
+int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
+                  int *count, int cap, void *stream);
+int launch_count_disp(const DevGrid &g, int comp, int32_t *counts,
+                      void *stream);
+int launch_fill_i16(int16_t *a, long long n, int16_t v, void *stream);
+int launch_pec_walls(const DevGrid &g, int16_t *mi[6], int faces, void *stream);
+int launch_paint(const DevGrid &g, int16_t *mi[6], int shape, const float *par,
+                 int16_t medium, const int32_t *prio, int nprio, void *stream);
+int exclusive_scan_i32(const int32_t *in, int32_t *out, int n, void *tmp,
+                       size_t *tmp_bytes, void *stream);
+
+}  // namespace sfd

@@ -69,6 +69,12 @@ class Problem:
     eps0: float = EPS0
     mu0: float = MU0
     name: str = ""
+    # device-side scene for grids too large to voxelise on the host: a list of
+    # ("fill", m) / ("pec", faces_mask) / ("sphere", cx, cy, cz, r, m) /
+    # ("box", x0, x1, y0, y1, z0, z1, m) applied in order with priorities
+    # ``prio``; used when ``mi`` is None.
+    scene: list = None
+    prio: np.ndarray = None
 
     def __post_init__(self):
         nm = self.nmedia
@@ -89,8 +95,11 @@ class Problem:
         self.npoles = np.ascontiguousarray(self.npoles, np.int32)
         self.eps11, self.sig11 = f32(self.eps11), f32(self.sig11)
         self.D = [f32(d) for d in self.D]
-        self.mi = np.ascontiguousarray(self.mi, np.int16)
-        assert self.mi.shape == (6,) + self.alloc_shape, (self.mi.shape, self.alloc_shape)
+        if self.mi is None:
+            assert self.scene is not None and self.prio is not None
+        else:
+            self.mi = np.ascontiguousarray(self.mi, np.int16)
+            assert self.mi.shape == (6,) + self.alloc_shape, (self.mi.shape, self.alloc_shape)
         for a, n in enumerate((self.nx, self.ny, self.nz)):
             assert self.D[a].shape == (n + 4,)
 
@@ -109,6 +118,30 @@ class Problem:
     @property
     def cells(self) -> int:
         return self.nx * self.ny * self.nz
+
+    def materialize(self) -> np.ndarray:
+        """Media index arrays of a scene-described problem, built on the host
+        with the same rules the device builders apply."""
+        if self.mi is not None:
+            return self.mi
+        n = (self.nx, self.ny, self.nz)
+        mi = new_media_index(*n)
+        for op in self.scene:
+            kind = op[0]
+            if kind == "fill":
+                mi[:] = op[1]
+            elif kind == "pec":
+                pec_walls(mi, n, tuple(bool(op[1] >> f & 1) for f in range(6)))
+            elif kind == "sphere":
+                paint_voxels(mi, sphere_cells(n, op[1:4], op[4]), op[5], self.prio)
+            elif kind == "box":
+                x0, x1, y0, y1, z0, z1, m = op[1:]
+                inside = np.zeros((self.nz, self.ny, self.nx), bool)
+                inside[max(z0, 0):z1, max(y0, 0):y1, max(x0, 0):x1] = True
+                paint_voxels(mi, inside, m, self.prio)
+            else:
+                raise ValueError(f"unknown scene op {kind}")
+        return mi
 
 
 # --------------------------------------------------------------------------
@@ -206,12 +239,16 @@ def paint_voxels(mi: np.ndarray, inside: np.ndarray, medium: int, prio: np.ndarr
 
 
 def sphere_cells(n, centre, radius) -> np.ndarray:
-    """Voxels whose centre lies inside the sphere (cells indexed (k, j, i))."""
+    """Voxels whose centre lies inside the sphere (cells indexed (k, j, i)),
+    evaluated in real(4) like the device builder."""
     nx, ny, nz = n
-    k, j, i = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5,
-                          indexing="ij")
-    cx, cy, cz = centre
-    return (i - cx) ** 2 + (j - cy) ** 2 + (k - cz) ** 2 <= radius ** 2
+    f = np.float32
+    k, j, i = np.meshgrid(np.arange(nz, dtype=f) + f(0.5), np.arange(ny, dtype=f) + f(0.5),
+                          np.arange(nx, dtype=f) + f(0.5), indexing="ij")
+    cx, cy, cz = (f(c) for c in centre)
+    dx, dy, dz = i - cx, j - cy, k - cz
+    r = f(radius)
+    return dx * dx + dy * dy + dz * dz <= r * r
 
 
 def gaussian_table(dt: float, nsamples: int, spread: float, delay: float) -> np.ndarray:
@@ -255,6 +292,70 @@ def lorentz_pair(f0: float, delta: float, deps: float, eps0: float = EPS0):
     cs = np.array([[c.real, c.imag], [c.real, -c.imag]])
     as_ = np.array([[a.real, a.imag], [a.real, -a.imag]])
     return cs, as_
+
+
+def heterogeneous_problem(n: int = 1024, L: int = 12, seed: int = 1003, nsteps: int = 2000,
+                          dx: float = 0.01, nspheres: int = 8, disp_fraction: float = 0.2,
+                          nprobes: int = 16, host_media: bool = False) -> Problem:
+    """Large heterogeneous single-GPU grid: n^3 total cells with an L-cell
+    CPML on every face, ``nspheres`` seeded voxelised spheres (radius
+    20-120 cells at n = 1024, scaled with n) each of one of 8 lossy
+    dielectrics (eps_r U[1,12], sigma U[0,0.5] S/m), and a 2-pole
+    complex-conjugate pole-residue medium (resonance U[1,10] GHz) filling
+    ``disp_fraction`` of the volume as a slab inside the CPML.  Excited by a
+    soft Ex line source across the slab-free part of the grid.  The media
+    matrices are described as a scene for the device builders
+    (``host_media`` materialises them on the host instead)."""
+    rng = np.random.default_rng(seed)
+    s = n / 1024.0
+    nd = 8
+    epr = [1.0, 1.0] + list(rng.uniform(1.0, 12.0, nd))
+    sig = [0.0, 0.0] + list(rng.uniform(0.0, 0.5, nd))
+    kind = [KIND_PEC, 0] + [0] * nd
+    mdisp = len(epr)
+    epr.append(2.0); sig.append(0.0); kind.append(KIND_EDISP)
+    nm = len(epr) - 1
+    f0 = float(rng.uniform(1e9, 10e9))
+    c, a = lorentz_pair(f0, 0.1 * 2 * np.pi * f0, 2.0)
+    pc = np.zeros((nm + 1, PMAX, 2)); pa = np.zeros((nm + 1, PMAX, 2))
+    pc[mdisp, :2] = c; pa[mdisp, :2] = a
+    npoles = np.zeros(nm + 1, np.int32); npoles[mdisp] = 2
+    eps11 = np.zeros(nm + 1, np.float32); eps11[mdisp] = np.float32(2.0 * EPS0)
+    sig11 = np.zeros(nm + 1, np.float32)
+    prio = np.array([1000, 0] + [20 + i for i in range(nd)] + [10], np.int64)
+    lo, hi = L, n - L
+    inner = hi - lo
+    thick = max(1, int(round(disp_fraction * n ** 3 / inner ** 2))) if disp_fraction > 0 else 0
+    zc = (lo + hi) // 2
+    scene = [("fill", 1)]
+    if thick:
+        z0 = max(lo, zc - thick // 2)
+        scene.append(("box", lo, hi, lo, hi, z0, min(hi, z0 + thick), mdisp))
+    for b in range(nspheres):
+        r = max(2.0, float(rng.uniform(20.0, 120.0)) * s)
+        r = min(r, 0.45 * inner)
+        ctr = [float(rng.uniform(lo + r + 1, hi - r - 1)) for _ in range(3)]
+        scene.append(("sphere", ctr[0], ctr[1], ctr[2], r, 2 + b % nd))
+    D = [uniform_D(n, dx)] * 3
+    dt = courant_dt((dx, dx, dx), cfl=0.9)
+    evol = gaussian_table(dt, nsteps + 4, 30.0, 120.0)
+    ys = lo + max(1, inner // 8)
+    zs = lo + max(1, inner // 8)
+    src = NodalSource(EX, False, (lo + 1, hi - 2, ys, ys, zs, zs), 1.0, evol,
+                      float(np.float32(dt)))
+    probes = []
+    for p in range(nprobes):
+        cc = p % 6
+        probes.append((cc,) + tuple(int(rng.integers(lo, hi)) for _ in range(3)))
+    pb = Problem(n, n, n, epr=epr, sigma=sig, mur=[1.0] * (nm + 1), sigmam=[0.0] * (nm + 1),
+                 kind=kind, mi=None, D=D, dt=dt, npoles=npoles, pole_c=pc, pole_a=pa,
+                 eps11=eps11, sig11=sig11, cpml_layers=[[L, L]] * 3,
+                 cpml_order=[[3.0, 3.0]] * 3, cpml_refl=[[1e-6, 1e-6]] * 3, nodal=[src],
+                 probes=probes, nsteps_max=nsteps, name=f"heterogeneous{n}", scene=scene,
+                 prio=prio)
+    if host_media:
+        pb.mi = pb.materialize()
+    return pb
 
 
 def scatterer_problem(n=(24, 22, 20), L=((4, 5), (3, 4), (5, 3)), seed: int = 7,

@@ -1,0 +1,260 @@
+"""Host-side mirror of semba-fdtd's ``launch_simulation`` for the time-stepping
+hot path, driving the gfx950 library through its C ABI.
+
+Set-up follows the reference order (timestepping.F90:305-1014): spacings,
+calc_G1G2Gm1Gm2, InitCPMLBorders (which stretches the inverse spacings),
+InitEDispersives (which overrides G1/G2 of the dispersive media), nodal
+sources, probes.  The step methods keep the reference's call structure:
+``advance_e(n)`` / ``advance_h(n)`` are the E and H halves of one iteration of
+``ciclo_temporal`` (timestepping.F90:1248-1981) and ``run`` replays whole steps
+from a captured HIP graph.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List
+import numpy as np
+
+from . import _lib
+from ._lib import FdtdError, f32p, f64p, i16p, i32p, ptr
+from .problem import Problem, PMAX, KIND_EDISP
+
+MODES = {"fast": _lib.MODE_FAST, "exact": _lib.MODE_EXACT}
+
+
+@dataclass
+class Tables:
+    g1: np.ndarray
+    g2: np.ndarray
+    gm1: np.ndarray
+    gm2: np.ndarray
+    ide: List[np.ndarray]
+    idh: List[np.ndarray]
+    pbe: List[np.ndarray]
+    pce: List[np.ndarray]
+    pbm: List[np.ndarray]
+    pcm: List[np.ndarray]
+    disp_media: np.ndarray
+    disp_npoles: np.ndarray
+    kappa: np.ndarray
+    beta: np.ndarray
+    g3: np.ndarray
+
+
+def _check(lib, h, rc, what):
+    if rc != 0:
+        msg = lib.fdtd_last_error(h)
+        raise FdtdError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def build_tables(pb: Problem) -> Tables:
+    """Every coefficient table of ``pb``, in the reference's set-up order."""
+    lib = _lib.load()
+    nm = pb.nmedia
+    z = lambda n: np.zeros(n, np.float32)
+    g1, g2, gm1, gm2 = z(nm + 1), z(nm + 1), z(nm + 1), z(nm + 1)
+    rc = lib.fdtd_build_g_tables(nm, ptr(pb.epr, f32p), ptr(pb.sigma, f32p), ptr(pb.mur, f32p),
+                                 ptr(pb.sigmam, f32p), ptr(pb.kind, i32p), pb.dt, pb.eps0,
+                                 pb.mu0, ptr(g1, f32p), ptr(g2, f32p), ptr(gm1, f32p),
+                                 ptr(gm2, f32p))
+    if rc:
+        raise FdtdError("fdtd_build_g_tables failed")
+    ide, idh, pbe, pce, pbm, pcm = [], [], [], [], [], []
+    for a, n in enumerate((pb.nx, pb.ny, pb.nz)):
+        de, dh, Ie, Ih = z(n + 3), z(n + 3), z(n + 3), z(n + 3)
+        lib.fdtd_build_spacing(n, ptr(pb.D[a], f32p), ptr(de, f32p), ptr(dh, f32p),
+                               ptr(Ie, f32p), ptr(Ih, f32p))
+        b1, c1, b2, c2 = np.ones(n + 3, np.float32), z(n + 3), np.ones(n + 3, np.float32), z(n + 3)
+        L = pb.cpml_layers[a]
+        if L[0] or L[1]:
+            rc = lib.fdtd_build_cpml_axis(
+                n, int(L[0]), int(L[1]), float(pb.cpml_order[a][0]), float(pb.cpml_order[a][1]),
+                float(pb.cpml_refl[a][0]), float(pb.cpml_refl[a][1]), pb.alphamax, pb.alphaorden,
+                pb.kappamax, pb.dt, pb.eps0, pb.mu0, ptr(de, f32p), ptr(dh, f32p), ptr(b1, f32p),
+                ptr(c1, f32p), ptr(b2, f32p), ptr(c2, f32p), ptr(Ie, f32p), ptr(Ih, f32p))
+            if rc:
+                raise FdtdError(f"fdtd_build_cpml_axis failed on axis {a}")
+        ide.append(Ie); idh.append(Ih)
+        pbe.append(b1); pce.append(c1); pbm.append(b2); pcm.append(c2)
+    disp = [m for m in range(1, nm + 1) if pb.kind[m] & KIND_EDISP]
+    nd = len(disp)
+    kappa = np.zeros((max(nd, 1), PMAX, 2)); beta = np.zeros_like(kappa); g3 = np.zeros_like(kappa)
+    for d, m in enumerate(disp):
+        a1, a2 = np.zeros(1, np.float32), np.zeros(1, np.float32)
+        c = np.ascontiguousarray(pb.pole_c[m]); a = np.ascontiguousarray(pb.pole_a[m])
+        kk = np.zeros((PMAX, 2)); bb = np.zeros((PMAX, 2)); gg = np.zeros((PMAX, 2))
+        rc = lib.fdtd_build_ade(int(pb.npoles[m]), ptr(c, f64p), ptr(a, f64p), float(pb.eps11[m]),
+                                float(pb.sig11[m]), pb.dt, ptr(kk, f64p), ptr(bb, f64p),
+                                ptr(gg, f64p), ptr(a1, f32p), ptr(a2, f32p))
+        if rc:
+            raise FdtdError(f"fdtd_build_ade failed for medium {m}")
+        kappa[d], beta[d], g3[d] = kk, bb, gg
+        g1[m], g2[m] = a1[0], a2[0]
+    return Tables(g1, g2, gm1, gm2, ide, idh, pbe, pce, pbm, pcm,
+                  np.array(disp, np.int32), np.array([pb.npoles[m] for m in disp], np.int32),
+                  kappa, beta, g3)
+
+
+class Solver:
+    """One grid resident in HBM.  ``mode`` is "fast" (FMA, folded fp32 poles)
+    or "exact" (bit-identical to the reference loops)."""
+
+    def __init__(self, pb: Problem, mode: str = "fast", device: int = 0, stream=None,
+                 tables: Tables | None = None, upload_media: bool = True, probe_buffer: int = 4096):
+        self.lib = lib = _lib.load()
+        self.pb = pb
+        self.mode = mode
+        self.t = t = tables if tables is not None else build_tables(pb)
+        d = _lib.FdtdGridDesc(pb.nx, pb.ny, pb.nz, MODES[mode], _lib.STORE_FP32, device,
+                              stream)
+        h = C.c_void_p()
+        rc = lib.fdtd_create(C.byref(d), C.byref(h))
+        if rc:
+            raise FdtdError(f"fdtd_create failed ({rc}): {lib.fdtd_last_error(None).decode()}")
+        self.h = h
+        self._keep = []
+        self._ck(lib.fdtd_set_media(h, pb.nmedia, ptr(t.g1, f32p), ptr(t.g2, f32p),
+                                    ptr(t.gm1, f32p), ptr(t.gm2, f32p), ptr(pb.kind, i32p)),
+                 "fdtd_set_media")
+        if pb.mi is None:
+            self._apply_scene()
+        elif upload_media:
+            for c in range(6):
+                m = np.ascontiguousarray(pb.mi[c], np.int16)
+                self._ck(lib.fdtd_set_media_index(h, c, ptr(m, i16p)), "fdtd_set_media_index")
+        ids = t.ide + t.idh
+        self._ck(lib.fdtd_set_spacing(h, *[ptr(a, f32p) for a in ids]), "fdtd_set_spacing")
+        if pb.has_cpml:
+            lay = np.array([pb.cpml_layers[a][s] for a in range(3) for s in range(2)], np.int32)
+            prof = []
+            for a in range(3):
+                prof += [t.pbe[a], t.pce[a], t.pbm[a], t.pcm[a]]
+            self._ck(lib.fdtd_set_cpml(h, ptr(lay, i32p), *[ptr(p, f32p) for p in prof]),
+                     "fdtd_set_cpml")
+        nd = len(t.disp_media)
+        if nd:
+            dm = np.ascontiguousarray(t.disp_media, np.int32)
+            dn = np.ascontiguousarray(t.disp_npoles, np.int32)
+            k = np.ascontiguousarray(t.kappa[:nd]); b = np.ascontiguousarray(t.beta[:nd])
+            g = np.ascontiguousarray(t.g3[:nd])
+            self._keep += [dm, dn, k, b, g]
+            self._ck(lib.fdtd_set_edispersive(h, nd, ptr(dm, i32p), ptr(dn, i32p), ptr(k, f64p),
+                                              ptr(b, f64p), ptr(g, f64p)), "fdtd_set_edispersive")
+        nlast = max(pb.nsteps_max + 2, 2)
+        self._ck(lib.fdtd_set_time(h, pb.dt, nlast, None), "fdtd_set_time")
+        for s in pb.nodal:
+            box = np.array(s.box, np.int32)
+            ev = np.ascontiguousarray(s.evol, np.float32)
+            self._ck(lib.fdtd_add_nodal(h, s.comp, int(s.hard), ptr(box, i32p), s.amp,
+                                        s.deltaevol, len(ev) - 1, ptr(ev, f32p)), "fdtd_add_nodal")
+        self.nprobe = len(pb.probes)
+        self.probe_buffer = probe_buffer
+        if self.nprobe:
+            P = np.array(pb.probes, np.int32).T.copy()
+            self._ck(lib.fdtd_set_probes(h, self.nprobe, ptr(P[0], i32p), ptr(P[1], i32p),
+                                         ptr(P[2], i32p), ptr(P[3], i32p), probe_buffer),
+                     "fdtd_set_probes")
+        self.n = 0
+        self._samples = []
+
+    def _ck(self, rc, what):
+        _check(self.lib, self.h, rc, what)
+
+    def _apply_scene(self):
+        """Build the media matrices in HBM from the problem's scene ops."""
+        lib, h, pb = self.lib, self.h, self.pb
+        prio = np.ascontiguousarray(pb.prio, np.int32)
+        npr = len(prio)
+        for op in pb.scene:
+            kind = op[0]
+            if kind == "fill":
+                rc = lib.fdtd_media_fill(h, int(op[1]))
+            elif kind == "pec":
+                rc = lib.fdtd_media_pec_walls(h, int(op[1]))
+            elif kind == "sphere":
+                rc = lib.fdtd_media_paint_sphere(h, *[float(v) for v in op[1:5]], int(op[5]),
+                                                 npr, ptr(prio, i32p))
+            elif kind == "box":
+                rc = lib.fdtd_media_paint_box(h, *[int(v) for v in op[1:7]], int(op[7]), npr,
+                                              ptr(prio, i32p))
+            else:
+                raise ValueError(f"unknown scene op {kind}")
+            self._ck(rc, f"scene op {kind}")
+
+    # -- the reference's call structure ------------------------------------
+    def advance_e(self, n: int):
+        self._ck(self.lib.fdtd_advance_e(self.h, n), "fdtd_advance_e")
+
+    def advance_h(self, n: int):
+        self._ck(self.lib.fdtd_advance_h(self.h, n), "fdtd_advance_h")
+
+    def step(self):
+        """One iteration of ciclo_temporal, then UpdateObservation."""
+        self.advance_e(self.n)
+        self.advance_h(self.n)
+        if self.nprobe:
+            self._drain_if_full(1)
+            self._ck(self.lib.fdtd_sample_probes(self.h), "fdtd_sample_probes")
+        self.n += 1
+
+    def run(self, nsteps: int, chunk: int | None = None):
+        """nsteps whole steps from the captured graph; probes are drained from
+        the device buffer between chunks."""
+        chunk = chunk or self.probe_buffer
+        left = nsteps
+        while left > 0:
+            k = min(left, chunk, self.probe_buffer)
+            self._drain_if_full(k)
+            self._ck(self.lib.fdtd_run(self.h, self.n, k), "fdtd_run")
+            self.n += k
+            left -= k
+
+    def _drain_if_full(self, need):
+        if self.nprobe and getattr(self, "_pending", 0) + need > self.probe_buffer:
+            self._drain()
+        self._pending = getattr(self, "_pending", 0) + need
+
+    def _drain(self):
+        if not self.nprobe:
+            return
+        out = np.zeros((self.probe_buffer, self.nprobe), np.float32)
+        ns = C.c_int32()
+        self._ck(self.lib.fdtd_read_probes(self.h, ptr(out, f32p), self.probe_buffer, C.byref(ns)),
+                 "fdtd_read_probes")
+        self._samples.append(out[:ns.value].copy())
+        self._pending = 0
+
+    def probes(self) -> np.ndarray:
+        """All probe samples so far, [step][probe]."""
+        self._drain()
+        if not self._samples:
+            return np.zeros((0, self.nprobe), np.float32)
+        return np.concatenate(self._samples, axis=0)
+
+    def synchronize(self):
+        self._ck(self.lib.fdtd_synchronize(self.h), "fdtd_synchronize")
+
+    def field(self, c: int) -> np.ndarray:
+        out = np.zeros(self.pb.alloc_shape, np.float32)
+        self._ck(self.lib.fdtd_get_field(self.h, c, ptr(out, f32p)), "fdtd_get_field")
+        return out
+
+    def set_field(self, c: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, np.float32)
+        self._ck(self.lib.fdtd_set_field(self.h, c, ptr(a, f32p)), "fdtd_set_field")
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self.lib.fdtd_device_bytes(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fdtd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

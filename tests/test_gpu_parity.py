@@ -1,0 +1,179 @@
+"""Parity of the gfx950 path (through the C ABI) with the oracle.
+
+EXACT mode must be bit-identical to the oracle (which is bit-identical to the
+reference's own modules, tests/test_oracle_reference.py); FAST mode (FMA,
+folded fp32 poles) must stay within probe relative L2 <= 1e-5, the fp32
+tolerance of the target.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import semba_fdtd_amd as sf
+from semba_fdtd_amd import problem as P
+
+pytestmark = pytest.mark.gpu
+
+FAST_TOL = 1e-5   # probe relative L2, fp32 fields with FMA contraction
+
+CASES = {
+    "cavity_hard": ("cavity_problem", dict(n=24, nsteps=300)),
+    "cavity_soft": ("cavity_problem", dict(n=20, nsteps=250, hard=False)),
+    "scatterer": ("scatterer_problem", dict(nsteps=300)),
+    "graded": ("scatterer_problem", dict(nsteps=250, graded_z=1.03)),
+    "gedney10_alpha_kappa": ("scatterer_problem", dict(nsteps=200, n=(34, 33, 32),
+                                                        L=((10, 10), (10, 10), (10, 10)),
+                                                        alphamax=0.1, kappamax=3.0, order=2.0)),
+    "one_sided_cpml": ("scatterer_problem", dict(nsteps=200, n=(70, 24, 22),
+                                                 L=((5, 0), (0, 6), (4, 4)), refl=1e-3)),
+    "wide_x": ("scatterer_problem", dict(nsteps=120, n=(131, 9, 11), L=((6, 7), (2, 2), (3, 2)))),
+}
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _problem(name):
+    builder, kw = CASES[name]
+    return getattr(P, builder)(**kw), kw["nsteps"]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_exact_mode_bit_identical(name):
+    print(f"case {name}", flush=True)
+    pb, ns = _problem(name)
+    ref = oracle.Oracle(pb)
+    want = ref.run(ns)
+    gpu = sf.Solver(pb, mode="exact")
+    gpu.run(ns)
+    got = gpu.probes()
+    assert got.shape == want.shape
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), rel_l2(got, want)
+    for c in range(6):
+        f = gpu.field(c)
+        assert np.array_equal(f.view(np.uint32), ref.fields[c].view(np.uint32)), c
+    assert np.abs(want).max() > 0
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fast_mode_within_tolerance(name):
+    print(f"case {name}", flush=True)
+    pb, ns = _problem(name)
+    want = oracle.Oracle(pb).run(ns)
+    gpu = sf.Solver(pb, mode="fast")
+    gpu.run(ns)
+    err = rel_l2(gpu.probes(), want)
+    assert err <= FAST_TOL, err
+
+
+def test_cavity64_2000_steps_fast_and_exact():
+    """The plain-method workload: 64^3 PEC cavity, hard Ez Gaussian source,
+    2000 steps, 8 probes."""
+    pb = P.cavity_problem(n=64, nsteps=2000)
+    want = oracle.Oracle(pb).run(2000)
+    fast = sf.Solver(pb, mode="fast")
+    fast.run(2000)
+    err = rel_l2(fast.probes(), want)
+    assert err <= FAST_TOL, err
+    exact = sf.Solver(pb, mode="exact")
+    exact.run(2000)
+    assert np.array_equal(exact.probes().view(np.uint32), want.view(np.uint32))
+
+
+def test_granular_entry_points_equal_fused():
+    """advance_e_yee + advance_e_cpml + advance_e_dispersive + nodal equals
+    the fused advance_e (and the same for H), bit for bit."""
+    pb = P.scatterer_problem(nsteps=60)
+    a = sf.Solver(pb, mode="exact")
+    b = sf.Solver(pb, mode="exact")
+    L = b.lib
+    for n in range(60):
+        a.advance_e(n)
+        a.advance_h(n)
+        for fn in (L.fdtd_advance_e_yee, L.fdtd_advance_e_cpml, L.fdtd_advance_e_dispersive):
+            assert fn(b.h) == 0
+        assert L.fdtd_advance_nodal_e(b.h, n) == 0
+        for fn in (L.fdtd_advance_h_yee, L.fdtd_advance_h_cpml):
+            assert fn(b.h) == 0
+        assert L.fdtd_advance_nodal_h(b.h, n) == 0
+    for c in range(6):
+        assert np.array_equal(a.field(c).view(np.uint32), b.field(c).view(np.uint32)), c
+
+
+def test_graph_replay_equals_stepping():
+    pb = P.scatterer_problem(nsteps=80)
+    a = sf.Solver(pb, mode="fast")
+    b = sf.Solver(pb, mode="fast")
+    a.run(80, chunk=7)
+    for _ in range(80):
+        b.step()
+    assert np.array_equal(a.probes().view(np.uint32), b.probes().view(np.uint32))
+    for c in range(6):
+        assert np.array_equal(a.field(c).view(np.uint32), b.field(c).view(np.uint32)), c
+
+
+def test_device_scene_equals_host_scene():
+    """fdtd_media_fill / pec_walls / paint_sphere / paint_box reproduce the
+    host voxeliser (CreateVolumeMM rules) exactly; the stepped fields match
+    the oracle run on the host-built matrices."""
+    pb = P.heterogeneous_problem(n=72, L=6, nsteps=60, nprobes=12)
+    host = P.heterogeneous_problem(n=72, L=6, nsteps=60, nprobes=12, host_media=True)
+    gpu = sf.Solver(pb, mode="exact")
+    ref = oracle.Oracle(host)
+    want = ref.run(60)
+    gpu.run(60)
+    assert np.array_equal(gpu.probes().view(np.uint32), want.view(np.uint32))
+    for c in range(6):
+        assert np.array_equal(gpu.field(c).view(np.uint32), ref.fields[c].view(np.uint32)), c
+
+
+def test_pec_walls_device_equals_host():
+    n = (20, 17, 15)
+    pb = P.Problem(*n, epr=[1, 1], sigma=[0, 0], mur=[1, 1], sigmam=[0, 0], kind=[1, 0],
+                   mi=None, D=[P.uniform_D(v, 0.01) for v in n], dt=1e-12,
+                   scene=[("fill", 1), ("pec", 0b101101)], prio=np.array([9, 0]))
+    host = pb.materialize()
+    gpu = sf.Solver(pb, mode="fast")
+    # media leave the device only through the stepping; a field of ones on
+    # every component is zeroed exactly where G1 = G2 = 0 (PEC) after one E/H
+    # Yee update with zero curl
+    for c in range(6):
+        gpu.set_field(c, np.ones(pb.alloc_shape, np.float32))
+    assert gpu.lib.fdtd_advance_e_yee(gpu.h) == 0
+    assert gpu.lib.fdtd_advance_h_yee(gpu.h) == 0
+    for c in range(6):
+        f = gpu.field(c)
+        sw = tuple(slice(1, P.sweep_hi(c, 2 - ax, n) + 2) for ax in range(3))
+        assert np.array_equal(f[sw] == 0.0, host[c][sw] == 0), c
+
+
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+def test_full_size_linearity(mode):
+    """Full-size heterogeneous grid (1024^3, 12-cell CPML, 8 lossy spheres,
+    2-pole medium in 20% of the volume): every operation of the path is
+    linear in the fields, so doubling the source doubles every probe
+    sample exactly (scaling by 2 is exact in binary floating point)."""
+    print(f"full size linearity {mode}", flush=True)
+    ns = 40
+    pb = P.heterogeneous_problem(n=1024, nsteps=ns)
+    _, (x0, x1, y, _, z, _) = 0, pb.nodal[0].box
+    xm = (x0 + x1) // 2
+    pb.probes = [(P.EX, xm, y, z), (P.EX, xm, y + 1, z), (P.EX, xm, y, z + 2),
+                 (P.HY, xm, y, z), (P.HZ, xm, y, z), (P.EY, xm, y, z), (P.EZ, xm, y, z)]
+    a = sf.Solver(pb, mode=mode)
+    a.run(ns)
+    pa = a.probes()
+    a.close()
+    pb.nodal[0].amp *= 2.0
+    b = sf.Solver(pb, mode=mode)
+    b.run(ns)
+    pbb = b.probes()
+    b.close()
+    assert np.isfinite(pa).all()
+    assert np.abs(pa[:, :5]).min(axis=0).max() > 0 or np.abs(pa).max() > 1e-6
+    # exact where the samples are normal numbers; subnormal leading-edge
+    # values may round differently after doubling
+    big = np.abs(pa) > 1e-30
+    assert np.array_equal((2.0 * pa)[big].view(np.uint32), pbb[big].view(np.uint32))
+    assert np.abs(2.0 * pa - pbb).max() <= 1e-35
