@@ -84,7 +84,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=192)
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--disp-fraction", type=float, default=0.2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 plain 1 GiB device copies (the known-size kernel that "
+                         "scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(2**28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read_per_launch": x.numel() * 4,
+                          "bytes_written_per_launch": y.numel() * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               f"(independent grids shard across GPUs, see DESIGN.md)", file=sys.stderr)
@@ -95,7 +109,7 @@ def main():
     from semba_fdtd_amd import problem as P
 
     total = args.warmup + args.steps + 2 * args.kernel_reps + 4
-    pb = P.heterogeneous_problem(n=args.n, nsteps=total)
+    pb = P.heterogeneous_problem(n=args.n, nsteps=total, disp_fraction=args.disp_fraction)
     print(f"[bench] building {pb.name} ({pb.cells / 1e9:.3f} G cells) mode={args.mode}",
           file=sys.stderr, flush=True)
     torch.cuda.init()

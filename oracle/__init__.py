@@ -7,7 +7,9 @@ TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
 Two checkers live here:
 
 * ``Oracle`` -- ctypes wrapper of ``fdtd_oracle.c``, a plain-C restatement of
-  the reference's loops and coefficient tables (no FMA contraction).
+  the reference's loops and coefficient tables (no FMA contraction), built in
+  the reference's real(4) kind (``liboracle_fdtd.so``) and in real(8)
+  (``liboracle_fdtd64.so``, the FP64 yardstick, ``precision=64``).
 * ``Reference`` -- ctypes wrapper of ``_ref/libsemba_ref.so``: the reference's
   own Fortran modules (calc_constants, borderscpml, electricdispersive,
   nodalsources) compiled from their sources by ``oracle/Makefile``, driven by
@@ -24,68 +26,90 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle_fdtd.so")
+ORACLE_SO = {32: os.path.join(HERE, "liboracle_fdtd.so"),
+             64: os.path.join(HERE, "liboracle_fdtd64.so")}
 REF_SO = os.path.join(HERE, "_ref", "libsemba_ref.so")
 PMAX = 8
+# vacuum constants as written in the reference (fdetypes.F90:219-222), kept
+# in full for the real(8) build
+EPS0_64 = 8.8541878176203898505365630317107502606083701665994498081024171524053950954599821142852891607182008932e-12
+MU0_64 = 1.2566370614359172953850573533118011536788677597500423283899778369231265625144835994512139301368468271e-6
 
 _f32p = C.POINTER(C.c_float)
 _f64p = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
 _i16p = C.POINTER(C.c_int16)
+_REAL = {32: (C.c_float, np.float32), 64: (C.c_double, np.float64)}
 
 
-class OrcNodal(C.Structure):
-    _fields_ = [("comp", C.c_int32), ("hard", C.c_int32),
-                ("xi", C.c_int32), ("xe", C.c_int32), ("yi", C.c_int32),
-                ("ye", C.c_int32), ("zi", C.c_int32), ("ze", C.c_int32),
-                ("amp", C.c_float), ("deltaevol", C.c_float),
-                ("numus", C.c_int32), ("evol", _f32p)]
+def _structs(prec):
+    rt = _REAL[prec][0]
+    rp = C.POINTER(rt)
+
+    class OrcNodal(C.Structure):
+        _fields_ = [("comp", C.c_int32), ("hard", C.c_int32),
+                    ("xi", C.c_int32), ("xe", C.c_int32), ("yi", C.c_int32),
+                    ("ye", C.c_int32), ("zi", C.c_int32), ("ze", C.c_int32),
+                    ("amp", rt), ("deltaevol", rt), ("numus", C.c_int32), ("evol", rp)]
+
+    class OrcGrid(C.Structure):
+        _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                    ("f", rp * 6), ("mi", _i16p * 6),
+                    ("g1", rp), ("g2", rp), ("gm1", rp), ("gm2", rp),
+                    ("nmedia", C.c_int32), ("id", rp * 6),
+                    ("L", (C.c_int32 * 2) * 3),
+                    ("pbe", rp * 3), ("pce", rp * 3), ("pbm", rp * 3), ("pcm", rp * 3),
+                    ("psi", rp * 12),
+                    ("ndisp", C.c_int32), ("disp_medium", _i32p),
+                    ("disp_npoles", _i32p), ("disp_kappa", _f64p),
+                    ("disp_beta", _f64p), ("disp_g3", _f64p),
+                    ("eprev", rp * 3), ("jcur", _f64p * 3), ("jstride", C.c_int32),
+                    ("dt", C.c_double), ("tiempo", _f64p),
+                    ("nnodal", C.c_int32), ("nodal", C.POINTER(OrcNodal)),
+                    ("medkind", _i32p)]
+    return OrcNodal, OrcGrid
 
 
-class OrcGrid(C.Structure):
-    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
-                ("f", _f32p * 6), ("mi", _i16p * 6),
-                ("g1", _f32p), ("g2", _f32p), ("gm1", _f32p), ("gm2", _f32p),
-                ("nmedia", C.c_int32), ("id", _f32p * 6),
-                ("L", (C.c_int32 * 2) * 3),
-                ("pbe", _f32p * 3), ("pce", _f32p * 3), ("pbm", _f32p * 3),
-                ("pcm", _f32p * 3), ("psi", _f32p * 12),
-                ("ndisp", C.c_int32), ("disp_medium", _i32p),
-                ("disp_npoles", _i32p), ("disp_kappa", _f64p),
-                ("disp_beta", _f64p), ("disp_g3", _f64p),
-                ("eprev", _f32p * 3), ("jcur", _f64p * 3), ("jstride", C.c_int32),
-                ("dt", C.c_double), ("tiempo", _f64p),
-                ("nnodal", C.c_int32), ("nodal", C.POINTER(OrcNodal)),
-                ("medkind", _i32p)]
+_STRUCTS = {p: _structs(p) for p in (32, 64)}
+OrcNodal, OrcGrid = _STRUCTS[32]
 
 
 def _p(a, t):
     return a.ctypes.data_as(t)
 
 
-def load_oracle():
-    if not os.path.exists(ORACLE_SO):
-        raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
-    lib = C.CDLL(ORACLE_SO)
+_LIBS = {}
+
+
+def load_oracle(precision: int = 32):
+    if precision in _LIBS:
+        return _LIBS[precision]
+    path = ORACLE_SO[precision]
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+    rt = _REAL[precision][0]
+    rp = C.POINTER(rt)
+    Nod, Grid = _STRUCTS[precision]
+    lib = C.CDLL(path)
     lib.orc_dt.restype = C.c_double
-    lib.orc_dt.argtypes = [C.c_float] * 5 + [C.c_float, C.c_int32, C.c_double]
-    lib.orc_g_tables.argtypes = [C.c_int32, _f32p, _f32p, _f32p, _f32p, _i32p,
-                                 C.c_double, C.c_float, C.c_float,
-                                 _f32p, _f32p, _f32p, _f32p]
-    lib.orc_spacing.argtypes = [C.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p]
-    lib.orc_cpml_axis.argtypes = ([C.c_int32] * 3 + [C.c_float] * 7 +
-                                  [C.c_double, C.c_float, C.c_float] + [_f32p] * 8)
-    lib.orc_ade_tables.argtypes = [C.c_int32, _f64p, _f64p, C.c_float, C.c_float,
-                                   C.c_double, _f64p, _f64p, _f64p, _f32p, _f32p]
+    lib.orc_dt.argtypes = [rt] * 6 + [C.c_int32, C.c_double]
+    lib.orc_g_tables.argtypes = [C.c_int32, rp, rp, rp, rp, _i32p, C.c_double, rt, rt,
+                                 rp, rp, rp, rp]
+    lib.orc_spacing.argtypes = [C.c_int32, rp, rp, rp, rp, rp]
+    lib.orc_cpml_axis.argtypes = ([C.c_int32] * 3 + [rt] * 7 + [C.c_double, rt, rt] +
+                                  [rp] * 8)
+    lib.orc_ade_tables.argtypes = [C.c_int32, _f64p, _f64p, rt, rt, C.c_double,
+                                   _f64p, _f64p, _f64p, rp, rp]
     lib.orc_time_vector.argtypes = [C.c_int32, C.c_double, _f64p]
-    lib.orc_evolucion.restype = C.c_float
-    lib.orc_evolucion.argtypes = [C.c_float, C.POINTER(OrcNodal)]
+    lib.orc_evolucion.restype = rt
+    lib.orc_evolucion.argtypes = [rt, C.POINTER(Nod)]
     for fn in ("orc_yee_e", "orc_yee_h", "orc_cpml_e", "orc_cpml_h", "orc_ade_e"):
-        getattr(lib, fn).argtypes = [C.POINTER(OrcGrid)]
+        getattr(lib, fn).argtypes = [C.POINTER(Grid)]
     for fn in ("orc_nodal_e", "orc_nodal_h", "orc_step"):
-        getattr(lib, fn).argtypes = [C.POINTER(OrcGrid), C.c_int32]
-    lib.orc_run.argtypes = [C.POINTER(OrcGrid), C.c_int32, C.c_int32, C.c_int32,
-                            _i32p, _i32p, _i32p, _i32p, _f32p]
+        getattr(lib, fn).argtypes = [C.POINTER(Grid), C.c_int32]
+    lib.orc_run.argtypes = [C.POINTER(Grid), C.c_int32, C.c_int32, C.c_int32,
+                            _i32p, _i32p, _i32p, _i32p, rp]
+    _LIBS[precision] = lib
     return lib
 
 
@@ -101,28 +125,39 @@ class Tables:
         self.disp_media, self.disp_npoles = disp_media, disp_npoles
 
 
-def oracle_tables(lib, pb) -> Tables:
+def _constants(pb, precision):
+    if precision == 64:
+        return EPS0_64, MU0_64
+    return pb.eps0, pb.mu0
+
+
+def oracle_tables(lib, pb, precision: int = 32) -> Tables:
     """Build every coefficient table with the oracle's restatement."""
+    rt, dt_ = _REAL[precision]
+    rp = C.POINTER(rt)
     nm = pb.nmedia
-    f = lambda n: np.zeros(n, np.float32)
+    f = lambda n: np.zeros(n, dt_)
+    r = lambda a: np.ascontiguousarray(a, dt_)
+    eps0, mu0 = _constants(pb, precision)
     g1, g2, gm1, gm2 = f(nm + 1), f(nm + 1), f(nm + 1), f(nm + 1)
-    lib.orc_g_tables(nm, _p(pb.epr, _f32p), _p(pb.sigma, _f32p), _p(pb.mur, _f32p),
-                     _p(pb.sigmam, _f32p), _p(pb.kind, _i32p), pb.dt, pb.eps0, pb.mu0,
-                     _p(g1, _f32p), _p(g2, _f32p), _p(gm1, _f32p), _p(gm2, _f32p))
+    epr, sig, mur, sgm = r(pb.epr), r(pb.sigma), r(pb.mur), r(pb.sigmam)
+    lib.orc_g_tables(nm, _p(epr, rp), _p(sig, rp), _p(mur, rp), _p(sgm, rp),
+                     _p(pb.kind, _i32p), pb.dt, eps0, mu0,
+                     _p(g1, rp), _p(g2, rp), _p(gm1, rp), _p(gm2, rp))
     ide, idh, pbe, pce, pbm, pcm = [], [], [], [], [], []
     for a, n in enumerate((pb.nx, pb.ny, pb.nz)):
         de, dh, Ie, Ih = f(n + 3), f(n + 3), f(n + 3), f(n + 3)
-        lib.orc_spacing(n, _p(pb.D[a], _f32p), _p(de, _f32p), _p(dh, _f32p),
-                        _p(Ie, _f32p), _p(Ih, _f32p))
+        D = r(pb.D[a])
+        lib.orc_spacing(n, _p(D, rp), _p(de, rp), _p(dh, rp), _p(Ie, rp), _p(Ih, rp))
         b1, c1, b2, c2 = f(n + 3), f(n + 3), f(n + 3), f(n + 3)
         L = pb.cpml_layers[a]
         if L[0] or L[1]:
             lib.orc_cpml_axis(n, int(L[0]), int(L[1]), float(pb.cpml_order[a][0]),
                               float(pb.cpml_order[a][1]), float(pb.cpml_refl[a][0]),
                               float(pb.cpml_refl[a][1]), pb.alphamax, pb.alphaorden,
-                              pb.kappamax, pb.dt, pb.eps0, pb.mu0,
-                              _p(de, _f32p), _p(dh, _f32p), _p(b1, _f32p), _p(c1, _f32p),
-                              _p(b2, _f32p), _p(c2, _f32p), _p(Ie, _f32p), _p(Ih, _f32p))
+                              pb.kappamax, pb.dt, eps0, mu0,
+                              _p(de, rp), _p(dh, rp), _p(b1, rp), _p(c1, rp),
+                              _p(b2, rp), _p(c2, rp), _p(Ie, rp), _p(Ih, rp))
         else:
             b1[:] = 1.0
             b2[:] = 1.0
@@ -132,13 +167,14 @@ def oracle_tables(lib, pb) -> Tables:
     nd = len(disp)
     kappa = np.zeros((max(nd, 1), PMAX, 2)); beta = np.zeros_like(kappa); g3 = np.zeros_like(kappa)
     for d, m in enumerate(disp):
-        a1, a2 = np.zeros(1, np.float32), np.zeros(1, np.float32)
+        a1, a2 = f(1), f(1)
         np_ = int(pb.npoles[m])
         cc = np.ascontiguousarray(pb.pole_c[m]); aa = np.ascontiguousarray(pb.pole_a[m])
         kk = np.zeros((PMAX, 2)); bb = np.zeros((PMAX, 2)); gg = np.zeros((PMAX, 2))
-        lib.orc_ade_tables(np_, _p(cc, _f64p), _p(aa, _f64p), float(pb.eps11[m]),
+        eps11 = float(pb.eps11[m]) if precision == 32 else float(pb.eps11[m]) / pb.eps0 * EPS0_64
+        lib.orc_ade_tables(np_, _p(cc, _f64p), _p(aa, _f64p), eps11,
                            float(pb.sig11[m]), pb.dt, _p(kk, _f64p), _p(bb, _f64p),
-                           _p(gg, _f64p), _p(a1, _f32p), _p(a2, _f32p))
+                           _p(gg, _f64p), _p(a1, rp), _p(a2, rp))
         kappa[d], beta[d], g3[d] = kk, bb, gg
         g1[m], g2[m] = a1[0], a2[0]
     return Tables(g1, g2, gm1, gm2, ide, idh, pbe, pce, pbm, pcm, kappa, beta, g3,
@@ -146,44 +182,49 @@ def oracle_tables(lib, pb) -> Tables:
 
 
 class Oracle:
-    """Runs a Problem through the C restatement."""
+    """Runs a Problem through the C restatement (precision 32 or 64)."""
 
-    def __init__(self, pb, tables: Tables | None = None):
-        self.lib = load_oracle()
+    def __init__(self, pb, tables: Tables | None = None, precision: int = 32):
+        self.precision = precision
+        self.lib = load_oracle(precision)
+        rt, dt_ = _REAL[precision]
+        rp = C.POINTER(rt)
+        Nod, Grid = _STRUCTS[precision]
         self.pb = pb
-        self.t = tables if tables is not None else oracle_tables(self.lib, pb)
+        self.t = tables if tables is not None else oracle_tables(self.lib, pb, precision)
         t = self.t
+        if pb.mi is None:
+            pb.mi = pb.materialize()
         shape = pb.alloc_shape
-        self.fields = np.zeros((6,) + shape, np.float32)
+        self.fields = np.zeros((6,) + shape, dt_)
         self.mi = np.ascontiguousarray(pb.mi, dtype=np.int16)
-        self.psi = np.zeros((12,) + shape, np.float32) if pb.has_cpml else None
+        self.psi = np.zeros((12,) + shape, dt_) if pb.has_cpml else None
         nd = len(t.disp_media)
-        self.eprev = np.zeros((3,) + shape, np.float32) if nd else None
+        self.eprev = np.zeros((3,) + shape, dt_) if nd else None
         self.jstride = int(max(t.disp_npoles)) if nd else 1
         self.jcur = np.zeros((3,) + shape + (self.jstride, 2)) if nd else None
         self.tiempo = np.zeros(pb.nsteps_max + 3)
         self.lib.orc_time_vector(pb.nsteps_max + 2, pb.dt, _p(self.tiempo, _f64p))
         self.medkind = np.ascontiguousarray(pb.kind, np.int32)
         self._keep = []
-        g = OrcGrid()
+        g = Grid()
         g.nx, g.ny, g.nz = pb.nx, pb.ny, pb.nz
         for c in range(6):
-            g.f[c] = self.fields[c].ctypes.data_as(_f32p)
+            g.f[c] = self.fields[c].ctypes.data_as(rp)
             g.mi[c] = self.mi[c].ctypes.data_as(_i16p)
-        g.g1, g.g2, g.gm1, g.gm2 = (_p(t.g1, _f32p), _p(t.g2, _f32p), _p(t.gm1, _f32p),
-                                    _p(t.gm2, _f32p))
+        g.g1, g.g2, g.gm1, g.gm2 = (_p(t.g1, rp), _p(t.g2, rp), _p(t.gm1, rp), _p(t.gm2, rp))
         g.nmedia = pb.nmedia
         ids = t.ide + t.idh
         for c in range(6):
-            g.id[c] = ids[c].ctypes.data_as(_f32p)
+            g.id[c] = ids[c].ctypes.data_as(rp)
         for a in range(3):
             for s in range(2):
                 g.L[a][s] = int(pb.cpml_layers[a][s])
-            g.pbe[a] = _p(t.pbe[a], _f32p); g.pce[a] = _p(t.pce[a], _f32p)
-            g.pbm[a] = _p(t.pbm[a], _f32p); g.pcm[a] = _p(t.pcm[a], _f32p)
+            g.pbe[a] = _p(t.pbe[a], rp); g.pce[a] = _p(t.pce[a], rp)
+            g.pbm[a] = _p(t.pbm[a], rp); g.pcm[a] = _p(t.pcm[a], rp)
         if self.psi is not None:
             for s in range(12):
-                g.psi[s] = self.psi[s].ctypes.data_as(_f32p)
+                g.psi[s] = self.psi[s].ctypes.data_as(rp)
         g.ndisp = nd
         if nd:
             self._dm = np.ascontiguousarray(t.disp_media, np.int32)
@@ -194,23 +235,23 @@ class Oracle:
             g.disp_kappa = _p(self._k, _f64p); g.disp_beta = _p(self._b, _f64p)
             g.disp_g3 = _p(self._g, _f64p)
             for c in range(3):
-                g.eprev[c] = self.eprev[c].ctypes.data_as(_f32p)
+                g.eprev[c] = self.eprev[c].ctypes.data_as(rp)
                 g.jcur[c] = self.jcur[c].ctypes.data_as(_f64p)
         g.jstride = self.jstride
         g.dt = pb.dt
         g.tiempo = _p(self.tiempo, _f64p)
         ns = len(pb.nodal)
-        self._nod = (OrcNodal * max(ns, 1))()
+        self._nod = (Nod * max(ns, 1))()
         for s, src in enumerate(pb.nodal):
-            ev = np.ascontiguousarray(src.evol, np.float32)
+            ev = np.ascontiguousarray(src.evol, dt_)
             self._keep.append(ev)
             o = self._nod[s]
             o.comp, o.hard = src.comp, int(src.hard)
             o.xi, o.xe, o.yi, o.ye, o.zi, o.ze = src.box
             o.amp, o.deltaevol, o.numus = src.amp, src.deltaevol, len(ev) - 1
-            o.evol = _p(ev, _f32p)
+            o.evol = _p(ev, rp)
         g.nnodal = ns
-        g.nodal = C.cast(self._nod, C.POINTER(OrcNodal))
+        g.nodal = C.cast(self._nod, C.POINTER(Nod))
         g.medkind = _p(self.medkind, _i32p)
         self.g = g
         self.n = 0
@@ -226,9 +267,10 @@ class Oracle:
         pi = np.array([p[1] for p in probes] or [0], np.int32)
         pj = np.array([p[2] for p in probes] or [0], np.int32)
         pk = np.array([p[3] for p in probes] or [0], np.int32)
-        out = np.zeros((nsteps, max(npb, 1)), np.float32)
+        out = np.zeros((nsteps, max(npb, 1)), _REAL[self.precision][1])
         self.lib.orc_run(C.byref(self.g), self.n, nsteps, npb, _p(pc, _i32p), _p(pi, _i32p),
-                         _p(pj, _i32p), _p(pk, _i32p), _p(out, _f32p))
+                         _p(pj, _i32p), _p(pk, _i32p),
+                         _p(out, C.POINTER(_REAL[self.precision][0])))
         self.n += nsteps
         return out[:, :npb]
 
@@ -259,6 +301,8 @@ class Reference:
         if lib is None:
             raise RuntimeError("reference harness not built (make -C oracle ref)")
         self.lib, self.pb = lib, pb
+        if pb.mi is None:
+            pb.mi = pb.materialize()
         nm = pb.nmedia
         pc = np.zeros((nm + 1, PMAX, 2)); pa = np.zeros((nm + 1, PMAX, 2))
         for m in range(nm + 1):

@@ -10,19 +10,33 @@
  * reference's default x86-64 build has no FMA either.
  *
  * Every function names the reference lines it restates (paths relative to the
- * reference root).  Precision follows the reference kinds: fields and tables
- * real(4) (-DCompileWithReal4, fdetypes.F90:140-162), time and dt real(8)
- * (RKIND_tiempo), dispersive currents complex(8) (CKIND).
+ * reference root).  The real kind is a build parameter, as in the reference
+ * (-DCompileWithReal4 / -DCompileWithReal8, fdetypes.F90:140-162): the
+ * default build keeps fields and tables in real(4) with time and dt in
+ * real(8) and dispersive currents in complex(8); -DORC_REAL64 builds the
+ * real(8) variant used as the FP64 yardstick for error statements.
  *
  * Parity pins (see tests/): tables and a full step loop are compared
  * bit-for-bit against the reference's own Fortran modules built from source
- * (oracle/_ref), and against closed forms (cavity eigenfrequencies, Fresnel /
- * CPML reflection, discrete Debye/Lorentz permittivity).
+ * (oracle/_ref), and against closed forms (Yee cavity eigenfrequencies,
+ * CPML reflection, discrete pole-residue permittivity).
  */
 #include "fdtd_oracle.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+typedef orc_real real;
+#define R(x) ((real)(x))
+#ifdef ORC_REAL64
+#define RSQRT sqrt
+#define RLOG log
+#define RPOW pow
+#else
+#define RSQRT sqrtf
+#define RLOG logf
+#define RPOW powf
+#endif
 
 #define AX(g) ((size_t)(g)->nx + 3)
 #define AY(g) ((size_t)(g)->ny + 3)
@@ -35,23 +49,18 @@
 #define ORC_KIND_EDISP 4
 
 /* --------------------------------------------------------------------------
- * dt: semba_fdtd.F90:1211-1250.  dtlay is formed in real(4) from the minimum
- * cell sizes; with -cfl the step is dtlay*cfl (real(4) product stored in the
- * real(8) sgg%dt), otherwise the input dt is clipped to heurCFL*dtlay
- * (heurCFL = 0.8, fdetypes.F90:186).  cluz = 1/sqrt(eps0*mu0) in real(4)
- * (semba_fdtd.F90:154-156).
+ * dt: semba_fdtd.F90:1211-1250.  dtlay is formed in the real kind from the
+ * minimum cell sizes; with -cfl the step is dtlay*cfl (stored in the real(8)
+ * sgg%dt), otherwise the input dt is clipped to heurCFL*dtlay (heurCFL = 0.8,
+ * fdetypes.F90:186).  cluz = 1/sqrt(eps0*mu0) (semba_fdtd.F90:154-156).
  * ------------------------------------------------------------------------ */
-double orc_dt(float dxmin, float dymin, float dzmin, float eps0, float mu0,
-              float cfl, int32_t forcecfl, double dt_in) {
-  float cluz = 1.0f / sqrtf(eps0 * mu0);
-  float ix = 1.0f / dxmin, iy = 1.0f / dymin, iz = 1.0f / dzmin;
-  float dtlay = 1.0f / (cluz * sqrtf(ix * ix + iy * iy + iz * iz));
-  float dt = dtlay;
-  if (forcecfl) {
-    float p = dt * cfl;
-    return (double)p;
-  }
-  if (dt_in > (double)(dt * 0.8f)) return (double)(dt * 0.8f);
+double orc_dt(real dxmin, real dymin, real dzmin, real eps0, real mu0, real cfl,
+              int32_t forcecfl, double dt_in) {
+  real cluz = R(1.0) / RSQRT(eps0 * mu0);
+  real ix = R(1.0) / dxmin, iy = R(1.0) / dymin, iz = R(1.0) / dzmin;
+  real dtlay = R(1.0) / (cluz * RSQRT(ix * ix + iy * iy + iz * iz));
+  if (forcecfl) return (double)(dtlay * cfl);
+  if (dt_in > (double)(dtlay * R(0.8))) return (double)(dtlay * R(0.8));
   return dt_in;
 }
 
@@ -59,34 +68,33 @@ double orc_dt(float dxmin, float dymin, float dzmin, float eps0, float mu0,
  * Per-medium update coefficients: calc_constants.F90:36-148.
  * kind: 0 lossy dielectric (:132-144), ORC_KIND_PEC (or r == 0) all zero
  * (:62-64), ORC_KIND_EDISP: G1/G2 placeholders 0 (overwritten by the ADE
- * init), GM from mu/sigmam (:109-118).  Expressions mix real(4) with the
- * real(8) dt exactly as written there.
+ * init), GM from mu/sigmam (:109-118).  Expressions mix the real kind with
+ * the real(8) dt exactly as written there.
  * ------------------------------------------------------------------------ */
-static void orc_lossy(double dt, float sig, float eps, float *g1, float *g2) {
+static void orc_lossy(double dt, real sig, real eps, real *g1, real *g2) {
   /* G1=(1 - S*dt/(2*eps))/(1 + S*dt/(2*eps)); G2=dt/eps/(1 + S*dt/(2*eps)) */
-  double a = (double)sig * dt / (double)(2.0f * eps);
+  double a = (double)sig * dt / (double)(R(2.0) * eps);
   double x1 = (1.0 - a) / (1.0 + a);
   double x2 = dt / (double)eps / (1.0 + a);
-  *g1 = (float)x1;
-  *g2 = (float)x2;
-  if (*g1 < 0.0f) { /* exponential time stepping, :135-138 */
-    *g1 = (float)exp(-(double)sig * dt / (double)eps);
-    *g2 = (1.0f - *g1) / sig;
+  *g1 = (real)x1;
+  *g2 = (real)x2;
+  if (*g1 < R(0.0)) { /* exponential time stepping, :135-138 */
+    *g1 = (real)exp(-(double)sig * dt / (double)eps);
+    *g2 = (R(1.0) - *g1) / sig;
   }
 }
 
-void orc_g_tables(int32_t nmedia, const float *epr, const float *sigma,
-                  const float *mur, const float *sigmam, const int32_t *kind,
-                  double dt, float eps0, float mu0,
-                  float *g1, float *g2, float *gm1, float *gm2) {
+void orc_g_tables(int32_t nmedia, const real *epr, const real *sigma, const real *mur,
+                  const real *sigmam, const int32_t *kind, double dt, real eps0, real mu0,
+                  real *g1, real *g2, real *gm1, real *gm2) {
   for (int32_t r = 0; r <= nmedia; ++r) {
-    float eps = eps0 * epr[r];
-    float mu = mu0 * mur[r];
+    real eps = eps0 * epr[r];
+    real mu = mu0 * mur[r];
     if ((kind[r] & ORC_KIND_PEC) || r == 0) {
-      g1[r] = g2[r] = gm1[r] = gm2[r] = 0.0f;
+      g1[r] = g2[r] = gm1[r] = gm2[r] = R(0.0);
     } else if (kind[r] & ORC_KIND_EDISP) {
-      g1[r] = 0.0f;
-      g2[r] = 0.0f;
+      g1[r] = R(0.0);
+      g2[r] = R(0.0);
       orc_lossy(dt, sigmam[r], mu, &gm1[r], &gm2[r]);
     } else {
       orc_lossy(dt, sigma[r], eps, &g1[r], &g2[r]);
@@ -97,16 +105,15 @@ void orc_g_tables(int32_t nmedia, const float *epr, const float *sigma,
 
 /* --------------------------------------------------------------------------
  * Spacings: timestepping.F90:314-370.  dxe(i) = DX(i) (primal, H side),
- * dxh(i) = (DX(i) + DX(i-1))/2 (dual, E side), inverses in real(4), over the
- * Alloc range [-1, n+1].  DX is indexed [-2, n+1] (DX[i+2]).
+ * dxh(i) = (DX(i) + DX(i-1))/2 (dual, E side), inverses, over the Alloc
+ * range [-1, n+1].  DX is indexed [-2, n+1] (DX[i+2]).
  * ------------------------------------------------------------------------ */
-void orc_spacing(int32_t n, const float *DX, float *de, float *dh, float *Ide,
-                 float *Idh) {
+void orc_spacing(int32_t n, const real *DX, real *de, real *dh, real *Ide, real *Idh) {
   for (int32_t i = -1; i <= n + 1; ++i) {
     de[i + 1] = DX[i + 2];
-    dh[i + 1] = (DX[i + 2] + DX[i + 1]) / 2.0f;
-    Ide[i + 1] = 1.0f / de[i + 1];
-    Idh[i + 1] = 1.0f / dh[i + 1];
+    dh[i + 1] = (DX[i + 2] + DX[i + 1]) / R(2.0);
+    Ide[i + 1] = R(1.0) / de[i + 1];
+    Idh[i + 1] = R(1.0) / dh[i + 1];
   }
 }
 
@@ -116,112 +123,108 @@ void orc_spacing(int32_t n, const float *DX, float *de, float *dh, float *Ide,
  * H).  sigma_max / alpha / kappa: calc_cpmlconstants, borderscpml.F90:
  * 1465-1733, including the 5/10-layer Gedney optimum (:1494-1497) and the
  * absent-side value 1e29 (:1504).  b = exp(-(s/k + a)*dt/eps0) in real(8)
- * stored real(4); c = s*(b-1)/(s + k*a)/k / delta; Id = 1/(k*delta).
+ * stored in the real kind; c = s*(b-1)/(s + k*a)/k / delta; Id = 1/(k*delta).
  * E uses i <= SIN_lo-1 / i >= SIN_hi+1 (:1545,1557); H uses i <= SIN_lo-1 /
  * i >= SIN_hi (:1627,1643), with SIN_lo = Llo, SIN_hi = n - Lhi in total
  * coordinates.  Arrays have length n+3 (index i+1).
  * ------------------------------------------------------------------------ */
-static float orc_sigmax(int32_t L, float orden, float R, float del, float eps0,
-                        float mu0) {
-  if (L == 0) return 1.0e29f;
-  if (L == 10 || L == 5)
-    return 0.8f * (orden + 1.0f) / (sqrtf(mu0 / eps0) * del);
-  return -((logf(R) * (orden + 1.0f)) /
-           (2.0f * sqrtf(mu0 / eps0) * (float)L * del));
+static real orc_sigmax(int32_t L, real orden, real R_, real del, real eps0, real mu0) {
+  if (L == 0) return R(1.0e29);
+  if (L == 10 || L == 5) return R(0.8) * (orden + R(1.0)) / (RSQRT(mu0 / eps0) * del);
+  return -((RLOG(R_) * (orden + R(1.0))) / (R(2.0) * RSQRT(mu0 / eps0) * (real)L * del));
 }
 
-void orc_cpml_axis(int32_t n, int32_t Llo, int32_t Lhi, float orden_lo,
-                   float orden_hi, float R_lo, float R_hi, float alphamaxpar,
-                   float alphaorden, float kappamaxpar, double dt, float eps0,
-                   float mu0, const float *de, const float *dh, float *pbe,
-                   float *pce, float *pbm, float *pcm, float *Ide, float *Idh) {
+void orc_cpml_axis(int32_t n, int32_t Llo, int32_t Lhi, real orden_lo, real orden_hi,
+                   real R_lo, real R_hi, real alphamaxpar, real alphaorden,
+                   real kappamaxpar, double dt, real eps0, real mu0, const real *de,
+                   const real *dh, real *pbe, real *pce, real *pbm, real *pcm, real *Ide,
+                   real *Idh) {
   const int32_t SI = Llo, SE = n - Lhi;
   /* del: dxe at Alloc(iEx) XI / XE = -1 / n (:1480-1485) */
-  float smax[2], amax[2], kmax[2];
-  float dlo = de[-1 + 1], dhi = de[n + 1];
+  real smax[2], amax[2], kmax[2];
+  real dlo = de[-1 + 1], dhi = de[n + 1];
   smax[0] = orc_sigmax(Llo, orden_lo, R_lo, dlo, eps0, mu0);
   smax[1] = orc_sigmax(Lhi, orden_hi, R_hi, dhi, eps0, mu0);
-  amax[0] = Llo ? alphamaxpar * smax[0] : 0.0f;
-  amax[1] = Lhi ? alphamaxpar * smax[1] : 0.0f;
-  kmax[0] = Llo ? kappamaxpar : 1.0f;
-  kmax[1] = Lhi ? kappamaxpar : 1.0f;
-  const float ord[2] = {orden_lo, orden_hi};
+  amax[0] = Llo ? alphamaxpar * smax[0] : R(0.0);
+  amax[1] = Lhi ? alphamaxpar * smax[1] : R(0.0);
+  kmax[0] = Llo ? kappamaxpar : R(1.0);
+  kmax[1] = Lhi ? kappamaxpar : R(1.0);
+  const real ord[2] = {orden_lo, orden_hi};
   const int32_t Ls[2] = {Llo, Lhi};
 
   for (int32_t i = -1; i <= n + 1; ++i) {
-    pbe[i + 1] = 1.0f; pce[i + 1] = 0.0f;
-    pbm[i + 1] = 1.0f; pcm[i + 1] = 0.0f;
+    pbe[i + 1] = R(1.0); pce[i + 1] = R(0.0);
+    pbm[i + 1] = R(1.0); pcm[i + 1] = R(0.0);
   }
   /* electric: i over Alloc(iEx) = [-1, n] */
   for (int32_t i = -1; i <= n; ++i) {
     int side = -1;
-    float ce = 0.0f, Ice = 0.0f;
+    real ce = R(0.0), Ice = R(0.0);
     if (i <= SI - 1) {
       side = 0;
-      ce = 1.0f * (float)(SI - i) / (float)Ls[0];
-      Ice = 1.0f * (float)(Ls[0] - (SI - i)) / (float)Ls[0];
+      ce = R(1.0) * (real)(SI - i) / (real)Ls[0];
+      Ice = R(1.0) * (real)(Ls[0] - (SI - i)) / (real)Ls[0];
     } else if (i >= SE + 1) {
       side = 1;
-      ce = 1.0f * (float)(i - SE) / (float)Ls[1];
-      Ice = 1.0f * (float)(Ls[1] - (i - SE)) / (float)Ls[1];
+      ce = R(1.0) * (real)(i - SE) / (real)Ls[1];
+      Ice = R(1.0) * (real)(Ls[1] - (i - SE)) / (real)Ls[1];
     }
     if (side < 0) continue;
-    float s, k;
-    if (ord[side] == 0.0f) {
+    real s, k;
+    if (ord[side] == R(0.0)) {
       s = smax[side];
-      k = 1.0f + (kmax[side] - 1.0f);
+      k = R(1.0) + (kmax[side] - R(1.0));
     } else {
-      s = smax[side] * powf(ce, ord[side]);
-      k = 1.0f + (kmax[side] - 1.0f) * powf(ce, ord[side]);
+      s = smax[side] * RPOW(ce, ord[side]);
+      k = R(1.0) + (kmax[side] - R(1.0)) * RPOW(ce, ord[side]);
     }
-    float a = amax[side] * powf(Ice, alphaorden);
-    float b = (float)exp((double)(-(s / k + a)) * dt / (double)eps0);
+    real a = amax[side] * RPOW(Ice, alphaorden);
+    real b = (real)exp((double)(-(s / k + a)) * dt / (double)eps0);
     pbe[i + 1] = b;
-    pce[i + 1] = (s * (b - 1.0f) / (s + k * a) / k) / dh[i + 1];
-    Idh[i + 1] = 1.0f / (k * dh[i + 1]);
+    pce[i + 1] = (s * (b - R(1.0)) / (s + k * a) / k) / dh[i + 1];
+    Idh[i + 1] = R(1.0) / (k * dh[i + 1]);
   }
   /* magnetic: i over Alloc(iHx) = [-1, n+1] */
   for (int32_t i = -1; i <= n + 1; ++i) {
     int side = -1;
-    float cm = 0.0f, Icm = 0.0f;
+    real cm = R(0.0), Icm = R(0.0);
     if (i <= SI - 1) {
       side = 0;
-      cm = 1.0f * ((float)SI - ((float)i + 0.5f)) / (float)Ls[0];
-      Icm = 1.0f * ((float)Ls[0] - ((float)SI - ((float)i + 0.5f))) / (float)Ls[0];
+      cm = R(1.0) * ((real)SI - ((real)i + R(0.5))) / (real)Ls[0];
+      Icm = R(1.0) * ((real)Ls[0] - ((real)SI - ((real)i + R(0.5)))) / (real)Ls[0];
     } else if (i >= SE) {
       side = 1;
-      cm = 1.0f * ((float)(i - SE) + 0.5f) / (float)Ls[1];
-      Icm = 1.0f * ((float)Ls[1] - ((float)(i - SE) + 0.5f)) / (float)Ls[1];
+      cm = R(1.0) * ((real)(i - SE) + R(0.5)) / (real)Ls[1];
+      Icm = R(1.0) * ((real)Ls[1] - ((real)(i - SE) + R(0.5))) / (real)Ls[1];
     }
     if (side < 0) continue;
-    float s, k;
-    if (ord[side] == 0.0f) {
+    real s, k;
+    if (ord[side] == R(0.0)) {
       s = smax[side];
-      k = 1.0f + (kmax[side] - 1.0f);
+      k = R(1.0) + (kmax[side] - R(1.0));
     } else {
-      s = smax[side] * powf(cm, ord[side]);
-      k = 1.0f + (kmax[side] - 1.0f) * powf(cm, ord[side]);
+      s = smax[side] * RPOW(cm, ord[side]);
+      k = R(1.0) + (kmax[side] - R(1.0)) * RPOW(cm, ord[side]);
     }
-    float a = amax[side] * powf(Icm, alphaorden);
-    float b = (float)exp((double)(-(s / k + a)) * dt / (double)eps0);
+    real a = amax[side] * RPOW(Icm, alphaorden);
+    real b = (real)exp((double)(-(s / k + a)) * dt / (double)eps0);
     pbm[i + 1] = b;
-    pcm[i + 1] = (s * (b - 1.0f) / (s + k * a) / k) / de[i + 1];
-    Ide[i + 1] = 1.0f / (k * de[i + 1]);
+    pcm[i + 1] = (s * (b - R(1.0)) / (s + k * a) / k) / de[i + 1];
+    Ide[i + 1] = R(1.0) / (k * de[i + 1]);
   }
 }
 
 /* --------------------------------------------------------------------------
  * Pole-residue ADE coefficients: electricdispersive.F90:98-146.
  * kappa_p = (1 + a_p dt/2)/(1 - a_p dt/2), beta_p = c_p dt/(1 - a_p dt/2)
- * (complex(8)); tempo = sum Re(beta_p) accumulated in real(4);
+ * (complex(8)); tempo = sum Re(beta_p) accumulated in the real kind;
  * G1 = (2 eps + tempo - s dt)/(2 eps + tempo + s dt),
- * G2 = 2 dt/(2 eps + tempo + s dt)   (real(8) expressions stored real(4));
- * G3_p = G2/2 * (1 + kappa_p) with G2/2 in real(4).
+ * G2 = 2 dt/(2 eps + tempo + s dt)   (real(8) expressions stored real);
+ * G3_p = G2/2 * (1 + kappa_p) with G2/2 in the real kind.
  * Complex division uses Smith's range-reduced form (gfortran's
  * -fcx-fortran-rules lowering).  Arrays are [p][2] (re, im).
  * ------------------------------------------------------------------------ */
-static void cdiv(double a, double b, double c, double d, double *re,
-                 double *im) {
+static void cdiv(double a, double b, double c, double d, double *re, double *im) {
   if (fabs(c) >= fabs(d)) {
     double r = d / c, den = c + d * r;
     *re = (a + b * r) / den;
@@ -233,12 +236,12 @@ static void cdiv(double a, double b, double c, double d, double *re,
   }
 }
 
-void orc_ade_tables(int32_t npoles, const double *c, const double *a,
-                    float eps11, float sigma11, double dt, double *kappa,
-                    double *beta, double *g3, float *g1_out, float *g2_out) {
+void orc_ade_tables(int32_t npoles, const double *c, const double *a, real eps11,
+                    real sigma11, double dt, double *kappa, double *beta, double *g3,
+                    real *g1_out, real *g2_out) {
   for (int32_t p = 0; p < npoles; ++p) {
     double ar = a[2 * p], ai = a[2 * p + 1];
-    /* a*dt/2 : complex * real(8), then / real(4) 2.0 */
+    /* a*dt/2 : complex * real(8), then / 2.0 */
     double hr = ar * dt / 2.0, hi = ai * dt / 2.0;
     double nr = 1.0 + hr, ni = hi;
     double dr = 1.0 - hr, di = -hi;
@@ -246,15 +249,15 @@ void orc_ade_tables(int32_t npoles, const double *c, const double *a,
     double cr = c[2 * p] * dt, ci = c[2 * p + 1] * dt;
     cdiv(cr, ci, dr, di, &beta[2 * p], &beta[2 * p + 1]);
   }
-  float tempo = 0.0f;
-  for (int32_t p = 0; p < npoles; ++p) tempo = (float)((double)tempo + beta[2 * p]);
+  real tempo = R(0.0);
+  for (int32_t p = 0; p < npoles; ++p) tempo = (real)((double)tempo + beta[2 * p]);
   double sdt = (double)sigma11 * dt;
-  float e2 = 2.0f * eps11 + tempo;
-  float g1 = (float)(((double)e2 - sdt) / ((double)e2 + sdt));
-  float g2 = (float)(2.0 * dt / ((double)e2 + sdt));
+  real e2 = R(2.0) * eps11 + tempo;
+  real g1 = (real)(((double)e2 - sdt) / ((double)e2 + sdt));
+  real g2 = (real)(2.0 * dt / ((double)e2 + sdt));
   *g1_out = g1;
   *g2_out = g2;
-  float h = g2 / 2.0f;
+  real h = g2 / R(2.0);
   for (int32_t p = 0; p < npoles; ++p) {
     double kr = 1.0 + kappa[2 * p], ki = kappa[2 * p + 1];
     g3[2 * p] = (double)h * kr;
@@ -272,12 +275,13 @@ void orc_time_vector(int32_t nlast, double dt, double *tiempo) {
 /* --------------------------------------------------------------------------
  * Yee E update: Advance_Ex/Ey/Ez, timestepping.F90:2165-2285.
  * Sweeps (total coordinates): Ex x[0,nx-1] y[0,ny] z[0,nz]; Ey x[0,nx]
- * y[0,ny-1] z[0,nz]; Ez x[0,nx] y[0,ny] z[0,nz-1].
+ * y[0,ny-1] z[0,nz]; Ez x[0,nx] y[0,ny] z[0,nz-1].  OpenMP over (k, j) as
+ * the reference's collapse(2) loops.
  * ------------------------------------------------------------------------ */
 void orc_yee_e(orc_grid *g) {
-  float *Ex = g->f[0], *Ey = g->f[1], *Ez = g->f[2];
-  const float *Hx = g->f[3], *Hy = g->f[4], *Hz = g->f[5];
-  const float *Idxh = g->id[3], *Idyh = g->id[4], *Idzh = g->id[5];
+  real *Ex = g->f[0], *Ey = g->f[1], *Ez = g->f[2];
+  const real *Hx = g->f[3], *Hy = g->f[4], *Hz = g->f[5];
+  const real *Idxh = g->id[3], *Idyh = g->id[4], *Idzh = g->id[5];
   const size_t sx = 1, sy = AX(g), sz = AX(g) * AY(g);
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k <= g->nz; ++k)
@@ -312,9 +316,9 @@ void orc_yee_e(orc_grid *g) {
  * Sweeps: Hx x[0,nx] y[0,ny-1] z[0,nz-1]; Hy x[0,nx-1] y[0,ny] z[0,nz-1];
  * Hz x[0,nx-1] y[0,ny-1] z[0,nz]. */
 void orc_yee_h(orc_grid *g) {
-  const float *Ex = g->f[0], *Ey = g->f[1], *Ez = g->f[2];
-  float *Hx = g->f[3], *Hy = g->f[4], *Hz = g->f[5];
-  const float *Idxe = g->id[0], *Idye = g->id[1], *Idze = g->id[2];
+  const real *Ex = g->f[0], *Ey = g->f[1], *Ez = g->f[2];
+  real *Hx = g->f[3], *Hy = g->f[4], *Hz = g->f[5];
+  const real *Idxe = g->id[0], *Idye = g->id[1], *Idze = g->id[2];
   const size_t sx = 1, sy = AX(g), sz = AX(g) * AY(g);
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k <= g->nz - 1; ++k)
@@ -378,19 +382,19 @@ static int slab_box(const orc_grid *g, int c, int a, int s, box3 *b) {
 }
 
 /* one E (or H) psi loop over a slab; sign applies to the field correction.
- * tgt: field updated; src: field differenced along axis a; forward: H-side
- * difference src(p+s)-src(p), otherwise src(p)-src(p-s). */
-static void cpml_loop(orc_grid *g, int c, int a, int side, int psislot,
-                      int src, int forward, float sign) {
+ * src: field differenced along axis a; forward: H-side difference
+ * src(p+s)-src(p), otherwise src(p)-src(p-s). */
+static void cpml_loop(orc_grid *g, int c, int a, int side, int psislot, int src,
+                      int forward, int sign) {
   box3 b;
   if (!slab_box(g, c, a, side, &b)) return;
-  float *F = g->f[c];
-  const float *S = g->f[src];
-  float *psi = g->psi[psislot];
+  real *F = g->f[c];
+  const real *S = g->f[src];
+  real *psi = g->psi[psislot];
   const int16_t *mi = g->mi[c];
-  const float *gg = (c < 3) ? g->g2 : g->gm2;
-  const float *pb = (c < 3) ? g->pbe[a] : g->pbm[a];
-  const float *pc = (c < 3) ? g->pce[a] : g->pcm[a];
+  const real *gg = (c < 3) ? g->g2 : g->gm2;
+  const real *pb = (c < 3) ? g->pbe[a] : g->pbm[a];
+  const real *pc = (c < 3) ? g->pce[a] : g->pcm[a];
   const size_t st[3] = {1, AX(g), AX(g) * AY(g)};
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = b.lo[2]; k <= b.hi[2]; ++k)
@@ -398,7 +402,7 @@ static void cpml_loop(orc_grid *g, int c, int a, int side, int psislot,
       for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
         size_t p = IDX(g, i, j, k);
         int q = a == 0 ? i : (a == 1 ? j : k);
-        float d = forward ? (S[p + st[a]] - S[p]) : (S[p] - S[p - st[a]]);
+        real d = forward ? (S[p + st[a]] - S[p]) : (S[p] - S[p - st[a]]);
         psi[p] = pb[q + 1] * psi[p] + d * pc[q + 1];
         int m = mi[p];
         if (sign > 0)
@@ -410,40 +414,41 @@ static void cpml_loop(orc_grid *g, int c, int a, int side, int psislot,
 
 void orc_cpml_e(orc_grid *g) {
   for (int s = 0; s < 2; ++s) { /* left, right */
-    cpml_loop(g, ORC_EX, 1, s, ORC_PSI_EXY, ORC_HZ, 0, +1.0f);
-    cpml_loop(g, ORC_EZ, 1, s, ORC_PSI_EZY, ORC_HX, 0, -1.0f);
+    cpml_loop(g, ORC_EX, 1, s, ORC_PSI_EXY, ORC_HZ, 0, +1);
+    cpml_loop(g, ORC_EZ, 1, s, ORC_PSI_EZY, ORC_HX, 0, -1);
   }
   for (int s = 0; s < 2; ++s) { /* down, up */
-    cpml_loop(g, ORC_EY, 2, s, ORC_PSI_EYZ, ORC_HX, 0, +1.0f);
-    cpml_loop(g, ORC_EX, 2, s, ORC_PSI_EXZ, ORC_HY, 0, -1.0f);
+    cpml_loop(g, ORC_EY, 2, s, ORC_PSI_EYZ, ORC_HX, 0, +1);
+    cpml_loop(g, ORC_EX, 2, s, ORC_PSI_EXZ, ORC_HY, 0, -1);
   }
   for (int s = 0; s < 2; ++s) { /* back, front */
-    cpml_loop(g, ORC_EZ, 0, s, ORC_PSI_EZX, ORC_HY, 0, +1.0f);
-    cpml_loop(g, ORC_EY, 0, s, ORC_PSI_EYX, ORC_HZ, 0, -1.0f);
+    cpml_loop(g, ORC_EZ, 0, s, ORC_PSI_EZX, ORC_HY, 0, +1);
+    cpml_loop(g, ORC_EY, 0, s, ORC_PSI_EYX, ORC_HZ, 0, -1);
   }
 }
 
 void orc_cpml_h(orc_grid *g) {
   for (int s = 0; s < 2; ++s) { /* left, right */
-    cpml_loop(g, ORC_HX, 1, s, ORC_PSI_HXY, ORC_EZ, 1, -1.0f);
-    cpml_loop(g, ORC_HZ, 1, s, ORC_PSI_HZY, ORC_EX, 1, +1.0f);
+    cpml_loop(g, ORC_HX, 1, s, ORC_PSI_HXY, ORC_EZ, 1, -1);
+    cpml_loop(g, ORC_HZ, 1, s, ORC_PSI_HZY, ORC_EX, 1, +1);
   }
   for (int s = 0; s < 2; ++s) { /* down, up */
-    cpml_loop(g, ORC_HY, 2, s, ORC_PSI_HYZ, ORC_EX, 1, -1.0f);
-    cpml_loop(g, ORC_HX, 2, s, ORC_PSI_HXZ, ORC_EY, 1, +1.0f);
+    cpml_loop(g, ORC_HY, 2, s, ORC_PSI_HYZ, ORC_EX, 1, -1);
+    cpml_loop(g, ORC_HX, 2, s, ORC_PSI_HXZ, ORC_EY, 1, +1);
   }
   for (int s = 0; s < 2; ++s) { /* back, front */
-    cpml_loop(g, ORC_HZ, 0, s, ORC_PSI_HZX, ORC_EY, 1, -1.0f);
-    cpml_loop(g, ORC_HY, 0, s, ORC_PSI_HYX, ORC_EZ, 1, +1.0f);
+    cpml_loop(g, ORC_HZ, 0, s, ORC_PSI_HZX, ORC_EY, 1, -1);
+    cpml_loop(g, ORC_HY, 0, s, ORC_PSI_HYX, ORC_EZ, 1, +1);
   }
 }
 
 /* --------------------------------------------------------------------------
  * ADE: AdvanceEDispersiveE, electricdispersive.F90:304-362.  Per dispersive
  * edge: E <- E - Re(G3_p J_p) one pole at a time (real(8) product, rounded to
- * real(4) after each pole); then J_p <- kappa_p J_p + beta_p (E - Eprev)/dt
- * with (E - Eprev) in real(4); then Eprev <- E.  Edges are those of the E
- * sweeps whose media index equals the dispersive medium (:148-244).
+ * the real kind after each pole); then J_p <- kappa_p J_p + beta_p
+ * (E - Eprev)/dt with (E - Eprev) in the real kind; then Eprev <- E.  Edges
+ * are those of the E sweeps whose media index equals the dispersive medium
+ * (:148-244); the reference walks them serially.
  * ------------------------------------------------------------------------ */
 void orc_ade_e(orc_grid *g) {
   for (int d = 0; d < g->ndisp; ++d) {
@@ -453,8 +458,8 @@ void orc_ade_e(orc_grid *g) {
     const double *B = g->disp_beta + (size_t)d * ORC_PMAX * 2;
     const double *G = g->disp_g3 + (size_t)d * ORC_PMAX * 2;
     for (int c = 0; c < 3; ++c) {
-      float *F = g->f[c];
-      float *Ep = g->eprev[c];
+      real *F = g->f[c];
+      real *Ep = g->eprev[c];
       double *J = g->jcur[c];
       int xe = sweep_hi(g, c, 0), ye = sweep_hi(g, c, 1), ze = sweep_hi(g, c, 2);
       for (int k = 0; k <= ze; ++k)
@@ -463,13 +468,13 @@ void orc_ade_e(orc_grid *g) {
             size_t p = IDX(g, i, j, k);
             if (g->mi[c][p] != med) continue;
             double *Jp = J + p * (size_t)g->jstride * 2;
-            float e = F[p];
+            real e = F[p];
             for (int q = 0; q < np; ++q) {
               double re = G[2 * q] * Jp[2 * q] - G[2 * q + 1] * Jp[2 * q + 1];
-              e = (float)((double)e - re);
+              e = (real)((double)e - re);
             }
             F[p] = e;
-            float diff = e - Ep[p];
+            real diff = e - Ep[p];
             for (int q = 0; q < np; ++q) {
               double kr = K[2 * q], ki = K[2 * q + 1];
               double jr = Jp[2 * q], ji = Jp[2 * q + 1];
@@ -489,29 +494,29 @@ void orc_ade_e(orc_grid *g) {
 /* --------------------------------------------------------------------------
  * Nodal sources: evolucion (linear interpolation of the excitation table,
  * nodalsources.F90:317-356) and AdvancenodalE/H (:396-578, :584-777).
- * E time stamp tiempo(n) cast to real(4) (:426); H stamp tiempo(n)+0.5*dt
- * (:618).  Hard: F = amp*f(t); soft: F = F - G2*Id_a*Id_b*amp*f(t) with the
- * reference's factor order.  PEC (E) / PMC (H) media are skipped; the H hard
- * sources read the Hx media matrix for Hy and Hz as the reference does
- * (:641,:715).
+ * E time stamp tiempo(n) cast to the real kind (:426); H stamp
+ * tiempo(n)+0.5*dt (:618).  Hard: F = amp*f(t); soft: F = F - G2*Id_a*Id_b*
+ * amp*f(t) with the reference's factor order.  PEC (E) / PMC (H) media are
+ * skipped; the H hard sources read the Hx media matrix for Hy and Hz as the
+ * reference does (:641,:715).
  * ------------------------------------------------------------------------ */
-float orc_evolucion(float t, const orc_nodal *s) {
-  float de = s->deltaevol;
+real orc_evolucion(real t, const orc_nodal *s) {
+  real de = s->deltaevol;
   long long nprev = (long long)(t / de);
-  if (nprev + 1 > s->numus || nprev + 1 <= 0) return 0.0f;
-  return (s->evol[nprev + 1] - s->evol[nprev]) / de * (t - (float)nprev * de) +
+  if (nprev + 1 > s->numus || nprev + 1 <= 0) return R(0.0);
+  return (s->evol[nprev + 1] - s->evol[nprev]) / de * (t - (real)nprev * de) +
          s->evol[nprev];
 }
 
-static void nodal_apply(orc_grid *g, const orc_nodal *s, float t) {
+static void nodal_apply(orc_grid *g, const orc_nodal *s, real t) {
   const int c = s->comp;
-  float *F = g->f[c];
+  real *F = g->f[c];
   const int16_t *mi = g->mi[(c >= 4 && s->hard) ? 3 : c];
   const int skipbit = c < 3 ? ORC_KIND_PEC : ORC_KIND_PMC;
-  const float *gg = c < 3 ? g->g2 : g->gm2;
-  const float *Ix = c < 3 ? g->id[3] : g->id[0];
-  const float *Iy = c < 3 ? g->id[4] : g->id[1];
-  const float *Iz = c < 3 ? g->id[5] : g->id[2];
+  const real *gg = c < 3 ? g->g2 : g->gm2;
+  const real *Ix = c < 3 ? g->id[3] : g->id[0];
+  const real *Iy = c < 3 ? g->id[4] : g->id[1];
+  const real *Iz = c < 3 ? g->id[5] : g->id[2];
   const int cc = c % 3;
   for (int k = s->zi; k <= s->ze; ++k)
     for (int j = s->yi; j <= s->ye; ++j)
@@ -522,7 +527,7 @@ static void nodal_apply(orc_grid *g, const orc_nodal *s, float t) {
         if (s->hard) {
           F[p] = s->amp * orc_evolucion(t, s);
         } else {
-          float ia, ib;
+          real ia, ib;
           if (cc == 0) { ia = Iy[j + 1]; ib = Iz[k + 1]; }
           else if (cc == 1) { ia = Ix[i + 1]; ib = Iz[k + 1]; }
           else { ia = Iy[j + 1]; ib = Ix[i + 1]; }
@@ -532,7 +537,7 @@ static void nodal_apply(orc_grid *g, const orc_nodal *s, float t) {
 }
 
 void orc_nodal_e(orc_grid *g, int32_t n) {
-  float t = (float)g->tiempo[n];
+  real t = (real)g->tiempo[n];
   for (int c = 0; c < 3; ++c)
     for (int hard = 1; hard >= 0; --hard)
       for (int s = 0; s < g->nnodal; ++s)
@@ -541,7 +546,7 @@ void orc_nodal_e(orc_grid *g, int32_t n) {
 }
 
 void orc_nodal_h(orc_grid *g, int32_t n) {
-  float t = (float)(g->tiempo[n] + (double)0.5f * g->dt);
+  real t = (real)(g->tiempo[n] + (double)R(0.5) * g->dt);
   for (int c = 3; c < 6; ++c)
     for (int hard = 1; hard >= 0; --hard)
       for (int s = 0; s < g->nnodal; ++s)
@@ -561,9 +566,8 @@ void orc_step(orc_grid *g, int32_t n) {
   if (g->nnodal) orc_nodal_h(g, n);
 }
 
-void orc_run(orc_grid *g, int32_t n0, int32_t nsteps, int32_t nprobe,
-             const int32_t *pc, const int32_t *pi, const int32_t *pj,
-             const int32_t *pk, float *out) {
+void orc_run(orc_grid *g, int32_t n0, int32_t nsteps, int32_t nprobe, const int32_t *pc,
+             const int32_t *pi, const int32_t *pj, const int32_t *pk, real *out) {
   for (int32_t s = 0; s < nsteps; ++s) {
     orc_step(g, n0 + s);
     for (int32_t p = 0; p < nprobe; ++p)

@@ -13,6 +13,9 @@
  * nx/ny/nz count the interior plus both CPML slabs.  1-D spacing tables cover
  * [-1, n+1] (index i+1); the raw cell-size array DX covers [-2, n+1]
  * (index i+2), as sgg%dx does (preprocess_geom.F90:5639-5666).
+ *
+ * orc_real is the reference's RKIND: real(4) by default, real(8) when built
+ * with -DORC_REAL64 (the FP64 yardstick).
  */
 #ifndef FDTD_ORACLE_H
 #define FDTD_ORACLE_H
@@ -21,6 +24,12 @@
 
 #ifdef __cplusplus
 extern "C" {
+#endif
+
+#ifdef ORC_REAL64
+typedef double orc_real;
+#else
+typedef float orc_real;
 #endif
 
 #define ORC_PMAX 8 /* max poles per dispersive medium */
@@ -36,23 +45,23 @@ typedef struct {
   int32_t comp;        /* ORC_EX..ORC_HZ */
   int32_t hard;        /* 1: E = amp*f(t) ; 0: soft current E -= G2*Id*Id*amp*f(t) */
   int32_t xi, xe, yi, ye, zi, ze; /* box, already clipped to the sweep */
-  float amp;
-  float deltaevol;     /* sample spacing of the excitation table */
+  orc_real amp;
+  orc_real deltaevol;  /* sample spacing of the excitation table */
   int32_t numus;       /* samples are evol[0..numus] */
-  const float *evol;
+  const orc_real *evol;
 } orc_nodal;
 
 typedef struct {
   int32_t nx, ny, nz;
-  float *f[6];                 /* Ex Ey Ez Hx Hy Hz, Alloc layout */
+  orc_real *f[6];              /* Ex Ey Ez Hx Hy Hz, Alloc layout */
   const int16_t *mi[6];        /* media index per component, Alloc layout */
-  const float *g1, *g2, *gm1, *gm2; /* 0..nmedia */
+  const orc_real *g1, *g2, *gm1, *gm2; /* 0..nmedia */
   int32_t nmedia;
-  const float *id[6];          /* Idxe Idye Idze Idxh Idyh Idzh, length n+3 */
+  const orc_real *id[6];       /* Idxe Idye Idze Idxh Idyh Idzh, length n+3 */
   /* CPML */
   int32_t L[3][2];             /* layers (axis, side) ; 0 = none */
-  const float *pbe[3], *pce[3], *pbm[3], *pcm[3]; /* per axis, length n+3 */
-  float *psi[12];              /* full Alloc-sized arrays, only slab entries used */
+  const orc_real *pbe[3], *pce[3], *pbm[3], *pcm[3]; /* per axis, length n+3 */
+  orc_real *psi[12];           /* full Alloc-sized arrays, only slab entries used */
   /* electric pole-residue ADE */
   int32_t ndisp;
   const int32_t *disp_medium;  /* [ndisp] media index */
@@ -60,7 +69,7 @@ typedef struct {
   const double *disp_kappa;    /* [ndisp][ORC_PMAX][2] */
   const double *disp_beta;     /* [ndisp][ORC_PMAX][2] */
   const double *disp_g3;       /* [ndisp][ORC_PMAX][2] */
-  float *eprev[3];             /* Alloc-sized, per E component */
+  orc_real *eprev[3];          /* Alloc-sized, per E component */
   double *jcur[3];             /* Alloc-sized * jstride * 2 per E component */
   int32_t jstride;             /* pole slots per edge (>= max poles) */
   double dt;
@@ -71,25 +80,25 @@ typedef struct {
 } orc_grid;
 
 /* ---- coefficient tables ---- */
-double orc_dt(float dxmin, float dymin, float dzmin, float eps0, float mu0,
-              float cfl, int32_t forcecfl, double dt_in);
-void orc_g_tables(int32_t nmedia, const float *epr, const float *sigma,
-                  const float *mur, const float *sigmam, const int32_t *kind,
-                  double dt, float eps0, float mu0,
-                  float *g1, float *g2, float *gm1, float *gm2);
-void orc_spacing(int32_t n, const float *DX, float *de, float *dh,
-                 float *Ide, float *Idh);
+double orc_dt(orc_real dxmin, orc_real dymin, orc_real dzmin, orc_real eps0, orc_real mu0,
+              orc_real cfl, int32_t forcecfl, double dt_in);
+void orc_g_tables(int32_t nmedia, const orc_real *epr, const orc_real *sigma,
+                  const orc_real *mur, const orc_real *sigmam, const int32_t *kind,
+                  double dt, orc_real eps0, orc_real mu0,
+                  orc_real *g1, orc_real *g2, orc_real *gm1, orc_real *gm2);
+void orc_spacing(int32_t n, const orc_real *DX, orc_real *de, orc_real *dh,
+                 orc_real *Ide, orc_real *Idh);
 void orc_cpml_axis(int32_t n, int32_t Llo, int32_t Lhi,
-                   float orden_lo, float orden_hi, float R_lo, float R_hi,
-                   float alphamaxpar, float alphaorden, float kappamaxpar,
-                   double dt, float eps0, float mu0,
-                   const float *de, const float *dh,
-                   float *pbe, float *pce, float *pbm, float *pcm,
-                   float *Ide, float *Idh);
+                   orc_real orden_lo, orc_real orden_hi, orc_real R_lo, orc_real R_hi,
+                   orc_real alphamaxpar, orc_real alphaorden, orc_real kappamaxpar,
+                   double dt, orc_real eps0, orc_real mu0,
+                   const orc_real *de, const orc_real *dh,
+                   orc_real *pbe, orc_real *pce, orc_real *pbm, orc_real *pcm,
+                   orc_real *Ide, orc_real *Idh);
 void orc_ade_tables(int32_t npoles, const double *c_re_im, const double *a_re_im,
-                    float eps11, float sigma11, double dt,
+                    orc_real eps11, orc_real sigma11, double dt,
                     double *kappa, double *beta, double *g3,
-                    float *g1_out, float *g2_out);
+                    orc_real *g1_out, orc_real *g2_out);
 void orc_time_vector(int32_t nlast, double dt, double *tiempo);
 
 /* ---- stepping ---- */
@@ -98,7 +107,7 @@ void orc_yee_h(orc_grid *g);
 void orc_cpml_e(orc_grid *g);
 void orc_cpml_h(orc_grid *g);
 void orc_ade_e(orc_grid *g);
-float orc_evolucion(float t, const orc_nodal *s);
+orc_real orc_evolucion(orc_real t, const orc_nodal *s);
 void orc_nodal_e(orc_grid *g, int32_t n);
 void orc_nodal_h(orc_grid *g, int32_t n);
 /* one full step n (E half, then H half) in the reference order */
@@ -107,7 +116,7 @@ void orc_step(orc_grid *g, int32_t n);
  * probe p reads component pc[p] at (pi,pj,pk) into out[s*nprobe + p] */
 void orc_run(orc_grid *g, int32_t n0, int32_t nsteps, int32_t nprobe,
              const int32_t *pc, const int32_t *pi, const int32_t *pj,
-             const int32_t *pk, float *out);
+             const int32_t *pk, orc_real *out);
 
 #ifdef __cplusplus
 }

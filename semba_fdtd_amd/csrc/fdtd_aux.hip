@@ -4,7 +4,6 @@
 // Built with -ffp-contract=off: the soft-source update keeps the reference's
 // rounding order.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 #include "fdtd_internal.h"
 
 namespace sfd {
@@ -70,33 +69,60 @@ int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
   return (int)hipGetLastError();
 }
 
-// number of dispersive edges of component comp in every 64-wide x segment
-// of every row (rows = (j,k) of the component's sweep).
-__global__ void k_count_disp(DevGrid g, int comp, int ylast, int zlast, int xlast,
-                             int32_t *counts) {
-  const int lane = threadIdx.x;
-  const long long w = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  const long long nrows = (long long)(ylast + 1) * (zlast + 1);
-  if (w >= nrows * g.nseg) return;
-  const int seg = (int)(w % g.nseg);
-  const long long row = w / g.nseg;
-  const int j = (int)(row % (ylast + 1)), k = (int)(row / (ylast + 1));
-  const int i = seg * kWave + (lane & (kWave - 1));
-  bool d = false;
-  if (i <= xlast) d = g.dslot[g.mi[comp][fidx(g, i, j, k)]] >= 0;
-  const unsigned long long m = __ballot(d);
-  if ((lane & (kWave - 1)) == 0) counts[w] = __popcll(m);
+// bounding box of the dispersive edges of component comp over its sweep
+// (the node lists InitEDispersives gathers, electricdispersive.F90:148-244).
+// Each thread keeps a running min/max, the block reduces in LDS and one lane
+// folds the block result into out[] with global atomics.
+__global__ void k_disp_bbox(DevGrid g, int comp, int xlast, int ylast, int zlast,
+                            int32_t *out) {
+  __shared__ int red[7][256];
+  int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+  int hi[3] = {-0x7fffffff - 1, -0x7fffffff - 1, -0x7fffffff - 1};
+  int cnt = 0;
+  const long long nxr = xlast + 1, nyr = ylast + 1;
+  const long long tot = nxr * nyr * (zlast + 1);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % nxr);
+    const int j = (int)((t / nxr) % nyr);
+    const int k = (int)(t / (nxr * nyr));
+    if (g.dslot[g.mi[comp][fidx(g, i, j, k)]] >= 0) {
+      lo[0] = min(lo[0], i); lo[1] = min(lo[1], j); lo[2] = min(lo[2], k);
+      hi[0] = max(hi[0], i); hi[1] = max(hi[1], j); hi[2] = max(hi[2], k);
+      ++cnt;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    red[a][threadIdx.x] = lo[a];
+    red[3 + a][threadIdx.x] = hi[a];
+  }
+  red[6][threadIdx.x] = cnt;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      for (int a = 0; a < 3; ++a) {
+        red[a][threadIdx.x] = min(red[a][threadIdx.x], red[a][threadIdx.x + w]);
+        red[3 + a][threadIdx.x] = max(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + w]);
+      }
+      red[6][threadIdx.x] += red[6][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&out[a], red[a][0]);
+      atomicMax(&out[3 + a], red[3 + a][0]);
+    }
+    atomicAdd(&out[6], red[6][0]);
+  }
 }
 
-int launch_count_disp(const DevGrid &g, int comp, int32_t *counts, void *stream) {
+int launch_disp_bbox(const DevGrid &g, int comp, int32_t *out, void *stream) {
   const int xl = comp == 0 ? g.nx - 1 : g.nx;
   const int yl = comp == 1 ? g.ny - 1 : g.ny;
   const int zl = comp == 2 ? g.nz - 1 : g.nz;
-  const long long nw = (long long)(yl + 1) * (zl + 1) * g.nseg;
-  const int wpb = 4;
-  const long long blocks = (nw + wpb - 1) / wpb;
-  hipLaunchKernelGGL(k_count_disp, dim3((unsigned)blocks), dim3(kWave * wpb), 0,
-                     (hipStream_t)stream, g, comp, yl, zl, xl, counts);
+  hipLaunchKernelGGL(k_disp_bbox, dim3(2048), dim3(256), 0, (hipStream_t)stream, g, comp,
+                     xl, yl, zl, out);
   return (int)hipGetLastError();
 }
 
@@ -107,11 +133,6 @@ int launch_tick(int *dn, void *stream) {
   return (int)hipGetLastError();
 }
 
-int exclusive_scan_i32(const int32_t *in, int32_t *out, int n, void *tmp,
-                       size_t *tmp_bytes, void *stream) {
-  return (int)hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, n,
-                                               (hipStream_t)stream);
-}
 
 __global__ void k_fill_i16(int16_t *a, long long n, int16_t v) {
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;

@@ -57,6 +57,7 @@ struct fdtd_handle {
   std::vector<double> dk, db, dg3;
   bool ade_ready = false;
   std::vector<void *> ade_allocs;
+  long long nedges[3] = {0, 0, 0};  // dispersive edges per E component
   // time
   double dt = 0.0;
   std::vector<double> tiempo;
@@ -221,41 +222,44 @@ int build_ade(fdtd_handle *h) {
   }
   int maxp = 0;
   for (int d = 0; d < h->ndisp; ++d) maxp = std::max(maxp, anp[d]);
-  // segment offsets per E component
-  h->g.nseg = (h->g.nx + 1 + kWave - 1) / kWave;
+  // dense state box per E component over its dispersive edges
+  int32_t *bb = nullptr;
+  if ((rc = dalloc(h, (void **)&bb, 8 * 4, &h->ade_allocs))) return rc;
   for (int c = 0; c < 3; ++c) {
-    const int yl = c == 1 ? h->g.ny - 1 : h->g.ny;
-    const int zl = c == 2 ? h->g.nz - 1 : h->g.nz;
-    const long long nseg_tot = (long long)(yl + 1) * (zl + 1) * h->g.nseg;
-    if (nseg_tot + 1 > 0x7fffffffLL) return fail(h, FDTD_ERR_ARG, "grid too large for segment table");
-    int32_t *counts = nullptr, *off = nullptr;
-    if ((rc = dalloc(h, (void **)&counts, (nseg_tot + 1) * 4, &h->ade_allocs))) return rc;
-    if ((rc = dalloc(h, (void **)&off, (nseg_tot + 1) * 4, &h->ade_allocs))) return rc;
-    if ((rc = dzero(h, counts, (nseg_tot + 1) * 4))) return rc;
-    HIPCHK(h, (hipError_t)launch_count_disp(h->g, c, counts, h->stream));
-    size_t tb = 0;
-    HIPCHK(h, (hipError_t)exclusive_scan_i32(counts, off, (int)nseg_tot + 1, nullptr, &tb, h->stream));
-    void *tmp = nullptr;
-    if ((rc = dalloc(h, &tmp, tb, &h->ade_allocs))) return rc;
-    HIPCHK(h, (hipError_t)exclusive_scan_i32(counts, off, (int)nseg_tot + 1, tmp, &tb, h->stream));
-    int32_t total = 0;
-    HIPCHK(h, hipMemcpyAsync(&total, off + nseg_tot, 4, hipMemcpyDeviceToHost, h->stream));
+    const int32_t init[7] = {0x7fffffff, 0x7fffffff, 0x7fffffff, (int32_t)0x80000000,
+                             (int32_t)0x80000000, (int32_t)0x80000000, 0};
+    HIPCHK(h, hipMemcpy(bb, init, sizeof init, hipMemcpyHostToDevice));
+    HIPCHK(h, (hipError_t)launch_disp_bbox(h->g, c, bb, h->stream));
+    int32_t box[7];
+    HIPCHK(h, hipMemcpyAsync(box, bb, sizeof box, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->g.segoff[c] = off;
+    h->nedges[c] = box[6];
+    long long total = 0;
+    if (box[0] <= box[3]) {
+      for (int a = 0; a < 3; ++a) {
+        h->g.abox[c][a] = box[a];
+        h->g.abox[c][3 + a] = box[3 + a] - box[a] + 1;
+      }
+      total = (long long)h->g.abox[c][3] * h->g.abox[c][4] * h->g.abox[c][5];
+    } else {
+      for (int a = 0; a < 6; ++a) h->g.abox[c][a] = 0;  // empty box: no edge inside
+    }
     h->g.nstate[c] = total;
     float *ep = nullptr;
     void *js = nullptr;
-    const size_t jbytes = (size_t)std::max(total, 1) * maxp *
-                          (h->mode == FDTD_MODE_EXACT ? 16 : 8);
-    if ((rc = dalloc(h, (void **)&ep, (size_t)std::max(total, 1) * 4, &h->ade_allocs))) return rc;
+    const size_t ns = (size_t)std::max(total, 1LL);
+    const size_t jbytes = ns * std::max(maxp, 1) * (h->mode == FDTD_MODE_EXACT ? 16 : 8);
+    if ((rc = dalloc(h, (void **)&ep, ns * 4, &h->ade_allocs))) return rc;
     if ((rc = dalloc(h, &js, jbytes, &h->ade_allocs))) return rc;
-    if ((rc = dzero(h, ep, (size_t)std::max(total, 1) * 4))) return rc;
+    if ((rc = dzero(h, ep, ns * 4))) return rc;
     if ((rc = dzero(h, js, jbytes))) return rc;
     h->g.eprev[c] = ep;
     h->g.jst[c] = js;
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->g.has_ade = 1;
+  h->g.ndisp = h->ndisp;
+  // no dispersive edge anywhere: run the passes without the pole stage
+  h->g.has_ade = (h->nedges[0] + h->nedges[1] + h->nedges[2]) > 0 ? 1 : 0;
   h->ade_ready = true;
   return FDTD_OK;
 }
@@ -448,8 +452,9 @@ int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
   }
   // tiles: 64 x 4 threads, z chunks of 64 planes (or fewer for thin grids)
   TileMap &tm = h->tm;
+  tm.bdy = 4;
   tm.tx = (d->nx + 1 + kWave - 1) / kWave;
-  tm.ty = (d->ny + 1 + 3) / 4;
+  tm.ty = (d->ny + 1 + tm.bdy - 1) / tm.bdy;
   tm.kc = std::min(64, d->nz + 1);
   tm.tz = (d->nz + 1 + tm.kc - 1) / tm.kc;
   tm.nblk = tm.tx * tm.ty * tm.tz;
@@ -806,7 +811,7 @@ int64_t fdtd_device_bytes(const fdtd_handle *h) { return h ? h->bytes : 0; }
 
 int64_t fdtd_dispersive_edges(const fdtd_handle *h, int32_t comp) {
   if (!h || comp < 0 || comp > 2 || !h->ade_ready) return -1;
-  return h->g.has_ade ? (int64_t)h->g.nstate[comp] : 0;
+  return h->g.has_ade ? (int64_t)h->nedges[comp] : 0;
 }
 
 int fdtd_media_fill(fdtd_handle *h, int16_t medium) {

@@ -14,10 +14,12 @@
 //               stretched axis only the low and high slab layers are stored
 //               (Llo + Lhi entries), the two other axes span the component's
 //               sweep; x-fastest.
-//   pole state  per E component a compact list of the dispersive edges in
-//               (k, j, i) order.  segoff[c][row*nseg + s] is the index of the
-//               first dispersive edge of the 64-wide x segment s of row
-//               (j, k); a wave finds its lanes' slots with one ballot.
+//   pole state  per E component a dense x-fastest box spanning the
+//               component's dispersive edges (Eprev and one complex state per
+//               stored pole); an edge finds its slot from its coordinates, so
+//               the state loads issue alongside the field loads.  Only
+//               dispersive edges touch it, so a sparse box costs capacity,
+//               not bandwidth.
 #pragma once
 #include <stdint.h>
 
@@ -45,9 +47,10 @@ struct DevGrid {
   int has_cpml;
   // electric pole-residue state
   int has_ade;
+  int ndisp;                    // dispersive media (slots)
   const int32_t *dslot;         // [nmedia+1] dispersive slot or -1
-  int nseg;                     // 64-wide x segments per row
-  const int32_t *segoff[3];
+  int abox[3][6];               // per E component: x0,y0,z0,bx,by,bz of the
+                                // box holding its dispersive edges
   float *eprev[3];
   void *jst[3];                 // FAST: float2[P][N], EXACT: double2[P][N]
   long long nstate[3];
@@ -65,6 +68,7 @@ struct TileMap {
   int kc;           // planes per z chunk
   int nblk;
   int remap;
+  int bdy;          // rows per block (block = 64 x bdy threads, <= 256)
 };
 
 struct NodalCell {   // one cell touched by a nodal source, in reference order
@@ -100,13 +104,12 @@ int launch_nodal(const DevGrid &g, const NodalCell *cells, int ncells,
 int launch_tick(int *dn, void *stream);
 int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
                   int *count, int cap, void *stream);
-int launch_count_disp(const DevGrid &g, int comp, int32_t *counts,
-                      void *stream);
+// bounding box {xmin,ymin,zmin,xmax,ymax,zmax} of the dispersive edges of
+// component comp; out must hold {INT_MAX x3, INT_MIN x3} on entry
+int launch_disp_bbox(const DevGrid &g, int comp, int32_t *out, void *stream);
 int launch_fill_i16(int16_t *a, long long n, int16_t v, void *stream);
 int launch_pec_walls(const DevGrid &g, int16_t *mi[6], int faces, void *stream);
 int launch_paint(const DevGrid &g, int16_t *mi[6], int shape, const float *par,
                  int16_t medium, const int32_t *prio, int nprio, void *stream);
-int exclusive_scan_i32(const int32_t *in, int32_t *out, int n, void *tmp,
-                       size_t *tmp_bytes, void *stream);
 
 }  // namespace sfd

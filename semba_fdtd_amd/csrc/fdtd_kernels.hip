@@ -20,8 +20,13 @@
 // registers, so the z neighbour costs no extra HBM read; the x and y
 // neighbours come from the same or an adjacent 128-byte line and hit L1/L2.
 // Loads of a wave are 64 consecutive x: 256 B of fp32, 128 B of int16.
-// The thread domain covers [0,nx]x[0,ny]x[0,nz]; each component updates the
-// part of it that is inside its own sweep.
+// The per-medium coefficients, the dispersive slot of each medium and the
+// pole coefficients are staged in LDS once per block, so the media-index
+// gather costs an LDS read instead of a dependent global load.  Each z step
+// first issues every independent load (fields, media indices, then the pole
+// state of dispersive edges) and only then computes, so the pole state's
+// HBM latency overlaps the curl arithmetic.  The thread domain covers
+// [0,nx]x[0,ny]x[0,nz]; each component updates the part inside its sweep.
 #include <hip/hip_runtime.h>
 #include "fdtd_internal.h"
 
@@ -34,6 +39,21 @@
 
 namespace sfd {
 namespace SFD_NS {
+
+// pole states prefetched into registers per edge: one Lorentz pair is two
+// complex(8) states in EXACT and one folded complex64 state in FAST
+#if SFD_EXACT
+typedef double2 jstate_t;  // complex(8) pole state, reference layout
+typedef double coef_t;
+constexpr int kPre = 2;
+#else
+typedef float2 jstate_t;   // complex64 state, conjugate pairs folded
+typedef float coef_t;
+#ifndef SFD_PREFETCH_POLES
+#define SFD_PREFETCH_POLES 1
+#endif
+constexpr int kPre = SFD_PREFETCH_POLES;
+#endif
 
 __device__ __forceinline__ void decode_tile(const TileMap &tm, int &bx, int &by,
                                             int &bz) {
@@ -75,94 +95,178 @@ __device__ __forceinline__ float psi_step(float *psi, long long s, float b,
   return v;
 }
 
-// Pole-residue update of one dispersive E edge (electricdispersive.F90:
-// 314-360): E -= sum_p Re(G3_p J_p); J_p = kappa_p J_p + beta_p (E-Eprev)/dt;
-// Eprev = E.
-template <int C>
-__device__ __forceinline__ float ade_edge(const DevGrid &g, float e, int slot,
-                                          long long s) {
-  const int np = g.anp[slot];
-  const long long N = g.nstate[C];
-  float *ep = g.eprev[C];
+// per-medium tables staged in LDS (TBL) or read from global memory
+struct MedTab {
+  const float2 *gg;     // {G1,G2} or {GM1,GM2}
+  const int *slot;      // dispersive slot per medium (-1: not dispersive)
+  const int *anp;       // stored poles per slot
+  const coef_t *acf;    // per slot per pole {kr,ki,br,bi,g3r,g3i}
+};
+
+__device__ __forceinline__ const coef_t *ade_coefs(const DevGrid &g) {
 #if SFD_EXACT
-  const double *cf = g.aded + (size_t)slot * kPMax * 6;
-  double2 *J = reinterpret_cast<double2 *>(g.jst[C]);
-  double2 jv[kPMax];
-#pragma unroll
-  for (int q = 0; q < kPMax; ++q) {
-    if (q < np) {
-      jv[q] = J[q * N + s];
-      const double re = cf[q * 6 + 4] * jv[q].x - cf[q * 6 + 5] * jv[q].y;
-      e = (float)((double)e - re);
+  return g.aded;
+#else
+  return g.adef;
+#endif
+}
+
+// LDS layout: gg[nm+1] | slot[nm+1] | anp[ndisp] | (8-byte aligned) acf
+template <bool TBL, bool ADE>
+__device__ __forceinline__ MedTab stage_tables(const DevGrid &g, const float *a,
+                                               const float *b, float2 *lds) {
+  MedTab t;
+  const int nm = g.nmedia;
+  if (TBL) {
+    int *ls = reinterpret_cast<int *>(lds + (nm + 1));
+    int *lnp = ls + (nm + 1);
+    const size_t off = ((size_t)(nm + 1) * 12 + (size_t)g.ndisp * 4 + 7) & ~(size_t)7;
+    coef_t *lcf = reinterpret_cast<coef_t *>(reinterpret_cast<char *>(lds) + off);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int nt = blockDim.x * blockDim.y;
+    for (int m = tid; m <= nm; m += nt) {
+      lds[m] = make_float2(a[m], b[m]);
+      if (ADE) ls[m] = g.dslot[m];
     }
+    if (ADE) {
+      const coef_t *cf = ade_coefs(g);
+      for (int d = tid; d < g.ndisp; d += nt) lnp[d] = g.anp[d];
+      for (int q = tid; q < g.ndisp * kPMax * 6; q += nt) lcf[q] = cf[q];
+    }
+    __syncthreads();
+    t.gg = lds;
+    t.slot = ls;
+    t.anp = lnp;
+    t.acf = lcf;
+  } else {
+    t.gg = nullptr;
+    t.slot = g.dslot;
+    t.anp = g.anp;
+    t.acf = ade_coefs(g);
   }
-  const float diff = e - ep[s];
+  return t;
+}
+
+template <bool TBL>
+__device__ __forceinline__ float2 coef(const MedTab &t, const float *a, const float *b, int m) {
+  if (TBL) return t.gg[m];
+  return make_float2(a[m], b[m]);
+}
+
+// ---- electric pole-residue ADE (electricdispersive.F90:304-362) ----------
+// The pole state of every dispersive edge of component C lives in a dense
+// x-fastest box (abox); s = the edge's slot in it.  AdePre holds the loaded
+// state of one edge.
+struct AdePre {
+  long long s;
+  int slot;
+  int np;
+  float ep;
+  jstate_t j[kPre];
+};
+
+// E -= sum_p Re(G3_p J_p); J_p = kappa_p J_p + beta_p (E - Eprev)/dt; Eprev = E
+template <int C>
+__device__ __forceinline__ float ade_apply(const DevGrid &g, const MedTab &t, const AdePre &a,
+                                           float e) {
+  const coef_t *cf = t.acf + (size_t)a.slot * kPMax * 6;
+  const long long N = g.nstate[C];
+  jstate_t *J = reinterpret_cast<jstate_t *>(g.jst[C]);
+#if SFD_EXACT
+  // one pole at a time, rounding E after each (electricdispersive.F90:319-321)
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
-    if (q < np) {
-      const double kr = cf[q * 6 + 0], ki = cf[q * 6 + 1];
-      const double ar = kr * jv[q].x - ki * jv[q].y;
-      const double ai = kr * jv[q].y + ki * jv[q].x;
-      const double br = cf[q * 6 + 2] * (double)diff / g.dt;
-      const double bi = cf[q * 6 + 3] * (double)diff / g.dt;
-      J[q * N + s] = make_double2(ar + br, ai + bi);
-    }
+    if (q >= a.np) break;
+    const double2 jv = q < kPre ? a.j[q] : J[q * N + a.s];
+    const double re = cf[q * 6 + 4] * jv.x - cf[q * 6 + 5] * jv.y;
+    e = (float)((double)e - re);
+  }
+  const float diff = e - a.ep;
+#pragma unroll
+  for (int q = 0; q < kPMax; ++q) {
+    if (q >= a.np) break;
+    const double2 jv = q < kPre ? a.j[q] : J[q * N + a.s];
+    const double kr = cf[q * 6 + 0], ki = cf[q * 6 + 1];
+    const double ar = kr * jv.x - ki * jv.y;
+    const double ai = kr * jv.y + ki * jv.x;
+    const double br = cf[q * 6 + 2] * (double)diff / g.dt;
+    const double bi = cf[q * 6 + 3] * (double)diff / g.dt;
+    J[q * N + a.s] = make_double2(ar + br, ai + bi);
   }
 #else
-  const float *cf = g.adef + (size_t)slot * kPMax * 6;
-  float2 *J = reinterpret_cast<float2 *>(g.jst[C]);
-  float2 jv[kPMax];
+  // folded fp32 form: beta is pre-divided by dt, G3 of a folded pair doubled
   float sum = 0.0f;
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
-    if (q < np) {
-      jv[q] = J[q * N + s];
-      sum += cf[q * 6 + 4] * jv[q].x - cf[q * 6 + 5] * jv[q].y;
-    }
+    if (q >= a.np) break;
+    const float2 jv = q < kPre ? a.j[q] : J[q * N + a.s];
+    sum += cf[q * 6 + 4] * jv.x - cf[q * 6 + 5] * jv.y;
   }
   e = e - sum;
-  const float diff = e - ep[s];
+  const float diff = e - a.ep;
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
-    if (q < np) {
-      const float kr = cf[q * 6 + 0], ki = cf[q * 6 + 1];
-      J[q * N + s] = make_float2(kr * jv[q].x - ki * jv[q].y + cf[q * 6 + 2] * diff,
-                                 kr * jv[q].y + ki * jv[q].x + cf[q * 6 + 3] * diff);
-    }
+    if (q >= a.np) break;
+    const float2 jv = q < kPre ? a.j[q] : J[q * N + a.s];
+    const float kr = cf[q * 6 + 0], ki = cf[q * 6 + 1];
+    J[q * N + a.s] = make_float2(kr * jv.x - ki * jv.y + cf[q * 6 + 2] * diff,
+                                 kr * jv.y + ki * jv.x + cf[q * 6 + 3] * diff);
   }
 #endif
-  ep[s] = e;
+  g.eprev[C][a.s] = e;
   return e;
 }
 
-// wave-cooperative ADE for component C: lanes whose edge is dispersive find
-// their compact slot as segment base + rank among the set ballot bits.
-template <int C>
-__device__ __forceinline__ float ade_wave(const DevGrid &g, bool valid, int m,
-                                         float e, long long row, int seg,
-                                         int lane, bool &touched) {
-  const int slot = valid ? g.dslot[m] : -1;
-  const unsigned long long mask = __ballot(slot >= 0);
-  if (mask) {
-    if (slot >= 0) {
-      const long long base = g.segoff[C][row * g.nseg + seg];
-      const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-      e = ade_edge<C>(g, e, slot, base + rank);
-      touched = true;
-    }
+// The dispersive-edge pass: AdvanceEDispersiveE over the pole-state box of
+// each E component (blockIdx.y), after the Yee+CPML E pass.  Edges of the box
+// are visited x-fastest, so the media index, E, Eprev and state loads of a
+// wave are each one contiguous run; non-dispersive edges cost only their
+// int16 media index.
+template <bool TBL>
+__global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
+  extern __shared__ float2 lds_tab[];
+  const MedTab tab = stage_tables<TBL, true>(g, g.g1, g.g2, lds_tab);
+  const int c = blockIdx.y;
+  const int *b = g.abox[c];
+  const long long bx = b[3], bxy = (long long)b[3] * b[4];
+  const long long n = bxy * b[5];
+  for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n;
+       s += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(s % bx), y = (int)((s / bx) % b[4]), z = (int)(s / bxy);
+    const long long p = (long long)(b[2] + z + 1) * g.pxy + (long long)(b[1] + y + 1) * g.px +
+                        kXOff + b[0] + x;
+    const int m = g.mi[c][p];
+    const int slot = tab.slot[m];
+    if (slot < 0) continue;
+    float *F = g.f[c];
+    AdePre a;
+    a.s = s;
+    a.slot = slot;
+    a.np = tab.anp[slot];
+    a.ep = g.eprev[c][s];
+    const jstate_t *J = reinterpret_cast<const jstate_t *>(g.jst[c]);
+#pragma unroll
+    for (int q = 0; q < kPre; ++q)
+      if (q < a.np) a.j[q] = J[q * g.nstate[c] + s];
+    const float e = F[p];
+    float r;
+    if (c == 0) r = ade_apply<0>(g, tab, a, e);
+    else if (c == 1) r = ade_apply<1>(g, tab, a, e);
+    else r = ade_apply<2>(g, tab, a, e);
+    F[p] = r;
   }
-  return e;
 }
 
-template <bool YEE, bool CPML, bool ADE>
+template <bool YEE, bool CPML, bool TBL>
 __global__ void __launch_bounds__(256) k_epass(DevGrid g, TileMap tm) {
+  extern __shared__ float2 lds_tab[];
+  const MedTab tab = stage_tables<TBL, false>(g, g.g1, g.g2, lds_tab);
   int bx, by, bz;
   decode_tile(tm, bx, by, bz);
-  const int lane = threadIdx.x;
-  const int i = bx * kWave + lane;
+  const int i = bx * kWave + threadIdx.x;
   const int j = by * blockDim.y + threadIdx.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const bool act = (i <= nx) && (j <= ny);
+  if (i > nx || j > ny) return;
   const int k0 = bz * tm.kc;
   const int k1 = min(k0 + tm.kc, nz + 1);
   const long long px = g.px, pxy = g.pxy;
@@ -174,121 +278,102 @@ __global__ void __launch_bounds__(256) k_epass(DevGrid g, TileMap tm) {
   const float *__restrict__ Hy = g.f[4];
   const float *__restrict__ Hz = g.f[5];
   constexpr bool NEEDH = YEE || CPML;
-  float hx_km = 0.0f, hy_km = 0.0f, idyh = 0.0f, idxh = 0.0f;
-  if (act) {
-    if (NEEDH) {
-      hx_km = Hx[p - pxy];
-      hy_km = Hy[p - pxy];
-    }
-    idyh = g.id[4][j + 1];
-    idxh = g.id[3][i + 1];
+  float hx_km = 0.0f, hy_km = 0.0f;
+  if (NEEDH) {
+    hx_km = Hx[p - pxy];
+    hy_km = Hy[p - pxy];
   }
+  const float idyh = g.id[4][j + 1];
+  const float idxh = g.id[3][i + 1];
+  const bool cx = i < nx, cy = j < ny;   // Ex needs i<nx, Ey j<ny, Ez k<nz
   // CPML slab membership along x and y is fixed for the thread
   int qxn = 0, qyn = 0;
   bool sxn = false, syn = false;
-  if (CPML && act) {
+  if (CPML) {
     sxn = slab_node(i, nx, g.L[0][0], g.L[0][1], qxn);
     syn = slab_node(j, ny, g.L[1][0], g.L[1][1], qyn);
   }
   for (int k = k0; k < k1; ++k, p += pxy) {
-    const bool vx = act && i < nx, vy = act && j < ny, vz = act && k < nz;
-    float exn = 0.0f, eyn = 0.0f, ezn = 0.0f;
+    const bool cz = k < nz;
+    // ---- issue the independent loads of this plane
+    const float idzh = g.id[5][k + 1];
+    float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f, hz_jm = 0.0f, hx_jm = 0.0f, hz_im = 0.0f,
+          hy_im = 0.0f;
+    if (NEEDH) {
+      hx0 = Hx[p]; hy0 = Hy[p]; hz0 = Hz[p];
+      hz_jm = Hz[p - px]; hx_jm = Hx[p - px];
+      hz_im = Hz[p - 1]; hy_im = Hy[p - 1];
+    }
     int mx = 0, my = 0, mz = 0;
-    float hx0 = 0.0f, hy0 = 0.0f;
-    if (act) {
-      const float idzh = g.id[5][k + 1];
-      float hz0 = 0.0f, hz_jm = 0.0f, hx_jm = 0.0f, hz_im = 0.0f, hy_im = 0.0f;
-      if (NEEDH) {
-        hx0 = Hx[p]; hy0 = Hy[p]; hz0 = Hz[p];
-        hz_jm = Hz[p - px]; hx_jm = Hx[p - px];
-        hz_im = Hz[p - 1]; hy_im = Hy[p - 1];
-      }
-      int qzn = 0;
-      const bool szn = CPML && slab_node(k, nz, g.L[2][0], g.L[2][1], qzn);
-      if (vx) {
-        mx = g.mi[0][p];
-        float e = Ex[p];
-        if (YEE)
-          e = g.g1[mx] * e + g.g2[mx] * ((hz0 - hz_jm) * idyh - (hy0 - hy_km) * idzh);
-        if (CPML) {
-          const float g2 = g.g2[mx];
-          if (syn) {  // psi_Exy, left/right (borderscpml.F90:621-623)
-            const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * nx + i;
-            e = e + g2 * psi_step(g.psi[PSI_EXY], s, g.pbe[1][j + 1], hz0 - hz_jm, g.pce[1][j + 1]);
-          }
-          if (szn) {  // psi_Exz, down/up (:712)
-            const long long s = ((long long)qzn * (ny + 1) + j) * nx + i;
-            e = e - g2 * psi_step(g.psi[PSI_EXZ], s, g.pbe[2][k + 1], hy0 - hy_km, g.pce[2][k + 1]);
-          }
+    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    if (cx) { mx = g.mi[0][p]; ex = Ex[p]; }
+    if (cy) { my = g.mi[1][p]; ey = Ey[p]; }
+    if (cz) { mz = g.mi[2][p]; ez = Ez[p]; }
+    int qzn = 0;
+    const bool szn = CPML && slab_node(k, nz, g.L[2][0], g.L[2][1], qzn);
+    // ---- compute
+    if (cx) {
+      float e = ex;
+      const float2 c = coef<TBL>(tab, g.g1, g.g2, mx);
+      if (YEE) e = c.x * e + c.y * ((hz0 - hz_jm) * idyh - (hy0 - hy_km) * idzh);
+      if (CPML) {
+        if (syn) {  // psi_Exy, left/right (borderscpml.F90:609-623)
+          const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * nx + i;
+          e = e + c.y * psi_step(g.psi[PSI_EXY], s, g.pbe[1][j + 1], hz0 - hz_jm, g.pce[1][j + 1]);
         }
-        exn = e;
-      }
-      if (vy) {
-        my = g.mi[1][p];
-        float e = Ey[p];
-        if (YEE)
-          e = g.g1[my] * e + g.g2[my] * ((hx0 - hx_km) * idzh - (hz0 - hz_im) * idxh);
-        if (CPML) {
-          const float g2 = g.g2[my];
-          if (szn) {  // psi_Eyz, down/up (:642... region down)
-            const long long s = ((long long)qzn * ny + j) * (nx + 1) + i;
-            e = e + g2 * psi_step(g.psi[PSI_EYZ], s, g.pbe[2][k + 1], hx0 - hx_km, g.pce[2][k + 1]);
-          }
-          if (sxn) {  // psi_Eyx, back/front
-            const long long s = ((long long)k * ny + j) * (g.L[0][0] + g.L[0][1]) + qxn;
-            e = e - g2 * psi_step(g.psi[PSI_EYX], s, g.pbe[0][i + 1], hz0 - hz_im, g.pce[0][i + 1]);
-          }
+        if (szn) {  // psi_Exz, down/up (:699-712)
+          const long long s = ((long long)qzn * (ny + 1) + j) * nx + i;
+          e = e - c.y * psi_step(g.psi[PSI_EXZ], s, g.pbe[2][k + 1], hy0 - hy_km, g.pce[2][k + 1]);
         }
-        eyn = e;
       }
-      if (vz) {
-        mz = g.mi[2][p];
-        float e = Ez[p];
-        if (YEE)
-          e = g.g1[mz] * e + g.g2[mz] * ((hy0 - hy_im) * idxh - (hx0 - hx_jm) * idyh);
-        if (CPML) {
-          const float g2 = g.g2[mz];
-          if (syn) {  // psi_Ezy, left/right
-            const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * (nx + 1) + i;
-            e = e - g2 * psi_step(g.psi[PSI_EZY], s, g.pbe[1][j + 1], hx0 - hx_jm, g.pce[1][j + 1]);
-          }
-          if (sxn) {  // psi_Ezx, back/front
-            const long long s = ((long long)k * (ny + 1) + j) * (g.L[0][0] + g.L[0][1]) + qxn;
-            e = e + g2 * psi_step(g.psi[PSI_EZX], s, g.pbe[0][i + 1], hy0 - hy_im, g.pce[0][i + 1]);
-          }
-        }
-        ezn = e;
-      }
+      Ex[p] = e;
     }
-    bool tx = false, ty = false, tz = false;
-    if (ADE) {
-      if (!YEE && !CPML) {
-        if (vx) { mx = g.mi[0][p]; exn = Ex[p]; }
-        if (vy) { my = g.mi[1][p]; eyn = Ey[p]; }
-        if (vz) { mz = g.mi[2][p]; ezn = Ez[p]; }
+    if (cy) {
+      float e = ey;
+      const float2 c = coef<TBL>(tab, g.g1, g.g2, my);
+      if (YEE) e = c.x * e + c.y * ((hx0 - hx_km) * idzh - (hz0 - hz_im) * idxh);
+      if (CPML) {
+        if (szn) {  // psi_Eyz, down/up (:680-693)
+          const long long s = ((long long)qzn * ny + j) * (nx + 1) + i;
+          e = e + c.y * psi_step(g.psi[PSI_EYZ], s, g.pbe[2][k + 1], hx0 - hx_km, g.pce[2][k + 1]);
+        }
+        if (sxn) {  // psi_Eyx, back/front (:790-802)
+          const long long s = ((long long)k * ny + j) * (g.L[0][0] + g.L[0][1]) + qxn;
+          e = e - c.y * psi_step(g.psi[PSI_EYX], s, g.pbe[0][i + 1], hz0 - hz_im, g.pce[0][i + 1]);
+        }
       }
-      exn = ade_wave<0>(g, vx, mx, exn, (long long)k * (ny + 1) + j, bx, lane, tx);
-      eyn = ade_wave<1>(g, vy, my, eyn, (long long)k * ny + j, bx, lane, ty);
-      ezn = ade_wave<2>(g, vz, mz, ezn, (long long)k * (ny + 1) + j, bx, lane, tz);
+      Ey[p] = e;
     }
-    constexpr bool ALL = YEE || CPML;
-    if (vx && (ALL || tx)) Ex[p] = exn;
-    if (vy && (ALL || ty)) Ey[p] = eyn;
-    if (vz && (ALL || tz)) Ez[p] = ezn;
+    if (cz) {
+      float e = ez;
+      const float2 c = coef<TBL>(tab, g.g1, g.g2, mz);
+      if (YEE) e = c.x * e + c.y * ((hy0 - hy_im) * idxh - (hx0 - hx_jm) * idyh);
+      if (CPML) {
+        if (syn) {  // psi_Ezy, left/right (:630-642)
+          const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * (nx + 1) + i;
+          e = e - c.y * psi_step(g.psi[PSI_EZY], s, g.pbe[1][j + 1], hx0 - hx_jm, g.pce[1][j + 1]);
+        }
+        if (sxn) {  // psi_Ezx, back/front (:771-783)
+          const long long s = ((long long)k * (ny + 1) + j) * (g.L[0][0] + g.L[0][1]) + qxn;
+          e = e + c.y * psi_step(g.psi[PSI_EZX], s, g.pbe[0][i + 1], hy0 - hy_im, g.pce[0][i + 1]);
+        }
+      }
+      Ez[p] = e;
+    }
     if (NEEDH) { hx_km = hx0; hy_km = hy0; }
   }
 }
 
-template <bool YEE, bool CPML>
+template <bool YEE, bool CPML, bool TBL>
 __global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm) {
+  extern __shared__ float2 lds_tab[];
+  const MedTab tab = stage_tables<TBL, false>(g, g.gm1, g.gm2, lds_tab);
   int bx, by, bz;
   decode_tile(tm, bx, by, bz);
-  const int lane = threadIdx.x;
-  const int i = bx * kWave + lane;
+  const int i = bx * kWave + threadIdx.x;
   const int j = by * blockDim.y + threadIdx.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const bool act = (i <= nx) && (j <= ny);
-  if (!act) return;  // no cross-lane operations in this pass
+  if (i > nx || j > ny) return;
   const int k0 = bz * tm.kc;
   const int k1 = min(k0 + tm.kc, nz + 1);
   const long long px = g.px, pxy = g.pxy;
@@ -317,55 +402,49 @@ __global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm) {
     int qzc = 0;
     const bool szc = CPML && slab_cell(k, nz, g.L[2][0], g.L[2][1], qzc);
     if (j < ny && k < nz) {  // Hx sweep: x[0,nx] y[0,ny-1] z[0,nz-1]
-      const int m = g.mi[3][p];
+      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[3][p]);
       float h = Hx[p];
-      if (YEE)
-        h = g.gm1[m] * h + g.gm2[m] * ((ey_kp - ey_c) * idze - (ez_jp - ez_c) * idye);
+      if (YEE) h = c.x * h + c.y * ((ey_kp - ey_c) * idze - (ez_jp - ez_c) * idye);
       if (CPML) {
-        const float gm2 = g.gm2[m];
-        if (syc) {  // psi_Hxy (borderscpml.F90:921)
+        if (syc) {  // psi_Hxy (borderscpml.F90:911-921)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyc) * (nx + 1) + i;
-          h = h - gm2 * psi_step(g.psi[PSI_HXY], s, g.pbm[1][j + 1], ez_jp - ez_c, g.pcm[1][j + 1]);
+          h = h - c.y * psi_step(g.psi[PSI_HXY], s, g.pbm[1][j + 1], ez_jp - ez_c, g.pcm[1][j + 1]);
         }
-        if (szc) {  // psi_Hxz (:1031)
+        if (szc) {  // psi_Hxz (:1021-1031)
           const long long s = ((long long)qzc * ny + j) * (nx + 1) + i;
-          h = h + gm2 * psi_step(g.psi[PSI_HXZ], s, g.pbm[2][k + 1], ey_kp - ey_c, g.pcm[2][k + 1]);
+          h = h + c.y * psi_step(g.psi[PSI_HXZ], s, g.pbm[2][k + 1], ey_kp - ey_c, g.pcm[2][k + 1]);
         }
       }
       Hx[p] = h;
     }
     if (i < nx && k < nz) {  // Hy sweep: x[0,nx-1] y[0,ny] z[0,nz-1]
-      const int m = g.mi[4][p];
+      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[4][p]);
       float h = Hy[p];
-      if (YEE)
-        h = g.gm1[m] * h + g.gm2[m] * ((ez_ip - ez_c) * idxe - (ex_kp - ex_c) * idze);
+      if (YEE) h = c.x * h + c.y * ((ez_ip - ez_c) * idxe - (ex_kp - ex_c) * idze);
       if (CPML) {
-        const float gm2 = g.gm2[m];
-        if (szc) {  // psi_Hyz (:1011)
+        if (szc) {  // psi_Hyz (:1001-1011)
           const long long s = ((long long)qzc * (ny + 1) + j) * nx + i;
-          h = h - gm2 * psi_step(g.psi[PSI_HYZ], s, g.pbm[2][k + 1], ex_kp - ex_c, g.pcm[2][k + 1]);
+          h = h - c.y * psi_step(g.psi[PSI_HYZ], s, g.pbm[2][k + 1], ex_kp - ex_c, g.pcm[2][k + 1]);
         }
-        if (sxc) {  // psi_Hyx (:1121)
+        if (sxc) {  // psi_Hyx (:1111-1121)
           const long long s = ((long long)k * (ny + 1) + j) * (g.L[0][0] + g.L[0][1]) + qxc;
-          h = h + gm2 * psi_step(g.psi[PSI_HYX], s, g.pbm[0][i + 1], ez_ip - ez_c, g.pcm[0][i + 1]);
+          h = h + c.y * psi_step(g.psi[PSI_HYX], s, g.pbm[0][i + 1], ez_ip - ez_c, g.pcm[0][i + 1]);
         }
       }
       Hy[p] = h;
     }
     if (i < nx && j < ny) {  // Hz sweep: x[0,nx-1] y[0,ny-1] z[0,nz]
-      const int m = g.mi[5][p];
+      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[5][p]);
       float h = Hz[p];
-      if (YEE)
-        h = g.gm1[m] * h + g.gm2[m] * ((ex_jp - ex_c) * idye - (ey_ip - ey_c) * idxe);
+      if (YEE) h = c.x * h + c.y * ((ex_jp - ex_c) * idye - (ey_ip - ey_c) * idxe);
       if (CPML) {
-        const float gm2 = g.gm2[m];
-        if (syc) {  // psi_Hzy (:941)
+        if (syc) {  // psi_Hzy (:931-941)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyc) * nx + i;
-          h = h + gm2 * psi_step(g.psi[PSI_HZY], s, g.pbm[1][j + 1], ex_jp - ex_c, g.pcm[1][j + 1]);
+          h = h + c.y * psi_step(g.psi[PSI_HZY], s, g.pbm[1][j + 1], ex_jp - ex_c, g.pcm[1][j + 1]);
         }
-        if (sxc) {  // psi_Hzx (:1101)
+        if (sxc) {  // psi_Hzx (:1091-1101)
           const long long s = ((long long)k * ny + j) * (g.L[0][0] + g.L[0][1]) + qxc;
-          h = h - gm2 * psi_step(g.psi[PSI_HZX], s, g.pbm[0][i + 1], ey_ip - ey_c, g.pcm[0][i + 1]);
+          h = h - c.y * psi_step(g.psi[PSI_HZX], s, g.pbm[0][i + 1], ey_ip - ey_c, g.pcm[0][i + 1]);
         }
       }
       Hz[p] = h;
@@ -375,35 +454,72 @@ __global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm) {
   }
 }
 
-static inline int grid_of(const TileMap &tm) { return tm.nblk; }
+// media tables fit in LDS up to this many entries
+constexpr int kLdsMedia = 4096;
+
+static size_t e_lds_bytes(const DevGrid &g, int ade) {
+  size_t b = (size_t)(g.nmedia + 1) * 12;
+  if (ade) b = ((b + (size_t)g.ndisp * 4 + 7) & ~(size_t)7) + (size_t)g.ndisp * kPMax * 6 * sizeof(coef_t);
+  return b;
+}
 
 int launch_e(const DevGrid &g, const TileMap &tm, int yee, int cpml, int ade,
              void *stream) {
-  const dim3 blk(kWave, (unsigned)(256 / kWave));
+  const dim3 blk(kWave, (unsigned)tm.bdy);
   hipStream_t s = (hipStream_t)stream;
-  const int nb = grid_of(tm);
-#define SFD_E(Y, C, A) hipLaunchKernelGGL((k_epass<Y, C, A>), dim3(nb), blk, 0, s, g, tm)
-  if (yee && cpml && ade) SFD_E(true, true, true);
-  else if (yee && cpml) SFD_E(true, true, false);
-  else if (yee && ade) SFD_E(true, false, true);
-  else if (yee) SFD_E(true, false, false);
-  else if (cpml && ade) SFD_E(false, true, true);
-  else if (cpml) SFD_E(false, true, false);
-  else if (ade) SFD_E(false, false, true);
-  else return 0;
+  const int nb = tm.nblk;
+  const size_t lds = e_lds_bytes(g, 0);
+  const bool tbl = g.nmedia + 1 <= kLdsMedia;
+#define SFD_E(Y, C)                                                            \
+  do {                                                                         \
+    if (tbl)                                                                   \
+      hipLaunchKernelGGL((k_epass<Y, C, true>), dim3(nb), blk, lds, s, g, tm);  \
+    else                                                                       \
+      hipLaunchKernelGGL((k_epass<Y, C, false>), dim3(nb), blk, 0, s, g, tm);   \
+  } while (0)
+  if (yee && cpml) SFD_E(true, true);
+  else if (yee) SFD_E(true, false);
+  else if (cpml) SFD_E(false, true);
 #undef SFD_E
+  if (ade) {
+    // the pole stage follows the Yee+CPML E pass (electricdispersive.F90 runs
+    // after AdvanceelectricCPML, timestepping.F90:1380-1417)
+    long long nmax = 0;
+    for (int c = 0; c < 3; ++c)
+      nmax = nmax > g.nstate[c] ? nmax : g.nstate[c];
+    if (nmax > 0) {
+      const size_t la = e_lds_bytes(g, 1);
+      const bool ta = g.nmedia + 1 <= kLdsMedia && la <= 48 * 1024;
+      long long want = (nmax + 255) / 256;
+      const unsigned gx = (unsigned)(want < 8192 ? want : 8192);
+      if (ta)
+        hipLaunchKernelGGL(k_ade_box<true>, dim3(gx, 3), dim3(256), la, s, g);
+      else
+        hipLaunchKernelGGL(k_ade_box<false>, dim3(gx, 3), dim3(256), 0, s, g);
+    }
+  }
   return (int)hipGetLastError();
 }
 
 int launch_h(const DevGrid &g, const TileMap &tm, int yee, int cpml,
              void *stream) {
-  const dim3 blk(kWave, (unsigned)(256 / kWave));
+  const dim3 blk(kWave, (unsigned)tm.bdy);
   hipStream_t s = (hipStream_t)stream;
-  const int nb = grid_of(tm);
-  if (yee && cpml) hipLaunchKernelGGL((k_hpass<true, true>), dim3(nb), blk, 0, s, g, tm);
-  else if (yee) hipLaunchKernelGGL((k_hpass<true, false>), dim3(nb), blk, 0, s, g, tm);
-  else if (cpml) hipLaunchKernelGGL((k_hpass<false, true>), dim3(nb), blk, 0, s, g, tm);
+  const int nb = tm.nblk;
+  const bool tbl = g.nmedia + 1 <= kLdsMedia;
+  const size_t lds = tbl ? (size_t)(g.nmedia + 1) * sizeof(float2) : 0;
+#define SFD_H(Y, C)                                                            \
+  do {                                                                         \
+    if (tbl)                                                                   \
+      hipLaunchKernelGGL((k_hpass<Y, C, true>), dim3(nb), blk, lds, s, g, tm);  \
+    else                                                                       \
+      hipLaunchKernelGGL((k_hpass<Y, C, false>), dim3(nb), blk, 0, s, g, tm);   \
+  } while (0)
+  if (yee && cpml) SFD_H(true, true);
+  else if (yee) SFD_H(true, false);
+  else if (cpml) SFD_H(false, true);
   else return 0;
+#undef SFD_H
   return (int)hipGetLastError();
 }
 
