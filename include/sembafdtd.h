@@ -12,11 +12,16 @@
  *  - Coordinates are total cells per axis (CPML slabs included); 0 is the
  *    outer face of the low slab.  nx, ny, nz are the reference's swept cell
  *    counts (fullsize, semba_fdtd.F90:1285-1309).
- *  - Host arrays of fields / media indices use the reference's Alloc box
- *    [-1, n+1] on every axis, x fastest:
- *        idx(i,j,k) = ((k+1)*(ny+3) + (j+1))*(nx+3) + (i+1)
- *    (all six components share the Alloc origin, timestepping.F90:282-290).
- *  - 1-D tables (inverse spacings, CPML profiles) have length n+3, index i+1.
+ *  - Host arrays of a field or media-index component are the reference's
+ *    arrays as allocated (timestepping.F90:380-386, semba_fdtd.F90:1285-1300):
+ *    the component's Alloc box [-1, e+1] on each axis, e being the last
+ *    swept index (Ex: nx-1,ny,nz; Ey: nx,ny-1,nz; Ez: nx,ny,nz-1;
+ *    Hx: nx,ny-1,nz-1; Hy: nx-1,ny,nz-1; Hz: nx-1,ny-1,nz), x fastest, so
+ *    idx(i,j,k) = ((k+1)*EY + (j+1))*EX + (i+1) with EX = e_x+3, EY = e_y+3.
+ *    A Fortran caller passes Ex(sgg%Alloc(iEx)%XI:...) etc. unchanged.
+ *  - 1-D tables are indexed i+1 from -1: Idxe/Idye/Idze and the CPML
+ *    profiles span Alloc of Hx/Hy/Hz (n+3 entries), Idxh/Idyh/Idzh span
+ *    Alloc of Ex/Ey/Ez (n+2 entries).
  *  - Every function returns 0 on success or a negative FDTD_ERR_* code;
  *    fdtd_last_error() returns the message.  A handle is bound to one HIP
  *    stream and is not re-entrant; distinct handles may run concurrently.
@@ -88,8 +93,8 @@ int fdtd_abi_version(void);
 int fdtd_set_media(fdtd_handle *h, int32_t nmedia, const float *g1,
                    const float *g2, const float *gm1, const float *gm2,
                    const int32_t *kind);
-/* Media index matrix of one component (sggMiEx..sggMiHz, host Alloc layout;
- * semba_fdtd.F90:93, built by read_geomData). */
+/* Media index matrix of one component (sggMiEx..sggMiHz in their Alloc
+ * shape; semba_fdtd.F90:93, built by read_geomData). */
 int fdtd_set_media_index(fdtd_handle *h, int32_t comp, const int16_t *mi);
 /* Inverse spacings Idxe,Idye,Idze,Idxh,Idyh,Idzh (timestepping.F90:314-370,
  * already stretched by the CPML, borderscpml.F90:1556-1724). */

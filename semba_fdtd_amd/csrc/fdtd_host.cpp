@@ -135,6 +135,35 @@ size_t field_elems(const fdtd_handle *h) { return (size_t)h->g.pxy * (size_t)(h-
 
 int axis_n(const fdtd_handle *h, int a) { return a == 0 ? h->g.nx : (a == 1 ? h->g.ny : h->g.nz); }
 
+// extents of component comp's Alloc box [-1, last swept + 1] per axis, the
+// shape launch_simulation allocates (timestepping.F90:380-386)
+void alloc_extent(const fdtd_handle *h, int comp, long long ext[3]) {
+  for (int a = 0; a < 3; ++a) {
+    const bool node = comp < 3 ? (comp != a) : (comp - 3 == a);
+    ext[a] = (long long)axis_n(h, a) + (node ? 3 : 2);
+  }
+}
+
+// host Alloc-box array <-> device grid array (element size es), one 2-D copy
+// per z plane
+int copy_box(fdtd_handle *h, int comp, void *dev, const void *host_src, void *host_dst,
+             size_t es) {
+  long long ext[3];
+  alloc_extent(h, comp, ext);
+  const size_t hrow = (size_t)ext[0] * es, hplane = hrow * (size_t)ext[1];
+  for (long long k = 0; k < ext[2]; ++k) {
+    char *d = (char *)dev + ((size_t)k * h->g.pxy + (kXOff - 1)) * es;
+    if (host_src)
+      HIPCHK(h, hipMemcpy2DAsync(d, (size_t)h->g.px * es, (const char *)host_src + k * hplane, hrow,
+                                 hrow, (size_t)ext[1], hipMemcpyHostToDevice, h->stream));
+    else
+      HIPCHK(h, hipMemcpy2DAsync((char *)host_dst + k * hplane, hrow, d, (size_t)h->g.px * es,
+                                 hrow, (size_t)ext[1], hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FDTD_OK;
+}
+
 int check_ready(fdtd_handle *h) {
   if (h->nmedia < 0) return fail(h, FDTD_ERR_STATE, "fdtd_set_media not called");
   if (h->media_max > h->nmedia)
@@ -506,9 +535,10 @@ int fdtd_set_media(fdtd_handle *h, int32_t nmedia, const float *g1, const float 
 int fdtd_set_media_index(fdtd_handle *h, int32_t comp, const int16_t *mi) {
   if (!h || comp < 0 || comp > 5 || !mi) return fail(h, FDTD_ERR_ARG, "bad media index arguments");
   hipSetDevice(h->device);
-  const DevGrid &g = h->g;
   int16_t mx = 0;
-  const long long tot = (long long)(g.nx + 3) * (g.ny + 3) * (g.nz + 3);
+  long long ext[3];
+  alloc_extent(h, comp, ext);
+  const long long tot = ext[0] * ext[1] * ext[2];
   for (long long t = 0; t < tot; ++t) {
     if (mi[t] < 0) return fail(h, FDTD_ERR_ARG, "negative media index at %lld", t);
     mx = std::max(mx, mi[t]);
@@ -516,10 +546,8 @@ int fdtd_set_media_index(fdtd_handle *h, int32_t comp, const int16_t *mi) {
   if (h->nmedia >= 0 && mx > h->nmedia)
     return fail(h, FDTD_ERR_ARG, "media index %d outside 0..%d", mx, h->nmedia);
   h->media_max = std::max(h->media_max, (int)mx);
-  HIPCHK(h, hipMemcpy2DAsync(h->mi[comp] + kXOff - 1, (size_t)g.px * 2, mi, (size_t)(g.nx + 3) * 2,
-                             (size_t)(g.nx + 3) * 2, (size_t)(g.ny + 3) * (g.nz + 3),
-                             hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  int rc = copy_box(h, comp, h->mi[comp], mi, nullptr, 2);
+  if (rc) return rc;
   h->ade_ready = false;
   h->nodal_ready = false;
   invalidate_graph(h);
@@ -534,10 +562,15 @@ int fdtd_set_spacing(fdtd_handle *h, const float *idxe, const float *idye, const
   const float *src[6] = {idxe, idye, idze, idxh, idyh, idzh};
   int rc;
   for (int c = 0; c < 6; ++c) {
+    // primal tables span Alloc(iHx..iHz) = n+3 entries, dual ones Alloc(iEx..
+    // iEz) = n+2 (timestepping.F90:314-327); the device copy keeps n+3
     const int n = axis_n(h, c % 3) + 3;
-    if ((rc = upload(h, &h->did[c], src[c], (size_t)n))) return rc;
+    const int len = c < 3 ? n : n - 1;
+    std::vector<float> t(src[c], src[c] + len);
+    t.resize(n, 0.0f);
+    if ((rc = upload(h, &h->did[c], t.data(), (size_t)n))) return rc;
     h->g.id[c] = h->did[c];
-    h->idh_[c].assign(src[c], src[c] + n);
+    h->idh_[c] = t;
   }
   h->nodal_ready = false;
   invalidate_graph(h);
@@ -781,23 +814,13 @@ int fdtd_read_probes(fdtd_handle *h, float *out, int32_t max_samples, int32_t *n
 int fdtd_get_field(fdtd_handle *h, int32_t comp, float *host) {
   if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field read");
   hipSetDevice(h->device);
-  const DevGrid &g = h->g;
-  HIPCHK(h, hipMemcpy2DAsync(host, (size_t)(g.nx + 3) * 4, h->fld[comp] + kXOff - 1, (size_t)g.px * 4,
-                             (size_t)(g.nx + 3) * 4, (size_t)(g.ny + 3) * (g.nz + 3),
-                             hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return FDTD_OK;
+  return copy_box(h, comp, h->fld[comp], nullptr, host, 4);
 }
 
 int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host) {
   if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field write");
   hipSetDevice(h->device);
-  const DevGrid &g = h->g;
-  HIPCHK(h, hipMemcpy2DAsync(h->fld[comp] + kXOff - 1, (size_t)g.px * 4, host, (size_t)(g.nx + 3) * 4,
-                             (size_t)(g.nx + 3) * 4, (size_t)(g.ny + 3) * (g.nz + 3),
-                             hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return FDTD_OK;
+  return copy_box(h, comp, h->fld[comp], host, nullptr, 4);
 }
 
 int fdtd_synchronize(fdtd_handle *h) {

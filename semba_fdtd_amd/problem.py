@@ -152,6 +152,18 @@ def sweep_hi(c: int, a: int, n: Tuple[int, int, int]) -> int:
     return n[a] if node else n[a] - 1
 
 
+def alloc_extent(c: int, n: Tuple[int, int, int]) -> Tuple[int, int, int]:
+    """(EX, EY, EZ): component c's Alloc box [-1, last swept + 1] per axis,
+    the shape launch_simulation allocates it with (timestepping.F90:380-386)."""
+    return tuple(sweep_hi(c, a, n) + 3 for a in range(3))
+
+
+def to_alloc_box(a: np.ndarray, c: int, n) -> np.ndarray:
+    """Component c's Alloc box out of a uniform (nz+3, ny+3, nx+3) array."""
+    ex, ey, ez = alloc_extent(c, n)
+    return np.ascontiguousarray(a[:ez, :ey, :ex])
+
+
 def courant_dt(dmin, cfl=None, dt_in=None, eps0=EPS0, mu0=MU0) -> float:
     """semba_fdtd.F90:1211-1250 with real(4) arithmetic."""
     f = np.float32

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import FdtdError, f32p, f64p, i16p, i32p, ptr
-from .problem import Problem, PMAX, KIND_EDISP
+from .problem import Problem, PMAX, KIND_EDISP, alloc_extent, to_alloc_box
 
 MODES = {"fast": _lib.MODE_FAST, "exact": _lib.MODE_EXACT}
 
@@ -104,6 +104,7 @@ class Solver:
                  tables: Tables | None = None, upload_media: bool = True, probe_buffer: int = 4096):
         self.lib = lib = _lib.load()
         self.pb = pb
+        self.n3 = (pb.nx, pb.ny, pb.nz)
         self.mode = mode
         self.t = t = tables if tables is not None else build_tables(pb)
         d = _lib.FdtdGridDesc(pb.nx, pb.ny, pb.nz, MODES[mode], _lib.STORE_FP32, device,
@@ -121,9 +122,10 @@ class Solver:
             self._apply_scene()
         elif upload_media:
             for c in range(6):
-                m = np.ascontiguousarray(pb.mi[c], np.int16)
+                m = to_alloc_box(pb.mi[c].astype(np.int16, copy=False), c, self.n3)
                 self._ck(lib.fdtd_set_media_index(h, c, ptr(m, i16p)), "fdtd_set_media_index")
-        ids = t.ide + t.idh
+        # Idx/y/zh span Alloc(iEx..iEz): one entry fewer than the primal tables
+        ids = t.ide + [np.ascontiguousarray(a[:-1]) for a in t.idh]
         self._ck(lib.fdtd_set_spacing(h, *[ptr(a, f32p) for a in ids]), "fdtd_set_spacing")
         if pb.has_cpml:
             lay = np.array([pb.cpml_layers[a][s] for a in range(3) for s in range(2)], np.int32)
@@ -236,13 +238,19 @@ class Solver:
         self._ck(self.lib.fdtd_synchronize(self.h), "fdtd_synchronize")
 
     def field(self, c: int) -> np.ndarray:
+        """Component c on the uniform (nz+3, ny+3, nx+3) box; positions
+        outside the component's own Alloc box read as zero."""
+        ex, ey, ez = alloc_extent(c, self.n3)
+        box = np.zeros((ez, ey, ex), np.float32)
+        self._ck(self.lib.fdtd_get_field(self.h, c, ptr(box, f32p)), "fdtd_get_field")
         out = np.zeros(self.pb.alloc_shape, np.float32)
-        self._ck(self.lib.fdtd_get_field(self.h, c, ptr(out, f32p)), "fdtd_get_field")
+        out[:ez, :ey, :ex] = box
         return out
 
     def set_field(self, c: int, a: np.ndarray):
-        a = np.ascontiguousarray(a, np.float32)
-        self._ck(self.lib.fdtd_set_field(self.h, c, ptr(a, f32p)), "fdtd_set_field")
+        """Load component c from a uniform (nz+3, ny+3, nx+3) array."""
+        box = to_alloc_box(np.asarray(a, np.float32), c, self.n3)
+        self._ck(self.lib.fdtd_set_field(self.h, c, ptr(box, f32p)), "fdtd_set_field")
 
     @property
     def device_bytes(self) -> int:
