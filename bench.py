@@ -8,7 +8,7 @@ CPML and pole state fused in, probes sampled), replayed from a captured HIP
 graph.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--n N] [--mode fast|exact]
-                    [--no-cpu] [--profile]
+                    [--kernel-reps R] [--no-cpu] [--cpu-n N] [--cpu-steps K]
 """
 import argparse
 import json
@@ -38,18 +38,37 @@ def psi_counts(pb):
 
 
 def algorithmic_bytes(pb, ndisp_edges, mode, stored_poles):
-    """Compulsory HBM bytes of one E pass and one H pass.
-    Per cell and pass: read the 3 updated components (12 B), the 3 curl
-    components (12 B) and 3 int16 media indices (6 B), write 3 components
-    (12 B) = 42 B.  CPML: 8 B per psi value (read + write).  Poles: per
-    dispersive edge 8 B of Eprev (r+w) + 16 B per stored complex64 state
+    """Compulsory HBM bytes per launch of the three step kernels.
+
+    E face pass (Yee E + CPML E) and H pass (Yee H + CPML H), per cell: read
+    the 3 updated components (12 B), the 3 curl components (12 B) and 3 int16
+    media indices (6 B), write 3 components (12 B) = 42 B; CPML adds 8 B per
+    psi value (read + write).  Pole pass, per dispersive edge: E read + write
+    (8 B), Eprev read + write (8 B) and 16 B per stored complex64 pole state
     (FAST, conjugate pairs folded) or 32 B per complex128 state (EXACT)."""
     cells = pb.cells
     pe, ph = psi_counts(pb)
     per_state = 16 if mode == "fast" else 32
-    e = 42 * cells + 8 * pe + ndisp_edges * (8 + per_state * stored_poles)
-    h = 42 * cells + 8 * ph
-    return e, h
+    return {"e_face": 42 * cells + 8 * pe,
+            "ade": ndisp_edges * (16 + per_state * stored_poles),
+            "h": 42 * cells + 8 * ph}
+
+
+def step_algorithmic_bytes(pb, ndisp_edges, mode, stored_poles):
+    """Compulsory HBM bytes of one whole step with half-step passes that touch
+    each array once: 84 B per cell (two passes of 42 B), 8 B per psi value,
+    and per dispersive edge 8 B of Eprev plus the pole state (its E access is
+    already in the 84 B)."""
+    pe, ph = psi_counts(pb)
+    per_state = 16 if mode == "fast" else 32
+    return 84 * pb.cells + 8 * (pe + ph) + ndisp_edges * (8 + per_state * stored_poles)
+
+
+KERNELS = {
+    "e_face": "k_epass (Yee E + CPML E, fused face pass)",
+    "ade": "k_ade_box (pole-residue ADE over the dispersive edges)",
+    "h": "k_hpass (Yee H + CPML H, fused face pass)",
+}
 
 
 def cpu_baseline(n_sample, steps):
@@ -82,8 +101,8 @@ def main():
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=192)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--disp-fraction", type=float, default=0.2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 1 GiB device copies (the known-size kernel that "
@@ -100,8 +119,9 @@ def main():
                           "bytes_written_per_launch": y.numel() * 4, "launches": 5}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(independent grids shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: the "
+              f"whole grid is resident on one GPU and nothing here spans several",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -135,7 +155,10 @@ def main():
     ms_per_step = ms / args.steps
     value = pb.cells * args.steps / (ms / 1e3) / 1e6
 
-    # per-pass kernel durations on the same stream (E pass: Yee + CPML + poles)
+    # per-kernel durations on the stream the kernels run on.  fdtd_advance_h
+    # launches k_hpass alone (this workload has no H sources),
+    # fdtd_advance_e_dispersive launches k_ade_box alone, and fdtd_advance_e
+    # launches k_epass then k_ade_box (+ the nodal E kernel).
     def timed(fn, reps):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
@@ -146,25 +169,30 @@ def main():
         return a.elapsed_time(b) / reps
     lib, h = s.lib, s.h
     n0 = s.n
-    e_ms = timed(lambda: lib.fdtd_advance_e(h, n0), args.kernel_reps)
-    h_ms = timed(lambda: lib.fdtd_advance_h(h, n0), args.kernel_reps)
+    reps = args.kernel_reps
+    ms_k = {"h": timed(lambda: lib.fdtd_advance_h(h, n0), reps)}
+    ms_e = timed(lambda: lib.fdtd_advance_e(h, n0), reps)
+    ms_k["ade"] = timed(lambda: lib.fdtd_advance_e_dispersive(h), reps) if ndisp else 0.0
+    ms_k["e_face"] = ms_e - ms_k["ade"]
     stored = 1 if args.mode == "fast" else 2
-    e_bytes, h_bytes = algorithmic_bytes(pb, ndisp, args.mode, stored)
-    e_gbs = e_bytes / (e_ms / 1e3) / 1e9
-    h_gbs = h_bytes / (h_ms / 1e3) / 1e9
-    step_bytes = e_bytes + h_bytes
+    nbytes = algorithmic_bytes(pb, ndisp, args.mode, stored)
+    gbs = {k: nbytes[k] / (ms_k[k] / 1e3) / 1e9 if ms_k[k] > 0 else 0.0 for k in ms_k}
+    dom = max(ms_k, key=ms_k.get)
+    step_bytes = step_algorithmic_bytes(pb, ndisp, args.mode, stored)
     step_gbs = step_bytes / (ms_per_step / 1e3) / 1e9
     probes = s.probes()
     finite = bool(np.isfinite(probes).all())
     s.close()
 
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # (scaled by a plain copy of known size) for this workload and mode
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "e_pass_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
         if t.get("workload") == pb.name and t.get("mode") == args.mode:
-            traffic = t.get("bytes_per_launch")
+            traffic = t.get("bytes_per_launch", {}).get(dom)
 
     cpu = None
     if not args.no_cpu:
@@ -191,21 +219,21 @@ def main():
             "mode": args.mode,
             "cells": pb.cells,
             "dispersive_edges": ndisp,
+            "step_algorithmic_bytes": step_bytes,
+            "step_algorithmic_gbs": round(step_gbs, 1),
             "step_hbm_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-            "e_pass_ms": round(e_ms, 4),
-            "h_pass_ms": round(h_ms, 4),
-            "h_pass_gbs": round(h_gbs, 1),
+            "kernels": {k: {"ms": round(ms_k[k], 4), "algorithmic_bytes": nbytes[k],
+                            "gbs": round(gbs[k], 1)} for k in ms_k},
             "probes_finite": finite,
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": round(e_gbs, 1),
+            "achieved": round(gbs[dom], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(e_gbs / HBM_PEAK_GBS, 4),
+            "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "E pass (Yee E + CPML E + pole-residue ADE), algorithmic "
-                      f"{e_bytes / pb.cells:.1f} B/cell",
+            "kernel": f"{KERNELS[dom]}, {nbytes[dom]} algorithmic B per launch",
         },
         "cpu_baseline": cpu,
     }
