@@ -1,0 +1,56 @@
+"""Writes the golden fixtures of tests/golden from the reference's own Fortran
+modules (oracle/_ref, built by `make -C oracle ref` from the reference tree).
+
+Each fixture holds, for one small problem of semba_fdtd_amd.problem, the probe
+series (float32 [step][probe]) and the SHA-256 of each final field array in
+the uniform (nz+3, ny+3, nx+3) layout.  Run from the repository root:
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from refrun import run_reference  # noqa: E402
+
+CASES = {
+    "cavity64_hertzian": ("cavity_problem", dict(n=64, nsteps=2000)),
+    "cavity16_soft": ("cavity_problem", dict(n=16, nsteps=300, hard=False)),
+    "scatterer": ("scatterer_problem", dict(nsteps=300)),
+    "scatterer_graded_z": ("scatterer_problem", dict(nsteps=250, graded_z=1.03)),
+    "gedney10_alpha_kappa": ("scatterer_problem", dict(nsteps=150, n=(30, 30, 30),
+                                                        L=((10, 10), (10, 10), (10, 10)),
+                                                        alphamax=0.1, kappamax=3.0, order=2.0)),
+    "one_sided_cpml": ("scatterer_problem", dict(nsteps=150, n=(40, 20, 22),
+                                                 L=((5, 0), (0, 6), (4, 4)), refl=1e-3)),
+}
+
+
+def field_hashes(fields):
+    return [hashlib.sha256(np.ascontiguousarray(f, np.float32).tobytes()).hexdigest()
+            for f in fields]
+
+
+def main():
+    for name, (builder, kw) in CASES.items():
+        with tempfile.TemporaryDirectory() as tmp:
+            out = run_reference(builder, kw, kw["nsteps"], tmp)
+        meta = {"builder": builder, "kwargs": kw, "nsteps": kw["nsteps"],
+                "field_sha256": field_hashes(out["fields"]),
+                "source": "oracle/_ref (reference Fortran modules)"}
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), probes=out["probes"],
+                            meta=np.array(json.dumps(meta)))
+        print(f"{name}: probes {out['probes'].shape}, |max| {np.abs(out['probes']).max():.3e}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,207 @@
+"""What the mathematics fixes, independently of the reference harness.
+
+The harness compiles the reference's CPML, dispersive, nodal-source and
+coefficient modules, but its Yee loops are a restatement; these checks pin
+the oracle's Yee updates, CPML and pole-residue tables with closed forms:
+
+* discrete cavity eigenmodes: leapfrog Yee keeps a discrete eigenmode exact,
+  E^s = shape * cos(w (s - 1/2) dt) / cos(w dt / 2) with
+  sin^2(w dt / 2) = (c dt / 2)^2 K^2, K^2 the discrete curl-curl eigenvalue;
+  one mode per E component covers all twelve curl terms, on unequal
+  spacings so that a transposed spacing table fails;
+* CPML absorption: a pulse in a CPML-bounded box matches a box large enough
+  for its walls to stay causally invisible, to -60 dB, while PEC walls do not;
+* pole-residue tables: the update is the trapezoidal rule of Ampere's law
+  with J = c s/(s - a) E, so its discrete admittance is exactly the analytic
+  one at the bilinear (Tustin) image of the frequency;
+* voxel rule: brute force over single voxels with the loop ranges of
+  CreateVolumeMM (healer.F90:135-231) equals the vectorised painter.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from semba_fdtd_amd import problem as P
+from semba_fdtd_amd import _lib
+
+
+def _cavity(n, d, nsteps):
+    nx, ny, nz = n
+    mi = P.new_media_index(nx, ny, nz)
+    P.pec_walls(mi, n)
+    dt = P.courant_dt(d, cfl=0.9)
+    return P.Problem(nx, ny, nz, epr=[1.0, 1.0], sigma=[0.0, 0.0], mur=[1.0, 1.0],
+                     sigmam=[0.0, 0.0], kind=[P.KIND_PEC, 0], mi=mi,
+                     D=[P.uniform_D(nx, d[0]), P.uniform_D(ny, d[1]), P.uniform_D(nz, d[2])],
+                     dt=dt, nsteps_max=nsteps, name="mode")
+
+
+# E component -> the two axes its mode varies along
+MODE_AXES = {P.EX: (1, 2), P.EY: (0, 2), P.EZ: (0, 1)}
+
+
+@pytest.mark.parametrize("comp", [P.EX, P.EY, P.EZ])
+def test_yee_keeps_discrete_eigenmode(comp):
+    n = (12, 10, 8)
+    d = (0.010, 0.013, 0.008)
+    steps = 400
+    pb = _cavity(n, d, steps + 2)
+    o = oracle.Oracle(pb, precision=64)
+    # the fields the FP64 oracle integrates with
+    eps0, mu0 = oracle.EPS0_64, oracle.MU0_64
+    a, b = MODE_AXES[comp]
+    d32 = [float(np.float32(x)) for x in d]      # cell sizes are stored in real(4)
+    K2 = sum((2.0 / d32[ax]) ** 2 * np.sin(np.pi / (2 * n[ax])) ** 2 for ax in (a, b))
+    w = 2.0 / pb.dt * np.arcsin(pb.dt / 2.0 * np.sqrt(K2 / (eps0 * mu0)))
+    # shape on the component's sweep: sin(pi*idx/n) along the two node axes
+    shape = np.zeros(pb.alloc_shape)
+    idx = np.meshgrid(np.arange(n[2] + 1), np.arange(n[1] + 1), np.arange(n[0] + 1), indexing="ij")
+    val = np.sin(np.pi * idx[2 - a] / n[a]) * np.sin(np.pi * idx[2 - b] / n[b])
+    hi = [P.sweep_hi(comp, ax, n) for ax in range(3)]
+    sl = (slice(1, hi[2] + 2), slice(1, hi[1] + 2), slice(1, hi[0] + 2))
+    shape[sl] = val[:hi[2] + 1, :hi[1] + 1, :hi[0] + 1]
+    o.fields[comp][...] = shape
+    for s in range(1, steps + 1):
+        o.step()
+        if s % 50 == 0:
+            want = shape * np.cos(w * (s - 0.5) * pb.dt) / np.cos(w * pb.dt / 2)
+            err = np.abs(o.fields[comp] - want).max()
+            assert err < 1e-9, (s, err)
+            for c in range(3):
+                if c != comp:
+                    assert not o.fields[c].any()
+    # the mode went through several periods
+    assert w * steps * pb.dt > 4 * np.pi
+
+
+def _open_box(n, L, nsteps, pec=False):
+    """Vacuum box with an L-cell CPML (or PEC walls), soft Ez point source at
+    the centre and Ez probes at 3 cells from it along each axis."""
+    nx = ny = nz = n
+    mi = P.new_media_index(nx, ny, nz)
+    if pec:
+        P.pec_walls(mi, (n, n, n))
+    dx = 0.01
+    dt = P.courant_dt((dx, dx, dx), cfl=0.9)
+    evol = P.gaussian_table(dt, nsteps + 4, 6.0, 24.0)
+    c = n // 2
+    src = P.NodalSource(P.EZ, False, (c, c, c, c, c, c), 1.0, evol, float(np.float32(dt)))
+    probes = [(P.EZ, c + 3, c, c), (P.EZ, c, c + 3, c), (P.EZ, c, c, c + 3), (P.EZ, c - 3, c - 2, c)]
+    lay = [[0, 0]] * 3 if pec else [[L, L]] * 3
+    return P.Problem(nx, ny, nz, epr=[1.0, 1.0], sigma=[0.0, 0.0], mur=[1.0, 1.0],
+                     sigmam=[0.0, 0.0], kind=[P.KIND_PEC, 0], mi=mi, D=[P.uniform_D(n, dx)] * 3,
+                     dt=dt, cpml_layers=lay, cpml_order=[[3.0, 3.0]] * 3,
+                     cpml_refl=[[1e-6, 1e-6]] * 3, nodal=[src], probes=probes,
+                     nsteps_max=nsteps, name="openbox")
+
+
+def test_cpml_absorbs():
+    steps = 160
+    small = oracle.Oracle(_open_box(36, 10, steps)).run(steps).astype(np.float64)
+    # the big box's walls are > 84 cells from the probes: their echo needs
+    # > 160 steps at c dt = 0.52 cells/step
+    big = oracle.Oracle(_open_box(200, 10, steps)).run(steps).astype(np.float64)
+    pec = oracle.Oracle(_open_box(36, 10, steps, pec=True)).run(steps).astype(np.float64)
+    ref = np.abs(big).max()
+    err_cpml = np.abs(small - big).max() / ref
+    err_pec = np.abs(pec - big).max() / ref
+    assert err_cpml < 1e-3, err_cpml        # -60 dB (measured about -98 dB)
+    assert err_pec > 1e-2, err_pec           # the walls alone reflect strongly
+
+
+def test_pole_residue_tables_are_the_bilinear_image_of_the_medium():
+    """The dispersive update (electricdispersive.F90:98-146, 304-362) is the
+    trapezoidal rule of Ampere's law eps_inf E' + sigma E + sum_p J_p = curl H
+    with J_p = c_p s/(s - a_p) E.  Trapezoidal in time is the bilinear map
+    s' = (2/dt)(z-1)/(z+1), z = exp(jw dt), so the discrete admittance must be
+    exactly Y(s') = s' eps_inf + sigma + sum_p c_p s'/(s' - a_p).  From the
+    library's tables: J_p = beta_p (z-1)/(dt (z-kappa_p)) E and the update
+    E(z - G1) = G2 X z^(1/2) - sum_p G3_p J_p (a conjugate pair makes the
+    real part of the sum the sum itself), with X z^(1/2) 2/(z+1) = Y."""
+    lib = _lib.load()
+    eps0 = P.EPS0
+    eps_inf_r, sigma, deps = 2.0, 0.005, 1.5
+    f0, delta = 5e9, 2e9
+    c, a = P.lorentz_pair(f0, delta, deps)
+    dt = P.courant_dt((0.002, 0.002, 0.002), cfl=0.9)
+    kk = np.zeros((8, 2)); bb = np.zeros((8, 2)); gg = np.zeros((8, 2))
+    g1 = np.zeros(1, np.float32); g2 = np.zeros(1, np.float32)
+    cc = np.ascontiguousarray(c); aa = np.ascontiguousarray(a)
+    f64p, f32p = _lib.f64p, _lib.f32p
+    eps_inf = float(np.float32(eps_inf_r * eps0))
+    rc = lib.fdtd_build_ade(2, _lib.ptr(cc, f64p), _lib.ptr(aa, f64p), eps_inf, sigma, dt,
+                            _lib.ptr(kk, f64p), _lib.ptr(bb, f64p), _lib.ptr(gg, f64p),
+                            _lib.ptr(g1, f32p), _lib.ptr(g2, f32p))
+    assert rc == 0
+    kap = kk[:2, 0] + 1j * kk[:2, 1]
+    bet = bb[:2, 0] + 1j * bb[:2, 1]
+    g3 = gg[:2, 0] + 1j * gg[:2, 1]
+    # conjugate input -> conjugate coefficients
+    assert np.allclose(kap[1], np.conj(kap[0]), rtol=1e-15)
+    assert np.allclose(bet[1], np.conj(bet[0]), rtol=1e-15)
+    assert np.allclose(g3[1], np.conj(g3[0]), rtol=1e-15)
+    ap = a[:, 0] + 1j * a[:, 1]
+    cp = c[:, 0] + 1j * c[:, 1]
+    sig32 = float(np.float32(sigma))
+    for f in (0.5e9, 1e9, 3e9, 4.5e9, 5e9, 5.5e9, 8e9, 20e9):
+        w = 2 * np.pi * f
+        z = np.exp(1j * w * dt)
+        J = bet * (z - 1) / (dt * (z - kap))
+        Y_d = ((z - float(g1[0])) + np.sum(g3 * J)) / float(g2[0]) * 2 / (z + 1)
+        s_ = 2 / dt * (z - 1) / (z + 1)
+        Y = s_ * eps_inf + sig32 + np.sum(cp * s_ / (s_ - ap))
+        rel = abs(Y_d - Y) / abs(Y)
+        # G1 and G2 are stored in real(4): G1's rounding is relative to |z - 1|
+        assert rel < 1e-6 + 4 * 6e-8 / abs(z - 1), (f, rel)
+    # the pair adds deps to the static permittivity
+    assert abs(np.sum(cp / (-ap)).real - deps * eps0) < 1e-6 * deps * eps0
+
+
+def test_voxel_rule_brute_force():
+    rng = np.random.default_rng(5)
+    n = (7, 6, 5)
+    nx, ny, nz = n
+    prio = np.array([100, 0, 10, 20, 10], np.int64)
+    mi_vec = P.new_media_index(nx, ny, nz)
+    mi_bf = P.new_media_index(nx, ny, nz)
+    for medium in (2, 3, 4, 2):
+        inside = rng.random((nz, ny, nx)) < 0.3
+        P.paint_voxels(mi_vec, inside, medium, prio)
+        # CreateVolumeMM for one voxel at a time: E edges claimed on strictly
+        # higher priority, H faces on higher or equal (healer.F90:135-231)
+        for k, j, i in zip(*np.nonzero(inside)):
+            ranges = {
+                P.EX: ((i, i), (j, j + 1), (k, k + 1)),
+                P.EY: ((i, i + 1), (j, j), (k, k + 1)),
+                P.EZ: ((i, i + 1), (j, j + 1), (k, k)),
+                P.HX: ((i, i + 1), (j, j), (k, k)),
+                P.HY: ((i, i), (j, j + 1), (k, k)),
+                P.HZ: ((i, i), (j, j), (k, k + 1)),
+            }
+            for c, ((x0, x1), (y0, y1), (z0, z1)) in ranges.items():
+                for zz in range(z0, z1 + 1):
+                    for yy in range(y0, y1 + 1):
+                        for xx in range(x0, x1 + 1):
+                            cur = mi_bf[c][zz + 1, yy + 1, xx + 1]
+                            win = prio[medium] > prio[cur] if c < 3 else prio[medium] >= prio[cur]
+                            if win:
+                                mi_bf[c][zz + 1, yy + 1, xx + 1] = medium
+    assert np.array_equal(mi_vec, mi_bf)
+
+
+def test_spacing_tables_closed_form():
+    """dxe(i) = DX(i), dxh(i) = (DX(i) + DX(i-1))/2 over [-1, n+1]
+    (timestepping.F90:314-370) on a graded axis."""
+    lib = _lib.load()
+    n = 20
+    D = P.graded_D(n, 0.01, 1.07, (3, 4))
+    de = np.zeros(n + 3, np.float32); dh = np.zeros_like(de)
+    ie = np.zeros_like(de); ih = np.zeros_like(de)
+    f32p = _lib.f32p
+    assert lib.fdtd_build_spacing(n, _lib.ptr(D, f32p), _lib.ptr(de, f32p), _lib.ptr(dh, f32p),
+                                  _lib.ptr(ie, f32p), _lib.ptr(ih, f32p)) == 0
+    for i in range(-1, n + 2):
+        assert de[i + 1] == D[i + 2]
+        assert dh[i + 1] == np.float32((np.float32(D[i + 2]) + np.float32(D[i + 1])) / np.float32(2))
+        assert ie[i + 1] == np.float32(1) / de[i + 1]
+        assert ih[i + 1] == np.float32(1) / dh[i + 1]
