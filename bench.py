@@ -107,7 +107,14 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 1 GiB device copies (the known-size kernel that "
                          "scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only run semba_fdtd_amd/hbm_probe (achievable HBM bandwidth per access "
+                         "shape) and exit")
     args = ap.parse_args()
+    if args.hbm_probe:
+        import subprocess
+        exe = os.path.join(ROOT, "semba_fdtd_amd", "hbm_probe")
+        sys.exit(subprocess.run([exe]).returncode)
     if args.copy_calibration:
         import torch
         x = torch.ones(2**28, dtype=torch.float32, device="cuda")
