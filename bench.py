@@ -221,7 +221,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{pb.name}: {pb.nx}^3 cells incl. 12-cell CPML, 8 lossy spheres, "
-                        f"2-pole medium in 20% of the volume, soft Ex line source, "
+                        f"2-pole medium in 20% of the volume, TF/SF plane wave (+z, Ex), "
                         f"{len(pb.probes)} probes",
             "mode": args.mode,
             "cells": pb.cells,

@@ -126,22 +126,40 @@ int fdtd_add_nodal(fdtd_handle *h, int32_t comp, int32_t hard,
                    const int32_t box[6], float amp, float deltaevol,
                    int32_t numus, const float *evol);
 
+/* Node coordinates of the discretisation lines (sgg%LineX/Y/Z,
+ * preprocess_geom.F90:5330-5347,5672-5712), each over [-1, n+1] (n+3 entries);
+ * the plane waves place their fields with them. */
+int fdtd_set_lines(fdtd_handle *h, const float *line_x, const float *line_y,
+                   const float *line_z);
+/* One TF/SF plane-wave box (sgg%PlaneWave(j), fdetypes.F90:458-465, set up by
+ * InitPlaneWave, planewaves.F90:72-770): box {esqx1,esqx2,esqy1,esqy2,esqz1,
+ * esqz2} in total cells, nmodes (<= 4) modes with direction p[m][3], E
+ * amplitude e[m][3] and start uncertainty incert[m], excitation samples
+ * evol[0..numus] spaced deltaevol, and the real(4) eps0/mu0 of the run. */
+int fdtd_add_planewave(fdtd_handle *h, const int32_t box[6], int32_t nmodes, const float *p,
+                       const float *e, const float *incert, float deltaevol, int32_t numus,
+                       const float *evol, float eps0, float mu0);
+
 /* ---- the hot path ----------------------------------------------------- */
 /* Fused E half step: Advance_Ex/Ey/Ez (timestepping.F90:1286-1297) +
- * AdvanceelectricCPML (:1381) + AdvanceEDispersiveE (:1416) in one HBM pass,
- * then AdvanceNodalE at tiempo(n) (:1438-1446). */
+ * AdvanceelectricCPML (:1381) in one HBM pass, AdvanceEDispersiveE (:1416)
+ * over the dispersive edges, then AdvancePlaneWaveE (:1425) and AdvanceNodalE
+ * (:1441) at tiempo(n). */
 int fdtd_advance_e(fdtd_handle *h, int32_t n);
 /* Fused H half step: Advance_Hx/Hy/Hz (:1497-1507) + AdvanceMagneticCPML
- * (:1535) in one pass, then AdvanceNodalH at tiempo(n)+dt/2 (:1596-1635). */
+ * (:1535) in one pass, then AdvancePlaneWaveH (:1598) and AdvanceNodalH
+ * (:1615) at tiempo(n)+dt/2. */
 int fdtd_advance_h(fdtd_handle *h, int32_t n);
 /* The same pieces one call site at a time, for drivers that interleave other
  * host modules between them. */
 int fdtd_advance_e_yee(fdtd_handle *h);        /* Advance_Ex/Ey/Ez */
 int fdtd_advance_e_cpml(fdtd_handle *h);       /* AdvanceelectricCPML */
 int fdtd_advance_e_dispersive(fdtd_handle *h); /* AdvanceEDispersiveE */
+int fdtd_advance_planewave_e(fdtd_handle *h, int32_t n); /* AdvancePlaneWaveE */
 int fdtd_advance_nodal_e(fdtd_handle *h, int32_t n); /* AdvanceNodalE */
 int fdtd_advance_h_yee(fdtd_handle *h);        /* Advance_Hx/Hy/Hz */
 int fdtd_advance_h_cpml(fdtd_handle *h);       /* AdvanceMagneticCPML */
+int fdtd_advance_planewave_h(fdtd_handle *h, int32_t n); /* AdvancePlaneWaveH */
 int fdtd_advance_nodal_h(fdtd_handle *h, int32_t n); /* AdvanceNodalH */
 /* nsteps full steps n0 .. n0+nsteps-1 (E half then H half, probes sampled
  * after each step as UpdateObservation does, timestepping.F90:1725-1742),
@@ -186,6 +204,9 @@ int fdtd_build_g_tables(int32_t nmedia, const float *epr, const float *sigma,
  * [-1..n+1] (length n+3). */
 int fdtd_build_spacing(int32_t n, const float *D, float *de, float *dh, float *ide,
                        float *idh);
+/* discretisation lines of one axis: D[-2..n+1] -> line[-1..n+1], origin 0
+ * at the first interior node Llo (preprocess_geom.F90:5330-5347,5690-5712). */
+int fdtd_build_lines(int32_t n, int32_t Llo, int32_t Lhi, const float *D, float *line);
 /* CPML profiles of one axis, calc_cpmlconstants (borderscpml.F90:1465-1733);
  * stretches ide/idh in place. */
 int fdtd_build_cpml_axis(int32_t n, int32_t Llo, int32_t Lhi, float orden_lo,

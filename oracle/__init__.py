@@ -30,6 +30,7 @@ ORACLE_SO = {32: os.path.join(HERE, "liboracle_fdtd.so"),
              64: os.path.join(HERE, "liboracle_fdtd64.so")}
 REF_SO = os.path.join(HERE, "_ref", "libsemba_ref.so")
 PMAX = 8
+PW_MODES = 4
 # vacuum constants as written in the reference (fdetypes.F90:219-222), kept
 # in full for the real(8) build
 EPS0_64 = 8.8541878176203898505365630317107502606083701665994498081024171524053950954599821142852891607182008932e-12
@@ -52,6 +53,12 @@ def _structs(prec):
                     ("ye", C.c_int32), ("zi", C.c_int32), ("ze", C.c_int32),
                     ("amp", rt), ("deltaevol", rt), ("numus", C.c_int32), ("evol", rp)]
 
+    class OrcPW(C.Structure):
+        _fields_ = [("esq", C.c_int32 * 6), ("nmodes", C.c_int32),
+                    ("p", (rt * 3) * PW_MODES), ("e", (rt * 3) * PW_MODES),
+                    ("incert", rt * PW_MODES), ("deltaevol", rt), ("numus", C.c_int32),
+                    ("evol", rp)]
+
     class OrcGrid(C.Structure):
         _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
                     ("f", rp * 6), ("mi", _i16p * 6),
@@ -66,12 +73,14 @@ def _structs(prec):
                     ("eprev", rp * 3), ("jcur", _f64p * 3), ("jstride", C.c_int32),
                     ("dt", C.c_double), ("tiempo", _f64p),
                     ("nnodal", C.c_int32), ("nodal", C.POINTER(OrcNodal)),
-                    ("medkind", _i32p)]
-    return OrcNodal, OrcGrid
+                    ("medkind", _i32p),
+                    ("npw", C.c_int32), ("pw", C.POINTER(OrcPW)), ("line", rp * 3),
+                    ("eps0", rt), ("mu0", rt), ("pws", C.c_void_p)]
+    return OrcNodal, OrcGrid, OrcPW
 
 
 _STRUCTS = {p: _structs(p) for p in (32, 64)}
-OrcNodal, OrcGrid = _STRUCTS[32]
+OrcNodal, OrcGrid, OrcPW = _STRUCTS[32]
 
 
 def _p(a, t):
@@ -89,7 +98,7 @@ def load_oracle(precision: int = 32):
         raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
     rt = _REAL[precision][0]
     rp = C.POINTER(rt)
-    Nod, Grid = _STRUCTS[precision]
+    Nod, Grid, _ = _STRUCTS[precision]
     lib = C.CDLL(path)
     lib.orc_dt.restype = C.c_double
     lib.orc_dt.argtypes = [rt] * 6 + [C.c_int32, C.c_double]
@@ -105,8 +114,12 @@ def load_oracle(precision: int = 32):
     lib.orc_evolucion.argtypes = [rt, C.POINTER(Nod)]
     for fn in ("orc_yee_e", "orc_yee_h", "orc_cpml_e", "orc_cpml_h", "orc_ade_e"):
         getattr(lib, fn).argtypes = [C.POINTER(Grid)]
-    for fn in ("orc_nodal_e", "orc_nodal_h", "orc_step"):
+    for fn in ("orc_nodal_e", "orc_nodal_h", "orc_step", "orc_pw_e", "orc_pw_h"):
         getattr(lib, fn).argtypes = [C.POINTER(Grid), C.c_int32]
+    lib.orc_pw_init.argtypes = [C.POINTER(Grid)]
+    lib.orc_pw_init.restype = C.c_int
+    lib.orc_pw_free.argtypes = [C.POINTER(Grid)]
+    lib.orc_lines.argtypes = [C.c_int32, C.c_int32, C.c_int32, rp, rp]
     lib.orc_run.argtypes = [C.POINTER(Grid), C.c_int32, C.c_int32, C.c_int32,
                             _i32p, _i32p, _i32p, _i32p, rp]
     _LIBS[precision] = lib
@@ -181,6 +194,19 @@ def oracle_tables(lib, pb, precision: int = 32) -> Tables:
                   np.array(disp, np.int32), np.array([pb.npoles[m] for m in disp], np.int32))
 
 
+def lines(lib, pb, precision: int = 32):
+    """Node coordinates LineX/Y/Z over [-1, n+1] (orc_lines)."""
+    rt, dt_ = _REAL[precision]
+    out = []
+    for a, n in enumerate((pb.nx, pb.ny, pb.nz)):
+        D = np.ascontiguousarray(pb.D[a], dt_)
+        ln = np.zeros(n + 3, dt_)
+        lib.orc_lines(n, int(pb.cpml_layers[a][0]), int(pb.cpml_layers[a][1]),
+                      _p(D, C.POINTER(rt)), _p(ln, C.POINTER(rt)))
+        out.append(ln)
+    return out
+
+
 class Oracle:
     """Runs a Problem through the C restatement (precision 32 or 64)."""
 
@@ -189,7 +215,7 @@ class Oracle:
         self.lib = load_oracle(precision)
         rt, dt_ = _REAL[precision]
         rp = C.POINTER(rt)
-        Nod, Grid = _STRUCTS[precision]
+        Nod, Grid, PW = _STRUCTS[precision]
         self.pb = pb
         self.t = tables if tables is not None else oracle_tables(self.lib, pb, precision)
         t = self.t
@@ -253,12 +279,44 @@ class Oracle:
         g.nnodal = ns
         g.nodal = C.cast(self._nod, C.POINTER(Nod))
         g.medkind = _p(self.medkind, _i32p)
+        eps0, mu0 = _constants(pb, precision)
+        g.eps0, g.mu0 = eps0, mu0
+        self.lines = lines(self.lib, pb, precision)
+        for a in range(3):
+            g.line[a] = self.lines[a].ctypes.data_as(rp)
+        npw = len(pb.planewaves)
+        self._pw = (PW * max(npw, 1))()
+        for q, w in enumerate(pb.planewaves):
+            o = self._pw[q]
+            for a in range(6):
+                o.esq[a] = int(w.box[a])
+            o.nmodes = len(w.p)
+            for m in range(len(w.p)):
+                for a in range(3):
+                    o.p[m][a] = w.p[m][a]
+                    o.e[m][a] = w.e[m][a]
+                o.incert[m] = w.incert[m]
+            ev = np.ascontiguousarray(w.evol, dt_)
+            self._keep.append(ev)
+            o.deltaevol, o.numus, o.evol = w.deltaevol, len(ev) - 1, _p(ev, rp)
+        g.npw = npw
+        g.pw = C.cast(self._pw, C.POINTER(PW))
+        g.pws = None
         self.g = g
+        if npw and self.lib.orc_pw_init(C.byref(g)) != 0:
+            raise MemoryError("orc_pw_init failed")
         self.n = 0
 
     def step(self):
         self.lib.orc_step(C.byref(self.g), self.n)
         self.n += 1
+
+    def __del__(self):
+        try:
+            if getattr(self, "g", None) is not None and self.g.pws:
+                self.lib.orc_pw_free(C.byref(self.g))
+        except Exception:
+            pass
 
     def run(self, nsteps, probes=None):
         probes = self.pb.probes if probes is None else probes
@@ -285,6 +343,9 @@ def load_reference():
     lib.ref_nodal.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _f32p, _f32p, _i32p, _f32p,
                               C.c_int32]
     lib.ref_step.argtypes = [C.c_int32]
+    lib.ref_time.argtypes = [C.c_int32]
+    lib.ref_planewave.argtypes = [_i32p, C.c_int32, _f32p, _f32p, _f32p, C.c_float, C.c_int32,
+                                  _f32p, _f32p, _f32p, _f32p]
     lib.ref_get_field.argtypes = [C.c_int32, _f32p]
     lib.ref_get_tables.argtypes = [_f32p] * 10
     return lib
@@ -338,6 +399,22 @@ class Reference:
             self._keep += [comp, hard, box, amp, de, nu, smp]
             lib.ref_nodal(ns, _p(comp, _i32p), _p(hard, _i32p), _p(box, _i32p), _p(amp, _f32p),
                           _p(de, _f32p), _p(nu, _i32p), _p(smp, _f32p), pb.nsteps_max)
+        lib.ref_time(pb.nsteps_max)
+        if len(pb.planewaves) > 1:
+            raise ValueError("the reference harness drives one plane wave")
+        for w in pb.planewaves:
+            # the node coordinates come from the oracle's restatement of the
+            # preprocessing (the reference builds them in preprocess_geom.F90)
+            ln = lines(load_oracle(32), pb, 32)
+            esq = np.array(w.box, np.int32)
+            pp = np.ascontiguousarray(np.array(w.p, np.float32))
+            ee = np.ascontiguousarray(np.array(w.e, np.float32))
+            inc = np.array(w.incert, np.float32)
+            ev = np.ascontiguousarray(w.evol, np.float32)
+            self._keep += [ln, esq, pp, ee, inc, ev]
+            lib.ref_planewave(_p(esq, _i32p), len(w.p), _p(pp, _f32p), _p(ee, _f32p),
+                              _p(inc, _f32p), w.deltaevol, len(ev) - 1, _p(ev, _f32p),
+                              _p(ln[0], _f32p), _p(ln[1], _f32p), _p(ln[2], _f32p))
         self.n = 0
 
     def tables(self):

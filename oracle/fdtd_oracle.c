@@ -16,6 +16,9 @@
  * real(8) and dispersive currents in complex(8); -DORC_REAL64 builds the
  * real(8) variant used as the FP64 yardstick for error statements.
  *
+ * TF/SF plane waves follow planewaves.F90 (InitPlaneWave, Incid,
+ * AdvancePlaneWaveE/H).
+ *
  * Parity pins (see tests/): tables and a full step loop are compared
  * bit-for-bit against the reference's own Fortran modules built from source
  * (oracle/_ref), and against closed forms (Yee cavity eigenfrequencies,
@@ -554,15 +557,282 @@ void orc_nodal_h(orc_grid *g, int32_t n) {
           nodal_apply(g, &g->nodal[s], t);
 }
 
+/* --------------------------------------------------------------------------
+ * TF/SF plane waves, planewaves.F90.
+ * orc_lines restates the discretisation lines of preprocess_geom.F90:
+ * 5330-5347 (interior: line(i+1) = d(i) + line(i), origin 0) and 5690-5712
+ * (CPML and padding extrapolated with the first/last interior cell).
+ * ------------------------------------------------------------------------ */
+void orc_lines(int32_t n, int32_t Llo, int32_t Lhi, const real *DX, real *line) {
+  const int lo = Llo, hi = n - Lhi;            /* interior nodes lo..hi */
+  line[lo + 1] = R(0.0);
+  for (int i = lo; i < hi; ++i) line[i + 2] = DX[i + 2] + line[i + 1];
+  real delta = line[lo + 2] - line[lo + 1];
+  for (int i = lo - 1; i >= -1; --i) line[i + 1] = line[i + 2] - delta;
+  delta = line[hi + 1] - line[hi];
+  for (int i = hi + 1; i <= n + 1; ++i) line[i + 1] = line[i] + delta;
+}
+
+/* face limits of one component (ehxyz entries of TrFr/IzDe/AbAr) */
+typedef struct { int lo[3], hi[3]; int fixed_lo, fixed_hi; } pw_face;
+
+struct orc_pw_state {
+  int npw;
+  real cluz;
+  real *phys[6][3];               /* PhysCoor(field)%x/y/z over [-1, n+1] */
+  struct {
+    real fpw[6][ORC_PW_MODES];    /* E and H amplitudes per mode */
+    real pn[ORC_PW_MODES][3];     /* normalised direction */
+    real dist0[ORC_PW_MODES];     /* distanciaInicial */
+    int ilum[6];                  /* Tr Fr Iz De Ab Ar */
+    pw_face tr[6], iz[6], ab[6];  /* by component: x faces, y faces, z faces */
+  } *w;
+};
+
+static int imax(int a, int b) { return a > b ? a : b; }
+static int imin(int a, int b) { return a < b ? a : b; }
+
+/* SINPMLSweep of component c along axis a (the sweep without the CPML) */
+static int sin_lo(const orc_grid *g, int c, int a) { (void)c; return g->L[a][0]; }
+static int sin_hi(const orc_grid *g, int c, int a) { return sweep_hi(g, c, a) - g->L[a][1]; }
+
+/* InitPlaneWave, planewaves.F90:72-766 */
+int orc_pw_init(orc_grid *g) {
+  orc_pw_free(g);
+  if (g->npw <= 0) return 0;
+  orc_pw_state *S = (orc_pw_state *)calloc(1, sizeof(orc_pw_state));
+  if (!S) return -1;
+  S->npw = g->npw;
+  S->w = calloc((size_t)g->npw, sizeof(*S->w));
+  const int n3[3] = {g->nx, g->ny, g->nz};
+  /* cluz and zvac in the real kind (:91-92) */
+  S->cluz = R(1.0) / RSQRT(g->eps0 * g->mu0);
+  const real zvac = RSQRT(g->mu0 / g->eps0);
+  /* physical coordinates of every component (:94-164): node axes read the
+   * line, cell axes the midpoint of two lines */
+  for (int c = 0; c < 6; ++c)
+    for (int a = 0; a < 3; ++a) {
+      real *v = (real *)calloc((size_t)n3[a] + 3, sizeof(real));
+      S->phys[c][a] = v;
+      const int node = c < 3 ? (c != a) : (c - 3 == a);
+      const int top = sweep_hi(g, c, a) + (node ? 1 : 0);
+      for (int i = -1; i <= top; ++i)
+        v[i + 1] = node ? g->line[a][i + 1]
+                        : (g->line[a][i + 1] + g->line[a][i + 2]) * R(0.5);
+    }
+  for (int q = 0; q < g->npw; ++q) {
+    const orc_planewave *P = &g->pw[q];
+    const int *E = P->esq;  /* x1 x2 y1 y2 z1 z2 */
+    /* illuminated faces (:203-214), against the H sweeps */
+    S->w[q].ilum[0] = E[0] >= sin_lo(g, ORC_HX, 0) && E[0] <= sin_hi(g, ORC_HX, 0);
+    S->w[q].ilum[1] = E[1] <= sin_hi(g, ORC_HX, 0) && E[1] >= sin_lo(g, ORC_HX, 0);
+    S->w[q].ilum[2] = E[2] >= sin_lo(g, ORC_HY, 1) && E[2] <= sin_hi(g, ORC_HY, 1);
+    S->w[q].ilum[3] = E[3] <= sin_hi(g, ORC_HY, 1) && E[3] >= sin_lo(g, ORC_HY, 1);
+    S->w[q].ilum[4] = E[4] >= sin_lo(g, ORC_HZ, 2) && E[4] <= sin_hi(g, ORC_HZ, 2);
+    S->w[q].ilum[5] = E[5] <= sin_hi(g, ORC_HZ, 2) && E[5] >= sin_lo(g, ORC_HZ, 2);
+    /* face limits (:216-295).  For the x faces of Ez/Ey, the y/z range of a
+     * component stops one short of the box end along its own axis. */
+    pw_face *tr = S->w[q].tr, *iz = S->w[q].iz, *ab = S->w[q].ab;
+    for (int c = 0; c < 3; ++c) {
+      for (int a = 0; a < 3; ++a) {
+        const int lo = imax(sin_lo(g, c, a), E[2 * a]);
+        const int hi_box = E[2 * a + 1] - (a == c ? 1 : 0);
+        const int hi = imin(sin_hi(g, c, a), hi_box);
+        tr[c].lo[a] = iz[c].lo[a] = ab[c].lo[a] = lo;
+        tr[c].hi[a] = iz[c].hi[a] = ab[c].hi[a] = hi;
+      }
+      tr[c].fixed_lo = imax(sin_lo(g, c, 0), E[0]);
+      tr[c].fixed_hi = imin(sin_hi(g, c, 0), E[1]);
+      iz[c].fixed_lo = imax(sin_lo(g, c, 1), E[2]);
+      iz[c].fixed_hi = imin(sin_hi(g, c, 1), E[3]);
+      ab[c].fixed_lo = imax(sin_lo(g, c, 2), E[4]);
+      ab[c].fixed_hi = imin(sin_hi(g, c, 2), E[5]);
+    }
+    /* H faces copy the E face they pair with, the low face one cell lower:
+     * Hy<-Ez and Hz<-Ey on x faces, Hz<-Ex and Hx<-Ez on y faces, Hx<-Ey and
+     * Hy<-Ex on z faces */
+    tr[ORC_HY] = tr[ORC_EZ]; tr[ORC_HY].fixed_lo -= 1;
+    tr[ORC_HZ] = tr[ORC_EY]; tr[ORC_HZ].fixed_lo -= 1;
+    iz[ORC_HZ] = iz[ORC_EX]; iz[ORC_HZ].fixed_lo -= 1;
+    iz[ORC_HX] = iz[ORC_EZ]; iz[ORC_HX].fixed_lo -= 1;
+    ab[ORC_HX] = ab[ORC_EY]; ab[ORC_HX].fixed_lo -= 1;
+    ab[ORC_HY] = ab[ORC_EX]; ab[ORC_HY].fixed_lo -= 1;
+    /* directions, amplitudes (:330-342) and H amplitudes (:1494-1509) */
+    for (int m = 0; m < P->nmodes; ++m) {
+      const real px = P->p[m][0], py = P->p[m][1], pz = P->p[m][2];
+      const real modulus = RSQRT(px * px + py * py + RPOW(pz, R(2.0)));
+      real *pn = S->w[q].pn[m];
+      pn[0] = px / modulus; pn[1] = py / modulus; pn[2] = pz / modulus;
+      S->w[q].fpw[0][m] = P->e[m][0];
+      S->w[q].fpw[1][m] = P->e[m][1];
+      S->w[q].fpw[2][m] = P->e[m][2];
+      S->w[q].fpw[3][m] = (pn[1] * P->e[m][2] - pn[2] * P->e[m][1]) / zvac;
+      S->w[q].fpw[4][m] = (pn[2] * P->e[m][0] - pn[0] * P->e[m][2]) / zvac;
+      S->w[q].fpw[5][m] = (pn[0] * P->e[m][1] - pn[1] * P->e[m][0]) / zvac;
+      /* null-phase corner (:372-407): per axis the low corner when the
+       * direction is >= 0, else the high one, clipped to the non-CPML region */
+      real c0[3];
+      const int hc[3] = {ORC_HX, ORC_HY, ORC_HZ};
+      real span2[3];
+      for (int a = 0; a < 3; ++a) {
+        const int clo = imax(E[2 * a] - 1, sin_lo(g, hc[a], a));
+        const int chi = imin(E[2 * a + 1] + 1, sin_hi(g, hc[a], a));
+        c0[a] = pn[a] >= R(0.0) ? g->line[a][clo + 1] : g->line[a][chi + 1];
+        const real dd = g->line[a][clo + 1] - g->line[a][chi + 1];
+        span2[a] = RPOW(dd, R(2.0));
+      }
+      const real diag = RSQRT(span2[0] + span2[1] + span2[2]);
+      S->w[q].dist0[m] = (c0[0] * pn[0] + c0[1] * pn[1] + c0[2] * pn[2]) - P->incert[m] * diag;
+    }
+  }
+  g->pws = S;
+  return 0;
+}
+
+void orc_pw_free(orc_grid *g) {
+  orc_pw_state *S = g->pws;
+  if (!S) return;
+  for (int c = 0; c < 6; ++c)
+    for (int a = 0; a < 3; ++a) free(S->phys[c][a]);
+  free(S->w);
+  free(S);
+  g->pws = NULL;
+}
+
+/* evolucion of the plane wave (:824-851): zero before the second sample and
+ * after the table */
+static real pw_evol(const orc_planewave *P, real cluz, real t, real d) {
+  real ev = R(0.0);
+  long long nprev = (long long)(int)((t - d / cluz) / P->deltaevol);
+  if (nprev + 1 <= P->numus) {
+    if (nprev > 0)
+      ev = (P->evol[nprev + 1] - P->evol[nprev]) / P->deltaevol *
+               ((t - d / cluz) - (real)nprev * P->deltaevol) + P->evol[nprev];
+  }
+  return ev;
+}
+
+/* Incid (:778-853) for plane wave q, component nf at (i,j,k) */
+static real pw_incid(const orc_grid *g, int q, int nf, real t, int i, int j, int k) {
+  const orc_pw_state *S = g->pws;
+  const orc_planewave *P = &g->pw[q];
+  const real xf = S->phys[nf][0][i + 1], yf = S->phys[nf][1][j + 1], zf = S->phys[nf][2][k + 1];
+  real ehi = R(0.0);
+  for (int m = 0; m < P->nmodes; ++m) {
+    const real *pn = S->w[q].pn[m];
+    const real d = (xf * pn[0] + yf * pn[1] + zf * pn[2]) - S->w[q].dist0[m];
+    ehi = ehi + S->w[q].fpw[nf][m] * pw_evol(P, S->cluz, t, d);
+  }
+  return ehi;
+}
+
+/* one face of one component: F(pos) += sign * G * Incid(nf, pos + off) * Id(fixed) */
+static void pw_face_apply(orc_grid *g, int q, int comp, int axis, int fixed,
+                          const pw_face *fc, int nf, int off, real sign, real G,
+                          const real *Id, real t) {
+  real *F = g->f[comp];
+  const real id = Id[fixed + 1];
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) { lo[a] = fc->lo[a]; hi[a] = fc->hi[a]; }
+  lo[axis] = hi[axis] = fixed;
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        const int ii = i + (axis == 0 ? off : 0);
+        const int jj = j + (axis == 1 ? off : 0);
+        const int kk = k + (axis == 2 ? off : 0);
+        const real inc = pw_incid(g, q, nf, t, ii, jj, kk);
+        const size_t p = IDX(g, i, j, k);
+        if (sign > R(0.0)) F[p] = F[p] + G * inc * id;
+        else F[p] = F[p] - G * inc * id;
+      }
+}
+
+/* AdvancePlaneWaveE, planewaves.F90:878-1174 (G2 of the background medium 1) */
+void orc_pw_e(orc_grid *g, int32_t n) {
+  if (!g->pws) return;
+  const real t = (real)g->tiempo[n];
+  const real G = g->g2[1];
+  const real *Ix = g->id[3], *Iy = g->id[4], *Iz = g->id[5];
+  for (int q = 0; q < g->npw; ++q) {
+    const int *il = g->pws->w[q].ilum;
+    const pw_face *tr = g->pws->w[q].tr, *iz = g->pws->w[q].iz, *ab = g->pws->w[q].ab;
+    if (il[0]) {  /* back */
+      pw_face_apply(g, q, ORC_EZ, 0, tr[ORC_EZ].fixed_lo, &tr[ORC_EZ], ORC_HY, -1, R(-1), G, Ix, t);
+      pw_face_apply(g, q, ORC_EY, 0, tr[ORC_EY].fixed_lo, &tr[ORC_EY], ORC_HZ, -1, R(1), G, Ix, t);
+    }
+    if (il[1]) {  /* front */
+      pw_face_apply(g, q, ORC_EZ, 0, tr[ORC_EZ].fixed_hi, &tr[ORC_EZ], ORC_HY, 0, R(1), G, Ix, t);
+      pw_face_apply(g, q, ORC_EY, 0, tr[ORC_EY].fixed_hi, &tr[ORC_EY], ORC_HZ, 0, R(-1), G, Ix, t);
+    }
+    if (il[2]) {  /* left */
+      pw_face_apply(g, q, ORC_EX, 1, iz[ORC_EX].fixed_lo, &iz[ORC_EX], ORC_HZ, -1, R(-1), G, Iy, t);
+      pw_face_apply(g, q, ORC_EZ, 1, iz[ORC_EZ].fixed_lo, &iz[ORC_EZ], ORC_HX, -1, R(1), G, Iy, t);
+    }
+    if (il[3]) {  /* right */
+      pw_face_apply(g, q, ORC_EZ, 1, iz[ORC_EZ].fixed_hi, &iz[ORC_EZ], ORC_HX, 0, R(-1), G, Iy, t);
+      pw_face_apply(g, q, ORC_EX, 1, iz[ORC_EX].fixed_hi, &iz[ORC_EX], ORC_HZ, 0, R(1), G, Iy, t);
+    }
+    if (il[4]) {  /* down */
+      pw_face_apply(g, q, ORC_EX, 2, ab[ORC_EX].fixed_lo, &ab[ORC_EX], ORC_HY, -1, R(1), G, Iz, t);
+      pw_face_apply(g, q, ORC_EY, 2, ab[ORC_EY].fixed_lo, &ab[ORC_EY], ORC_HX, -1, R(-1), G, Iz, t);
+    }
+    if (il[5]) {  /* up */
+      pw_face_apply(g, q, ORC_EX, 2, ab[ORC_EX].fixed_hi, &ab[ORC_EX], ORC_HY, 0, R(-1), G, Iz, t);
+      pw_face_apply(g, q, ORC_EY, 2, ab[ORC_EY].fixed_hi, &ab[ORC_EY], ORC_HX, 0, R(1), G, Iz, t);
+    }
+  }
+}
+
+/* AdvancePlaneWaveH, planewaves.F90:1180-1475 (GM2 of medium 1), at
+ * tiempo(n) + dt/2 */
+void orc_pw_h(orc_grid *g, int32_t n) {
+  if (!g->pws) return;
+  const real t = (real)(g->tiempo[n] + (double)R(0.5) * g->dt);
+  const real G = g->gm2[1];
+  const real *Ix = g->id[0], *Iy = g->id[1], *Iz = g->id[2];
+  for (int q = 0; q < g->npw; ++q) {
+    const int *il = g->pws->w[q].ilum;
+    const pw_face *tr = g->pws->w[q].tr, *iz = g->pws->w[q].iz, *ab = g->pws->w[q].ab;
+    if (il[0]) {
+      pw_face_apply(g, q, ORC_HZ, 0, tr[ORC_HZ].fixed_lo, &tr[ORC_HZ], ORC_EY, 1, R(1), G, Ix, t);
+      pw_face_apply(g, q, ORC_HY, 0, tr[ORC_HY].fixed_lo, &tr[ORC_HY], ORC_EZ, 1, R(-1), G, Ix, t);
+    }
+    if (il[1]) {
+      pw_face_apply(g, q, ORC_HZ, 0, tr[ORC_HZ].fixed_hi, &tr[ORC_HZ], ORC_EY, 0, R(-1), G, Ix, t);
+      pw_face_apply(g, q, ORC_HY, 0, tr[ORC_HY].fixed_hi, &tr[ORC_HY], ORC_EZ, 0, R(1), G, Ix, t);
+    }
+    if (il[2]) {
+      pw_face_apply(g, q, ORC_HX, 1, iz[ORC_HX].fixed_lo, &iz[ORC_HX], ORC_EZ, 1, R(1), G, Iy, t);
+      pw_face_apply(g, q, ORC_HZ, 1, iz[ORC_HZ].fixed_lo, &iz[ORC_HZ], ORC_EX, 1, R(-1), G, Iy, t);
+    }
+    if (il[3]) {
+      pw_face_apply(g, q, ORC_HX, 1, iz[ORC_HX].fixed_hi, &iz[ORC_HX], ORC_EZ, 0, R(-1), G, Iy, t);
+      pw_face_apply(g, q, ORC_HZ, 1, iz[ORC_HZ].fixed_hi, &iz[ORC_HZ], ORC_EX, 0, R(1), G, Iy, t);
+    }
+    if (il[4]) {
+      pw_face_apply(g, q, ORC_HX, 2, ab[ORC_HX].fixed_lo, &ab[ORC_HX], ORC_EY, 1, R(-1), G, Iz, t);
+      pw_face_apply(g, q, ORC_HY, 2, ab[ORC_HY].fixed_lo, &ab[ORC_HY], ORC_EX, 1, R(1), G, Iz, t);
+    }
+    if (il[5]) {
+      pw_face_apply(g, q, ORC_HX, 2, ab[ORC_HX].fixed_hi, &ab[ORC_HX], ORC_EY, 0, R(1), G, Iz, t);
+      pw_face_apply(g, q, ORC_HY, 2, ab[ORC_HY].fixed_hi, &ab[ORC_HY], ORC_EX, 0, R(-1), G, Iz, t);
+    }
+  }
+}
+
 /* One full step in the order of ciclo_temporal, timestepping.F90:1248-1981:
- * E Yee -> E CPML -> E ADE -> nodal E -> H Yee -> H CPML -> nodal H. */
+ * E Yee -> E CPML -> E ADE -> plane wave E -> nodal E -> H Yee -> H CPML ->
+ * plane wave H -> nodal H (:1286-1441, :1497-1615). */
 void orc_step(orc_grid *g, int32_t n) {
   orc_yee_e(g);
   orc_cpml_e(g);
   if (g->ndisp) orc_ade_e(g);
+  if (g->pws) orc_pw_e(g, n);
   if (g->nnodal) orc_nodal_e(g, n);
   orc_yee_h(g);
   orc_cpml_h(g);
+  if (g->pws) orc_pw_h(g, n);
   if (g->nnodal) orc_nodal_h(g, n);
 }
 

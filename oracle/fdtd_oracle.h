@@ -51,6 +51,22 @@ typedef struct {
   const orc_real *evol;
 } orc_nodal;
 
+#define ORC_PW_MODES 4 /* max modes per plane wave */
+
+/* One TF/SF plane-wave box (planeonde_t, fdetypes.F90:458-465) */
+typedef struct {
+  int32_t esq[6];      /* esqx1, esqx2, esqy1, esqy2, esqz1, esqz2 (total cells) */
+  int32_t nmodes;
+  orc_real p[ORC_PW_MODES][3];   /* propagation direction (normalised at init) */
+  orc_real e[ORC_PW_MODES][3];   /* E amplitude */
+  orc_real incert[ORC_PW_MODES];
+  orc_real deltaevol;
+  int32_t numus;
+  const orc_real *evol;          /* evol[0..numus] */
+} orc_planewave;
+
+typedef struct orc_pw_state orc_pw_state;
+
 typedef struct {
   int32_t nx, ny, nz;
   orc_real *f[6];              /* Ex Ey Ez Hx Hy Hz, Alloc layout */
@@ -77,6 +93,12 @@ typedef struct {
   int32_t nnodal;
   const orc_nodal *nodal;
   const int32_t *medkind;      /* [nmedia+1] bits: 1 PEC, 2 PMC, 4 E-dispersive */
+  /* TF/SF plane waves */
+  int32_t npw;
+  const orc_planewave *pw;
+  const orc_real *line[3];     /* node coordinates LineX/Y/Z over [-1, n+1] */
+  orc_real eps0, mu0;
+  orc_pw_state *pws;           /* derived by orc_pw_init */
 } orc_grid;
 
 /* ---- coefficient tables ---- */
@@ -110,6 +132,18 @@ void orc_ade_e(orc_grid *g);
 orc_real orc_evolucion(orc_real t, const orc_nodal *s);
 void orc_nodal_e(orc_grid *g, int32_t n);
 void orc_nodal_h(orc_grid *g, int32_t n);
+/* TF/SF plane waves (planewaves.F90): orc_pw_init derives the constants,
+ * coordinates and face limits of InitPlaneWave; orc_pw_e / orc_pw_h are
+ * AdvancePlaneWaveE / AdvancePlaneWaveH at step n. */
+int orc_pw_init(orc_grid *g);
+void orc_pw_free(orc_grid *g);
+void orc_pw_e(orc_grid *g, int32_t n);
+void orc_pw_h(orc_grid *g, int32_t n);
+/* node coordinates of one axis (preprocess_geom.F90:5330-5347, 5690-5712):
+ * origin 0 at the first interior node (total index Llo), cumulative interior
+ * cell sizes, CPML and padding extrapolated with the edge interior cells;
+ * DX over [-2, n+1], line over [-1, n+1]. */
+void orc_lines(int32_t n, int32_t Llo, int32_t Lhi, const orc_real *DX, orc_real *line);
 /* one full step n (E half, then H half) in the reference order */
 void orc_step(orc_grid *g, int32_t n);
 /* run steps n0..n0+nsteps-1, sampling probes after each step:

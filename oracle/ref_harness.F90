@@ -2,8 +2,9 @@
 !
 ! A driver of the reference's own Fortran modules, built from their sources
 ! where they lie (fdetypes, errorreport, calc_constants, borderscpml,
-! electricdispersive, nodalsources; see oracle/Makefile), so that the tests can
-! compare the C restatement in fdtd_oracle.c against the reference itself:
+! electricdispersive, nodalsources, planewaves; see oracle/Makefile), so that
+! the tests can compare the C restatement in fdtd_oracle.c against the
+! reference itself:
 !   - calc_G1G2Gm1Gm2          calc_constants.F90:36-148
 !   - InitCPMLBorders, AdvanceelectricCPML, AdvanceMagneticCPML
 !                              borderscpml.F90:90-473, 584-1176, 1465-1733
@@ -11,6 +12,8 @@
 !                              electricdispersive.F90:77-362
 !   - InitnodalSources, AdvanceNodalE/H
 !                              nodalsources.F90:76-305, 396-777
+!   - InitPlaneWave, AdvancePlaneWaveE/H
+!                              planewaves.F90:72-770, 878-1475
 ! The six Yee loops live in timestepping.F90, which cannot be compiled here
 ! without stand-ins for the MTLN modules it uses, so they are restated below
 ! from timestepping.F90:2165-2409 (one statement each).  Everything else that
@@ -28,6 +31,7 @@ module ref_harness
    use borders_cpml
    use edispersives
    use nodalsources
+   use ilumina
    implicit none
 
    type(SGGFDTDINFO), save, target :: sgg
@@ -40,6 +44,7 @@ module ref_harness
    real(kind=RKIND), allocatable, dimension(:), save :: dxe, dye, dze, dxh, dyh, dzh, &
       Idxe, Idye, Idze, Idxh, Idyh, Idzh
    logical, save :: thereCPML, thereEdisp, thereNodE, thereNodH
+   logical, save :: therePW = .false., stillPW = .true.
    real(kind=RKIND), save :: eps0h, mu0h
    type(limit_t), dimension(1:6), save :: sinpml
 
@@ -104,6 +109,7 @@ contains
 
       gnx = nx; gny = ny; gnz = nz
       eps0h = g_eps0; mu0h = g_mu0
+      nullify(sgg%tiempo)
       sgg%dt = dt
       sgg%NumMedia = nmedia
       ! fullsize (preprocess_geom.F90:5563-5591) in total coordinates
@@ -257,13 +263,60 @@ contains
          off = off + numus(s) + 1
       end do
       call InitnodalSources(sgg, 0, nsrc, sgg%NodalSource, sgg%Sweep, thereNodE, thereNodH)
-      ! crea_timevector, timestepping.F90:3230-3240
+      call ref_time(nsteps)
+   end subroutine ref_nodal
+
+   ! crea_timevector, timestepping.F90:3230-3240 (once)
+   subroutine ref_time(nsteps) bind(C, name="ref_time")
+      integer(c_int), value :: nsteps
+      integer :: i
+      if (associated(sgg%tiempo)) return
       allocate(sgg%tiempo(0:nsteps+2))
       sgg%tiempo(0) = 0.0_RKIND_tiempo
       do i = 1, nsteps + 2
          sgg%tiempo(i) = sgg%tiempo(i-1) + sgg%dt
       end do
-   end subroutine ref_nodal
+   end subroutine ref_time
+
+   ! One TF/SF plane wave through the reference's InitPlaneWave
+   ! (planewaves.F90:72): box esq = x1,x2,y1,y2,z1,z2, modes p(3,m), e(3,m),
+   ! incert(m), excitation samples and the node coordinates of each axis over
+   ! [-1, n+1] (sgg%LineX/Y/Z).
+   subroutine ref_planewave(esq, nmodes, p, e, incert, delta, numus, samples, lx, ly, lz) &
+         bind(C, name="ref_planewave")
+      integer(c_int), intent(in) :: esq(6)
+      integer(c_int), value :: nmodes, numus
+      real(c_float), intent(in) :: p(3, nmodes), e(3, nmodes), incert(nmodes)
+      real(c_float), value :: delta
+      real(c_float), intent(in) :: samples(0:numus)
+      real(c_float), intent(in) :: lx(-1:gnx+1), ly(-1:gny+1), lz(-1:gnz+1)
+      logical :: resume
+      sgg%NumPlaneWaves = 1
+      allocate(sgg%PlaneWave(1))
+      sgg%PlaneWave(1)%esqx1 = esq(1); sgg%PlaneWave(1)%esqx2 = esq(2)
+      sgg%PlaneWave(1)%esqy1 = esq(3); sgg%PlaneWave(1)%esqy2 = esq(4)
+      sgg%PlaneWave(1)%esqz1 = esq(5); sgg%PlaneWave(1)%esqz2 = esq(6)
+      sgg%PlaneWave(1)%nummodes = nmodes
+      sgg%PlaneWave(1)%isRC = .false.
+      sgg%PlaneWave(1)%INCERTMAX = 0.0_RKIND
+      allocate(sgg%PlaneWave(1)%px(nmodes), sgg%PlaneWave(1)%py(nmodes), sgg%PlaneWave(1)%pz(nmodes), &
+               sgg%PlaneWave(1)%ex(nmodes), sgg%PlaneWave(1)%ey(nmodes), sgg%PlaneWave(1)%ez(nmodes), &
+               sgg%PlaneWave(1)%incert(nmodes))
+      sgg%PlaneWave(1)%px = p(1, :); sgg%PlaneWave(1)%py = p(2, :); sgg%PlaneWave(1)%pz = p(3, :)
+      sgg%PlaneWave(1)%ex = e(1, :); sgg%PlaneWave(1)%ey = e(2, :); sgg%PlaneWave(1)%ez = e(3, :)
+      sgg%PlaneWave(1)%incert = incert
+      sgg%PlaneWave(1)%Fichero%Name = 'harness'
+      sgg%PlaneWave(1)%Fichero%NumSamples = numus
+      sgg%PlaneWave(1)%Fichero%DeltaSamples = delta
+      allocate(sgg%PlaneWave(1)%Fichero%Samples(0:numus))
+      sgg%PlaneWave(1)%Fichero%Samples = samples
+      allocate(sgg%LineX(-1:gnx+1), sgg%LineY(-1:gny+1), sgg%LineZ(-1:gnz+1))
+      sgg%LineX = lx; sgg%LineY = ly; sgg%LineZ = lz
+      resume = .false.
+      call InitPlaneWave(sgg, miEx, miEy, miEz, miHx, miHy, miHz, 0, 1, sinpml, therePW, resume, &
+                         eps0h, mu0h)
+      stillPW = .true.
+   end subroutine ref_planewave
 
    ! Yee loops restated from timestepping.F90:2187-2197, 2228-2236, 2272-2280,
    ! 2316-2325, 2356-2364, 2396-2404 (global indices instead of offsets)
@@ -329,10 +382,12 @@ contains
       call yee_e()
       if (thereCPML) call AdvanceelectricCPML(sgg%NumMedia, b, miEx, miEy, miEz, G2, Ex, Ey, Ez, Hx, Hy, Hz)
       if (thereEdisp) call AdvanceEDispersiveE(sgg)
+      if (therePW .and. stillPW) call AdvancePlaneWaveE(sgg, n, b, G2, Idxh, Idyh, Idzh, Ex, Ey, Ez, stillPW)
       if (thereNodE) call AdvanceNodalE(sgg, miEx, miEy, miEz, sgg%NumMedia, n, b, G2, Idxh, Idyh, Idzh, &
                                         Ex, Ey, Ez, .false.)
       call yee_h()
       if (thereCPML) call AdvanceMagneticCPML(sgg%NumMedia, b, miHx, miHy, miHz, GM2, Hx, Hy, Hz, Ex, Ey, Ez)
+      if (therePW .and. stillPW) call AdvancePlaneWaveH(sgg, n, b, GM2, Idxe, Idye, Idze, Hx, Hy, Hz, stillPW)
       if (thereNodH) call AdvanceNodalH(sgg, miHx, miHy, miHz, sgg%NumMedia, n, b, GM2, Idxe, Idye, Idze, &
                                         Hx, Hy, Hz, .false.)
    end subroutine

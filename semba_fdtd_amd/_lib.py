@@ -47,6 +47,12 @@ SIGNATURES = {
     "fdtd_advance_e_yee": (C.c_int, [vp]),
     "fdtd_advance_e_cpml": (C.c_int, [vp]),
     "fdtd_advance_e_dispersive": (C.c_int, [vp]),
+    "fdtd_advance_planewave_e": (C.c_int, [vp, C.c_int32]),
+    "fdtd_advance_planewave_h": (C.c_int, [vp, C.c_int32]),
+    "fdtd_set_lines": (C.c_int, [vp, f32p, f32p, f32p]),
+    "fdtd_add_planewave": (C.c_int, [vp, i32p, C.c_int32, f32p, f32p, f32p, C.c_float,
+                                     C.c_int32, f32p, C.c_float, C.c_float]),
+    "fdtd_build_lines": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, f32p, f32p]),
     "fdtd_advance_nodal_e": (C.c_int, [vp, C.c_int32]),
     "fdtd_advance_h_yee": (C.c_int, [vp]),
     "fdtd_advance_h_cpml": (C.c_int, [vp]),

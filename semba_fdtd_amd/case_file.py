@@ -19,6 +19,9 @@ Layout: little-endian stream, no record markers.
     int32 nnodal; per source: int32 comp, hard, box[6]; float32 amp, deltaevol;
         int32 numus; float32 evol[numus+1]
     int32 nprobe; int32 comp[nprobe], i[nprobe], j[nprobe], k[nprobe]
+    int32 nplanewave; per wave: int32 box[6] (esqx1,esqx2,esqy1,esqy2,esqz1,
+        esqz2), int32 nmodes; float32 p[nmodes][3], e[nmodes][3],
+        incert[nmodes], deltaevol; int32 numus; float32 evol[numus+1]
 
 Probe output written by the driver: int32 nsamples, nprobe; float32
 samples[nsamples][nprobe].
@@ -30,7 +33,7 @@ import numpy as np
 from .problem import Problem, PMAX, to_alloc_box
 
 MAGIC = b"SFDC"
-VERSION = 1
+VERSION = 2
 
 
 def write_case(pb: Problem, path: str, nsteps: int) -> None:
@@ -69,6 +72,12 @@ def write_case(pb: Problem, path: str, nsteps: int) -> None:
         f.write(i32(len(P)))
         for col in range(4):
             f.write(np.ascontiguousarray(P[:, col]).tobytes())
+        f.write(i32(len(pb.planewaves)))
+        for w in pb.planewaves:
+            ev = np.ascontiguousarray(w.evol, "<f4")
+            f.write(i32(*w.box) + i32(len(w.p)))
+            f.write(f32(np.array(w.p, np.float32).ravel()) + f32(np.array(w.e, np.float32).ravel()))
+            f.write(f32(w.incert) + f32([w.deltaevol]) + i32(len(ev) - 1) + ev.tobytes())
 
 
 def read_probe_file(path: str) -> np.ndarray:

@@ -1,5 +1,5 @@
 // fdtd_aux.hip -- small gfx950 kernels around the two passes: nodal sources,
-// point probes, dispersive-edge counting for the compact pole state, and the
+// TF/SF plane-wave faces, point probes, dispersive-edge counting for the compact pole state, and the
 // device-side scene builders used for grids too large to voxelise on the host.
 // Built with -ffp-contract=off: the soft-source update keeps the reference's
 // rounding order.
@@ -42,6 +42,67 @@ int launch_nodal(const DevGrid &g, const NodalCell *cells, int ncells,
   const int blocks = serial ? 1 : (ncells + threads - 1) / threads;
   hipLaunchKernelGGL(k_nodal, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, g,
                      cells, ncells, values, nsrc, n, dn, serial);
+  return (int)hipGetLastError();
+}
+
+// ---- TF/SF plane waves (planewaves.F90) ----------------------------------
+// evolucion of the plane wave (planewaves.F90:824-851): linear interpolation
+// of the excitation at the retarded time t - d/c, zero before the second
+// sample and after the table.
+__device__ __forceinline__ float pw_evol(const PwWave &w, float t, float d) {
+  float ev = 0.0f;
+  const long long nprev = (long long)(int)((t - d / w.cluz) / w.deltaevol);
+  if (nprev + 1 <= w.numus && nprev > 0)
+    ev = (w.evol[nprev + 1] - w.evol[nprev]) / w.deltaevol *
+             ((t - d / w.cluz) - (float)nprev * w.deltaevol) + w.evol[nprev];
+  return ev;
+}
+
+// one face of one component of AdvancePlaneWaveE/H (planewaves.F90:878-1475):
+// Incid (:778-853) at the neighbouring incident component, then
+// F = F +- G * Incid * Id.  Threads tile the two free axes, x first.
+__global__ void __launch_bounds__(256) k_tfsf(DevGrid g, PwFace f, const PwWave *waves,
+                                              const float *phys, int pstride,
+                                              const double *tiempo, int n, const int *dn,
+                                              int half, double dt, float G) {
+  const int ua = f.axis == 0 ? 1 : 0;
+  const int va = f.axis == 2 ? 1 : 2;
+  const int u = f.lo[ua] + blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = f.lo[va] + blockIdx.y * blockDim.y + threadIdx.y;
+  if (u > f.hi[ua] || v > f.hi[va]) return;
+  int c[3];
+  c[f.axis] = f.fixed;
+  c[ua] = u;
+  c[va] = v;
+  if (dn) n = *dn;
+  const float t = half ? (float)(tiempo[n] + (double)0.5f * dt) : (float)tiempo[n];
+  int q[3] = {c[0], c[1], c[2]};
+  q[f.axis] += f.off;
+  const float xf = phys[(size_t)(f.nf * 3 + 0) * pstride + q[0] + 1];
+  const float yf = phys[(size_t)(f.nf * 3 + 1) * pstride + q[1] + 1];
+  const float zf = phys[(size_t)(f.nf * 3 + 2) * pstride + q[2] + 1];
+  const PwWave &w = waves[f.wave];
+  float ehi = 0.0f;
+  for (int m = 0; m < w.nmodes; ++m) {
+    const float d = (xf * w.pn[m][0] + yf * w.pn[m][1] + zf * w.pn[m][2]) - w.dist0[m];
+    ehi = ehi + w.fpw[f.nf][m] * pw_evol(w, t, d);
+  }
+  float *F = g.f[f.comp];
+  const long long p = fidx(g, c[0], c[1], c[2]);
+  F[p] = f.sign > 0.0f ? F[p] + G * ehi * f.id : F[p] - G * ehi * f.id;
+}
+
+int launch_tfsf(const DevGrid &g, const PwFace &f, const PwWave *waves, const float *phys,
+                int pstride, const double *tiempo, int n, const int *dn, int half, double dt,
+                float G, void *stream) {
+  const int ua = f.axis == 0 ? 1 : 0;
+  const int va = f.axis == 2 ? 1 : 2;
+  const int nu = f.hi[ua] - f.lo[ua] + 1, nv = f.hi[va] - f.lo[va] + 1;
+  if (nu <= 0 || nv <= 0) return 0;
+  const dim3 blk(64, 4);
+  const dim3 grd((unsigned)((nu + 63) / 64), (unsigned)((nv + 3) / 4));
+  hipLaunchKernelGGL(k_tfsf, grd, blk, 0, (hipStream_t)stream, g, f, waves, phys, pstride,
+                     tiempo, n, dn, half, dt, G);
   return (int)hipGetLastError();
 }
 

@@ -25,6 +25,15 @@ struct Source {
   int numus;
   std::vector<float> evol;
 };
+
+struct PlaneWaveIn {     // sgg%PlaneWave(j) (planeonde_t, fdetypes.F90:458-465)
+  int esq[6];            // esqx1, esqx2, esqy1, esqy2, esqz1, esqz2
+  int nmodes;
+  float p[kPwModes][3], e[kPwModes][3], incert[kPwModes];
+  float deltaevol;
+  std::vector<float> evol;
+  float eps0, mu0;
+};
 }  // namespace
 
 struct fdtd_handle {
@@ -70,6 +79,16 @@ struct fdtd_handle {
   int serial[2] = {0, 0};
   float *dvalues = nullptr;
   int nvalues_steps = 0;
+  // TF/SF plane waves
+  std::vector<PlaneWaveIn> pwin;
+  std::vector<float> line[3];           // LineX/Y/Z over [-1, n+1]
+  bool pw_ready = false;
+  std::vector<void *> pw_allocs;
+  std::vector<PwFace> pw_faces[2];      // E faces, H faces, in reference order
+  PwWave *dwaves = nullptr;
+  float *dphys = nullptr;
+  int pstride = 0;
+  double *dtiempo = nullptr;
   // probes
   ProbeDev *dprobes = nullptr;
   int nprobe = 0;
@@ -378,11 +397,147 @@ int build_nodal(fdtd_handle *h) {
   return FDTD_OK;
 }
 
+// ---- TF/SF plane waves ------------------------------------------------------
+int sweep_last(const fdtd_handle *h, int c, int a) {
+  const bool node = c < 3 ? (c != a) : (c - 3 == a);
+  return node ? axis_n(h, a) : axis_n(h, a) - 1;
+}
+// SINPMLSweep of component c along axis a: the sweep without the CPML slabs
+int sin_lo(const fdtd_handle *h, int, int a) { return h->g.L[a][0]; }
+int sin_hi(const fdtd_handle *h, int c, int a) { return sweep_last(h, c, a) - h->g.L[a][1]; }
+
+// InitPlaneWave (planewaves.F90:72-766) and calc_planewaveconstants
+// (:1494-1509) on the host, in real(4) with the reference's expression order;
+// the faces of AdvancePlaneWaveE/H (:878-1475) become one PwFace each, in the
+// order the reference applies them.
+int build_planewaves(fdtd_handle *h) {
+  if (h->pw_ready) return FDTD_OK;
+  for (void *p : h->pw_allocs) hipFree(p);
+  h->pw_allocs.clear();
+  h->pw_faces[0].clear();
+  h->pw_faces[1].clear();
+  h->dwaves = nullptr; h->dphys = nullptr; h->dtiempo = nullptr;
+  if (h->pwin.empty()) { h->pw_ready = true; return FDTD_OK; }
+  if (h->tiempo.empty()) return fail(h, FDTD_ERR_STATE, "fdtd_set_time not called");
+  if (h->line[0].empty()) return fail(h, FDTD_ERR_STATE, "fdtd_set_lines not called");
+  if (h->nmedia < 1) return fail(h, FDTD_ERR_STATE, "plane waves need the background medium 1");
+  const int n3[3] = {h->g.nx, h->g.ny, h->g.nz};
+  const int S = std::max(n3[0], std::max(n3[1], n3[2])) + 3;
+  // physical coordinates (:94-164): node axes on the line, cell axes midway
+  std::vector<float> phys((size_t)18 * S, 0.0f);
+  for (int c = 0; c < 6; ++c)
+    for (int a = 0; a < 3; ++a) {
+      const bool node = c < 3 ? (c != a) : (c - 3 == a);
+      const int top = sweep_last(h, c, a) + (node ? 1 : 0);
+      float *v = &phys[(size_t)(c * 3 + a) * S];
+      const std::vector<float> &L = h->line[a];
+      for (int i = -1; i <= top; ++i)
+        v[i + 1] = node ? L[i + 1] : (L[i + 1] + L[i + 2]) * 0.5f;
+    }
+  std::vector<PwWave> waves(h->pwin.size());
+  std::vector<std::vector<float>> evols(h->pwin.size());
+  for (size_t q = 0; q < h->pwin.size(); ++q) {
+    const PlaneWaveIn &P = h->pwin[q];
+    const int *E = P.esq;
+    PwWave &w = waves[q];
+    std::memset(&w, 0, sizeof w);
+    w.nmodes = P.nmodes;
+    w.numus = (int)P.evol.size() - 1;
+    w.deltaevol = P.deltaevol;
+    w.cluz = 1.0f / std::sqrt(P.eps0 * P.mu0);
+    const float zvac = std::sqrt(P.mu0 / P.eps0);
+    for (int m = 0; m < P.nmodes; ++m) {
+      const float px = P.p[m][0], py = P.p[m][1], pz = P.p[m][2];
+      const float modulus = std::sqrt(px * px + py * py + std::pow(pz, 2.0f));
+      float *pn = w.pn[m];
+      pn[0] = px / modulus; pn[1] = py / modulus; pn[2] = pz / modulus;
+      w.fpw[0][m] = P.e[m][0]; w.fpw[1][m] = P.e[m][1]; w.fpw[2][m] = P.e[m][2];
+      w.fpw[3][m] = (pn[1] * P.e[m][2] - pn[2] * P.e[m][1]) / zvac;
+      w.fpw[4][m] = (pn[2] * P.e[m][0] - pn[0] * P.e[m][2]) / zvac;
+      w.fpw[5][m] = (pn[0] * P.e[m][1] - pn[1] * P.e[m][0]) / zvac;
+      // null-phase corner (:372-407) and box diagonal (:408-411)
+      const int hc[3] = {3, 4, 5};
+      float c0[3], sq[3];
+      for (int a = 0; a < 3; ++a) {
+        const int clo = std::max(E[2 * a] - 1, sin_lo(h, hc[a], a));
+        const int chi = std::min(E[2 * a + 1] + 1, sin_hi(h, hc[a], a));
+        const std::vector<float> &L = h->line[a];
+        c0[a] = pn[a] >= 0.0f ? L[clo + 1] : L[chi + 1];
+        sq[a] = std::pow(L[clo + 1] - L[chi + 1], 2.0f);
+      }
+      const float diag = std::sqrt(sq[0] + sq[1] + sq[2]);
+      w.dist0[m] = (c0[0] * pn[0] + c0[1] * pn[1] + c0[2] * pn[2]) - P.incert[m] * diag;
+    }
+    evols[q] = P.evol;
+    // illuminated faces (:203-214)
+    const bool il[6] = {E[0] >= sin_lo(h, 3, 0) && E[0] <= sin_hi(h, 3, 0),
+                        E[1] <= sin_hi(h, 3, 0) && E[1] >= sin_lo(h, 3, 0),
+                        E[2] >= sin_lo(h, 4, 1) && E[2] <= sin_hi(h, 4, 1),
+                        E[3] <= sin_hi(h, 4, 1) && E[3] >= sin_lo(h, 4, 1),
+                        E[4] >= sin_lo(h, 5, 2) && E[4] <= sin_hi(h, 5, 2),
+                        E[5] <= sin_hi(h, 5, 2) && E[5] >= sin_lo(h, 5, 2)};
+    // free-axis ranges of each E component (:216-295): along its own axis the
+    // range stops one short of the box end; H faces copy the paired E face
+    int rlo[6][3], rhi[6][3], flo[6][3], fhi[6][3];
+    for (int c = 0; c < 3; ++c)
+      for (int a = 0; a < 3; ++a) {
+        rlo[c][a] = std::max(sin_lo(h, c, a), E[2 * a]);
+        rhi[c][a] = std::min(sin_hi(h, c, a), E[2 * a + 1] - (a == c ? 1 : 0));
+        flo[c][a] = std::max(sin_lo(h, c, a), E[2 * a]);
+        fhi[c][a] = std::min(sin_hi(h, c, a), E[2 * a + 1]);
+      }
+    // H component -> (paired E component) per face axis: x faces Hy<-Ez,
+    // Hz<-Ey; y faces Hz<-Ex, Hx<-Ez; z faces Hx<-Ey, Hy<-Ex
+    auto face = [&](int half, int comp, int axis, bool high, int nf, int off, float sign) {
+      int ec = comp;
+      if (comp >= 3) {
+        static const int pair[3][6] = {{0, 0, 0, 0, 2, 1}, {0, 0, 0, 2, 0, 0}, {0, 0, 0, 1, 0, 0}};
+        ec = pair[axis][comp];
+      }
+      PwFace f{};
+      f.comp = comp; f.axis = axis; f.nf = nf; f.off = off; f.sign = sign;
+      f.wave = (int)q;
+      for (int a = 0; a < 3; ++a) { f.lo[a] = rlo[ec][a]; f.hi[a] = rhi[ec][a]; }
+      f.fixed = high ? fhi[ec][axis] : flo[ec][axis] - (comp >= 3 ? 1 : 0);
+      const float *I = h->idh_[half ? axis : 3 + axis].data();
+      f.id = I[f.fixed + 1];
+      h->pw_faces[half].push_back(f);
+    };
+    // AdvancePlaneWaveE (:933-1170)
+    if (il[0]) { face(0, 2, 0, false, 4, -1, -1.f); face(0, 1, 0, false, 5, -1, 1.f); }
+    if (il[1]) { face(0, 2, 0, true, 4, 0, 1.f); face(0, 1, 0, true, 5, 0, -1.f); }
+    if (il[2]) { face(0, 0, 1, false, 5, -1, -1.f); face(0, 2, 1, false, 3, -1, 1.f); }
+    if (il[3]) { face(0, 2, 1, true, 3, 0, -1.f); face(0, 0, 1, true, 5, 0, 1.f); }
+    if (il[4]) { face(0, 0, 2, false, 4, -1, 1.f); face(0, 1, 2, false, 3, -1, -1.f); }
+    if (il[5]) { face(0, 0, 2, true, 4, 0, -1.f); face(0, 1, 2, true, 3, 0, 1.f); }
+    // AdvancePlaneWaveH (:1213-1470)
+    if (il[0]) { face(1, 5, 0, false, 1, 1, 1.f); face(1, 4, 0, false, 2, 1, -1.f); }
+    if (il[1]) { face(1, 5, 0, true, 1, 0, -1.f); face(1, 4, 0, true, 2, 0, 1.f); }
+    if (il[2]) { face(1, 3, 1, false, 2, 1, 1.f); face(1, 5, 1, false, 0, 1, -1.f); }
+    if (il[3]) { face(1, 3, 1, true, 2, 0, -1.f); face(1, 5, 1, true, 0, 0, 1.f); }
+    if (il[4]) { face(1, 3, 2, false, 1, 1, -1.f); face(1, 4, 2, false, 0, 1, 1.f); }
+    if (il[5]) { face(1, 3, 2, true, 1, 0, 1.f); face(1, 4, 2, true, 0, 0, -1.f); }
+  }
+  int rc;
+  for (size_t q = 0; q < waves.size(); ++q) {
+    float *de = nullptr;
+    if ((rc = upload(h, &de, evols[q].data(), evols[q].size(), &h->pw_allocs))) return rc;
+    waves[q].evol = de;
+  }
+  if ((rc = upload(h, &h->dwaves, waves.data(), waves.size(), &h->pw_allocs))) return rc;
+  if ((rc = upload(h, &h->dphys, phys.data(), phys.size(), &h->pw_allocs))) return rc;
+  if ((rc = upload(h, &h->dtiempo, h->tiempo.data(), h->tiempo.size(), &h->pw_allocs))) return rc;
+  h->pstride = S;
+  h->pw_ready = true;
+  return FDTD_OK;
+}
+
 int prepare(fdtd_handle *h) {
   int rc = check_ready(h);
   if (rc) return rc;
   if ((rc = build_ade(h))) return rc;
   if ((rc = build_nodal(h))) return rc;
+  if ((rc = build_planewaves(h))) return rc;
   return FDTD_OK;
 }
 
@@ -400,6 +555,22 @@ int run_h(fdtd_handle *h, int yee, int cpml) {
   int e = h->mode == FDTD_MODE_EXACT ? exact::launch_h(h->g, h->tm, yee, c, h->stream)
                                      : fast::launch_h(h->g, h->tm, yee, c, h->stream);
   if (e) return fail(h, FDTD_ERR_HIP, "H pass launch failed: %s", hipGetErrorString((hipError_t)e));
+  return FDTD_OK;
+}
+
+// AdvancePlaneWaveE (half 0, after the pole stage) / AdvancePlaneWaveH
+// (half 1, after the CPML), one launch per face in the reference's order
+int run_pw(fdtd_handle *h, int half, int n, const int *dn) {
+  if (h->pw_faces[half].empty()) return FDTD_OK;
+  if (!dn && (n < 0 || n >= (int)h->tiempo.size()))
+    return fail(h, FDTD_ERR_ARG, "step %d outside the time vector (0..%d)", n,
+                (int)h->tiempo.size() - 1);
+  const float G = half ? h->gm2h[1] : h->g2h[1];
+  for (const PwFace &f : h->pw_faces[half]) {
+    int e = launch_tfsf(h->g, f, h->dwaves, h->dphys, h->pstride, h->dtiempo, n, dn, half, h->dt,
+                        G, h->stream);
+    if (e) return fail(h, FDTD_ERR_HIP, "plane-wave launch failed: %s", hipGetErrorString((hipError_t)e));
+  }
   return FDTD_OK;
 }
 
@@ -500,6 +671,7 @@ int fdtd_destroy(fdtd_handle *h) {
   invalidate_graph(h);
   for (void *p : h->ade_allocs) hipFree(p);
   for (void *p : h->nodal_allocs) hipFree(p);
+  for (void *p : h->pw_allocs) hipFree(p);
   for (void *p : h->allocs) hipFree(p);
   if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -528,6 +700,7 @@ int fdtd_set_media(fdtd_handle *h, int32_t nmedia, const float *g1, const float 
   h->g.nmedia = nmedia;
   h->ade_ready = false;
   h->nodal_ready = false;
+  h->pw_ready = false;
   invalidate_graph(h);
   return FDTD_OK;
 }
@@ -573,6 +746,7 @@ int fdtd_set_spacing(fdtd_handle *h, const float *idxe, const float *idye, const
     h->idh_[c] = t;
   }
   h->nodal_ready = false;
+  h->pw_ready = false;
   invalidate_graph(h);
   return FDTD_OK;
 }
@@ -629,6 +803,7 @@ int fdtd_set_cpml(fdtd_handle *h, const int32_t layers[6], const float *pbe_x,
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   g.has_cpml = any ? 1 : 0;
+  h->pw_ready = false;
   invalidate_graph(h);
   return FDTD_OK;
 }
@@ -664,6 +839,7 @@ int fdtd_set_time(fdtd_handle *h, double dt, int32_t nlast, const double *tiempo
   }
   h->ade_ready = false;  // FAST coefficients pre-divide by dt
   h->nodal_ready = false;
+  h->pw_ready = false;
   invalidate_graph(h);
   return FDTD_OK;
 }
@@ -690,11 +866,53 @@ int fdtd_add_nodal(fdtd_handle *h, int32_t comp, int32_t hard, const int32_t box
   return FDTD_OK;
 }
 
+int fdtd_set_lines(fdtd_handle *h, const float *line_x, const float *line_y,
+                   const float *line_z) {
+  if (!h || !line_x || !line_y || !line_z) return fail(h, FDTD_ERR_ARG, "null line coordinates");
+  const float *src[3] = {line_x, line_y, line_z};
+  for (int a = 0; a < 3; ++a) h->line[a].assign(src[a], src[a] + axis_n(h, a) + 3);
+  h->pw_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_add_planewave(fdtd_handle *h, const int32_t box[6], int32_t nmodes, const float *p,
+                       const float *e, const float *incert, float deltaevol, int32_t numus,
+                       const float *evol, float eps0, float mu0) {
+  if (!h || !box || nmodes < 1 || nmodes > kPwModes || !p || !e || !incert || numus < 1 ||
+      !evol || !(deltaevol > 0.0f) || !(eps0 > 0.0f) || !(mu0 > 0.0f))
+    return fail(h, FDTD_ERR_ARG, "bad plane-wave arguments");
+  const int n[3] = {h->g.nx, h->g.ny, h->g.nz};
+  for (int a = 0; a < 3; ++a)
+    if (box[2 * a] < 0 || box[2 * a + 1] > n[a] || box[2 * a] >= box[2 * a + 1])
+      return fail(h, FDTD_ERR_ARG, "plane-wave box outside the grid on axis %d", a);
+  for (int m = 0; m < nmodes; ++m)
+    if (p[3 * m] == 0.0f && p[3 * m + 1] == 0.0f && p[3 * m + 2] == 0.0f)
+      return fail(h, FDTD_ERR_ARG, "plane-wave mode %d has no direction", m);
+  PlaneWaveIn w{};
+  std::copy(box, box + 6, w.esq);
+  w.nmodes = nmodes;
+  for (int m = 0; m < nmodes; ++m) {
+    for (int a = 0; a < 3; ++a) { w.p[m][a] = p[3 * m + a]; w.e[m][a] = e[3 * m + a]; }
+    w.incert[m] = incert[m];
+  }
+  w.deltaevol = deltaevol;
+  w.evol.assign(evol, evol + numus + 1);
+  w.eps0 = eps0;
+  w.mu0 = mu0;
+  h->pwin.push_back(std::move(w));
+  h->pw_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
 int fdtd_advance_e(fdtd_handle *h, int32_t n) {
   if (!h) return FDTD_ERR_ARG;
   hipSetDevice(h->device);
   int rc;
-  if ((rc = prepare(h)) || (rc = run_e(h, 1, 1, 1)) || (rc = run_nodal(h, 0, n, nullptr))) return rc;
+  if ((rc = prepare(h)) || (rc = run_e(h, 1, 1, 1)) || (rc = run_pw(h, 0, n, nullptr)) ||
+      (rc = run_nodal(h, 0, n, nullptr)))
+    return rc;
   return FDTD_OK;
 }
 
@@ -702,7 +920,9 @@ int fdtd_advance_h(fdtd_handle *h, int32_t n) {
   if (!h) return FDTD_ERR_ARG;
   hipSetDevice(h->device);
   int rc;
-  if ((rc = prepare(h)) || (rc = run_h(h, 1, 1)) || (rc = run_nodal(h, 1, n, nullptr))) return rc;
+  if ((rc = prepare(h)) || (rc = run_h(h, 1, 1)) || (rc = run_pw(h, 1, n, nullptr)) ||
+      (rc = run_nodal(h, 1, n, nullptr)))
+    return rc;
   return FDTD_OK;
 }
 
@@ -717,9 +937,11 @@ int fdtd_advance_h(fdtd_handle *h, int32_t n) {
 SFD_GRANULAR(fdtd_advance_e_yee(fdtd_handle *h), run_e(h, 1, 0, 0))
 SFD_GRANULAR(fdtd_advance_e_cpml(fdtd_handle *h), run_e(h, 0, 1, 0))
 SFD_GRANULAR(fdtd_advance_e_dispersive(fdtd_handle *h), run_e(h, 0, 0, 1))
+SFD_GRANULAR(fdtd_advance_planewave_e(fdtd_handle *h, int32_t n), run_pw(h, 0, n, nullptr))
 SFD_GRANULAR(fdtd_advance_nodal_e(fdtd_handle *h, int32_t n), run_nodal(h, 0, n, nullptr))
 SFD_GRANULAR(fdtd_advance_h_yee(fdtd_handle *h), run_h(h, 1, 0))
 SFD_GRANULAR(fdtd_advance_h_cpml(fdtd_handle *h), run_h(h, 0, 1))
+SFD_GRANULAR(fdtd_advance_planewave_h(fdtd_handle *h, int32_t n), run_pw(h, 1, n, nullptr))
 SFD_GRANULAR(fdtd_advance_nodal_h(fdtd_handle *h, int32_t n), run_nodal(h, 1, n, nullptr))
 #undef SFD_GRANULAR
 
@@ -731,6 +953,8 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
   if (nsteps == 0) return FDTD_OK;
   if (!h->srcs.empty() && n0 + nsteps > h->nvalues_steps)
     return fail(h, FDTD_ERR_ARG, "steps %d..%d beyond the time vector", n0, n0 + nsteps - 1);
+  if (!h->pwin.empty() && (n0 < 0 || n0 + nsteps > (int)h->tiempo.size()))
+    return fail(h, FDTD_ERR_ARG, "steps %d..%d beyond the time vector", n0, n0 + nsteps - 1);
   // the step counter lives on the device so one captured step replays for
   // every n
   HIPCHK(h, hipMemcpyAsync(h->dstep, &n0, 4, hipMemcpyHostToDevice, h->stream));
@@ -738,8 +962,10 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
     invalidate_graph(h);
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int r2 = run_e(h, 1, 1, 1);
+    if (!r2) r2 = run_pw(h, 0, 0, h->dstep);
     if (!r2) r2 = run_nodal(h, 0, 0, h->dstep);
     if (!r2) r2 = run_h(h, 1, 1);
+    if (!r2) r2 = run_pw(h, 1, 0, h->dstep);
     if (!r2) r2 = run_nodal(h, 1, 0, h->dstep);
     if (!r2 && h->nprobe)
       r2 = launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap, h->stream)

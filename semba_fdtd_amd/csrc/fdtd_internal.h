@@ -81,6 +81,30 @@ struct NodalCell {   // one cell touched by a nodal source, in reference order
   float amp;
 };
 
+// TF/SF plane waves (planewaves.F90).  PwWave: the constants InitPlaneWave
+// derives for one wave; PwFace: one face of one component, as one loop of
+// AdvancePlaneWaveE/H applies it.
+constexpr int kPwModes = 4;
+
+struct PwWave {
+  float pn[kPwModes][3];      // normalised directions
+  float fpw[6][kPwModes];     // incident amplitudes of Ex..Hz per mode
+  float dist0[kPwModes];      // distanciaInicial
+  int nmodes;
+  int numus;
+  float deltaevol;
+  float cluz;                 // 1/sqrt(eps0 mu0) in real(4)
+  const float *evol;          // device, numus+1 samples
+};
+
+struct PwFace {
+  int comp, axis, fixed;      // updated component, face normal, face index
+  int lo[3], hi[3];           // ranges along the two free axes
+  int nf, off;                // incident component, its offset along axis
+  float sign, id;             // +1/-1 and the inverse spacing at `fixed`
+  int wave;
+};
+
 struct ProbeDev {
   long long p;
   int comp;
@@ -102,6 +126,12 @@ int launch_nodal(const DevGrid &g, const NodalCell *cells, int ncells,
                  const float *values, int nsrc, int n, const int *dn, int serial,
                  void *stream);
 int launch_tick(int *dn, void *stream);
+// one TF/SF face at step n (or *dn): F += sign * G * Incid(t) * id, t the real(4)
+// time stamp of the half step (E: tiempo(n), H: tiempo(n) + dt/2).  phys holds
+// the physical coordinates of each (component, axis) at stride pstride, index i+1.
+int launch_tfsf(const DevGrid &g, const PwFace &f, const PwWave *waves, const float *phys,
+                int pstride, const double *tiempo, int n, const int *dn, int half, double dt,
+                float G, void *stream);
 int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
                   int *count, int cap, void *stream);
 // bounding box {xmin,ymin,zmin,xmax,ymax,zmax} of the dispersive edges of

@@ -114,6 +114,21 @@ int fdtd_build_spacing(int32_t n, const float *D, float *de, float *dh, float *i
   return FDTD_OK;
 }
 
+int fdtd_build_lines(int32_t n, int32_t Llo, int32_t Lhi, const float *D, float *line) {
+  // discretisation lines, preprocess_geom.F90:5330-5347 (interior: origin 0,
+  // line(i+1) = d(i) + line(i)) and 5690-5712 (CPML and padding extended with
+  // the first / last interior cell); D covers [-2, n+1], line [-1, n+1]
+  if (n <= 0 || Llo < 0 || Lhi < 0 || Llo + Lhi >= n || !D || !line) return FDTD_ERR_ARG;
+  const int32_t lo = Llo, hi = n - Lhi;
+  line[lo + 1] = 0.0f;
+  for (int32_t i = lo; i < hi; ++i) line[i + 2] = D[i + 2] + line[i + 1];
+  float delta = line[lo + 2] - line[lo + 1];
+  for (int32_t i = lo - 1; i >= -1; --i) line[i + 1] = line[i + 2] - delta;
+  delta = line[hi + 1] - line[hi];
+  for (int32_t i = hi + 1; i <= n + 1; ++i) line[i + 1] = line[i] + delta;
+  return FDTD_OK;
+}
+
 int fdtd_build_cpml_axis(int32_t n, int32_t Llo, int32_t Lhi, float orden_lo,
                          float orden_hi, float R_lo, float R_hi, float alphamaxpar,
                          float alphaorden, float kappamaxpar, double dt, float eps0,

@@ -2,7 +2,7 @@
 !
 ! Keeps the call structure of semba-fdtd: the set-up of launch_simulation
 ! (timestepping.F90:305-1014: spacings, calc_G1G2Gm1Gm2, InitCPMLBorders,
-! InitEDispersives, crea_timevector, nodal sources, probes) done in Fortran
+! InitEDispersives, crea_timevector, nodal sources, plane waves, probes) done in Fortran
 ! through the library's table builders, then the loop of ciclo_temporal
 ! (timestepping.F90:1248-1981) with one library call per reference call site.
 !
@@ -56,6 +56,9 @@ program fdtd_gpu_driver
    real(c_double), allocatable :: tiempo(:)
    real(c_float), allocatable :: samples(:,:), chunk(:,:)
    integer(c_int32_t) :: nsamp, got, bufsteps, left, k, n0
+   integer(c_int32_t) :: npw, pwbox(6), nmodes
+   real(c_float), allocatable :: pwp(:), pwe(:), pwinc(:)
+   type(r4vec) :: lines(3)
    logical :: has_cpml
    type(fdtd_grid_desc) :: desc
    type(c_ptr) :: h
@@ -80,7 +83,7 @@ program fdtd_gpu_driver
          status='old', action='read', iostat=ios)
    if (ios /= 0) call die('cannot open '//trim(casefile))
    read (u) magic, version
-   if (magic /= 'SFDC' .or. version /= 1) call die('not a version-1 case file')
+   if (magic /= 'SFDC' .or. version /= 2) call die('not a version-2 case file')
    read (u) nx, ny, nz, nmedia, nsteps
    nn = [nx, ny, nz]
    read (u) eps0, mu0, dt
@@ -189,6 +192,27 @@ program fdtd_gpu_driver
    read (u) nprobe
    allocate (pc(max(nprobe, 1)), pi(max(nprobe, 1)), pj(max(nprobe, 1)), pk(max(nprobe, 1)))
    if (nprobe > 0) read (u) pc(1:nprobe), pi(1:nprobe), pj(1:nprobe), pk(1:nprobe)
+   ! TF/SF plane waves (InitPlaneWave, planewaves.F90:72) placed on the
+   ! discretisation lines of the preprocessing (preprocess_geom.F90:5672-5712)
+   read (u) npw
+   if (npw > 0) then
+      do ax = 1, 3
+         allocate (lines(ax)%v(-1:nn(ax)+1))
+         call check_host(fdtd_build_lines(nn(ax), layers(2*ax-1), layers(2*ax), dd(ax)%v, &
+                                          lines(ax)%v), 'fdtd_build_lines')
+      end do
+      call check(fdtd_set_lines(h, lines(1)%v, lines(2)%v, lines(3)%v), 'fdtd_set_lines')
+   end if
+   do s = 1, npw
+      read (u) pwbox, nmodes
+      allocate (pwp(3*nmodes), pwe(3*nmodes), pwinc(nmodes))
+      read (u) pwp, pwe, pwinc, deltaevol, numus
+      allocate (evol(0:numus))
+      read (u) evol
+      call check(fdtd_add_planewave(h, pwbox, nmodes, pwp, pwe, pwinc, deltaevol, numus, evol, &
+                                    eps0, mu0), 'fdtd_add_planewave')
+      deallocate (pwp, pwe, pwinc, evol)
+   end do
    close (u)
    bufsteps = max(1, min(nsteps, 4096))
    if (nprobe > 0) then
@@ -215,12 +239,16 @@ program fdtd_gpu_driver
             if (has_cpml) call check(fdtd_advance_e_cpml(h), 'fdtd_advance_e_cpml')
             ! AdvanceEDispersiveE (timestepping.F90:1416)
             if (nd > 0) call check(fdtd_advance_e_dispersive(h), 'fdtd_advance_e_dispersive')
+            ! AdvancePlaneWaveE (timestepping.F90:1425)
+            if (npw > 0) call check(fdtd_advance_planewave_e(h, n), 'fdtd_advance_planewave_e')
             ! AdvanceNodalE (timestepping.F90:1438-1446)
             call check(fdtd_advance_nodal_e(h, n), 'fdtd_advance_nodal_e')
             ! Advance_Hx/Hy/Hz (timestepping.F90:1497-1507)
             call check(fdtd_advance_h_yee(h), 'fdtd_advance_h_yee')
             ! AdvanceMagneticCPML (timestepping.F90:1535)
             if (has_cpml) call check(fdtd_advance_h_cpml(h), 'fdtd_advance_h_cpml')
+            ! AdvancePlaneWaveH (timestepping.F90:1598)
+            if (npw > 0) call check(fdtd_advance_planewave_h(h, n), 'fdtd_advance_planewave_h')
             ! AdvanceNodalH (timestepping.F90:1596-1635)
             call check(fdtd_advance_nodal_h(h, n), 'fdtd_advance_nodal_h')
          else

@@ -27,7 +27,9 @@ module sembafdtd_gpu
 
    public :: fdtd_create, fdtd_destroy, fdtd_set_media, fdtd_set_media_index, &
              fdtd_set_spacing, fdtd_set_cpml, fdtd_set_edispersive, fdtd_set_time, &
-             fdtd_add_nodal, fdtd_advance_e, fdtd_advance_h, fdtd_advance_e_yee, &
+             fdtd_set_lines, fdtd_add_planewave, fdtd_advance_planewave_e, &
+             fdtd_advance_planewave_h, fdtd_build_lines, fdtd_add_nodal, fdtd_advance_e, &
+             fdtd_advance_h, fdtd_advance_e_yee, &
              fdtd_advance_e_cpml, fdtd_advance_e_dispersive, fdtd_advance_nodal_e, &
              fdtd_advance_h_yee, fdtd_advance_h_cpml, fdtd_advance_nodal_h, fdtd_run, &
              fdtd_set_probes, fdtd_read_probes, fdtd_sample_probes, fdtd_get_field, &
@@ -99,6 +101,22 @@ module sembafdtd_gpu
          integer(c_int32_t), value :: nlast
          real(c_double), intent(in) :: tiempo(*)
       end function
+      ! sgg%LineX/Y/Z over [-1, n+1] (preprocess_geom.F90:5672-5712)
+      integer(c_int) function fdtd_set_lines(h, line_x, line_y, line_z) bind(C, name="fdtd_set_lines")
+         import :: c_int, c_ptr, c_float
+         type(c_ptr), value :: h
+         real(c_float), intent(in) :: line_x(*), line_y(*), line_z(*)
+      end function
+      ! sgg%PlaneWave(j) (InitPlaneWave, planewaves.F90:72-770)
+      integer(c_int) function fdtd_add_planewave(h, box, nmodes, p, e, incert, deltaevol, numus, &
+            evol, eps0, mu0) bind(C, name="fdtd_add_planewave")
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(in) :: box(6)
+         integer(c_int32_t), value :: nmodes, numus
+         real(c_float), intent(in) :: p(*), e(*), incert(*), evol(*)
+         real(c_float), value :: deltaevol, eps0, mu0
+      end function
       integer(c_int) function fdtd_add_nodal(h, comp, hard, box, amp, deltaevol, numus, evol) &
             bind(C, name="fdtd_add_nodal")
          import :: c_int, c_ptr, c_int32_t, c_float
@@ -130,6 +148,16 @@ module sembafdtd_gpu
       integer(c_int) function fdtd_advance_e_dispersive(h) bind(C, name="fdtd_advance_e_dispersive")
          import :: c_int, c_ptr
          type(c_ptr), value :: h
+      end function
+      integer(c_int) function fdtd_advance_planewave_e(h, n) bind(C, name="fdtd_advance_planewave_e")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: n
+      end function
+      integer(c_int) function fdtd_advance_planewave_h(h, n) bind(C, name="fdtd_advance_planewave_h")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: n
       end function
       integer(c_int) function fdtd_advance_nodal_e(h, n) bind(C, name="fdtd_advance_nodal_e")
          import :: c_int, c_ptr, c_int32_t
@@ -226,6 +254,12 @@ module sembafdtd_gpu
          real(c_float), intent(in) :: de(*), dh(*)
          real(c_float), intent(out) :: pbe(*), pce(*), pbm(*), pcm(*)
          real(c_float), intent(inout) :: ide(*), idh(*)
+      end function
+      integer(c_int) function fdtd_build_lines(n, llo, lhi, d, line) bind(C, name="fdtd_build_lines")
+         import :: c_int, c_int32_t, c_float
+         integer(c_int32_t), value :: n, llo, lhi
+         real(c_float), intent(in) :: d(*)
+         real(c_float), intent(out) :: line(*)
       end function
       integer(c_int) function fdtd_build_ade(npoles, c, a, eps11, sigma11, dt, kappa, beta, g3, &
             g1, g2) bind(C, name="fdtd_build_ade")

@@ -21,6 +21,7 @@ import numpy as np
 EPS0 = float(np.float32(8.8541878176203898505365630317107502606083701665994498081024171524053950954599821142852891607182008932e-12))
 MU0 = float(np.float32(1.2566370614359172953850573533118011536788677597500423283899778369231265625144835994512139301368468271e-6))
 PMAX = 8
+PW_MODES = 4
 KIND_PEC, KIND_PMC, KIND_EDISP = 1, 2, 4
 EX, EY, EZ, HX, HY, HZ = range(6)
 
@@ -37,6 +38,26 @@ class NodalSource:
     amp: float
     evol: np.ndarray
     deltaevol: float
+
+
+@dataclass
+class PlaneWave:
+    """A TF/SF plane-wave box (planeonde_t, fdetypes.F90:458-465; injected by
+    planewaves.F90).  ``box`` = (esqx1, esqx2, esqy1, esqy2, esqz1, esqz2) in
+    total cells; per mode a propagation direction ``p``, an E amplitude ``e``
+    and a start-time uncertainty ``incert`` (fraction of the box diagonal);
+    ``evol`` = excitation samples 0..numus spaced ``deltaevol`` seconds."""
+    box: Tuple[int, int, int, int, int, int]
+    p: List[Tuple[float, float, float]]
+    e: List[Tuple[float, float, float]]
+    evol: np.ndarray
+    deltaevol: float
+    incert: List[float] = None
+
+    def __post_init__(self):
+        if self.incert is None:
+            self.incert = [0.0] * len(self.p)
+        assert len(self.p) == len(self.e) == len(self.incert) <= PW_MODES
 
 
 @dataclass
@@ -65,6 +86,7 @@ class Problem:
     kappamax: float = 1.0
     nodal: List[NodalSource] = field(default_factory=list)
     probes: List[Tuple[int, int, int, int]] = field(default_factory=list)
+    planewaves: List[PlaneWave] = field(default_factory=list)
     nsteps_max: int = 0
     eps0: float = EPS0
     mu0: float = MU0
@@ -315,7 +337,8 @@ def heterogeneous_problem(n: int = 1024, L: int = 12, seed: int = 1003, nsteps: 
     dielectrics (eps_r U[1,12], sigma U[0,0.5] S/m), and a 2-pole
     complex-conjugate pole-residue medium (resonance U[1,10] GHz) filling
     ``disp_fraction`` of the volume as a slab inside the CPML.  Excited by a
-    soft Ex line source across the slab-free part of the grid.  The media
+    TF/SF Gaussian plane wave (+z incidence, Ex polarisation) on a box one
+    cell inside the CPML; the bodies stay clear of its faces.  The media
     matrices are described as a scene for the device builders
     (``host_media`` materialises them on the host instead)."""
     rng = np.random.default_rng(seed)
@@ -335,34 +358,34 @@ def heterogeneous_problem(n: int = 1024, L: int = 12, seed: int = 1003, nsteps: 
     eps11 = np.zeros(nm + 1, np.float32); eps11[mdisp] = np.float32(2.0 * EPS0)
     sig11 = np.zeros(nm + 1, np.float32)
     prio = np.array([1000, 0] + [20 + i for i in range(nd)] + [10], np.int64)
-    lo, hi = L, n - L
+    lo, hi = L, n - L                 # interior cells [lo, hi)
+    blo, bhi = lo + 1, hi - 1         # TF/SF box (cells)
     inner = hi - lo
-    thick = max(1, int(round(disp_fraction * n ** 3 / inner ** 2))) if disp_fraction > 0 else 0
+    thick = max(1, int(round(disp_fraction * n ** 3 / (inner - 4) ** 2))) if disp_fraction > 0 else 0
     zc = (lo + hi) // 2
     scene = [("fill", 1)]
     if thick:
-        z0 = max(lo, zc - thick // 2)
-        scene.append(("box", lo, hi, lo, hi, z0, min(hi, z0 + thick), mdisp))
+        z0 = max(blo + 2, zc - thick // 2)
+        scene.append(("box", blo + 2, bhi - 2, blo + 2, bhi - 2, z0, min(bhi - 2, z0 + thick),
+                      mdisp))
     for b in range(nspheres):
         r = max(2.0, float(rng.uniform(20.0, 120.0)) * s)
-        r = min(r, 0.45 * inner)
-        ctr = [float(rng.uniform(lo + r + 1, hi - r - 1)) for _ in range(3)]
+        r = min(r, 0.4 * inner)
+        ctr = [float(rng.uniform(blo + r + 3, bhi - r - 3)) for _ in range(3)]
         scene.append(("sphere", ctr[0], ctr[1], ctr[2], r, 2 + b % nd))
     D = [uniform_D(n, dx)] * 3
     dt = courant_dt((dx, dx, dx), cfl=0.9)
     evol = gaussian_table(dt, nsteps + 4, 30.0, 120.0)
-    ys = lo + max(1, inner // 8)
-    zs = lo + max(1, inner // 8)
-    src = NodalSource(EX, False, (lo + 1, hi - 2, ys, ys, zs, zs), 1.0, evol,
-                      float(np.float32(dt)))
+    pw = PlaneWave((blo, bhi, blo, bhi, blo, bhi), [(0.0, 0.0, 1.0)], [(1.0, 0.0, 0.0)], evol,
+                   float(np.float32(dt)))
     probes = []
     for p in range(nprobes):
         cc = p % 6
-        probes.append((cc,) + tuple(int(rng.integers(lo, hi)) for _ in range(3)))
+        probes.append((cc,) + tuple(int(rng.integers(blo + 1, bhi - 1)) for _ in range(3)))
     pb = Problem(n, n, n, epr=epr, sigma=sig, mur=[1.0] * (nm + 1), sigmam=[0.0] * (nm + 1),
                  kind=kind, mi=None, D=D, dt=dt, npoles=npoles, pole_c=pc, pole_a=pa,
                  eps11=eps11, sig11=sig11, cpml_layers=[[L, L]] * 3,
-                 cpml_order=[[3.0, 3.0]] * 3, cpml_refl=[[1e-6, 1e-6]] * 3, nodal=[src],
+                 cpml_order=[[3.0, 3.0]] * 3, cpml_refl=[[1e-6, 1e-6]] * 3, planewaves=[pw],
                  probes=probes, nsteps_max=nsteps, name=f"heterogeneous{n}", scene=scene,
                  prio=prio)
     if host_media:
@@ -423,3 +446,37 @@ def scatterer_problem(n=(24, 22, 20), L=((4, 5), (3, 4), (5, 3)), seed: int = 7,
                    cpml_layers=[list(l) for l in L], cpml_order=[[order, order]] * 3,
                    cpml_refl=[[refl, refl]] * 3, alphamax=alphamax, kappamax=kappamax,
                    nodal=srcs, probes=probes, nsteps_max=nsteps, name="scatterer")
+
+
+def open_scatterer_problem(n: int = 256, L: int = 10, radius: float = 40.0, grading: float = 1.03,
+                           nsteps: int = 4000, dx: float = 0.01, margin: int = 3, eps_r: float = 4.0,
+                           sigma: float = 0.01, nprobes: int = 16, seed: int = 1002,
+                           spread: float = 30.0, delay: float = 120.0,
+                           direction=(0.0, 0.0, 1.0), polar=(1.0, 0.0, 0.0)) -> Problem:
+    """Open scatterer: n^3 total cells with an L-cell CPML on every face
+    (grading m = 3, R0 = 1e-6, kappa_max = 1, alpha = 0), a centred dielectric
+    sphere (radius in cells, eps_r, sigma), the z mesh graded geometrically by
+    ``grading`` from the centre outwards, and a TF/SF Gaussian plane wave
+    (+z incidence, Ex polarisation by default) on a box ``margin`` cells inside
+    the CPML.  ``nprobes`` seeded probes on all six components."""
+    rng = np.random.default_rng(seed)
+    mi = new_media_index(n, n, n)
+    prio = np.array([100, 0, 10], np.int64)
+    ctr = [n / 2.0] * 3
+    paint_voxels(mi, sphere_cells((n, n, n), ctr, radius), 2, prio)
+    D = [uniform_D(n, dx), uniform_D(n, dx),
+         graded_D(n, dx, grading, (L, L)) if grading != 1.0 else uniform_D(n, dx)]
+    dt = courant_dt([float(d.min()) for d in D], cfl=0.9)
+    lo, hi = L + margin, n - L - margin
+    evol = gaussian_table(dt, nsteps + 4, spread, delay)
+    pw = PlaneWave((lo, hi, lo, hi, lo, hi), [tuple(direction)], [tuple(polar)], evol,
+                   float(np.float32(dt)))
+    probes = []
+    for p in range(nprobes):
+        c = p % 6
+        probes.append((c,) + tuple(int(rng.integers(L + 1, n - L - 1)) for _ in range(3)))
+    return Problem(n, n, n, epr=[1.0, 1.0, eps_r], sigma=[0.0, 0.0, sigma], mur=[1.0] * 3,
+                   sigmam=[0.0] * 3, kind=[KIND_PEC, 0, 0], mi=mi, D=D, dt=dt,
+                   cpml_layers=[[L, L]] * 3, cpml_order=[[3.0, 3.0]] * 3,
+                   cpml_refl=[[1e-6, 1e-6]] * 3, planewaves=[pw], probes=probes,
+                   nsteps_max=nsteps, name=f"open_scatterer{n}")

@@ -4,7 +4,7 @@ hot path, driving the gfx950 library through its C ABI.
 Set-up follows the reference order (timestepping.F90:305-1014): spacings,
 calc_G1G2Gm1Gm2, InitCPMLBorders (which stretches the inverse spacings),
 InitEDispersives (which overrides G1/G2 of the dispersive media), nodal
-sources, probes.  The step methods keep the reference's call structure:
+sources, plane waves, probes.  The step methods keep the reference's call structure:
 ``advance_e(n)`` / ``advance_h(n)`` are the E and H halves of one iteration of
 ``ciclo_temporal`` (timestepping.F90:1248-1981) and ``run`` replays whole steps
 from a captured HIP graph.
@@ -40,6 +40,7 @@ class Tables:
     kappa: np.ndarray
     beta: np.ndarray
     g3: np.ndarray
+    lines: List[np.ndarray] = None
 
 
 def _check(lib, h, rc, what):
@@ -91,9 +92,17 @@ def build_tables(pb: Problem) -> Tables:
             raise FdtdError(f"fdtd_build_ade failed for medium {m}")
         kappa[d], beta[d], g3[d] = kk, bb, gg
         g1[m], g2[m] = a1[0], a2[0]
+    lines = []
+    for a, n in enumerate((pb.nx, pb.ny, pb.nz)):
+        ln = z(n + 3)
+        rc = lib.fdtd_build_lines(n, int(pb.cpml_layers[a][0]), int(pb.cpml_layers[a][1]),
+                                  ptr(pb.D[a], f32p), ptr(ln, f32p))
+        if rc:
+            raise FdtdError(f"fdtd_build_lines failed on axis {a}")
+        lines.append(ln)
     return Tables(g1, g2, gm1, gm2, ide, idh, pbe, pce, pbm, pcm,
                   np.array(disp, np.int32), np.array([pb.npoles[m] for m in disp], np.int32),
-                  kappa, beta, g3)
+                  kappa, beta, g3, lines)
 
 
 class Solver:
@@ -150,6 +159,20 @@ class Solver:
             ev = np.ascontiguousarray(s.evol, np.float32)
             self._ck(lib.fdtd_add_nodal(h, s.comp, int(s.hard), ptr(box, i32p), s.amp,
                                         s.deltaevol, len(ev) - 1, ptr(ev, f32p)), "fdtd_add_nodal")
+        if pb.planewaves:
+            self._ck(lib.fdtd_set_lines(h, *[ptr(t.lines[a], f32p) for a in range(3)]),
+                     "fdtd_set_lines")
+        for w in pb.planewaves:
+            box = np.array(w.box, np.int32)
+            pp = np.ascontiguousarray(np.array(w.p, np.float32))
+            ee = np.ascontiguousarray(np.array(w.e, np.float32))
+            inc = np.array(w.incert, np.float32)
+            ev = np.ascontiguousarray(w.evol, np.float32)
+            self._keep += [box, pp, ee, inc, ev]
+            self._ck(lib.fdtd_add_planewave(h, ptr(box, i32p), len(w.p), ptr(pp, f32p),
+                                            ptr(ee, f32p), ptr(inc, f32p), w.deltaevol,
+                                            len(ev) - 1, ptr(ev, f32p), pb.eps0, pb.mu0),
+                     "fdtd_add_planewave")
         self.nprobe = len(pb.probes)
         self.probe_buffer = probe_buffer
         if self.nprobe:

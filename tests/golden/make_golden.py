@@ -32,6 +32,12 @@ CASES = {
                                                         alphamax=0.1, kappamax=3.0, order=2.0)),
     "one_sided_cpml": ("scatterer_problem", dict(nsteps=150, n=(40, 20, 22),
                                                  L=((5, 0), (0, 6), (4, 4)), refl=1e-3)),
+    "planewave_z_sphere": ("open_scatterer_problem", dict(n=32, L=6, radius=5, nsteps=250,
+                                                          spread=10, delay=40)),
+    "planewave_oblique_graded": ("open_scatterer_problem",
+                                 dict(n=30, L=5, radius=4, nsteps=200, spread=8, delay=30,
+                                      grading=1.05, direction=(0.3, -0.5, 0.8),
+                                      polar=(0.0, 0.848, 0.53))),
 }
 
 

@@ -103,6 +103,9 @@ def test_driver_rejects_bad_case(tmp_path):
 
 GPU_CASES = {
     "scatterer": lambda: (P.scatterer_problem(nsteps=200), 200),
+    "planewave_oblique_graded": lambda: (P.open_scatterer_problem(
+        n=30, L=5, radius=4, nsteps=150, spread=8, delay=30, grading=1.05,
+        direction=(0.3, -0.5, 0.8), polar=(0.0, 0.848, 0.53)), 150),
     "cavity64_hertzian_2000": lambda: (P.cavity_problem(n=64, nsteps=2000), 2000),
 }
 

@@ -37,6 +37,12 @@ CASES = {
     "one_sided_cpml": ("scatterer_problem", dict(nsteps=200, n=(70, 24, 22),
                                                  L=((5, 0), (0, 6), (4, 4)), refl=1e-3)),
     "wide_x": ("scatterer_problem", dict(nsteps=120, n=(131, 9, 11), L=((6, 7), (2, 2), (3, 2)))),
+    "planewave_z": ("open_scatterer_problem", dict(n=32, L=6, radius=5, nsteps=200, spread=10,
+                                                   delay=40)),
+    "planewave_oblique_graded": ("open_scatterer_problem",
+                                 dict(n=30, L=5, radius=4, nsteps=150, spread=8, delay=30,
+                                      grading=1.05, direction=(0.3, -0.5, 0.8),
+                                      polar=(0.0, 0.848, 0.53))),
 }
 
 
@@ -93,10 +99,15 @@ def test_cavity64_2000_steps_fast_and_exact():
     assert np.array_equal(exact.probes().view(np.uint32), want.view(np.uint32))
 
 
-def test_granular_entry_points_equal_fused():
-    """advance_e_yee + advance_e_cpml + advance_e_dispersive + nodal equals
-    the fused advance_e (and the same for H), bit for bit."""
+@pytest.mark.parametrize("planewave", [False, True])
+def test_granular_entry_points_equal_fused(planewave):
+    """advance_e_yee + advance_e_cpml + advance_e_dispersive + plane wave +
+    nodal equals the fused advance_e (and the same for H), bit for bit."""
     pb = P.scatterer_problem(nsteps=60)
+    if planewave:
+        pw = P.open_scatterer_problem(n=32, nsteps=60, spread=8, delay=20).planewaves[0]
+        pw.box = (6, 18, 5, 16, 7, 15)
+        pb.planewaves = [pw]
     a = sf.Solver(pb, mode="exact")
     b = sf.Solver(pb, mode="exact")
     L = b.lib
@@ -105,16 +116,20 @@ def test_granular_entry_points_equal_fused():
         a.advance_h(n)
         for fn in (L.fdtd_advance_e_yee, L.fdtd_advance_e_cpml, L.fdtd_advance_e_dispersive):
             assert fn(b.h) == 0
+        assert L.fdtd_advance_planewave_e(b.h, n) == 0
         assert L.fdtd_advance_nodal_e(b.h, n) == 0
         for fn in (L.fdtd_advance_h_yee, L.fdtd_advance_h_cpml):
             assert fn(b.h) == 0
+        assert L.fdtd_advance_planewave_h(b.h, n) == 0
         assert L.fdtd_advance_nodal_h(b.h, n) == 0
     for c in range(6):
         assert np.array_equal(a.field(c).view(np.uint32), b.field(c).view(np.uint32)), c
 
 
-def test_graph_replay_equals_stepping():
-    pb = P.scatterer_problem(nsteps=80)
+@pytest.mark.parametrize("builder", ["scatterer_problem", "open_scatterer_problem"])
+def test_graph_replay_equals_stepping(builder):
+    pb = (P.scatterer_problem(nsteps=80) if builder == "scatterer_problem" else
+          P.open_scatterer_problem(n=30, L=5, radius=4, nsteps=80, spread=8, delay=30))
     a = sf.Solver(pb, mode="fast")
     b = sf.Solver(pb, mode="fast")
     a.run(80, chunk=7)
@@ -163,27 +178,30 @@ def test_pec_walls_device_equals_host():
 @pytest.mark.parametrize("mode", ["fast", "exact"])
 def test_full_size_linearity(mode):
     """Full-size heterogeneous grid (1024^3, 12-cell CPML, 8 lossy spheres,
-    2-pole medium in 20% of the volume): every operation of the path is
-    linear in the fields, so doubling the source doubles every probe
-    sample exactly (scaling by 2 is exact in binary floating point)."""
+    2-pole medium in 20% of the volume, TF/SF plane wave): every operation of
+    the path is linear in the fields, so doubling the incident amplitude
+    doubles every probe sample exactly (scaling by 2 is exact in binary
+    floating point)."""
     print(f"full size linearity {mode}", flush=True)
-    ns = 40
+    ns = 60
     pb = P.heterogeneous_problem(n=1024, nsteps=ns)
-    _, (x0, x1, y, _, z, _) = 0, pb.nodal[0].box
-    xm = (x0 + x1) // 2
-    pb.probes = [(P.EX, xm, y, z), (P.EX, xm, y + 1, z), (P.EX, xm, y, z + 2),
-                 (P.HY, xm, y, z), (P.HZ, xm, y, z), (P.EY, xm, y, z), (P.EZ, xm, y, z)]
+    w = pb.planewaves[0]
+    z0 = w.box[4]
+    c = pb.nx // 2
+    pb.probes = [(P.EX, c, c, z0), (P.EX, c, c, z0 + 1), (P.EX, c + 7, c - 3, z0 + 2),
+                 (P.HY, c, c, z0), (P.HY, c, c, z0 - 1), (P.EX, c, c, z0 - 1),
+                 (P.EZ, c, c, z0)]
     a = sf.Solver(pb, mode=mode)
     a.run(ns)
     pa = a.probes()
     a.close()
-    pb.nodal[0].amp *= 2.0
+    w.e = [(2.0, 0.0, 0.0)]
     b = sf.Solver(pb, mode=mode)
     b.run(ns)
     pbb = b.probes()
     b.close()
     assert np.isfinite(pa).all()
-    assert np.abs(pa[:, :5]).min(axis=0).max() > 0 or np.abs(pa).max() > 1e-6
+    assert np.abs(pa[:, :4]).max(axis=0).min() > 0
     # exact where the samples are normal numbers; subnormal leading-edge
     # values may round differently after doubling
     big = np.abs(pa) > 1e-30

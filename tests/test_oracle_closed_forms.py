@@ -15,7 +15,11 @@ the oracle's Yee updates, CPML and pole-residue tables with closed forms:
   with J = c s/(s - a) E, so its discrete admittance is exactly the analytic
   one at the bilinear (Tustin) image of the frequency;
 * voxel rule: brute force over single voxels with the loop ranges of
-  CreateVolumeMM (healer.F90:135-231) equals the vectorised painter.
+  CreateVolumeMM (healer.F90:135-231) equals the vectorised painter;
+* TF/SF plane wave in vacuum: inside the box the total field is the incident
+  Gaussian delayed by z/c, outside it (the scattered-field region) the field
+  stays at the level of the grid's numerical dispersion;
+* discretisation lines: cumulative cell sizes from the first interior node.
 """
 import numpy as np
 import pytest
@@ -205,3 +209,61 @@ def test_spacing_tables_closed_form():
         assert dh[i + 1] == np.float32((np.float32(D[i + 2]) + np.float32(D[i + 1])) / np.float32(2))
         assert ie[i + 1] == np.float32(1) / de[i + 1]
         assert ih[i + 1] == np.float32(1) / dh[i + 1]
+
+
+def test_tfsf_plane_wave_in_vacuum():
+    # leakage into the scattered-field region is the grid's numerical
+    # dispersion against the analytic incident field: 2.7e-2, 2.6e-3, 5.8e-4
+    # of the peak for Gaussians of 10, 20, 30 steps
+    n, L, margin, steps, spread = 40, 6, 3, 300, 20.0
+    pb = P.open_scatterer_problem(n=n, L=L, radius=0.0, eps_r=1.0, sigma=0.0, grading=1.0,
+                                  nsteps=steps, margin=margin, spread=spread, delay=4 * spread)
+    o = oracle.Oracle(pb, precision=64)
+    lo, hi = L + margin, n - L - margin
+    c = n // 2
+    zs = list(range(lo + 2, hi - 1, 4))
+    rec = np.zeros((steps, len(zs)))
+    leak = 0.0
+    for s in range(steps):
+        o.step()
+        ex = o.fields[P.EX]
+        rec[s] = [ex[z + 1, c + 1, c + 1] for z in zs]
+        # scattered-field region: the slabs between the box and the CPML
+        sf = np.concatenate([ex[L + 2:lo - 0, :, :].ravel(), ex[hi + 2:n - L, :, :].ravel(),
+                             ex[:, L + 2:lo, :].ravel(), ex[:, hi + 2:n - L, :].ravel()])
+        leak = max(leak, float(np.abs(sf).max()))
+    peak = np.abs(rec).max()
+    assert 0.9 < peak < 1.1, peak
+    # the pulse passes each probe plane later by dz / c
+    dt = pb.dt
+    arrival = [int(np.argmax(rec[:, q])) for q in range(len(zs))]
+    cdt = 1.0 / np.sqrt(oracle.EPS0_64 * oracle.MU0_64) * dt / 0.01   # cells per step
+    for q in range(1, len(zs)):
+        expect = (zs[q] - zs[0]) / cdt
+        assert abs((arrival[q] - arrival[0]) - expect) <= 2.0, (arrival, expect)
+    # Gaussian shape at the first probe plane
+    t = np.arange(steps)
+    g = np.exp(-(((t - arrival[0]) / spread) ** 2))
+    assert np.abs(rec[:, 0] - g).max() < 0.03
+    assert leak < 0.01 * peak, leak
+
+
+def test_lines_closed_form():
+    lib = _lib.load()
+    n, Llo, Lhi = 30, 4, 6
+    D = P.graded_D(n, 0.01, 1.05, (Llo, Lhi))
+    ln = np.zeros(n + 3, np.float32)
+    assert lib.fdtd_build_lines(n, Llo, Lhi, _lib.ptr(D, _lib.f32p), _lib.ptr(ln, _lib.f32p)) == 0
+    want = np.zeros(n + 3)
+    want[Llo + 1:n - Lhi + 2] = np.concatenate([[0.0], np.cumsum(D[Llo + 2:n - Lhi + 2])])
+    d0 = D[Llo + 2]
+    d1 = D[n - Lhi + 1]
+    for i in range(-1, Llo):
+        want[i + 1] = -(Llo - i) * d0
+    for i in range(n - Lhi + 1, n + 2):
+        want[i + 1] = want[n - Lhi + 1] + (i - (n - Lhi)) * d1
+    assert np.allclose(ln, want, rtol=1e-6, atol=1e-8)
+    o = oracle.Oracle(P.open_scatterer_problem(n=n, L=5, radius=3, nsteps=4))
+    t = __import__("semba_fdtd_amd").build_tables(o.pb)
+    for a in range(3):
+        assert np.array_equal(o.lines[a].view(np.uint32), t.lines[a].view(np.uint32))

@@ -1,9 +1,9 @@
 """The oracle against the reference's own Fortran modules (oracle/_ref).
 
 The reference harness compiles calc_constants.F90, borderscpml.F90,
-electricdispersive.F90 and nodalsources.F90 from their sources and calls
-them in ciclo_temporal's order (timestepping.F90:1248-1981); the Yee loops in
-it restate timestepping.F90:2165-2409.  The C oracle must reproduce its probe
+electricdispersive.F90, nodalsources.F90 and planewaves.F90 from their
+sources and calls them in ciclo_temporal's order (timestepping.F90:
+1248-1981); the Yee loops in it restate timestepping.F90:2165-2409.  The C oracle must reproduce its probe
 series and final fields bit for bit (both built without FMA contraction).
 Skipped where the reference tree is absent; tests/golden holds the same
 series as committed fixtures for that case (test_golden.py).
@@ -27,6 +27,16 @@ CASES = {
     "one_sided_cpml": ("scatterer_problem", dict(nsteps=150, n=(40, 20, 22),
                                                  L=((5, 0), (0, 6), (4, 4)), refl=1e-3)),
     "hard_sources": ("scatterer_problem", dict(nsteps=150, soft=False)),
+    "planewave_z": ("open_scatterer_problem", dict(n=32, L=6, radius=5, nsteps=200, spread=10,
+                                                   delay=40)),
+    "planewave_oblique_graded": ("open_scatterer_problem",
+                                 dict(n=30, L=5, radius=4, nsteps=150, spread=8, delay=30,
+                                      grading=1.05, direction=(0.3, -0.5, 0.8),
+                                      polar=(0.0, 0.848, 0.53))),
+    "planewave_backward": ("open_scatterer_problem",
+                           dict(n=28, L=4, radius=3, nsteps=150, spread=8, delay=30,
+                                direction=(-0.6, 0.0, -0.8), polar=(0.8, 0.0, -0.6),
+                                margin=2)),
 }
 
 
