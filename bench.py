@@ -9,6 +9,14 @@ graph.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--n N] [--mode fast|exact]
                     [--kernel-reps R] [--no-cpu] [--cpu-n N] [--cpu-steps K]
+                    [--no-batch] [--batch-instances N] [--batch-steps K]
+                    [--no-serve] [--serve-requests N] [--serve-rate R]
+
+Besides the 1024^3 line it measures the batched-instances workload (256
+uneven grids, one persistent launch per half step, against the plain
+one-at-a-time path) and a serving stream (continuous batching against the
+plain path: p50/p99 latency and aggregate Mcell-updates/s), reported in the
+"batch" and "serve" objects of the same JSON line.
 """
 import argparse
 import json
@@ -92,6 +100,75 @@ def cpu_baseline(n_sample, steps):
                       f"{dt:.1f} s"}
 
 
+def batch_leg(count, steps, plain_sample, mode):
+    """Batched uneven instances (log-uniform 32-128 cells per axis, 8-cell
+    CPML, 1-6 blobs, plane wave): all instances advanced together, against a
+    sample of them solved one at a time on their own graphs."""
+    import torch
+    import semba_fdtd_amd as sf
+    from semba_fdtd_amd import problem as P
+    from semba_fdtd_amd.batch import Batch
+    insts = P.batch_instances(count=count, nsteps=steps)
+    cells = sum(p.cells for p in insts)
+    print(f"[bench] batch: {count} instances, {cells / 1e6:.1f} M cells, {steps} steps",
+          file=sys.stderr, flush=True)
+    sol = [sf.Solver(p, mode=mode, probe_buffer=steps) for p in insts]
+    b = Batch()
+    for s in sol:
+        b.admit(s)
+    b.step(2)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps - 2)
+    b.synchronize()
+    tb = time.perf_counter() - t0
+    batched = cells * (steps - 2) / tb / 1e6
+    finite = all(bool(np.isfinite(s.probes()).all()) for s in sol)
+    b.close()
+    for s in sol:
+        s.close()
+    del sol
+    # plain path on a sample: each instance alone, one after the other
+    sample = insts[:plain_sample]
+    scells = sum(p.cells for p in sample)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in sample:
+        s = sf.Solver(p, mode=mode, probe_buffer=steps)
+        s.run(steps)
+        s.synchronize()
+        s.close()
+    tp = time.perf_counter() - t0
+    plain = scells * steps / tp / 1e6
+    return {"instances": count, "cells": cells, "steps": steps, "mode": mode,
+            "batched_mcell_updates_per_s": round(batched, 1),
+            "plain_mcell_updates_per_s": round(plain, 1),
+            "plain_sample_instances": len(sample),
+            "speedup": round(batched / plain, 2), "probes_finite": finite,
+            "note": "batched: one persistent face-pass launch per half step for all instances, "
+                    "graph-replayed; plain: each instance alone on its own graph (set-up "
+                    "included), throughput measured on the first instances"}
+
+
+def serve_leg(count, rate, mode):
+    """Serving stream (Poisson arrivals, cubic 48-192, 500-4000 steps, 30%
+    with a 1-pole region, 64-frequency FP64 DFT): continuous batching against
+    the plain one-solve-at-a-time path on the same stream."""
+    from semba_fdtd_amd import problem as P
+    from semba_fdtd_amd import serve
+    reqs = P.serving_requests(count=count, rate=rate)
+    print(f"[bench] serve: {count} requests at {rate}/s", file=sys.stderr, flush=True)
+    plain = serve.serve_plain(reqs, mode=mode)
+    batched = serve.serve_batched(reqs, mode=mode)
+    same = True
+    pm = {r.name: r for r in plain}
+    for r in batched:
+        same = same and np.array_equal(r.probes.view(np.uint32), pm[r.name].probes.view(np.uint32))
+    return {"requests": count, "arrival_rate_per_s": rate, "mode": mode,
+            "batched": serve.summarize(batched), "plain": serve.summarize(plain),
+            "results_identical": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +184,13 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 1 GiB device copies (the known-size kernel that "
                          "scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=1500)
+    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only run semba_fdtd_amd/hbm_probe (achievable HBM bandwidth per access "
                          "shape) and exit")
@@ -201,6 +285,13 @@ def main():
         if t.get("workload") == pb.name and t.get("mode") == args.mode:
             traffic = t.get("bytes_per_launch", {}).get(dom)
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(args.batch_instances, args.batch_steps, args.plain_sample, args.mode)
+    served = None
+    if not args.no_serve:
+        served = serve_leg(args.serve_requests, args.serve_rate, args.mode)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -243,6 +334,8 @@ def main():
             "kernel": f"{KERNELS[dom]}, {nbytes[dom]} algorithmic B per launch",
         },
         "cpu_baseline": cpu,
+        "batch": batch,
+        "serve": served,
     }
     print(json.dumps(out), flush=True)
 

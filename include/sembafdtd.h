@@ -140,6 +140,13 @@ int fdtd_add_planewave(fdtd_handle *h, const int32_t box[6], int32_t nmodes, con
                        const float *e, const float *incert, float deltaevol, int32_t numus,
                        const float *evol, float eps0, float mu0);
 
+/* Work decomposition of the face passes: blocks of 64 x rows threads (rows =
+ * 1, 2 or 4) that march zchunk z planes each.  No counterpart in the
+ * reference (its loops are OpenMP over k); chosen by the autotuner
+ * (semba_fdtd_amd.tuning) or left at the default 4 rows x 32 planes. */
+int fdtd_set_tiling(fdtd_handle *h, int32_t rows, int32_t zchunk);
+int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk);
+
 /* ---- the hot path ----------------------------------------------------- */
 /* Fused E half step: Advance_Ex/Ey/Ez (timestepping.F90:1286-1297) +
  * AdvanceelectricCPML (:1381) in one HBM pass, AdvanceEDispersiveE (:1416)
@@ -161,9 +168,9 @@ int fdtd_advance_h_yee(fdtd_handle *h);        /* Advance_Hx/Hy/Hz */
 int fdtd_advance_h_cpml(fdtd_handle *h);       /* AdvanceMagneticCPML */
 int fdtd_advance_planewave_h(fdtd_handle *h, int32_t n); /* AdvancePlaneWaveH */
 int fdtd_advance_nodal_h(fdtd_handle *h, int32_t n); /* AdvanceNodalH */
-/* nsteps full steps n0 .. n0+nsteps-1 (E half then H half, probes sampled
- * after each step as UpdateObservation does, timestepping.F90:1725-1742),
- * replayed from a captured HIP graph. */
+/* nsteps full steps n0 .. n0+nsteps-1 (E half then H half, probes sampled and
+ * frequency-domain sums advanced after each step as UpdateObservation does,
+ * timestepping.F90:1725-1742), replayed from a captured HIP graph. */
 int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps);
 
 /* ---- observation ------------------------------------------------------ */
@@ -178,11 +185,34 @@ int fdtd_read_probes(fdtd_handle *h, float *out, int32_t max_samples,
                      int32_t *nsamples);
 /* Sample probes once now (used by drivers that step with fdtd_advance_*). */
 int fdtd_sample_probes(fdtd_handle *h);
+/* Frequency-domain observation box (freq-domain probes, observation.F90:
+ * 3527-3600): kind 0 sums Ex,Ey,Ez, kind 1 Hx,Hy,Hz at the points of box
+ * {i1,i2,j1,j2,k1,k2} whose coordinates are multiples of stride[3], for nfreq
+ * real(4) frequencies [Hz]: complex(8) running sums of dt*F(t_n)*exp(-j2 pi f
+ * t_n) (H at t_n + dt/2).  *id names the box. */
+int fdtd_add_dft(fdtd_handle *h, int32_t kind, const int32_t box[6], const int32_t stride[3],
+                 int32_t nfreq, const float *freq, int32_t *id);
+/* Complex values held by box id (points x 3 x nfreq), -1 for a bad id. */
+int64_t fdtd_dft_size(const fdtd_handle *h, int32_t id);
+/* The running sums of box id as [k'][j'][i'][comp][freq][re,im] doubles
+ * (valor3DComplex's order); capacity counts complex values. */
+int fdtd_read_dft(fdtd_handle *h, int32_t id, double *out, int64_t capacity);
+/* UpdateObservation at step n (timestepping.F90:1727): sample the point probes
+ * and add step n to the frequency-domain sums.  fdtd_run does this itself. */
+int fdtd_update_observation(fdtd_handle *h, int32_t n);
 
 /* ---- field access ----------------------------------------------------- */
 int fdtd_get_field(fdtd_handle *h, int32_t comp, float *host_alloc);
 int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host_alloc);
 int fdtd_synchronize(fdtd_handle *h);
+/* Checkpoint / resume (StoreFields and the modules' restart files,
+ * resuming.F90:394-496): the whole time-stepping state -- fields, CPML psi,
+ * pole state, frequency-domain sums -- with the step n to resume at, as one
+ * opaque byte string of fdtd_state_size bytes.  A state loads into a grid with
+ * the same set-up and mode; *n receives the saved step. */
+int64_t fdtd_state_size(fdtd_handle *h);
+int fdtd_save_state(fdtd_handle *h, int32_t n, void *buf, int64_t size);
+int fdtd_load_state(fdtd_handle *h, const void *buf, int64_t size, int32_t *n);
 /* Device bytes held by the handle. */
 int64_t fdtd_device_bytes(const fdtd_handle *h);
 /* Number of dispersive E edges of component comp (0..2) holding pole state
@@ -221,6 +251,31 @@ int fdtd_build_ade(int32_t npoles, const double *c, const double *a, float eps11
                    float *g1, float *g2);
 /* evolucion, nodalsources.F90:317-356. */
 float fdtd_evolucion(float t, float deltaevol, int32_t numus, const float *evol);
+
+/* ---- batches of independent grids ---------------------------------------
+ * No counterpart in the reference (one solve per process there).  A batch
+ * advances many grids, each set up as above, together: one persistent launch
+ * per half step covers the face passes of every grid, the rest of the step
+ * is grouped across grids, and the step is replayed from a HIP graph.  Each
+ * grid keeps its own step counter, so grids join (admit) and leave (retire)
+ * between steps: continuous batching.  Results equal the grid advanced
+ * alone. */
+typedef struct fdtd_batch fdtd_batch;
+int fdtd_batch_create(int32_t device, void *stream, fdtd_batch **out);
+int fdtd_batch_destroy(fdtd_batch *b);
+const char *fdtd_batch_last_error(const fdtd_batch *b);
+/* Admit grid h at its step n0; *slot names it in the batch.  While admitted
+ * its own calls (reads, set-up) run on the batch's stream. */
+int fdtd_batch_admit(fdtd_batch *b, fdtd_handle *h, int32_t n0, int32_t *slot);
+int fdtd_batch_retire(fdtd_batch *b, int32_t slot);
+/* Step the grid of slot will run next (-1 for an empty slot). */
+int64_t fdtd_batch_step_of(const fdtd_batch *b, int32_t slot);
+/* Tiling of the batched face passes (rows 1, 2 or 4; zchunk planes) and the
+ * persistent launch's block count (0: 2048). */
+int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t blocks);
+/* nsteps steps of every admitted grid. */
+int fdtd_batch_step(fdtd_batch *b, int32_t nsteps);
+int fdtd_batch_synchronize(fdtd_batch *b);
 
 /* ---- device-side scene building for grids too large to build on the host.
  * Same rules as the reference preprocessing, applied in HBM. */

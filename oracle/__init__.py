@@ -120,6 +120,9 @@ def load_oracle(precision: int = 32):
     lib.orc_pw_init.restype = C.c_int
     lib.orc_pw_free.argtypes = [C.POINTER(Grid)]
     lib.orc_lines.argtypes = [C.c_int32, C.c_int32, C.c_int32, rp, rp]
+    lib.orc_dft_aux.argtypes = [C.c_int32, _f32p, C.c_double, C.c_double, C.c_int32, _f64p]
+    lib.orc_dft_accum.argtypes = [C.POINTER(Grid), C.c_int32, _i32p, _i32p, C.c_int32, _f64p,
+                                  _f64p]
     lib.orc_run.argtypes = [C.POINTER(Grid), C.c_int32, C.c_int32, C.c_int32,
                             _i32p, _i32p, _i32p, _i32p, rp]
     _LIBS[precision] = lib
@@ -305,10 +308,27 @@ class Oracle:
         self.g = g
         if npw and self.lib.orc_pw_init(C.byref(g)) != 0:
             raise MemoryError("orc_pw_init failed")
+        self.dft = [np.zeros(d.shape + (2,)) for d in pb.dfts]
         self.n = 0
+
+    def _accumulate_dft(self, n):
+        for d, acc in zip(self.pb.dfts, self.dft):
+            aux = np.zeros((len(d.freqs), 2))
+            self.lib.orc_dft_aux(len(d.freqs), _p(d.freqs, _f32p), self.pb.dt, self.tiempo[n],
+                                 int(d.kind), _p(aux, _f64p))
+            box = np.array(d.box, np.int32)
+            st = np.array(d.stride, np.int32)
+            self.lib.orc_dft_accum(C.byref(self.g), int(d.kind), _p(box, _i32p), _p(st, _i32p),
+                                   len(d.freqs), _p(aux, _f64p), _p(acc, _f64p))
+
+    def dft_sums(self):
+        """Running sums of each DftBox as complex128 arrays of DftBox.shape."""
+        return [a[..., 0] + 1j * a[..., 1] for a in self.dft]
 
     def step(self):
         self.lib.orc_step(C.byref(self.g), self.n)
+        if self.pb.dfts:
+            self._accumulate_dft(self.n)
         self.n += 1
 
     def __del__(self):
@@ -320,6 +340,13 @@ class Oracle:
 
     def run(self, nsteps, probes=None):
         probes = self.pb.probes if probes is None else probes
+        if self.pb.dfts:
+            # UpdateObservation after every step: probes, then the DFT sums
+            out = [self._run(1, probes) for _ in range(nsteps)]
+            return np.concatenate(out, axis=0) if out else self._run(0, probes)
+        return self._run(nsteps, probes)
+
+    def _run(self, nsteps, probes):
         npb = len(probes)
         pc = np.array([p[0] for p in probes] or [0], np.int32)
         pi = np.array([p[1] for p in probes] or [0], np.int32)
@@ -329,6 +356,9 @@ class Oracle:
         self.lib.orc_run(C.byref(self.g), self.n, nsteps, npb, _p(pc, _i32p), _p(pi, _i32p),
                          _p(pj, _i32p), _p(pk, _i32p),
                          _p(out, C.POINTER(_REAL[self.precision][0])))
+        for s in range(nsteps):
+            if self.pb.dfts:
+                self._accumulate_dft(self.n + s)
         self.n += nsteps
         return out[:, :npb]
 

@@ -821,6 +821,68 @@ void orc_pw_h(orc_grid *g, int32_t n) {
   }
 }
 
+/* --------------------------------------------------------------------------
+ * Frequency-domain probes, observation.F90:3527-3600.  mcPI2 is formed in
+ * real(4) and stored in complex(8) (fdetypes.F90:187-190): (-0, -2*pi_4).
+ * exp of a complex with a (signed) zero real part is (cos y, sin y); the
+ * products of a complex(8) by a real factor reduce to scaling both parts.
+ * ------------------------------------------------------------------------ */
+void orc_dft_aux(int32_t nfreq, const float *freq, double dt, double at, int32_t half,
+                 double *aux) {
+  const float pi4 = 3.14159265358979323846f;
+  const double mc = -(double)(2.0f * pi4);          /* imaginary part of mcPI2 */
+  for (int f = 0; f < nfreq; ++f) {
+    const double F = (double)freq[f];
+    const double y = (mc * F) * at;
+    double re = dt * cos(y), im = dt * sin(y);       /* (dt, 0) * exp(j y) */
+    if (half) {
+      const double y2 = ((mc * F) * dt) * 0.5;
+      const double c = cos(y2), s = sin(y2);
+      const double r2 = re * c - im * s, i2 = re * s + im * c;
+      re = r2;
+      im = i2;
+    }
+    aux[2 * f] = re;
+    aux[2 * f + 1] = im;
+  }
+}
+
+static int dft_count(int lo, int hi, int st) {
+  int c = 0;
+  for (int v = lo; v <= hi; ++v) c += (v % st == 0);
+  return c;
+}
+
+void orc_dft_accum(const orc_grid *g, int32_t kind, const int32_t box[6],
+                   const int32_t stride[3], int32_t nfreq, const double *aux, double *acc) {
+  const int ni = dft_count(box[0], box[1], stride[0]);
+  const int nj = dft_count(box[2], box[3], stride[1]);
+  int kt = 0;
+  for (int k = box[4]; k <= box[5]; ++k) {
+    if (k % stride[2]) continue;
+    int jt = 0;
+    for (int j = box[2]; j <= box[3]; ++j) {
+      if (j % stride[1]) continue;
+      int it = 0;
+      for (int i = box[0]; i <= box[1]; ++i) {
+        if (i % stride[0]) continue;
+        const size_t pt = ((size_t)kt * nj + jt) * ni + it;
+        for (int c = 0; c < 3; ++c) {
+          const double v = (double)g->f[3 * kind + c][IDX(g, i, j, k)];
+          double *a = acc + (pt * 3 + c) * (size_t)nfreq * 2;
+          for (int f = 0; f < nfreq; ++f) {
+            a[2 * f] = a[2 * f] + aux[2 * f] * v;
+            a[2 * f + 1] = a[2 * f + 1] + aux[2 * f + 1] * v;
+          }
+        }
+        ++it;
+      }
+      ++jt;
+    }
+    ++kt;
+  }
+}
+
 /* One full step in the order of ciclo_temporal, timestepping.F90:1248-1981:
  * E Yee -> E CPML -> E ADE -> plane wave E -> nodal E -> H Yee -> H CPML ->
  * plane wave H -> nodal H (:1286-1441, :1497-1615). */

@@ -144,6 +144,18 @@ void orc_pw_h(orc_grid *g, int32_t n);
  * cell sizes, CPML and padding extrapolated with the edge interior cells;
  * DX over [-2, n+1], line over [-1, n+1]. */
 void orc_lines(int32_t n, int32_t Llo, int32_t Lhi, const orc_real *DX, orc_real *line);
+/* Frequency-domain observation (observation.F90:3527-3600, 3532-3533):
+ * orc_dft_aux fills aux[f] = dt * exp(mcPI2 * freq[f] * at) for E (half 0,
+ * at = tiempo(n)) or that times exp(mcPI2 * freq[f] * dt * 0.5) for H (half 1),
+ * with mcPI2 = -j*2*pi formed in real(4) (fdetypes.F90:190); aux is
+ * [nfreq][re,im].  orc_dft_accum adds aux * F for the three components of
+ * kind (0: Ex,Ey,Ez; 1: Hx,Hy,Hz) at the points i1..i2 step si (and j, k) whose
+ * coordinates are multiples of the stride, into acc
+ * [k'][j'][i'][comp][freq][re,im] (valor3DComplex's Fortran order). */
+void orc_dft_aux(int32_t nfreq, const float *freq, double dt, double at, int32_t half,
+                 double *aux);
+void orc_dft_accum(const orc_grid *g, int32_t kind, const int32_t box[6],
+                   const int32_t stride[3], int32_t nfreq, const double *aux, double *acc);
 /* one full step n (E half, then H half) in the reference order */
 void orc_step(orc_grid *g, int32_t n);
 /* run steps n0..n0+nsteps-1, sampling probes after each step:

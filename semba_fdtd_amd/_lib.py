@@ -50,6 +50,24 @@ SIGNATURES = {
     "fdtd_advance_planewave_e": (C.c_int, [vp, C.c_int32]),
     "fdtd_advance_planewave_h": (C.c_int, [vp, C.c_int32]),
     "fdtd_set_lines": (C.c_int, [vp, f32p, f32p, f32p]),
+    "fdtd_set_tiling": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "fdtd_state_size": (C.c_int64, [vp]),
+    "fdtd_save_state": (C.c_int, [vp, C.c_int32, vp, C.c_int64]),
+    "fdtd_load_state": (C.c_int, [vp, vp, C.c_int64, i32p]),
+    "fdtd_batch_create": (C.c_int, [C.c_int32, vp, C.POINTER(vp)]),
+    "fdtd_batch_destroy": (C.c_int, [vp]),
+    "fdtd_batch_last_error": (C.c_char_p, [vp]),
+    "fdtd_batch_admit": (C.c_int, [vp, vp, C.c_int32, i32p]),
+    "fdtd_batch_retire": (C.c_int, [vp, C.c_int32]),
+    "fdtd_batch_step_of": (C.c_int64, [vp, C.c_int32]),
+    "fdtd_batch_set_tiling": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+    "fdtd_batch_step": (C.c_int, [vp, C.c_int32]),
+    "fdtd_batch_synchronize": (C.c_int, [vp]),
+    "fdtd_add_dft": (C.c_int, [vp, C.c_int32, i32p, i32p, C.c_int32, f32p, i32p]),
+    "fdtd_dft_size": (C.c_int64, [vp, C.c_int32]),
+    "fdtd_read_dft": (C.c_int, [vp, C.c_int32, f64p, C.c_int64]),
+    "fdtd_update_observation": (C.c_int, [vp, C.c_int32]),
+    "fdtd_get_tiling": (C.c_int, [vp, i32p, i32p]),
     "fdtd_add_planewave": (C.c_int, [vp, i32p, C.c_int32, f32p, f32p, f32p, C.c_float,
                                      C.c_int32, f32p, C.c_float, C.c_float]),
     "fdtd_build_lines": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, f32p, f32p]),

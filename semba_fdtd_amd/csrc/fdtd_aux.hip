@@ -58,51 +58,98 @@ __device__ __forceinline__ float pw_evol(const PwWave &w, float t, float d) {
   return ev;
 }
 
-// one face of one component of AdvancePlaneWaveE/H (planewaves.F90:878-1475):
-// Incid (:778-853) at the neighbouring incident component, then
-// F = F +- G * Incid * Id.  Threads tile the two free axes, x first.
-__global__ void __launch_bounds__(256) k_tfsf(DevGrid g, PwFace f, const PwWave *waves,
-                                              const float *phys, int pstride,
-                                              const double *tiempo, int n, const int *dn,
-                                              int half, double dt, float G) {
+// A group of TF/SF faces of AdvancePlaneWaveE/H (planewaves.F90:878-1475),
+// possibly of several grids, in one launch: thread t takes cell t of the
+// concatenated faces (the face is found by binary search in the prefix of face
+// areas), evaluates Incid (:778-853) at the neighbouring incident component and
+// applies F = F +- G * Incid * Id.
+__global__ void __launch_bounds__(256) k_tfsf_list(const PwRun *runs, const long long *start,
+                                                   int nruns, long long total, int n) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = nruns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  const PwRun &r = runs[lo];
+  const PwFace &f = r.f;
+  const long long c = t - start[lo];
   const int ua = f.axis == 0 ? 1 : 0;
   const int va = f.axis == 2 ? 1 : 2;
-  const int u = f.lo[ua] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int v = f.lo[va] + blockIdx.y * blockDim.y + threadIdx.y;
-  if (u > f.hi[ua] || v > f.hi[va]) return;
-  int c[3];
-  c[f.axis] = f.fixed;
-  c[ua] = u;
-  c[va] = v;
-  if (dn) n = *dn;
-  const float t = half ? (float)(tiempo[n] + (double)0.5f * dt) : (float)tiempo[n];
-  int q[3] = {c[0], c[1], c[2]};
+  int cc[3];
+  cc[f.axis] = f.fixed;
+  cc[ua] = f.lo[ua] + (int)(c % r.nu);
+  cc[va] = f.lo[va] + (int)(c / r.nu);
+  const int step = r.dn ? *r.dn : n;
+  const float tm = r.half ? (float)(r.tiempo[step] + (double)0.5f * r.dt) : (float)r.tiempo[step];
+  int q[3] = {cc[0], cc[1], cc[2]};
   q[f.axis] += f.off;
-  const float xf = phys[(size_t)(f.nf * 3 + 0) * pstride + q[0] + 1];
-  const float yf = phys[(size_t)(f.nf * 3 + 1) * pstride + q[1] + 1];
-  const float zf = phys[(size_t)(f.nf * 3 + 2) * pstride + q[2] + 1];
-  const PwWave &w = waves[f.wave];
+  const float xf = r.phys[(size_t)(f.nf * 3 + 0) * r.pstride + q[0] + 1];
+  const float yf = r.phys[(size_t)(f.nf * 3 + 1) * r.pstride + q[1] + 1];
+  const float zf = r.phys[(size_t)(f.nf * 3 + 2) * r.pstride + q[2] + 1];
+  const PwWave &w = r.wave[f.wave];
   float ehi = 0.0f;
   for (int m = 0; m < w.nmodes; ++m) {
     const float d = (xf * w.pn[m][0] + yf * w.pn[m][1] + zf * w.pn[m][2]) - w.dist0[m];
-    ehi = ehi + w.fpw[f.nf][m] * pw_evol(w, t, d);
+    ehi = ehi + w.fpw[f.nf][m] * pw_evol(w, tm, d);
   }
-  float *F = g.f[f.comp];
-  const long long p = fidx(g, c[0], c[1], c[2]);
-  F[p] = f.sign > 0.0f ? F[p] + G * ehi * f.id : F[p] - G * ehi * f.id;
+  const long long p = (long long)(cc[2] + 1) * r.pxy + (long long)(cc[1] + 1) * r.px + kXOff + cc[0];
+  float *F = r.F;
+  F[p] = f.sign > 0.0f ? F[p] + r.G * ehi * f.id : F[p] - r.G * ehi * f.id;
 }
 
-int launch_tfsf(const DevGrid &g, const PwFace &f, const PwWave *waves, const float *phys,
-                int pstride, const double *tiempo, int n, const int *dn, int half, double dt,
-                float G, void *stream) {
-  const int ua = f.axis == 0 ? 1 : 0;
-  const int va = f.axis == 2 ? 1 : 2;
-  const int nu = f.hi[ua] - f.lo[ua] + 1, nv = f.hi[va] - f.lo[va] + 1;
-  if (nu <= 0 || nv <= 0) return 0;
-  const dim3 blk(64, 4);
-  const dim3 grd((unsigned)((nu + 63) / 64), (unsigned)((nv + 3) / 4));
-  hipLaunchKernelGGL(k_tfsf, grd, blk, 0, (hipStream_t)stream, g, f, waves, phys, pstride,
-                     tiempo, n, dn, half, dt, G);
+int launch_tfsf_list(const PwRun *runs, const long long *start, int nruns, long long total,
+                     int n, void *stream) {
+  if (total <= 0 || nruns <= 0) return 0;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  hipLaunchKernelGGL(k_tfsf_list, dim3(blocks), dim3(256), 0, (hipStream_t)stream, runs, start,
+                     nruns, total, n);
+  return (int)hipGetLastError();
+}
+
+__global__ void k_tick_many(int *const *counters, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) *counters[t] += 1;
+}
+
+int launch_tick_many(int *const *counters, int n, void *stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_tick_many, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     counters, n);
+  return (int)hipGetLastError();
+}
+
+// Frequency-domain sums of one box at step n (or *dn), observation.F90:
+// 3553-3600: one thread per (point, frequency); the three components of the
+// point are read once per thread and multiplied by the complex(8) phase
+// factor of the step (each part scaled by the real sample).
+__global__ void __launch_bounds__(256) k_dft(DevGrid g, DftDev d, int n, const int *dn) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long npt = (long long)d.cnt[0] * d.cnt[1] * d.cnt[2];
+  if (t >= npt * d.nfreq) return;
+  if (dn) n = *dn;
+  const long long pt = t / d.nfreq;
+  const int f = (int)(t - pt * d.nfreq);
+  const int it = (int)(pt % d.cnt[0]);
+  const int jt = (int)((pt / d.cnt[0]) % d.cnt[1]);
+  const int kt = (int)(pt / ((long long)d.cnt[0] * d.cnt[1]));
+  const long long p = fidx(g, d.first[0] + it * d.stride[0], d.first[1] + jt * d.stride[1],
+                           d.first[2] + kt * d.stride[2]);
+  const double2 a = d.aux[(long long)n * d.nfreq + f];
+  for (int c = 0; c < 3; ++c) {
+    const double v = (double)g.f[3 * d.kind + c][p];
+    double2 *acc = d.acc + (pt * 3 + c) * d.nfreq + f;
+    const double2 o = *acc;
+    *acc = make_double2(o.x + a.x * v, o.y + a.y * v);
+  }
+}
+
+int launch_dft(const DevGrid &g, const DftDev &d, int n, const int *dn, void *stream) {
+  const long long work = (long long)d.cnt[0] * d.cnt[1] * d.cnt[2] * d.nfreq;
+  if (work <= 0) return 0;
+  const unsigned blocks = (unsigned)((work + 255) / 256);
+  hipLaunchKernelGGL(k_dft, dim3(blocks), dim3(256), 0, (hipStream_t)stream, g, d, n, dn);
   return (int)hipGetLastError();
 }
 

@@ -1,0 +1,395 @@
+// fdtd_batch.cpp -- batches of independent grids advanced together.
+//
+// A batch holds grids that were set up as single grids (fdtd_create and the
+// set-up calls); admitting one hands its device state to the batch without
+// copying.  One step of the batch runs, for every grid at its own step:
+//   E face pass of all grids          one persistent launch (k_pass_batch)
+//   pole stage                        one launch per dispersive grid
+//   plane-wave E faces                one launch per (wave, face axis) group
+//   nodal E                           one launch per grid with E sources
+//   H face pass of all grids          one persistent launch
+//   plane-wave H faces, nodal H       as above
+//   probes, frequency-domain sums     per grid
+//   step counters                     one launch for all grids
+// captured once per membership in a HIP graph and replayed.  Each grid keeps
+// its own step counter (its dstep), so grids admitted at different times
+// advance in lock-step from different n.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "fdtd_handle.h"
+
+using namespace sfd;
+using namespace sfd_host;
+
+namespace {
+thread_local std::string g_batch_error;
+
+// the plane-wave groups of one (half, wave, axis) key across the batch
+struct BatchPwGroup {
+  int half;
+  std::vector<PwRun> runs;
+  std::vector<long long> start;
+  PwRun *druns = nullptr;
+  long long *dstart = nullptr;
+};
+}  // namespace
+
+struct fdtd_batch {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int mode = -1;
+  std::vector<fdtd_handle *> slots;   // admitted grids; retired slots are null
+  std::vector<long long> step;        // host copy of each slot's step counter
+  std::vector<unsigned> version;      // each grid's state version at the last build
+  int rows = 4, zchunk = 32, blocks = 0;
+  // device descriptors, rebuilt when the membership changes
+  bool ready = false;
+  std::vector<void *> allocs;
+  BatchDev bd{};
+  size_t lds = 0;
+  std::vector<BatchPwGroup> pw;
+  int **dcounters = nullptr;
+  int ncounters = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+int bfail(fdtd_batch *b, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+int bfail(fdtd_batch *b, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf; else g_batch_error = buf;
+  return code;
+}
+
+#define BCHK(b, call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return bfail(b, FDTD_ERR_HIP, "%s: %s (%s:%d)", #call,                   \
+                   hipGetErrorString(e_), __FILE__, __LINE__);                 \
+  } while (0)
+
+void drop_graph(fdtd_batch *b) {
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+void drop_descriptors(fdtd_batch *b) {
+  drop_graph(b);
+  for (void *p : b->allocs) hipFree(p);
+  b->allocs.clear();
+  b->pw.clear();
+  b->ready = false;
+}
+
+template <class T>
+int bupload(fdtd_batch *b, T **dst, const T *src, size_t n) {
+  void *p = nullptr;
+  BCHK(b, hipMalloc(&p, std::max<size_t>(n * sizeof(T), 16)));
+  b->allocs.push_back(p);
+  if (n) BCHK(b, hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, b->stream));
+  *dst = (T *)p;
+  return FDTD_OK;
+}
+
+// tile map of one grid under the batch's tiling
+TileMap batch_tiles(const fdtd_batch *b, const DevGrid &g) {
+  TileMap tm{};
+  tm.bdy = b->rows;
+  tm.tx = (g.nx + 1 + kWave - 1) / kWave;
+  tm.ty = (g.ny + 1 + b->rows - 1) / b->rows;
+  tm.kc = std::min(b->zchunk, g.nz + 1);
+  tm.tz = (g.nz + 1 + tm.kc - 1) / tm.kc;
+  tm.nblk = tm.tx * tm.ty * tm.tz;
+  tm.remap = 0;
+  return tm;
+}
+
+int build_descriptors(fdtd_batch *b) {
+  if (b->ready) return FDTD_OK;
+  drop_descriptors(b);
+  std::vector<DevGrid> grids;
+  std::vector<TileMap> tms;
+  std::vector<int> start(1, 0);
+  std::vector<int *> counters;
+  size_t lds = 16;
+  int maxwaves = 0;
+  int rc;
+  for (size_t s = 0; s < b->slots.size(); ++s) {
+    fdtd_handle *h = b->slots[s];
+    if (!h) continue;
+    // a grid changed since admission (set-up calls) rebuilds its device state
+    if ((rc = prepare(h))) return bfail(b, rc, "slot %zu: %s", s, h->err.c_str());
+    b->version[s] = h->version;
+    grids.push_back(h->g);
+    tms.push_back(batch_tiles(b, h->g));
+    if ((long long)start.back() + tms.back().nblk > 0x7fffffffLL)
+      return bfail(b, FDTD_ERR_ARG, "batch holds too many tiles");
+    start.push_back(start.back() + tms.back().nblk);
+    counters.push_back(h->dstep);
+    lds = std::max(lds, (size_t)(h->g.nmedia + 1) * 12);
+    maxwaves = std::max(maxwaves, (int)h->pwin.size());
+  }
+  DevGrid *dg = nullptr;
+  TileMap *dt = nullptr;
+  int *ds = nullptr;
+  if ((rc = bupload(b, &dg, grids.data(), grids.size())) ||
+      (rc = bupload(b, &dt, tms.data(), tms.size())) ||
+      (rc = bupload(b, &ds, start.data(), start.size())) ||
+      (rc = bupload(b, &b->dcounters, counters.data(), counters.size())))
+    return rc;
+  b->bd.grids = dg;
+  b->bd.tms = dt;
+  b->bd.start = ds;
+  b->bd.ngrid = (int)grids.size();
+  b->bd.total = start.back();
+  b->ncounters = (int)counters.size();
+  b->lds = lds;
+  // plane-wave groups across the batch, in each grid's face order: per half,
+  // per wave index, per face axis (the runs read each grid's own counter)
+  for (int half = 0; half < 2; ++half)
+    for (int w = 0; w < maxwaves; ++w)
+      for (int axis = 0; axis < 3; ++axis) {
+        BatchPwGroup grp;
+        grp.half = half;
+        grp.start.push_back(0);
+        for (fdtd_handle *h : b->slots) {
+          if (!h) continue;
+          for (const PwGroup &g : h->pw_groups) {
+            if (g.half != half || g.runs.empty() || g.runs[0].f.wave != w ||
+                g.runs[0].f.axis != axis)
+              continue;
+            for (size_t r = 0; r < g.runs.size(); ++r) {
+              PwRun run = g.runs[r];
+              run.dn = h->dstep;
+              grp.runs.push_back(run);
+              grp.start.push_back(grp.start.back() + (g.start[r + 1] - g.start[r]));
+            }
+          }
+        }
+        if (grp.runs.empty()) continue;
+        if ((rc = bupload(b, &grp.druns, grp.runs.data(), grp.runs.size())) ||
+            (rc = bupload(b, &grp.dstart, grp.start.data(), grp.start.size())))
+          return rc;
+        b->pw.push_back(std::move(grp));
+      }
+  BCHK(b, hipStreamSynchronize(b->stream));
+  b->ready = true;
+  return FDTD_OK;
+}
+
+int launch_step(fdtd_batch *b) {
+  const bool exact = b->mode == FDTD_MODE_EXACT;
+  const int blocks = b->blocks > 0 ? b->blocks : 2048;
+  int e, rc;
+  for (int half = 0; half < 2; ++half) {
+    e = exact ? exact::launch_pass_batch(b->bd, half, blocks, b->rows, b->lds, b->stream)
+              : fast::launch_pass_batch(b->bd, half, blocks, b->rows, b->lds, b->stream);
+    if (e) return bfail(b, FDTD_ERR_HIP, "batch face pass failed: %s", hipGetErrorString((hipError_t)e));
+    if (half == 0)
+      for (fdtd_handle *h : b->slots)
+        if (h && h->g.has_ade && (rc = run_ade(h)))
+          return bfail(b, rc, "grid: %s", h->err.c_str());
+    for (const BatchPwGroup &g : b->pw) {
+      if (g.half != half) continue;
+      e = launch_tfsf_list(g.druns, g.dstart, (int)g.runs.size(), g.start.back(), 0, b->stream);
+      if (e) return bfail(b, FDTD_ERR_HIP, "batch plane-wave launch failed");
+    }
+    for (fdtd_handle *h : b->slots)
+      if (h && (rc = run_nodal(h, half, 0, h->dstep)))
+        return bfail(b, rc, "grid: %s", h->err.c_str());
+  }
+  for (fdtd_handle *h : b->slots) {
+    if (!h) continue;
+    if (h->nprobe && launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap,
+                                   b->stream))
+      return bfail(b, FDTD_ERR_HIP, "batch probe launch failed");
+    if ((rc = run_dft(h, 0, h->dstep))) return bfail(b, rc, "grid: %s", h->err.c_str());
+  }
+  if (launch_tick_many(b->dcounters, b->ncounters, b->stream))
+    return bfail(b, FDTD_ERR_HIP, "batch step counter launch failed");
+  return FDTD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fdtd_batch_last_error(const fdtd_batch *b) {
+  return b ? b->err.c_str() : g_batch_error.c_str();
+}
+
+int fdtd_batch_create(int32_t device, void *stream, fdtd_batch **out) {
+  if (!out) return bfail(nullptr, FDTD_ERR_ARG, "null argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return bfail(nullptr, FDTD_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev)
+    return bfail(nullptr, FDTD_ERR_DEVICE, "device %d out of range", device);
+  fdtd_batch *b = new fdtd_batch();
+  b->device = device;
+  hipSetDevice(device);
+  if (stream) {
+    b->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete b;
+      return bfail(nullptr, FDTD_ERR_HIP, "hipStreamCreate failed");
+    }
+    b->own_stream = true;
+  }
+  *out = b;
+  return FDTD_OK;
+}
+
+int fdtd_batch_destroy(fdtd_batch *b) {
+  if (!b) return FDTD_OK;
+  hipSetDevice(b->device);
+  hipStreamSynchronize(b->stream);
+  for (fdtd_handle *h : b->slots)
+    if (h && h->solo_stream) {
+      h->stream = h->solo_stream;
+      h->solo_stream = nullptr;
+      invalidate_graph(h);
+    }
+  drop_descriptors(b);
+  if (b->own_stream) hipStreamDestroy(b->stream);
+  delete b;
+  return FDTD_OK;
+}
+
+int fdtd_batch_admit(fdtd_batch *b, fdtd_handle *h, int32_t n0, int32_t *slot) {
+  if (!b || !h || !slot || n0 < 0) return bfail(b, FDTD_ERR_ARG, "bad admit arguments");
+  if (h->device != b->device) return bfail(b, FDTD_ERR_ARG, "grid lives on another device");
+  for (fdtd_handle *o : b->slots)
+    if (o == h) return bfail(b, FDTD_ERR_ARG, "grid already admitted");
+  if (b->mode >= 0 && h->mode != b->mode)
+    return bfail(b, FDTD_ERR_ARG, "grids of one batch share the arithmetic mode");
+  hipSetDevice(b->device);
+  int rc = prepare(h);
+  if (rc) return bfail(b, rc, "grid not ready: %s", h->err.c_str());
+  if (h->nmedia + 1 > 4096) return bfail(b, FDTD_ERR_ARG, "batched grids hold at most 4095 media");
+  // the grid's work is ordered after anything still queued on its own stream
+  BCHK(b, hipStreamSynchronize(h->stream));
+  BCHK(b, hipMemcpyAsync(h->dstep, &n0, 4, hipMemcpyHostToDevice, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  invalidate_graph(h);
+  // while admitted, the grid's own launches (set-up, reads) go to the batch's
+  // stream, so they are ordered with the batch steps
+  h->solo_stream = h->stream;
+  h->stream = b->stream;
+  b->mode = h->mode;
+  int s = -1;
+  for (size_t i = 0; i < b->slots.size(); ++i)
+    if (!b->slots[i]) { s = (int)i; break; }
+  if (s < 0) {
+    s = (int)b->slots.size();
+    b->slots.push_back(nullptr);
+    b->step.push_back(0);
+    b->version.push_back(0);
+  }
+  b->slots[s] = h;
+  b->step[s] = n0;
+  b->ready = false;
+  *slot = s;
+  return FDTD_OK;
+}
+
+int fdtd_batch_retire(fdtd_batch *b, int32_t slot) {
+  if (!b || slot < 0 || slot >= (int32_t)b->slots.size() || !b->slots[slot])
+    return bfail(b, FDTD_ERR_ARG, "no grid in slot %d", slot);
+  hipSetDevice(b->device);
+  BCHK(b, hipStreamSynchronize(b->stream));
+  fdtd_handle *h = b->slots[slot];
+  if (h->solo_stream) {
+    h->stream = h->solo_stream;
+    h->solo_stream = nullptr;
+  }
+  invalidate_graph(h);
+  b->slots[slot] = nullptr;
+  b->ready = false;
+  bool any = false;
+  for (fdtd_handle *h : b->slots) any = any || h;
+  if (!any) b->mode = -1;
+  return FDTD_OK;
+}
+
+int64_t fdtd_batch_step_of(const fdtd_batch *b, int32_t slot) {
+  if (!b || slot < 0 || slot >= (int32_t)b->slots.size() || !b->slots[slot]) return -1;
+  return b->step[slot];
+}
+
+int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t blocks) {
+  if (!b || !(rows == 1 || rows == 2 || rows == 4) || zchunk < 1 || blocks < 0)
+    return bfail(b, FDTD_ERR_ARG, "tiling %d rows x %d planes, %d blocks not supported", rows,
+                 zchunk, blocks);
+  b->rows = rows;
+  b->zchunk = zchunk;
+  b->blocks = blocks;
+  b->ready = false;
+  return FDTD_OK;
+}
+
+int fdtd_batch_step(fdtd_batch *b, int32_t nsteps) {
+  if (!b || nsteps < 0) return bfail(b, FDTD_ERR_ARG, "bad step count");
+  hipSetDevice(b->device);
+  bool any = false;
+  for (size_t s = 0; s < b->slots.size(); ++s) {
+    fdtd_handle *h = b->slots[s];
+    if (!h) continue;
+    any = true;
+    const long long last = b->step[s] + nsteps;
+    if ((!h->srcs.empty() && last > h->nvalues_steps) ||
+        ((!h->pwin.empty() || !h->dftin.empty()) && last > (long long)h->tiempo.size()))
+      return bfail(b, FDTD_ERR_ARG, "slot %zu: steps up to %lld beyond its time vector", s,
+                   last - 1);
+    if (h->nprobe && h->pcap < nsteps)
+      return bfail(b, FDTD_ERR_ARG, "slot %zu: probe buffer holds %d steps, %d asked", s,
+                   h->pcap, nsteps);
+  }
+  if (!any || nsteps == 0) return FDTD_OK;
+  for (size_t s = 0; s < b->slots.size(); ++s)
+    if (b->slots[s] && b->slots[s]->version != b->version[s]) b->ready = false;
+  int rc = build_descriptors(b);
+  if (rc) return rc;
+  if (!b->gexec) {
+    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int r2 = launch_step(b);
+    hipGraph_t graph = nullptr;
+    hipError_t ec = hipStreamEndCapture(b->stream, &graph);
+    if (r2) { if (graph) hipGraphDestroy(graph); return r2; }
+    if (ec != hipSuccess) return bfail(b, FDTD_ERR_HIP, "batch capture failed: %s", hipGetErrorString(ec));
+    b->graph = graph;
+    BCHK(b, hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
+  }
+  for (int32_t s = 0; s < nsteps; ++s) BCHK(b, hipGraphLaunch(b->gexec, b->stream));
+  for (size_t s = 0; s < b->slots.size(); ++s)
+    if (b->slots[s]) b->step[s] += nsteps;
+  return FDTD_OK;
+}
+
+int fdtd_batch_synchronize(fdtd_batch *b) {
+  if (!b) return FDTD_ERR_ARG;
+  hipSetDevice(b->device);
+  BCHK(b, hipStreamSynchronize(b->stream));
+  return FDTD_OK;
+}
+
+}  // extern "C"

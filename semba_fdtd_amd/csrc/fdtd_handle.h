@@ -1,0 +1,134 @@
+// fdtd_handle.h -- host-side state of one grid (the opaque fdtd_handle of
+// the C ABI) and the helpers the host translation units share.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fdtd_internal.h"
+#include "sembafdtd.h"
+
+namespace sfd_host {
+using namespace sfd;
+struct Source {
+  int comp, hard;
+  int box[6];
+  float amp, deltaevol;
+  int numus;
+  std::vector<float> evol;
+};
+
+struct DftIn {           // one frequency-domain observation box
+  int kind;
+  int box[6], stride[3];
+  std::vector<float> freq;
+};
+
+// one launch of TF/SF faces: the faces of one wave and one face axis
+struct PwGroup {
+  int half;                       // 0: E faces, 1: H faces
+  std::vector<PwRun> runs;        // step from the launch (dn = null)
+  std::vector<long long> start;   // prefix of face areas, runs.size()+1 entries
+  PwRun *druns[2] = {};           // device copies: [0] launch step, [1] device counter
+  long long *dstart = nullptr;
+};
+
+struct PlaneWaveIn {     // sgg%PlaneWave(j) (planeonde_t, fdetypes.F90:458-465)
+  int esq[6];            // esqx1, esqx2, esqy1, esqy2, esqz1, esqz2
+  int nmodes;
+  float p[kPwModes][3], e[kPwModes][3], incert[kPwModes];
+  float deltaevol;
+  std::vector<float> evol;
+  float eps0, mu0;
+};
+}  // namespace sfd_host
+
+struct fdtd_handle {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;       // the stream every launch goes to
+  hipStream_t solo_stream = nullptr;  // its own stream while lent to a batch
+  bool own_stream = false;
+  int mode = FDTD_MODE_FAST;
+  int storage = FDTD_STORE_FP32;
+  sfd::DevGrid g{};
+  sfd::TileMap tm{};
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+  float *fld[6] = {};
+  int16_t *mi[6] = {};
+  // media
+  int nmedia = -1;
+  int media_max = 0;  // largest media index written so far
+  std::vector<float> g2h, gm2h;
+  std::vector<int32_t> kindh;
+  float *dg[4] = {};
+  // spacing
+  std::vector<float> idh_[6];
+  float *did[6] = {};
+  // cpml
+  float *dprof[12] = {};
+  long long psilen[12] = {};       // psi values per slab array
+  // dispersive
+  int ndisp = 0;
+  std::vector<int32_t> dmed, dnp;
+  std::vector<double> dk, db, dg3;
+  bool ade_ready = false;
+  std::vector<void *> ade_allocs;
+  long long nedges[3] = {0, 0, 0};  // dispersive edges per E component
+  // time
+  double dt = 0.0;
+  std::vector<double> tiempo;
+  // nodal
+  std::vector<sfd_host::Source> srcs;
+  bool nodal_ready = false;
+  std::vector<void *> nodal_allocs;
+  sfd::NodalCell *dcells[2] = {};
+  int ncells[2] = {0, 0};
+  int serial[2] = {0, 0};
+  float *dvalues = nullptr;
+  int nvalues_steps = 0;
+  // TF/SF plane waves
+  std::vector<sfd_host::PlaneWaveIn> pwin;
+  std::vector<float> line[3];           // LineX/Y/Z over [-1, n+1]
+  bool pw_ready = false;
+  std::vector<void *> pw_allocs;
+  std::vector<sfd::PwFace> pw_faces[2];      // E faces, H faces, in reference order
+  std::vector<sfd_host::PwGroup> pw_groups;  // launch groups, in reference order
+  sfd::PwWave *dwaves = nullptr;
+  float *dphys = nullptr;
+  int pstride = 0;
+  double *dtiempo = nullptr;
+  // frequency-domain observation
+  std::vector<sfd_host::DftIn> dftin;
+  std::vector<sfd::DftDev> dfts;
+  bool dft_ready = false;
+  std::vector<void *> dft_allocs;     // phase tables (rebuilt with the time vector)
+  std::vector<void *> dft_sums;       // running sums (kept across rebuilds)
+  // probes
+  sfd::ProbeDev *dprobes = nullptr;
+  int nprobe = 0;
+  float *dpbuf = nullptr;
+  int *dpcount = nullptr;
+  int pcap = 0;
+  // graph
+  int *dstep = nullptr;           // device step counter of the captured step
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool graph_ok = false;
+  unsigned version = 0;           // bumped by every change that drops the graph
+};
+
+
+// helpers shared by the host translation units (fdtd_host.cpp)
+namespace sfd_host {
+int fail(fdtd_handle *h, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+int dalloc(fdtd_handle *h, void **p, size_t bytes, std::vector<void *> *list = nullptr);
+int prepare(fdtd_handle *h);
+int run_pw(fdtd_handle *h, int half, int n, const int *dn);
+int run_nodal(fdtd_handle *h, int half, int n, const int *dn);
+int run_dft(fdtd_handle *h, int n, const int *dn);
+int run_ade(fdtd_handle *h);
+void invalidate_graph(fdtd_handle *h);
+}  // namespace sfd_host

@@ -10,101 +10,17 @@
 #include <string>
 #include <vector>
 
-#include "fdtd_internal.h"
-#include "sembafdtd.h"
+#include "fdtd_handle.h"
 
 using namespace sfd;
+using namespace sfd_host;
 
 namespace {
 thread_local std::string g_create_error;
-
-struct Source {
-  int comp, hard;
-  int box[6];
-  float amp, deltaevol;
-  int numus;
-  std::vector<float> evol;
-};
-
-struct PlaneWaveIn {     // sgg%PlaneWave(j) (planeonde_t, fdetypes.F90:458-465)
-  int esq[6];            // esqx1, esqx2, esqy1, esqy2, esqz1, esqz2
-  int nmodes;
-  float p[kPwModes][3], e[kPwModes][3], incert[kPwModes];
-  float deltaevol;
-  std::vector<float> evol;
-  float eps0, mu0;
-};
 }  // namespace
 
-struct fdtd_handle {
-  std::string err;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-  int mode = FDTD_MODE_FAST;
-  int storage = FDTD_STORE_FP32;
-  DevGrid g{};
-  TileMap tm{};
-  std::vector<void *> allocs;
-  int64_t bytes = 0;
-  float *fld[6] = {};
-  int16_t *mi[6] = {};
-  // media
-  int nmedia = -1;
-  int media_max = 0;  // largest media index written so far
-  std::vector<float> g2h, gm2h;
-  std::vector<int32_t> kindh;
-  float *dg[4] = {};
-  // spacing
-  std::vector<float> idh_[6];
-  float *did[6] = {};
-  // cpml
-  float *dprof[12] = {};
-  // dispersive
-  int ndisp = 0;
-  std::vector<int32_t> dmed, dnp;
-  std::vector<double> dk, db, dg3;
-  bool ade_ready = false;
-  std::vector<void *> ade_allocs;
-  long long nedges[3] = {0, 0, 0};  // dispersive edges per E component
-  // time
-  double dt = 0.0;
-  std::vector<double> tiempo;
-  // nodal
-  std::vector<Source> srcs;
-  bool nodal_ready = false;
-  std::vector<void *> nodal_allocs;
-  NodalCell *dcells[2] = {};
-  int ncells[2] = {0, 0};
-  int serial[2] = {0, 0};
-  float *dvalues = nullptr;
-  int nvalues_steps = 0;
-  // TF/SF plane waves
-  std::vector<PlaneWaveIn> pwin;
-  std::vector<float> line[3];           // LineX/Y/Z over [-1, n+1]
-  bool pw_ready = false;
-  std::vector<void *> pw_allocs;
-  std::vector<PwFace> pw_faces[2];      // E faces, H faces, in reference order
-  PwWave *dwaves = nullptr;
-  float *dphys = nullptr;
-  int pstride = 0;
-  double *dtiempo = nullptr;
-  // probes
-  ProbeDev *dprobes = nullptr;
-  int nprobe = 0;
-  float *dpbuf = nullptr;
-  int *dpcount = nullptr;
-  int pcap = 0;
-  // graph
-  int *dstep = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  bool graph_ok = false;
-};
+namespace sfd_host {
 
-namespace {
-
-int fail(fdtd_handle *h, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
 int fail(fdtd_handle *h, int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
@@ -123,7 +39,7 @@ int fail(fdtd_handle *h, int code, const char *fmt, ...) {
                   hipGetErrorString(e_), __FILE__, __LINE__);                  \
   } while (0)
 
-int dalloc(fdtd_handle *h, void **p, size_t bytes, std::vector<void *> *list = nullptr) {
+int dalloc(fdtd_handle *h, void **p, size_t bytes, std::vector<void *> *list) {
   if (bytes == 0) bytes = 16;
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess)
@@ -192,7 +108,21 @@ int check_ready(fdtd_handle *h) {
   return FDTD_OK;
 }
 
+// blocks of 64 x rows threads march z in chunks of zchunk planes
+void set_tiles(fdtd_handle *h, int rows, int zchunk) {
+  TileMap &tm = h->tm;
+  const DevGrid &g = h->g;
+  tm.bdy = rows;
+  tm.tx = (g.nx + 1 + kWave - 1) / kWave;
+  tm.ty = (g.ny + 1 + rows - 1) / rows;
+  tm.kc = std::min(zchunk, g.nz + 1);
+  tm.tz = (g.nz + 1 + tm.kc - 1) / tm.kc;
+  tm.nblk = tm.tx * tm.ty * tm.tz;
+  tm.remap = 1;
+}
+
 void invalidate_graph(fdtd_handle *h) {
+  ++h->version;
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   if (h->graph) hipGraphDestroy(h->graph);
   h->gexec = nullptr;
@@ -528,7 +458,96 @@ int build_planewaves(fdtd_handle *h) {
   if ((rc = upload(h, &h->dphys, phys.data(), phys.size(), &h->pw_allocs))) return rc;
   if ((rc = upload(h, &h->dtiempo, h->tiempo.data(), h->tiempo.size(), &h->pw_allocs))) return rc;
   h->pstride = S;
+  // launch groups: per half, per wave, per face axis (faces of a group are
+  // parallel planes, so no two touch the same cell); the group order is the
+  // reference's face order
+  h->pw_groups.clear();
+  for (int half = 0; half < 2; ++half)
+    for (size_t q = 0; q < h->pwin.size(); ++q)
+      for (int axis = 0; axis < 3; ++axis) {
+        PwGroup grp;
+        grp.half = half;
+        grp.start.push_back(0);
+        for (const PwFace &f : h->pw_faces[half]) {
+          if (f.wave != (int)q || f.axis != axis) continue;
+          const int ua = f.axis == 0 ? 1 : 0, va = f.axis == 2 ? 1 : 2;
+          const long long nu = f.hi[ua] - f.lo[ua] + 1, nv = f.hi[va] - f.lo[va] + 1;
+          if (nu <= 0 || nv <= 0) continue;
+          PwRun r{};
+          r.f = f;
+          r.F = h->fld[f.comp];
+          r.px = h->g.px;
+          r.pxy = h->g.pxy;
+          r.wave = h->dwaves;
+          r.phys = h->dphys;
+          r.pstride = S;
+          r.half = half;
+          r.tiempo = h->dtiempo;
+          r.dn = nullptr;
+          r.dt = h->dt;
+          r.G = half ? h->gm2h[1] : h->g2h[1];
+          r.nu = (int)nu;
+          grp.runs.push_back(r);
+          grp.start.push_back(grp.start.back() + nu * nv);
+        }
+        if (grp.runs.empty()) continue;
+        std::vector<PwRun> counted = grp.runs;
+        for (PwRun &r : counted) r.dn = h->dstep;
+        if ((rc = upload(h, &grp.druns[0], grp.runs.data(), grp.runs.size(), &h->pw_allocs)))
+          return rc;
+        if ((rc = upload(h, &grp.druns[1], counted.data(), counted.size(), &h->pw_allocs)))
+          return rc;
+        if ((rc = upload(h, &grp.dstart, grp.start.data(), grp.start.size(), &h->pw_allocs)))
+          return rc;
+        h->pw_groups.push_back(std::move(grp));
+      }
   h->pw_ready = true;
+  return FDTD_OK;
+}
+
+// ---- frequency-domain observation ------------------------------------------
+// Phase factors of observation.F90:3532-3533 per step and frequency:
+// E: dt * exp(mcPI2 f t_n); H: that times exp(mcPI2 f dt / 2).  mcPI2 =
+// -j 2 pi with pi in real(4) (fdetypes.F90:187-190); f is real(4), the
+// products run in real(8), exp of the purely imaginary argument is
+// (cos y, sin y).
+void dft_phases(const DftIn &d, const std::vector<double> &tiempo, double dt,
+                std::vector<double2> &aux) {
+  const float pi4 = 3.14159265358979323846f;
+  const double mc = -(double)(2.0f * pi4);
+  const size_t nf = d.freq.size();
+  aux.resize(tiempo.size() * nf);
+  for (size_t n = 0; n < tiempo.size(); ++n)
+    for (size_t f = 0; f < nf; ++f) {
+      const double F = (double)d.freq[f];
+      const double y = (mc * F) * tiempo[n];
+      double re = dt * std::cos(y), im = dt * std::sin(y);
+      if (d.kind == 1) {
+        const double y2 = ((mc * F) * dt) * 0.5;
+        const double c = std::cos(y2), sn = std::sin(y2);
+        const double r2 = re * c - im * sn, i2 = re * sn + im * c;
+        re = r2;
+        im = i2;
+      }
+      aux[n * nf + f] = make_double2(re, im);
+    }
+}
+
+int build_dfts(fdtd_handle *h) {
+  if (h->dft_ready) return FDTD_OK;
+  for (void *p : h->dft_allocs) hipFree(p);
+  h->dft_allocs.clear();
+  if (h->dftin.empty()) { h->dft_ready = true; return FDTD_OK; }
+  if (h->tiempo.empty()) return fail(h, FDTD_ERR_STATE, "fdtd_set_time not called");
+  int rc;
+  for (size_t q = 0; q < h->dftin.size(); ++q) {
+    std::vector<double2> aux;
+    dft_phases(h->dftin[q], h->tiempo, h->dt, aux);
+    double2 *da = nullptr;
+    if ((rc = upload(h, &da, aux.data(), aux.size(), &h->dft_allocs))) return rc;
+    h->dfts[q].aux = da;
+  }
+  h->dft_ready = true;
   return FDTD_OK;
 }
 
@@ -538,6 +557,7 @@ int prepare(fdtd_handle *h) {
   if ((rc = build_ade(h))) return rc;
   if ((rc = build_nodal(h))) return rc;
   if ((rc = build_planewaves(h))) return rc;
+  if ((rc = build_dfts(h))) return rc;
   return FDTD_OK;
 }
 
@@ -550,11 +570,26 @@ int run_e(fdtd_handle *h, int yee, int cpml, int ade) {
   return FDTD_OK;
 }
 
+int run_ade(fdtd_handle *h) { return run_e(h, 0, 0, 1); }
+
 int run_h(fdtd_handle *h, int yee, int cpml) {
   const int c = cpml && h->g.has_cpml;
   int e = h->mode == FDTD_MODE_EXACT ? exact::launch_h(h->g, h->tm, yee, c, h->stream)
                                      : fast::launch_h(h->g, h->tm, yee, c, h->stream);
   if (e) return fail(h, FDTD_ERR_HIP, "H pass launch failed: %s", hipGetErrorString((hipError_t)e));
+  return FDTD_OK;
+}
+
+// UpdateObservation's frequency-domain sums at step n (or *dn)
+int run_dft(fdtd_handle *h, int n, const int *dn) {
+  if (h->dfts.empty()) return FDTD_OK;
+  if (!dn && (n < 0 || n >= (int)h->tiempo.size()))
+    return fail(h, FDTD_ERR_ARG, "step %d outside the time vector (0..%d)", n,
+                (int)h->tiempo.size() - 1);
+  for (const DftDev &d : h->dfts) {
+    int e = launch_dft(h->g, d, n, dn, h->stream);
+    if (e) return fail(h, FDTD_ERR_HIP, "DFT launch failed: %s", hipGetErrorString((hipError_t)e));
+  }
   return FDTD_OK;
 }
 
@@ -565,10 +600,11 @@ int run_pw(fdtd_handle *h, int half, int n, const int *dn) {
   if (!dn && (n < 0 || n >= (int)h->tiempo.size()))
     return fail(h, FDTD_ERR_ARG, "step %d outside the time vector (0..%d)", n,
                 (int)h->tiempo.size() - 1);
-  const float G = half ? h->gm2h[1] : h->g2h[1];
-  for (const PwFace &f : h->pw_faces[half]) {
-    int e = launch_tfsf(h->g, f, h->dwaves, h->dphys, h->pstride, h->dtiempo, n, dn, half, h->dt,
-                        G, h->stream);
+  for (PwGroup &grp : h->pw_groups) {
+    if (grp.half != half) continue;
+    // the captured step reads the device step counter, eager calls pass n
+    int e = launch_tfsf_list(grp.druns[dn ? 1 : 0], grp.dstart, (int)grp.runs.size(), grp.start.back(), n,
+                             h->stream);
     if (e) return fail(h, FDTD_ERR_HIP, "plane-wave launch failed: %s", hipGetErrorString((hipError_t)e));
   }
   return FDTD_OK;
@@ -650,15 +686,8 @@ int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
     fdtd_destroy(h);
     return rc;
   }
-  // tiles: 64 x 4 threads, z chunks of 64 planes (or fewer for thin grids)
-  TileMap &tm = h->tm;
-  tm.bdy = 4;
-  tm.tx = (d->nx + 1 + kWave - 1) / kWave;
-  tm.ty = (d->ny + 1 + tm.bdy - 1) / tm.bdy;
-  tm.kc = std::min(64, d->nz + 1);
-  tm.tz = (d->nz + 1 + tm.kc - 1) / tm.kc;
-  tm.nblk = tm.tx * tm.ty * tm.tz;
-  tm.remap = 1;
+  // tiles: 64 x 4 threads, z chunks of 32 planes (or fewer for thin grids)
+  set_tiles(h, 4, 32);
   hipStreamSynchronize(h->stream);
   *out = h;
   return FDTD_OK;
@@ -672,7 +701,10 @@ int fdtd_destroy(fdtd_handle *h) {
   for (void *p : h->ade_allocs) hipFree(p);
   for (void *p : h->nodal_allocs) hipFree(p);
   for (void *p : h->pw_allocs) hipFree(p);
+  for (void *p : h->dft_allocs) hipFree(p);
+  for (void *p : h->dft_sums) hipFree(p);
   for (void *p : h->allocs) hipFree(p);
+  if (h->solo_stream) h->stream = h->solo_stream;
   if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
   delete h;
   return FDTD_OK;
@@ -796,9 +828,11 @@ int fdtd_set_cpml(fdtd_handle *h, const int32_t layers[6], const float *pbe_x,
       (long long)nx * Ly * (nz + 1)};       // Hzy
   for (int s = 0; s < 12; ++s) {
     int rc;
+    if (len[s] != h->psilen[s]) g.psi[s] = nullptr;  // a different layout: fresh array
     if (!g.psi[s] && len[s] > 0) {
       if ((rc = dalloc(h, (void **)&g.psi[s], (size_t)len[s] * 4))) return rc;
     }
+    h->psilen[s] = len[s];
     if (len[s] > 0 && (rc = dzero(h, g.psi[s], (size_t)len[s] * 4))) return rc;
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -840,6 +874,7 @@ int fdtd_set_time(fdtd_handle *h, double dt, int32_t nlast, const double *tiempo
   h->ade_ready = false;  // FAST coefficients pre-divide by dt
   h->nodal_ready = false;
   h->pw_ready = false;
+  h->dft_ready = false;
   invalidate_graph(h);
   return FDTD_OK;
 }
@@ -864,6 +899,93 @@ int fdtd_add_nodal(fdtd_handle *h, int32_t comp, int32_t hard, const int32_t box
   h->nodal_ready = false;
   invalidate_graph(h);
   return FDTD_OK;
+}
+
+int fdtd_set_tiling(fdtd_handle *h, int32_t rows, int32_t zchunk) {
+  if (!h || !(rows == 1 || rows == 2 || rows == 4) || zchunk < 1)
+    return fail(h, FDTD_ERR_ARG, "tiling %d rows x %d planes not supported", rows, zchunk);
+  const long long nblk = (long long)((h->g.nx + kWave) / kWave) * ((h->g.ny + rows) / rows) *
+                         ((h->g.nz + zchunk) / zchunk);
+  if (nblk > 0x7fffffffLL) return fail(h, FDTD_ERR_ARG, "tiling gives too many blocks");
+  set_tiles(h, rows, zchunk);
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk) {
+  if (!h || !rows || !zchunk) return FDTD_ERR_ARG;
+  *rows = h->tm.bdy;
+  *zchunk = h->tm.kc;
+  return FDTD_OK;
+}
+
+int fdtd_add_dft(fdtd_handle *h, int32_t kind, const int32_t box[6], const int32_t stride[3],
+                 int32_t nfreq, const float *freq, int32_t *id) {
+  if (!h || (kind != 0 && kind != 1) || !box || !stride || nfreq < 1 || !freq || !id)
+    return fail(h, FDTD_ERR_ARG, "bad frequency-domain probe arguments");
+  hipSetDevice(h->device);
+  DftIn d;
+  d.kind = kind;
+  DftDev dv{};
+  dv.kind = kind;
+  dv.nfreq = nfreq;
+  long long npt = 1;
+  for (int a = 0; a < 3; ++a) {
+    const int lo = box[2 * a], hi = box[2 * a + 1], st = stride[a];
+    if (st < 1 || lo > hi || lo < -1 || hi > axis_n(h, a) + 1)
+      return fail(h, FDTD_ERR_ARG, "frequency-domain box outside the grid on axis %d", a);
+    int first = lo;
+    while (first <= hi && first % st != 0) ++first;
+    int cnt = 0;
+    for (int v = first; v <= hi; v += st) ++cnt;
+    dv.first[a] = first;
+    dv.stride[a] = st;
+    dv.cnt[a] = cnt;
+    npt *= cnt;
+    d.box[2 * a] = lo; d.box[2 * a + 1] = hi; d.stride[a] = st;
+  }
+  if (npt == 0) return fail(h, FDTD_ERR_ARG, "frequency-domain box holds no point");
+  d.freq.assign(freq, freq + nfreq);
+  const size_t bytes = (size_t)npt * 3 * nfreq * sizeof(double2);
+  int rc;
+  if ((rc = dalloc(h, (void **)&dv.acc, bytes, &h->dft_sums))) return rc;
+  HIPCHK(h, hipMemset(dv.acc, 0, bytes));
+  h->dftin.push_back(std::move(d));
+  h->dfts.push_back(dv);
+  h->dft_ready = false;
+  invalidate_graph(h);
+  *id = (int32_t)h->dfts.size() - 1;
+  return FDTD_OK;
+}
+
+int64_t fdtd_dft_size(const fdtd_handle *h, int32_t id) {
+  if (!h || id < 0 || id >= (int32_t)h->dfts.size()) return -1;
+  const DftDev &d = h->dfts[id];
+  return (int64_t)d.cnt[0] * d.cnt[1] * d.cnt[2] * 3 * d.nfreq;
+}
+
+int fdtd_read_dft(fdtd_handle *h, int32_t id, double *out, int64_t capacity) {
+  if (!h || id < 0 || id >= (int32_t)h->dfts.size() || !out)
+    return fail(h, FDTD_ERR_ARG, "bad frequency-domain probe id");
+  const int64_t n = fdtd_dft_size(h, id);
+  if (capacity < n) return fail(h, FDTD_ERR_ARG, "output holds %lld of %lld complex values",
+                                (long long)capacity, (long long)n);
+  hipSetDevice(h->device);
+  HIPCHK(h, hipMemcpyAsync(out, h->dfts[id].acc, (size_t)n * sizeof(double2),
+                           hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FDTD_OK;
+}
+
+int fdtd_update_observation(fdtd_handle *h, int32_t n) {
+  if (!h) return FDTD_ERR_ARG;
+  hipSetDevice(h->device);
+  int rc = prepare(h);
+  if (rc) return rc;
+  if (h->nprobe && launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap,
+                                 h->stream))
+    return fail(h, FDTD_ERR_HIP, "probe launch failed");
+  return run_dft(h, n, nullptr);
 }
 
 int fdtd_set_lines(fdtd_handle *h, const float *line_x, const float *line_y,
@@ -953,7 +1075,7 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
   if (nsteps == 0) return FDTD_OK;
   if (!h->srcs.empty() && n0 + nsteps > h->nvalues_steps)
     return fail(h, FDTD_ERR_ARG, "steps %d..%d beyond the time vector", n0, n0 + nsteps - 1);
-  if (!h->pwin.empty() && (n0 < 0 || n0 + nsteps > (int)h->tiempo.size()))
+  if ((!h->pwin.empty() || !h->dftin.empty()) && (n0 < 0 || n0 + nsteps > (int)h->tiempo.size()))
     return fail(h, FDTD_ERR_ARG, "steps %d..%d beyond the time vector", n0, n0 + nsteps - 1);
   // the step counter lives on the device so one captured step replays for
   // every n
@@ -970,6 +1092,7 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
     if (!r2 && h->nprobe)
       r2 = launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap, h->stream)
                ? fail(h, FDTD_ERR_HIP, "probe launch failed") : 0;
+    if (!r2) r2 = run_dft(h, 0, h->dstep);
     if (!r2) r2 = launch_tick(h->dstep, h->stream) ? fail(h, FDTD_ERR_HIP, "tick failed") : 0;
     hipGraph_t graph = nullptr;
     hipError_t ec = hipStreamEndCapture(h->stream, &graph);
@@ -1129,6 +1252,122 @@ int fdtd_media_paint_box(fdtd_handle *h, int32_t x0, int32_t x1, int32_t y0, int
   par[6] = (float)lx; par[7] = (float)ly; par[8] = (float)lz;
   par[9] = (float)(hx - lx + 1); par[10] = (float)(hy - ly + 1); par[11] = (float)(hz - lz + 1);
   return paint(h, 1, par, medium, nprio, prio);
+}
+
+}  // extern "C"
+
+// ---- checkpoint ------------------------------------------------------------
+// The complete time-stepping state of a grid: the six fields (Alloc boxes, as
+// StoreFields writes them, resuming.F90:394-496), the CPML psi arrays (the
+// reference's StoreFieldsCPMLBorders counterpart), the pole state (Eprev and
+// the pole currents, StoreFieldsEDispersives) and the frequency-domain sums,
+// behind a header naming the step.  The layout of the pole state depends on
+// the mode, so a state loads into a grid of the same set-up and mode.
+namespace {
+
+struct StateHeader {
+  char magic[4];
+  int32_t version;
+  int32_t mode;
+  int32_t nx, ny, nz;
+  int32_t step;
+  int32_t nsections;
+};
+
+struct Section {
+  void *dev;
+  size_t bytes;
+  int comp;   // >= 0: a field copied through its Alloc box
+};
+
+std::vector<Section> state_sections(fdtd_handle *h) {
+  std::vector<Section> v;
+  for (int c = 0; c < 6; ++c) {
+    long long ext[3];
+    alloc_extent(h, c, ext);
+    v.push_back({h->fld[c], (size_t)(ext[0] * ext[1] * ext[2]) * 4, c});
+  }
+  for (int s = 0; s < 12; ++s)
+    if (h->g.psi[s] && h->psilen[s] > 0) v.push_back({h->g.psi[s], (size_t)h->psilen[s] * 4, -1});
+  if (h->g.has_ade) {
+    int maxp = 0;
+    for (int d = 0; d < h->ndisp; ++d) maxp = std::max(maxp, h->dnp[d]);
+    const size_t js = h->mode == FDTD_MODE_EXACT ? 16 : 8;
+    for (int c = 0; c < 3; ++c) {
+      const size_t ns = (size_t)std::max(h->g.nstate[c], 1LL);
+      v.push_back({h->g.eprev[c], ns * 4, -1});
+      v.push_back({h->g.jst[c], ns * std::max(maxp, 1) * js, -1});
+    }
+  }
+  for (const DftDev &d : h->dfts)
+    v.push_back({d.acc, (size_t)d.cnt[0] * d.cnt[1] * d.cnt[2] * 3 * d.nfreq * sizeof(double2), -1});
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t fdtd_state_size(fdtd_handle *h) {
+  if (!h) return -1;
+  hipSetDevice(h->device);
+  if (prepare(h)) return -1;
+  int64_t n = sizeof(StateHeader);
+  for (const Section &s : state_sections(h)) n += (int64_t)s.bytes;
+  return n;
+}
+
+int fdtd_save_state(fdtd_handle *h, int32_t n, void *buf, int64_t size) {
+  if (!h || !buf) return fail(h, FDTD_ERR_ARG, "null state buffer");
+  const int64_t need = fdtd_state_size(h);
+  if (need < 0) return FDTD_ERR_STATE;
+  if (size < need) return fail(h, FDTD_ERR_ARG, "state needs %lld bytes, %lld given",
+                                 (long long)need, (long long)size);
+  std::vector<Section> secs = state_sections(h);
+  StateHeader hd{{'S', 'F', 'D', 'S'}, 1, h->mode, h->g.nx, h->g.ny, h->g.nz, n,
+                 (int32_t)secs.size()};
+  char *p = (char *)buf;
+  std::memcpy(p, &hd, sizeof hd);
+  p += sizeof hd;
+  for (const Section &s : secs) {
+    if (s.comp >= 0) {
+      int rc = copy_box(h, s.comp, s.dev, nullptr, p, 4);
+      if (rc) return rc;
+    } else {
+      HIPCHK(h, hipMemcpyAsync(p, s.dev, s.bytes, hipMemcpyDeviceToHost, h->stream));
+    }
+    p += s.bytes;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FDTD_OK;
+}
+
+int fdtd_load_state(fdtd_handle *h, const void *buf, int64_t size, int32_t *n) {
+  if (!h || !buf || !n) return fail(h, FDTD_ERR_ARG, "null state buffer");
+  const int64_t need = fdtd_state_size(h);
+  if (need < 0) return FDTD_ERR_STATE;
+  StateHeader hd;
+  if (size < (int64_t)sizeof hd) return fail(h, FDTD_ERR_ARG, "state too short");
+  std::memcpy(&hd, buf, sizeof hd);
+  std::vector<Section> secs = state_sections(h);
+  if (std::memcmp(hd.magic, "SFDS", 4) != 0 || hd.version != 1)
+    return fail(h, FDTD_ERR_ARG, "not a version-1 state");
+  if (hd.mode != h->mode || hd.nx != h->g.nx || hd.ny != h->g.ny || hd.nz != h->g.nz ||
+      hd.nsections != (int32_t)secs.size() || size != need)
+    return fail(h, FDTD_ERR_ARG, "state of a different grid, mode or set-up");
+  const char *p = (const char *)buf + sizeof hd;
+  for (const Section &s : secs) {
+    if (s.comp >= 0) {
+      int rc = copy_box(h, s.comp, s.dev, p, nullptr, 4);
+      if (rc) return rc;
+    } else {
+      HIPCHK(h, hipMemcpyAsync(s.dev, p, s.bytes, hipMemcpyHostToDevice, h->stream));
+    }
+    p += s.bytes;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  *n = hd.step;
+  return FDTD_OK;
 }
 
 }  // extern "C"

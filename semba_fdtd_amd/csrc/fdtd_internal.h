@@ -105,9 +105,48 @@ struct PwFace {
   int wave;
 };
 
+// One TF/SF face of one grid with everything its launch needs, so that the
+// faces of one group (one wave, one face axis) of many grids run in a single
+// launch: faces of a group never share a cell within a grid.
+struct PwRun {
+  PwFace f;
+  float *F;                   // the updated component
+  long long px, pxy;          // its row pitch and plane stride
+  const PwWave *wave;
+  const float *phys;
+  int pstride;
+  int half;
+  const double *tiempo;
+  const int *dn;              // device step counter (or null: the launch's n)
+  double dt;
+  float G;
+  int nu;                     // cells along the first free axis
+};
+
+// Frequency-domain observation box (observation.F90:3527-3600): complex(8)
+// running sums acc[pt][comp][freq] += aux[n][freq] * F_comp(pt), the points
+// being the stride multiples of the box (first[a] + t*stride[a], cnt[a] of them).
+struct DftDev {
+  int kind;                   // 0: Ex,Ey,Ez  1: Hx,Hy,Hz
+  int first[3], stride[3], cnt[3];
+  int nfreq;
+  const double2 *aux;         // [step][freq]
+  double2 *acc;               // [pt][3][freq], pt = (kt*cnt1 + jt)*cnt0 + it
+};
+
 struct ProbeDev {
   long long p;
   int comp;
+};
+
+// a batch of independent grids for the persistent face-pass launch: tile t of
+// the batch belongs to the grid b with start[b] <= t < start[b+1]
+struct BatchDev {
+  const DevGrid *grids;       // device copies of the grids' descriptors
+  const TileMap *tms;
+  const int *start;           // [ngrid+1] prefix of tile counts
+  int ngrid;
+  int total;
 };
 
 // launchers, one set per arithmetic mode
@@ -117,6 +156,8 @@ struct ProbeDev {
                int ade, void *stream);                                       \
   int launch_h(const DevGrid &g, const TileMap &tm, int yee, int cpml,       \
                void *stream);                                                \
+  int launch_pass_batch(const BatchDev &bd, int half, int blocks, int rows,  \
+                        size_t lds, void *stream);                           \
   }
 SFD_DECLARE_LAUNCHERS(fast)
 SFD_DECLARE_LAUNCHERS(exact)
@@ -126,12 +167,14 @@ int launch_nodal(const DevGrid &g, const NodalCell *cells, int ncells,
                  const float *values, int nsrc, int n, const int *dn, int serial,
                  void *stream);
 int launch_tick(int *dn, void *stream);
-// one TF/SF face at step n (or *dn): F += sign * G * Incid(t) * id, t the real(4)
-// time stamp of the half step (E: tiempo(n), H: tiempo(n) + dt/2).  phys holds
-// the physical coordinates of each (component, axis) at stride pstride, index i+1.
-int launch_tfsf(const DevGrid &g, const PwFace &f, const PwWave *waves, const float *phys,
-                int pstride, const double *tiempo, int n, const int *dn, int half, double dt,
-                float G, void *stream);
+int launch_dft(const DevGrid &g, const DftDev &d, int n, const int *dn, void *stream);
+// a group of TF/SF faces at step n (or each run's *dn): F += sign * G *
+// Incid(t) * id, t the real(4) time stamp of the half step (E: tiempo(n), H:
+// tiempo(n) + dt/2); runs[r] covers cells start[r] .. start[r+1]-1 of total
+int launch_tfsf_list(const PwRun *runs, const long long *start, int nruns, long long total,
+                     int n, void *stream);
+// increments every counter of the list (per-grid step counters of a batch)
+int launch_tick_many(int *const *counters, int n, void *stream);
 int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
                   int *count, int cap, void *stream);
 // bounding box {xmin,ymin,zmin,xmax,ymax,zmax} of the dispersive edges of

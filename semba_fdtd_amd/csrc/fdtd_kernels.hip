@@ -55,9 +55,8 @@ typedef float coef_t;
 constexpr int kPre = SFD_PREFETCH_POLES;
 #endif
 
-__device__ __forceinline__ void decode_tile(const TileMap &tm, int &bx, int &by,
+__device__ __forceinline__ void decode_tile(const TileMap &tm, int b, int &bx, int &by,
                                             int &bz) {
-  int b = blockIdx.x;
   if (tm.remap) {
     // hardware dispatches block b to XCD b % 8; give each XCD one contiguous
     // range of tiles so that x/y neighbours share that XCD's L2.
@@ -257,12 +256,12 @@ __global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
   }
 }
 
+// one tile (64 x blockDim.y lanes, one z chunk) of the E face pass
 template <bool YEE, bool CPML, bool TBL>
-__global__ void __launch_bounds__(256) k_epass(DevGrid g, TileMap tm) {
-  extern __shared__ float2 lds_tab[];
-  const MedTab tab = stage_tables<TBL, false>(g, g.g1, g.g2, lds_tab);
+__device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
+                                           const MedTab &tab, int blk) {
   int bx, by, bz;
-  decode_tile(tm, bx, by, bz);
+  decode_tile(tm, blk, bx, by, bz);
   const int i = bx * kWave + threadIdx.x;
   const int j = by * blockDim.y + threadIdx.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -364,12 +363,12 @@ __global__ void __launch_bounds__(256) k_epass(DevGrid g, TileMap tm) {
   }
 }
 
+// one tile of the H face pass
 template <bool YEE, bool CPML, bool TBL>
-__global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm) {
-  extern __shared__ float2 lds_tab[];
-  const MedTab tab = stage_tables<TBL, false>(g, g.gm1, g.gm2, lds_tab);
+__device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
+                                           const MedTab &tab, int blk) {
   int bx, by, bz;
-  decode_tile(tm, bx, by, bz);
+  decode_tile(tm, blk, bx, by, bz);
   const int i = bx * kWave + threadIdx.x;
   const int j = by * blockDim.y + threadIdx.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -452,6 +451,62 @@ __global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm) {
     ex_c = ex_kp;
     ey_c = ey_kp;
   }
+}
+
+template <bool YEE, bool CPML, bool TBL>
+__global__ void __launch_bounds__(256) k_epass(DevGrid g, TileMap tm) {
+  extern __shared__ float2 lds_tab[];
+  const MedTab tab = stage_tables<TBL, false>(g, g.g1, g.g2, lds_tab);
+  epass_tile<YEE, CPML, TBL>(g, tm, tab, blockIdx.x);
+}
+
+template <bool YEE, bool CPML, bool TBL>
+__global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm) {
+  extern __shared__ float2 lds_tab[];
+  const MedTab tab = stage_tables<TBL, false>(g, g.gm1, g.gm2, lds_tab);
+  hpass_tile<YEE, CPML, TBL>(g, tm, tab, blockIdx.x);
+}
+
+// ---- batches of independent grids ----------------------------------------
+// One persistent launch per half step covers every tile of every grid of the
+// batch: block b takes tiles b, b + gridDim.x, ...; the grid of a tile is
+// found by binary search in the prefix of tile counts, its media tables are
+// staged in LDS before the tile runs.
+__device__ __forceinline__ int batch_grid(const int *start, int ngrid, int t) {
+  int lo = 0, hi = ngrid - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool H>
+__global__ void __launch_bounds__(256) k_pass_batch(BatchDev bd) {
+  extern __shared__ float2 lds_tab[];
+  for (int t = blockIdx.x; t < bd.total; t += gridDim.x) {
+    const int b = batch_grid(bd.start, bd.ngrid, t);
+    const DevGrid &g = bd.grids[b];
+    const TileMap &tm = bd.tms[b];
+    __syncthreads();  // the previous tile is done with the LDS tables
+    const MedTab tab = H ? stage_tables<true, false>(g, g.gm1, g.gm2, lds_tab)
+                         : stage_tables<true, false>(g, g.g1, g.g2, lds_tab);
+    // the CPML variant serves grids without CPML too: with no slab layers
+    // no cell of the sweep falls in a slab
+    const int blk = t - bd.start[b];
+    if (H) hpass_tile<true, true, true>(g, tm, tab, blk);
+    else epass_tile<true, true, true>(g, tm, tab, blk);
+  }
+}
+
+int launch_pass_batch(const BatchDev &bd, int half, int blocks, int rows, size_t lds,
+                      void *stream) {
+  if (bd.total <= 0) return 0;
+  const dim3 blk(kWave, (unsigned)rows);
+  hipStream_t s = (hipStream_t)stream;
+  if (half) hipLaunchKernelGGL(k_pass_batch<true>, dim3(blocks), blk, lds, s, bd);
+  else hipLaunchKernelGGL(k_pass_batch<false>, dim3(blocks), blk, lds, s, bd);
+  return (int)hipGetLastError();
 }
 
 // media tables fit in LDS up to this many entries

@@ -61,6 +61,35 @@ class PlaneWave:
 
 
 @dataclass
+class DftBox:
+    """Frequency-domain observation of a box (observation.F90:3527-3600):
+    ``kind`` 0 accumulates Ex, Ey, Ez, 1 accumulates Hx, Hy, Hz, at the
+    points of ``box`` = (i1, i2, j1, j2, k1, k2) whose coordinates are
+    multiples of ``stride``, for the real(4) frequencies ``freqs`` [Hz], in
+    complex(8) running sums."""
+    kind: int
+    box: Tuple[int, int, int, int, int, int]
+    stride: Tuple[int, int, int]
+    freqs: np.ndarray
+
+    def __post_init__(self):
+        self.freqs = np.ascontiguousarray(self.freqs, np.float32)
+
+    def counts(self):
+        c = []
+        for a in range(3):
+            lo, hi, st = self.box[2 * a], self.box[2 * a + 1], self.stride[a]
+            c.append(sum(1 for v in range(lo, hi + 1) if v % st == 0))
+        return tuple(c)
+
+    @property
+    def shape(self):
+        """(nk, nj, ni, 3, nfreq): the running sums, complex128."""
+        ni, nj, nk = self.counts()
+        return (nk, nj, ni, 3, len(self.freqs))
+
+
+@dataclass
 class Problem:
     nx: int
     ny: int
@@ -87,6 +116,7 @@ class Problem:
     nodal: List[NodalSource] = field(default_factory=list)
     probes: List[Tuple[int, int, int, int]] = field(default_factory=list)
     planewaves: List[PlaneWave] = field(default_factory=list)
+    dfts: List[DftBox] = field(default_factory=list)
     nsteps_max: int = 0
     eps0: float = EPS0
     mu0: float = MU0
@@ -480,3 +510,104 @@ def open_scatterer_problem(n: int = 256, L: int = 10, radius: float = 40.0, grad
                    cpml_layers=[[L, L]] * 3, cpml_order=[[3.0, 3.0]] * 3,
                    cpml_refl=[[1e-6, 1e-6]] * 3, planewaves=[pw], probes=probes,
                    nsteps_max=nsteps, name=f"open_scatterer{n}")
+
+
+def ellipsoid_cells(n, centre, axes) -> np.ndarray:
+    """Cells whose centre lies inside the axis-aligned ellipsoid, in real(4)."""
+    nx, ny, nz = n
+    f = np.float32
+    k, j, i = np.meshgrid(np.arange(nz, dtype=f) + f(0.5), np.arange(ny, dtype=f) + f(0.5),
+                          np.arange(nx, dtype=f) + f(0.5), indexing="ij")
+    q = [((v - f(c)) / f(a)) for v, c, a in zip((i, j, k), centre, axes)]
+    return q[0] * q[0] + q[1] * q[1] + q[2] * q[2] <= f(1.0)
+
+
+def blob_problem(n, nsteps: int, rng, L: int = 8, nblobs=(1, 6), eps_range=(1.0, 10.0),
+                 dispersive: bool = False, dx: float = 0.01, nprobes: int = 4,
+                 nfreq: int = 0, name: str = "blobs") -> Problem:
+    """An independent scattering solve: n = (nx, ny, nz) total cells with an
+    L-cell CPML, a few seeded dielectric ellipsoids (eps_r U[eps_range]), an
+    optional 1-pole (Debye-like) dispersive box, a TF/SF Gaussian plane wave
+    (+z, Ex) two cells inside the CPML, point probes and, with nfreq > 0, a
+    frequency-domain box over the probe region at nfreq frequencies."""
+    nx, ny, nz = n
+    mi = new_media_index(nx, ny, nz)
+    nb = int(rng.integers(nblobs[0], nblobs[1] + 1))
+    epr = [1.0, 1.0] + [float(rng.uniform(*eps_range)) for _ in range(nb)]
+    sig = [0.0] * len(epr)
+    kind = [KIND_PEC, 0] + [0] * nb
+    prio = [100, 0] + [10 + b for b in range(nb)]
+    lo = [L + 2] * 3
+    hi = [v - L - 2 for v in n]
+    for b in range(nb):
+        axes = [float(rng.uniform(2.0, max(2.5, 0.25 * (h - l)))) for l, h in zip(lo, hi)]
+        ctr = [float(rng.uniform(l + a + 1, max(l + a + 1.5, h - a - 1)))
+               for l, h, a in zip(lo, hi, axes)]
+        paint_voxels(mi, ellipsoid_cells(n, ctr, axes), 2 + b, np.array(prio + [0], np.int64))
+    npoles = [0] * len(epr)
+    pc = np.zeros((len(epr) + 1, PMAX, 2)); pa = np.zeros((len(epr) + 1, PMAX, 2))
+    eps11 = np.zeros(len(epr) + 1, np.float32); sig11 = np.zeros(len(epr) + 1, np.float32)
+    if dispersive:
+        md = len(epr)
+        epr.append(2.0); sig.append(0.0); kind.append(KIND_EDISP); npoles.append(1)
+        prio.append(5)
+        a = -2.0 * np.pi * float(rng.uniform(1e9, 10e9))     # real pole
+        pa[md, 0] = (a, 0.0)
+        pc[md, 0] = (-1.5 * EPS0 * a, 0.0)                     # static increment 1.5
+        eps11[md] = np.float32(2.0 * EPS0)
+        box = np.zeros((nz, ny, nx), bool)
+        box[lo[2] + 1:hi[2] - 1, lo[1] + 1:(lo[1] + hi[1]) // 2, lo[0] + 1:hi[0] - 1] = True
+        paint_voxels(mi, box, md, np.array(prio, np.int64))
+    nm = len(epr) - 1
+    D = [uniform_D(v, dx) for v in n]
+    dt = courant_dt((dx, dx, dx), cfl=0.9)
+    evol = gaussian_table(dt, nsteps + 4, 20.0, 80.0)
+    pw = PlaneWave((L + 2, nx - L - 2, L + 2, ny - L - 2, L + 2, nz - L - 2), [(0.0, 0.0, 1.0)],
+                   [(1.0, 0.0, 0.0)], evol, float(np.float32(dt)))
+    probes = []
+    for p in range(nprobes):
+        probes.append((p % 6,) + tuple(int(rng.integers(lo[a], hi[a])) for a in range(3)))
+    dfts = []
+    if nfreq > 0:
+        c = [v // 2 for v in n]
+        freqs = np.linspace(0.5e9, 8e9, nfreq).astype(np.float32)
+        dfts.append(DftBox(0, (c[0] - 1, c[0] + 1, c[1] - 1, c[1] + 1, c[2], c[2]), (1, 1, 1),
+                           freqs))
+    return Problem(nx, ny, nz, epr=epr, sigma=sig, mur=[1.0] * (nm + 1), sigmam=[0.0] * (nm + 1),
+                   kind=kind, mi=mi, D=D, dt=dt, npoles=np.array(npoles[:nm + 1]),
+                   pole_c=pc[:nm + 1], pole_a=pa[:nm + 1], eps11=eps11[:nm + 1],
+                   sig11=sig11[:nm + 1], cpml_layers=[[L, L]] * 3, cpml_order=[[3.0, 3.0]] * 3,
+                   cpml_refl=[[1e-6, 1e-6]] * 3, planewaves=[pw], probes=probes, dfts=dfts,
+                   nsteps_max=nsteps, name=name)
+
+
+def batch_instances(count: int = 256, nsteps: int = 1500, seed: int = 1004, lo: int = 32,
+                    hi: int = 128):
+    """Batched uneven instances: edge lengths log-uniform in [lo, hi] per
+    axis, 8-cell CPML, 1-6 ellipsoidal blobs (eps_r U[1,10]), a plane wave."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(count):
+        n = tuple(int(round(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(3))
+        out.append(blob_problem(n, nsteps, rng, name=f"instance{q}"))
+    return out
+
+
+def serving_requests(count: int = 48, rate: float = 8.0, seed: int = 1005, nmin: int = 48,
+                     nmax: int = 192, steps=(500, 4000), nfreq: int = 64,
+                     disp_share: float = 0.3):
+    """Serving stream: Poisson arrivals at ``rate`` per second of cubic solves
+    U{nmin..nmax}^3, U[steps] steps, 8-cell CPML, random blobs, a 1-pole
+    dispersive box in ``disp_share`` of the requests and a 64-frequency
+    FP64 DFT over the centre.  Returns [(arrival_s, Problem, nsteps)]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for q in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        n = int(rng.integers(nmin, nmax + 1))
+        ns = int(rng.integers(steps[0], steps[1] + 1))
+        pb = blob_problem((n, n, n), ns, rng, dispersive=bool(rng.random() < disp_share),
+                          nfreq=nfreq, name=f"request{q}")
+        out.append((t, pb, ns))
+    return out

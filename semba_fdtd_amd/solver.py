@@ -180,6 +180,15 @@ class Solver:
             self._ck(lib.fdtd_set_probes(h, self.nprobe, ptr(P[0], i32p), ptr(P[1], i32p),
                                          ptr(P[2], i32p), ptr(P[3], i32p), probe_buffer),
                      "fdtd_set_probes")
+        self.dft_ids = []
+        for d in pb.dfts:
+            box = np.array(d.box, np.int32)
+            st = np.array(d.stride, np.int32)
+            did = C.c_int32()
+            self._ck(lib.fdtd_add_dft(h, int(d.kind), ptr(box, i32p), ptr(st, i32p),
+                                      len(d.freqs), ptr(d.freqs, f32p), C.byref(did)),
+                     "fdtd_add_dft")
+            self.dft_ids.append(did.value)
         self.n = 0
         self._samples = []
 
@@ -218,10 +227,21 @@ class Solver:
         """One iteration of ciclo_temporal, then UpdateObservation."""
         self.advance_e(self.n)
         self.advance_h(self.n)
-        if self.nprobe:
+        if self.nprobe or self.dft_ids:
             self._drain_if_full(1)
-            self._ck(self.lib.fdtd_sample_probes(self.h), "fdtd_sample_probes")
+            self._ck(self.lib.fdtd_update_observation(self.h, self.n), "fdtd_update_observation")
         self.n += 1
+
+    def dft_sums(self):
+        """Running sums of each DftBox of the problem, complex128 arrays of
+        DftBox.shape."""
+        out = []
+        for d, did in zip(self.pb.dfts, self.dft_ids):
+            n = int(self.lib.fdtd_dft_size(self.h, did))
+            buf = np.zeros((n, 2))
+            self._ck(self.lib.fdtd_read_dft(self.h, did, ptr(buf, f64p), n), "fdtd_read_dft")
+            out.append((buf[:, 0] + 1j * buf[:, 1]).reshape(d.shape))
+        return out
 
     def run(self, nsteps: int, chunk: int | None = None):
         """nsteps whole steps from the captured graph; probes are drained from
@@ -274,6 +294,23 @@ class Solver:
         """Load component c from a uniform (nz+3, ny+3, nx+3) array."""
         box = to_alloc_box(np.asarray(a, np.float32), c, self.n3)
         self._ck(self.lib.fdtd_set_field(self.h, c, ptr(box, f32p)), "fdtd_set_field")
+
+    def save_state(self) -> bytes:
+        """The whole time-stepping state at the current step (a checkpoint)."""
+        self._drain()
+        size = int(self.lib.fdtd_state_size(self.h))
+        if size < 0:
+            raise FdtdError(f"fdtd_state_size failed: {self.lib.fdtd_last_error(self.h).decode()}")
+        buf = C.create_string_buffer(size)
+        self._ck(self.lib.fdtd_save_state(self.h, self.n, buf, size), "fdtd_save_state")
+        return buf.raw
+
+    def load_state(self, state: bytes):
+        """Resume from a checkpoint of a grid with the same set-up and mode."""
+        n = C.c_int32()
+        buf = C.create_string_buffer(state, len(state))
+        self._ck(self.lib.fdtd_load_state(self.h, buf, len(state), C.byref(n)), "fdtd_load_state")
+        self.n = n.value
 
     @property
     def device_bytes(self) -> int:

@@ -1,0 +1,91 @@
+"""Batches of uneven grids (persistent face-pass launch, graph-replayed step,
+continuous admission): every grid's probes, fields and frequency-domain sums
+equal those of the same grid advanced alone, bit for bit, in both modes."""
+import numpy as np
+import pytest
+
+import semba_fdtd_amd as sf
+from semba_fdtd_amd import problem as P
+from semba_fdtd_amd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    a = P.scatterer_problem(nsteps=150)                       # CPML, poles, nodal E and H
+    b = P.open_scatterer_problem(n=30, L=5, radius=4, nsteps=120, spread=8, delay=30,
+                                 grading=1.05, direction=(0.3, -0.5, 0.8),
+                                 polar=(0.0, 0.848, 0.53))    # TF/SF, graded z
+    b.dfts = [P.DftBox(0, (8, 20, 9, 18, 10, 12), (2, 1, 1),
+                       np.array([0.5e9, 2e9, 3.5e9], np.float32))]
+    c = P.cavity_problem(n=20, nsteps=100)                    # no CPML, hard Ez
+    return [(a, 150), (b, 120), (c, 100)]
+
+
+def _state(s):
+    return s.probes(), [s.field(k) for k in range(6)], s.dft_sums()
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_batch_equals_alone(mode):
+    print(f"batch {mode}", flush=True)
+    probs = _problems()
+    alone = []
+    for pb, ns in probs:
+        s = sf.Solver(pb, mode=mode)
+        s.run(ns)
+        alone.append(_state(s))
+        s.close()
+    sol = [sf.Solver(pb, mode=mode) for pb, _ in probs]
+    batch = Batch(rows=2, zchunk=8, blocks=64)
+    sa = batch.admit(sol[0])
+    sb = batch.admit(sol[1])
+    batch.step(20)
+    sc = batch.admit(sol[2])                       # joins while the others run
+    batch.step(100)
+    batch.retire(sb)
+    batch.retire(sc)
+    batch.step(30)
+    batch.retire(sa)
+    assert [s.n for s in sol] == [150, 120, 100]
+    for q, s in enumerate(sol):
+        got = _state(s)
+        want = alone[q]
+        assert np.array_equal(got[0].view(np.uint32), want[0].view(np.uint32)), q
+        for k in range(6):
+            assert np.array_equal(got[1][k].view(np.uint32), want[1][k].view(np.uint32)), (q, k)
+        for g, w in zip(got[2], want[2]):
+            assert np.array_equal(g.view(np.uint64), w.view(np.uint64)), q
+    batch.close()
+
+
+def test_retired_grid_continues_alone():
+    pb = P.scatterer_problem(nsteps=80)
+    ref = sf.Solver(pb, mode="exact")
+    ref.run(80)
+    s = sf.Solver(pb, mode="exact")
+    batch = Batch()
+    slot = batch.admit(s)
+    batch.step(30)
+    batch.retire(slot)
+    s.run(50)                                      # back on its own graph
+    assert np.array_equal(s.probes().view(np.uint32), ref.probes().view(np.uint32))
+    batch.close()
+
+
+def test_serving_results_equal_plain():
+    """Continuous batching: requests arriving while others run give the same
+    probe series and frequency-domain sums as the one-at-a-time path."""
+    from semba_fdtd_amd import serve
+    reqs = P.serving_requests(count=5, rate=40.0, nmin=24, nmax=36, steps=(40, 120), nfreq=8)
+    plain = {r.name: r for r in serve.serve_plain(reqs, mode="fast")}
+    batched = {r.name: r for r in serve.serve_batched(reqs, mode="fast", max_chunk=25)}
+    assert set(plain) == set(batched)
+    for name in plain:
+        a, b = plain[name], batched[name]
+        assert a.probes.shape == b.probes.shape and a.probes.shape[0] == a.steps
+        assert np.array_equal(a.probes.view(np.uint32), b.probes.view(np.uint32)), name
+        for x, y in zip(a.dft, b.dft):
+            assert np.array_equal(x.view(np.uint64), y.view(np.uint64)), name
+    s = serve.summarize(list(batched.values()))
+    assert s["requests"] == 5 and s["p99_latency_s"] >= s["p50_latency_s"] > 0

@@ -207,3 +207,34 @@ def test_full_size_linearity(mode):
     big = np.abs(pa) > 1e-30
     assert np.array_equal((2.0 * pa)[big].view(np.uint32), pbb[big].view(np.uint32))
     assert np.abs(2.0 * pa - pbb).max() <= 1e-35
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_dft_sums(mode):
+    """Frequency-domain boxes (E and H, strided, 4 frequencies) through the
+    graph and through step-wise UpdateObservation: EXACT mode equals the
+    oracle's running sums bit for bit; FAST mode within the FP32 tolerance
+    of the FP64 oracle."""
+    print(f"dft {mode}", flush=True)
+    ns = 150
+    pb = P.open_scatterer_problem(n=28, L=4, radius=3, nsteps=ns, spread=8, delay=30)
+    freqs = np.array([0.3e9, 1.1e9, 2.5e9, 4.0e9], np.float32)
+    pb.dfts = [P.DftBox(0, (6, 20, 7, 19, 9, 14), (2, 3, 1), freqs),
+               P.DftBox(1, (5, 21, 8, 12, 10, 11), (1, 2, 1), freqs[::-1].copy())]
+    ref = oracle.Oracle(pb, precision=32 if mode == "exact" else 64)
+    ref.run(ns)
+    want = ref.dft_sums()
+    a = sf.Solver(pb, mode=mode)
+    a.run(ns)
+    b = sf.Solver(pb, mode=mode)
+    for _ in range(ns):
+        b.step()
+    for q in range(len(pb.dfts)):
+        ga, gb = a.dft_sums()[q], b.dft_sums()[q]
+        assert np.array_equal(ga.view(np.uint64), gb.view(np.uint64))
+        if mode == "exact":
+            assert np.array_equal(ga.view(np.uint64), want[q].view(np.uint64))
+        else:
+            err = np.linalg.norm(ga - want[q]) / np.linalg.norm(want[q])
+            print(f"dft box {q} rel-L2 vs FP64 {err:.3e}", flush=True)
+            assert err <= 1e-5

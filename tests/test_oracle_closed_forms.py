@@ -19,7 +19,10 @@ the oracle's Yee updates, CPML and pole-residue tables with closed forms:
 * TF/SF plane wave in vacuum: inside the box the total field is the incident
   Gaussian delayed by z/c, outside it (the scattered-field region) the field
   stays at the level of the grid's numerical dispersion;
-* discretisation lines: cumulative cell sizes from the first interior node.
+* discretisation lines: cumulative cell sizes from the first interior node;
+* frequency-domain probes: the running sums equal a direct FP64 sum
+  dt * sum_n F(t_n) exp(-j 2 pi f t_n) over the recorded time series (H at
+  t_n + dt/2), with the reference's real(4) 2*pi and frequencies.
 """
 import numpy as np
 import pytest
@@ -267,3 +270,33 @@ def test_lines_closed_form():
     t = __import__("semba_fdtd_amd").build_tables(o.pb)
     for a in range(3):
         assert np.array_equal(o.lines[a].view(np.uint32), t.lines[a].view(np.uint32))
+
+
+def test_dft_running_sums_equal_direct_sum():
+    pb = P.open_scatterer_problem(n=24, L=4, radius=3, nsteps=120, spread=8, delay=30)
+    freqs = np.array([0.3e9, 1.1e9, 2.5e9, 4.0e9], np.float32)
+    pb.dfts = [P.DftBox(0, (6, 17, 7, 16, 9, 14), (2, 3, 1), freqs),
+               P.DftBox(1, (5, 18, 8, 12, 10, 11), (1, 2, 1), freqs[::-1].copy())]
+    o = oracle.Oracle(pb, precision=64)
+    rec = [[] for _ in pb.dfts]
+    for s in range(120):
+        o.step()
+        for q, d in enumerate(pb.dfts):
+            i1, i2, j1, j2, k1, k2 = d.box
+            pts = [(i, j, k) for k in range(k1, k2 + 1) if k % d.stride[2] == 0
+                   for j in range(j1, j2 + 1) if j % d.stride[1] == 0
+                   for i in range(i1, i2 + 1) if i % d.stride[0] == 0]
+            rec[q].append([[o.fields[3 * d.kind + c][k + 1, j + 1, i + 1] for c in range(3)]
+                           for (i, j, k) in pts])
+    twopi = np.float64(np.float32(2.0) * np.float32(np.pi))
+    tn = o.tiempo[:120]
+    for q, d in enumerate(pb.dfts):
+        series = np.array(rec[q])                      # [step][point][comp]
+        f = d.freqs.astype(np.float64)
+        t = tn + (0.5 * pb.dt if d.kind == 1 else 0.0)
+        ph = np.exp(-1j * twopi * np.outer(t, f))       # [step][freq]
+        want = pb.dt * np.einsum("npc,nf->pcf", series, ph)
+        got = o.dft_sums()[q].reshape(-1, 3, len(f))
+        scale = np.abs(want).max()
+        assert scale > 0
+        assert np.abs(got - want).max() <= 1e-9 * scale, np.abs(got - want).max() / scale
