@@ -56,14 +56,16 @@ enum {
 enum { FDTD_MODE_FAST = 0, FDTD_MODE_EXACT = 1 };
 
 /* Field storage in HBM.  FP16 keeps each field as binary16 times a per-array
- * scale with fp32 arithmetic in registers (CPML psi and pole state stay fp32). */
+ * scale with fp32 arithmetic in registers (CPML psi and pole state stay fp32);
+ * FAST mode only; default scales 1 for E and 1/376.73 for H
+ * (fdtd_set_field_scale).  The granular entry points need FP32 storage. */
 enum { FDTD_STORE_FP32 = 0, FDTD_STORE_FP16 = 1 };
 
 /* field components */
 enum { FDTD_EX = 0, FDTD_EY, FDTD_EZ, FDTD_HX, FDTD_HY, FDTD_HZ };
 
 /* media kind bits (per media index) */
-enum { FDTD_KIND_PEC = 1, FDTD_KIND_PMC = 2, FDTD_KIND_EDISP = 4 };
+enum { FDTD_KIND_PEC = 1, FDTD_KIND_PMC = 2, FDTD_KIND_EDISP = 4, FDTD_KIND_MDISP = 8 };
 
 typedef struct fdtd_handle fdtd_handle;
 
@@ -116,6 +118,17 @@ int fdtd_set_cpml(fdtd_handle *h, const int32_t layers[6],
 int fdtd_set_edispersive(fdtd_handle *h, int32_t ndisp, const int32_t *medium,
                          const int32_t *npoles, const double *kappa,
                          const double *beta, const double *g3);
+/* Magnetic pole-residue media (InitMdispersives, magneticdispersive.F90:
+ * 77-290): the same recursion on H with mu11/Sigmam11, coefficients
+ * kappa_p, beta_p, GM3_p as [ndisp][8][re,im].  Pole state starts at zero. */
+int fdtd_set_mdispersive(fdtd_handle *h, int32_t ndisp, const int32_t *medium,
+                         const int32_t *npoles, const double *kappa,
+                         const double *beta, const double *gm3);
+/* First-order Mur absorbing faces (InitMURBorders, bordersmur.F90:79-314):
+ * faces[6] = x-, x+, y-, y+, z-, z+ (nonzero: absorbing), cab1 the
+ * [6][nmedia+1] coefficients of fdtd_build_mur.  Call after fdtd_set_media.
+ * The stored previous-step planes start at zero. */
+int fdtd_set_mur(fdtd_handle *h, const int32_t faces[6], const float *cab1);
 /* Time step and time vector (crea_timevector, timestepping.F90:3230-3240):
  * tiempo[0..nlast]; NULL builds tiempo(n) = tiempo(n-1) + dt from 0. */
 int fdtd_set_time(fdtd_handle *h, double dt, int32_t nlast, const double *tiempo);
@@ -141,11 +154,20 @@ int fdtd_add_planewave(fdtd_handle *h, const int32_t box[6], int32_t nmodes, con
                        const float *evol, float eps0, float mu0);
 
 /* Work decomposition of the face passes: blocks of 64 x rows threads (rows =
- * 1, 2 or 4) that march zchunk z planes each.  No counterpart in the
+ * 1, 2 or 4) that march zchunk z planes each, every lane owning xcells (1 or
+ * 4) consecutive x cells of the fused passes.  No counterpart in the
  * reference (its loops are OpenMP over k); chosen by the autotuner
- * (semba_fdtd_amd.tuning) or left at the default 4 rows x 32 planes. */
-int fdtd_set_tiling(fdtd_handle *h, int32_t rows, int32_t zchunk);
-int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk);
+ * (semba_fdtd_amd.tuning) or left at the default 4 rows x 32 planes x 1 cell.
+ * Binary16 storage always runs 4 cells per lane. */
+int fdtd_set_tiling(fdtd_handle *h, int32_t rows, int32_t zchunk, int32_t xcells);
+int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk, int32_t *xcells);
+
+/* Average duration in ms, over reps launches on the grid's stream, of the
+ * fused E pass with its pole update (ms[0]), the fused H pass (ms[1]) and the
+ * pole update alone (ms[2]); sources and probes are not run.  The fields
+ * advance while this runs: save the state first to keep it.  No counterpart
+ * in the reference; used by the autotuner. */
+int fdtd_time_halves(fdtd_handle *h, int32_t reps, float ms[3]);
 
 /* ---- the hot path ----------------------------------------------------- */
 /* Fused E half step: Advance_Ex/Ey/Ez (timestepping.F90:1286-1297) +
@@ -166,8 +188,10 @@ int fdtd_advance_planewave_e(fdtd_handle *h, int32_t n); /* AdvancePlaneWaveE */
 int fdtd_advance_nodal_e(fdtd_handle *h, int32_t n); /* AdvanceNodalE */
 int fdtd_advance_h_yee(fdtd_handle *h);        /* Advance_Hx/Hy/Hz */
 int fdtd_advance_h_cpml(fdtd_handle *h);       /* AdvanceMagneticCPML */
+int fdtd_advance_h_dispersive(fdtd_handle *h); /* AdvanceMdispersiveH */
 int fdtd_advance_planewave_h(fdtd_handle *h, int32_t n); /* AdvancePlaneWaveH */
 int fdtd_advance_nodal_h(fdtd_handle *h, int32_t n); /* AdvanceNodalH */
+int fdtd_advance_mur(fdtd_handle *h);          /* AdvanceMagneticMUR */
 /* nsteps full steps n0 .. n0+nsteps-1 (E half then H half, probes sampled and
  * frequency-domain sums advanced after each step as UpdateObservation does,
  * timestepping.F90:1725-1742), replayed from a captured HIP graph. */
@@ -204,6 +228,9 @@ int fdtd_update_observation(fdtd_handle *h, int32_t n);
 /* ---- field access ----------------------------------------------------- */
 int fdtd_get_field(fdtd_handle *h, int32_t comp, float *host_alloc);
 int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host_alloc);
+/* Scale of a binary16 field array (value = stored * scale); stored values keep
+ * their physical value.  No effect on FP32 storage. */
+int fdtd_set_field_scale(fdtd_handle *h, int32_t comp, float scale);
 int fdtd_synchronize(fdtd_handle *h);
 /* Checkpoint / resume (StoreFields and the modules' restart files,
  * resuming.F90:394-496): the whole time-stepping state -- fields, CPML psi,
@@ -215,9 +242,10 @@ int fdtd_save_state(fdtd_handle *h, int32_t n, void *buf, int64_t size);
 int fdtd_load_state(fdtd_handle *h, const void *buf, int64_t size, int32_t *n);
 /* Device bytes held by the handle. */
 int64_t fdtd_device_bytes(const fdtd_handle *h);
-/* Number of dispersive E edges of component comp (0..2) holding pole state
- * (the node lists of InitEDispersives, electricdispersive.F90:148-244);
- * valid after the first advance/run call, -1 before. */
+/* Number of dispersive edges of component comp (0..2 E, 3..5 H) holding pole
+ * state (the node lists of InitEDispersives, electricdispersive.F90:148-244,
+ * and InitMdispersives, magneticdispersive.F90:139-265); valid after the
+ * first advance/run call, -1 before. */
 int64_t fdtd_dispersive_edges(const fdtd_handle *h, int32_t comp);
 
 /* ---- host-side coefficient tables (built once, uploaded with the set-up
@@ -249,6 +277,13 @@ int fdtd_build_cpml_axis(int32_t n, int32_t Llo, int32_t Lhi, float orden_lo,
 int fdtd_build_ade(int32_t npoles, const double *c, const double *a, float eps11,
                    float sigma11, double dt, double *kappa, double *beta, double *g3,
                    float *g1, float *g2);
+/* First-order Mur coefficients CAB1 (calc_murconstants, bordersmur.F90:
+ * 316-363) for the six faces x-, x+, y-, y+, z-, z+: idh_face holds the dual
+ * inverse spacing at the face's end of the Alloc range (Idxh(-1), Idxh(nx),
+ * Idyh(-1), Idyh(ny), Idzh(-1), Idzh(nz), after any CPML stretching);
+ * cab1 is [6][nmedia+1]. */
+int fdtd_build_mur(int32_t nmedia, const float *epr, const float *mur, double dt, float eps0,
+                   float mu0, const float idh_face[6], float *cab1);
 /* evolucion, nodalsources.F90:317-356. */
 float fdtd_evolucion(float t, float deltaevol, int32_t numus, const float *evol);
 
@@ -270,9 +305,11 @@ int fdtd_batch_admit(fdtd_batch *b, fdtd_handle *h, int32_t n0, int32_t *slot);
 int fdtd_batch_retire(fdtd_batch *b, int32_t slot);
 /* Step the grid of slot will run next (-1 for an empty slot). */
 int64_t fdtd_batch_step_of(const fdtd_batch *b, int32_t slot);
-/* Tiling of the batched face passes (rows 1, 2 or 4; zchunk planes) and the
- * persistent launch's block count (0: 2048). */
-int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t blocks);
+/* Tiling of the batched face passes (rows 1, 2 or 4; zchunk planes; xcells 1
+ * or 4 x cells per lane, always 4 with binary16 storage) and the persistent
+ * launch's block count (0: 2048). */
+int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t xcells,
+                          int32_t blocks);
 /* nsteps steps of every admitted grid. */
 int fdtd_batch_step(fdtd_batch *b, int32_t nsteps);
 int fdtd_batch_synchronize(fdtd_batch *b);

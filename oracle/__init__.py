@@ -12,8 +12,9 @@ Two checkers live here:
   (``liboracle_fdtd64.so``, the FP64 yardstick, ``precision=64``).
 * ``Reference`` -- ctypes wrapper of ``_ref/libsemba_ref.so``: the reference's
   own Fortran modules (calc_constants, borderscpml, electricdispersive,
-  nodalsources) compiled from their sources by ``oracle/Makefile``, driven by
-  ``ref_harness.F90``.  Only built where the reference tree is present.
+  magneticdispersive, bordersmur, nodalsources, planewaves) compiled from
+  their sources by ``oracle/Makefile``, driven by ``ref_harness.F90``.  Only
+  built where the reference tree is present.
 
 Both consume a ``semba_fdtd_amd.problem.Problem`` (plain numpy data: media
 tables, int16 media-index arrays in the Alloc layout, cell sizes, CPML
@@ -75,7 +76,12 @@ def _structs(prec):
                     ("nnodal", C.c_int32), ("nodal", C.POINTER(OrcNodal)),
                     ("medkind", _i32p),
                     ("npw", C.c_int32), ("pw", C.POINTER(OrcPW)), ("line", rp * 3),
-                    ("eps0", rt), ("mu0", rt), ("pws", C.c_void_p)]
+                    ("eps0", rt), ("mu0", rt), ("pws", C.c_void_p),
+                    ("nmdisp", C.c_int32), ("mdisp_medium", _i32p),
+                    ("mdisp_npoles", _i32p), ("mdisp_kappa", _f64p),
+                    ("mdisp_beta", _f64p), ("mdisp_gm3", _f64p),
+                    ("hprev", rp * 3), ("mjcur", _f64p * 3),
+                    ("mur", C.c_int32 * 6), ("cab1", rp * 6), ("murpast", rp * 3)]
     return OrcNodal, OrcGrid, OrcPW
 
 
@@ -110,9 +116,11 @@ def load_oracle(precision: int = 32):
     lib.orc_ade_tables.argtypes = [C.c_int32, _f64p, _f64p, rt, rt, C.c_double,
                                    _f64p, _f64p, _f64p, rp, rp]
     lib.orc_time_vector.argtypes = [C.c_int32, C.c_double, _f64p]
+    lib.orc_mur_table.argtypes = [C.c_int32, rp, rp, C.c_double, rt, rt, rt, rp]
     lib.orc_evolucion.restype = rt
     lib.orc_evolucion.argtypes = [rt, C.POINTER(Nod)]
-    for fn in ("orc_yee_e", "orc_yee_h", "orc_cpml_e", "orc_cpml_h", "orc_ade_e"):
+    for fn in ("orc_yee_e", "orc_yee_h", "orc_cpml_e", "orc_cpml_h", "orc_ade_e", "orc_ade_h",
+               "orc_mur_h"):
         getattr(lib, fn).argtypes = [C.POINTER(Grid)]
     for fn in ("orc_nodal_e", "orc_nodal_h", "orc_step", "orc_pw_e", "orc_pw_h"):
         getattr(lib, fn).argtypes = [C.POINTER(Grid), C.c_int32]
@@ -133,12 +141,20 @@ class Tables:
     """Coefficient tables of one problem (as the reference builds them)."""
 
     def __init__(self, g1, g2, gm1, gm2, ide, idh, pbe, pce, pbm, pcm,
-                 kappa, beta, g3, disp_media, disp_npoles):
+                 kappa, beta, g3, disp_media, disp_npoles, mkappa=None, mbeta=None,
+                 gm3=None, mdisp_media=None, mdisp_npoles=None, cab1=None):
         self.g1, self.g2, self.gm1, self.gm2 = g1, g2, gm1, gm2
         self.ide, self.idh = ide, idh          # lists of 3 arrays (x,y,z)
         self.pbe, self.pce, self.pbm, self.pcm = pbe, pce, pbm, pcm
         self.kappa, self.beta, self.g3 = kappa, beta, g3
         self.disp_media, self.disp_npoles = disp_media, disp_npoles
+        z = np.zeros((1, PMAX, 2))
+        self.mkappa = z if mkappa is None else mkappa
+        self.mbeta = z if mbeta is None else mbeta
+        self.gm3 = z if gm3 is None else gm3
+        self.mdisp_media = np.zeros(0, np.int32) if mdisp_media is None else mdisp_media
+        self.mdisp_npoles = np.zeros(0, np.int32) if mdisp_npoles is None else mdisp_npoles
+        self.cab1 = cab1                       # [6][nmedia+1] or None
 
 
 def _constants(pb, precision):
@@ -167,7 +183,9 @@ def oracle_tables(lib, pb, precision: int = 32) -> Tables:
         lib.orc_spacing(n, _p(D, rp), _p(de, rp), _p(dh, rp), _p(Ie, rp), _p(Ih, rp))
         b1, c1, b2, c2 = f(n + 3), f(n + 3), f(n + 3), f(n + 3)
         L = pb.cpml_layers[a]
-        if L[0] or L[1]:
+        # InitCPMLBorders computes the profiles of all three axes as soon as
+        # one face has layers (borderscpml.F90:228-300, 1465-1733)
+        if pb.has_cpml:
             lib.orc_cpml_axis(n, int(L[0]), int(L[1]), float(pb.cpml_order[a][0]),
                               float(pb.cpml_order[a][1]), float(pb.cpml_refl[a][0]),
                               float(pb.cpml_refl[a][1]), pb.alphamax, pb.alphaorden,
@@ -193,8 +211,37 @@ def oracle_tables(lib, pb, precision: int = 32) -> Tables:
                            _p(gg, _f64p), _p(a1, rp), _p(a2, rp))
         kappa[d], beta[d], g3[d] = kk, bb, gg
         g1[m], g2[m] = a1[0], a2[0]
+    # magnetic media after the electric ones (InitMDispersives follows
+    # InitEDispersives, timestepping.F90:927-943)
+    mdisp = [m for m in range(1, nm + 1) if pb.kind[m] & 8]
+    nmd = len(mdisp)
+    mkappa = np.zeros((max(nmd, 1), PMAX, 2)); mbeta = np.zeros_like(mkappa)
+    gm3 = np.zeros_like(mkappa)
+    for d, m in enumerate(mdisp):
+        a1, a2 = f(1), f(1)
+        cc = np.ascontiguousarray(pb.mpole_c[m]); aa = np.ascontiguousarray(pb.mpole_a[m])
+        kk = np.zeros((PMAX, 2)); bb = np.zeros((PMAX, 2)); gg = np.zeros((PMAX, 2))
+        mu11 = float(pb.mu11[m]) if precision == 32 else float(pb.mu11[m]) / pb.mu0 * MU0_64
+        lib.orc_ade_tables(int(pb.mnpoles[m]), _p(cc, _f64p), _p(aa, _f64p), mu11,
+                           float(pb.sigm11[m]), pb.dt, _p(kk, _f64p), _p(bb, _f64p),
+                           _p(gg, _f64p), _p(a1, rp), _p(a2, rp))
+        mkappa[d], mbeta[d], gm3[d] = kk, bb, gg
+        gm1[m], gm2[m] = a1[0], a2[0]
+    # first-order Mur (InitMURBorders after InitCPMLBorders, :635-669): the
+    # dual inverse spacings at the Alloc ends, after any CPML stretching
+    cab1 = None
+    if any(pb.mur_faces):
+        cab1 = np.zeros((6, nm + 1), dt_)
+        for fce in range(6):
+            a, hi = fce // 2, fce & 1
+            n = (pb.nx, pb.ny, pb.nz)[a]
+            idh_ = idh[a][n + 1] if hi else idh[a][0]
+            lib.orc_mur_table(nm, _p(epr, rp), _p(mur, rp), pb.dt, eps0, mu0, idh_,
+                              _p(cab1[fce], rp))
     return Tables(g1, g2, gm1, gm2, ide, idh, pbe, pce, pbm, pcm, kappa, beta, g3,
-                  np.array(disp, np.int32), np.array([pb.npoles[m] for m in disp], np.int32))
+                  np.array(disp, np.int32), np.array([pb.npoles[m] for m in disp], np.int32),
+                  mkappa, mbeta, gm3, np.array(mdisp, np.int32),
+                  np.array([pb.mnpoles[m] for m in mdisp], np.int32), cab1)
 
 
 def lines(lib, pb, precision: int = 32):
@@ -229,9 +276,13 @@ class Oracle:
         self.mi = np.ascontiguousarray(pb.mi, dtype=np.int16)
         self.psi = np.zeros((12,) + shape, dt_) if pb.has_cpml else None
         nd = len(t.disp_media)
+        nmd = len(t.mdisp_media)
         self.eprev = np.zeros((3,) + shape, dt_) if nd else None
-        self.jstride = int(max(t.disp_npoles)) if nd else 1
+        self.hprev = np.zeros((3,) + shape, dt_) if nmd else None
+        self.jstride = int(max(list(t.disp_npoles) + list(t.mdisp_npoles) + [1]))
         self.jcur = np.zeros((3,) + shape + (self.jstride, 2)) if nd else None
+        self.mjcur = np.zeros((3,) + shape + (self.jstride, 2)) if nmd else None
+        self.murpast = np.zeros((3,) + shape, dt_) if t.cab1 is not None else None
         self.tiempo = np.zeros(pb.nsteps_max + 3)
         self.lib.orc_time_vector(pb.nsteps_max + 2, pb.dt, _p(self.tiempo, _f64p))
         self.medkind = np.ascontiguousarray(pb.kind, np.int32)
@@ -266,6 +317,25 @@ class Oracle:
             for c in range(3):
                 g.eprev[c] = self.eprev[c].ctypes.data_as(rp)
                 g.jcur[c] = self.jcur[c].ctypes.data_as(_f64p)
+        g.nmdisp = nmd
+        if nmd:
+            self._mdm = np.ascontiguousarray(t.mdisp_media, np.int32)
+            self._mdn = np.ascontiguousarray(t.mdisp_npoles, np.int32)
+            self._mk = np.ascontiguousarray(t.mkappa); self._mb = np.ascontiguousarray(t.mbeta)
+            self._mg = np.ascontiguousarray(t.gm3)
+            g.mdisp_medium = _p(self._mdm, _i32p); g.mdisp_npoles = _p(self._mdn, _i32p)
+            g.mdisp_kappa = _p(self._mk, _f64p); g.mdisp_beta = _p(self._mb, _f64p)
+            g.mdisp_gm3 = _p(self._mg, _f64p)
+            for c in range(3):
+                g.hprev[c] = self.hprev[c].ctypes.data_as(rp)
+                g.mjcur[c] = self.mjcur[c].ctypes.data_as(_f64p)
+        if t.cab1 is not None:
+            self._cab1 = np.ascontiguousarray(t.cab1)
+            for fce in range(6):
+                g.mur[fce] = int(bool(pb.mur_faces[fce]))
+                g.cab1[fce] = self._cab1[fce].ctypes.data_as(rp)
+            for c in range(3):
+                g.murpast[c] = self.murpast[c].ctypes.data_as(rp)
         g.jstride = self.jstride
         g.dt = pb.dt
         g.tiempo = _p(self.tiempo, _f64p)
@@ -377,6 +447,8 @@ def load_reference():
     lib.ref_planewave.argtypes = [_i32p, C.c_int32, _f32p, _f32p, _f32p, C.c_float, C.c_int32,
                                   _f32p, _f32p, _f32p, _f32p]
     lib.ref_get_field.argtypes = [C.c_int32, _f32p]
+    lib.ref_mdisp.argtypes = [C.c_int32, _i32p, _f64p, _f64p, _f32p, _f32p]
+    lib.ref_mur.argtypes = [_i32p]
     lib.ref_get_tables.argtypes = [_f32p] * 10
     return lib
 
@@ -416,6 +488,21 @@ class Reference:
                      _p(orden, _f32p), _p(refl, _f32p), pb.alphamax, pb.alphaorden,
                      pb.kappamax, pb.dt, _p(pb.D[0], _f32p), _p(pb.D[1], _f32p),
                      _p(pb.D[2], _f32p), _p(mi, _i16p), pb.eps0, pb.mu0)
+        if any(pb.kind[m] & 8 for m in range(nm + 1)):
+            mpc = np.zeros((nm + 1, PMAX, 2)); mpa = np.zeros((nm + 1, PMAX, 2))
+            for m in range(nm + 1):
+                if pb.kind[m] & 8:
+                    k = int(pb.mnpoles[m])
+                    mpc[m, :k] = pb.mpole_c[m][:k]
+                    mpa[m, :k] = pb.mpole_a[m][:k]
+            mnp = np.ascontiguousarray(pb.mnpoles, np.int32)
+            self._keep += [mpc, mpa, mnp]
+            lib.ref_mdisp(nm, _p(mnp, _i32p), _p(mpc, _f64p), _p(mpa, _f64p),
+                          _p(pb.mu11, _f32p), _p(pb.sigm11, _f32p))
+        if any(pb.mur_faces):
+            faces = np.array(pb.mur_faces, np.int32)
+            self._keep.append(faces)
+            lib.ref_mur(_p(faces, _i32p))
         ns = len(pb.nodal)
         if ns:
             comp = np.array([s.comp for s in pb.nodal], np.int32)

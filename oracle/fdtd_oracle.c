@@ -46,10 +46,11 @@ typedef orc_real real;
 #define IDX(g, i, j, k) \
   ((((size_t)((k) + 1)) * AY(g) + (size_t)((j) + 1)) * AX(g) + (size_t)((i) + 1))
 
-/* media kind bits: PEC, PMC, electric dispersive */
+/* media kind bits: PEC, PMC, electric dispersive, magnetic dispersive */
 #define ORC_KIND_PEC 1
 #define ORC_KIND_PMC 2
 #define ORC_KIND_EDISP 4
+#define ORC_KIND_MDISP 8
 
 /* --------------------------------------------------------------------------
  * dt: semba_fdtd.F90:1211-1250.  dtlay is formed in the real kind from the
@@ -70,8 +71,9 @@ double orc_dt(real dxmin, real dymin, real dzmin, real eps0, real mu0, real cfl,
 /* --------------------------------------------------------------------------
  * Per-medium update coefficients: calc_constants.F90:36-148.
  * kind: 0 lossy dielectric (:132-144), ORC_KIND_PEC (or r == 0) all zero
- * (:62-64), ORC_KIND_EDISP: G1/G2 placeholders 0 (overwritten by the ADE
- * init), GM from mu/sigmam (:109-118).  Expressions mix the real kind with
+ * (:62-64), ORC_KIND_EDISP alone: G1/G2 placeholders 0 (overwritten by the
+ * ADE init), GM from mu/sigmam (:109-118); ORC_KIND_MDISP alone: the mirror
+ * image (:119-128); both: lossy G and GM, both overwritten by the inits.  Expressions mix the real kind with
  * the real(8) dt exactly as written there.
  * ------------------------------------------------------------------------ */
 static void orc_lossy(double dt, real sig, real eps, real *g1, real *g2) {
@@ -93,12 +95,17 @@ void orc_g_tables(int32_t nmedia, const real *epr, const real *sigma, const real
   for (int32_t r = 0; r <= nmedia; ++r) {
     real eps = eps0 * epr[r];
     real mu = mu0 * mur[r];
+    const int ed = (kind[r] & ORC_KIND_EDISP) != 0, md = (kind[r] & ORC_KIND_MDISP) != 0;
     if ((kind[r] & ORC_KIND_PEC) || r == 0) {
       g1[r] = g2[r] = gm1[r] = gm2[r] = R(0.0);
-    } else if (kind[r] & ORC_KIND_EDISP) {
+    } else if (ed && !md) {
       g1[r] = R(0.0);
       g2[r] = R(0.0);
       orc_lossy(dt, sigmam[r], mu, &gm1[r], &gm2[r]);
+    } else if (md && !ed) { /* :119-128, GM placeholders for InitMdispersives */
+      orc_lossy(dt, sigma[r], eps, &g1[r], &g2[r]);
+      gm1[r] = R(0.0);
+      gm2[r] = R(0.0);
     } else {
       orc_lossy(dt, sigma[r], eps, &g1[r], &g2[r]);
       orc_lossy(dt, sigmam[r], mu, &gm1[r], &gm2[r]);
@@ -446,30 +453,35 @@ void orc_cpml_h(orc_grid *g) {
 }
 
 /* --------------------------------------------------------------------------
- * ADE: AdvanceEDispersiveE, electricdispersive.F90:304-362.  Per dispersive
- * edge: E <- E - Re(G3_p J_p) one pole at a time (real(8) product, rounded to
- * the real kind after each pole); then J_p <- kappa_p J_p + beta_p
- * (E - Eprev)/dt with (E - Eprev) in the real kind; then Eprev <- E.  Edges
- * are those of the E sweeps whose media index equals the dispersive medium
- * (:148-244); the reference walks them serially.
+ * ADE: AdvanceEDispersiveE, electricdispersive.F90:304-362, and
+ * AdvanceMdispersiveH, magneticdispersive.F90:306-362 (the same statements on
+ * H with GM3).  Per dispersive edge: F <- F - Re(G3_p J_p) one pole at a time
+ * (real(8) product, rounded to the real kind after each pole); then J_p <-
+ * kappa_p J_p + beta_p (F - Fprev)/dt with (F - Fprev) in the real kind; then
+ * Fprev <- F.  Edges are those of the field's sweeps whose media index equals
+ * the dispersive medium (electricdispersive.F90:148-244,
+ * magneticdispersive.F90:139-265); the reference walks them serially.
  * ------------------------------------------------------------------------ */
-void orc_ade_e(orc_grid *g) {
-  for (int d = 0; d < g->ndisp; ++d) {
-    const int med = g->disp_medium[d];
-    const int np = g->disp_npoles[d];
-    const double *K = g->disp_kappa + (size_t)d * ORC_PMAX * 2;
-    const double *B = g->disp_beta + (size_t)d * ORC_PMAX * 2;
-    const double *G = g->disp_g3 + (size_t)d * ORC_PMAX * 2;
+static void ade_field(orc_grid *g, int side) {
+  const int nd = side ? g->nmdisp : g->ndisp;
+  for (int d = 0; d < nd; ++d) {
+    const int med = side ? g->mdisp_medium[d] : g->disp_medium[d];
+    const int np = side ? g->mdisp_npoles[d] : g->disp_npoles[d];
+    const size_t o = (size_t)d * ORC_PMAX * 2;
+    const double *K = (side ? g->mdisp_kappa : g->disp_kappa) + o;
+    const double *B = (side ? g->mdisp_beta : g->disp_beta) + o;
+    const double *G = (side ? g->mdisp_gm3 : g->disp_g3) + o;
     for (int c = 0; c < 3; ++c) {
-      real *F = g->f[c];
-      real *Ep = g->eprev[c];
-      double *J = g->jcur[c];
-      int xe = sweep_hi(g, c, 0), ye = sweep_hi(g, c, 1), ze = sweep_hi(g, c, 2);
+      const int f = 3 * side + c;
+      real *F = g->f[f];
+      real *Ep = side ? g->hprev[c] : g->eprev[c];
+      double *J = side ? g->mjcur[c] : g->jcur[c];
+      int xe = sweep_hi(g, f, 0), ye = sweep_hi(g, f, 1), ze = sweep_hi(g, f, 2);
       for (int k = 0; k <= ze; ++k)
         for (int j = 0; j <= ye; ++j)
           for (int i = 0; i <= xe; ++i) {
             size_t p = IDX(g, i, j, k);
-            if (g->mi[c][p] != med) continue;
+            if (g->mi[f][p] != med) continue;
             double *Jp = J + p * (size_t)g->jstride * 2;
             real e = F[p];
             for (int q = 0; q < np; ++q) {
@@ -489,6 +501,96 @@ void orc_ade_e(orc_grid *g) {
               Jp[2 * q + 1] = ai + bi;
             }
             Ep[p] = e;
+          }
+    }
+  }
+}
+
+void orc_ade_e(orc_grid *g) { ade_field(g, 0); }
+void orc_ade_h(orc_grid *g) { ade_field(g, 1); }
+
+/* --------------------------------------------------------------------------
+ * First-order Mur, AdvanceMagneticMUR (bordersmur.F90:1129-1383, the
+ * "first order" branch; the second-order one stops with an error in the
+ * reference, :519).  On each absorbing face the two tangential H components
+ * of the plane just outside their sweep are advanced from the first plane
+ * inside:  H_b = Past_1 + CAB1[m] (H_1 - Past_b), m = media index of the
+ * inside cell; then (:1388-1600) Past <- H on both planes.  The face regions
+ * span the components' sweeps along the two other axes (:113-160); the
+ * updated cells of different faces never coincide and never feed each other,
+ * so one Past array per component serves every face.
+ * ------------------------------------------------------------------------ */
+void orc_mur_table(int32_t nmedia, const real *epr, const real *mur, double dt, real eps0,
+                   real mu0, real idh, real *cab1) {
+  const real cluz = R(1.0) / RSQRT(eps0 * mu0);
+  for (int32_t m = 0; m <= nmedia; ++m) {
+    const real v = RSQRT(epr[m] * mur[m]);
+    const real cnum = (real)((double)(R(1.0) / idh) / (dt * (double)cluz / (double)v));
+    cab1[m] = (R(1.0) - cnum) / (R(1.0) + cnum);
+  }
+}
+
+/* the tangential H components of face f (0 back .. 5 up) in the reference's
+ * loop order, and the face's normal axis and side */
+static const int mur_comp[6][2] = {{ORC_HZ, ORC_HY}, {ORC_HZ, ORC_HY}, {ORC_HX, ORC_HZ},
+                                   {ORC_HX, ORC_HZ}, {ORC_HY, ORC_HX}, {ORC_HY, ORC_HX}};
+
+static void mur_plane(orc_grid *g, int f, int c, int *ghost, int *inner) {
+  const int a = f / 2, hi = f & 1;
+  const int last = sweep_hi(g, c, a);
+  *ghost = hi ? last + 1 : -1;
+  *inner = hi ? last : 0;
+}
+
+void orc_mur_h(orc_grid *g) {
+  /* reference face order: left, right, down, up, back, front */
+  static const int order[6] = {2, 3, 4, 5, 0, 1};
+  for (int o = 0; o < 6; ++o) {
+    const int f = order[o];
+    if (!g->mur[f]) continue;
+    const int a = f / 2;
+    for (int t = 0; t < 2; ++t) {
+      const int c = mur_comp[f][t];
+      int ghost, inner;
+      mur_plane(g, f, c, &ghost, &inner);
+      real *H = g->f[c];
+      real *P = g->murpast[c - 3];
+      const int hi0 = sweep_hi(g, c, 0), hi1 = sweep_hi(g, c, 1), hi2 = sweep_hi(g, c, 2);
+      for (int k = 0; k <= hi2; ++k)
+        for (int j = 0; j <= hi1; ++j)
+          for (int i = 0; i <= hi0; ++i) {
+            int cb[3] = {i, j, k}, ci[3] = {i, j, k};
+            if ((a == 0 && i != 0) || (a == 1 && j != 0) || (a == 2 && k != 0)) continue;
+            cb[a] = ghost;
+            ci[a] = inner;
+            const size_t pb = IDX(g, cb[0], cb[1], cb[2]);
+            const size_t pi = IDX(g, ci[0], ci[1], ci[2]);
+            const int m = g->mi[c][pi];
+            H[pb] = P[pi] + g->cab1[f][m] * (H[pi] - P[pb]);
+          }
+    }
+  }
+  for (int o = 0; o < 6; ++o) {
+    const int f = order[o];
+    if (!g->mur[f]) continue;
+    const int a = f / 2;
+    for (int t = 0; t < 2; ++t) {
+      const int c = mur_comp[f][t];
+      int ghost, inner;
+      mur_plane(g, f, c, &ghost, &inner);
+      real *H = g->f[c];
+      real *P = g->murpast[c - 3];
+      const int hi0 = sweep_hi(g, c, 0), hi1 = sweep_hi(g, c, 1), hi2 = sweep_hi(g, c, 2);
+      for (int k = 0; k <= hi2; ++k)
+        for (int j = 0; j <= hi1; ++j)
+          for (int i = 0; i <= hi0; ++i) {
+            if ((a == 0 && i != 0) || (a == 1 && j != 0) || (a == 2 && k != 0)) continue;
+            for (int w = 0; w < 2; ++w) {
+              int cc[3] = {i, j, k};
+              cc[a] = w ? inner : ghost;
+              const size_t p = IDX(g, cc[0], cc[1], cc[2]);
+              P[p] = H[p];
+            }
           }
     }
   }
@@ -885,7 +987,8 @@ void orc_dft_accum(const orc_grid *g, int32_t kind, const int32_t box[6],
 
 /* One full step in the order of ciclo_temporal, timestepping.F90:1248-1981:
  * E Yee -> E CPML -> E ADE -> plane wave E -> nodal E -> H Yee -> H CPML ->
- * plane wave H -> nodal H (:1286-1441, :1497-1615). */
+ * H ADE -> plane wave H -> nodal H -> Mur (:1286-1441, :1497-1615,
+ * :1585-1586, :1711-1712). */
 void orc_step(orc_grid *g, int32_t n) {
   orc_yee_e(g);
   orc_cpml_e(g);
@@ -894,8 +997,11 @@ void orc_step(orc_grid *g, int32_t n) {
   if (g->nnodal) orc_nodal_e(g, n);
   orc_yee_h(g);
   orc_cpml_h(g);
+  if (g->nmdisp) orc_ade_h(g);
   if (g->pws) orc_pw_h(g, n);
   if (g->nnodal) orc_nodal_h(g, n);
+  if (g->mur[0] || g->mur[1] || g->mur[2] || g->mur[3] || g->mur[4] || g->mur[5])
+    orc_mur_h(g);
 }
 
 void orc_run(orc_grid *g, int32_t n0, int32_t nsteps, int32_t nprobe, const int32_t *pc,

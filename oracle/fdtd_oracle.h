@@ -99,6 +99,21 @@ typedef struct {
   const orc_real *line[3];     /* node coordinates LineX/Y/Z over [-1, n+1] */
   orc_real eps0, mu0;
   orc_pw_state *pws;           /* derived by orc_pw_init */
+  /* magnetic pole-residue ADE (magneticdispersive.F90), the E layout on H */
+  int32_t nmdisp;
+  const int32_t *mdisp_medium; /* [nmdisp] media index */
+  const int32_t *mdisp_npoles; /* [nmdisp] */
+  const double *mdisp_kappa;   /* [nmdisp][ORC_PMAX][2] */
+  const double *mdisp_beta;    /* [nmdisp][ORC_PMAX][2] */
+  const double *mdisp_gm3;     /* [nmdisp][ORC_PMAX][2] */
+  orc_real *hprev[3];          /* Alloc-sized, per H component */
+  double *mjcur[3];            /* Alloc-sized * jstride * 2 per H component */
+  /* first-order Mur faces (bordersmur.F90), order back, front, left, right,
+   * down, up (x-, x+, y-, y+, z-, z+) */
+  int32_t mur[6];              /* face absorbs */
+  const orc_real *cab1[6];     /* [nmedia+1] per face */
+  orc_real *murpast[3];        /* Alloc-sized per H component: the field of the
+                                  previous step on the Mur planes */
 } orc_grid;
 
 /* ---- coefficient tables ---- */
@@ -122,6 +137,12 @@ void orc_ade_tables(int32_t npoles, const double *c_re_im, const double *a_re_im
                     double *kappa, double *beta, double *g3,
                     orc_real *g1_out, orc_real *g2_out);
 void orc_time_vector(int32_t nlast, double dt, double *tiempo);
+/* First-order Mur coefficient CAB1 of one face for every medium
+ * (calc_murconstants, bordersmur.F90:316-363): idh = the dual inverse
+ * spacing at the face's end of the Alloc range (Idxh(XI) / Idxh(XE) ...);
+ * cnum = (1/idh) / (dt * c0/sqrt(epr*mur)), CAB1 = (1-cnum)/(1+cnum). */
+void orc_mur_table(int32_t nmedia, const orc_real *epr, const orc_real *mur, double dt,
+                   orc_real eps0, orc_real mu0, orc_real idh, orc_real *cab1);
 
 /* ---- stepping ---- */
 void orc_yee_e(orc_grid *g);
@@ -129,6 +150,8 @@ void orc_yee_h(orc_grid *g);
 void orc_cpml_e(orc_grid *g);
 void orc_cpml_h(orc_grid *g);
 void orc_ade_e(orc_grid *g);
+void orc_ade_h(orc_grid *g);
+void orc_mur_h(orc_grid *g);
 orc_real orc_evolucion(orc_real t, const orc_nodal *s);
 void orc_nodal_e(orc_grid *g, int32_t n);
 void orc_nodal_h(orc_grid *g, int32_t n);

@@ -2,7 +2,8 @@
 !
 ! A driver of the reference's own Fortran modules, built from their sources
 ! where they lie (fdetypes, errorreport, calc_constants, borderscpml,
-! electricdispersive, nodalsources, planewaves; see oracle/Makefile), so that
+! electricdispersive, nodalsources, planewaves, magneticdispersive, bordersmur;
+! see oracle/Makefile), so that
 ! the tests can compare the C restatement in fdtd_oracle.c against the
 ! reference itself:
 !   - calc_G1G2Gm1Gm2          calc_constants.F90:36-148
@@ -14,6 +15,10 @@
 !                              nodalsources.F90:76-305, 396-777
 !   - InitPlaneWave, AdvancePlaneWaveE/H
 !                              planewaves.F90:72-770, 878-1475
+!   - InitMdispersives, AdvanceMdispersiveH
+!                              magneticdispersive.F90:77-362
+!   - InitMURBorders, AdvanceMagneticMUR
+!                              bordersmur.F90:79-363, 502-1600
 ! The six Yee loops live in timestepping.F90, which cannot be compiled here
 ! without stand-ins for the MTLN modules it uses, so they are restated below
 ! from timestepping.F90:2165-2409 (one statement each).  Everything else that
@@ -32,6 +37,8 @@ module ref_harness
    use edispersives
    use nodalsources
    use ilumina
+   use Mdispersives
+   use Borders_MUR
    implicit none
 
    type(SGGFDTDINFO), save, target :: sgg
@@ -45,6 +52,7 @@ module ref_harness
       Idxe, Idye, Idze, Idxh, Idyh, Idzh
    logical, save :: thereCPML, thereEdisp, thereNodE, thereNodH
    logical, save :: therePW = .false., stillPW = .true.
+   logical, save :: thereMdisp = .false., thereMur = .false.
    real(kind=RKIND), save :: eps0h, mu0h
    type(limit_t), dimension(1:6), save :: sinpml
 
@@ -160,6 +168,7 @@ contains
          sgg%Med(r)%Is%PEC = (iand(kind(r), 1) /= 0) .or. (r == 0)
          sgg%Med(r)%Is%PMC = (iand(kind(r), 2) /= 0)
          sgg%Med(r)%Is%EDispersive = (iand(kind(r), 4) /= 0)
+         sgg%Med(r)%Is%MDispersive = (iand(kind(r), 8) /= 0)
          if (sgg%Med(r)%Is%EDispersive) then
             allocate(sgg%Med(r)%EDispersive(1))
             sgg%Med(r)%EDispersive(1)%NumPolRes11 = npoles(r)
@@ -191,6 +200,9 @@ contains
             sgg%PML%CoeffReflPML(o, p) = refl(p, o)
          end do
       end do
+      sgg%Border%IsBackMUR = .false.; sgg%Border%IsFrontMUR = .false.
+      sgg%Border%IsLeftMUR = .false.; sgg%Border%IsRightMUR = .false.
+      sgg%Border%IsDownMUR = .false.; sgg%Border%IsUpMUR = .false.
       sgg%Border%IsBackPML = lay(1, 1) > 0; sgg%Border%IsFrontPML = lay(2, 1) > 0
       sgg%Border%IsLeftPML = lay(1, 2) > 0; sgg%Border%IsRightPML = lay(2, 2) > 0
       sgg%Border%IsDownPML = lay(1, 3) > 0; sgg%Border%IsUpPML = lay(2, 3) > 0
@@ -265,6 +277,45 @@ contains
       call InitnodalSources(sgg, 0, nsrc, sgg%NodalSource, sgg%Sweep, thereNodE, thereNodH)
       call ref_time(nsteps)
    end subroutine ref_nodal
+
+   ! Magnetic pole-residue media through the reference's InitMdispersives
+   ! (magneticdispersive.F90:77), after InitEDispersives as in
+   ! timestepping.F90:927-943: per medium its pole count, residues/poles
+   ! (2, 8, 0:nmedia) and mu11 / Sigmam11.
+   subroutine ref_mdisp(nmedia, npoles, pc, pa, mu11, sigm11) bind(C, name="ref_mdisp")
+      integer(c_int), value :: nmedia
+      integer(c_int), intent(in) :: npoles(0:nmedia)
+      real(c_double), intent(in) :: pc(2, 8, 0:nmedia), pa(2, 8, 0:nmedia)
+      real(c_float), intent(in) :: mu11(0:nmedia), sigm11(0:nmedia)
+      integer :: r, p
+      logical :: resume
+      do r = 0, nmedia
+         if (.not. sgg%Med(r)%Is%MDispersive) cycle
+         allocate(sgg%Med(r)%MDispersive(1))
+         sgg%Med(r)%MDispersive(1)%NumPolRes11 = npoles(r)
+         allocate(sgg%Med(r)%MDispersive(1)%C11(npoles(r)), sgg%Med(r)%MDispersive(1)%A11(npoles(r)))
+         do p = 1, npoles(r)
+            sgg%Med(r)%MDispersive(1)%C11(p) = cmplx(pc(1, p, r), pc(2, p, r), kind=CKIND)
+            sgg%Med(r)%MDispersive(1)%A11(p) = cmplx(pa(1, p, r), pa(2, p, r), kind=CKIND)
+         end do
+         sgg%Med(r)%MDispersive(1)%mu11 = mu11(r)
+         sgg%Med(r)%MDispersive(1)%Sigmam11 = sigm11(r)
+      end do
+      resume = .false.
+      call InitMdispersives(sgg, miHx, miHy, miHz, thereMdisp, resume, GM1, GM2, Hx, Hy, Hz)
+   end subroutine ref_mdisp
+
+   ! First-order Mur faces through the reference's InitMURBorders
+   ! (bordersmur.F90:79), faces = back, front, left, right, down, up.
+   subroutine ref_mur(faces) bind(C, name="ref_mur")
+      integer(c_int), intent(in) :: faces(6)
+      logical :: resume
+      sgg%Border%IsBackMUR = faces(1) /= 0; sgg%Border%IsFrontMUR = faces(2) /= 0
+      sgg%Border%IsLeftMUR = faces(3) /= 0; sgg%Border%IsRightMUR = faces(4) /= 0
+      sgg%Border%IsDownMUR = faces(5) /= 0; sgg%Border%IsUpMUR = faces(6) /= 0
+      resume = .false.
+      call InitMURBorders(sgg, thereMur, resume, Idxh, Idyh, Idzh, eps0h, mu0h)
+   end subroutine ref_mur
 
    ! crea_timevector, timestepping.F90:3230-3240 (once)
    subroutine ref_time(nsteps) bind(C, name="ref_time")
@@ -387,9 +438,11 @@ contains
                                         Ex, Ey, Ez, .false.)
       call yee_h()
       if (thereCPML) call AdvanceMagneticCPML(sgg%NumMedia, b, miHx, miHy, miHz, GM2, Hx, Hy, Hz, Ex, Ey, Ez)
+      if (thereMdisp) call AdvanceMdispersiveH(sgg)
       if (therePW .and. stillPW) call AdvancePlaneWaveH(sgg, n, b, GM2, Idxe, Idye, Idze, Hx, Hy, Hz, stillPW)
       if (thereNodH) call AdvanceNodalH(sgg, miHx, miHy, miHz, sgg%NumMedia, n, b, GM2, Idxe, Idye, Idze, &
                                         Hx, Hy, Hz, .false.)
+      if (thereMur) call AdvanceMagneticMUR(b, sgg, miHx, miHy, miHz, Hx, Hy, Hz, .false.)
    end subroutine
 
    subroutine ref_get_field(c, out) bind(C, name="ref_get_field")

@@ -39,6 +39,7 @@ SIGNATURES = {
     "fdtd_set_spacing": (C.c_int, [vp] + [f32p] * 6),
     "fdtd_set_cpml": (C.c_int, [vp, i32p] + [f32p] * 12),
     "fdtd_set_edispersive": (C.c_int, [vp, C.c_int32, i32p, i32p, f64p, f64p, f64p]),
+    "fdtd_set_mdispersive": (C.c_int, [vp, C.c_int32, i32p, i32p, f64p, f64p, f64p]),
     "fdtd_set_time": (C.c_int, [vp, C.c_double, C.c_int32, f64p]),
     "fdtd_add_nodal": (C.c_int, [vp, C.c_int32, C.c_int32, i32p, C.c_float, C.c_float,
                                  C.c_int32, f32p]),
@@ -50,8 +51,9 @@ SIGNATURES = {
     "fdtd_advance_planewave_e": (C.c_int, [vp, C.c_int32]),
     "fdtd_advance_planewave_h": (C.c_int, [vp, C.c_int32]),
     "fdtd_set_lines": (C.c_int, [vp, f32p, f32p, f32p]),
-    "fdtd_set_tiling": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "fdtd_set_tiling": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
     "fdtd_state_size": (C.c_int64, [vp]),
+    "fdtd_set_field_scale": (C.c_int, [vp, C.c_int32, C.c_float]),
     "fdtd_save_state": (C.c_int, [vp, C.c_int32, vp, C.c_int64]),
     "fdtd_load_state": (C.c_int, [vp, vp, C.c_int64, i32p]),
     "fdtd_batch_create": (C.c_int, [C.c_int32, vp, C.POINTER(vp)]),
@@ -60,20 +62,24 @@ SIGNATURES = {
     "fdtd_batch_admit": (C.c_int, [vp, vp, C.c_int32, i32p]),
     "fdtd_batch_retire": (C.c_int, [vp, C.c_int32]),
     "fdtd_batch_step_of": (C.c_int64, [vp, C.c_int32]),
-    "fdtd_batch_set_tiling": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+    "fdtd_batch_set_tiling": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "fdtd_batch_step": (C.c_int, [vp, C.c_int32]),
     "fdtd_batch_synchronize": (C.c_int, [vp]),
     "fdtd_add_dft": (C.c_int, [vp, C.c_int32, i32p, i32p, C.c_int32, f32p, i32p]),
     "fdtd_dft_size": (C.c_int64, [vp, C.c_int32]),
     "fdtd_read_dft": (C.c_int, [vp, C.c_int32, f64p, C.c_int64]),
     "fdtd_update_observation": (C.c_int, [vp, C.c_int32]),
-    "fdtd_get_tiling": (C.c_int, [vp, i32p, i32p]),
+    "fdtd_get_tiling": (C.c_int, [vp, i32p, i32p, i32p]),
+    "fdtd_time_halves": (C.c_int, [vp, C.c_int32, f32p]),
     "fdtd_add_planewave": (C.c_int, [vp, i32p, C.c_int32, f32p, f32p, f32p, C.c_float,
                                      C.c_int32, f32p, C.c_float, C.c_float]),
     "fdtd_build_lines": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, f32p, f32p]),
     "fdtd_advance_nodal_e": (C.c_int, [vp, C.c_int32]),
     "fdtd_advance_h_yee": (C.c_int, [vp]),
     "fdtd_advance_h_cpml": (C.c_int, [vp]),
+    "fdtd_advance_h_dispersive": (C.c_int, [vp]),
+    "fdtd_advance_mur": (C.c_int, [vp]),
+    "fdtd_set_mur": (C.c_int, [vp, i32p, f32p]),
     "fdtd_advance_nodal_h": (C.c_int, [vp, C.c_int32]),
     "fdtd_run": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "fdtd_set_probes": (C.c_int, [vp, C.c_int32, i32p, i32p, i32p, i32p, C.c_int32]),
@@ -92,6 +98,8 @@ SIGNATURES = {
                              [C.c_double, C.c_float, C.c_float] + [f32p] * 8),
     "fdtd_build_ade": (C.c_int, [C.c_int32, f64p, f64p, C.c_float, C.c_float, C.c_double,
                                  f64p, f64p, f64p, f32p, f32p]),
+    "fdtd_build_mur": (C.c_int, [C.c_int32, f32p, f32p, C.c_double, C.c_float, C.c_float, f32p,
+                                 f32p]),
     "fdtd_evolucion": (C.c_float, [C.c_float, C.c_float, C.c_int32, f32p]),
     "fdtd_media_fill": (C.c_int, [vp, C.c_int16]),
     "fdtd_media_pec_walls": (C.c_int, [vp, C.c_int32]),

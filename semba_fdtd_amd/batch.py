@@ -18,7 +18,7 @@ from ._lib import FdtdError
 
 class Batch:
     def __init__(self, device: int = 0, stream=None, rows: int = 4, zchunk: int = 32,
-                 blocks: int = 0):
+                 blocks: int = 0, xcells: int = 1):
         self.lib = lib = _lib.load()
         b = C.c_void_p()
         rc = lib.fdtd_batch_create(device, stream, C.byref(b))
@@ -27,15 +27,18 @@ class Batch:
                             f"{lib.fdtd_batch_last_error(None).decode()}")
         self.b = b
         self.members = {}          # slot -> Solver
-        self.set_tiling(rows, zchunk, blocks)
+        self.set_tiling(rows, zchunk, blocks, xcells)
 
     def _ck(self, rc, what):
         if rc:
             msg = self.lib.fdtd_batch_last_error(self.b)
             raise FdtdError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def set_tiling(self, rows: int, zchunk: int, blocks: int = 0):
-        self._ck(self.lib.fdtd_batch_set_tiling(self.b, rows, zchunk, blocks),
+    def set_tiling(self, rows: int, zchunk: int, blocks: int = 0, xcells: int = 1):
+        """Blocks of 64 x rows threads marching zchunk z planes, xcells (1 or
+        4) x cells per lane, and the persistent launch's block count (0:
+        2048)."""
+        self._ck(self.lib.fdtd_batch_set_tiling(self.b, rows, zchunk, xcells, blocks),
                  "fdtd_batch_set_tiling")
 
     def admit(self, solver) -> int:

@@ -26,11 +26,10 @@ __global__ void k_nodal(DevGrid g, const NodalCell *cells, int ncells,
     const NodalCell cell = cells[c];
     if (cell.skip) continue;
     const float v = values[(long long)n * nsrc + cell.src];
-    float *F = g.f[cell.comp];
     if (cell.hard)
-      F[cell.p] = cell.amp * v;
+      field_st(g, cell.comp, cell.p, cell.amp * v);
     else
-      F[cell.p] = F[cell.p] - cell.coef * cell.amp * v;
+      field_st(g, cell.comp, cell.p, field_ld(g, cell.comp, cell.p) - cell.coef * cell.amp * v);
   }
 }
 
@@ -95,8 +94,14 @@ __global__ void __launch_bounds__(256) k_tfsf_list(const PwRun *runs, const long
     ehi = ehi + w.fpw[f.nf][m] * pw_evol(w, tm, d);
   }
   const long long p = (long long)(cc[2] + 1) * r.pxy + (long long)(cc[1] + 1) * r.px + kXOff + cc[0];
-  float *F = r.F;
-  F[p] = f.sign > 0.0f ? F[p] + r.G * ehi * f.id : F[p] - r.G * ehi * f.id;
+  if (r.fp16) {
+    __half *F = reinterpret_cast<__half *>(r.F);
+    const float old = __half2float(F[p]) * r.fs;
+    F[p] = __float2half_rn((f.sign > 0.0f ? old + r.G * ehi * f.id : old - r.G * ehi * f.id) * r.fi);
+  } else {
+    float *F = r.F;
+    F[p] = f.sign > 0.0f ? F[p] + r.G * ehi * f.id : F[p] - r.G * ehi * f.id;
+  }
 }
 
 int launch_tfsf_list(const PwRun *runs, const long long *start, int nruns, long long total,
@@ -138,7 +143,7 @@ __global__ void __launch_bounds__(256) k_dft(DevGrid g, DftDev d, int n, const i
                            d.first[2] + kt * d.stride[2]);
   const double2 a = d.aux[(long long)n * d.nfreq + f];
   for (int c = 0; c < 3; ++c) {
-    const double v = (double)g.f[3 * d.kind + c][p];
+    const double v = (double)field_ld(g, 3 * d.kind + c, p);
     double2 *acc = d.acc + (pt * 3 + c) * d.nfreq + f;
     const double2 o = *acc;
     *acc = make_double2(o.x + a.x * v, o.y + a.y * v);
@@ -163,7 +168,7 @@ __global__ void k_probes(DevGrid g, const ProbeDev *pr, int np, float *buf,
   __syncthreads();
   if (row < cap) {
     for (int t = threadIdx.x; t < np; t += blockDim.x)
-      buf[(long long)row * np + t] = g.f[pr[t].comp][pr[t].p];
+      buf[(long long)row * np + t] = field_ld(g, pr[t].comp, pr[t].p);
   }
   __syncthreads();
   if (threadIdx.x == 0 && row < cap) *count = row + 1;
@@ -178,7 +183,8 @@ int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
 }
 
 // bounding box of the dispersive edges of component comp over its sweep
-// (the node lists InitEDispersives gathers, electricdispersive.F90:148-244).
+// (the node lists InitEDispersives and InitMdispersives gather,
+// electricdispersive.F90:148-244, magneticdispersive.F90:139-265).
 // Each thread keeps a running min/max, the block reduces in LDS and one lane
 // folds the block result into out[] with global atomics.
 __global__ void k_disp_bbox(DevGrid g, int comp, int xlast, int ylast, int zlast,
@@ -194,7 +200,7 @@ __global__ void k_disp_bbox(DevGrid g, int comp, int xlast, int ylast, int zlast
     const int i = (int)(t % nxr);
     const int j = (int)((t / nxr) % nyr);
     const int k = (int)(t / (nxr * nyr));
-    if (g.dslot[g.mi[comp][fidx(g, i, j, k)]] >= 0) {
+    if (g.ade[comp / 3].dslot[g.mi[comp][fidx(g, i, j, k)]] >= 0) {
       lo[0] = min(lo[0], i); lo[1] = min(lo[1], j); lo[2] = min(lo[2], k);
       hi[0] = max(hi[0], i); hi[1] = max(hi[1], j); hi[2] = max(hi[2], k);
       ++cnt;
@@ -225,12 +231,59 @@ __global__ void k_disp_bbox(DevGrid g, int comp, int xlast, int ylast, int zlast
   }
 }
 
-int launch_disp_bbox(const DevGrid &g, int comp, int32_t *out, void *stream) {
-  const int xl = comp == 0 ? g.nx - 1 : g.nx;
-  const int yl = comp == 1 ? g.ny - 1 : g.ny;
-  const int zl = comp == 2 ? g.nz - 1 : g.nz;
-  hipLaunchKernelGGL(k_disp_bbox, dim3(2048), dim3(256), 0, (hipStream_t)stream, g, comp,
-                     xl, yl, zl, out);
+// side 0: E component c (sweep n-1 along c, n across); side 1: H component
+// c (n along c, n-1 across)
+int launch_disp_bbox(const DevGrid &g, int side, int c, int32_t *out, void *stream) {
+  const int o = side ? 0 : 1, a = side ? 1 : 0;
+  const int xl = c == 0 ? g.nx - o : g.nx - a;
+  const int yl = c == 1 ? g.ny - o : g.ny - a;
+  const int zl = c == 2 ? g.nz - o : g.nz - a;
+  hipLaunchKernelGGL(k_disp_bbox, dim3(2048), dim3(256), 0, (hipStream_t)stream, g,
+                     3 * side + c, xl, yl, zl, out);
+  return (int)hipGetLastError();
+}
+
+// ---- first-order Mur, AdvanceMagneticMUR (bordersmur.F90:1129-1600) -----
+// One thread per face cell: H_ghost = Past_inner + CAB1[m] (H_inner -
+// Past_ghost), m the inner cell's media index, then both Past planes take the
+// new field.  Ghost cells of different faces never coincide and no face reads
+// another's ghost plane, so one launch equals the reference's face-by-face
+// loops followed by its Past updates.  EXACT: no contraction (the file is
+// built without it); FAST: one fused multiply-add.
+template <bool EXACT>
+__global__ void k_mur(DevGrid g, const MurRun *runs, int nrun, long long total) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    int r = 0;
+    while (r + 1 < nrun && runs[r + 1].start <= t) ++r;
+    const MurRun &m = runs[r];
+    const long long q = t - m.start;
+    const int u = (int)(q % m.nu), v = (int)(q / m.nu);
+    int cb[3], ci[3];
+    cb[m.ua] = ci[m.ua] = u;
+    cb[m.va] = ci[m.va] = v;
+    cb[m.axis] = m.ghost;
+    ci[m.axis] = m.inner;
+    const long long pb = fidx(g, cb[0], cb[1], cb[2]);
+    const long long pi = fidx(g, ci[0], ci[1], ci[2]);
+    const long long plane = (long long)m.nu * m.nv;
+    const float past_b = m.past[q], past_i = m.past[plane + q];
+    const float h_i = field_ld(g, m.comp, pi);
+    const float c = m.cab1[g.mi[m.comp][pi]];
+    const float h_b = EXACT ? past_i + c * (h_i - past_b) : __builtin_fmaf(c, h_i - past_b, past_i);
+    field_st(g, m.comp, pb, h_b);
+    m.past[q] = field_ld(g, m.comp, pb);
+    m.past[plane + q] = h_i;
+  }
+}
+
+int launch_mur(const DevGrid &g, const MurRun *runs, int nrun, long long total, int exact,
+               void *stream) {
+  if (total <= 0) return 0;
+  const long long want = (total + 255) / 256;
+  const dim3 grid((unsigned)(want < 4096 ? want : 4096));
+  if (exact) hipLaunchKernelGGL(k_mur<true>, grid, dim3(256), 0, (hipStream_t)stream, g, runs, nrun, total);
+  else hipLaunchKernelGGL(k_mur<false>, grid, dim3(256), 0, (hipStream_t)stream, g, runs, nrun, total);
   return (int)hipGetLastError();
 }
 

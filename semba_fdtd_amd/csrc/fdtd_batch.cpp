@@ -46,10 +46,11 @@ struct fdtd_batch {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int mode = -1;
+  int fp16 = 0;                       // field storage shared by the grids
   std::vector<fdtd_handle *> slots;   // admitted grids; retired slots are null
   std::vector<long long> step;        // host copy of each slot's step counter
   std::vector<unsigned> version;      // each grid's state version at the last build
-  int rows = 4, zchunk = 32, blocks = 0;
+  int rows = 4, zchunk = 32, xcells = 1, blocks = 0;
   // device descriptors, rebuilt when the membership changes
   bool ready = false;
   std::vector<void *> allocs;
@@ -66,6 +67,7 @@ namespace {
 
 int bfail(fdtd_batch *b, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
 int bfail(fdtd_batch *b, int code, const char *fmt, ...) {
+  (void)hipGetLastError();
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -112,7 +114,10 @@ int bupload(fdtd_batch *b, T **dst, const T *src, size_t n) {
 TileMap batch_tiles(const fdtd_batch *b, const DevGrid &g) {
   TileMap tm{};
   tm.bdy = b->rows;
-  tm.tx = (g.nx + 1 + kWave - 1) / kWave;
+  // one or four x cells per lane; binary16 storage always runs four
+  const int w = (b->xcells == 4 || g.fp16) ? 4 * kWave : kWave;
+  tm.vec = w / kWave;
+  tm.tx = (g.nx + 1 + w - 1) / w;
   tm.ty = (g.ny + 1 + b->rows - 1) / b->rows;
   tm.kc = std::min(b->zchunk, g.nz + 1);
   tm.tz = (g.nz + 1 + tm.kc - 1) / tm.kc;
@@ -199,13 +204,15 @@ int launch_step(fdtd_batch *b) {
   const int blocks = b->blocks > 0 ? b->blocks : 2048;
   int e, rc;
   for (int half = 0; half < 2; ++half) {
-    e = exact ? exact::launch_pass_batch(b->bd, half, blocks, b->rows, b->lds, b->stream)
-              : fast::launch_pass_batch(b->bd, half, blocks, b->rows, b->lds, b->stream);
+    const int vec = (b->fp16 || b->xcells == 4) ? 4 : 1;
+    e = exact ? exact::launch_pass_batch(b->bd, half, blocks, b->rows, vec, b->lds, 0, b->stream)
+              : fast::launch_pass_batch(b->bd, half, blocks, b->rows, vec, b->lds, b->fp16,
+                                        b->stream);
     if (e) return bfail(b, FDTD_ERR_HIP, "batch face pass failed: %s", hipGetErrorString((hipError_t)e));
-    if (half == 0)
-      for (fdtd_handle *h : b->slots)
-        if (h && h->g.has_ade && (rc = run_ade(h)))
-          return bfail(b, rc, "grid: %s", h->err.c_str());
+    // each grid's pole stage follows the face pass of its field
+    for (fdtd_handle *h : b->slots)
+      if (h && h->g.ade[half].has && (rc = run_ade(h, half)))
+        return bfail(b, rc, "grid: %s", h->err.c_str());
     for (const BatchPwGroup &g : b->pw) {
       if (g.half != half) continue;
       e = launch_tfsf_list(g.druns, g.dstart, (int)g.runs.size(), g.start.back(), 0, b->stream);
@@ -215,6 +222,8 @@ int launch_step(fdtd_batch *b) {
       if (h && (rc = run_nodal(h, half, 0, h->dstep)))
         return bfail(b, rc, "grid: %s", h->err.c_str());
   }
+  for (fdtd_handle *h : b->slots)
+    if (h && (rc = run_mur(h))) return bfail(b, rc, "grid: %s", h->err.c_str());
   for (fdtd_handle *h : b->slots) {
     if (!h) continue;
     if (h->nprobe && launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap,
@@ -280,8 +289,8 @@ int fdtd_batch_admit(fdtd_batch *b, fdtd_handle *h, int32_t n0, int32_t *slot) {
   if (h->device != b->device) return bfail(b, FDTD_ERR_ARG, "grid lives on another device");
   for (fdtd_handle *o : b->slots)
     if (o == h) return bfail(b, FDTD_ERR_ARG, "grid already admitted");
-  if (b->mode >= 0 && h->mode != b->mode)
-    return bfail(b, FDTD_ERR_ARG, "grids of one batch share the arithmetic mode");
+  if (b->mode >= 0 && (h->mode != b->mode || h->g.fp16 != b->fp16))
+    return bfail(b, FDTD_ERR_ARG, "grids of one batch share the arithmetic mode and storage");
   hipSetDevice(b->device);
   int rc = prepare(h);
   if (rc) return bfail(b, rc, "grid not ready: %s", h->err.c_str());
@@ -296,6 +305,7 @@ int fdtd_batch_admit(fdtd_batch *b, fdtd_handle *h, int32_t n0, int32_t *slot) {
   h->solo_stream = h->stream;
   h->stream = b->stream;
   b->mode = h->mode;
+  b->fp16 = h->g.fp16;
   int s = -1;
   for (size_t i = 0; i < b->slots.size(); ++i)
     if (!b->slots[i]) { s = (int)i; break; }
@@ -336,12 +346,15 @@ int64_t fdtd_batch_step_of(const fdtd_batch *b, int32_t slot) {
   return b->step[slot];
 }
 
-int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t blocks) {
-  if (!b || !(rows == 1 || rows == 2 || rows == 4) || zchunk < 1 || blocks < 0)
-    return bfail(b, FDTD_ERR_ARG, "tiling %d rows x %d planes, %d blocks not supported", rows,
-                 zchunk, blocks);
+int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t xcells,
+                          int32_t blocks) {
+  if (!b || !(rows == 1 || rows == 2 || rows == 4) || zchunk < 1 ||
+      !(xcells == 1 || xcells == 4) || blocks < 0)
+    return bfail(b, FDTD_ERR_ARG, "tiling %d rows x %d planes x %d cells, %d blocks not supported",
+                 rows, zchunk, xcells, blocks);
   b->rows = rows;
   b->zchunk = zchunk;
+  b->xcells = xcells;
   b->blocks = blocks;
   b->ready = false;
   return FDTD_OK;

@@ -19,6 +19,14 @@ struct Source {
   std::vector<float> evol;
 };
 
+struct AdeIn {           // the dispersive media of one field, as set by the caller
+  int ndisp = 0;
+  std::vector<int32_t> dmed, dnp;
+  std::vector<double> dk, db, dg3;
+  long long nedges[3] = {0, 0, 0};  // dispersive edges per component
+  size_t jbytes[3] = {0, 0, 0};      // pole-current bytes per component
+};
+
 struct DftIn {           // one frequency-domain observation box
   int kind;
   int box[6], stride[3];
@@ -53,7 +61,8 @@ struct fdtd_handle {
   int mode = FDTD_MODE_FAST;
   int storage = FDTD_STORE_FP32;
   sfd::DevGrid g{};
-  sfd::TileMap tm{};
+  sfd::TileMap tm{};           // 64-cell x tiles (granular passes)
+  sfd::TileMap tm4{};          // 256-cell x tiles (fused passes, 4 cells per lane)
   std::vector<void *> allocs;
   int64_t bytes = 0;
   float *fld[6] = {};
@@ -70,13 +79,18 @@ struct fdtd_handle {
   // cpml
   float *dprof[12] = {};
   long long psilen[12] = {};       // psi values per slab array
-  // dispersive
-  int ndisp = 0;
-  std::vector<int32_t> dmed, dnp;
-  std::vector<double> dk, db, dg3;
+  // dispersive media: [0] electric (on E), [1] magnetic (on H)
+  sfd_host::AdeIn adein[2];
+  // first-order Mur faces
+  int mur_faces[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<float> cab1h;            // [6][nmedia+1]
+  bool mur_ready = false;
+  std::vector<void *> mur_allocs;
+  std::vector<sfd::MurRun> mur_runs;   // host copy (past pointers on the device)
+  sfd::MurRun *dmur = nullptr;
+  long long mur_total = 0;
   bool ade_ready = false;
   std::vector<void *> ade_allocs;
-  long long nedges[3] = {0, 0, 0};  // dispersive edges per E component
   // time
   double dt = 0.0;
   std::vector<double> tiempo;
@@ -129,6 +143,7 @@ int prepare(fdtd_handle *h);
 int run_pw(fdtd_handle *h, int half, int n, const int *dn);
 int run_nodal(fdtd_handle *h, int half, int n, const int *dn);
 int run_dft(fdtd_handle *h, int n, const int *dn);
-int run_ade(fdtd_handle *h);
+int run_mur(fdtd_handle *h);
+int run_ade(fdtd_handle *h, int side);
 void invalidate_graph(fdtd_handle *h);
 }  // namespace sfd_host

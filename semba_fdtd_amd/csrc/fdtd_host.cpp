@@ -22,6 +22,7 @@ thread_local std::string g_create_error;
 namespace sfd_host {
 
 int fail(fdtd_handle *h, int code, const char *fmt, ...) {
+  (void)hipGetLastError();  // a failed call must not surface again in the next launch check
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -68,6 +69,9 @@ int upload(fdtd_handle *h, T **dst, const T *src, size_t n,
 
 size_t field_elems(const fdtd_handle *h) { return (size_t)h->g.pxy * (size_t)(h->g.nz + 3); }
 
+// bytes per stored field value
+size_t field_bytes(const fdtd_handle *h) { return h->storage == FDTD_STORE_FP16 ? 2 : 4; }
+
 int axis_n(const fdtd_handle *h, int a) { return a == 0 ? h->g.nx : (a == 1 ? h->g.ny : h->g.nz); }
 
 // extents of component comp's Alloc box [-1, last swept + 1] per axis, the
@@ -99,6 +103,25 @@ int copy_box(fdtd_handle *h, int comp, void *dev, const void *host_src, void *ho
   return FDTD_OK;
 }
 
+// a field component to / from the host as fp32, whatever the storage
+int field_io(fdtd_handle *h, int comp, const float *src, float *dst) {
+  if (h->storage == FDTD_STORE_FP32)
+    return copy_box(h, comp, h->fld[comp], src, dst, 4);
+  long long ext[3];
+  alloc_extent(h, comp, ext);
+  const size_t n = (size_t)(ext[0] * ext[1] * ext[2]);
+  std::vector<__half> tmp(n);
+  const float s = h->g.fs[comp], is = h->g.fi[comp];
+  if (src) {
+    for (size_t t = 0; t < n; ++t) tmp[t] = __float2half_rn(src[t] * is);
+    return copy_box(h, comp, h->fld[comp], tmp.data(), nullptr, 2);
+  }
+  int rc = copy_box(h, comp, h->fld[comp], nullptr, tmp.data(), 2);
+  if (rc) return rc;
+  for (size_t t = 0; t < n; ++t) dst[t] = __half2float(tmp[t]) * s;
+  return FDTD_OK;
+}
+
 int check_ready(fdtd_handle *h) {
   if (h->nmedia < 0) return fail(h, FDTD_ERR_STATE, "fdtd_set_media not called");
   if (h->media_max > h->nmedia)
@@ -109,16 +132,20 @@ int check_ready(fdtd_handle *h) {
 }
 
 // blocks of 64 x rows threads march z in chunks of zchunk planes
-void set_tiles(fdtd_handle *h, int rows, int zchunk) {
-  TileMap &tm = h->tm;
+void set_tiles(fdtd_handle *h, int rows, int zchunk, int xcells) {
   const DevGrid &g = h->g;
-  tm.bdy = rows;
-  tm.tx = (g.nx + 1 + kWave - 1) / kWave;
-  tm.ty = (g.ny + 1 + rows - 1) / rows;
-  tm.kc = std::min(zchunk, g.nz + 1);
-  tm.tz = (g.nz + 1 + tm.kc - 1) / tm.kc;
-  tm.nblk = tm.tx * tm.ty * tm.tz;
-  tm.remap = 1;
+  for (int v = 0; v < 2; ++v) {
+    TileMap &tm = v ? h->tm4 : h->tm;
+    const int w = v ? 4 * kWave : kWave;
+    tm.bdy = rows;
+    tm.tx = (g.nx + 1 + w - 1) / w;
+    tm.ty = (g.ny + 1 + rows - 1) / rows;
+    tm.kc = std::min(zchunk, g.nz + 1);
+    tm.tz = (g.nz + 1 + tm.kc - 1) / tm.kc;
+    tm.nblk = tm.tx * tm.ty * tm.tz;
+    tm.remap = 1;
+    tm.vec = xcells;
+  }
 }
 
 void invalidate_graph(fdtd_handle *h) {
@@ -131,31 +158,34 @@ void invalidate_graph(fdtd_handle *h) {
 }
 
 // ---- compact pole state --------------------------------------------------
-int build_ade(fdtd_handle *h) {
-  if (h->ade_ready) return FDTD_OK;
-  for (void *p : h->ade_allocs) hipFree(p);
-  h->ade_allocs.clear();
-  h->g.has_ade = 0;
-  if (h->ndisp == 0) { h->ade_ready = true; return FDTD_OK; }
+// One field's pole state (side 0: electric media on E, 1: magnetic media on
+// H): per-medium coefficients, and per component a dense box over its
+// dispersive edges.
+static int build_ade_side(fdtd_handle *h, int side) {
+  sfd_host::AdeIn &in = h->adein[side];
+  AdeSet &ad = h->g.ade[side];
+  ad = AdeSet{};
+  for (int c = 0; c < 3; ++c) { in.nedges[c] = 0; in.jbytes[c] = 0; }
+  if (in.ndisp == 0) return FDTD_OK;
   const int nm = h->nmedia;
   std::vector<int32_t> slot(nm + 1, -1);
-  for (int d = 0; d < h->ndisp; ++d) {
-    if (h->dmed[d] < 1 || h->dmed[d] > nm)
-      return fail(h, FDTD_ERR_ARG, "dispersive medium %d out of range", h->dmed[d]);
-    slot[h->dmed[d]] = d;
+  for (int d = 0; d < in.ndisp; ++d) {
+    if (in.dmed[d] < 1 || in.dmed[d] > nm)
+      return fail(h, FDTD_ERR_ARG, "dispersive medium %d out of range", in.dmed[d]);
+    slot[in.dmed[d]] = d;
   }
   int32_t *dslot = nullptr;
   int rc = upload(h, &dslot, slot.data(), slot.size(), &h->ade_allocs);
   if (rc) return rc;
-  h->g.dslot = dslot;
+  ad.dslot = dslot;
   // coefficients
-  std::vector<int32_t> anp(h->ndisp);
-  std::vector<float> cf((size_t)h->ndisp * kPMax * 6, 0.0f);
-  std::vector<double> cd((size_t)h->ndisp * kPMax * 6, 0.0);
-  for (int d = 0; d < h->ndisp; ++d) {
-    const int np = h->dnp[d];
-    const double *K = &h->dk[(size_t)d * kPMax * 2], *B = &h->db[(size_t)d * kPMax * 2],
-                 *G = &h->dg3[(size_t)d * kPMax * 2];
+  std::vector<int32_t> anp(in.ndisp);
+  std::vector<float> cf((size_t)in.ndisp * kPMax * 6, 0.0f);
+  std::vector<double> cd((size_t)in.ndisp * kPMax * 6, 0.0);
+  for (int d = 0; d < in.ndisp; ++d) {
+    const int np = in.dnp[d];
+    const double *K = &in.dk[(size_t)d * kPMax * 2], *B = &in.db[(size_t)d * kPMax * 2],
+                 *G = &in.dg3[(size_t)d * kPMax * 2];
     for (int q = 0; q < np; ++q) {
       double *o = &cd[((size_t)d * kPMax + q) * 6];
       o[0] = K[2 * q]; o[1] = K[2 * q + 1]; o[2] = B[2 * q]; o[3] = B[2 * q + 1];
@@ -188,57 +218,133 @@ int build_ade(fdtd_handle *h) {
   }
   int32_t *danp = nullptr;
   if ((rc = upload(h, &danp, anp.data(), anp.size(), &h->ade_allocs))) return rc;
-  h->g.anp = danp;
+  ad.anp = danp;
   if (h->mode == FDTD_MODE_EXACT) {
     double *p = nullptr;
     if ((rc = upload(h, &p, cd.data(), cd.size(), &h->ade_allocs))) return rc;
-    h->g.aded = p;
+    ad.aded = p;
   } else {
     float *p = nullptr;
     if ((rc = upload(h, &p, cf.data(), cf.size(), &h->ade_allocs))) return rc;
-    h->g.adef = p;
+    ad.adef = p;
   }
+  ad.ndisp = in.ndisp;
   int maxp = 0;
-  for (int d = 0; d < h->ndisp; ++d) maxp = std::max(maxp, anp[d]);
-  // dense state box per E component over its dispersive edges
+  for (int d = 0; d < in.ndisp; ++d) maxp = std::max(maxp, anp[d]);
+  // dense state box per component over its dispersive edges (the bounding
+  // box kernel reads the slot table through the grid descriptor)
   int32_t *bb = nullptr;
   if ((rc = dalloc(h, (void **)&bb, 8 * 4, &h->ade_allocs))) return rc;
   for (int c = 0; c < 3; ++c) {
     const int32_t init[7] = {0x7fffffff, 0x7fffffff, 0x7fffffff, (int32_t)0x80000000,
                              (int32_t)0x80000000, (int32_t)0x80000000, 0};
     HIPCHK(h, hipMemcpy(bb, init, sizeof init, hipMemcpyHostToDevice));
-    HIPCHK(h, (hipError_t)launch_disp_bbox(h->g, c, bb, h->stream));
+    HIPCHK(h, (hipError_t)launch_disp_bbox(h->g, side, c, bb, h->stream));
     int32_t box[7];
     HIPCHK(h, hipMemcpyAsync(box, bb, sizeof box, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->nedges[c] = box[6];
+    in.nedges[c] = box[6];
     long long total = 0;
     if (box[0] <= box[3]) {
       for (int a = 0; a < 3; ++a) {
-        h->g.abox[c][a] = box[a];
-        h->g.abox[c][3 + a] = box[3 + a] - box[a] + 1;
+        ad.abox[c][a] = box[a];
+        ad.abox[c][3 + a] = box[3 + a] - box[a] + 1;
       }
-      total = (long long)h->g.abox[c][3] * h->g.abox[c][4] * h->g.abox[c][5];
+      total = (long long)ad.abox[c][3] * ad.abox[c][4] * ad.abox[c][5];
     } else {
-      for (int a = 0; a < 6; ++a) h->g.abox[c][a] = 0;  // empty box: no edge inside
+      for (int a = 0; a < 6; ++a) ad.abox[c][a] = 0;  // empty box: no edge inside
     }
-    h->g.nstate[c] = total;
+    ad.nstate[c] = total;
     float *ep = nullptr;
     void *js = nullptr;
     const size_t ns = (size_t)std::max(total, 1LL);
     const size_t jbytes = ns * std::max(maxp, 1) * (h->mode == FDTD_MODE_EXACT ? 16 : 8);
     if ((rc = dalloc(h, (void **)&ep, ns * 4, &h->ade_allocs))) return rc;
     if ((rc = dalloc(h, &js, jbytes, &h->ade_allocs))) return rc;
+    in.jbytes[c] = jbytes;
     if ((rc = dzero(h, ep, ns * 4))) return rc;
     if ((rc = dzero(h, js, jbytes))) return rc;
-    h->g.eprev[c] = ep;
-    h->g.jst[c] = js;
+    ad.prev[c] = ep;
+    ad.jst[c] = js;
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->g.ndisp = h->ndisp;
-  // no dispersive edge anywhere: run the passes without the pole stage
-  h->g.has_ade = (h->nedges[0] + h->nedges[1] + h->nedges[2]) > 0 ? 1 : 0;
+  // no dispersive edge anywhere: the passes run without this pole stage
+  ad.has = (in.nedges[0] + in.nedges[1] + in.nedges[2]) > 0 ? 1 : 0;
+  return FDTD_OK;
+}
+
+int build_ade(fdtd_handle *h) {
+  if (h->ade_ready) return FDTD_OK;
+  for (void *p : h->ade_allocs) hipFree(p);
+  h->ade_allocs.clear();
+  for (int side = 0; side < 2; ++side) {
+    const int rc = build_ade_side(h, side);
+    if (rc) return rc;
+  }
   h->ade_ready = true;
+  return FDTD_OK;
+}
+
+// ---- first-order Mur faces -------------------------------------------------
+// Runs in the reference's face order (left, right, down, up, back, front,
+// bordersmur.F90:1129-1383), each face's two tangential H components; the
+// components' sweeps give the face extents (InitMURBorders, :113-160).
+static int build_mur(fdtd_handle *h) {
+  if (h->mur_ready) return FDTD_OK;
+  for (void *p : h->mur_allocs) hipFree(p);
+  h->mur_allocs.clear();
+  h->mur_runs.clear();
+  h->dmur = nullptr;
+  h->mur_total = 0;
+  bool any = false;
+  for (int f = 0; f < 6; ++f) any = any || h->mur_faces[f];
+  if (!any) { h->mur_ready = true; return FDTD_OK; }
+  if ((int)h->cab1h.size() != 6 * (h->nmedia + 1))
+    return fail(h, FDTD_ERR_STATE, "Mur coefficients do not match the media table");
+  float *dcab = nullptr;
+  int rc = upload(h, &dcab, h->cab1h.data(), h->cab1h.size(), &h->mur_allocs);
+  if (rc) return rc;
+  static const int order[6] = {2, 3, 4, 5, 0, 1};
+  static const int comps[6][2] = {{5, 4}, {5, 4}, {3, 5}, {3, 5}, {4, 3}, {4, 3}};
+  const int n[3] = {h->g.nx, h->g.ny, h->g.nz};
+  for (int o = 0; o < 6; ++o) {
+    const int f = order[o];
+    if (!h->mur_faces[f]) continue;
+    const int a = f / 2, hi = f & 1;
+    for (int t = 0; t < 2; ++t) {
+      MurRun r{};
+      r.comp = comps[f][t];
+      const int c = r.comp - 3;
+      // H component c sweeps [0, n] along its own axis, [0, n-1] across
+      auto last = [&](int ax) { return ax == c ? n[ax] : n[ax] - 1; };
+      r.axis = a;
+      r.ghost = hi ? last(a) + 1 : -1;
+      r.inner = hi ? last(a) : 0;
+      r.ua = a == 0 ? 1 : 0;
+      r.va = a == 2 ? 1 : 2;
+      r.nu = last(r.ua) + 1;
+      r.nv = last(r.va) + 1;
+      r.start = h->mur_total;
+      r.cab1 = dcab + (size_t)f * (h->nmedia + 1);
+      const size_t bytes = (size_t)2 * r.nu * r.nv * 4;
+      if ((rc = dalloc(h, (void **)&r.past, bytes, &h->mur_allocs))) return rc;
+      if ((rc = dzero(h, r.past, bytes))) return rc;
+      h->mur_total += (long long)r.nu * r.nv;
+      h->mur_runs.push_back(r);
+    }
+  }
+  if ((rc = upload(h, &h->dmur, h->mur_runs.data(), h->mur_runs.size(), &h->mur_allocs)))
+    return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->mur_ready = true;
+  return FDTD_OK;
+}
+
+int run_mur(fdtd_handle *h) {
+  if (h->mur_runs.empty()) return FDTD_OK;
+  if (launch_mur(h->g, h->dmur, (int)h->mur_runs.size(), h->mur_total,
+                 h->mode == FDTD_MODE_EXACT, h->stream))
+    return fail(h, FDTD_ERR_HIP, "Mur launch failed");
   return FDTD_OK;
 }
 
@@ -301,6 +407,7 @@ int build_nodal(fdtd_handle *h) {
     }
   int rc;
   for (void *p : h->nodal_allocs) hipFree(p);
+  for (void *p : h->mur_allocs) hipFree(p);
   h->nodal_allocs.clear();
   h->dvalues = nullptr;
   h->dcells[0] = h->dcells[1] = nullptr;
@@ -476,6 +583,9 @@ int build_planewaves(fdtd_handle *h) {
           PwRun r{};
           r.f = f;
           r.F = h->fld[f.comp];
+          r.fp16 = h->g.fp16;
+          r.fs = h->g.fs[f.comp];
+          r.fi = h->g.fi[f.comp];
           r.px = h->g.px;
           r.pxy = h->g.pxy;
           r.wave = h->dwaves;
@@ -555,6 +665,7 @@ int prepare(fdtd_handle *h) {
   int rc = check_ready(h);
   if (rc) return rc;
   if ((rc = build_ade(h))) return rc;
+  if ((rc = build_mur(h))) return rc;
   if ((rc = build_nodal(h))) return rc;
   if ((rc = build_planewaves(h))) return rc;
   if ((rc = build_dfts(h))) return rc;
@@ -562,23 +673,30 @@ int prepare(fdtd_handle *h) {
 }
 
 int run_e(fdtd_handle *h, int yee, int cpml, int ade) {
-  const int c = cpml && h->g.has_cpml;
-  const int a = ade && h->g.has_ade;
-  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_e(h->g, h->tm, yee, c, a, h->stream)
-                                     : fast::launch_e(h->g, h->tm, yee, c, a, h->stream);
+  if (h->g.fp16 && (yee != cpml || h->g.nmedia + 1 > 4096))
+    return fail(h, FDTD_ERR_STATE, "binary16 storage runs the fused passes only (<= 4095 media)");
+  // the fused pass (yee and cpml) always runs the CPML kernel: without slab
+  // layers it updates no psi
+  const int c = cpml && (h->g.has_cpml || yee);
+  const int a = ade && h->g.ade[0].has;
+  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_e(h->g, h->tm, h->tm4, yee, c, a, h->stream)
+                                     : fast::launch_e(h->g, h->tm, h->tm4, yee, c, a, h->stream);
   if (e) return fail(h, FDTD_ERR_HIP, "E pass launch failed: %s", hipGetErrorString((hipError_t)e));
   return FDTD_OK;
 }
 
-int run_ade(fdtd_handle *h) { return run_e(h, 0, 0, 1); }
-
-int run_h(fdtd_handle *h, int yee, int cpml) {
-  const int c = cpml && h->g.has_cpml;
-  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_h(h->g, h->tm, yee, c, h->stream)
-                                     : fast::launch_h(h->g, h->tm, yee, c, h->stream);
+int run_h(fdtd_handle *h, int yee, int cpml, int ade) {
+  if (h->g.fp16 && (yee != cpml || h->g.nmedia + 1 > 4096))
+    return fail(h, FDTD_ERR_STATE, "binary16 storage runs the fused passes only (<= 4095 media)");
+  const int c = cpml && (h->g.has_cpml || yee);
+  const int a = ade && h->g.ade[1].has;
+  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_h(h->g, h->tm, h->tm4, yee, c, a, h->stream)
+                                     : fast::launch_h(h->g, h->tm, h->tm4, yee, c, a, h->stream);
   if (e) return fail(h, FDTD_ERR_HIP, "H pass launch failed: %s", hipGetErrorString((hipError_t)e));
   return FDTD_OK;
 }
+
+int run_ade(fdtd_handle *h, int side) { return side ? run_h(h, 0, 0, 1) : run_e(h, 0, 0, 1); }
 
 // UpdateObservation's frequency-domain sums at step n (or *dn)
 int run_dft(fdtd_handle *h, int n, const int *dn) {
@@ -637,8 +755,10 @@ int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
     return fail(nullptr, FDTD_ERR_ARG, "grid %dx%dx%d must be at least 1 cell per axis", d->nx, d->ny, d->nz);
   if (d->mode != FDTD_MODE_FAST && d->mode != FDTD_MODE_EXACT)
     return fail(nullptr, FDTD_ERR_ARG, "unknown mode %d", d->mode);
-  if (d->storage != FDTD_STORE_FP32)
-    return fail(nullptr, FDTD_ERR_ARG, "storage %d not available in this build", d->storage);
+  if (d->storage != FDTD_STORE_FP32 && d->storage != FDTD_STORE_FP16)
+    return fail(nullptr, FDTD_ERR_ARG, "unknown storage %d", d->storage);
+  if (d->storage == FDTD_STORE_FP16 && d->mode != FDTD_MODE_FAST)
+    return fail(nullptr, FDTD_ERR_ARG, "binary16 field storage runs in FAST mode only");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(nullptr, FDTD_ERR_DEVICE, "no HIP device available");
@@ -666,13 +786,20 @@ int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
   }
   DevGrid &g = h->g;
   g.nx = d->nx; g.ny = d->ny; g.nz = d->nz;
-  g.px = ((long long)kXOff + d->nx + 2 + 31) / 32 * 32;
+  g.px = ((long long)kXOff + d->nx + 8 + 31) / 32 * 32;
   g.pxy = g.px * (long long)(d->ny + 3);
   const size_t ne = field_elems(h);
+  // binary16 storage: E scaled by 1 (V/m), H by 1/eta0 so both sit near 1
+  g.fp16 = d->storage == FDTD_STORE_FP16 ? 1 : 0;
+  for (int c = 0; c < 6; ++c) {
+    g.fs[c] = (g.fp16 && c >= 3) ? 1.0f / 376.730313f : 1.0f;
+    g.fi[c] = 1.0f / g.fs[c];
+  }
+  const size_t fes = field_bytes(h);
   int rc;
   for (int c = 0; c < 6; ++c) {
-    if ((rc = dalloc(h, (void **)&h->fld[c], ne * 4)) ||
-        (rc = dalloc(h, (void **)&h->mi[c], ne * 2)) || (rc = dzero(h, h->fld[c], ne * 4)) ||
+    if ((rc = dalloc(h, (void **)&h->fld[c], ne * fes)) ||
+        (rc = dalloc(h, (void **)&h->mi[c], ne * 2)) || (rc = dzero(h, h->fld[c], ne * fes)) ||
         (rc = dzero(h, h->mi[c], ne * 2))) {
       g_create_error = h->err;
       fdtd_destroy(h);
@@ -686,8 +813,9 @@ int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
     fdtd_destroy(h);
     return rc;
   }
-  // tiles: 64 x 4 threads, z chunks of 32 planes (or fewer for thin grids)
-  set_tiles(h, 4, 32);
+  // tiles: 64 x 4 threads, z chunks of 32 planes (or fewer for thin grids),
+  // one x cell per lane
+  set_tiles(h, 4, 32, 1);
   hipStreamSynchronize(h->stream);
   *out = h;
   return FDTD_OK;
@@ -731,6 +859,7 @@ int fdtd_set_media(fdtd_handle *h, int32_t nmedia, const float *g1, const float 
   h->nmedia = nmedia;
   h->g.nmedia = nmedia;
   h->ade_ready = false;
+  h->mur_ready = false;
   h->nodal_ready = false;
   h->pw_ready = false;
   invalidate_graph(h);
@@ -842,21 +971,50 @@ int fdtd_set_cpml(fdtd_handle *h, const int32_t layers[6], const float *pbe_x,
   return FDTD_OK;
 }
 
-int fdtd_set_edispersive(fdtd_handle *h, int32_t ndisp, const int32_t *medium,
-                         const int32_t *npoles, const double *kappa, const double *beta,
-                         const double *g3) {
+static int set_dispersive(fdtd_handle *h, int side, int32_t ndisp, const int32_t *medium,
+                          const int32_t *npoles, const double *kappa, const double *beta,
+                          const double *g3) {
   if (!h || ndisp < 0 || (ndisp > 0 && (!medium || !npoles || !kappa || !beta || !g3)))
     return fail(h, FDTD_ERR_ARG, "bad dispersive arguments");
   for (int d = 0; d < ndisp; ++d)
     if (npoles[d] < 0 || npoles[d] > kPMax)
       return fail(h, FDTD_ERR_ARG, "medium %d has %d poles (max %d)", medium[d], npoles[d], kPMax);
-  h->ndisp = ndisp;
-  h->dmed.assign(medium, medium + ndisp);
-  h->dnp.assign(npoles, npoles + ndisp);
-  h->dk.assign(kappa, kappa + (size_t)ndisp * kPMax * 2);
-  h->db.assign(beta, beta + (size_t)ndisp * kPMax * 2);
-  h->dg3.assign(g3, g3 + (size_t)ndisp * kPMax * 2);
+  sfd_host::AdeIn &in = h->adein[side];
+  in.ndisp = ndisp;
+  in.dmed.assign(medium, medium + ndisp);
+  in.dnp.assign(npoles, npoles + ndisp);
+  in.dk.assign(kappa, kappa + (size_t)ndisp * kPMax * 2);
+  in.db.assign(beta, beta + (size_t)ndisp * kPMax * 2);
+  in.dg3.assign(g3, g3 + (size_t)ndisp * kPMax * 2);
   h->ade_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_set_edispersive(fdtd_handle *h, int32_t ndisp, const int32_t *medium,
+                         const int32_t *npoles, const double *kappa, const double *beta,
+                         const double *g3) {
+  return set_dispersive(h, 0, ndisp, medium, npoles, kappa, beta, g3);
+}
+
+int fdtd_set_mdispersive(fdtd_handle *h, int32_t ndisp, const int32_t *medium,
+                         const int32_t *npoles, const double *kappa, const double *beta,
+                         const double *gm3) {
+  return set_dispersive(h, 1, ndisp, medium, npoles, kappa, beta, gm3);
+}
+
+int fdtd_set_mur(fdtd_handle *h, const int32_t faces[6], const float *cab1) {
+  if (!h || !faces) return fail(h, FDTD_ERR_ARG, "bad Mur arguments");
+  if (h->nmedia < 0) return fail(h, FDTD_ERR_STATE, "fdtd_set_media not called");
+  bool any = false;
+  for (int f = 0; f < 6; ++f) {
+    h->mur_faces[f] = faces[f] ? 1 : 0;
+    any = any || faces[f];
+  }
+  if (any && !cab1) return fail(h, FDTD_ERR_ARG, "Mur coefficients missing");
+  if (any) h->cab1h.assign(cab1, cab1 + (size_t)6 * (h->nmedia + 1));
+  else h->cab1h.clear();
+  h->mur_ready = false;
   invalidate_graph(h);
   return FDTD_OK;
 }
@@ -901,22 +1059,60 @@ int fdtd_add_nodal(fdtd_handle *h, int32_t comp, int32_t hard, const int32_t box
   return FDTD_OK;
 }
 
-int fdtd_set_tiling(fdtd_handle *h, int32_t rows, int32_t zchunk) {
-  if (!h || !(rows == 1 || rows == 2 || rows == 4) || zchunk < 1)
-    return fail(h, FDTD_ERR_ARG, "tiling %d rows x %d planes not supported", rows, zchunk);
+int fdtd_set_tiling(fdtd_handle *h, int32_t rows, int32_t zchunk, int32_t xcells) {
+  if (!h || !(rows == 1 || rows == 2 || rows == 4) || zchunk < 1 || !(xcells == 1 || xcells == 4))
+    return fail(h, FDTD_ERR_ARG, "tiling %d rows x %d planes x %d cells not supported", rows,
+                zchunk, xcells);
   const long long nblk = (long long)((h->g.nx + kWave) / kWave) * ((h->g.ny + rows) / rows) *
                          ((h->g.nz + zchunk) / zchunk);
   if (nblk > 0x7fffffffLL) return fail(h, FDTD_ERR_ARG, "tiling gives too many blocks");
-  set_tiles(h, rows, zchunk);
+  set_tiles(h, rows, zchunk, xcells);
   invalidate_graph(h);
   return FDTD_OK;
 }
 
-int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk) {
-  if (!h || !rows || !zchunk) return FDTD_ERR_ARG;
+int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk, int32_t *xcells) {
+  if (!h || !rows || !zchunk || !xcells) return FDTD_ERR_ARG;
   *rows = h->tm.bdy;
   *zchunk = h->tm.kc;
+  *xcells = h->g.fp16 ? 4 : h->tm4.vec;
   return FDTD_OK;
+}
+
+int fdtd_time_halves(fdtd_handle *h, int32_t reps, float ms[3]) {
+  if (!h || reps < 1 || !ms) return FDTD_ERR_ARG;
+  hipSetDevice(h->device);
+  int rc = prepare(h);
+  if (rc) return rc;
+  hipEvent_t ev[4];
+  for (int i = 0; i < 4; ++i)
+    if (hipEventCreate(&ev[i]) != hipSuccess) {
+      for (int j = 0; j < i; ++j) hipEventDestroy(ev[j]);
+      return fail(h, FDTD_ERR_HIP, "hipEventCreate failed");
+    }
+  // one untimed E+H step, then reps of each half
+  if (!(rc = run_e(h, 1, 1, 1)) && !(rc = run_h(h, 1, 1, 1))) {
+    hipEventRecord(ev[0], h->stream);
+    for (int r = 0; r < reps && !rc; ++r) rc = run_e(h, 1, 1, 1);
+    hipEventRecord(ev[1], h->stream);
+    for (int r = 0; r < reps && !rc; ++r) rc = run_h(h, 1, 1, 1);
+    hipEventRecord(ev[2], h->stream);
+    for (int r = 0; r < reps && !rc; ++r) rc = run_e(h, 0, 0, 1);
+    hipEventRecord(ev[3], h->stream);
+  }
+  if (!rc) {
+    if (hipEventSynchronize(ev[3]) != hipSuccess) {
+      rc = fail(h, FDTD_ERR_HIP, "timing events failed");
+    } else {
+      for (int i = 0; i < 3; ++i) {
+        float t = 0.f;
+        hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+        ms[i] = t / reps;
+      }
+    }
+  }
+  for (int i = 0; i < 4; ++i) hipEventDestroy(ev[i]);
+  return rc;
 }
 
 int fdtd_add_dft(fdtd_handle *h, int32_t kind, const int32_t box[6], const int32_t stride[3],
@@ -1042,8 +1238,8 @@ int fdtd_advance_h(fdtd_handle *h, int32_t n) {
   if (!h) return FDTD_ERR_ARG;
   hipSetDevice(h->device);
   int rc;
-  if ((rc = prepare(h)) || (rc = run_h(h, 1, 1)) || (rc = run_pw(h, 1, n, nullptr)) ||
-      (rc = run_nodal(h, 1, n, nullptr)))
+  if ((rc = prepare(h)) || (rc = run_h(h, 1, 1, 1)) || (rc = run_pw(h, 1, n, nullptr)) ||
+      (rc = run_nodal(h, 1, n, nullptr)) || (rc = run_mur(h)))
     return rc;
   return FDTD_OK;
 }
@@ -1061,8 +1257,10 @@ SFD_GRANULAR(fdtd_advance_e_cpml(fdtd_handle *h), run_e(h, 0, 1, 0))
 SFD_GRANULAR(fdtd_advance_e_dispersive(fdtd_handle *h), run_e(h, 0, 0, 1))
 SFD_GRANULAR(fdtd_advance_planewave_e(fdtd_handle *h, int32_t n), run_pw(h, 0, n, nullptr))
 SFD_GRANULAR(fdtd_advance_nodal_e(fdtd_handle *h, int32_t n), run_nodal(h, 0, n, nullptr))
-SFD_GRANULAR(fdtd_advance_h_yee(fdtd_handle *h), run_h(h, 1, 0))
-SFD_GRANULAR(fdtd_advance_h_cpml(fdtd_handle *h), run_h(h, 0, 1))
+SFD_GRANULAR(fdtd_advance_h_yee(fdtd_handle *h), run_h(h, 1, 0, 0))
+SFD_GRANULAR(fdtd_advance_h_cpml(fdtd_handle *h), run_h(h, 0, 1, 0))
+SFD_GRANULAR(fdtd_advance_h_dispersive(fdtd_handle *h), run_h(h, 0, 0, 1))
+SFD_GRANULAR(fdtd_advance_mur(fdtd_handle *h), run_mur(h))
 SFD_GRANULAR(fdtd_advance_planewave_h(fdtd_handle *h, int32_t n), run_pw(h, 1, n, nullptr))
 SFD_GRANULAR(fdtd_advance_nodal_h(fdtd_handle *h, int32_t n), run_nodal(h, 1, n, nullptr))
 #undef SFD_GRANULAR
@@ -1086,9 +1284,10 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
     int r2 = run_e(h, 1, 1, 1);
     if (!r2) r2 = run_pw(h, 0, 0, h->dstep);
     if (!r2) r2 = run_nodal(h, 0, 0, h->dstep);
-    if (!r2) r2 = run_h(h, 1, 1);
+    if (!r2) r2 = run_h(h, 1, 1, 1);
     if (!r2) r2 = run_pw(h, 1, 0, h->dstep);
     if (!r2) r2 = run_nodal(h, 1, 0, h->dstep);
+    if (!r2) r2 = run_mur(h);
     if (!r2 && h->nprobe)
       r2 = launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap, h->stream)
                ? fail(h, FDTD_ERR_HIP, "probe launch failed") : 0;
@@ -1163,13 +1362,33 @@ int fdtd_read_probes(fdtd_handle *h, float *out, int32_t max_samples, int32_t *n
 int fdtd_get_field(fdtd_handle *h, int32_t comp, float *host) {
   if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field read");
   hipSetDevice(h->device);
-  return copy_box(h, comp, h->fld[comp], nullptr, host, 4);
+  return field_io(h, comp, nullptr, host);
 }
 
 int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host) {
   if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field write");
   hipSetDevice(h->device);
-  return copy_box(h, comp, h->fld[comp], host, nullptr, 4);
+  return field_io(h, comp, host, nullptr);
+}
+
+int fdtd_set_field_scale(fdtd_handle *h, int32_t comp, float scale) {
+  if (!h || comp < 0 || comp > 5 || !(scale > 0.0f) || !std::isfinite(scale))
+    return fail(h, FDTD_ERR_ARG, "bad field scale");
+  if (h->storage != FDTD_STORE_FP16) return FDTD_OK;  // fp32 fields carry no scale
+  hipSetDevice(h->device);
+  // keep the stored values: read at the old scale, write at the new one
+  long long ext[3];
+  alloc_extent(h, comp, ext);
+  std::vector<float> tmp((size_t)(ext[0] * ext[1] * ext[2]));
+  int rc = field_io(h, comp, nullptr, tmp.data());
+  if (rc) return rc;
+  h->g.fs[comp] = scale;
+  h->g.fi[comp] = 1.0f / scale;
+  rc = field_io(h, comp, tmp.data(), nullptr);
+  if (rc) return rc;
+  h->pw_ready = false;
+  invalidate_graph(h);
+  return FDTD_OK;
 }
 
 int fdtd_synchronize(fdtd_handle *h) {
@@ -1182,8 +1401,9 @@ int fdtd_synchronize(fdtd_handle *h) {
 int64_t fdtd_device_bytes(const fdtd_handle *h) { return h ? h->bytes : 0; }
 
 int64_t fdtd_dispersive_edges(const fdtd_handle *h, int32_t comp) {
-  if (!h || comp < 0 || comp > 2 || !h->ade_ready) return -1;
-  return h->g.has_ade ? (int64_t)h->nedges[comp] : 0;
+  if (!h || comp < 0 || comp > 5 || !h->ade_ready) return -1;
+  const int side = comp / 3;
+  return h->g.ade[side].has ? (int64_t)h->adein[side].nedges[comp % 3] : 0;
 }
 
 int fdtd_media_fill(fdtd_handle *h, int16_t medium) {
@@ -1259,8 +1479,10 @@ int fdtd_media_paint_box(fdtd_handle *h, int32_t x0, int32_t x1, int32_t y0, int
 // ---- checkpoint ------------------------------------------------------------
 // The complete time-stepping state of a grid: the six fields (Alloc boxes, as
 // StoreFields writes them, resuming.F90:394-496), the CPML psi arrays (the
-// reference's StoreFieldsCPMLBorders counterpart), the pole state (Eprev and
-// the pole currents, StoreFieldsEDispersives) and the frequency-domain sums,
+// reference's StoreFieldsCPMLBorders counterpart), the pole state (previous
+// field and pole currents of the electric then the magnetic media,
+// StoreFieldsEDispersives, StoreFieldsMdispersives) and the frequency-domain
+// sums,
 // behind a header naming the step.  The layout of the pole state depends on
 // the mode, so a state loads into a grid of the same set-up and mode.
 namespace {
@@ -1285,20 +1507,20 @@ std::vector<Section> state_sections(fdtd_handle *h) {
   for (int c = 0; c < 6; ++c) {
     long long ext[3];
     alloc_extent(h, c, ext);
-    v.push_back({h->fld[c], (size_t)(ext[0] * ext[1] * ext[2]) * 4, c});
+    v.push_back({h->fld[c], (size_t)(ext[0] * ext[1] * ext[2]) * field_bytes(h), c});
   }
   for (int s = 0; s < 12; ++s)
     if (h->g.psi[s] && h->psilen[s] > 0) v.push_back({h->g.psi[s], (size_t)h->psilen[s] * 4, -1});
-  if (h->g.has_ade) {
-    int maxp = 0;
-    for (int d = 0; d < h->ndisp; ++d) maxp = std::max(maxp, h->dnp[d]);
-    const size_t js = h->mode == FDTD_MODE_EXACT ? 16 : 8;
+  for (int side = 0; side < 2; ++side) {
+    const AdeSet &ad = h->g.ade[side];
+    if (!ad.has) continue;
     for (int c = 0; c < 3; ++c) {
-      const size_t ns = (size_t)std::max(h->g.nstate[c], 1LL);
-      v.push_back({h->g.eprev[c], ns * 4, -1});
-      v.push_back({h->g.jst[c], ns * std::max(maxp, 1) * js, -1});
+      const size_t ns = (size_t)std::max(ad.nstate[c], 1LL);
+      v.push_back({ad.prev[c], ns * 4, -1});
+      v.push_back({ad.jst[c], h->adein[side].jbytes[c], -1});
     }
   }
+  for (const MurRun &r : h->mur_runs) v.push_back({r.past, (size_t)2 * r.nu * r.nv * 4, -1});
   for (const DftDev &d : h->dfts)
     v.push_back({d.acc, (size_t)d.cnt[0] * d.cnt[1] * d.cnt[2] * 3 * d.nfreq * sizeof(double2), -1});
   return v;
@@ -1324,14 +1546,14 @@ int fdtd_save_state(fdtd_handle *h, int32_t n, void *buf, int64_t size) {
   if (size < need) return fail(h, FDTD_ERR_ARG, "state needs %lld bytes, %lld given",
                                  (long long)need, (long long)size);
   std::vector<Section> secs = state_sections(h);
-  StateHeader hd{{'S', 'F', 'D', 'S'}, 1, h->mode, h->g.nx, h->g.ny, h->g.nz, n,
+  StateHeader hd{{'S', 'F', 'D', 'S'}, 1, h->mode * 2 + (h->g.fp16 ? 1 : 0), h->g.nx, h->g.ny, h->g.nz, n,
                  (int32_t)secs.size()};
   char *p = (char *)buf;
   std::memcpy(p, &hd, sizeof hd);
   p += sizeof hd;
   for (const Section &s : secs) {
     if (s.comp >= 0) {
-      int rc = copy_box(h, s.comp, s.dev, nullptr, p, 4);
+      int rc = copy_box(h, s.comp, s.dev, nullptr, p, field_bytes(h));
       if (rc) return rc;
     } else {
       HIPCHK(h, hipMemcpyAsync(p, s.dev, s.bytes, hipMemcpyDeviceToHost, h->stream));
@@ -1352,13 +1574,13 @@ int fdtd_load_state(fdtd_handle *h, const void *buf, int64_t size, int32_t *n) {
   std::vector<Section> secs = state_sections(h);
   if (std::memcmp(hd.magic, "SFDS", 4) != 0 || hd.version != 1)
     return fail(h, FDTD_ERR_ARG, "not a version-1 state");
-  if (hd.mode != h->mode || hd.nx != h->g.nx || hd.ny != h->g.ny || hd.nz != h->g.nz ||
+  if (hd.mode != h->mode * 2 + (h->g.fp16 ? 1 : 0) || hd.nx != h->g.nx || hd.ny != h->g.ny || hd.nz != h->g.nz ||
       hd.nsections != (int32_t)secs.size() || size != need)
     return fail(h, FDTD_ERR_ARG, "state of a different grid, mode or set-up");
   const char *p = (const char *)buf + sizeof hd;
   for (const Section &s : secs) {
     if (s.comp >= 0) {
-      int rc = copy_box(h, s.comp, s.dev, p, nullptr, 4);
+      int rc = copy_box(h, s.comp, s.dev, p, nullptr, field_bytes(h));
       if (rc) return rc;
     } else {
       HIPCHK(h, hipMemcpyAsync(s.dev, p, s.bytes, hipMemcpyHostToDevice, h->stream));

@@ -5,8 +5,10 @@
 //   fields      6 arrays (Ex Ey Ez Hx Hy Hz), each (nz+3) planes x (ny+3) rows
 //               x px elements; cell (i,j,k) at ((k+1)*(ny+3)+(j+1))*px + xoff+i.
 //               xoff = 32 puts x = 0 on a 128-byte line boundary in every row
-//               (px is a multiple of 32), the Alloc padding planes/rows/columns
-//               (-1 and n+1) stay zero forever.
+//               (px is a multiple of 32 with room for x up to n+5, so the last
+//               16-byte vector of a row and its right neighbour stay in the
+//               row), the Alloc padding planes/rows/columns (-1 and n+1) stay
+//               zero forever.
 //   media idx   6 int16 arrays with the same geometry as the fields.
 //   tables      G1/G2/GM1/GM2 and kind bits per media index; inverse spacings
 //               (length n+3, index i+1); CPML profiles per axis.
@@ -14,7 +16,8 @@
 //               stretched axis only the low and high slab layers are stored
 //               (Llo + Lhi entries), the two other axes span the component's
 //               sweep; x-fastest.
-//   pole state  per E component a dense x-fastest box spanning the
+//   pole state  per E component (electric media) and per H component
+//               (magnetic media) a dense x-fastest box spanning the
 //               component's dispersive edges (Eprev and one complex state per
 //               stored pole); an edge finds its slot from its coordinates, so
 //               the state loads issue alongside the field loads.  Only
@@ -22,6 +25,9 @@
 //               not bandwidth.
 #pragma once
 #include <stdint.h>
+#ifdef __HIPCC__
+#include <hip/hip_fp16.h>
+#endif
 
 namespace sfd {
 
@@ -32,11 +38,30 @@ constexpr int kXOff = 32;    // x = 0 offset inside a row (elements)
 enum { PSI_EXY = 0, PSI_EXZ, PSI_EYX, PSI_EYZ, PSI_EZX, PSI_EZY,
        PSI_HXY, PSI_HXZ, PSI_HYX, PSI_HYZ, PSI_HZX, PSI_HZY };
 
+// Pole-residue state of one field: the electric media acting on E
+// (electricdispersive.F90) or the magnetic media acting on H
+// (magneticdispersive.F90); the two share one recursion.
+struct AdeSet {
+  int has;                      // some edge of the field is dispersive
+  int ndisp;                    // dispersive media (slots)
+  const int32_t *dslot;         // [nmedia+1] dispersive slot or -1
+  int abox[3][6];               // per component: x0,y0,z0,bx,by,bz of the
+                                // box holding its dispersive edges
+  float *prev[3];               // field at the previous step, per box slot
+  void *jst[3];                 // FAST: float2[P][N], EXACT: double2[P][N]
+  long long nstate[3];
+  const int32_t *anp;           // per slot: stored poles
+  const float *adef;            // FAST: per slot per pole {kr,ki,bdtr,bdti,g3r,g3i}
+  const double *aded;           // EXACT: per slot per pole {kr,ki,br,bi,g3r,g3i}
+};
+
 struct DevGrid {
   int nx, ny, nz;
   int nmedia;
   long long px, pxy;            // row pitch and plane stride in elements
-  float *f[6];
+  float *f[6];                  // fp32 fields, or binary16 ones when fp16
+  int fp16;                     // fields stored as binary16 * fs[c]
+  float fs[6], fi[6];           // per-array scale and its inverse
   const int16_t *mi[6];
   const float *g1, *g2, *gm1, *gm2;
   const float *id[6];           // Idxe Idye Idze Idxh Idyh Idzh
@@ -45,18 +70,7 @@ struct DevGrid {
   const float *pbe[3], *pce[3], *pbm[3], *pcm[3];
   float *psi[12];
   int has_cpml;
-  // electric pole-residue state
-  int has_ade;
-  int ndisp;                    // dispersive media (slots)
-  const int32_t *dslot;         // [nmedia+1] dispersive slot or -1
-  int abox[3][6];               // per E component: x0,y0,z0,bx,by,bz of the
-                                // box holding its dispersive edges
-  float *eprev[3];
-  void *jst[3];                 // FAST: float2[P][N], EXACT: double2[P][N]
-  long long nstate[3];
-  const int32_t *anp;           // per slot: stored poles
-  const float *adef;            // FAST: per slot per pole {kr,ki,bdtr,bdti,g3r,g3i}
-  const double *aded;           // EXACT: per slot per pole {kr,ki,br,bi,g3r,g3i}
+  AdeSet ade[2];                // 0: electric media (E), 1: magnetic media (H)
   double dt;
 };
 
@@ -69,6 +83,21 @@ struct TileMap {
   int nblk;
   int remap;
   int bdy;          // rows per block (block = 64 x bdy threads, <= 256)
+  int vec;          // x cells per lane of the fused passes (1 or 4)
+};
+
+// First-order Mur (bordersmur.F90): one tangential H component on one
+// absorbing face.  The plane just outside the component's sweep (ghost) is
+// advanced from the first plane inside (inner); past keeps both planes of
+// the previous step.
+struct MurRun {
+  int comp;                // 3..5
+  int axis, ghost, inner;  // face normal and the two planes along it
+  int ua, va;              // the two other axes (u fastest)
+  int nu, nv;              // extents: the component's sweep along ua, va
+  long long start;         // first cell of this run in the launch
+  float *past;             // [2][nv][nu]: ghost plane, then inner plane
+  const float *cab1;       // [nmedia+1] coefficients of the face
 };
 
 struct NodalCell {   // one cell touched by a nodal source, in reference order
@@ -111,6 +140,8 @@ struct PwFace {
 struct PwRun {
   PwFace f;
   float *F;                   // the updated component
+  int fp16;                   // F holds binary16 values * fs
+  float fs, fi;
   long long px, pxy;          // its row pitch and plane stride
   const PwWave *wave;
   const float *phys;
@@ -149,15 +180,27 @@ struct BatchDev {
   int total;
 };
 
+#ifdef __HIPCC__
+// one field value of component c at p, whatever the storage
+__device__ __forceinline__ float field_ld(const DevGrid &g, int c, long long p) {
+  return g.fp16 ? __half2float(reinterpret_cast<const __half *>(g.f[c])[p]) * g.fs[c]
+                : g.f[c][p];
+}
+__device__ __forceinline__ void field_st(const DevGrid &g, int c, long long p, float v) {
+  if (g.fp16) reinterpret_cast<__half *>(g.f[c])[p] = __float2half_rn(v * g.fi[c]);
+  else g.f[c][p] = v;
+}
+#endif
+
 // launchers, one set per arithmetic mode
 #define SFD_DECLARE_LAUNCHERS(NS)                                              \
   namespace NS {                                                             \
-  int launch_e(const DevGrid &g, const TileMap &tm, int yee, int cpml,       \
-               int ade, void *stream);                                       \
-  int launch_h(const DevGrid &g, const TileMap &tm, int yee, int cpml,       \
-               void *stream);                                                \
+  int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4,      \
+               int yee, int cpml, int ade, void *stream);                    \
+  int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4,      \
+               int yee, int cpml, int ade, void *stream);                    \
   int launch_pass_batch(const BatchDev &bd, int half, int blocks, int rows,  \
-                        size_t lds, void *stream);                           \
+                        int vec, size_t lds, int fp16, void *stream);        \
   }
 SFD_DECLARE_LAUNCHERS(fast)
 SFD_DECLARE_LAUNCHERS(exact)
@@ -179,7 +222,9 @@ int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
                   int *count, int cap, void *stream);
 // bounding box {xmin,ymin,zmin,xmax,ymax,zmax} of the dispersive edges of
 // component comp; out must hold {INT_MAX x3, INT_MIN x3} on entry
-int launch_disp_bbox(const DevGrid &g, int comp, int32_t *out, void *stream);
+int launch_disp_bbox(const DevGrid &g, int side, int c, int32_t *out, void *stream);
+int launch_mur(const DevGrid &g, const MurRun *runs, int nrun, long long total, int exact,
+               void *stream);
 int launch_fill_i16(int16_t *a, long long n, int16_t v, void *stream);
 int launch_pec_walls(const DevGrid &g, int16_t *mi[6], int faces, void *stream);
 int launch_paint(const DevGrid &g, int16_t *mi[6], int shape, const float *par,

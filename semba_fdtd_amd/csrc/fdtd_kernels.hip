@@ -27,6 +27,7 @@
 // state of dispersive edges) and only then computes, so the pole state's
 // HBM latency overlaps the curl arithmetic.  The thread domain covers
 // [0,nx]x[0,ny]x[0,nz]; each component updates the part inside its sweep.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include "fdtd_internal.h"
 
@@ -86,10 +87,39 @@ __device__ __forceinline__ bool slab_cell(int c, int n, int L0, int L1, int &q) 
   return false;
 }
 
+// The update arithmetic.  EXACT: plain products and sums (the file is built
+// without contraction), the reference's order.  FAST: explicit fused
+// multiply-adds, so that every variant of a pass (one or four cells per lane,
+// one grid or a batch) rounds the same way and gives the same bits.
+// a*ia - b*ib
+__device__ __forceinline__ float curl2(float a, float ia, float b, float ib) {
+#if SFD_EXACT
+  return a * ia - b * ib;
+#else
+  return __builtin_fmaf(a, ia, -(b * ib));
+#endif
+}
+// g1*f + g2*cu
+__device__ __forceinline__ float yee_upd(float g1, float f, float g2, float cu) {
+#if SFD_EXACT
+  return g1 * f + g2 * cu;
+#else
+  return __builtin_fmaf(g1, f, g2 * cu);
+#endif
+}
+// x + g*v
+__device__ __forceinline__ float madd(float x, float g, float v) {
+#if SFD_EXACT
+  return x + g * v;
+#else
+  return __builtin_fmaf(g, v, x);
+#endif
+}
+
 // psi <- b*psi + d*c ; returns the new psi
 __device__ __forceinline__ float psi_step(float *psi, long long s, float b,
                                           float d, float c) {
-  const float v = b * psi[s] + d * c;
+  const float v = madd(d * c, b, psi[s]);
   psi[s] = v;
   return v;
 }
@@ -102,35 +132,38 @@ struct MedTab {
   const coef_t *acf;    // per slot per pole {kr,ki,br,bi,g3r,g3i}
 };
 
-__device__ __forceinline__ const coef_t *ade_coefs(const DevGrid &g) {
+__device__ __forceinline__ const coef_t *ade_coefs(const AdeSet &ad) {
 #if SFD_EXACT
-  return g.aded;
+  return ad.aded;
 #else
-  return g.adef;
+  return ad.adef;
 #endif
 }
 
-// LDS layout: gg[nm+1] | slot[nm+1] | anp[ndisp] | (8-byte aligned) acf
+// LDS layout: gg[nm+1] | slot[nm+1] | anp[ndisp] | (8-byte aligned) acf.
+// ad: the pole state whose tables are staged (ADE only).
 template <bool TBL, bool ADE>
 __device__ __forceinline__ MedTab stage_tables(const DevGrid &g, const float *a,
-                                               const float *b, float2 *lds) {
+                                               const float *b, float2 *lds,
+                                               const AdeSet *ad = nullptr) {
   MedTab t;
   const int nm = g.nmedia;
   if (TBL) {
     int *ls = reinterpret_cast<int *>(lds + (nm + 1));
     int *lnp = ls + (nm + 1);
-    const size_t off = ((size_t)(nm + 1) * 12 + (size_t)g.ndisp * 4 + 7) & ~(size_t)7;
+    const int nd = ADE ? ad->ndisp : 0;
+    const size_t off = ((size_t)(nm + 1) * 12 + (size_t)nd * 4 + 7) & ~(size_t)7;
     coef_t *lcf = reinterpret_cast<coef_t *>(reinterpret_cast<char *>(lds) + off);
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     const int nt = blockDim.x * blockDim.y;
     for (int m = tid; m <= nm; m += nt) {
       lds[m] = make_float2(a[m], b[m]);
-      if (ADE) ls[m] = g.dslot[m];
+      if (ADE) ls[m] = ad->dslot[m];
     }
     if (ADE) {
-      const coef_t *cf = ade_coefs(g);
-      for (int d = tid; d < g.ndisp; d += nt) lnp[d] = g.anp[d];
-      for (int q = tid; q < g.ndisp * kPMax * 6; q += nt) lcf[q] = cf[q];
+      const coef_t *cf = ade_coefs(*ad);
+      for (int d = tid; d < nd; d += nt) lnp[d] = ad->anp[d];
+      for (int q = tid; q < nd * kPMax * 6; q += nt) lcf[q] = cf[q];
     }
     __syncthreads();
     t.gg = lds;
@@ -139,9 +172,9 @@ __device__ __forceinline__ MedTab stage_tables(const DevGrid &g, const float *a,
     t.acf = lcf;
   } else {
     t.gg = nullptr;
-    t.slot = g.dslot;
-    t.anp = g.anp;
-    t.acf = ade_coefs(g);
+    t.slot = ADE ? ad->dslot : nullptr;
+    t.anp = ADE ? ad->anp : nullptr;
+    t.acf = ADE ? ade_coefs(*ad) : nullptr;
   }
   return t;
 }
@@ -152,7 +185,8 @@ __device__ __forceinline__ float2 coef(const MedTab &t, const float *a, const fl
   return make_float2(a[m], b[m]);
 }
 
-// ---- electric pole-residue ADE (electricdispersive.F90:304-362) ----------
+// ---- pole-residue ADE (electricdispersive.F90:304-362 on E,
+// magneticdispersive.F90:306-362 on H, the same recursion) -----------------
 // The pole state of every dispersive edge of component C lives in a dense
 // x-fastest box (abox); s = the edge's slot in it.  AdePre holds the loaded
 // state of one edge.
@@ -164,13 +198,14 @@ struct AdePre {
   jstate_t j[kPre];
 };
 
-// E -= sum_p Re(G3_p J_p); J_p = kappa_p J_p + beta_p (E - Eprev)/dt; Eprev = E
+// F -= sum_p Re(G3_p J_p); J_p = kappa_p J_p + beta_p (F - Fprev)/dt;
+// Fprev = F  (F = E with G3, or H with GM3)
 template <int C>
-__device__ __forceinline__ float ade_apply(const DevGrid &g, const MedTab &t, const AdePre &a,
-                                           float e) {
+__device__ __forceinline__ float ade_apply(const AdeSet &ad, double dt, const MedTab &t,
+                                           const AdePre &a, float e) {
   const coef_t *cf = t.acf + (size_t)a.slot * kPMax * 6;
-  const long long N = g.nstate[C];
-  jstate_t *J = reinterpret_cast<jstate_t *>(g.jst[C]);
+  const long long N = ad.nstate[C];
+  jstate_t *J = reinterpret_cast<jstate_t *>(ad.jst[C]);
 #if SFD_EXACT
   // one pole at a time, rounding E after each (electricdispersive.F90:319-321)
 #pragma unroll
@@ -188,8 +223,8 @@ __device__ __forceinline__ float ade_apply(const DevGrid &g, const MedTab &t, co
     const double kr = cf[q * 6 + 0], ki = cf[q * 6 + 1];
     const double ar = kr * jv.x - ki * jv.y;
     const double ai = kr * jv.y + ki * jv.x;
-    const double br = cf[q * 6 + 2] * (double)diff / g.dt;
-    const double bi = cf[q * 6 + 3] * (double)diff / g.dt;
+    const double br = cf[q * 6 + 2] * (double)diff / dt;
+    const double bi = cf[q * 6 + 3] * (double)diff / dt;
     J[q * N + a.s] = make_double2(ar + br, ai + bi);
   }
 #else
@@ -212,21 +247,25 @@ __device__ __forceinline__ float ade_apply(const DevGrid &g, const MedTab &t, co
                                  kr * jv.y + ki * jv.x + cf[q * 6 + 3] * diff);
   }
 #endif
-  g.eprev[C][a.s] = e;
+  ad.prev[C][a.s] = e;
   return e;
 }
 
-// The dispersive-edge pass: AdvanceEDispersiveE over the pole-state box of
-// each E component (blockIdx.y), after the Yee+CPML E pass.  Edges of the box
-// are visited x-fastest, so the media index, E, Eprev and state loads of a
-// wave are each one contiguous run; non-dispersive edges cost only their
-// int16 media index.
-template <bool TBL>
+// The dispersive-edge pass: AdvanceEDispersiveE (SIDE 0) or
+// AdvanceMdispersiveH (SIDE 1) over the pole-state box of each component
+// (blockIdx.y), after the Yee+CPML pass of that field.  Edges of the box are
+// visited x-fastest, so the media index, field, previous-field and state
+// loads of a wave are each one contiguous run; non-dispersive edges cost only
+// their int16 media index.
+template <bool TBL, int SIDE>
 __global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
   extern __shared__ float2 lds_tab[];
-  const MedTab tab = stage_tables<TBL, true>(g, g.g1, g.g2, lds_tab);
+  const AdeSet &ad = g.ade[SIDE];
+  const MedTab tab = SIDE ? stage_tables<TBL, true>(g, g.gm1, g.gm2, lds_tab, &ad)
+                          : stage_tables<TBL, true>(g, g.g1, g.g2, lds_tab, &ad);
   const int c = blockIdx.y;
-  const int *b = g.abox[c];
+  const int f = 3 * SIDE + c;   // field component
+  const int *b = ad.abox[c];
   const long long bx = b[3], bxy = (long long)b[3] * b[4];
   const long long n = bxy * b[5];
   for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n;
@@ -234,25 +273,24 @@ __global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
     const int x = (int)(s % bx), y = (int)((s / bx) % b[4]), z = (int)(s / bxy);
     const long long p = (long long)(b[2] + z + 1) * g.pxy + (long long)(b[1] + y + 1) * g.px +
                         kXOff + b[0] + x;
-    const int m = g.mi[c][p];
+    const int m = g.mi[f][p];
     const int slot = tab.slot[m];
     if (slot < 0) continue;
-    float *F = g.f[c];
     AdePre a;
     a.s = s;
     a.slot = slot;
     a.np = tab.anp[slot];
-    a.ep = g.eprev[c][s];
-    const jstate_t *J = reinterpret_cast<const jstate_t *>(g.jst[c]);
+    a.ep = ad.prev[c][s];
+    const jstate_t *J = reinterpret_cast<const jstate_t *>(ad.jst[c]);
 #pragma unroll
     for (int q = 0; q < kPre; ++q)
-      if (q < a.np) a.j[q] = J[q * g.nstate[c] + s];
-    const float e = F[p];
+      if (q < a.np) a.j[q] = J[q * ad.nstate[c] + s];
+    const float e = field_ld(g, f, p);
     float r;
-    if (c == 0) r = ade_apply<0>(g, tab, a, e);
-    else if (c == 1) r = ade_apply<1>(g, tab, a, e);
-    else r = ade_apply<2>(g, tab, a, e);
-    F[p] = r;
+    if (c == 0) r = ade_apply<0>(ad, g.dt, tab, a, e);
+    else if (c == 1) r = ade_apply<1>(ad, g.dt, tab, a, e);
+    else r = ade_apply<2>(ad, g.dt, tab, a, e);
+    field_st(g, f, p, r);
   }
 }
 
@@ -314,15 +352,15 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
     if (cx) {
       float e = ex;
       const float2 c = coef<TBL>(tab, g.g1, g.g2, mx);
-      if (YEE) e = c.x * e + c.y * ((hz0 - hz_jm) * idyh - (hy0 - hy_km) * idzh);
+      if (YEE) e = yee_upd(c.x, e, c.y, curl2(hz0 - hz_jm, idyh, hy0 - hy_km, idzh));
       if (CPML) {
         if (syn) {  // psi_Exy, left/right (borderscpml.F90:609-623)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * nx + i;
-          e = e + c.y * psi_step(g.psi[PSI_EXY], s, g.pbe[1][j + 1], hz0 - hz_jm, g.pce[1][j + 1]);
+          e = madd(e, c.y, psi_step(g.psi[PSI_EXY], s, g.pbe[1][j + 1], hz0 - hz_jm, g.pce[1][j + 1]));
         }
         if (szn) {  // psi_Exz, down/up (:699-712)
           const long long s = ((long long)qzn * (ny + 1) + j) * nx + i;
-          e = e - c.y * psi_step(g.psi[PSI_EXZ], s, g.pbe[2][k + 1], hy0 - hy_km, g.pce[2][k + 1]);
+          e = madd(e, -c.y, psi_step(g.psi[PSI_EXZ], s, g.pbe[2][k + 1], hy0 - hy_km, g.pce[2][k + 1]));
         }
       }
       Ex[p] = e;
@@ -330,15 +368,15 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
     if (cy) {
       float e = ey;
       const float2 c = coef<TBL>(tab, g.g1, g.g2, my);
-      if (YEE) e = c.x * e + c.y * ((hx0 - hx_km) * idzh - (hz0 - hz_im) * idxh);
+      if (YEE) e = yee_upd(c.x, e, c.y, curl2(hx0 - hx_km, idzh, hz0 - hz_im, idxh));
       if (CPML) {
         if (szn) {  // psi_Eyz, down/up (:680-693)
           const long long s = ((long long)qzn * ny + j) * (nx + 1) + i;
-          e = e + c.y * psi_step(g.psi[PSI_EYZ], s, g.pbe[2][k + 1], hx0 - hx_km, g.pce[2][k + 1]);
+          e = madd(e, c.y, psi_step(g.psi[PSI_EYZ], s, g.pbe[2][k + 1], hx0 - hx_km, g.pce[2][k + 1]));
         }
         if (sxn) {  // psi_Eyx, back/front (:790-802)
           const long long s = ((long long)k * ny + j) * (g.L[0][0] + g.L[0][1]) + qxn;
-          e = e - c.y * psi_step(g.psi[PSI_EYX], s, g.pbe[0][i + 1], hz0 - hz_im, g.pce[0][i + 1]);
+          e = madd(e, -c.y, psi_step(g.psi[PSI_EYX], s, g.pbe[0][i + 1], hz0 - hz_im, g.pce[0][i + 1]));
         }
       }
       Ey[p] = e;
@@ -346,15 +384,15 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
     if (cz) {
       float e = ez;
       const float2 c = coef<TBL>(tab, g.g1, g.g2, mz);
-      if (YEE) e = c.x * e + c.y * ((hy0 - hy_im) * idxh - (hx0 - hx_jm) * idyh);
+      if (YEE) e = yee_upd(c.x, e, c.y, curl2(hy0 - hy_im, idxh, hx0 - hx_jm, idyh));
       if (CPML) {
         if (syn) {  // psi_Ezy, left/right (:630-642)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * (nx + 1) + i;
-          e = e - c.y * psi_step(g.psi[PSI_EZY], s, g.pbe[1][j + 1], hx0 - hx_jm, g.pce[1][j + 1]);
+          e = madd(e, -c.y, psi_step(g.psi[PSI_EZY], s, g.pbe[1][j + 1], hx0 - hx_jm, g.pce[1][j + 1]));
         }
         if (sxn) {  // psi_Ezx, back/front (:771-783)
           const long long s = ((long long)k * (ny + 1) + j) * (g.L[0][0] + g.L[0][1]) + qxn;
-          e = e + c.y * psi_step(g.psi[PSI_EZX], s, g.pbe[0][i + 1], hy0 - hy_im, g.pce[0][i + 1]);
+          e = madd(e, c.y, psi_step(g.psi[PSI_EZX], s, g.pbe[0][i + 1], hy0 - hy_im, g.pce[0][i + 1]));
         }
       }
       Ez[p] = e;
@@ -403,15 +441,15 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
     if (j < ny && k < nz) {  // Hx sweep: x[0,nx] y[0,ny-1] z[0,nz-1]
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[3][p]);
       float h = Hx[p];
-      if (YEE) h = c.x * h + c.y * ((ey_kp - ey_c) * idze - (ez_jp - ez_c) * idye);
+      if (YEE) h = yee_upd(c.x, h, c.y, curl2(ey_kp - ey_c, idze, ez_jp - ez_c, idye));
       if (CPML) {
         if (syc) {  // psi_Hxy (borderscpml.F90:911-921)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyc) * (nx + 1) + i;
-          h = h - c.y * psi_step(g.psi[PSI_HXY], s, g.pbm[1][j + 1], ez_jp - ez_c, g.pcm[1][j + 1]);
+          h = madd(h, -c.y, psi_step(g.psi[PSI_HXY], s, g.pbm[1][j + 1], ez_jp - ez_c, g.pcm[1][j + 1]));
         }
         if (szc) {  // psi_Hxz (:1021-1031)
           const long long s = ((long long)qzc * ny + j) * (nx + 1) + i;
-          h = h + c.y * psi_step(g.psi[PSI_HXZ], s, g.pbm[2][k + 1], ey_kp - ey_c, g.pcm[2][k + 1]);
+          h = madd(h, c.y, psi_step(g.psi[PSI_HXZ], s, g.pbm[2][k + 1], ey_kp - ey_c, g.pcm[2][k + 1]));
         }
       }
       Hx[p] = h;
@@ -419,15 +457,15 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
     if (i < nx && k < nz) {  // Hy sweep: x[0,nx-1] y[0,ny] z[0,nz-1]
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[4][p]);
       float h = Hy[p];
-      if (YEE) h = c.x * h + c.y * ((ez_ip - ez_c) * idxe - (ex_kp - ex_c) * idze);
+      if (YEE) h = yee_upd(c.x, h, c.y, curl2(ez_ip - ez_c, idxe, ex_kp - ex_c, idze));
       if (CPML) {
         if (szc) {  // psi_Hyz (:1001-1011)
           const long long s = ((long long)qzc * (ny + 1) + j) * nx + i;
-          h = h - c.y * psi_step(g.psi[PSI_HYZ], s, g.pbm[2][k + 1], ex_kp - ex_c, g.pcm[2][k + 1]);
+          h = madd(h, -c.y, psi_step(g.psi[PSI_HYZ], s, g.pbm[2][k + 1], ex_kp - ex_c, g.pcm[2][k + 1]));
         }
         if (sxc) {  // psi_Hyx (:1111-1121)
           const long long s = ((long long)k * (ny + 1) + j) * (g.L[0][0] + g.L[0][1]) + qxc;
-          h = h + c.y * psi_step(g.psi[PSI_HYX], s, g.pbm[0][i + 1], ez_ip - ez_c, g.pcm[0][i + 1]);
+          h = madd(h, c.y, psi_step(g.psi[PSI_HYX], s, g.pbm[0][i + 1], ez_ip - ez_c, g.pcm[0][i + 1]));
         }
       }
       Hy[p] = h;
@@ -435,15 +473,15 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
     if (i < nx && j < ny) {  // Hz sweep: x[0,nx-1] y[0,ny-1] z[0,nz]
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[5][p]);
       float h = Hz[p];
-      if (YEE) h = c.x * h + c.y * ((ex_jp - ex_c) * idye - (ey_ip - ey_c) * idxe);
+      if (YEE) h = yee_upd(c.x, h, c.y, curl2(ex_jp - ex_c, idye, ey_ip - ey_c, idxe));
       if (CPML) {
         if (syc) {  // psi_Hzy (:931-941)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyc) * nx + i;
-          h = h + c.y * psi_step(g.psi[PSI_HZY], s, g.pbm[1][j + 1], ex_jp - ex_c, g.pcm[1][j + 1]);
+          h = madd(h, c.y, psi_step(g.psi[PSI_HZY], s, g.pbm[1][j + 1], ex_jp - ex_c, g.pcm[1][j + 1]));
         }
         if (sxc) {  // psi_Hzx (:1091-1101)
           const long long s = ((long long)k * ny + j) * (g.L[0][0] + g.L[0][1]) + qxc;
-          h = h - c.y * psi_step(g.psi[PSI_HZX], s, g.pbm[0][i + 1], ey_ip - ey_c, g.pcm[0][i + 1]);
+          h = madd(h, -c.y, psi_step(g.psi[PSI_HZX], s, g.pbm[0][i + 1], ey_ip - ey_c, g.pcm[0][i + 1]));
         }
       }
       Hz[p] = h;
@@ -451,6 +489,320 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
     ex_c = ex_kp;
     ey_c = ey_kp;
   }
+}
+
+// ---- four consecutive x cells per lane ------------------------------------
+// The fused passes move each field value with 16-byte (fp32) accesses: lane l
+// of a wave owns cells 4l..4l+3 of a 256-cell x segment.  The x neighbour one
+// cell down (E pass) or up (H pass) comes from the next lane by a cross-lane
+// shuffle, and from one extra load at the wave's edge.  The row pitch leaves
+// room for the last vector of a row, whose cells beyond the sweep are written
+// back unchanged.
+struct F4 {
+  float v[4];
+};
+
+template <class T> struct Store;
+// fp32 fields: plain 16-byte vectors (the scale arguments are unused)
+template <> struct Store<float> {
+  static __device__ __forceinline__ F4 ld4(const float *b, long long p, float) {
+    const float4 t = *reinterpret_cast<const float4 *>(b + p);
+    return {{t.x, t.y, t.z, t.w}};
+  }
+  static __device__ __forceinline__ float ld1(const float *b, long long p, float) { return b[p]; }
+  static __device__ __forceinline__ void st4(float *b, long long p, float, const F4 &v) {
+    *reinterpret_cast<float4 *>(b + p) = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+  }
+};
+// binary16 fields with a per-array scale: value = half * s, stored as
+// round(value / s); 8-byte vectors, fp32 arithmetic
+template <> struct Store<__half> {
+  static __device__ __forceinline__ F4 ld4(const float *b, long long p, float s) {
+    const uint2 t = *reinterpret_cast<const uint2 *>(reinterpret_cast<const __half *>(b) + p);
+    const float2 lo = __half22float2(*reinterpret_cast<const __half2 *>(&t.x));
+    const float2 hi = __half22float2(*reinterpret_cast<const __half2 *>(&t.y));
+    return {{lo.x * s, lo.y * s, hi.x * s, hi.y * s}};
+  }
+  static __device__ __forceinline__ float ld1(const float *b, long long p, float s) {
+    return __half2float(reinterpret_cast<const __half *>(b)[p]) * s;
+  }
+  static __device__ __forceinline__ void st4(float *b, long long p, float is, const F4 &v) {
+    const __half2 lo = __floats2half2_rn(v.v[0] * is, v.v[1] * is);
+    const __half2 hi = __floats2half2_rn(v.v[2] * is, v.v[3] * is);
+    uint2 t;
+    t.x = *reinterpret_cast<const unsigned *>(&lo);
+    t.y = *reinterpret_cast<const unsigned *>(&hi);
+    *reinterpret_cast<uint2 *>(reinterpret_cast<__half *>(b) + p) = t;
+  }
+};
+
+__device__ __forceinline__ void ld_mi4(const int16_t *m, long long p, int (&o)[4]) {
+  const short4 t = *reinterpret_cast<const short4 *>(m + p);
+  o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+
+// E face pass (Yee E + CPML E), four cells per lane
+template <bool TBL, class T>
+__device__ __forceinline__ void epass4_tile(const DevGrid &g, const TileMap &tm,
+                                            const MedTab &tab, int blk) {
+  int bx, by, bz;
+  decode_tile(tm, blk, bx, by, bz);
+  const int lane = threadIdx.x;
+  const int i0 = bx * (kWave * 4) + 4 * lane;
+  const int j = by * blockDim.y + threadIdx.y;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (j > ny) return;                       // whole wave: one row per wave
+  const bool live = i0 <= nx;               // lanes past the sweep only shuffle
+  const int k0 = bz * tm.kc;
+  const int k1 = min(k0 + tm.kc, nz + 1);
+  const long long px = g.px, pxy = g.pxy;
+  long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + kXOff + i0;
+  float *__restrict__ Ex = g.f[0];
+  float *__restrict__ Ey = g.f[1];
+  float *__restrict__ Ez = g.f[2];
+  const float *__restrict__ Hx = g.f[3];
+  const float *__restrict__ Hy = g.f[4];
+  const float *__restrict__ Hz = g.f[5];
+  F4 hx_km = {}, hy_km = {};
+  if (live) {
+    hx_km = Store<T>::ld4(Hx, p - pxy, g.fs[3]);
+    hy_km = Store<T>::ld4(Hy, p - pxy, g.fs[4]);
+  }
+  const float idyh = g.id[4][j + 1];
+  float idxh[4];
+  int qxn[4];
+  bool sxn[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int i = min(i0 + c, nx + 1);
+    idxh[c] = g.id[3][i + 1];
+    sxn[c] = slab_node(i0 + c, nx, g.L[0][0], g.L[0][1], qxn[c]);
+  }
+  int qyn = 0;
+  const bool syn = slab_node(j, ny, g.L[1][0], g.L[1][1], qyn);
+  const bool cy = j < ny;
+  const int Lx = g.L[0][0] + g.L[0][1], Ly = g.L[1][0] + g.L[1][1];
+  for (int k = k0; k < k1; ++k, p += pxy) {
+    const bool cz = k < nz;
+    const float idzh = g.id[5][k + 1];
+    F4 hx0 = {}, hy0 = {}, hz0 = {}, hz_jm = {}, hx_jm = {}, ex = {}, ey = {}, ez = {};
+    int mx[4] = {}, my[4] = {}, mz[4] = {};
+    float hz_edge = 0.0f, hy_edge = 0.0f;
+    if (live) {
+      hx0 = Store<T>::ld4(Hx, p, g.fs[3]); hy0 = Store<T>::ld4(Hy, p, g.fs[4]); hz0 = Store<T>::ld4(Hz, p, g.fs[5]);
+      hz_jm = Store<T>::ld4(Hz, p - px, g.fs[5]); hx_jm = Store<T>::ld4(Hx, p - px, g.fs[3]);
+      ex = Store<T>::ld4(Ex, p, g.fs[0]); ld_mi4(g.mi[0], p, mx);
+      if (cy) { ey = Store<T>::ld4(Ey, p, g.fs[1]); ld_mi4(g.mi[1], p, my); }
+      if (cz) { ez = Store<T>::ld4(Ez, p, g.fs[2]); ld_mi4(g.mi[2], p, mz); }
+      if (lane == 0) { hz_edge = Store<T>::ld1(Hz, p - 1, g.fs[5]); hy_edge = Store<T>::ld1(Hy, p - 1, g.fs[4]); }
+    }
+    // x - 1 neighbours: the previous lane's last cell
+    float hz_im[4], hy_im[4];
+    const float hz_up = __shfl_up(hz0.v[3], 1), hy_up = __shfl_up(hy0.v[3], 1);
+    hz_im[0] = lane == 0 ? hz_edge : hz_up;
+    hy_im[0] = lane == 0 ? hy_edge : hy_up;
+#pragma unroll
+    for (int c = 1; c < 4; ++c) { hz_im[c] = hz0.v[c - 1]; hy_im[c] = hy0.v[c - 1]; }
+    int qzn = 0;
+    const bool szn = slab_node(k, nz, g.L[2][0], g.L[2][1], qzn);
+    if (live) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = i0 + c;
+        if (i < nx) {  // Ex sweep x[0,nx-1]
+          float e = ex.v[c];
+          const float2 cf = coef<TBL>(tab, g.g1, g.g2, mx[c]);
+          e = yee_upd(cf.x, e, cf.y, curl2(hz0.v[c] - hz_jm.v[c], idyh, hy0.v[c] - hy_km.v[c], idzh));
+          if (syn) {  // psi_Exy, left/right (borderscpml.F90:609-623)
+            const long long s = ((long long)k * Ly + qyn) * nx + i;
+            e = madd(e, cf.y, psi_step(g.psi[PSI_EXY], s, g.pbe[1][j + 1], hz0.v[c] - hz_jm.v[c],
+                                    g.pce[1][j + 1]));
+          }
+          if (szn) {  // psi_Exz, down/up (:699-712)
+            const long long s = ((long long)qzn * (ny + 1) + j) * nx + i;
+            e = madd(e, -cf.y, psi_step(g.psi[PSI_EXZ], s, g.pbe[2][k + 1], hy0.v[c] - hy_km.v[c],
+                                    g.pce[2][k + 1]));
+          }
+          ex.v[c] = e;
+        }
+        if (cy && i <= nx) {  // Ey sweep x[0,nx] y[0,ny-1]
+          float e = ey.v[c];
+          const float2 cf = coef<TBL>(tab, g.g1, g.g2, my[c]);
+          e = yee_upd(cf.x, e, cf.y, curl2(hx0.v[c] - hx_km.v[c], idzh, hz0.v[c] - hz_im[c], idxh[c]));
+          if (szn) {  // psi_Eyz, down/up (:680-693)
+            const long long s = ((long long)qzn * ny + j) * (nx + 1) + i;
+            e = madd(e, cf.y, psi_step(g.psi[PSI_EYZ], s, g.pbe[2][k + 1], hx0.v[c] - hx_km.v[c],
+                                    g.pce[2][k + 1]));
+          }
+          if (sxn[c]) {  // psi_Eyx, back/front (:790-802)
+            const long long s = ((long long)k * ny + j) * Lx + qxn[c];
+            e = madd(e, -cf.y, psi_step(g.psi[PSI_EYX], s, g.pbe[0][i + 1], hz0.v[c] - hz_im[c],
+                                    g.pce[0][i + 1]));
+          }
+          ey.v[c] = e;
+        }
+        if (cz && i <= nx) {  // Ez sweep x[0,nx] z[0,nz-1]
+          float e = ez.v[c];
+          const float2 cf = coef<TBL>(tab, g.g1, g.g2, mz[c]);
+          e = yee_upd(cf.x, e, cf.y, curl2(hy0.v[c] - hy_im[c], idxh[c], hx0.v[c] - hx_jm.v[c], idyh));
+          if (syn) {  // psi_Ezy, left/right (:630-642)
+            const long long s = ((long long)k * Ly + qyn) * (nx + 1) + i;
+            e = madd(e, -cf.y, psi_step(g.psi[PSI_EZY], s, g.pbe[1][j + 1], hx0.v[c] - hx_jm.v[c],
+                                    g.pce[1][j + 1]));
+          }
+          if (sxn[c]) {  // psi_Ezx, back/front (:771-783)
+            const long long s = ((long long)k * (ny + 1) + j) * Lx + qxn[c];
+            e = madd(e, cf.y, psi_step(g.psi[PSI_EZX], s, g.pbe[0][i + 1], hy0.v[c] - hy_im[c],
+                                    g.pce[0][i + 1]));
+          }
+          ez.v[c] = e;
+        }
+      }
+      Store<T>::st4(Ex, p, g.fi[0], ex);
+      if (cy) Store<T>::st4(Ey, p, g.fi[1], ey);
+      if (cz) Store<T>::st4(Ez, p, g.fi[2], ez);
+    }
+    hx_km = hx0;
+    hy_km = hy0;
+  }
+}
+
+// H face pass (Yee H + CPML H), four cells per lane
+template <bool TBL, class T>
+__device__ __forceinline__ void hpass4_tile(const DevGrid &g, const TileMap &tm,
+                                            const MedTab &tab, int blk) {
+  int bx, by, bz;
+  decode_tile(tm, blk, bx, by, bz);
+  const int lane = threadIdx.x;
+  const int i0 = bx * (kWave * 4) + 4 * lane;
+  const int j = by * blockDim.y + threadIdx.y;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (j > ny) return;
+  const bool live = i0 <= nx;
+  const int k0 = bz * tm.kc;
+  const int k1 = min(k0 + tm.kc, nz + 1);
+  const long long px = g.px, pxy = g.pxy;
+  long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + kXOff + i0;
+  const float *__restrict__ Ex = g.f[0];
+  const float *__restrict__ Ey = g.f[1];
+  const float *__restrict__ Ez = g.f[2];
+  float *__restrict__ Hx = g.f[3];
+  float *__restrict__ Hy = g.f[4];
+  float *__restrict__ Hz = g.f[5];
+  const float idye = g.id[1][j + 1];
+  float idxe[4];
+  int qxc[4];
+  bool sxc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int i = min(i0 + c, nx + 1);
+    idxe[c] = g.id[0][i + 1];
+    sxc[c] = slab_cell(i0 + c, nx, g.L[0][0], g.L[0][1], qxc[c]);
+  }
+  int qyc = 0;
+  const bool syc = slab_cell(j, ny, g.L[1][0], g.L[1][1], qyc);
+  const bool cy = j < ny;
+  const int Lx = g.L[0][0] + g.L[0][1], Ly = g.L[1][0] + g.L[1][1];
+  F4 ex_c = {}, ey_c = {};
+  if (live) { ex_c = Store<T>::ld4(Ex, p, g.fs[0]); ey_c = Store<T>::ld4(Ey, p, g.fs[1]); }
+  for (int k = k0; k < k1; ++k, p += pxy) {
+    const float idze = g.id[2][k + 1];
+    const bool cz = k < nz;
+    F4 ex_kp = {}, ey_kp = {}, ez_c = {}, ez_jp = {}, ex_jp = {}, hx = {}, hy = {}, hz = {};
+    int mx[4] = {}, my[4] = {}, mz[4] = {};
+    float ez_edge = 0.0f, ey_edge = 0.0f;
+    if (live) {
+      ex_kp = Store<T>::ld4(Ex, p + pxy, g.fs[0]); ey_kp = Store<T>::ld4(Ey, p + pxy, g.fs[1]);
+      ez_c = Store<T>::ld4(Ez, p, g.fs[2]);
+      ez_jp = Store<T>::ld4(Ez, p + px, g.fs[2]); ex_jp = Store<T>::ld4(Ex, p + px, g.fs[0]);
+      if (cy && cz) { hx = Store<T>::ld4(Hx, p, g.fs[3]); ld_mi4(g.mi[3], p, mx); }
+      if (cz) { hy = Store<T>::ld4(Hy, p, g.fs[4]); ld_mi4(g.mi[4], p, my); }
+      if (cy) { hz = Store<T>::ld4(Hz, p, g.fs[5]); ld_mi4(g.mi[5], p, mz); }
+      if (lane == kWave - 1) { ez_edge = Store<T>::ld1(Ez, p + 4, g.fs[2]); ey_edge = Store<T>::ld1(Ey, p + 4, g.fs[1]); }
+    }
+    // x + 1 neighbours: the next lane's first cell
+    float ez_ip[4], ey_ip[4];
+    const float ez_dn = __shfl_down(ez_c.v[0], 1), ey_dn = __shfl_down(ey_c.v[0], 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { ez_ip[c] = ez_c.v[c + 1]; ey_ip[c] = ey_c.v[c + 1]; }
+    ez_ip[3] = lane == kWave - 1 ? ez_edge : ez_dn;
+    ey_ip[3] = lane == kWave - 1 ? ey_edge : ey_dn;
+    int qzc = 0;
+    const bool szc = slab_cell(k, nz, g.L[2][0], g.L[2][1], qzc);
+    if (live) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = i0 + c;
+        if (cy && cz && i <= nx) {  // Hx sweep: x[0,nx] y[0,ny-1] z[0,nz-1]
+          const float2 cf = coef<TBL>(tab, g.gm1, g.gm2, mx[c]);
+          float h = hx.v[c];
+          h = yee_upd(cf.x, h, cf.y, curl2(ey_kp.v[c] - ey_c.v[c], idze, ez_jp.v[c] - ez_c.v[c], idye));
+          if (syc) {  // psi_Hxy (borderscpml.F90:911-921)
+            const long long s = ((long long)k * Ly + qyc) * (nx + 1) + i;
+            h = madd(h, -cf.y, psi_step(g.psi[PSI_HXY], s, g.pbm[1][j + 1], ez_jp.v[c] - ez_c.v[c],
+                                    g.pcm[1][j + 1]));
+          }
+          if (szc) {  // psi_Hxz (:1021-1031)
+            const long long s = ((long long)qzc * ny + j) * (nx + 1) + i;
+            h = madd(h, cf.y, psi_step(g.psi[PSI_HXZ], s, g.pbm[2][k + 1], ey_kp.v[c] - ey_c.v[c],
+                                    g.pcm[2][k + 1]));
+          }
+          hx.v[c] = h;
+        }
+        if (cz && i < nx) {  // Hy sweep: x[0,nx-1] y[0,ny] z[0,nz-1]
+          const float2 cf = coef<TBL>(tab, g.gm1, g.gm2, my[c]);
+          float h = hy.v[c];
+          h = yee_upd(cf.x, h, cf.y, curl2(ez_ip[c] - ez_c.v[c], idxe[c], ex_kp.v[c] - ex_c.v[c], idze));
+          if (szc) {  // psi_Hyz (:1001-1011)
+            const long long s = ((long long)qzc * (ny + 1) + j) * nx + i;
+            h = madd(h, -cf.y, psi_step(g.psi[PSI_HYZ], s, g.pbm[2][k + 1], ex_kp.v[c] - ex_c.v[c],
+                                    g.pcm[2][k + 1]));
+          }
+          if (sxc[c]) {  // psi_Hyx (:1111-1121)
+            const long long s = ((long long)k * (ny + 1) + j) * Lx + qxc[c];
+            h = madd(h, cf.y, psi_step(g.psi[PSI_HYX], s, g.pbm[0][i + 1], ez_ip[c] - ez_c.v[c],
+                                    g.pcm[0][i + 1]));
+          }
+          hy.v[c] = h;
+        }
+        if (cy && i < nx) {  // Hz sweep: x[0,nx-1] y[0,ny-1] z[0,nz]
+          const float2 cf = coef<TBL>(tab, g.gm1, g.gm2, mz[c]);
+          float h = hz.v[c];
+          h = yee_upd(cf.x, h, cf.y, curl2(ex_jp.v[c] - ex_c.v[c], idye, ey_ip[c] - ey_c.v[c], idxe[c]));
+          if (syc) {  // psi_Hzy (:931-941)
+            const long long s = ((long long)k * Ly + qyc) * nx + i;
+            h = madd(h, cf.y, psi_step(g.psi[PSI_HZY], s, g.pbm[1][j + 1], ex_jp.v[c] - ex_c.v[c],
+                                    g.pcm[1][j + 1]));
+          }
+          if (sxc[c]) {  // psi_Hzx (:1091-1101)
+            const long long s = ((long long)k * ny + j) * Lx + qxc[c];
+            h = madd(h, -cf.y, psi_step(g.psi[PSI_HZX], s, g.pbm[0][i + 1], ey_ip[c] - ey_c.v[c],
+                                    g.pcm[0][i + 1]));
+          }
+          hz.v[c] = h;
+        }
+      }
+      if (cy && cz) Store<T>::st4(Hx, p, g.fi[3], hx);
+      if (cz) Store<T>::st4(Hy, p, g.fi[4], hy);
+      if (cy) Store<T>::st4(Hz, p, g.fi[5], hz);
+    }
+    ex_c = ex_kp;
+    ey_c = ey_kp;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_epass4(DevGrid g, TileMap tm) {
+  extern __shared__ float2 lds_tab[];
+  const MedTab tab = stage_tables<true, false>(g, g.g1, g.g2, lds_tab);
+  epass4_tile<true, T>(g, tm, tab, blockIdx.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_hpass4(DevGrid g, TileMap tm) {
+  extern __shared__ float2 lds_tab[];
+  const MedTab tab = stage_tables<true, false>(g, g.gm1, g.gm2, lds_tab);
+  hpass4_tile<true, T>(g, tm, tab, blockIdx.x);
 }
 
 template <bool YEE, bool CPML, bool TBL>
@@ -481,7 +833,7 @@ __device__ __forceinline__ int batch_grid(const int *start, int ngrid, int t) {
   return lo;
 }
 
-template <bool H>
+template <bool H, class T, int VEC>
 __global__ void __launch_bounds__(256) k_pass_batch(BatchDev bd) {
   extern __shared__ float2 lds_tab[];
   for (int t = blockIdx.x; t < bd.total; t += gridDim.x) {
@@ -494,36 +846,72 @@ __global__ void __launch_bounds__(256) k_pass_batch(BatchDev bd) {
     // the CPML variant serves grids without CPML too: with no slab layers
     // no cell of the sweep falls in a slab
     const int blk = t - bd.start[b];
-    if (H) hpass_tile<true, true, true>(g, tm, tab, blk);
-    else epass_tile<true, true, true>(g, tm, tab, blk);
+    if (VEC == 4) {
+      if (H) hpass4_tile<true, T>(g, tm, tab, blk);
+      else epass4_tile<true, T>(g, tm, tab, blk);
+    } else {
+      if (H) hpass_tile<true, true, true>(g, tm, tab, blk);
+      else epass_tile<true, true, true>(g, tm, tab, blk);
+    }
   }
 }
 
-int launch_pass_batch(const BatchDev &bd, int half, int blocks, int rows, size_t lds,
-                      void *stream) {
+int launch_pass_batch(const BatchDev &bd, int half, int blocks, int rows, int vec, size_t lds,
+                      int fp16, void *stream) {
   if (bd.total <= 0) return 0;
   const dim3 blk(kWave, (unsigned)rows);
   hipStream_t s = (hipStream_t)stream;
-  if (half) hipLaunchKernelGGL(k_pass_batch<true>, dim3(blocks), blk, lds, s, bd);
-  else hipLaunchKernelGGL(k_pass_batch<false>, dim3(blocks), blk, lds, s, bd);
+  if (fp16) {
+    if (half) hipLaunchKernelGGL((k_pass_batch<true, __half, 4>), dim3(blocks), blk, lds, s, bd);
+    else hipLaunchKernelGGL((k_pass_batch<false, __half, 4>), dim3(blocks), blk, lds, s, bd);
+  } else if (vec == 4) {
+    if (half) hipLaunchKernelGGL((k_pass_batch<true, float, 4>), dim3(blocks), blk, lds, s, bd);
+    else hipLaunchKernelGGL((k_pass_batch<false, float, 4>), dim3(blocks), blk, lds, s, bd);
+  } else {
+    if (half) hipLaunchKernelGGL((k_pass_batch<true, float, 1>), dim3(blocks), blk, lds, s, bd);
+    else hipLaunchKernelGGL((k_pass_batch<false, float, 1>), dim3(blocks), blk, lds, s, bd);
+  }
   return (int)hipGetLastError();
 }
 
 // media tables fit in LDS up to this many entries
 constexpr int kLdsMedia = 4096;
 
-static size_t e_lds_bytes(const DevGrid &g, int ade) {
-  size_t b = (size_t)(g.nmedia + 1) * 12;
-  if (ade) b = ((b + (size_t)g.ndisp * 4 + 7) & ~(size_t)7) + (size_t)g.ndisp * kPMax * 6 * sizeof(coef_t);
-  return b;
+static size_t e_lds_bytes(const DevGrid &g) { return (size_t)(g.nmedia + 1) * 12; }
+
+static size_t ade_lds_bytes(const DevGrid &g, int side) {
+  const size_t nd = (size_t)g.ade[side].ndisp;
+  return ((e_lds_bytes(g) + nd * 4 + 7) & ~(size_t)7) + nd * kPMax * 6 * sizeof(coef_t);
 }
 
-int launch_e(const DevGrid &g, const TileMap &tm, int yee, int cpml, int ade,
-             void *stream) {
+// the pole stage of one field, after its Yee+CPML pass (AdvanceEDispersiveE
+// follows AdvanceelectricCPML, timestepping.F90:1380-1417; AdvanceMdispersiveH
+// follows the magnetic CPML and PMC clone, :1533-1587)
+static void launch_ade(const DevGrid &g, int side, hipStream_t s) {
+  const AdeSet &ad = g.ade[side];
+  if (!ad.has) return;
+  long long nmax = 0;
+  for (int c = 0; c < 3; ++c) nmax = nmax > ad.nstate[c] ? nmax : ad.nstate[c];
+  if (nmax <= 0) return;
+  const size_t la = ade_lds_bytes(g, side);
+  const bool ta = g.nmedia + 1 <= kLdsMedia && la <= 48 * 1024;
+  const long long want = (nmax + 255) / 256;
+  const dim3 grid((unsigned)(want < 8192 ? want : 8192), 3);
+  if (side == 0) {
+    if (ta) hipLaunchKernelGGL((k_ade_box<true, 0>), grid, dim3(256), la, s, g);
+    else hipLaunchKernelGGL((k_ade_box<false, 0>), grid, dim3(256), 0, s, g);
+  } else {
+    if (ta) hipLaunchKernelGGL((k_ade_box<true, 1>), grid, dim3(256), la, s, g);
+    else hipLaunchKernelGGL((k_ade_box<false, 1>), grid, dim3(256), 0, s, g);
+  }
+}
+
+int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, int cpml,
+             int ade, void *stream) {
   const dim3 blk(kWave, (unsigned)tm.bdy);
   hipStream_t s = (hipStream_t)stream;
   const int nb = tm.nblk;
-  const size_t lds = e_lds_bytes(g, 0);
+  const size_t lds = e_lds_bytes(g);
   const bool tbl = g.nmedia + 1 <= kLdsMedia;
 #define SFD_E(Y, C)                                                            \
   do {                                                                         \
@@ -532,32 +920,21 @@ int launch_e(const DevGrid &g, const TileMap &tm, int yee, int cpml, int ade,
     else                                                                       \
       hipLaunchKernelGGL((k_epass<Y, C, false>), dim3(nb), blk, 0, s, g, tm);   \
   } while (0)
-  if (yee && cpml) SFD_E(true, true);
+  if (yee && cpml && tbl && (tm4.vec == 4 || g.fp16)) {
+    // fused pass, four cells per lane (the CPML variant also serves grids
+    // without CPML)
+    if (g.fp16) hipLaunchKernelGGL(k_epass4<__half>, dim3(tm4.nblk), blk, lds, s, g, tm4);
+    else hipLaunchKernelGGL(k_epass4<float>, dim3(tm4.nblk), blk, lds, s, g, tm4);
+  } else if (yee && cpml) SFD_E(true, true);
   else if (yee) SFD_E(true, false);
   else if (cpml) SFD_E(false, true);
 #undef SFD_E
-  if (ade) {
-    // the pole stage follows the Yee+CPML E pass (electricdispersive.F90 runs
-    // after AdvanceelectricCPML, timestepping.F90:1380-1417)
-    long long nmax = 0;
-    for (int c = 0; c < 3; ++c)
-      nmax = nmax > g.nstate[c] ? nmax : g.nstate[c];
-    if (nmax > 0) {
-      const size_t la = e_lds_bytes(g, 1);
-      const bool ta = g.nmedia + 1 <= kLdsMedia && la <= 48 * 1024;
-      long long want = (nmax + 255) / 256;
-      const unsigned gx = (unsigned)(want < 8192 ? want : 8192);
-      if (ta)
-        hipLaunchKernelGGL(k_ade_box<true>, dim3(gx, 3), dim3(256), la, s, g);
-      else
-        hipLaunchKernelGGL(k_ade_box<false>, dim3(gx, 3), dim3(256), 0, s, g);
-    }
-  }
+  if (ade) launch_ade(g, 0, s);
   return (int)hipGetLastError();
 }
 
-int launch_h(const DevGrid &g, const TileMap &tm, int yee, int cpml,
-             void *stream) {
+int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, int cpml,
+             int ade, void *stream) {
   const dim3 blk(kWave, (unsigned)tm.bdy);
   hipStream_t s = (hipStream_t)stream;
   const int nb = tm.nblk;
@@ -570,11 +947,14 @@ int launch_h(const DevGrid &g, const TileMap &tm, int yee, int cpml,
     else                                                                       \
       hipLaunchKernelGGL((k_hpass<Y, C, false>), dim3(nb), blk, 0, s, g, tm);   \
   } while (0)
-  if (yee && cpml) SFD_H(true, true);
+  if (yee && cpml && tbl && (tm4.vec == 4 || g.fp16)) {
+    if (g.fp16) hipLaunchKernelGGL(k_hpass4<__half>, dim3(tm4.nblk), blk, lds, s, g, tm4);
+    else hipLaunchKernelGGL(k_hpass4<float>, dim3(tm4.nblk), blk, lds, s, g, tm4);
+  } else if (yee && cpml) SFD_H(true, true);
   else if (yee) SFD_H(true, false);
   else if (cpml) SFD_H(false, true);
-  else return 0;
 #undef SFD_H
+  if (ade) launch_ade(g, 1, s);
   return (int)hipGetLastError();
 }
 

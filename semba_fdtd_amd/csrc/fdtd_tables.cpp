@@ -89,12 +89,20 @@ int fdtd_build_g_tables(int32_t nmedia, const float *epr, const float *sigma,
       g1[r] = g2[r] = gm1[r] = gm2[r] = 0.0f;
       continue;
     }
-    lossy_pair(dt, sigmam[r], mu0 * mur[r], gm1[r], gm2[r]);
-    if (kind[r] & FDTD_KIND_EDISP) {  // placeholders, :109-118
+    // one dispersive kind alone: placeholders the dispersive init
+    // overwrites (:109-128); both kinds fall to the lossy branch (:129-140)
+    const bool ed = kind[r] & FDTD_KIND_EDISP, md = kind[r] & FDTD_KIND_MDISP;
+    if (ed && !md) {
       g1[r] = 0.0f;
       g2[r] = 0.0f;
     } else {
       lossy_pair(dt, sigma[r], eps0 * epr[r], g1[r], g2[r]);
+    }
+    if (md && !ed) {
+      gm1[r] = 0.0f;
+      gm2[r] = 0.0f;
+    } else {
+      lossy_pair(dt, sigmam[r], mu0 * mur[r], gm1[r], gm2[r]);
     }
   }
   return FDTD_OK;
@@ -202,6 +210,21 @@ int fdtd_build_ade(int32_t npoles, const double *c, const double *a, float eps11
     g3[2 * p] = half * (1.0 + kappa[2 * p]);
     g3[2 * p + 1] = half * kappa[2 * p + 1];
   }
+  return FDTD_OK;
+}
+
+int fdtd_build_mur(int32_t nmedia, const float *epr, const float *mur, double dt, float eps0,
+                   float mu0, const float idh_face[6], float *cab1) {
+  // calc_murconstants, bordersmur.F90:316-363: per face and medium
+  // cnum = (1/Idh at the face) / (dt * c0 / sqrt(epr*mur)), real(4) result
+  if (nmedia < 0 || !epr || !mur || !idh_face || !cab1) return FDTD_ERR_ARG;
+  const float c0 = 1.0f / std::sqrt(eps0 * mu0);
+  for (int f = 0; f < 6; ++f)
+    for (int32_t m = 0; m <= nmedia; ++m) {
+      const float v = std::sqrt(epr[m] * mur[m]);
+      const float cnum = (float)((double)(1.0f / idh_face[f]) / (dt * (double)c0 / (double)v));
+      cab1[(size_t)f * (nmedia + 1) + m] = (1.0f - cnum) / (1.0f + cnum);
+    }
   return FDTD_OK;
 }
 

@@ -17,7 +17,7 @@ module sembafdtd_gpu
    integer(c_int), parameter, public :: FDTD_EX = 0, FDTD_EY = 1, FDTD_EZ = 2, &
                                         FDTD_HX = 3, FDTD_HY = 4, FDTD_HZ = 5
    integer(c_int), parameter, public :: FDTD_KIND_PEC = 1, FDTD_KIND_PMC = 2, &
-                                        FDTD_KIND_EDISP = 4
+                                        FDTD_KIND_EDISP = 4, FDTD_KIND_MDISP = 8
 
    type, bind(C), public :: fdtd_grid_desc
       integer(c_int32_t) :: nx, ny, nz      ! swept cells per axis, CPML included
@@ -37,7 +37,13 @@ module sembafdtd_gpu
              fdtd_build_spacing, fdtd_build_cpml_axis, fdtd_build_ade, fdtd_evolucion, &
              fdtd_abi_version, fdtd_device_bytes, fdtd_dispersive_edges, fdtd_media_fill, &
              fdtd_media_pec_walls, fdtd_media_paint_sphere, fdtd_media_paint_box, &
-             fdtd_error_message
+             fdtd_set_mdispersive, fdtd_advance_h_dispersive, fdtd_set_mur, fdtd_build_mur, &
+             fdtd_advance_mur, fdtd_set_tiling, fdtd_get_tiling, fdtd_time_halves, &
+             fdtd_add_dft, fdtd_dft_size, fdtd_read_dft, fdtd_update_observation, &
+             fdtd_set_field_scale, fdtd_state_size, fdtd_save_state, fdtd_load_state, &
+             fdtd_batch_create, fdtd_batch_destroy, fdtd_batch_admit, fdtd_batch_retire, &
+             fdtd_batch_step_of, fdtd_batch_set_tiling, fdtd_batch_step, &
+             fdtd_batch_synchronize, fdtd_error_message
 
    interface
       integer(c_int) function fdtd_create(desc, h) bind(C, name="fdtd_create")
@@ -317,6 +323,149 @@ module sembafdtd_gpu
          integer(c_int16_t), value :: medium
          integer(c_int32_t), value :: nprio
          integer(c_int32_t), intent(in) :: prio(*)
+      end function
+      ! ---- magnetic pole-residue media (magneticdispersive.F90:77-362)
+      integer(c_int) function fdtd_set_mdispersive(h, ndisp, medium, npoles, kappa, beta, gm3) &
+            bind(C, name="fdtd_set_mdispersive")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: ndisp
+         integer(c_int32_t), intent(in) :: medium(*), npoles(*)
+         real(c_double), intent(in) :: kappa(*), beta(*), gm3(*)
+      end function
+      integer(c_int) function fdtd_advance_h_dispersive(h) bind(C, name="fdtd_advance_h_dispersive")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function
+      ! ---- first-order Mur faces (bordersmur.F90:79-363, 502-1600)
+      integer(c_int) function fdtd_set_mur(h, faces, cab1) bind(C, name="fdtd_set_mur")
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(in) :: faces(6)
+         real(c_float), intent(in) :: cab1(*)
+      end function
+      integer(c_int) function fdtd_build_mur(nmedia, epr, mur, dt, eps0, mu0, idh_face, cab1) &
+            bind(C, name="fdtd_build_mur")
+         import :: c_int, c_int32_t, c_float, c_double
+         integer(c_int32_t), value :: nmedia
+         real(c_float), intent(in) :: epr(*), mur(*)
+         real(c_double), value :: dt
+         real(c_float), value :: eps0, mu0
+         real(c_float), intent(in) :: idh_face(6)
+         real(c_float), intent(out) :: cab1(*)
+      end function
+      integer(c_int) function fdtd_advance_mur(h) bind(C, name="fdtd_advance_mur")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function
+      ! ---- work decomposition of the face passes and its timing
+      integer(c_int) function fdtd_set_tiling(h, rows, zchunk, xcells) bind(C, name="fdtd_set_tiling")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: rows, zchunk, xcells
+      end function
+      integer(c_int) function fdtd_get_tiling(h, rows, zchunk, xcells) bind(C, name="fdtd_get_tiling")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(out) :: rows, zchunk, xcells
+      end function
+      integer(c_int) function fdtd_time_halves(h, reps, ms) bind(C, name="fdtd_time_halves")
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: reps
+         real(c_float), intent(out) :: ms(3)
+      end function
+      ! ---- frequency-domain observation (observation.F90:3527-3600)
+      integer(c_int) function fdtd_add_dft(h, kind, box, stride, nfreq, freq, id) &
+            bind(C, name="fdtd_add_dft")
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: kind, nfreq
+         integer(c_int32_t), intent(in) :: box(6), stride(3)
+         real(c_float), intent(in) :: freq(*)
+         integer(c_int32_t), intent(out) :: id
+      end function
+      integer(c_int64_t) function fdtd_dft_size(h, id) bind(C, name="fdtd_dft_size")
+         import :: c_int64_t, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: id
+      end function
+      integer(c_int) function fdtd_read_dft(h, id, out, capacity) bind(C, name="fdtd_read_dft")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: id
+         real(c_double), intent(out) :: out(*)
+         integer(c_int64_t), value :: capacity
+      end function
+      integer(c_int) function fdtd_update_observation(h, n) bind(C, name="fdtd_update_observation")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: n
+      end function
+      ! ---- binary16 storage scale, checkpoint / resume (resuming.F90:394-496)
+      integer(c_int) function fdtd_set_field_scale(h, comp, scale) bind(C, name="fdtd_set_field_scale")
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: comp
+         real(c_float), value :: scale
+      end function
+      integer(c_int64_t) function fdtd_state_size(h) bind(C, name="fdtd_state_size")
+         import :: c_int64_t, c_ptr
+         type(c_ptr), value :: h
+      end function
+      integer(c_int) function fdtd_save_state(h, n, buf, size) bind(C, name="fdtd_save_state")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: h, buf
+         integer(c_int32_t), value :: n
+         integer(c_int64_t), value :: size
+      end function
+      integer(c_int) function fdtd_load_state(h, buf, size, n) bind(C, name="fdtd_load_state")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value :: h, buf
+         integer(c_int64_t), value :: size
+         integer(c_int32_t), intent(out) :: n
+      end function
+      ! ---- batches of independent grids
+      integer(c_int) function fdtd_batch_create(device, stream, b) bind(C, name="fdtd_batch_create")
+         import :: c_int, c_ptr, c_int32_t
+         integer(c_int32_t), value :: device
+         type(c_ptr), value :: stream
+         type(c_ptr), intent(out) :: b
+      end function
+      integer(c_int) function fdtd_batch_destroy(b) bind(C, name="fdtd_batch_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: b
+      end function
+      integer(c_int) function fdtd_batch_admit(b, h, n0, slot) bind(C, name="fdtd_batch_admit")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: b, h
+         integer(c_int32_t), value :: n0
+         integer(c_int32_t), intent(out) :: slot
+      end function
+      integer(c_int) function fdtd_batch_retire(b, slot) bind(C, name="fdtd_batch_retire")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: b
+         integer(c_int32_t), value :: slot
+      end function
+      integer(c_int64_t) function fdtd_batch_step_of(b, slot) bind(C, name="fdtd_batch_step_of")
+         import :: c_int64_t, c_ptr, c_int32_t
+         type(c_ptr), value :: b
+         integer(c_int32_t), value :: slot
+      end function
+      integer(c_int) function fdtd_batch_set_tiling(b, rows, zchunk, xcells, blocks) &
+            bind(C, name="fdtd_batch_set_tiling")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: b
+         integer(c_int32_t), value :: rows, zchunk, xcells, blocks
+      end function
+      integer(c_int) function fdtd_batch_step(b, nsteps) bind(C, name="fdtd_batch_step")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: b
+         integer(c_int32_t), value :: nsteps
+      end function
+      integer(c_int) function fdtd_batch_synchronize(b) bind(C, name="fdtd_batch_synchronize")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: b
       end function
    end interface
 

@@ -9,7 +9,8 @@ as ``array[k+1, j+1, i+1]``.  The raw cell sizes ``D[a]`` cover [-2, n+1]
 
 Media follow the reference conventions: index 0 is PEC
 (preprocess_geom.F90:470-476), index 1 the background, larger indices are
-bodies.  ``kind`` bits: 1 PEC, 2 PMC, 4 electric pole-residue dispersive.
+bodies.  ``kind`` bits: 1 PEC, 2 PMC, 4 electric pole-residue dispersive,
+8 magnetic pole-residue dispersive.
 """
 from __future__ import annotations
 
@@ -22,7 +23,7 @@ EPS0 = float(np.float32(8.854187817620389850536563031710750260608370166599449808
 MU0 = float(np.float32(1.2566370614359172953850573533118011536788677597500423283899778369231265625144835994512139301368468271e-6))
 PMAX = 8
 PW_MODES = 4
-KIND_PEC, KIND_PMC, KIND_EDISP = 1, 2, 4
+KIND_PEC, KIND_PMC, KIND_EDISP, KIND_MDISP = 1, 2, 4, 8
 EX, EY, EZ, HX, HY, HZ = range(6)
 
 
@@ -107,6 +108,14 @@ class Problem:
     pole_a: np.ndarray = None           # (nmedia+1, PMAX, 2) poles a_p
     eps11: np.ndarray = None            # absolute permittivity of dispersive media
     sig11: np.ndarray = None
+    # magnetic pole-residue media (kind bit 8): poles, absolute mu, sigma_m
+    mnpoles: np.ndarray = None
+    mpole_c: np.ndarray = None
+    mpole_a: np.ndarray = None
+    mu11: np.ndarray = None
+    sigm11: np.ndarray = None
+    # first-order Mur faces x-, x+, y-, y+, z-, z+ (bordersmur.F90)
+    mur_faces: tuple = (0, 0, 0, 0, 0, 0)
     cpml_layers: list = field(default_factory=lambda: [[0, 0], [0, 0], [0, 0]])
     cpml_order: list = field(default_factory=lambda: [[2.0, 2.0]] * 3)
     cpml_refl: list = field(default_factory=lambda: [[1e-3, 1e-3]] * 3)
@@ -146,6 +155,19 @@ class Problem:
             self.sig11 = np.zeros(nm + 1, np.float32)
         self.npoles = np.ascontiguousarray(self.npoles, np.int32)
         self.eps11, self.sig11 = f32(self.eps11), f32(self.sig11)
+        if self.mnpoles is None:
+            self.mnpoles = np.zeros(nm + 1, np.int32)
+        if self.mpole_c is None:
+            self.mpole_c = np.zeros((nm + 1, PMAX, 2))
+        if self.mpole_a is None:
+            self.mpole_a = np.zeros((nm + 1, PMAX, 2))
+        if self.mu11 is None:
+            self.mu11 = np.zeros(nm + 1, np.float32)
+        if self.sigm11 is None:
+            self.sigm11 = np.zeros(nm + 1, np.float32)
+        self.mnpoles = np.ascontiguousarray(self.mnpoles, np.int32)
+        self.mu11, self.sigm11 = f32(self.mu11), f32(self.sigm11)
+        self.mur_faces = tuple(int(bool(v)) for v in self.mur_faces)
         self.D = [f32(d) for d in self.D]
         if self.mi is None:
             assert self.scene is not None and self.prio is not None
@@ -427,11 +449,13 @@ def scatterer_problem(n=(24, 22, 20), L=((4, 5), (3, 4), (5, 3)), seed: int = 7,
                       nsteps: int = 300, dx: float = 0.01, graded_z: float = 1.0,
                       order: float = 3.0, refl: float = 1e-6, disp: bool = True,
                       soft: bool = True, alphamax: float = 0.0, kappamax: float = 1.0,
-                      nprobes: int = 12) -> Problem:
+                      nprobes: int = 12, mdisp: bool = False, mur=None) -> Problem:
     """Small open problem exercising every piece of the path: CPML of uneven
     thickness on all faces, optional geometric z grading, a lossy dielectric
     sphere, a 2-pole Lorentz blob, a soft Ex and a hard Hy source and probes
-    on all six components."""
+    on all six components.  ``mdisp`` adds a magnetic 2-pole Lorentz block
+    and makes the Lorentz blob magnetic-dispersive too (one real pole);
+    ``mur`` = 6 flags (x-, x+, y-, y+, z-, z+) of first-order Mur faces."""
     rng = np.random.default_rng(seed)
     nx, ny, nz = n
     mi = new_media_index(nx, ny, nz)
@@ -444,8 +468,8 @@ def scatterer_problem(n=(24, 22, 20), L=((4, 5), (3, 4), (5, 3)), seed: int = 7,
     rad = 0.3 * min(hi - lo for lo, hi in inner)
     paint_voxels(mi, sphere_cells(n, ctr, rad), 2, prio)
     npoles = [0, 0, 0]
-    pc = np.zeros((4, PMAX, 2)); pa = np.zeros((4, PMAX, 2))
-    eps11 = np.zeros(4, np.float32); sig11 = np.zeros(4, np.float32)
+    pc = np.zeros((6, PMAX, 2)); pa = np.zeros((6, PMAX, 2))
+    eps11 = np.zeros(6, np.float32); sig11 = np.zeros(6, np.float32)
     if disp:
         epr.append(2.0); sig.append(0.005); kind.append(KIND_EDISP); npoles.append(2)
         c, a = lorentz_pair(5e9, 2e9, 1.5)
@@ -455,6 +479,28 @@ def scatterer_problem(n=(24, 22, 20), L=((4, 5), (3, 4), (5, 3)), seed: int = 7,
         x0, y0, z0 = (inner[0][0] + 1, inner[1][0] + 1, inner[2][0] + 1)
         box[z0:z0 + 4, y0:y0 + 5, x0:x0 + 6] = True
         paint_voxels(mi, box, 3, prio)
+    mnp = [0] * 6
+    mpc = np.zeros((6, PMAX, 2)); mpa = np.zeros((6, PMAX, 2))
+    mu11 = np.zeros(6, np.float32); sigm11 = np.zeros(6, np.float32)
+    mur_rel = [1.0] * 6
+    if mdisp:
+        if disp:  # the electric Lorentz blob also carries one real magnetic pole
+            kind[3] |= KIND_MDISP
+            mnp[3] = 1
+            mpc[3, 0] = (0.4 * MU0 * 2e9, 0.0); mpa[3, 0] = (-2e9, 0.0)
+            mu11[3] = np.float32(MU0); sigm11[3] = 0.0
+        m = len(epr)
+        epr.append(1.0); sig.append(0.0); kind.append(KIND_MDISP)
+        prio = np.concatenate([prio[:m], [30]]).astype(np.int64)
+        mnp[m] = 2
+        c, a = lorentz_pair(3e9, 1e9, 0.8, eps0=MU0)
+        mpc[m, :2] = c; mpa[m, :2] = a
+        mu11[m] = np.float32(1.5 * MU0); sigm11[m] = 40.0
+        mur_rel[m] = 1.5
+        box = np.zeros((nz, ny, nx), bool)
+        x1, y1, z1 = inner[0][1] - 1, inner[1][1] - 1, inner[2][1] - 1
+        box[z1 - 4:z1, y1 - 5:y1, x1 - 6:x1] = True
+        paint_voxels(mi, box, m, prio)
     nm = len(epr) - 1
     D = [uniform_D(nx, dx), uniform_D(ny, dx),
          graded_D(nz, dx, graded_z, L[2]) if graded_z != 1.0 else uniform_D(nz, dx)]
@@ -470,9 +516,13 @@ def scatterer_problem(n=(24, 22, 20), L=((4, 5), (3, 4), (5, 3)), seed: int = 7,
     for p in range(nprobes):
         c = p % 6
         probes.append((c,) + tuple(int(rng.integers(0, sweep_hi(c, a, n) + 1)) for a in range(3)))
-    return Problem(nx, ny, nz, epr=epr, sigma=sig, mur=[1.0] * (nm + 1), sigmam=[0.0] * (nm + 1),
+    npoles += [0] * (nm + 1 - len(npoles))
+    return Problem(nx, ny, nz, epr=epr, sigma=sig, mur=mur_rel[:nm + 1], sigmam=[0.0] * (nm + 1),
                    kind=kind, mi=mi, D=D, dt=dt, npoles=npoles, pole_c=pc[:nm + 1],
                    pole_a=pa[:nm + 1], eps11=eps11[:nm + 1], sig11=sig11[:nm + 1],
+                   mnpoles=mnp[:nm + 1], mpole_c=mpc[:nm + 1], mpole_a=mpa[:nm + 1],
+                   mu11=mu11[:nm + 1], sigm11=sigm11[:nm + 1],
+                   mur_faces=tuple(mur) if mur else (0,) * 6,
                    cpml_layers=[list(l) for l in L], cpml_order=[[order, order]] * 3,
                    cpml_refl=[[refl, refl]] * 3, alphamax=alphamax, kappamax=kappamax,
                    nodal=srcs, probes=probes, nsteps_max=nsteps, name="scatterer")

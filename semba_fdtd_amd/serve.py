@@ -70,12 +70,12 @@ def serve_plain(requests, mode: str = "fast") -> List[Result]:
 
 def serve_batched(requests, mode: str = "fast", mem_budget: int = 200 << 30,
                   max_chunk: int = 250, rows: int = 4, zchunk: int = 32,
-                  blocks: int = 0) -> List[Result]:
+                  blocks: int = 0, xcells: int = 1) -> List[Result]:
     """Continuous batching: admit arrived requests while the memory budget
     allows, advance the batch in chunks that end when the next admitted grid
     finishes (or after max_chunk steps), retire finished grids."""
     t0 = time.perf_counter()
-    batch = Batch(rows=rows, zchunk=zchunk, blocks=blocks)
+    batch = Batch(rows=rows, zchunk=zchunk, blocks=blocks, xcells=xcells)
     waiting = list(requests)
     active = {}          # slot -> (solver, arrival, steps)
     used = 0

@@ -18,9 +18,10 @@ import numpy as np
 
 from . import _lib
 from ._lib import FdtdError, f32p, f64p, i16p, i32p, ptr
-from .problem import Problem, PMAX, KIND_EDISP, alloc_extent, to_alloc_box
+from .problem import Problem, PMAX, KIND_EDISP, KIND_MDISP, alloc_extent, to_alloc_box
 
 MODES = {"fast": _lib.MODE_FAST, "exact": _lib.MODE_EXACT}
+STORAGE = {"fp32": _lib.STORE_FP32, "fp16": _lib.STORE_FP16}
 
 
 @dataclass
@@ -41,6 +42,12 @@ class Tables:
     beta: np.ndarray
     g3: np.ndarray
     lines: List[np.ndarray] = None
+    mdisp_media: np.ndarray = None
+    mdisp_npoles: np.ndarray = None
+    mkappa: np.ndarray = None
+    mbeta: np.ndarray = None
+    gm3: np.ndarray = None
+    cab1: np.ndarray = None             # [6][nmedia+1] first-order Mur, or None
 
 
 def _check(lib, h, rc, what):
@@ -68,7 +75,9 @@ def build_tables(pb: Problem) -> Tables:
                                ptr(Ie, f32p), ptr(Ih, f32p))
         b1, c1, b2, c2 = np.ones(n + 3, np.float32), z(n + 3), np.ones(n + 3, np.float32), z(n + 3)
         L = pb.cpml_layers[a]
-        if L[0] or L[1]:
+        # the reference computes the profiles of all three axes as soon as one
+        # face has layers (borderscpml.F90:228-300, 1465-1733)
+        if pb.has_cpml:
             rc = lib.fdtd_build_cpml_axis(
                 n, int(L[0]), int(L[1]), float(pb.cpml_order[a][0]), float(pb.cpml_order[a][1]),
                 float(pb.cpml_refl[a][0]), float(pb.cpml_refl[a][1]), pb.alphamax, pb.alphaorden,
@@ -92,6 +101,35 @@ def build_tables(pb: Problem) -> Tables:
             raise FdtdError(f"fdtd_build_ade failed for medium {m}")
         kappa[d], beta[d], g3[d] = kk, bb, gg
         g1[m], g2[m] = a1[0], a2[0]
+    # magnetic media after the electric ones (InitMDispersives follows
+    # InitEDispersives, timestepping.F90:927-943)
+    mdisp = [m for m in range(1, nm + 1) if pb.kind[m] & KIND_MDISP]
+    nmd = len(mdisp)
+    mkappa = np.zeros((max(nmd, 1), PMAX, 2)); mbeta = np.zeros_like(mkappa)
+    gm3 = np.zeros_like(mkappa)
+    for d, m in enumerate(mdisp):
+        a1, a2 = np.zeros(1, np.float32), np.zeros(1, np.float32)
+        c = np.ascontiguousarray(pb.mpole_c[m]); a = np.ascontiguousarray(pb.mpole_a[m])
+        kk = np.zeros((PMAX, 2)); bb = np.zeros((PMAX, 2)); gg = np.zeros((PMAX, 2))
+        rc = lib.fdtd_build_ade(int(pb.mnpoles[m]), ptr(c, f64p), ptr(a, f64p),
+                                float(pb.mu11[m]), float(pb.sigm11[m]), pb.dt, ptr(kk, f64p),
+                                ptr(bb, f64p), ptr(gg, f64p), ptr(a1, f32p), ptr(a2, f32p))
+        if rc:
+            raise FdtdError(f"fdtd_build_ade failed for magnetic medium {m}")
+        mkappa[d], mbeta[d], gm3[d] = kk, bb, gg
+        gm1[m], gm2[m] = a1[0], a2[0]
+    # first-order Mur after the CPML stretching (InitMURBorders follows
+    # InitCPMLBorders, timestepping.F90:635-669)
+    cab1 = None
+    if any(pb.mur_faces):
+        idf = np.array([idh[a][(n + 1) if hi else 0]
+                        for a, n in enumerate((pb.nx, pb.ny, pb.nz)) for hi in (0, 1)],
+                       np.float32)
+        cab1 = np.zeros((6, nm + 1), np.float32)
+        rc = lib.fdtd_build_mur(nm, ptr(pb.epr, f32p), ptr(pb.mur, f32p), pb.dt, pb.eps0,
+                                pb.mu0, ptr(idf, f32p), ptr(cab1, f32p))
+        if rc:
+            raise FdtdError("fdtd_build_mur failed")
     lines = []
     for a, n in enumerate((pb.nx, pb.ny, pb.nz)):
         ln = z(n + 3)
@@ -102,21 +140,52 @@ def build_tables(pb: Problem) -> Tables:
         lines.append(ln)
     return Tables(g1, g2, gm1, gm2, ide, idh, pbe, pce, pbm, pcm,
                   np.array(disp, np.int32), np.array([pb.npoles[m] for m in disp], np.int32),
-                  kappa, beta, g3, lines)
+                  kappa, beta, g3, lines, np.array(mdisp, np.int32),
+                  np.array([pb.mnpoles[m] for m in mdisp], np.int32), mkappa, mbeta, gm3, cab1)
+
+
+ETA0 = 376.730313
+
+
+def fp16_scales(pb: Problem, t: Tables) -> List[float]:
+    """Per-array scales for binary16 field storage: the peak field the sources
+    can drive, estimated from their amplitudes, mapped to 4096 so that values
+    keep a factor 16 of headroom below the binary16 maximum (65504).  H arrays
+    take the E scale over the vacuum impedance."""
+    peak = 0.0
+    for w in pb.planewaves:
+        amp = max(float(np.linalg.norm(e)) for e in w.e)
+        peak = max(peak, 2.0 * amp * float(np.abs(w.evol).max()))
+    dmin = min(float(d.min()) for d in pb.D)
+    for src in pb.nodal:
+        ev = np.abs(np.asarray(src.evol, np.float64))
+        m = abs(src.amp) * float(ev.max())
+        if src.comp >= 3:
+            m *= ETA0
+        elif not src.hard:
+            # a soft current integrates G2 * amp * f / d^2 over the pulse
+            width = max(1, int((ev > 0.5 * ev.max()).sum()))
+            m *= float(np.max(t.g2)) / (dmin * dmin) * width
+        peak = max(peak, m)
+    e = (peak if peak > 0 else 1.0) / 4096.0
+    return [e, e, e, e / ETA0, e / ETA0, e / ETA0]
 
 
 class Solver:
     """One grid resident in HBM.  ``mode`` is "fast" (FMA, folded fp32 poles)
-    or "exact" (bit-identical to the reference loops)."""
+    or "exact" (bit-identical to the reference loops); ``storage`` "fp16"
+    keeps the fields as scaled binary16 (FAST only)."""
 
     def __init__(self, pb: Problem, mode: str = "fast", device: int = 0, stream=None,
-                 tables: Tables | None = None, upload_media: bool = True, probe_buffer: int = 4096):
+                 tables: Tables | None = None, upload_media: bool = True, probe_buffer: int = 4096,
+                 storage: str = "fp32"):
         self.lib = lib = _lib.load()
         self.pb = pb
         self.n3 = (pb.nx, pb.ny, pb.nz)
         self.mode = mode
+        self.storage = storage
         self.t = t = tables if tables is not None else build_tables(pb)
-        d = _lib.FdtdGridDesc(pb.nx, pb.ny, pb.nz, MODES[mode], _lib.STORE_FP32, device,
+        d = _lib.FdtdGridDesc(pb.nx, pb.ny, pb.nz, MODES[mode], STORAGE[storage], device,
                               stream)
         h = C.c_void_p()
         rc = lib.fdtd_create(C.byref(d), C.byref(h))
@@ -124,6 +193,9 @@ class Solver:
             raise FdtdError(f"fdtd_create failed ({rc}): {lib.fdtd_last_error(None).decode()}")
         self.h = h
         self._keep = []
+        if storage == "fp16":
+            for c, sc in enumerate(fp16_scales(pb, t)):
+                self._ck(lib.fdtd_set_field_scale(h, c, sc), "fdtd_set_field_scale")
         self._ck(lib.fdtd_set_media(h, pb.nmedia, ptr(t.g1, f32p), ptr(t.g2, f32p),
                                     ptr(t.gm1, f32p), ptr(t.gm2, f32p), ptr(pb.kind, i32p)),
                  "fdtd_set_media")
@@ -152,6 +224,19 @@ class Solver:
             self._keep += [dm, dn, k, b, g]
             self._ck(lib.fdtd_set_edispersive(h, nd, ptr(dm, i32p), ptr(dn, i32p), ptr(k, f64p),
                                               ptr(b, f64p), ptr(g, f64p)), "fdtd_set_edispersive")
+        nmd = len(t.mdisp_media) if t.mdisp_media is not None else 0
+        if nmd:
+            dm = np.ascontiguousarray(t.mdisp_media, np.int32)
+            dn = np.ascontiguousarray(t.mdisp_npoles, np.int32)
+            k = np.ascontiguousarray(t.mkappa[:nmd]); b = np.ascontiguousarray(t.mbeta[:nmd])
+            g = np.ascontiguousarray(t.gm3[:nmd])
+            self._keep += [dm, dn, k, b, g]
+            self._ck(lib.fdtd_set_mdispersive(h, nmd, ptr(dm, i32p), ptr(dn, i32p), ptr(k, f64p),
+                                              ptr(b, f64p), ptr(g, f64p)), "fdtd_set_mdispersive")
+        if t.cab1 is not None:
+            faces = np.array(pb.mur_faces, np.int32)
+            cab = np.ascontiguousarray(t.cab1, np.float32)
+            self._ck(lib.fdtd_set_mur(h, ptr(faces, i32p), ptr(cab, f32p)), "fdtd_set_mur")
         nlast = max(pb.nsteps_max + 2, 2)
         self._ck(lib.fdtd_set_time(h, pb.dt, nlast, None), "fdtd_set_time")
         for s in pb.nodal:
@@ -311,6 +396,24 @@ class Solver:
         buf = C.create_string_buffer(state, len(state))
         self._ck(self.lib.fdtd_load_state(self.h, buf, len(state), C.byref(n)), "fdtd_load_state")
         self.n = n.value
+
+    def set_tiling(self, rows: int, zchunk: int, xcells: int = 1):
+        """Blocks of 64 x rows threads marching zchunk z planes, xcells x
+        cells per lane of the fused passes."""
+        self._ck(self.lib.fdtd_set_tiling(self.h, rows, zchunk, xcells), "fdtd_set_tiling")
+
+    def get_tiling(self):
+        r, k, x = C.c_int32(), C.c_int32(), C.c_int32()
+        self._ck(self.lib.fdtd_get_tiling(self.h, C.byref(r), C.byref(k), C.byref(x)),
+                 "fdtd_get_tiling")
+        return r.value, k.value, x.value
+
+    def time_halves(self, reps: int = 5):
+        """Average ms of the fused E pass (with poles), the fused H pass and
+        the pole update alone.  Advances the fields."""
+        ms = np.zeros(3, np.float32)
+        self._ck(self.lib.fdtd_time_halves(self.h, reps, ptr(ms, f32p)), "fdtd_time_halves")
+        return tuple(float(v) for v in ms)
 
     @property
     def device_bytes(self) -> int:

@@ -38,6 +38,11 @@ CASES = {
                                  dict(n=30, L=5, radius=4, nsteps=200, spread=8, delay=30,
                                       grading=1.05, direction=(0.3, -0.5, 0.8),
                                       polar=(0.0, 0.848, 0.53))),
+    "magnetic_dispersive": ("scatterer_problem", dict(nsteps=200, mdisp=True)),
+    "mur_walls": ("scatterer_problem", dict(nsteps=200, L=((0, 0), (0, 0), (0, 0)),
+                                            mur=(1, 1, 1, 1, 1, 1))),
+    "mur_after_cpml_mixed": ("scatterer_problem", dict(nsteps=150, L=((3, 4), (2, 3), (4, 2)),
+                                                       mur=(1, 1, 0, 1, 1, 0), mdisp=True)),
 }
 
 

@@ -37,6 +37,16 @@ def test_python_binding_covers_the_header():
     assert set(declared_symbols()) == set(_lib.SIGNATURES)
 
 
+def test_fortran_module_binds_the_header():
+    """The iso_c_binding module (the Fortran callers' side) binds every entry
+    point except the ones returning C strings, which it wraps as
+    fdtd_error_message."""
+    text = open(os.path.join(ROOT, "semba_fdtd_amd", "fortran", "sembafdtd_gpu.F90")).read()
+    bound = set(re.findall(r'name="(fdtd_[a-z0-9_]+)"', text))
+    want = set(declared_symbols()) - {"fdtd_batch_last_error"}
+    assert not (want - bound), sorted(want - bound)
+
+
 def test_abi_version():
     assert sf.load_library().fdtd_abi_version() == 1
 

@@ -19,16 +19,19 @@ def _problems():
     b.dfts = [P.DftBox(0, (8, 20, 9, 18, 10, 12), (2, 1, 1),
                        np.array([0.5e9, 2e9, 3.5e9], np.float32))]
     c = P.cavity_problem(n=20, nsteps=100)                    # no CPML, hard Ez
-    return [(a, 150), (b, 120), (c, 100)]
+    d = P.scatterer_problem(nsteps=90, n=(20, 18, 16), L=((3, 3), (2, 3), (3, 2)),
+                            mur=(1, 0, 1, 0, 0, 1), mdisp=True)  # Mur, magnetic poles
+    return [(a, 150), (b, 120), (c, 100), (d, 90)]
 
 
 def _state(s):
     return s.probes(), [s.field(k) for k in range(6)], s.dft_sums()
 
 
+@pytest.mark.parametrize("xcells", [1, 4])
 @pytest.mark.parametrize("mode", ["exact", "fast"])
-def test_batch_equals_alone(mode):
-    print(f"batch {mode}", flush=True)
+def test_batch_equals_alone(mode, xcells):
+    print(f"batch {mode} xcells {xcells}", flush=True)
     probs = _problems()
     alone = []
     for pb, ns in probs:
@@ -37,17 +40,20 @@ def test_batch_equals_alone(mode):
         alone.append(_state(s))
         s.close()
     sol = [sf.Solver(pb, mode=mode) for pb, _ in probs]
-    batch = Batch(rows=2, zchunk=8, blocks=64)
+    batch = Batch(rows=2, zchunk=8, blocks=64, xcells=xcells)
     sa = batch.admit(sol[0])
     sb = batch.admit(sol[1])
     batch.step(20)
     sc = batch.admit(sol[2])                       # joins while the others run
-    batch.step(100)
+    sd = batch.admit(sol[3])
+    batch.step(90)
+    batch.retire(sd)
+    batch.step(10)
     batch.retire(sb)
     batch.retire(sc)
     batch.step(30)
     batch.retire(sa)
-    assert [s.n for s in sol] == [150, 120, 100]
+    assert [s.n for s in sol] == [150, 120, 100, 90]
     for q, s in enumerate(sol):
         got = _state(s)
         want = alone[q]

@@ -1,6 +1,7 @@
 """Checkpoint / resume: a run stopped at step 50, saved, loaded into a fresh
 grid with the same set-up and continued equals the uninterrupted run bit for
-bit (fields, CPML psi, pole state, plane wave, nodal sources, DFT sums)."""
+bit (fields, CPML psi, electric and magnetic pole state, Mur planes, plane
+wave, nodal sources, DFT sums)."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _problem():
-    pb = P.scatterer_problem(nsteps=120)
+    pb = P.scatterer_problem(nsteps=120, mdisp=True, mur=(1, 1, 0, 1, 1, 0))
     pw = P.open_scatterer_problem(n=32, nsteps=120, spread=8, delay=20).planewaves[0]
     pw.box = (6, 18, 5, 16, 7, 15)
     pb.planewaves = [pw]

@@ -37,6 +37,21 @@ CASES = {
                            dict(n=28, L=4, radius=3, nsteps=150, spread=8, delay=30,
                                 direction=(-0.6, 0.0, -0.8), polar=(0.8, 0.0, -0.6),
                                 margin=2)),
+    # magnetic pole-residue media: a medium both electric and magnetic
+    # dispersive, and a magnetic-only Lorentz block
+    "magnetic_dispersive": ("scatterer_problem", dict(nsteps=150, mdisp=True)),
+    "magnetic_only": ("scatterer_problem", dict(nsteps=150, mdisp=True, disp=False)),
+    # first-order Mur: on every face of a grid without CPML, on top of CPML
+    # faces (MurAfterPML), and mixed with CPML and bare faces
+    "mur_walls": ("scatterer_problem", dict(nsteps=150, L=((0, 0), (0, 0), (0, 0)),
+                                            mur=(1, 1, 1, 1, 1, 1))),
+    "mur_after_cpml": ("scatterer_problem", dict(nsteps=150, mur=(1, 0, 0, 1, 0, 1))),
+    "mur_mixed": ("scatterer_problem", dict(nsteps=150, L=((3, 4), (2, 3), (4, 2)),
+                                            mur=(1, 1, 0, 1, 1, 0), mdisp=True)),
+    "mur_partial": ("scatterer_problem", dict(nsteps=150, L=((0, 0), (0, 0), (0, 0)),
+                                              mur=(1, 0, 1, 0, 0, 1))),
+    # CPML on one axis only: the reference still builds all three profiles
+    "cpml_x_only": ("scatterer_problem", dict(nsteps=120, L=((4, 5), (0, 0), (0, 0)))),
 }
 
 
