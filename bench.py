@@ -184,6 +184,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 1 GiB device copies (the known-size kernel that "
                          "scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tiling instead of timing the candidates")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1500)
@@ -227,6 +229,15 @@ def main():
     stream = torch.cuda.Stream()
     t0 = time.perf_counter()
     s = sf.Solver(pb, mode=args.mode, stream=stream.cuda_stream, probe_buffer=total)
+    tiling = None
+    if not args.no_tune:
+        # the grid is at rest: the source-free passes leave it at zero, so the
+        # sweep needs no checkpoint
+        from semba_fdtd_amd import tuning
+        tuned = tuning.autotune(s, reps=3, keep_state=False)
+        tiling = tuned["tiling"]
+        print(f"[bench] tiling rows={tiling[0]} zchunk={tiling[1]} xcells={tiling[2]} "
+              f"(E+H {tuned['ms']:.2f} ms)", file=sys.stderr, flush=True)
     s.run(args.warmup)  # builds pole state, captures the graph
     s.synchronize()
     setup_s = time.perf_counter() - t0
@@ -317,6 +328,7 @@ def main():
             "mode": args.mode,
             "cells": pb.cells,
             "dispersive_edges": ndisp,
+            "tiling": tiling,
             "step_algorithmic_bytes": step_bytes,
             "step_algorithmic_gbs": round(step_gbs, 1),
             "step_hbm_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),

@@ -13,6 +13,9 @@ Layout: little-endian stream, no record markers.
     float32 epr, sigma, mur, sigmam [nmedia+1]; int32 kind [nmedia+1]
     int32 npoles [nmedia+1]; float64 pole_c, pole_a [nmedia+1][8][2]
     float32 eps11, sig11 [nmedia+1]
+    int32 mnpoles [nmedia+1]; float64 mpole_c, mpole_a [nmedia+1][8][2]
+    float32 mu11, sigm11 [nmedia+1]                      (magnetic poles)
+    int32 mur_faces[6]                                   (first-order Mur)
     float32 D_x [nx+4], D_y [ny+4], D_z [nz+4]        (index -2 .. n+1)
     int32 layers[6]; float32 order[6], refl[6], alphamax, alphaorden, kappamax
     int16 media index of Ex..Hz, each over its own Alloc box, x fastest
@@ -33,7 +36,7 @@ import numpy as np
 from .problem import Problem, PMAX, to_alloc_box
 
 MAGIC = b"SFDC"
-VERSION = 2
+VERSION = 3
 
 
 def write_case(pb: Problem, path: str, nsteps: int) -> None:
@@ -55,6 +58,12 @@ def write_case(pb: Problem, path: str, nsteps: int) -> None:
             assert a.shape == (nm + 1, PMAX, 2)
             f.write(f64(a))
         f.write(f32(pb.eps11) + f32(pb.sig11))
+        f.write(np.ascontiguousarray(pb.mnpoles, "<i4").tobytes())
+        for a in (pb.mpole_c, pb.mpole_a):
+            assert a.shape == (nm + 1, PMAX, 2)
+            f.write(f64(a))
+        f.write(f32(pb.mu11) + f32(pb.sigm11))
+        f.write(i32(*pb.mur_faces))
         for a in range(3):
             f.write(f32(pb.D[a]))
         f.write(i32(*[pb.cpml_layers[a][s] for a in range(3) for s in range(2)]))

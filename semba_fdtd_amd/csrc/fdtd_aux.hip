@@ -26,10 +26,8 @@ __global__ void k_nodal(DevGrid g, const NodalCell *cells, int ncells,
     const NodalCell cell = cells[c];
     if (cell.skip) continue;
     const float v = values[(long long)n * nsrc + cell.src];
-    if (cell.hard)
-      field_st(g, cell.comp, cell.p, cell.amp * v);
-    else
-      field_st(g, cell.comp, cell.p, field_ld(g, cell.comp, cell.p) - cell.coef * cell.amp * v);
+    field_st(g, cell.comp, cell.p,
+             nodal_apply(cell.hard ? 0.0f : field_ld(g, cell.comp, cell.p), cell, v));
   }
 }
 
@@ -45,18 +43,6 @@ int launch_nodal(const DevGrid &g, const NodalCell *cells, int ncells,
 }
 
 // ---- TF/SF plane waves (planewaves.F90) ----------------------------------
-// evolucion of the plane wave (planewaves.F90:824-851): linear interpolation
-// of the excitation at the retarded time t - d/c, zero before the second
-// sample and after the table.
-__device__ __forceinline__ float pw_evol(const PwWave &w, float t, float d) {
-  float ev = 0.0f;
-  const long long nprev = (long long)(int)((t - d / w.cluz) / w.deltaevol);
-  if (nprev + 1 <= w.numus && nprev > 0)
-    ev = (w.evol[nprev + 1] - w.evol[nprev]) / w.deltaevol *
-             ((t - d / w.cluz) - (float)nprev * w.deltaevol) + w.evol[nprev];
-  return ev;
-}
-
 // A group of TF/SF faces of AdvancePlaneWaveE/H (planewaves.F90:878-1475),
 // possibly of several grids, in one launch: thread t takes cell t of the
 // concatenated faces (the face is found by binary search in the prefix of face
@@ -82,25 +68,15 @@ __global__ void __launch_bounds__(256) k_tfsf_list(const PwRun *runs, const long
   cc[va] = f.lo[va] + (int)(c / r.nu);
   const int step = r.dn ? *r.dn : n;
   const float tm = r.half ? (float)(r.tiempo[step] + (double)0.5f * r.dt) : (float)r.tiempo[step];
-  int q[3] = {cc[0], cc[1], cc[2]};
-  q[f.axis] += f.off;
-  const float xf = r.phys[(size_t)(f.nf * 3 + 0) * r.pstride + q[0] + 1];
-  const float yf = r.phys[(size_t)(f.nf * 3 + 1) * r.pstride + q[1] + 1];
-  const float zf = r.phys[(size_t)(f.nf * 3 + 2) * r.pstride + q[2] + 1];
-  const PwWave &w = r.wave[f.wave];
-  float ehi = 0.0f;
-  for (int m = 0; m < w.nmodes; ++m) {
-    const float d = (xf * w.pn[m][0] + yf * w.pn[m][1] + zf * w.pn[m][2]) - w.dist0[m];
-    ehi = ehi + w.fpw[f.nf][m] * pw_evol(w, tm, d);
-  }
+  const float ehi = pw_incid(f, r.wave, r.phys, r.pstride, tm, cc);
   const long long p = (long long)(cc[2] + 1) * r.pxy + (long long)(cc[1] + 1) * r.px + kXOff + cc[0];
   if (r.fp16) {
     __half *F = reinterpret_cast<__half *>(r.F);
     const float old = __half2float(F[p]) * r.fs;
-    F[p] = __float2half_rn((f.sign > 0.0f ? old + r.G * ehi * f.id : old - r.G * ehi * f.id) * r.fi);
+    F[p] = __float2half_rn(pw_apply(old, f.sign, r.G, ehi, f.id) * r.fi);
   } else {
     float *F = r.F;
-    F[p] = f.sign > 0.0f ? F[p] + r.G * ehi * f.id : F[p] - r.G * ehi * f.id;
+    F[p] = pw_apply(F[p], f.sign, r.G, ehi, f.id);
   }
 }
 

@@ -407,7 +407,6 @@ int build_nodal(fdtd_handle *h) {
     }
   int rc;
   for (void *p : h->nodal_allocs) hipFree(p);
-  for (void *p : h->mur_allocs) hipFree(p);
   h->nodal_allocs.clear();
   h->dvalues = nullptr;
   h->dcells[0] = h->dcells[1] = nullptr;
@@ -828,6 +827,7 @@ int fdtd_destroy(fdtd_handle *h) {
   invalidate_graph(h);
   for (void *p : h->ade_allocs) hipFree(p);
   for (void *p : h->nodal_allocs) hipFree(p);
+  for (void *p : h->mur_allocs) hipFree(p);
   for (void *p : h->pw_allocs) hipFree(p);
   for (void *p : h->dft_allocs) hipFree(p);
   for (void *p : h->dft_sums) hipFree(p);

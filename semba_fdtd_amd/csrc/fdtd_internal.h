@@ -170,6 +170,40 @@ struct ProbeDev {
   int comp;
 };
 
+constexpr int kStepPlanes = 8;
+
+// One full step of one grid in a single z-marching launch (E then H of each
+// plane, both fields read from gi and written to go; see k_step in
+// fdtd_kernels.hip).  Tiles of 64 x R threads overlap their neighbours by one
+// column and one row, and each z chunk recomputes the E plane above it, so
+// every H value a tile writes has its E neighbours computed in the same
+// block.  The plane-wave faces and nodal cells are applied inline, in the
+// reference's order.
+struct StepDev {
+  DevGrid gi, go;             // read (step n) and write (step n+1) views
+  TileMap tm;                 // tx, ty, tz over strides 63 x (R-1) x kc
+  // TF/SF faces of the E and H halves, in the reference's face order
+  int npw[2];
+  const PwFace *pwf[2];
+  const PwWave *waves;
+  const float *phys;
+  int pstride;
+  const double *tiempo;
+  double dt;
+  float pwG[2];               // G2(1), GM2(1)
+  // the planes holding faces, per half and axis (npl -1: too many to list,
+  // every wave checks the faces)
+  int npl[2][3];
+  int pl[2][3][kStepPlanes];
+  // nodal cells of the E and H halves, in the reference's order
+  int ncell[2];
+  const NodalCell *cells[2];
+  int cellbox[2][6];          // bounding box of each half's cells
+  const float *values;        // [step][source]
+  int nsrc;
+  const int *dn;              // device step counter
+};
+
 // a batch of independent grids for the persistent face-pass launch: tile t of
 // the batch belongs to the grid b with start[b] <= t < start[b+1]
 struct BatchDev {
@@ -190,6 +224,54 @@ __device__ __forceinline__ void field_st(const DevGrid &g, int c, long long p, f
   if (g.fp16) reinterpret_cast<__half *>(g.f[c])[p] = __float2half_rn(v * g.fi[c]);
   else g.f[c][p] = v;
 }
+
+// The TF/SF and nodal arithmetic, shared by their own kernels and the
+// one-launch step.  Written without contraction in every build, so both give
+// the same bits.
+// evolucion of the plane wave (planewaves.F90:824-851): linear interpolation
+// of the excitation at the retarded time t - d/c, zero before the second
+// sample and after the table.
+__device__ __forceinline__ float pw_evol(const PwWave &w, float t, float d) {
+#pragma clang fp contract(off)
+  float ev = 0.0f;
+  const long long nprev = (long long)(int)((t - d / w.cluz) / w.deltaevol);
+  if (nprev + 1 <= w.numus && nprev > 0)
+    ev = (w.evol[nprev + 1] - w.evol[nprev]) / w.deltaevol *
+             ((t - d / w.cluz) - (float)nprev * w.deltaevol) + w.evol[nprev];
+  return ev;
+}
+
+// Incid (planewaves.F90:778-853) of face f at cell cc and time tm: the
+// incident component f.nf at the neighbouring position, summed over modes
+__device__ __forceinline__ float pw_incid(const PwFace &f, const PwWave *waves,
+                                          const float *phys, long long pstride, float tm,
+                                          const int cc[3]) {
+#pragma clang fp contract(off)
+  int q[3] = {cc[0], cc[1], cc[2]};
+  q[f.axis] += f.off;
+  const float xf = phys[(size_t)(f.nf * 3 + 0) * pstride + q[0] + 1];
+  const float yf = phys[(size_t)(f.nf * 3 + 1) * pstride + q[1] + 1];
+  const float zf = phys[(size_t)(f.nf * 3 + 2) * pstride + q[2] + 1];
+  const PwWave &w = waves[f.wave];
+  float ehi = 0.0f;
+  for (int m = 0; m < w.nmodes; ++m) {
+    const float d = (xf * w.pn[m][0] + yf * w.pn[m][1] + zf * w.pn[m][2]) - w.dist0[m];
+    ehi = ehi + w.fpw[f.nf][m] * pw_evol(w, tm, d);
+  }
+  return ehi;
+}
+
+// F +- G * Incid * Id (AdvancePlaneWaveE/H)
+__device__ __forceinline__ float pw_apply(float v, float sign, float G, float ehi, float id) {
+#pragma clang fp contract(off)
+  return sign > 0.0f ? v + G * ehi * id : v - G * ehi * id;
+}
+
+// AdvanceNodalE/H on one cell: hard F = amp*f, soft F = F - coef*amp*f
+__device__ __forceinline__ float nodal_apply(float v, const NodalCell &c, float f) {
+#pragma clang fp contract(off)
+  return c.hard ? c.amp * f : v - c.coef * c.amp * f;
+}
 #endif
 
 // launchers, one set per arithmetic mode
@@ -199,6 +281,8 @@ __device__ __forceinline__ void field_st(const DevGrid &g, int c, long long p, f
                int yee, int cpml, int ade, void *stream);                    \
   int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4,      \
                int yee, int cpml, int ade, void *stream);                    \
+  size_t step_lds_bytes(const DevGrid &g, int rows);                         \
+  int launch_step(const StepDev &sd, int rows, void *stream);                \
   int launch_pass_batch(const BatchDev &bd, int half, int blocks, int rows,  \
                         int vec, size_t lds, int fp16, void *stream);        \
   }

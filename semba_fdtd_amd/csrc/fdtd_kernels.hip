@@ -158,7 +158,7 @@ __device__ __forceinline__ MedTab stage_tables(const DevGrid &g, const float *a,
     const int nt = blockDim.x * blockDim.y;
     for (int m = tid; m <= nm; m += nt) {
       lds[m] = make_float2(a[m], b[m]);
-      if (ADE) ls[m] = ad->dslot[m];
+      if (ADE) ls[m] = nd ? ad->dslot[m] : -1;
     }
     if (ADE) {
       const coef_t *cf = ade_coefs(*ad);
@@ -199,19 +199,24 @@ struct AdePre {
 };
 
 // F -= sum_p Re(G3_p J_p); J_p = kappa_p J_p + beta_p (F - Fprev)/dt;
-// Fprev = F  (F = E with G3, or H with GM3)
+// Fprev = F  (F = E with G3, or H with GM3).  The state is read from ai and
+// the new state written to ao when wr (the same set for in-place updates).
+// FAST arithmetic is written out as fused multiply-adds so that every kernel
+// that inlines it rounds the same way.
 template <int C>
-__device__ __forceinline__ float ade_apply(const AdeSet &ad, double dt, const MedTab &t,
-                                           const AdePre &a, float e) {
+__device__ __forceinline__ float ade_apply(const AdeSet &ai, const AdeSet &ao, bool wr,
+                                           double dt, const MedTab &t, const AdePre &a,
+                                           float e) {
   const coef_t *cf = t.acf + (size_t)a.slot * kPMax * 6;
-  const long long N = ad.nstate[C];
-  jstate_t *J = reinterpret_cast<jstate_t *>(ad.jst[C]);
+  const long long N = ai.nstate[C];
+  const jstate_t *Ji = reinterpret_cast<const jstate_t *>(ai.jst[C]);
+  jstate_t *Jo = reinterpret_cast<jstate_t *>(ao.jst[C]);
 #if SFD_EXACT
   // one pole at a time, rounding E after each (electricdispersive.F90:319-321)
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
     if (q >= a.np) break;
-    const double2 jv = q < kPre ? a.j[q] : J[q * N + a.s];
+    const double2 jv = q < kPre ? a.j[q] : Ji[q * N + a.s];
     const double re = cf[q * 6 + 4] * jv.x - cf[q * 6 + 5] * jv.y;
     e = (float)((double)e - re);
   }
@@ -219,35 +224,37 @@ __device__ __forceinline__ float ade_apply(const AdeSet &ad, double dt, const Me
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
     if (q >= a.np) break;
-    const double2 jv = q < kPre ? a.j[q] : J[q * N + a.s];
+    const double2 jv = q < kPre ? a.j[q] : Ji[q * N + a.s];
     const double kr = cf[q * 6 + 0], ki = cf[q * 6 + 1];
-    const double ar = kr * jv.x - ki * jv.y;
-    const double ai = kr * jv.y + ki * jv.x;
+    const double kjr = kr * jv.x - ki * jv.y;
+    const double kji = kr * jv.y + ki * jv.x;
     const double br = cf[q * 6 + 2] * (double)diff / dt;
     const double bi = cf[q * 6 + 3] * (double)diff / dt;
-    J[q * N + a.s] = make_double2(ar + br, ai + bi);
+    if (wr) Jo[q * N + a.s] = make_double2(kjr + br, kji + bi);
   }
 #else
   // folded fp32 form: beta is pre-divided by dt, G3 of a folded pair doubled
+  (void)dt;
   float sum = 0.0f;
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
     if (q >= a.np) break;
-    const float2 jv = q < kPre ? a.j[q] : J[q * N + a.s];
-    sum += cf[q * 6 + 4] * jv.x - cf[q * 6 + 5] * jv.y;
+    const float2 jv = q < kPre ? a.j[q] : Ji[q * N + a.s];
+    sum = sum + __builtin_fmaf(cf[q * 6 + 4], jv.x, -(cf[q * 6 + 5] * jv.y));
   }
   e = e - sum;
   const float diff = e - a.ep;
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
     if (q >= a.np) break;
-    const float2 jv = q < kPre ? a.j[q] : J[q * N + a.s];
+    const float2 jv = q < kPre ? a.j[q] : Ji[q * N + a.s];
     const float kr = cf[q * 6 + 0], ki = cf[q * 6 + 1];
-    J[q * N + a.s] = make_float2(kr * jv.x - ki * jv.y + cf[q * 6 + 2] * diff,
-                                 kr * jv.y + ki * jv.x + cf[q * 6 + 3] * diff);
+    const float nr = __builtin_fmaf(kr, jv.x, __builtin_fmaf(-ki, jv.y, cf[q * 6 + 2] * diff));
+    const float ni = __builtin_fmaf(kr, jv.y, __builtin_fmaf(ki, jv.x, cf[q * 6 + 3] * diff));
+    if (wr) Jo[q * N + a.s] = make_float2(nr, ni);
   }
 #endif
-  ad.prev[C][a.s] = e;
+  if (wr) ao.prev[C][a.s] = e;
   return e;
 }
 
@@ -287,9 +294,9 @@ __global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
       if (q < a.np) a.j[q] = J[q * ad.nstate[c] + s];
     const float e = field_ld(g, f, p);
     float r;
-    if (c == 0) r = ade_apply<0>(ad, g.dt, tab, a, e);
-    else if (c == 1) r = ade_apply<1>(ad, g.dt, tab, a, e);
-    else r = ade_apply<2>(ad, g.dt, tab, a, e);
+    if (c == 0) r = ade_apply<0>(ad, ad, true, g.dt, tab, a, e);
+    else if (c == 1) r = ade_apply<1>(ad, ad, true, g.dt, tab, a, e);
+    else r = ade_apply<2>(ad, ad, true, g.dt, tab, a, e);
     field_st(g, f, p, r);
   }
 }
@@ -955,6 +962,346 @@ int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
   else if (cpml) SFD_H(false, true);
 #undef SFD_H
   if (ade) launch_ade(g, 1, s);
+  return (int)hipGetLastError();
+}
+
+// ---- one launch per step ---------------------------------------------------
+// k_step advances every field of the grid by one step (E to n+1, then H to
+// n+1/2), reading the state of step n from sd.gi and writing step n+1 to
+// sd.go: fields, CPML psi and pole state all live twice, and the two sets
+// alternate.  Per cell the arithmetic and its order are those of the two
+// passes (Yee, psi corrections, poles, plane-wave faces, nodal cells), so the
+// result is the same bit for bit; only the traffic changes: each field value
+// is read once and written once per step instead of twice.
+//
+// A block of 64 x R threads marches one z chunk of a tile.  Lanes 0..62 and
+// rows 0..R-2 own their cells; lane 63 and row R-1 overlap the neighbouring
+// tiles and compute only the E values the owned H cells need, and the plane
+// above the chunk is computed the same way.  E values travel to the x - 1
+// neighbour by a cross-lane shuffle and to the y - 1 neighbour through LDS;
+// H values of step n are read from memory (gi is never written during the
+// step).
+
+// psi <- b*psi + d*c from gi's array, written to go's array when wr
+__device__ __forceinline__ float psi_next(const float *pi, float *po, bool wr, long long s,
+                                          float b, float d, float c) {
+  const float v = madd(d * c, b, pi[s]);
+  if (wr) po[s] = v;
+  return v;
+}
+
+// the pole state of one dispersive edge of component C (side: E or H)
+template <int C>
+__device__ __forceinline__ float step_poles(const AdeSet &ai, const AdeSet &ao, bool wr,
+                                            double dt, const MedTab &t, int slot, int i, int j,
+                                            int k, float e) {
+  const int *b = ai.abox[C];
+  AdePre a;
+  a.s = ((long long)(k - b[2]) * b[4] + (j - b[1])) * b[3] + (i - b[0]);
+  a.slot = slot;
+  a.np = t.anp[slot];
+  a.ep = ai.prev[C][a.s];
+  const jstate_t *J = reinterpret_cast<const jstate_t *>(ai.jst[C]);
+#pragma unroll
+  for (int q = 0; q < kPre; ++q)
+    if (q < a.np) a.j[q] = J[q * ai.nstate[C] + a.s];
+  return ade_apply<C>(ai, ao, wr, dt, t, a, e);
+}
+
+// plane-wave faces and nodal cells of one half acting on component comp at
+// cell cc (offset p), in the reference's order
+__device__ __noinline__ float step_sources(const StepDev &sd, int half, int comp, long long p,
+                                           int i, int j, int k, float v, bool pw, bool nd,
+                                           int n) {
+  if (pw) {
+    const int cc[3] = {i, j, k};
+    const float tm = half ? (float)(sd.tiempo[n] + (double)0.5f * sd.dt) : (float)sd.tiempo[n];
+    for (int q = 0; q < sd.npw[half]; ++q) {
+      const PwFace &f = sd.pwf[half][q];
+      if (f.comp != comp || cc[f.axis] != f.fixed) continue;
+      const int ua = f.axis == 0 ? 1 : 0, va = f.axis == 2 ? 1 : 2;
+      if (cc[ua] < f.lo[ua] || cc[ua] > f.hi[ua] || cc[va] < f.lo[va] || cc[va] > f.hi[va])
+        continue;
+      const float ehi = pw_incid(f, sd.waves, sd.phys, sd.pstride, tm, cc);
+      v = pw_apply(v, f.sign, sd.pwG[half], ehi, f.id);
+    }
+  }
+  if (nd) {
+    for (int q = 0; q < sd.ncell[half]; ++q) {
+      const NodalCell &c = sd.cells[half][q];
+      if (c.comp != comp || c.p != p || c.skip) continue;
+      v = nodal_apply(v, c, sd.values[(long long)n * sd.nsrc + c.src]);
+    }
+  }
+  return v;
+}
+
+// does a wave of row j, plane k, columns [x0, x0+63] touch a TF/SF face of
+// the half: the x and y tests do not change along the z march
+__device__ __forceinline__ bool step_pw_xy(const StepDev &sd, int half, int x0, int j) {
+  if (sd.npw[half] == 0) return false;
+  if (sd.npl[half][0] < 0 || sd.npl[half][1] < 0 || sd.npl[half][2] < 0) return true;
+  for (int q = 0; q < sd.npl[half][0]; ++q)
+    if (sd.pl[half][0][q] >= x0 && sd.pl[half][0][q] <= x0 + kWave - 1) return true;
+  for (int q = 0; q < sd.npl[half][1]; ++q)
+    if (sd.pl[half][1][q] == j) return true;
+  return false;
+}
+__device__ __forceinline__ bool step_pw_z(const StepDev &sd, int half, int k) {
+  for (int q = 0; q < sd.npl[half][2]; ++q)
+    if (sd.pl[half][2][q] == k) return true;
+  return false;
+}
+__device__ __forceinline__ bool step_nd(const StepDev &sd, int half, int x0, int j, int k) {
+  const int *b = sd.cellbox[half];
+  return sd.ncell[half] > 0 && j >= b[1] && j <= b[4] && k >= b[2] && k <= b[5] &&
+         x0 + kWave - 1 >= b[0] && x0 <= b[3];
+}
+
+static __host__ __device__ __forceinline__ size_t step_tab_bytes(int nmedia, int nd) {
+  return (((size_t)(nmedia + 1) * 12 + (size_t)nd * 4 + 7) & ~(size_t)7) +
+         (size_t)nd * kPMax * 6 * sizeof(coef_t);
+}
+
+// ADE: grids with pole state on either field; SRC: grids with plane waves or
+// nodal sources (the variants without them carry no code for them)
+template <int R, bool ADE, bool SRC>
+__global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
+  extern __shared__ float2 lds[];
+  const DevGrid &gi = sd.gi;
+  const DevGrid &go = sd.go;
+  const size_t offh = step_tab_bytes(gi.nmedia, gi.ade[0].ndisp);
+  const size_t offx = offh + step_tab_bytes(gi.nmedia, gi.ade[1].ndisp);
+  const MedTab te = stage_tables<true, true>(gi, gi.g1, gi.g2, lds, &gi.ade[0]);
+  const MedTab th = stage_tables<true, true>(
+      gi, gi.gm1, gi.gm2, reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offh),
+      &gi.ade[1]);
+  float2 *xch = reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offx);
+
+  int bx, by, bz;
+  decode_tile(sd.tm, blockIdx.x, bx, by, bz);
+  const int lane = threadIdx.x, row = threadIdx.y;
+  const int x0 = bx * (kWave - 1);
+  const int i = x0 + lane, j = by * (R - 1) + row;
+  const int nx = gi.nx, ny = gi.ny, nz = gi.nz;
+  const bool in = i <= nx && j <= ny;
+  const bool own = in && lane < kWave - 1 && row < R - 1;
+  const int ic = min(i, nx + 1), jc = min(j, ny + 1);
+  const int k0 = bz * sd.tm.kc;
+  const int k1 = min(k0 + sd.tm.kc, nz + 1);
+  const long long px = gi.px, pxy = gi.pxy;
+  long long p = (long long)(k0 + 1) * pxy + (long long)(jc + 1) * px + kXOff + ic;
+  const int n = *sd.dn;
+  const double dt = gi.dt;
+  const int Lx = gi.L[0][0] + gi.L[0][1], Ly = gi.L[1][0] + gi.L[1][1];
+  // E side: node positions along x and y; H side: cell positions
+  int qxn = 0, qyn = 0, qxc = 0, qyc = 0;
+  const bool sxn = slab_node(i, nx, gi.L[0][0], gi.L[0][1], qxn);
+  const bool syn = slab_node(j, ny, gi.L[1][0], gi.L[1][1], qyn);
+  const bool sxc = slab_cell(i, nx, gi.L[0][0], gi.L[0][1], qxc);
+  const bool syc = slab_cell(j, ny, gi.L[1][0], gi.L[1][1], qyc);
+  const float idyh = gi.id[4][jc + 1], idxh = gi.id[3][ic + 1];
+  const float idye = gi.id[1][jc + 1], idxe = gi.id[0][ic + 1];
+  const bool pwxy_e = SRC && step_pw_xy(sd, 0, x0, j), pwxy_h = SRC && step_pw_xy(sd, 1, x0, j);
+  const bool ade_e = ADE && gi.ade[0].has, ade_h = ADE && gi.ade[1].has;
+
+  // H of step n on plane k0 - 1: the E update's z - 1 neighbours
+  float hx_km = gi.f[3][p - pxy], hy_km = gi.f[4][p - pxy];
+  // carried from plane k - 1 to the H update of that plane
+  float hx_c = 0.0f, hy_c = 0.0f, hz_c = 0.0f;          // H of step n
+  float ex_c = 0.0f, ey_c = 0.0f, ez_c = 0.0f;          // E of step n+1
+  float ey_ip_c = 0.0f, ez_ip_c = 0.0f, ex_jp_c = 0.0f, ez_jp_c = 0.0f;
+  int mhx_c = 0, mhy_c = 0, mhz_c = 0;
+
+  for (int k = k0; k <= k1; ++k, p += pxy) {
+    float ex_n = 0.0f, ey_n = 0.0f, ez_n = 0.0f;
+    float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f;
+    int mhx = 0, mhy = 0, mhz = 0;
+    if (k <= nz) {
+      // ---- issue the loads of plane k
+      hx0 = gi.f[3][p]; hy0 = gi.f[4][p]; hz0 = gi.f[5][p];
+      const float hz_jm = gi.f[5][p - px], hx_jm = gi.f[3][p - px];
+      const float hz_im = gi.f[5][p - 1], hy_im = gi.f[4][p - 1];
+      ex_n = gi.f[0][p]; ey_n = gi.f[1][p]; ez_n = gi.f[2][p];
+      const int mx = gi.mi[0][p], my = gi.mi[1][p], mz = gi.mi[2][p];
+      mhx = gi.mi[3][p]; mhy = gi.mi[4][p]; mhz = gi.mi[5][p];
+      const float idzh = gi.id[5][k + 1];
+      int qzn = 0;
+      const bool szn = slab_node(k, nz, gi.L[2][0], gi.L[2][1], qzn);
+      const bool wr = own && k < k1;
+      const bool pw = SRC && (pwxy_e || step_pw_z(sd, 0, k));
+      const bool nd = SRC && step_nd(sd, 0, x0, j, k);
+      // ---- E of step n+1 (Advance_Ex/Ey/Ez, AdvanceelectricCPML,
+      // AdvanceEDispersiveE, AdvancePlaneWaveE, AdvanceNodalE)
+      if (in && i < nx) {
+        const float2 c = te.gg[mx];
+        float e = yee_upd(c.x, ex_n, c.y, curl2(hz0 - hz_jm, idyh, hy0 - hy_km, idzh));
+        if (syn)
+          e = madd(e, c.y, psi_next(gi.psi[PSI_EXY], go.psi[PSI_EXY], wr,
+                                    ((long long)k * Ly + qyn) * nx + i, gi.pbe[1][j + 1],
+                                    hz0 - hz_jm, gi.pce[1][j + 1]));
+        if (szn)
+          e = madd(e, -c.y, psi_next(gi.psi[PSI_EXZ], go.psi[PSI_EXZ], wr,
+                                     ((long long)qzn * (ny + 1) + j) * nx + i, gi.pbe[2][k + 1],
+                                     hy0 - hy_km, gi.pce[2][k + 1]));
+        if (ade_e) {
+          const int sl = te.slot[mx];
+          if (sl >= 0) e = step_poles<0>(gi.ade[0], go.ade[0], wr, dt, te, sl, i, j, k, e);
+        }
+        if (pw || nd) e = step_sources(sd, 0, 0, p, i, j, k, e, pw, nd, n);
+        ex_n = e;
+        if (wr) go.f[0][p] = e;
+      }
+      if (in && j < ny) {
+        const float2 c = te.gg[my];
+        float e = yee_upd(c.x, ey_n, c.y, curl2(hx0 - hx_km, idzh, hz0 - hz_im, idxh));
+        if (szn)
+          e = madd(e, c.y, psi_next(gi.psi[PSI_EYZ], go.psi[PSI_EYZ], wr,
+                                    ((long long)qzn * ny + j) * (nx + 1) + i, gi.pbe[2][k + 1],
+                                    hx0 - hx_km, gi.pce[2][k + 1]));
+        if (sxn)
+          e = madd(e, -c.y, psi_next(gi.psi[PSI_EYX], go.psi[PSI_EYX], wr,
+                                     ((long long)k * ny + j) * Lx + qxn, gi.pbe[0][i + 1],
+                                     hz0 - hz_im, gi.pce[0][i + 1]));
+        if (ade_e) {
+          const int sl = te.slot[my];
+          if (sl >= 0) e = step_poles<1>(gi.ade[0], go.ade[0], wr, dt, te, sl, i, j, k, e);
+        }
+        if (pw || nd) e = step_sources(sd, 0, 1, p, i, j, k, e, pw, nd, n);
+        ey_n = e;
+        if (wr) go.f[1][p] = e;
+      }
+      if (in && k < nz) {
+        const float2 c = te.gg[mz];
+        float e = yee_upd(c.x, ez_n, c.y, curl2(hy0 - hy_im, idxh, hx0 - hx_jm, idyh));
+        if (syn)
+          e = madd(e, -c.y, psi_next(gi.psi[PSI_EZY], go.psi[PSI_EZY], wr,
+                                     ((long long)k * Ly + qyn) * (nx + 1) + i, gi.pbe[1][j + 1],
+                                     hx0 - hx_jm, gi.pce[1][j + 1]));
+        if (sxn)
+          e = madd(e, c.y, psi_next(gi.psi[PSI_EZX], go.psi[PSI_EZX], wr,
+                                    ((long long)k * (ny + 1) + j) * Lx + qxn, gi.pbe[0][i + 1],
+                                    hy0 - hy_im, gi.pce[0][i + 1]));
+        if (ade_e) {
+          const int sl = te.slot[mz];
+          if (sl >= 0) e = step_poles<2>(gi.ade[0], go.ade[0], wr, dt, te, sl, i, j, k, e);
+        }
+        if (pw || nd) e = step_sources(sd, 0, 2, p, i, j, k, e, pw, nd, n);
+        ez_n = e;
+        if (wr) go.f[2][p] = e;
+      }
+    }
+    // ---- E of plane k at x + 1 (next lane) and y + 1 (next row)
+    const float ey_ip_n = __shfl_down(ey_n, 1), ez_ip_n = __shfl_down(ez_n, 1);
+    const int par = (k - k0) & 1;
+    xch[(par * R + row) * kWave + lane] = make_float2(ex_n, ez_n);
+    __syncthreads();
+    float ex_jp_n = 0.0f, ez_jp_n = 0.0f;
+    if (row + 1 < R) {
+      const float2 v = xch[(par * R + row + 1) * kWave + lane];
+      ex_jp_n = v.x;
+      ez_jp_n = v.y;
+    }
+    // ---- H of step n+1/2 on plane k - 1 (Advance_Hx/Hy/Hz,
+    // AdvanceMagneticCPML, AdvanceMdispersiveH, AdvancePlaneWaveH,
+    // AdvanceNodalH)
+    if (k > k0 && own) {
+      const int kh = k - 1;
+      const long long ph = p - pxy;
+      const float idze = gi.id[2][kh + 1];
+      int qzc = 0;
+      const bool szc = slab_cell(kh, nz, gi.L[2][0], gi.L[2][1], qzc);
+      const bool pw = SRC && (pwxy_h || step_pw_z(sd, 1, kh));
+      const bool nd = SRC && step_nd(sd, 1, x0, j, kh);
+      if (j < ny && kh < nz) {
+        const float2 c = th.gg[mhx_c];
+        float h = yee_upd(c.x, hx_c, c.y, curl2(ey_n - ey_c, idze, ez_jp_c - ez_c, idye));
+        if (syc)
+          h = madd(h, -c.y, psi_next(gi.psi[PSI_HXY], go.psi[PSI_HXY], true,
+                                     ((long long)kh * Ly + qyc) * (nx + 1) + i,
+                                     gi.pbm[1][j + 1], ez_jp_c - ez_c, gi.pcm[1][j + 1]));
+        if (szc)
+          h = madd(h, c.y, psi_next(gi.psi[PSI_HXZ], go.psi[PSI_HXZ], true,
+                                    ((long long)qzc * ny + j) * (nx + 1) + i, gi.pbm[2][kh + 1],
+                                    ey_n - ey_c, gi.pcm[2][kh + 1]));
+        if (ade_h) {
+          const int sl = th.slot[mhx_c];
+          if (sl >= 0) h = step_poles<0>(gi.ade[1], go.ade[1], true, dt, th, sl, i, j, kh, h);
+        }
+        if (pw || nd) h = step_sources(sd, 1, 3, ph, i, j, kh, h, pw, nd, n);
+        go.f[3][ph] = h;
+      }
+      if (i < nx && kh < nz) {
+        const float2 c = th.gg[mhy_c];
+        float h = yee_upd(c.x, hy_c, c.y, curl2(ez_ip_c - ez_c, idxe, ex_n - ex_c, idze));
+        if (szc)
+          h = madd(h, -c.y, psi_next(gi.psi[PSI_HYZ], go.psi[PSI_HYZ], true,
+                                     ((long long)qzc * (ny + 1) + j) * nx + i,
+                                     gi.pbm[2][kh + 1], ex_n - ex_c, gi.pcm[2][kh + 1]));
+        if (sxc)
+          h = madd(h, c.y, psi_next(gi.psi[PSI_HYX], go.psi[PSI_HYX], true,
+                                    ((long long)kh * (ny + 1) + j) * Lx + qxc, gi.pbm[0][i + 1],
+                                    ez_ip_c - ez_c, gi.pcm[0][i + 1]));
+        if (ade_h) {
+          const int sl = th.slot[mhy_c];
+          if (sl >= 0) h = step_poles<1>(gi.ade[1], go.ade[1], true, dt, th, sl, i, j, kh, h);
+        }
+        if (pw || nd) h = step_sources(sd, 1, 4, ph, i, j, kh, h, pw, nd, n);
+        go.f[4][ph] = h;
+      }
+      if (i < nx && j < ny) {
+        const float2 c = th.gg[mhz_c];
+        float h = yee_upd(c.x, hz_c, c.y, curl2(ex_jp_c - ex_c, idye, ey_ip_c - ey_c, idxe));
+        if (syc)
+          h = madd(h, c.y, psi_next(gi.psi[PSI_HZY], go.psi[PSI_HZY], true,
+                                    ((long long)kh * Ly + qyc) * nx + i, gi.pbm[1][j + 1],
+                                    ex_jp_c - ex_c, gi.pcm[1][j + 1]));
+        if (sxc)
+          h = madd(h, -c.y, psi_next(gi.psi[PSI_HZX], go.psi[PSI_HZX], true,
+                                     ((long long)kh * ny + j) * Lx + qxc, gi.pbm[0][i + 1],
+                                     ey_ip_c - ey_c, gi.pcm[0][i + 1]));
+        if (ade_h) {
+          const int sl = th.slot[mhz_c];
+          if (sl >= 0) h = step_poles<2>(gi.ade[1], go.ade[1], true, dt, th, sl, i, j, kh, h);
+        }
+        if (pw || nd) h = step_sources(sd, 1, 5, ph, i, j, kh, h, pw, nd, n);
+        go.f[5][ph] = h;
+      }
+    }
+    // ---- carry plane k to the next iteration
+    hx_km = hx0; hy_km = hy0;
+    hx_c = hx0; hy_c = hy0; hz_c = hz0;
+    ex_c = ex_n; ey_c = ey_n; ez_c = ez_n;
+    ey_ip_c = ey_ip_n; ez_ip_c = ez_ip_n; ex_jp_c = ex_jp_n; ez_jp_c = ez_jp_n;
+    mhx_c = mhx; mhy_c = mhy; mhz_c = mhz;
+  }
+}
+
+size_t step_lds_bytes(const DevGrid &g, int rows) {
+  return step_tab_bytes(g.nmedia, g.ade[0].ndisp) + step_tab_bytes(g.nmedia, g.ade[1].ndisp) +
+         (size_t)2 * rows * kWave * sizeof(float2);
+}
+
+int launch_step(const StepDev &sd, int rows, void *stream) {
+  const size_t lds = step_lds_bytes(sd.gi, rows);
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)sd.tm.nblk);
+  const bool ade = sd.gi.ade[0].has || sd.gi.ade[1].has;
+  const bool src = sd.npw[0] || sd.npw[1] || sd.ncell[0] || sd.ncell[1];
+#define SFD_STEP(RR, A, S) \
+  hipLaunchKernelGGL((k_step<RR, A, S>), grid, dim3(kWave, RR), lds, s, sd)
+#define SFD_STEP_R(RR)                     \
+  do {                                     \
+    if (ade && src) SFD_STEP(RR, true, true);   \
+    else if (ade) SFD_STEP(RR, true, false);    \
+    else if (src) SFD_STEP(RR, false, true);    \
+    else SFD_STEP(RR, false, false);            \
+  } while (0)
+  if (rows == 4) SFD_STEP_R(4);
+  else if (rows == 8) SFD_STEP_R(8);
+  else return (int)hipErrorInvalidValue;
+#undef SFD_STEP_R
+#undef SFD_STEP
   return (int)hipGetLastError();
 }
 

@@ -2,7 +2,8 @@
 !
 ! Keeps the call structure of semba-fdtd: the set-up of launch_simulation
 ! (timestepping.F90:305-1014: spacings, calc_G1G2Gm1Gm2, InitCPMLBorders,
-! InitEDispersives, crea_timevector, nodal sources, plane waves, probes) done in Fortran
+! InitMURBorders, InitEDispersives, InitMDispersives, crea_timevector, nodal
+! sources, plane waves, probes) done in Fortran
 ! through the library's table builders, then the loop of ciclo_temporal
 ! (timestepping.F90:1248-1981) with one library call per reference call site.
 !
@@ -15,7 +16,8 @@
 !   graph      fdtd_run over the whole run (captured HIP graph)
 !   tables     stop after the host set-up and write the coefficient tables to
 !              OUT instead (g1,g2,gm1,gm2; per axis ide,idh,pbe,pce,pbm,pcm;
-!              kappa,beta,g3 of the dispersive media); needs no GPU
+!              kappa,beta,g3 of the electric and then the magnetic dispersive
+!              media; the Mur coefficients); needs no GPU
 program fdtd_gpu_driver
    use iso_c_binding
    use sembafdtd_gpu
@@ -39,6 +41,12 @@ program fdtd_gpu_driver
    real(c_float), allocatable :: epr(:), sigma(:), mur(:), sigmam(:), eps11(:), sig11(:)
    integer(c_int32_t), allocatable :: kind(:), npoles(:)
    real(c_double), allocatable :: pole_c(:,:,:), pole_a(:,:,:)
+   real(c_float), allocatable :: mu11(:), sigm11(:)
+   integer(c_int32_t), allocatable :: mnpoles(:)
+   real(c_double), allocatable :: mpole_c(:,:,:), mpole_a(:,:,:)
+   integer(c_int32_t) :: murf(6)
+   real(c_float) :: idhf(6)
+   real(c_float), allocatable :: cab1(:)
    real(c_float), allocatable :: g1(:), g2(:), gm1(:), gm2(:)
    type(r4vec) :: dd(3), de(3), dh(3), ide(3), idh(3), pbe(3), pce(3), pbm(3), pcm(3)
    integer(c_int32_t) :: layers(6)
@@ -49,8 +57,10 @@ program fdtd_gpu_driver
    real(c_float), allocatable :: evol(:)
    integer(c_int32_t) :: nprobe
    integer(c_int32_t), allocatable :: pc(:), pi(:), pj(:), pk(:)
-   integer(c_int32_t), allocatable :: dmed(:), dnp(:)
+   integer(c_int32_t), allocatable :: dmed(:), dnp(:), mdmed(:), mdnp(:)
    real(c_double), allocatable :: kappa(:,:,:), beta(:,:,:), g3(:,:,:)
+   real(c_double), allocatable :: mkappa(:,:,:), mbeta(:,:,:), gm3(:,:,:)
+   integer :: nmd
    real(c_double) :: kk(2, PMAX), bb(2, PMAX), gg(2, PMAX)
    real(c_float) :: a1(1), a2(1)
    real(c_double), allocatable :: tiempo(:)
@@ -83,7 +93,7 @@ program fdtd_gpu_driver
          status='old', action='read', iostat=ios)
    if (ios /= 0) call die('cannot open '//trim(casefile))
    read (u) magic, version
-   if (magic /= 'SFDC' .or. version /= 2) call die('not a version-2 case file')
+   if (magic /= 'SFDC' .or. version /= 3) call die('not a version-3 case file')
    read (u) nx, ny, nz, nmedia, nsteps
    nn = [nx, ny, nz]
    read (u) eps0, mu0, dt
@@ -91,6 +101,9 @@ program fdtd_gpu_driver
              kind(0:nmedia), npoles(0:nmedia), eps11(0:nmedia), sig11(0:nmedia))
    allocate (pole_c(2, PMAX, 0:nmedia), pole_a(2, PMAX, 0:nmedia))
    read (u) epr, sigma, mur, sigmam, kind, npoles, pole_c, pole_a, eps11, sig11
+   allocate (mnpoles(0:nmedia), mpole_c(2, PMAX, 0:nmedia), mpole_a(2, PMAX, 0:nmedia), &
+             mu11(0:nmedia), sigm11(0:nmedia))
+   read (u) mnpoles, mpole_c, mpole_a, mu11, sigm11, murf
    do ax = 1, 3
       allocate (dd(ax)%v(-2:nn(ax)+1))
       read (u) dd(ax)%v
@@ -117,7 +130,8 @@ program fdtd_gpu_driver
       call check_host(fdtd_build_spacing(n, dd(ax)%v, de(ax)%v, dh(ax)%v, ide(ax)%v, idh(ax)%v), &
                       'fdtd_build_spacing')
       pbe(ax)%v = 1.0; pce(ax)%v = 0.0; pbm(ax)%v = 1.0; pcm(ax)%v = 0.0
-      if (layers(2*ax-1) /= 0 .or. layers(2*ax) /= 0) then
+      ! InitCPMLBorders builds all three axes once any face has layers
+      if (has_cpml) then
          call check_host(fdtd_build_cpml_axis(n, layers(2*ax-1), layers(2*ax), order(2*ax-1), &
                  order(2*ax), refl(2*ax-1), refl(2*ax), alphamax, alphaorden, kappamax, dt, eps0, &
                  mu0, de(ax)%v, dh(ax)%v, pbe(ax)%v, pce(ax)%v, pbm(ax)%v, pcm(ax)%v, &
@@ -140,6 +154,31 @@ program fdtd_gpu_driver
       ! InitEDispersives overrides G1/G2 of the medium (electricdispersive.F90:140-146)
       g1(m) = a1(1); g2(m) = a2(1)
    end do
+   ! InitMDispersives (timestepping.F90:943): the same tables from mu11 and
+   ! Sigmam11, overriding GM1/GM2 (magneticdispersive.F90:122-135)
+   nmd = count(iand(kind(1:), FDTD_KIND_MDISP) /= 0)
+   allocate (mdmed(max(nmd, 1)), mdnp(max(nmd, 1)))
+   allocate (mkappa(2, PMAX, max(nmd, 1)), mbeta(2, PMAX, max(nmd, 1)), gm3(2, PMAX, max(nmd, 1)))
+   mkappa = 0; mbeta = 0; gm3 = 0
+   d = 0
+   do m = 1, nmedia
+      if (iand(kind(m), FDTD_KIND_MDISP) == 0) cycle
+      d = d + 1
+      kk = 0; bb = 0; gg = 0
+      call check_host(fdtd_build_ade(mnpoles(m), mpole_c(:, :, m), mpole_a(:, :, m), mu11(m), &
+                      sigm11(m), dt, kk, bb, gg, a1, a2), 'fdtd_build_ade')
+      mdmed(d) = m; mdnp(d) = mnpoles(m)
+      mkappa(:, :, d) = kk; mbeta(:, :, d) = bb; gm3(:, :, d) = gg
+      gm1(m) = a1(1); gm2(m) = a2(1)
+   end do
+   ! InitMURBorders (timestepping.F90:669): dual inverse spacings at the Alloc
+   ! ends after the CPML stretching
+   allocate (cab1(6 * (nmedia + 1)))
+   cab1 = 0
+   if (any(murf /= 0)) then
+      idhf = [idh(1)%v(-1), idh(1)%v(nx), idh(2)%v(-1), idh(2)%v(ny), idh(3)%v(-1), idh(3)%v(nz)]
+      call check_host(fdtd_build_mur(nmedia, epr, mur, dt, eps0, mu0, idhf, cab1), 'fdtd_build_mur')
+   end if
    allocate (tiempo(0:nsteps+2))
    tiempo(0) = 0.0d0
    do i = 1, nsteps + 2
@@ -154,6 +193,8 @@ program fdtd_gpu_driver
          write (u) ide(ax)%v, idh(ax)%v, pbe(ax)%v, pce(ax)%v, pbm(ax)%v, pcm(ax)%v
       end do
       write (u) nd, kappa, beta, g3
+      write (u) nmd, mkappa, mbeta, gm3
+      write (u) cab1
       close (u)
       stop
    end if
@@ -179,6 +220,11 @@ program fdtd_gpu_driver
    if (nd > 0) then
       call check(fdtd_set_edispersive(h, nd, dmed, dnp, kappa, beta, g3), 'fdtd_set_edispersive')
    end if
+   if (nmd > 0) then
+      call check(fdtd_set_mdispersive(h, nmd, mdmed, mdnp, mkappa, mbeta, gm3), &
+                 'fdtd_set_mdispersive')
+   end if
+   if (any(murf /= 0)) call check(fdtd_set_mur(h, murf, cab1), 'fdtd_set_mur')
    call check(fdtd_set_time(h, dt, nsteps + 2, tiempo), 'fdtd_set_time')
    read (u) nnodal
    do s = 1, nnodal
@@ -247,10 +293,14 @@ program fdtd_gpu_driver
             call check(fdtd_advance_h_yee(h), 'fdtd_advance_h_yee')
             ! AdvanceMagneticCPML (timestepping.F90:1535)
             if (has_cpml) call check(fdtd_advance_h_cpml(h), 'fdtd_advance_h_cpml')
+            ! AdvanceMDispersiveH (timestepping.F90:1586)
+            if (nmd > 0) call check(fdtd_advance_h_dispersive(h), 'fdtd_advance_h_dispersive')
             ! AdvancePlaneWaveH (timestepping.F90:1598)
             if (npw > 0) call check(fdtd_advance_planewave_h(h, n), 'fdtd_advance_planewave_h')
             ! AdvanceNodalH (timestepping.F90:1596-1635)
             call check(fdtd_advance_nodal_h(h, n), 'fdtd_advance_nodal_h')
+            ! AdvanceMagneticMUR (timestepping.F90:1711-1712)
+            if (any(murf /= 0)) call check(fdtd_advance_mur(h), 'fdtd_advance_mur')
          else
             call check(fdtd_advance_e(h, n), 'fdtd_advance_e')
             call check(fdtd_advance_h(h, n), 'fdtd_advance_h')

@@ -48,15 +48,24 @@ def _read_tables(path, pb):
     nd = int(take("<i4", 1)[0])
     m = max(nd, 1) * 8 * 2
     kappa, beta, g3 = (take("<f8", m).reshape(max(nd, 1), 8, 2) for _ in range(3))
+    nmd = int(take("<i4", 1)[0])
+    m = max(nmd, 1) * 8 * 2
+    mkappa, mbeta, gm3 = (take("<f8", m).reshape(max(nmd, 1), 8, 2) for _ in range(3))
+    cab1 = take("<f4", 6 * (nm + 1)).reshape(6, nm + 1)
     assert off == len(raw)
-    return g, axes, nd, kappa, beta, g3
+    return g, axes, (nd, kappa, beta, g3), (nmd, mkappa, mbeta, gm3), cab1
 
 
 @needs_driver
-@pytest.mark.parametrize("case", ["scatterer", "graded_gedney", "cavity"])
+@pytest.mark.parametrize("case", ["scatterer", "graded_gedney", "cavity", "magnetic_mur",
+                                  "x_only_cpml"])
 def test_driver_tables_equal_python_setup(tmp_path, case):
     if case == "scatterer":
         pb = P.scatterer_problem(nsteps=10)
+    elif case == "magnetic_mur":
+        pb = P.scatterer_problem(nsteps=10, mdisp=True, mur=(1, 1, 0, 1, 1, 0), graded_z=1.05)
+    elif case == "x_only_cpml":
+        pb = P.scatterer_problem(nsteps=10, L=((4, 5), (0, 0), (0, 0)), mur=(0, 0, 1, 1, 0, 0))
     elif case == "graded_gedney":
         pb = P.scatterer_problem(nsteps=10, n=(34, 33, 32), L=((10, 10), (10, 10), (10, 10)),
                                  alphamax=0.1, kappamax=3.0, order=2.0, graded_z=1.03)
@@ -66,7 +75,7 @@ def test_driver_tables_equal_python_setup(tmp_path, case):
     write_case(pb, cf, 10)
     r = _run(cf, out, "fast", "tables")
     assert r.returncode == 0, r.stderr
-    g, axes, nd, kappa, beta, g3 = _read_tables(out, pb)
+    g, axes, (nd, kappa, beta, g3), (nmd, mkappa, mbeta, gm3), cab1 = _read_tables(out, pb)
     t = sf.build_tables(pb)
     for got, want in zip(g, (t.g1, t.g2, t.gm1, t.gm2)):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
@@ -79,6 +88,13 @@ def test_driver_tables_equal_python_setup(tmp_path, case):
         assert np.array_equal(kappa, t.kappa[:nd])
         assert np.array_equal(beta, t.beta[:nd])
         assert np.array_equal(g3, t.g3[:nd])
+    assert nmd == len(t.mdisp_media)
+    if nmd:
+        assert np.array_equal(mkappa, t.mkappa[:nmd])
+        assert np.array_equal(mbeta, t.mbeta[:nmd])
+        assert np.array_equal(gm3, t.gm3[:nmd])
+    if any(pb.mur_faces):
+        assert np.array_equal(cab1.view(np.uint32), t.cab1.view(np.uint32))
 
 
 @needs_driver
@@ -107,6 +123,8 @@ GPU_CASES = {
         n=30, L=5, radius=4, nsteps=150, spread=8, delay=30, grading=1.05,
         direction=(0.3, -0.5, 0.8), polar=(0.0, 0.848, 0.53)), 150),
     "cavity64_hertzian_2000": lambda: (P.cavity_problem(n=64, nsteps=2000), 2000),
+    "magnetic_mur": lambda: (P.scatterer_problem(nsteps=150, L=((3, 4), (2, 3), (4, 2)),
+                                                 mur=(1, 1, 0, 1, 1, 0), mdisp=True), 150),
 }
 
 
