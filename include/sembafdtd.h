@@ -169,6 +169,20 @@ int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk, int32_
  * in the reference; used by the autotuner. */
 int fdtd_time_halves(fdtd_handle *h, int32_t reps, float ms[3]);
 
+/* One launch per step in fdtd_run: blocks of 64 x rows threads (rows 4 or 8;
+ * 0 turns it off) marching zchunk z planes compute E and then H of each
+ * plane, reading the state of step n from one copy and writing step n+1 to a
+ * second copy held for the purpose (fields, CPML psi and pole state: twice
+ * the state's memory).  fdtd_run advances pairs of steps so that the state is
+ * back in its usual place when it returns; an odd last step runs the two
+ * passes.  The results equal the two-pass step bit for bit.  fp32 storage
+ * only.  No counterpart in the reference. */
+int fdtd_set_step_mode(fdtd_handle *h, int32_t rows, int32_t zchunk);
+int fdtd_get_step_mode(const fdtd_handle *h, int32_t *rows, int32_t *zchunk);
+/* Average ms of one one-launch step (the step kernel alone), over reps pairs;
+ * the fields advance. */
+int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms);
+
 /* ---- the hot path ----------------------------------------------------- */
 /* Fused E half step: Advance_Ex/Ey/Ez (timestepping.F90:1286-1297) +
  * AdvanceelectricCPML (:1381) in one HBM pass, AdvanceEDispersiveE (:1416)

@@ -131,6 +131,20 @@ struct fdtd_handle {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   bool graph_ok = false;
+  // one-launch steps (k_step): a second copy of the time-stepping state (B),
+  // the A->B and B->A descriptors and a captured pair of steps
+  int step_rows = 0;              // 0: two passes per step; 4 or 8 rows per block
+  int step_kc = 32;
+  bool step_ready = false;
+  bool step_stale = true;         // B's fields must be refreshed from A
+  std::vector<void *> step_allocs;       // the second state
+  std::vector<void *> step_small;        // descriptors' tables
+  std::vector<size_t> step_sizes;        // layout of the second state
+  sfd::DevGrid gB;
+  sfd::StepDev sd[2];             // [0] A -> B, [1] B -> A
+  hipGraphExec_t pexec = nullptr;
+  hipGraph_t pgraph = nullptr;
+  std::vector<sfd::NodalCell> hcells[2];   // host copies of the nodal cells
   unsigned version = 0;           // bumped by every change that drops the graph
 };
 

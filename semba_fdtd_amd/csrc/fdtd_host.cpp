@@ -152,9 +152,14 @@ void invalidate_graph(fdtd_handle *h) {
   ++h->version;
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   if (h->graph) hipGraphDestroy(h->graph);
+  if (h->pexec) hipGraphExecDestroy(h->pexec);
+  if (h->pgraph) hipGraphDestroy(h->pgraph);
   h->gexec = nullptr;
   h->graph = nullptr;
+  h->pexec = nullptr;
+  h->pgraph = nullptr;
   h->graph_ok = false;
+  h->step_ready = false;
 }
 
 // ---- compact pole state --------------------------------------------------
@@ -348,6 +353,165 @@ int run_mur(fdtd_handle *h) {
   return FDTD_OK;
 }
 
+// ---- one launch per step --------------------------------------------------
+// k_step reads the state of step n from one copy and writes step n+1 to the
+// other; fdtd_run advances pairs of steps A -> B -> A, so between calls the
+// state is always in A and every other entry point sees it there.
+bool step_supported(const fdtd_handle *h, int rows) {
+  if (h->g.fp16 || h->nmedia + 1 > 4096) return false;
+  DevGrid g = h->g;
+  const size_t lds = h->mode == FDTD_MODE_EXACT ? exact::step_lds_bytes(g, rows)
+                                                : fast::step_lds_bytes(g, rows);
+  return lds <= 48 * 1024;
+}
+
+static int build_step(fdtd_handle *h) {
+  if (h->step_ready) return FDTD_OK;
+  const int R = h->step_rows;
+  if (!step_supported(h, R))
+    return fail(h, FDTD_ERR_STATE, "one-launch steps need fp32 storage and small media tables");
+  // layout of the state: fields, psi arrays, pole state of both fields
+  const size_t ne = field_elems(h);
+  std::vector<size_t> sizes(6, ne * 4);
+  for (int s = 0; s < 12; ++s) sizes.push_back(h->g.psi[s] ? (size_t)h->psilen[s] * 4 : 0);
+  for (int side = 0; side < 2; ++side)
+    for (int c = 0; c < 3; ++c) {
+      const AdeSet &a = h->g.ade[side];
+      sizes.push_back(a.prev[c] ? (size_t)std::max(a.nstate[c], 1LL) * 4 : 0);
+      sizes.push_back(a.jst[c] ? h->adein[side].jbytes[c] : 0);
+    }
+  int rc;
+  std::vector<void *> ptr(sizes.size(), nullptr);
+  if (sizes != h->step_sizes) {
+    for (void *p : h->step_allocs) { hipFree(p); }
+    h->step_allocs.clear();
+    h->step_sizes.clear();
+    for (size_t q = 0; q < sizes.size(); ++q) {
+      if (!sizes[q]) continue;
+      if ((rc = dalloc(h, &ptr[q], sizes[q], &h->step_allocs)) || (rc = dzero(h, ptr[q], sizes[q])))
+        return rc;
+    }
+    h->step_sizes = sizes;
+    h->step_stale = true;
+    h->gB = h->g;
+    for (int c = 0; c < 6; ++c) h->gB.f[c] = (float *)ptr[c];
+    for (int s = 0; s < 12; ++s) h->gB.psi[s] = (float *)ptr[6 + s];
+    for (int side = 0; side < 2; ++side)
+      for (int c = 0; c < 3; ++c) {
+        h->gB.ade[side].prev[c] = (float *)ptr[18 + side * 6 + c * 2];
+        h->gB.ade[side].jst[c] = ptr[18 + side * 6 + c * 2 + 1];
+      }
+  }
+  // everything but the state pointers follows A
+  DevGrid gb = h->g;
+  for (int c = 0; c < 6; ++c) gb.f[c] = h->gB.f[c];
+  for (int s = 0; s < 12; ++s) gb.psi[s] = h->gB.psi[s];
+  for (int side = 0; side < 2; ++side)
+    for (int c = 0; c < 3; ++c) {
+      gb.ade[side].prev[c] = h->gB.ade[side].prev[c];
+      gb.ade[side].jst[c] = h->gB.ade[side].jst[c];
+    }
+  h->gB = gb;
+  for (void *p : h->step_small) hipFree(p);
+  h->step_small.clear();
+  StepDev sd{};
+  TileMap &tm = sd.tm;
+  tm.bdy = R;
+  tm.vec = 1;
+  tm.tx = (h->g.nx + 1 + kWave - 2) / (kWave - 1);
+  tm.ty = (h->g.ny + 1 + R - 2) / (R - 1);
+  tm.kc = std::min(h->step_kc, h->g.nz + 1);
+  tm.tz = (h->g.nz + 1 + tm.kc - 1) / tm.kc;
+  const long long nblk = (long long)tm.tx * tm.ty * tm.tz;
+  if (nblk > 0x7fffffffLL) return fail(h, FDTD_ERR_ARG, "step tiling gives too many blocks");
+  tm.nblk = (int)nblk;
+  tm.remap = 1;
+  for (int half = 0; half < 2; ++half) {
+    const std::vector<PwFace> &F = h->pw_faces[half];
+    sd.npw[half] = (int)F.size();
+    PwFace *df = nullptr;
+    if (!F.empty() && (rc = upload(h, &df, F.data(), F.size(), &h->step_small))) return rc;
+    sd.pwf[half] = df;
+    for (int a = 0; a < 3; ++a) {
+      std::vector<int> v;
+      for (const PwFace &f : F)
+        if (f.axis == a && std::find(v.begin(), v.end(), f.fixed) == v.end()) v.push_back(f.fixed);
+      if ((int)v.size() > kStepPlanes) {
+        sd.npl[half][a] = -1;
+      } else {
+        sd.npl[half][a] = (int)v.size();
+        for (size_t q = 0; q < v.size(); ++q) sd.pl[half][a][q] = v[q];
+      }
+    }
+    const std::vector<NodalCell> &C = h->hcells[half];
+    sd.ncell[half] = (int)C.size();
+    sd.cells[half] = h->dcells[half];
+    int *b = sd.cellbox[half];
+    b[0] = b[1] = b[2] = 0x7fffffff;
+    b[3] = b[4] = b[5] = -0x7fffffff;
+    for (const NodalCell &c : C) {
+      const long long k = c.p / h->g.pxy - 1, r = c.p % h->g.pxy;
+      const long long j = r / h->g.px - 1, i = r % h->g.px - kXOff;
+      const long long q[3] = {i, j, k};
+      for (int a = 0; a < 3; ++a) {
+        b[a] = (int)std::min<long long>(b[a], q[a]);
+        b[3 + a] = (int)std::max<long long>(b[3 + a], q[a]);
+      }
+    }
+  }
+  sd.waves = h->dwaves;
+  sd.phys = h->dphys;
+  sd.pstride = h->pstride;
+  sd.tiempo = h->dtiempo;
+  sd.dt = h->dt;
+  sd.pwG[0] = h->nmedia >= 1 ? h->g2h[1] : 0.0f;
+  sd.pwG[1] = h->nmedia >= 1 ? h->gm2h[1] : 0.0f;
+  sd.values = h->dvalues;
+  sd.nsrc = (int)h->srcs.size();
+  sd.dn = h->dstep;
+  h->sd[0] = sd;
+  h->sd[0].gi = h->g;
+  h->sd[0].go = h->gB;
+  h->sd[1] = sd;
+  h->sd[1].gi = h->gB;
+  h->sd[1].go = h->g;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->step_ready = true;
+  return FDTD_OK;
+}
+
+// cells outside the sweeps are never written by a step, so B's must equal
+// A's: refreshed after the fields were set from outside
+static int sync_b(fdtd_handle *h) {
+  if (!h->step_stale) return FDTD_OK;
+  const size_t ne = field_elems(h);
+  for (int c = 0; c < 6; ++c)
+    HIPCHK(h, hipMemcpyAsync(h->gB.f[c], h->fld[c], ne * 4, hipMemcpyDeviceToDevice, h->stream));
+  h->step_stale = false;
+  return FDTD_OK;
+}
+
+// one step of the pair, from sd[dir].gi to sd[dir].go, then the H-side
+// boundary, observation and step counter on the written copy
+static int run_step(fdtd_handle *h, int dir) {
+  const StepDev &sd = h->sd[dir];
+  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_step(sd, h->step_rows, h->stream)
+                                     : fast::launch_step(sd, h->step_rows, h->stream);
+  if (e) return fail(h, FDTD_ERR_HIP, "step launch failed: %s", hipGetErrorString((hipError_t)e));
+  if (!h->mur_runs.empty() &&
+      launch_mur(sd.go, h->dmur, (int)h->mur_runs.size(), h->mur_total,
+                 h->mode == FDTD_MODE_EXACT, h->stream))
+    return fail(h, FDTD_ERR_HIP, "Mur launch failed");
+  if (h->nprobe &&
+      launch_probes(sd.go, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap, h->stream))
+    return fail(h, FDTD_ERR_HIP, "probe launch failed");
+  for (const DftDev &d : h->dfts)
+    if (launch_dft(sd.go, d, 0, h->dstep, h->stream))
+      return fail(h, FDTD_ERR_HIP, "DFT launch failed");
+  if (launch_tick(h->dstep, h->stream)) return fail(h, FDTD_ERR_HIP, "tick failed");
+  return FDTD_OK;
+}
+
 // ---- nodal sources --------------------------------------------------------
 int build_nodal(fdtd_handle *h) {
   if (h->nodal_ready) return FDTD_OK;
@@ -422,6 +586,7 @@ int build_nodal(fdtd_handle *h) {
     std::sort(key.begin(), key.end());
     h->serial[half] = std::adjacent_find(key.begin(), key.end()) != key.end();
     h->dcells[half] = nullptr;
+    h->hcells[half] = cells[half];
     if (!cells[half].empty()) {
       if ((rc = dalloc(h, (void **)&h->dcells[half], cells[half].size() * sizeof(NodalCell),
                        &h->nodal_allocs))) return rc;
@@ -828,6 +993,8 @@ int fdtd_destroy(fdtd_handle *h) {
   for (void *p : h->ade_allocs) hipFree(p);
   for (void *p : h->nodal_allocs) hipFree(p);
   for (void *p : h->mur_allocs) hipFree(p);
+  for (void *p : h->step_allocs) hipFree(p);
+  for (void *p : h->step_small) hipFree(p);
   for (void *p : h->pw_allocs) hipFree(p);
   for (void *p : h->dft_allocs) hipFree(p);
   for (void *p : h->dft_sums) hipFree(p);
@@ -1278,8 +1445,31 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
   // the step counter lives on the device so one captured step replays for
   // every n
   HIPCHK(h, hipMemcpyAsync(h->dstep, &n0, 4, hipMemcpyHostToDevice, h->stream));
+  // one-launch steps go in pairs (A -> B -> A); an odd last step, and every
+  // step when they are off, replays the two-pass step
+  const bool paired = h->step_rows > 0 && nsteps >= 2;
+  const int pairs = paired ? nsteps / 2 : 0;
+  const int singles = paired ? nsteps % 2 : nsteps;
+  if (pairs > 0 && !h->pexec) {
+    if ((rc = build_step(h))) return rc;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int r2 = run_step(h, 0);
+    if (!r2) r2 = run_step(h, 1);
+    hipGraph_t graph = nullptr;
+    hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+    if (r2) { if (graph) hipGraphDestroy(graph); return r2; }
+    if (ec != hipSuccess) return fail(h, FDTD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+    h->pgraph = graph;
+    HIPCHK(h, hipGraphInstantiate(&h->pexec, graph, nullptr, nullptr, 0));
+  }
+  if (pairs > 0 && (rc = sync_b(h))) return rc;
+  for (int32_t s = 0; s < pairs; ++s) HIPCHK(h, hipGraphLaunch(h->pexec, h->stream));
+  if (singles == 0) return FDTD_OK;
   if (!h->graph_ok) {
-    invalidate_graph(h);
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int r2 = run_e(h, 1, 1, 1);
     if (!r2) r2 = run_pw(h, 0, 0, h->dstep);
@@ -1301,8 +1491,68 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
     HIPCHK(h, hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0));
     h->graph_ok = true;
   }
-  for (int32_t s = 0; s < nsteps; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+  for (int32_t s = 0; s < singles; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
   return FDTD_OK;
+}
+
+int fdtd_set_step_mode(fdtd_handle *h, int32_t rows, int32_t zchunk) {
+  if (!h || !(rows == 0 || rows == 4 || rows == 8) || zchunk < 1)
+    return fail(h, FDTD_ERR_ARG, "step mode %d rows x %d planes not supported", rows, zchunk);
+  if (rows && !step_supported(h, rows))
+    return fail(h, FDTD_ERR_STATE, "one-launch steps need fp32 storage and small media tables");
+  h->step_rows = rows;
+  h->step_kc = zchunk;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_get_step_mode(const fdtd_handle *h, int32_t *rows, int32_t *zchunk) {
+  if (!h || !rows || !zchunk) return FDTD_ERR_ARG;
+  *rows = h->step_rows;
+  *zchunk = h->step_kc;
+  return FDTD_OK;
+}
+
+int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms) {
+  if (!h || reps < 1 || !ms) return FDTD_ERR_ARG;
+  if (h->step_rows == 0) return fail(h, FDTD_ERR_STATE, "one-launch steps are off");
+  hipSetDevice(h->device);
+  int rc;
+  if ((rc = prepare(h)) || (rc = build_step(h)) || (rc = sync_b(h))) return rc;
+  hipEvent_t ev[2];
+  if (hipEventCreate(&ev[0]) != hipSuccess) return fail(h, FDTD_ERR_HIP, "hipEventCreate failed");
+  if (hipEventCreate(&ev[1]) != hipSuccess) {
+    hipEventDestroy(ev[0]);
+    return fail(h, FDTD_ERR_HIP, "hipEventCreate failed");
+  }
+  // the step kernels alone, in pairs so that the state ends in A
+  auto pair = [&]() -> int {
+    for (int dir = 0; dir < 2; ++dir) {
+      const int e = h->mode == FDTD_MODE_EXACT
+                        ? exact::launch_step(h->sd[dir], h->step_rows, h->stream)
+                        : fast::launch_step(h->sd[dir], h->step_rows, h->stream);
+      if (e) return fail(h, FDTD_ERR_HIP, "step launch failed: %s", hipGetErrorString((hipError_t)e));
+    }
+    return FDTD_OK;
+  };
+  rc = pair();
+  if (!rc) {
+    hipEventRecord(ev[0], h->stream);
+    for (int r = 0; r < reps && !rc; ++r) rc = pair();
+    hipEventRecord(ev[1], h->stream);
+  }
+  if (!rc) {
+    if (hipEventSynchronize(ev[1]) != hipSuccess) {
+      rc = fail(h, FDTD_ERR_HIP, "timing events failed");
+    } else {
+      float t = 0.f;
+      hipEventElapsedTime(&t, ev[0], ev[1]);
+      *ms = t / (2.0f * reps);
+    }
+  }
+  hipEventDestroy(ev[0]);
+  hipEventDestroy(ev[1]);
+  return rc;
 }
 
 int fdtd_set_probes(fdtd_handle *h, int32_t nprobe, const int32_t *comp, const int32_t *i,
@@ -1366,6 +1616,7 @@ int fdtd_get_field(fdtd_handle *h, int32_t comp, float *host) {
 }
 
 int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host) {
+  if (h) h->step_stale = true;
   if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field write");
   hipSetDevice(h->device);
   return field_io(h, comp, host, nullptr);
@@ -1565,6 +1816,7 @@ int fdtd_save_state(fdtd_handle *h, int32_t n, void *buf, int64_t size) {
 }
 
 int fdtd_load_state(fdtd_handle *h, const void *buf, int64_t size, int32_t *n) {
+  if (h) h->step_stale = true;
   if (!h || !buf || !n) return fail(h, FDTD_ERR_ARG, "null state buffer");
   const int64_t need = fdtd_state_size(h);
   if (need < 0) return FDTD_ERR_STATE;

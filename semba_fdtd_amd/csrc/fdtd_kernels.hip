@@ -1086,7 +1086,9 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
   const int nx = gi.nx, ny = gi.ny, nz = gi.nz;
   const bool in = i <= nx && j <= ny;
   const bool own = in && lane < kWave - 1 && row < R - 1;
-  const int ic = min(i, nx + 1), jc = min(j, ny + 1);
+  // threads past the grid's edge read the edge cell (their results are
+  // never used): every load and table index stays inside its array
+  const int ic = min(i, nx), jc = min(j, ny);
   const int k0 = bz * sd.tm.kc;
   const int k1 = min(k0 + sd.tm.kc, nz + 1);
   const long long px = gi.px, pxy = gi.pxy;

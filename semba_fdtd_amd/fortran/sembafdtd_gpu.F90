@@ -43,7 +43,8 @@ module sembafdtd_gpu
              fdtd_set_field_scale, fdtd_state_size, fdtd_save_state, fdtd_load_state, &
              fdtd_batch_create, fdtd_batch_destroy, fdtd_batch_admit, fdtd_batch_retire, &
              fdtd_batch_step_of, fdtd_batch_set_tiling, fdtd_batch_step, &
-             fdtd_batch_synchronize, fdtd_error_message
+             fdtd_batch_synchronize, fdtd_set_step_mode, fdtd_get_step_mode, fdtd_time_step, &
+             fdtd_error_message
 
    interface
       integer(c_int) function fdtd_create(desc, h) bind(C, name="fdtd_create")
@@ -374,6 +375,23 @@ module sembafdtd_gpu
          type(c_ptr), value :: h
          integer(c_int32_t), value :: reps
          real(c_float), intent(out) :: ms(3)
+      end function
+      ! ---- one launch per step in fdtd_run
+      integer(c_int) function fdtd_set_step_mode(h, rows, zchunk) bind(C, name="fdtd_set_step_mode")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: rows, zchunk
+      end function
+      integer(c_int) function fdtd_get_step_mode(h, rows, zchunk) bind(C, name="fdtd_get_step_mode")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(out) :: rows, zchunk
+      end function
+      integer(c_int) function fdtd_time_step(h, reps, ms) bind(C, name="fdtd_time_step")
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: reps
+         real(c_float), intent(out) :: ms
       end function
       ! ---- frequency-domain observation (observation.F90:3527-3600)
       integer(c_int) function fdtd_add_dft(h, kind, box, stride, nfreq, freq, id) &

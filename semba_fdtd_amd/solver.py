@@ -408,6 +408,23 @@ class Solver:
                  "fdtd_get_tiling")
         return r.value, k.value, x.value
 
+    def set_step_mode(self, rows: int, zchunk: int = 32):
+        """One launch per step in run() (rows 4 or 8 per block, 0: two
+        passes per step).  Needs fp32 storage; holds a second copy of the
+        time-stepping state."""
+        self._ck(self.lib.fdtd_set_step_mode(self.h, rows, zchunk), "fdtd_set_step_mode")
+
+    def get_step_mode(self):
+        r, k = C.c_int32(), C.c_int32()
+        self._ck(self.lib.fdtd_get_step_mode(self.h, C.byref(r), C.byref(k)), "fdtd_get_step_mode")
+        return r.value, k.value
+
+    def time_step(self, reps: int = 3) -> float:
+        """Average ms of one one-launch step.  Advances the fields."""
+        ms = np.zeros(1, np.float32)
+        self._ck(self.lib.fdtd_time_step(self.h, reps, ptr(ms, f32p)), "fdtd_time_step")
+        return float(ms[0])
+
     def time_halves(self, reps: int = 5):
         """Average ms of the fused E pass (with poles), the fused H pass and
         the pole update alone.  Advances the fields."""

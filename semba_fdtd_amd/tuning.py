@@ -1,12 +1,18 @@
-"""Autotuner for the face-pass work decomposition of one grid.
+"""Autotuner for the work decomposition of one grid.
 
-The fused E and H passes are tiled as blocks of 64 x rows threads that march
-zchunk z planes, each lane owning xcells consecutive x cells.  The best choice
-depends on the grid's shape (ragged x edges, thin z), on binary16 storage and
-on the share of dispersive cells, so ``autotune`` times each candidate on the
-grid itself with device events (``fdtd_time_halves``) and keeps the fastest.
-The grid's state is checkpointed before the sweep and restored after it, so
-tuning leaves the fields, CPML and pole state as they were.
+Two ways to advance a step are timed on the grid itself with device events:
+
+* the two passes (E, then H), tiled as blocks of 64 x rows threads marching
+  zchunk z planes, each lane owning xcells consecutive x cells
+  (``fdtd_time_halves``);
+* one launch per step (``fdtd_set_step_mode``, ``fdtd_time_step``), blocks of
+  64 x rows threads marching zchunk planes, which reads every field once and
+  writes it once per step but keeps a second copy of the state.
+
+The best choice depends on the grid's shape (ragged x edges, thin z), on
+binary16 storage and on the share of dispersive cells, so ``autotune`` keeps
+the fastest.  The grid's state is checkpointed before the sweep and restored
+after it, so tuning leaves the fields, CPML and pole state as they were.
 
 Results are cached per (shape, mode, storage) for the life of the process, and
 optionally in a JSON file that later processes read.
@@ -17,18 +23,21 @@ import json
 import os
 from typing import Dict, Iterable, Optional, Tuple
 
+from ._lib import FdtdError
+
 Tiling = Tuple[int, int, int]          # rows, zchunk, xcells
+StepShape = Tuple[int, int]            # rows, zchunk (rows 0: two passes)
 
 DEFAULT_CANDIDATES: Tuple[Tiling, ...] = tuple(
     (r, k, x) for x in (1, 4) for r in (1, 2, 4) for k in (16, 32, 64))
+STEP_CANDIDATES: Tuple[StepShape, ...] = ((4, 16), (4, 32), (8, 16), (8, 32), (8, 64))
 
-_cache: Dict[str, Tiling] = {}
+_cache: Dict[str, tuple] = {}
 
 
 def _key(solver) -> str:
     nx, ny, nz = solver.n3
-    fp16 = int(getattr(solver, "storage", "fp32") == "fp16")
-    return f"{nx}x{ny}x{nz}:{solver.mode}:{'fp16' if fp16 else 'fp32'}"
+    return f"{nx}x{ny}x{nz}:{solver.mode}:{getattr(solver, 'storage', 'fp32')}"
 
 
 def _load(path: Optional[str]) -> None:
@@ -46,27 +55,37 @@ def _save(path: Optional[str]) -> None:
         os.replace(tmp, path)
 
 
-def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: int = 4,
-             cache: Optional[str] = None, verbose: bool = False,
-             keep_state: bool = True) -> dict:
-    """Time every candidate tiling on ``solver``'s grid, set the fastest and
-    return {"tiling": (rows, zchunk, xcells), "ms": E+H ms, "table": [...]}.
+def _apply(solver, choice: tuple) -> None:
+    rows, zchunk, xcells, srows, skc = choice
+    solver.set_tiling(rows, zchunk, xcells)
+    solver.set_step_mode(srows, skc)
 
-    Binary16 storage always runs four cells per lane, so its candidates with
-    one cell per lane are skipped.  ``keep_state=False`` skips the checkpoint
-    around the sweep: right for a grid at rest (all fields and pole state
-    zero, which the source-free passes leave zero), where copying the state
-    out and back would cost more than the sweep."""
+
+def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: int = 4,
+             cache: Optional[str] = None, verbose: bool = False, keep_state: bool = True,
+             steps: Iterable[StepShape] = STEP_CANDIDATES) -> dict:
+    """Time every candidate on ``solver``'s grid and set the fastest.
+
+    Returns {"tiling": (rows, zchunk, xcells), "step": (rows, zchunk), "ms":
+    ms per step of the choice, "table": [...], "step_table": [...]}; step
+    rows 0 means the two passes won.  Binary16 storage always runs four cells
+    per lane and has no one-launch step.  ``keep_state=False`` skips the
+    checkpoint around the sweep: right for a grid at rest (all fields and pole
+    state zero, which the source-free passes leave zero), where copying the
+    state out and back would cost more than the sweep."""
     _load(cache)
     key = _key(solver)
     if key in _cache:
-        solver.set_tiling(*_cache[key])
-        return {"tiling": _cache[key], "ms": None, "table": [], "cached": True}
+        _apply(solver, _cache[key])
+        c = _cache[key]
+        return {"tiling": c[:3], "step": c[3:], "ms": None, "table": [], "step_table": [],
+                "cached": True}
     fp16 = getattr(solver, "storage", "fp32") == "fp16"
     state = solver.save_state() if keep_state else None
-    before = solver.get_tiling()
-    table = []
+    before = solver.get_tiling() + solver.get_step_mode()
+    table, step_table = [], []
     try:
+        solver.set_step_mode(0, 32)
         for rows, zchunk, xcells in candidates:
             if fp16 and xcells != 4:
                 continue
@@ -76,14 +95,34 @@ def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: in
             if verbose:
                 print(f"tiling rows={rows} zchunk={zchunk} xcells={xcells}: "
                       f"E {e:.3f} ms (poles {ade:.3f}) H {h:.3f} ms", flush=True)
+        if not fp16:
+            for rows, zchunk in steps:
+                try:
+                    solver.set_step_mode(rows, zchunk)
+                except FdtdError:
+                    continue
+                ms = solver.time_step(reps)
+                step_table.append(((rows, zchunk), ms))
+                if verbose:
+                    print(f"one launch per step rows={rows} zchunk={zchunk}: {ms:.3f} ms",
+                          flush=True)
     finally:
         if state is not None:
             solver.load_state(state)
     if not table:
-        solver.set_tiling(*before)
-        return {"tiling": before, "ms": None, "table": [], "cached": False}
+        _apply(solver, before)
+        return {"tiling": before[:3], "step": before[3:], "ms": None, "table": [],
+                "step_table": step_table, "cached": False}
     best = min(table, key=lambda r: r[1] + r[2])
-    solver.set_tiling(*best[0])
-    _cache[key] = best[0]
+    ms = best[1] + best[2]
+    step = (0, 32)
+    if step_table:
+        bs = min(step_table, key=lambda r: r[1])
+        if bs[1] < ms:
+            step, ms = bs[0], bs[1]
+    choice = tuple(best[0]) + tuple(step)
+    _apply(solver, choice)
+    _cache[key] = choice
     _save(cache)
-    return {"tiling": best[0], "ms": best[1] + best[2], "table": table, "cached": False}
+    return {"tiling": best[0], "step": step, "ms": ms, "table": table,
+            "step_table": step_table, "cached": False}

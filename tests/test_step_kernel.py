@@ -1,0 +1,138 @@
+"""One launch per step (k_step): E then H of each plane in one z march, the
+state of step n read from one copy and step n+1 written to the other.  Per
+cell the arithmetic is that of the two passes, so EXACT mode equals the
+oracle bit for bit and FAST mode equals the two-pass FAST run bit for bit,
+for every block shape, with CPML, electric and magnetic poles, Mur faces,
+plane waves, nodal sources and frequency-domain sums, for even and odd step
+counts."""
+import numpy as np
+import pytest
+
+import oracle
+import semba_fdtd_amd as sf
+from semba_fdtd_amd import problem as P
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "cavity_hard": lambda: P.cavity_problem(n=24, nsteps=120),
+    "scatterer": lambda: P.scatterer_problem(nsteps=120),
+    "graded_one_sided": lambda: P.scatterer_problem(nsteps=100, n=(70, 24, 22),
+                                                    L=((5, 0), (0, 6), (4, 4)), graded_z=1.03),
+    "wide_x": lambda: P.scatterer_problem(nsteps=80, n=(131, 9, 11),
+                                          L=((6, 7), (2, 2), (3, 2))),
+    "planewave_oblique": lambda: P.open_scatterer_problem(
+        n=30, L=5, radius=4, nsteps=101, spread=8, delay=30, grading=1.05,
+        direction=(0.3, -0.5, 0.8), polar=(0.0, 0.848, 0.53)),
+    "magnetic_mur": lambda: P.scatterer_problem(nsteps=101, L=((3, 4), (2, 3), (4, 2)),
+                                                mur=(1, 1, 0, 1, 1, 0), mdisp=True),
+    "mur_walls": lambda: P.scatterer_problem(nsteps=90, L=((0, 0), (0, 0), (0, 0)),
+                                             mur=(1, 1, 1, 1, 1, 1)),
+}
+SHAPES = [(4, 32), (8, 7), (4, 3), (8, 64)]
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_step_exact_equals_oracle(name):
+    pb = CASES[name]()
+    ns = pb.nsteps_max
+    ref = oracle.Oracle(pb)
+    want = ref.run(ns)
+    for rows, kc in SHAPES:
+        print(f"step case {name} rows {rows} zchunk {kc}", flush=True)
+        s = sf.Solver(pb, mode="exact")
+        s.set_step_mode(rows, kc)
+        assert s.get_step_mode() == (rows, kc)
+        s.run(ns)
+        got = s.probes()
+        assert np.array_equal(_bits(got), _bits(want)), (rows, kc)
+        for c in range(6):
+            assert np.array_equal(_bits(s.field(c)), _bits(ref.fields[c])), (rows, kc, c)
+        s.close()
+
+
+@pytest.mark.parametrize("name", ["scatterer", "planewave_oblique", "magnetic_mur"])
+def test_step_fast_equals_two_pass(name):
+    pb = CASES[name]()
+    ns = pb.nsteps_max
+    a = sf.Solver(pb, mode="fast")
+    a.run(ns)
+    for rows, kc in SHAPES[:2]:
+        print(f"step fast {name} rows {rows} zchunk {kc}", flush=True)
+        b = sf.Solver(pb, mode="fast")
+        b.set_step_mode(rows, kc)
+        b.run(ns)
+        assert np.array_equal(_bits(b.probes()), _bits(a.probes())), (rows, kc)
+        for c in range(6):
+            assert np.array_equal(_bits(b.field(c)), _bits(a.field(c))), (rows, kc, c)
+        b.close()
+
+
+def test_step_with_dft_and_chunks():
+    """Frequency-domain sums and probes across run() calls of odd and even
+    lengths equal the two-pass run."""
+    pb = CASES["planewave_oblique"]()
+    pb.dfts = [P.DftBox(0, (8, 20, 9, 18, 10, 12), (2, 1, 1),
+                        np.array([0.5e9, 2e9, 3.5e9], np.float32)),
+               P.DftBox(1, (6, 16, 6, 14, 8, 10), (2, 2, 1), np.array([1e9], np.float32))]
+    ns = 100
+    a = sf.Solver(pb, mode="exact")
+    a.run(ns)
+    b = sf.Solver(pb, mode="exact")
+    b.set_step_mode(8, 16)
+    for k in (7, 30, 1, 2, 60):
+        b.run(k)
+    assert b.n == ns
+    assert np.array_equal(_bits(b.probes()), _bits(a.probes()))
+    for x, y in zip(b.dft_sums(), a.dft_sums()):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+
+
+def test_step_after_set_field():
+    """Fields set from outside refresh the second copy: a grid stepped in
+    step mode, then given new fields, follows the two-pass run from them
+    (a lossless cavity: no psi, pole or Mur state carries over)."""
+    pb = CASES["cavity_hard"]()
+    ns = 60
+    rng = np.random.default_rng(3)
+    init = [rng.standard_normal(pb.alloc_shape).astype(np.float32) * 0.01 for _ in range(6)]
+    a = sf.Solver(pb, mode="exact")
+    for c in range(6):
+        a.set_field(c, init[c])
+    a.run(ns)
+    b = sf.Solver(pb, mode="exact")
+    b.set_step_mode(4, 16)
+    b.run(10)                       # the second copy now holds other values
+    head = b.probes().shape[0]
+    for c in range(6):
+        b.set_field(c, init[c])
+    b.n = 0                         # the same steps again from the set fields
+    b.run(ns)
+    assert np.array_equal(_bits(b.probes()[head:]), _bits(a.probes()))
+    for c in range(6):
+        assert np.array_equal(_bits(b.field(c)), _bits(a.field(c))), c
+
+
+def test_step_checkpoint_resume():
+    """A run checkpointed after 41 step-mode steps and resumed in step mode
+    equals the uninterrupted two-pass run (psi, poles, Mur planes)."""
+    pb = CASES["magnetic_mur"]()
+    ns = 90
+    a = sf.Solver(pb, mode="exact")
+    a.run(ns)
+    b = sf.Solver(pb, mode="exact")
+    b.set_step_mode(4, 16)
+    b.run(41)
+    state = b.save_state()
+    head = b.probes()
+    c3 = sf.Solver(pb, mode="exact")
+    c3.set_step_mode(8, 32)
+    c3.load_state(state)
+    c3.run(ns - 41)
+    for c in range(6):
+        assert np.array_equal(_bits(c3.field(c)), _bits(a.field(c))), c
+    assert np.array_equal(_bits(np.concatenate([head, c3.probes()])), _bits(a.probes()))
