@@ -169,8 +169,8 @@ int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk, int32_
  * in the reference; used by the autotuner. */
 int fdtd_time_halves(fdtd_handle *h, int32_t reps, float ms[3]);
 
-/* One launch per step in fdtd_run: blocks of 64 x rows threads (rows 4 or 8;
- * 0 turns it off) marching zchunk z planes compute E and then H of each
+/* One launch per step in fdtd_run: blocks of 64 x rows threads (rows 4, 8 or
+ * 16; 0 turns it off) marching zchunk z planes compute E and then H of each
  * plane, reading the state of step n from one copy and writing step n+1 to a
  * second copy held for the purpose (fields, CPML psi and pole state: twice
  * the state's memory).  fdtd_run advances pairs of steps so that the state is

@@ -4,6 +4,8 @@
 // Built with -ffp-contract=off: the soft-source update keeps the reference's
 // rounding order.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+
 #include "fdtd_internal.h"
 
 namespace sfd {
@@ -86,6 +88,39 @@ int launch_tfsf_list(const PwRun *runs, const long long *start, int nruns, long 
   const unsigned blocks = (unsigned)((total + 255) / 256);
   hipLaunchKernelGGL(k_tfsf_list, dim3(blocks), dim3(256), 0, (hipStream_t)stream, runs, start,
                      nruns, total, n);
+  return (int)hipGetLastError();
+}
+
+// Incid of every TF/SF face cell for the one-launch step (the same
+// arithmetic as k_tfsf_list, which applies it in place)
+__global__ void __launch_bounds__(256) k_pw_incident(const PwFace *faces, int nf0,
+                                                     const StepSrc *sp, const int *dn,
+                                                     float *inc) {
+  const int q = blockIdx.y;
+  const PwFace &f = faces[q];
+  const int half = q >= nf0 ? 1 : 0;
+  const int ua = f.axis == 0 ? 1 : 0;
+  const int va = f.axis == 2 ? 1 : 2;
+  const long long nu = f.hi[ua] - f.lo[ua] + 1, nv = f.hi[va] - f.lo[va] + 1;
+  const StepSrc &S = *sp;
+  const int n = *dn;
+  const float tm = half ? (float)(S.tiempo[n] + (double)0.5f * S.dt) : (float)S.tiempo[n];
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < nu * nv;
+       c += (long long)gridDim.x * blockDim.x) {
+    int cc[3];
+    cc[f.axis] = f.fixed;
+    cc[ua] = f.lo[ua] + (int)(c % nu);
+    cc[va] = f.lo[va] + (int)(c / nu);
+    inc[f.inc + c] = pw_incid(f, S.waves, S.phys, S.pstride, tm, cc);
+  }
+}
+
+int launch_pw_incident(const PwFace *faces, int nf0, int nf, long long maxcells,
+                       const StepSrc *src, const int *dn, float *inc, void *stream) {
+  if (nf <= 0 || maxcells <= 0) return 0;
+  const long long b = std::min<long long>((maxcells + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_pw_incident, dim3((unsigned)b, (unsigned)nf), dim3(256), 0,
+                     (hipStream_t)stream, faces, nf0, src, dn, inc);
   return (int)hipGetLastError();
 }
 

@@ -142,6 +142,17 @@ struct fdtd_handle {
   std::vector<size_t> step_sizes;        // layout of the second state
   sfd::DevGrid gB;
   sfd::StepDev sd[2];             // [0] A -> B, [1] B -> A
+  // tiles grouped by class (bit 0 CPML, 1 poles, 2 sources): one launch per
+  // class with tiles, each over its slice of the device tile list
+  std::vector<int> step_cls, step_cnt, step_off;
+  int *dtiles = nullptr;
+  // TF/SF faces of both halves for the step kernels and the buffer of their
+  // incident field, refreshed before every step
+  sfd::PwFace *dsfaces = nullptr;
+  int step_nf0 = 0, step_nf = 0;
+  long long step_maxcells = 0;
+  float *dinc = nullptr;
+  sfd::StepSrc *dstepsrc = nullptr;
   hipGraphExec_t pexec = nullptr;
   hipGraph_t pgraph = nullptr;
   std::vector<sfd::NodalCell> hcells[2];   // host copies of the nodal cells

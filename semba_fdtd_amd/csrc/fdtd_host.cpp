@@ -236,6 +236,7 @@ static int build_ade_side(fdtd_handle *h, int side) {
   ad.ndisp = in.ndisp;
   int maxp = 0;
   for (int d = 0; d < in.ndisp; ++d) maxp = std::max(maxp, anp[d]);
+  ad.maxp = std::max(maxp, 1);
   // dense state box per component over its dispersive edges (the bounding
   // box kernel reads the slot table through the grid descriptor)
   int32_t *bb = nullptr;
@@ -358,10 +359,15 @@ int run_mur(fdtd_handle *h) {
 // other; fdtd_run advances pairs of steps A -> B -> A, so between calls the
 // state is always in A and every other entry point sees it there.
 bool step_supported(const fdtd_handle *h, int rows) {
-  if (h->g.fp16 || h->nmedia + 1 > 4096) return false;
+  // lanes address a plane by a 32-bit byte offset
+  if (h->g.fp16 || h->nmedia + 1 > 4096 || h->g.pxy * 4 >= (1LL << 31)) return false;
+  if ((int)h->pw_faces[0].size() > kStepFaces || (int)h->pw_faces[1].size() > kStepFaces)
+    return false;
   DevGrid g = h->g;
-  const size_t lds = h->mode == FDTD_MODE_EXACT ? exact::step_lds_bytes(g, rows)
-                                                : fast::step_lds_bytes(g, rows);
+  const int npw = (int)(h->pw_faces[0].size() + h->pw_faces[1].size());
+  const int kc = std::min(h->step_kc, h->g.nz + 1);
+  const size_t lds = h->mode == FDTD_MODE_EXACT ? exact::step_lds_bytes(g, rows, npw, kc)
+                                                : fast::step_lds_bytes(g, rows, npw, kc);
   return lds <= 48 * 1024;
 }
 
@@ -369,7 +375,7 @@ static int build_step(fdtd_handle *h) {
   if (h->step_ready) return FDTD_OK;
   const int R = h->step_rows;
   if (!step_supported(h, R))
-    return fail(h, FDTD_ERR_STATE, "one-launch steps need fp32 storage and small media tables");
+    return fail(h, FDTD_ERR_STATE, "one-launch steps need fp32 storage, small media tables and at most 64 TF/SF faces per half");
   // layout of the state: fields, psi arrays, pole state of both fields
   const size_t ne = field_elems(h);
   std::vector<size_t> sizes(6, ne * 4);
@@ -426,26 +432,40 @@ static int build_step(fdtd_handle *h) {
   if (nblk > 0x7fffffffLL) return fail(h, FDTD_ERR_ARG, "step tiling gives too many blocks");
   tm.nblk = (int)nblk;
   tm.remap = 1;
+  StepSrc src{};
+  // the faces of both halves in one list, each with its slice of the
+  // incident-field buffer
+  {
+    std::vector<PwFace> all;
+    long long cells = 0, maxc = 0;
+    for (int half = 0; half < 2; ++half)
+      for (PwFace f : h->pw_faces[half]) {
+        const int ua = f.axis == 0 ? 1 : 0, va = f.axis == 2 ? 1 : 2;
+        const long long nc = (long long)(f.hi[ua] - f.lo[ua] + 1) * (f.hi[va] - f.lo[va] + 1);
+        f.inc = cells;
+        cells += std::max(nc, 0LL);
+        maxc = std::max(maxc, nc);
+        all.push_back(f);
+      }
+    h->step_nf0 = (int)h->pw_faces[0].size();
+    h->step_nf = (int)all.size();
+    h->step_maxcells = maxc;
+    h->dsfaces = nullptr;
+    h->dinc = nullptr;
+    if (!all.empty()) {
+      if ((rc = upload(h, &h->dsfaces, all.data(), all.size(), &h->step_small))) return rc;
+      if ((rc = dalloc(h, (void **)&h->dinc, (size_t)std::max(cells, 1LL) * 4, &h->step_small)))
+        return rc;
+    }
+    sd.inc = h->dinc;
+  }
   for (int half = 0; half < 2; ++half) {
     const std::vector<PwFace> &F = h->pw_faces[half];
-    sd.npw[half] = (int)F.size();
-    PwFace *df = nullptr;
-    if (!F.empty() && (rc = upload(h, &df, F.data(), F.size(), &h->step_small))) return rc;
-    sd.pwf[half] = df;
-    for (int a = 0; a < 3; ++a) {
-      std::vector<int> v;
-      for (const PwFace &f : F)
-        if (f.axis == a && std::find(v.begin(), v.end(), f.fixed) == v.end()) v.push_back(f.fixed);
-      if ((int)v.size() > kStepPlanes) {
-        sd.npl[half][a] = -1;
-      } else {
-        sd.npl[half][a] = (int)v.size();
-        for (size_t q = 0; q < v.size(); ++q) sd.pl[half][a][q] = v[q];
-      }
-    }
+    sd.npw[half] = src.npw[half] = (int)F.size();
+    src.pwf[half] = F.empty() ? nullptr : h->dsfaces + (half ? h->step_nf0 : 0);
     const std::vector<NodalCell> &C = h->hcells[half];
-    sd.ncell[half] = (int)C.size();
-    sd.cells[half] = h->dcells[half];
+    sd.ncell[half] = src.ncell[half] = (int)C.size();
+    src.cells[half] = h->dcells[half];
     int *b = sd.cellbox[half];
     b[0] = b[1] = b[2] = 0x7fffffff;
     b[3] = b[4] = b[5] = -0x7fffffff;
@@ -459,22 +479,108 @@ static int build_step(fdtd_handle *h) {
       }
     }
   }
-  sd.waves = h->dwaves;
-  sd.phys = h->dphys;
-  sd.pstride = h->pstride;
-  sd.tiempo = h->dtiempo;
-  sd.dt = h->dt;
-  sd.pwG[0] = h->nmedia >= 1 ? h->g2h[1] : 0.0f;
-  sd.pwG[1] = h->nmedia >= 1 ? h->gm2h[1] : 0.0f;
-  sd.values = h->dvalues;
-  sd.nsrc = (int)h->srcs.size();
+  // tile classes: a tile covers lanes x0..x0+63, rows y0..y0+R-1 and planes
+  // k0..k1 (E is recomputed on k1), halo lanes and rows included
+  {
+    struct Box { int lo[3], hi[3]; };
+    std::vector<Box> ade, srcb;
+    for (int side = 0; side < 2; ++side) {
+      const AdeSet &A = h->g.ade[side];
+      if (!A.has) continue;
+      for (int c = 0; c < 3; ++c) {
+        const int *b = A.abox[c];
+        if (b[3] <= 0 || b[4] <= 0 || b[5] <= 0) continue;
+        ade.push_back({{b[0], b[1], b[2]}, {b[0] + b[3] - 1, b[1] + b[4] - 1, b[2] + b[5] - 1}});
+      }
+    }
+    for (int half = 0; half < 2; ++half) {
+      for (const PwFace &f : h->pw_faces[half]) {
+        Box b;
+        for (int a = 0; a < 3; ++a) { b.lo[a] = f.lo[a]; b.hi[a] = f.hi[a]; }
+        b.lo[f.axis] = b.hi[f.axis] = f.fixed;
+        srcb.push_back(b);
+      }
+      if (sd.ncell[half] > 0) {
+        const int *c = sd.cellbox[half];
+        srcb.push_back({{c[0], c[1], c[2]}, {c[3], c[4], c[5]}});
+      }
+    }
+    const int nn[3] = {h->g.nx, h->g.ny, h->g.nz};
+    std::vector<std::vector<int>> lists(8);
+    for (int bz = 0; bz < tm.tz; ++bz)
+      for (int by = 0; by < tm.ty; ++by)
+        for (int bx = 0; bx < tm.tx; ++bx) {
+          const int lo[3] = {bx * (kWave - 1), by * (R - 1), bz * tm.kc};
+          const int hi[3] = {lo[0] + kWave - 1, lo[1] + R - 1,
+                             std::min(lo[2] + tm.kc, h->g.nz + 1)};
+          int cls = 0;
+          if (h->g.has_cpml)
+            for (int a = 0; a < 3; ++a)
+              if ((h->g.L[a][0] > 0 && lo[a] < h->g.L[a][0]) ||
+                  (h->g.L[a][1] > 0 && hi[a] >= nn[a] - h->g.L[a][1]))
+                cls |= 1;
+          auto meets = [&](const std::vector<Box> &v) {
+            for (const Box &b : v)
+              if (lo[0] <= b.hi[0] && b.lo[0] <= hi[0] && lo[1] <= b.hi[1] &&
+                  b.lo[1] <= hi[1] && lo[2] <= b.hi[2] && b.lo[2] <= hi[2])
+                return true;
+            return false;
+          };
+          if (meets(ade)) cls |= 2;
+          if (meets(srcb)) cls |= 4;
+          lists[cls].push_back((bz * tm.ty + by) * tm.tx + bx);
+        }
+    std::vector<int> all;
+    h->step_cls.clear(); h->step_cnt.clear(); h->step_off.clear();
+    for (int c = 0; c < 8; ++c) {
+      if (lists[c].empty()) continue;
+      h->step_cls.push_back(c);
+      h->step_cnt.push_back((int)lists[c].size());
+      h->step_off.push_back((int)all.size());
+      all.insert(all.end(), lists[c].begin(), lists[c].end());
+    }
+    h->dtiles = nullptr;
+    if (!all.empty() && (rc = upload(h, &h->dtiles, all.data(), all.size(), &h->step_small)))
+      return rc;
+  }
+  src.waves = h->dwaves;
+  src.phys = h->dphys;
+  src.pstride = h->pstride;
+  src.tiempo = h->dtiempo;
+  src.dt = h->dt;
+  src.pwG[0] = h->nmedia >= 1 ? h->g2h[1] : 0.0f;
+  src.pwG[1] = h->nmedia >= 1 ? h->gm2h[1] : 0.0f;
+  src.values = h->dvalues;
+  src.nsrc = (int)h->srcs.size();
+  StepSrc *dsrc = nullptr;
+  if ((rc = upload(h, &dsrc, &src, 1, &h->step_small))) return rc;
+  sd.src = dsrc;
+  h->dstepsrc = dsrc;
+  sd.pwG[0] = src.pwG[0];
+  sd.pwG[1] = src.pwG[1];
   sd.dn = h->dstep;
-  h->sd[0] = sd;
-  h->sd[0].gi = h->g;
-  h->sd[0].go = h->gB;
-  h->sd[1] = sd;
-  h->sd[1].gi = h->gB;
-  h->sd[1].go = h->g;
+  // CPML and pole state of each direction, read by the slab and dispersive
+  // cells
+  for (int dir = 0; dir < 2; ++dir) {
+    const DevGrid &gi = dir ? h->gB : h->g;
+    const DevGrid &go = dir ? h->g : h->gB;
+    StepAux a{};
+    for (int s = 0; s < 12; ++s) { a.psi_i[s] = gi.psi[s]; a.psi_o[s] = go.psi[s]; }
+    for (int ax = 0; ax < 3; ++ax) {
+      a.pbe[ax] = gi.pbe[ax]; a.pce[ax] = gi.pce[ax];
+      a.pbm[ax] = gi.pbm[ax]; a.pcm[ax] = gi.pcm[ax];
+      a.L[ax][0] = gi.L[ax][0]; a.L[ax][1] = gi.L[ax][1];
+    }
+    for (int side = 0; side < 2; ++side) { a.ai[side] = gi.ade[side]; a.ao[side] = go.ade[side]; }
+    a.nx = gi.nx; a.ny = gi.ny; a.nz = gi.nz;
+    a.dt = gi.dt;
+    StepAux *da = nullptr;
+    if ((rc = upload(h, &da, &a, 1, &h->step_small))) return rc;
+    h->sd[dir] = sd;
+    h->sd[dir].gi = gi;
+    h->sd[dir].go = go;
+    h->sd[dir].aux = da;
+  }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->step_ready = true;
   return FDTD_OK;
@@ -493,11 +599,28 @@ static int sync_b(fdtd_handle *h) {
 
 // one step of the pair, from sd[dir].gi to sd[dir].go, then the H-side
 // boundary, observation and step counter on the written copy
+// the step kernels of one direction: one launch per tile class
+static int launch_step_kernels(fdtd_handle *h, int dir) {
+  if (h->step_nf > 0 &&
+      launch_pw_incident(h->dsfaces, h->step_nf0, h->step_nf, h->step_maxcells, h->dstepsrc,
+                         h->dstep, h->dinc, h->stream))
+    return fail(h, FDTD_ERR_HIP, "incident-field launch failed");
+  for (size_t q = 0; q < h->step_cls.size(); ++q) {
+    StepDev sd = h->sd[dir];
+    sd.tiles = h->dtiles + h->step_off[q];
+    sd.tm.nblk = h->step_cnt[q];
+    const int e = h->mode == FDTD_MODE_EXACT
+                      ? exact::launch_step(sd, h->step_rows, h->step_cls[q], h->stream)
+                      : fast::launch_step(sd, h->step_rows, h->step_cls[q], h->stream);
+    if (e) return fail(h, FDTD_ERR_HIP, "step launch failed: %s", hipGetErrorString((hipError_t)e));
+  }
+  return FDTD_OK;
+}
+
 static int run_step(fdtd_handle *h, int dir) {
   const StepDev &sd = h->sd[dir];
-  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_step(sd, h->step_rows, h->stream)
-                                     : fast::launch_step(sd, h->step_rows, h->stream);
-  if (e) return fail(h, FDTD_ERR_HIP, "step launch failed: %s", hipGetErrorString((hipError_t)e));
+  int e = launch_step_kernels(h, dir);
+  if (e) return e;
   if (!h->mur_runs.empty() &&
       launch_mur(sd.go, h->dmur, (int)h->mur_runs.size(), h->mur_total,
                  h->mode == FDTD_MODE_EXACT, h->stream))
@@ -1496,12 +1619,15 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
 }
 
 int fdtd_set_step_mode(fdtd_handle *h, int32_t rows, int32_t zchunk) {
-  if (!h || !(rows == 0 || rows == 4 || rows == 8) || zchunk < 1)
+  if (!h || !(rows == 0 || rows == 4 || rows == 8 || rows == 16) || zchunk < 1)
     return fail(h, FDTD_ERR_ARG, "step mode %d rows x %d planes not supported", rows, zchunk);
-  if (rows && !step_supported(h, rows))
-    return fail(h, FDTD_ERR_STATE, "one-launch steps need fp32 storage and small media tables");
-  h->step_rows = rows;
+  const int kc_before = h->step_kc;
   h->step_kc = zchunk;
+  if (rows && !step_supported(h, rows)) {
+    h->step_kc = kc_before;
+    return fail(h, FDTD_ERR_STATE, "one-launch steps need fp32 storage, small media tables and at most 64 TF/SF faces per half");
+  }
+  h->step_rows = rows;
   invalidate_graph(h);
   return FDTD_OK;
 }
@@ -1528,10 +1654,8 @@ int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms) {
   // the step kernels alone, in pairs so that the state ends in A
   auto pair = [&]() -> int {
     for (int dir = 0; dir < 2; ++dir) {
-      const int e = h->mode == FDTD_MODE_EXACT
-                        ? exact::launch_step(h->sd[dir], h->step_rows, h->stream)
-                        : fast::launch_step(h->sd[dir], h->step_rows, h->stream);
-      if (e) return fail(h, FDTD_ERR_HIP, "step launch failed: %s", hipGetErrorString((hipError_t)e));
+      const int e = launch_step_kernels(h, dir);
+      if (e) return e;
     }
     return FDTD_OK;
   };

@@ -50,6 +50,7 @@ struct AdeSet {
   float *prev[3];               // field at the previous step, per box slot
   void *jst[3];                 // FAST: float2[P][N], EXACT: double2[P][N]
   long long nstate[3];
+  int maxp;                     // pole planes of each state array (>= 1)
   const int32_t *anp;           // per slot: stored poles
   const float *adef;            // FAST: per slot per pole {kr,ki,bdtr,bdti,g3r,g3i}
   const double *aded;           // EXACT: per slot per pole {kr,ki,br,bi,g3r,g3i}
@@ -132,6 +133,8 @@ struct PwFace {
   int nf, off;                // incident component, its offset along axis
   float sign, id;             // +1/-1 and the inverse spacing at `fixed`
   int wave;
+  long long inc;              // one-launch steps: the face's first cell in the
+                              // incident-field buffer (cells ua fastest)
 };
 
 // One TF/SF face of one grid with everything its launch needs, so that the
@@ -170,7 +173,9 @@ struct ProbeDev {
   int comp;
 };
 
-constexpr int kStepPlanes = 8;
+// TF/SF faces per half that a one-launch step handles (one bit each in the
+// lane and plane masks)
+constexpr int kStepFaces = 64;
 
 // One full step of one grid in a single z-marching launch (E then H of each
 // plane, both fields read from gi and written to go; see k_step in
@@ -179,9 +184,9 @@ constexpr int kStepPlanes = 8;
 // every H value a tile writes has its E neighbours computed in the same
 // block.  The plane-wave faces and nodal cells are applied inline, in the
 // reference's order.
-struct StepDev {
-  DevGrid gi, go;             // read (step n) and write (step n+1) views
-  TileMap tm;                 // tx, ty, tz over strides 63 x (R-1) x kc
+// the plane-wave faces and nodal cells of a grid, in device memory (read by
+// the few cells they touch)
+struct StepSrc {
   // TF/SF faces of the E and H halves, in the reference's face order
   int npw[2];
   const PwFace *pwf[2];
@@ -191,17 +196,40 @@ struct StepDev {
   const double *tiempo;
   double dt;
   float pwG[2];               // G2(1), GM2(1)
-  // the planes holding faces, per half and axis (npl -1: too many to list,
-  // every wave checks the faces)
-  int npl[2][3];
-  int pl[2][3][kStepPlanes];
   // nodal cells of the E and H halves, in the reference's order
   int ncell[2];
   const NodalCell *cells[2];
-  int cellbox[2][6];          // bounding box of each half's cells
   const float *values;        // [step][source]
   int nsrc;
+};
+
+// the CPML psi and pole state of one step direction, in device memory (read
+// by the slab and dispersive cells only)
+struct StepAux {
+  const float *psi_i[12];     // read copy
+  float *psi_o[12];           // written copy
+  const float *pbe[3], *pce[3], *pbm[3], *pcm[3];
+  AdeSet ai[2], ao[2];        // pole state: read and written copies, E and H
+  int L[3][2];
+  int nx, ny, nz;
+  double dt;
+};
+
+struct StepDev {
+  DevGrid gi, go;             // read (step n) and write (step n+1) views
+  const StepAux *aux;
+  TileMap tm;                 // tx, ty, tz over strides 63 x (R-1) x kc;
+                              // nblk: the tiles of this launch
+  const int *tiles;           // tile numbers of this launch's class
+  // TF/SF faces per half (staged in LDS by the blocks that meet them), and
+  // the bounding box of each half's nodal cells
+  int npw[2];
+  int ncell[2];
+  int cellbox[2][6];
+  const StepSrc *src;
   const int *dn;              // device step counter
+  const float *inc;           // incident field of every face cell this step
+  float pwG[2];               // G2(1), GM2(1)
 };
 
 // a batch of independent grids for the persistent face-pass launch: tile t of
@@ -281,8 +309,8 @@ __device__ __forceinline__ float nodal_apply(float v, const NodalCell &c, float 
                int yee, int cpml, int ade, void *stream);                    \
   int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4,      \
                int yee, int cpml, int ade, void *stream);                    \
-  size_t step_lds_bytes(const DevGrid &g, int rows);                         \
-  int launch_step(const StepDev &sd, int rows, void *stream);                \
+  size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc);       \
+  int launch_step(const StepDev &sd, int rows, int cls, void *stream);       \
   int launch_pass_batch(const BatchDev &bd, int half, int blocks, int rows,  \
                         int vec, size_t lds, int fp16, void *stream);        \
   }
@@ -300,6 +328,12 @@ int launch_dft(const DevGrid &g, const DftDev &d, int n, const int *dn, void *st
 // tiempo(n) + dt/2); runs[r] covers cells start[r] .. start[r+1]-1 of total
 int launch_tfsf_list(const PwRun *runs, const long long *start, int nruns, long long total,
                      int n, void *stream);
+// the incident field of every TF/SF face cell at step *dn for the one-launch
+// step: faces[0..nf0) act on E (time tiempo(n)), faces[nf0..nf) on H
+// (tiempo(n) + dt/2); face q's cells go to inc[faces[q].inc ...], the face's
+// first free axis fastest
+int launch_pw_incident(const PwFace *faces, int nf0, int nf, long long maxcells,
+                       const StepSrc *src, const int *dn, float *inc, void *stream);
 // increments every counter of the list (per-grid step counters of a batch)
 int launch_tick_many(int *const *counters, int n, void *stream);
 int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,

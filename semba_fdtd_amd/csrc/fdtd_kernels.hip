@@ -56,15 +56,17 @@ typedef float coef_t;
 constexpr int kPre = SFD_PREFETCH_POLES;
 #endif
 
+// hardware dispatches block b to XCD b % 8; give each XCD one contiguous
+// range of tiles so that x/y neighbours share that XCD's L2
+__device__ __forceinline__ int xcd_remap(int b, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int xcd = b & 7, loc = b >> 3;
+  return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+}
+
 __device__ __forceinline__ void decode_tile(const TileMap &tm, int b, int &bx, int &by,
                                             int &bz) {
-  if (tm.remap) {
-    // hardware dispatches block b to XCD b % 8; give each XCD one contiguous
-    // range of tiles so that x/y neighbours share that XCD's L2.
-    const int q = tm.nblk >> 3, r = tm.nblk & 7;
-    const int xcd = b & 7, loc = b >> 3;
-    b = (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
-  }
+  if (tm.remap) b = xcd_remap(b, tm.nblk);
   bx = b % tm.tx;
   b /= tm.tx;
   by = b % tm.ty;
@@ -199,38 +201,33 @@ struct AdePre {
 };
 
 // F -= sum_p Re(G3_p J_p); J_p = kappa_p J_p + beta_p (F - Fprev)/dt;
-// Fprev = F  (F = E with G3, or H with GM3).  The state is read from ai and
-// the new state written to ao when wr (the same set for in-place updates).
-// FAST arithmetic is written out as fused multiply-adds so that every kernel
-// that inlines it rounds the same way.
-template <int C>
-__device__ __forceinline__ float ade_apply(const AdeSet &ai, const AdeSet &ao, bool wr,
-                                           double dt, const MedTab &t, const AdePre &a,
-                                           float e) {
-  const coef_t *cf = t.acf + (size_t)a.slot * kPMax * 6;
-  const long long N = ai.nstate[C];
-  const jstate_t *Ji = reinterpret_cast<const jstate_t *>(ai.jst[C]);
-  jstate_t *Jo = reinterpret_cast<jstate_t *>(ao.jst[C]);
+// Fprev = F  (F = E with G3, or H with GM3).  cf: the slot's coefficients,
+// ep: Fprev; jget(q) gives the state of pole q and jput(q, v) stores its new
+// state.  Returns the new F.  FAST arithmetic is written out as fused
+// multiply-adds so that every kernel that inlines it rounds the same way.
+template <class JGet, class JPut>
+__device__ __forceinline__ float ade_core(const coef_t *cf, int np, float ep, double dt,
+                                          float e, JGet jget, JPut jput) {
 #if SFD_EXACT
   // one pole at a time, rounding E after each (electricdispersive.F90:319-321)
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
-    if (q >= a.np) break;
-    const double2 jv = q < kPre ? a.j[q] : Ji[q * N + a.s];
+    if (q >= np) break;
+    const double2 jv = jget(q);
     const double re = cf[q * 6 + 4] * jv.x - cf[q * 6 + 5] * jv.y;
     e = (float)((double)e - re);
   }
-  const float diff = e - a.ep;
+  const float diff = e - ep;
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
-    if (q >= a.np) break;
-    const double2 jv = q < kPre ? a.j[q] : Ji[q * N + a.s];
+    if (q >= np) break;
+    const double2 jv = jget(q);
     const double kr = cf[q * 6 + 0], ki = cf[q * 6 + 1];
     const double kjr = kr * jv.x - ki * jv.y;
     const double kji = kr * jv.y + ki * jv.x;
     const double br = cf[q * 6 + 2] * (double)diff / dt;
     const double bi = cf[q * 6 + 3] * (double)diff / dt;
-    if (wr) Jo[q * N + a.s] = make_double2(kjr + br, kji + bi);
+    jput(q, make_double2(kjr + br, kji + bi));
   }
 #else
   // folded fp32 form: beta is pre-divided by dt, G3 of a folded pair doubled
@@ -238,22 +235,39 @@ __device__ __forceinline__ float ade_apply(const AdeSet &ai, const AdeSet &ao, b
   float sum = 0.0f;
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
-    if (q >= a.np) break;
-    const float2 jv = q < kPre ? a.j[q] : Ji[q * N + a.s];
+    if (q >= np) break;
+    const float2 jv = jget(q);
     sum = sum + __builtin_fmaf(cf[q * 6 + 4], jv.x, -(cf[q * 6 + 5] * jv.y));
   }
   e = e - sum;
-  const float diff = e - a.ep;
+  const float diff = e - ep;
 #pragma unroll
   for (int q = 0; q < kPMax; ++q) {
-    if (q >= a.np) break;
-    const float2 jv = q < kPre ? a.j[q] : Ji[q * N + a.s];
+    if (q >= np) break;
+    const float2 jv = jget(q);
     const float kr = cf[q * 6 + 0], ki = cf[q * 6 + 1];
     const float nr = __builtin_fmaf(kr, jv.x, __builtin_fmaf(-ki, jv.y, cf[q * 6 + 2] * diff));
     const float ni = __builtin_fmaf(kr, jv.y, __builtin_fmaf(ki, jv.x, cf[q * 6 + 3] * diff));
-    if (wr) Jo[q * N + a.s] = make_float2(nr, ni);
+    jput(q, make_float2(nr, ni));
   }
 #endif
+  return e;
+}
+
+// the recursion on edge a.s of component C: state read from ai, the new state
+// written to ao when wr (the same set for in-place updates)
+template <int C>
+__device__ __forceinline__ float ade_apply(const AdeSet &ai, const AdeSet &ao, bool wr,
+                                           double dt, const MedTab &t, const AdePre &a,
+                                           float e) {
+  const long long N = ai.nstate[C];
+  const jstate_t *Ji = reinterpret_cast<const jstate_t *>(ai.jst[C]) + a.s;
+  jstate_t *Jo = reinterpret_cast<jstate_t *>(ao.jst[C]) + a.s;
+  e = ade_core(t.acf + (size_t)a.slot * kPMax * 6, a.np, a.ep, dt, e,
+               [&](int q) { return q < kPre ? a.j[q] : Ji[q * N]; },
+               [&](int q, jstate_t v) {
+                 if (wr) Jo[q * N] = v;
+               });
   if (wr) ao.prev[C][a.s] = e;
   return e;
 }
@@ -982,80 +996,178 @@ int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
 // H values of step n are read from memory (gi is never written during the
 // step).
 
-// psi <- b*psi + d*c from gi's array, written to go's array when wr
-__device__ __forceinline__ float psi_next(const float *pi, float *po, bool wr, long long s,
-                                          float b, float d, float c) {
-  const float v = madd(d * c, b, pi[s]);
-  if (wr) po[s] = v;
+// Loads and stores of the march address a plane through a base pointer that
+// is the same for every lane (scalar registers) plus the lane's byte offset
+// inside the plane (one 32-bit register), instead of one 64-bit address per
+// array and lane.
+template <class T>
+__device__ __forceinline__ T ld_at(const T *base, unsigned ob) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + ob);
+}
+template <class T>
+__device__ __forceinline__ void st_at(T *base, unsigned ob, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + ob) = v;
+}
+
+// the base of one plane of an array, made opaque to the compiler: the lane's
+// address is then this scalar base plus its 32-bit offset, rather than a
+// 64-bit (array base + offset) pair per array held across the march
+#define SFD_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ SFD_GLOBAL T *plane_base(T *base, long long P) {
+  SFD_GLOBAL T *q = (SFD_GLOBAL T *)(base + P);
+  asm volatile("" : "+s"(q));
+  return q;
+}
+template <class T>
+__device__ __forceinline__ T ld_at(const SFD_GLOBAL T *base, unsigned ob) {
+  return *reinterpret_cast<const SFD_GLOBAL T *>(reinterpret_cast<const SFD_GLOBAL char *>(base) +
+                                                 ob);
+}
+template <class T>
+__device__ __forceinline__ void st_at(SFD_GLOBAL T *base, unsigned ob, T v) {
+  *reinterpret_cast<SFD_GLOBAL T *>(reinterpret_cast<SFD_GLOBAL char *>(base) + ob) = v;
+}
+
+// an element of a table the kernel never writes, at an index the same for
+// every lane: read through the constant address space, i.e. by a scalar load
+// that does not join the vector-memory counter shared with the stores
+__device__ __forceinline__ float ro_uniform(const float *t, int idx) {
+  return ((const __attribute__((address_space(4))) float *)t)[idx];
+}
+
+// the descriptor pointer made opaque to the compiler once per plane: what is
+// read through it is read where it is used (slab and dispersive cells), not
+// hoisted out of the march into scalar registers that the plain cells pay for
+typedef const __attribute__((address_space(4))) StepAux CStepAux;
+__device__ __forceinline__ CStepAux *per_plane(const StepAux *p) {
+  CStepAux *q = (CStepAux *)p;
+  asm volatile("" : "+s"(q));
+  return q;
+}
+
+// CPML psi of one correction: the new value from the step-n value old,
+// written to the other copy when wr
+__device__ __forceinline__ float psi_new(float old, float *po, bool wr, float b, float d,
+                                         float c) {
+  const float v = madd(d * c, b, old);
+  if (wr) *po = v;
   return v;
 }
 
-// the pole state of one dispersive edge of component C (side: E or H)
-template <int C>
-__device__ __forceinline__ float step_poles(const AdeSet &ai, const AdeSet &ao, bool wr,
-                                            double dt, const MedTab &t, int slot, int i, int j,
-                                            int k, float e) {
-  const int *b = ai.abox[C];
-  AdePre a;
-  a.s = ((long long)(k - b[2]) * b[4] + (j - b[1])) * b[3] + (i - b[0]);
-  a.slot = slot;
-  a.np = t.anp[slot];
-  a.ep = ai.prev[C][a.s];
-  const jstate_t *J = reinterpret_cast<const jstate_t *>(ai.jst[C]);
+// The pole state of one field (E or H) at one plane k, per component: the
+// lane's offset in the xy extent of the component's state box (~0u outside),
+// fixed for the lane, and the plane's offset zo in the box.
+__device__ __forceinline__ void pole_xy(const AdeSet &A, int i, int j, unsigned *axy) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int *b = A.abox[c];
+    axy[c] = (i >= b[0] && i < b[0] + b[3] && j >= b[1] && j < b[1] + b[4])
+                 ? (unsigned)((j - b[1]) * b[3] + (i - b[0]))
+                 : ~0u;
+  }
+}
+template <class AS>
+__device__ __forceinline__ bool pole_plane(const AS &A, int c, int k, long long &zo) {
+  const auto &b = A.abox[c];
+  zo = (long long)(k - b[2]) * b[3] * b[4];
+  return k >= b[2] && k < b[2] + b[5];
+}
+// load the previous field and the first kPre pole states of edge zo + axy
+template <class AS>
+__device__ __forceinline__ void pole_load(const AS &A, int c, long long zo, unsigned axy,
+                                          float &ep, jstate_t *pj) {
+  ep = (A.prev[c] + zo)[axy];
+  const jstate_t *J = reinterpret_cast<const jstate_t *>(A.jst[c]) + zo;
 #pragma unroll
   for (int q = 0; q < kPre; ++q)
-    if (q < a.np) a.j[q] = J[q * ai.nstate[C] + a.s];
-  return ade_apply<C>(ai, ao, wr, dt, t, a, e);
+    if (q < A.maxp) pj[q] = (J + q * A.nstate[c])[axy];
+}
+// the new field of that edge (slot: its dispersive medium); the rest of its
+// poles are read here, the new state is written to ao when wr
+template <class AS>
+__device__ __forceinline__ float pole_new(const AS &ai, const AS &ao, int c,
+                                          long long zo, unsigned axy, bool wr, double dt,
+                                          const MedTab &t, int slot, float ep,
+                                          const jstate_t *pj, float e) {
+  const long long N = ai.nstate[c];
+  const jstate_t *Ji = reinterpret_cast<const jstate_t *>(ai.jst[c]) + zo;
+  jstate_t *Jo = reinterpret_cast<jstate_t *>(ao.jst[c]) + zo;
+  e = ade_core(t.acf + (size_t)slot * kPMax * 6, t.anp[slot], ep, dt, e,
+               [&](int q) { return q < kPre ? pj[q] : (Ji + q * N)[axy]; },
+               [&](int q, jstate_t v) {
+                 if (wr) (Jo + q * N)[axy] = v;
+               });
+  if (wr) (ao.prev[c] + zo)[axy] = e;
+  return e;
 }
 
-// plane-wave faces and nodal cells of one half acting on component comp at
-// cell cc (offset p), in the reference's order
-__device__ __noinline__ float step_sources(const StepDev &sd, int half, int comp, long long p,
-                                           int i, int j, int k, float v, bool pw, bool nd,
-                                           int n) {
-  if (pw) {
-    const int cc[3] = {i, j, k};
-    const float tm = half ? (float)(sd.tiempo[n] + (double)0.5f * sd.dt) : (float)sd.tiempo[n];
-    for (int q = 0; q < sd.npw[half]; ++q) {
-      const PwFace &f = sd.pwf[half][q];
-      if (f.comp != comp || cc[f.axis] != f.fixed) continue;
-      const int ua = f.axis == 0 ? 1 : 0, va = f.axis == 2 ? 1 : 2;
-      if (cc[ua] < f.lo[ua] || cc[ua] > f.hi[ua] || cc[va] < f.lo[va] || cc[va] > f.hi[va])
-        continue;
-      const float ehi = pw_incid(f, sd.waves, sd.phys, sd.pstride, tm, cc);
-      v = pw_apply(v, f.sign, sd.pwG[half], ehi, f.id);
-    }
-  }
-  if (nd) {
-    for (int q = 0; q < sd.ncell[half]; ++q) {
-      const NodalCell &c = sd.cells[half][q];
-      if (c.comp != comp || c.p != p || c.skip) continue;
-      v = nodal_apply(v, c, sd.values[(long long)n * sd.nsrc + c.src]);
-    }
+// nodal cells of one half acting on component comp at field offset p, in the
+// reference's order
+__device__ __noinline__ float step_nodal(const StepSrc *sp, int half, int comp, long long p,
+                                         float v, int n) {
+  const StepSrc &sd = *sp;
+  for (int q = 0; q < sd.ncell[half]; ++q) {
+    const NodalCell &c = sd.cells[half][q];
+    if (c.comp != comp || c.p != p || c.skip) continue;
+    v = nodal_apply(v, c, sd.values[(long long)n * sd.nsrc + c.src]);
   }
   return v;
 }
 
-// does a wave of row j, plane k, columns [x0, x0+63] touch a TF/SF face of
-// the half: the x and y tests do not change along the z march
-__device__ __forceinline__ bool step_pw_xy(const StepDev &sd, int half, int x0, int j) {
-  if (sd.npw[half] == 0) return false;
-  if (sd.npl[half][0] < 0 || sd.npl[half][1] < 0 || sd.npl[half][2] < 0) return true;
-  for (int q = 0; q < sd.npl[half][0]; ++q)
-    if (sd.pl[half][0][q] >= x0 && sd.pl[half][0][q] <= x0 + kWave - 1) return true;
-  for (int q = 0; q < sd.npl[half][1]; ++q)
-    if (sd.pl[half][1][q] == j) return true;
-  return false;
-}
-__device__ __forceinline__ bool step_pw_z(const StepDev &sd, int half, int k) {
-  for (int q = 0; q < sd.npl[half][2]; ++q)
-    if (sd.pl[half][2][q] == k) return true;
-  return false;
-}
-__device__ __forceinline__ bool step_nd(const StepDev &sd, int half, int x0, int j, int k) {
+// can a wave of row j, columns [x0, x0+63] hold a nodal cell of the half
+__device__ __forceinline__ bool step_nd_xy(const StepDev &sd, int half, int x0, int j) {
   const int *b = sd.cellbox[half];
-  return sd.ncell[half] > 0 && j >= b[1] && j <= b[4] && k >= b[2] && k <= b[5] &&
-         x0 + kWave - 1 >= b[0] && x0 <= b[3];
+  return sd.ncell[half] > 0 && j >= b[1] && j <= b[4] && x0 + kWave - 1 >= b[0] && x0 <= b[3];
+}
+
+// the incident field of face f at cell (i, j, k), from the buffer filled
+// before the step
+__device__ __forceinline__ float face_inc(const float *inc, const PwFace &f, int i, int j,
+                                          int k) {
+  const int cc[3] = {i, j, k};
+  const int ua = f.axis == 0 ? 1 : 0, va = f.axis == 2 ? 1 : 2;
+  const long long nu = f.hi[ua] - f.lo[ua] + 1;
+  return inc[f.inc + (long long)(cc[va] - f.lo[va]) * nu + (cc[ua] - f.lo[ua])];
+}
+// the faces in bits applied in order to v; the first face's incident value
+// was loaded with the plane (first)
+__device__ __forceinline__ float faces_apply(const PwFace *F, unsigned long long bits,
+                                             float first, const float *inc, float G, int i,
+                                             int j, int k, float v) {
+  int q = __builtin_ctzll(bits);
+  v = pw_apply(v, F[q].sign, G, first, F[q].id);
+  bits &= bits - 1;
+  while (bits) {
+    q = __builtin_ctzll(bits);
+    bits &= bits - 1;
+    v = pw_apply(v, F[q].sign, G, face_inc(inc, F[q], i, j, k), F[q].id);
+  }
+  return v;
+}
+
+// does face f of a half cross column (i, j) of the grid (any plane)
+__device__ __forceinline__ bool face_column(const PwFace &f, int i, int j) {
+  if (f.axis == 0) return i == f.fixed && j >= f.lo[1] && j <= f.hi[1];
+  if (f.axis == 1) return j == f.fixed && i >= f.lo[0] && i <= f.hi[0];
+  return i >= f.lo[0] && i <= f.hi[0] && j >= f.lo[1] && j <= f.hi[1];
+}
+// does face f cross plane k
+__device__ __forceinline__ bool face_plane(const PwFace &f, int k) {
+  return f.axis == 2 ? k == f.fixed : (k >= f.lo[2] && k <= f.hi[2]);
+}
+
+// LDS of one block: the E and H media tables, the E exchange rows, and for
+// grids with plane waves the faces of both halves, their per-plane masks over
+// the chunk's kc + 1 planes and their per-component masks
+static __host__ __device__ __forceinline__ size_t step_lds_faces(size_t base, int npw, int kc,
+                                                                size_t *offz, size_t *offc) {
+  const size_t f = base + (size_t)npw * sizeof(PwFace);
+  const size_t z = (f + 7) & ~(size_t)7;
+  if (offz) *offz = z;
+  const size_t c = z + (size_t)2 * (kc + 1) * 8;
+  if (offc) *offc = c;
+  return c + 6 * 8;
 }
 
 static __host__ __device__ __forceinline__ size_t step_tab_bytes(int nmedia, int nd) {
@@ -1063,134 +1175,345 @@ static __host__ __device__ __forceinline__ size_t step_tab_bytes(int nmedia, int
          (size_t)nd * kPMax * 6 * sizeof(coef_t);
 }
 
-// ADE: grids with pole state on either field; SRC: grids with plane waves or
-// nodal sources (the variants without them carry no code for them)
-template <int R, bool ADE, bool SRC>
+// Tiles are launched in classes: CPML, tiles that touch a CPML slab; ADE,
+// tiles that meet the pole-state box of some component; SRC, tiles that
+// touch a TF/SF face or a nodal cell.  A class's kernel carries only the code
+// its tiles need, so interior tiles run the plain Yee march.
+template <int R, bool CPML, bool ADE, bool SRC>
 __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
   extern __shared__ float2 lds[];
   const DevGrid &gi = sd.gi;
   const DevGrid &go = sd.go;
   const size_t offh = step_tab_bytes(gi.nmedia, gi.ade[0].ndisp);
   const size_t offx = offh + step_tab_bytes(gi.nmedia, gi.ade[1].ndisp);
-  const MedTab te = stage_tables<true, true>(gi, gi.g1, gi.g2, lds, &gi.ade[0]);
-  const MedTab th = stage_tables<true, true>(
+  const MedTab te = stage_tables<true, ADE>(gi, gi.g1, gi.g2, lds, &gi.ade[0]);
+  const MedTab th = stage_tables<true, ADE>(
       gi, gi.gm1, gi.gm2, reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offh),
       &gi.ade[1]);
   float2 *xch = reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offx);
+  const int t = sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)];
+  const int bx = t % sd.tm.tx, by = (t / sd.tm.tx) % sd.tm.ty, bz = t / (sd.tm.tx * sd.tm.ty);
+  const int k0 = bz * sd.tm.kc;
+  // TF/SF faces of both halves in LDS (E faces, then H faces), with the mask
+  // of the faces crossing each plane of the chunk (E: k0..k0+kc, H: the same
+  // numbering) and the mask of each component's faces
+  PwFace *lf = nullptr;
+  unsigned long long *lzm = nullptr, *lcm = nullptr;
+  const int np0 = SRC ? sd.npw[0] : 0, np1 = SRC ? sd.npw[1] : 0;
+  const int kc1 = sd.tm.kc + 1;
+  if (SRC && np0 + np1 > 0) {
+    size_t offz = 0, offc = 0;
+    step_lds_faces(offx + (size_t)2 * R * kWave * sizeof(float2), np0 + np1, sd.tm.kc, &offz,
+                   &offc);
+    lf = reinterpret_cast<PwFace *>(reinterpret_cast<char *>(lds) + offx +
+                                    (size_t)2 * R * kWave * sizeof(float2));
+    lzm = reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(lds) + offz);
+    lcm = reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(lds) + offc);
+    const StepSrc &S = *sd.src;
+    const int tid = threadIdx.y * kWave + threadIdx.x, nt = kWave * R;
+    for (int q = tid; q < np0 + np1; q += nt) lf[q] = q < np0 ? S.pwf[0][q] : S.pwf[1][q - np0];
+    for (int u = tid; u < 2 * kc1; u += nt) {
+      const int hf = u / kc1, k = k0 + u % kc1;
+      const PwFace *F = S.pwf[hf];
+      unsigned long long m = 0;
+      for (int q = 0; q < (hf ? np1 : np0); ++q)
+        if (face_plane(F[q], k)) m |= 1ull << q;
+      lzm[u] = m;
+    }
+    if (tid < 6) {
+      const int hf = tid / 3, comp = tid;
+      const PwFace *F = S.pwf[hf];
+      unsigned long long m = 0;
+      for (int q = 0; q < (hf ? np1 : np0); ++q)
+        if (F[q].comp == comp) m |= 1ull << q;
+      lcm[tid] = m;
+    }
+  }
+  __syncthreads();
 
-  int bx, by, bz;
-  decode_tile(sd.tm, blockIdx.x, bx, by, bz);
   const int lane = threadIdx.x, row = threadIdx.y;
   const int x0 = bx * (kWave - 1);
   const int i = x0 + lane, j = by * (R - 1) + row;
   const int nx = gi.nx, ny = gi.ny, nz = gi.nz;
   const bool in = i <= nx && j <= ny;
   const bool own = in && lane < kWave - 1 && row < R - 1;
-  // threads past the grid's edge read the edge cell (their results are
+  // lanes past the grid's edge address the edge cell (their results are
   // never used): every load and table index stays inside its array
   const int ic = min(i, nx), jc = min(j, ny);
-  const int k0 = bz * sd.tm.kc;
   const int k1 = min(k0 + sd.tm.kc, nz + 1);
-  const long long px = gi.px, pxy = gi.pxy;
-  long long p = (long long)(k0 + 1) * pxy + (long long)(jc + 1) * px + kXOff + ic;
-  const int n = *sd.dn;
-  const double dt = gi.dt;
-  const int Lx = gi.L[0][0] + gi.L[0][1], Ly = gi.L[1][0] + gi.L[1][1];
-  // E side: node positions along x and y; H side: cell positions
+  const long long pxy = gi.pxy;
+  const unsigned pxb = (unsigned)gi.px * 4u;
+  // byte offsets inside a plane: the cell and its x - 1 and y - 1 neighbours;
+  // om: the cell in the int16 media arrays
+  const unsigned ob = (unsigned)(jc + 1) * pxb + (unsigned)(kXOff + ic) * 4u;
+  const unsigned obx = ob - 4u, oby = ob - pxb, om = ob >> 1;
+  const bool ex_on = in && i < nx, ey_on = in && j < ny;
+  const bool hx_on = own && j < ny, hy_on = own && i < nx, hz_on = own && i < nx && j < ny;
+  const StepAux *ax = sd.aux;
+  // CPML: slab coordinates along x and y and the x/y profile coefficients,
+  // fixed for the lane (node positions for E, cell positions for H)
   int qxn = 0, qyn = 0, qxc = 0, qyc = 0;
-  const bool sxn = slab_node(i, nx, gi.L[0][0], gi.L[0][1], qxn);
-  const bool syn = slab_node(j, ny, gi.L[1][0], gi.L[1][1], qyn);
-  const bool sxc = slab_cell(i, nx, gi.L[0][0], gi.L[0][1], qxc);
-  const bool syc = slab_cell(j, ny, gi.L[1][0], gi.L[1][1], qyc);
+  bool sxn = false, syn = false, sxc = false, syc = false;
+  float bey = 0.f, cey = 0.f, bex = 0.f, cex = 0.f, bmy = 0.f, cmy = 0.f, bmx = 0.f, cmx = 0.f;
+  long long Lx = 0, Ly = 0;
+  const int Lz0 = gi.L[2][0], Lz1 = gi.L[2][1];
+  if (CPML) {
+    sxn = slab_node(i, nx, gi.L[0][0], gi.L[0][1], qxn);
+    syn = slab_node(j, ny, gi.L[1][0], gi.L[1][1], qyn);
+    sxc = slab_cell(i, nx, gi.L[0][0], gi.L[0][1], qxc);
+    syc = slab_cell(j, ny, gi.L[1][0], gi.L[1][1], qyc);
+    Lx = gi.L[0][0] + gi.L[0][1];
+    Ly = gi.L[1][0] + gi.L[1][1];
+    if (syn) { bey = gi.pbe[1][jc + 1]; cey = gi.pce[1][jc + 1]; }
+    if (sxn) { bex = gi.pbe[0][ic + 1]; cex = gi.pce[0][ic + 1]; }
+    if (syc) { bmy = gi.pbm[1][jc + 1]; cmy = gi.pcm[1][jc + 1]; }
+    if (sxc) { bmx = gi.pbm[0][ic + 1]; cmx = gi.pcm[0][ic + 1]; }
+  }
   const float idyh = gi.id[4][jc + 1], idxh = gi.id[3][ic + 1];
   const float idye = gi.id[1][jc + 1], idxe = gi.id[0][ic + 1];
-  const bool pwxy_e = SRC && step_pw_xy(sd, 0, x0, j), pwxy_h = SRC && step_pw_xy(sd, 1, x0, j);
+  const float *idzh_t = gi.id[5], *idze_t = gi.id[2];
+  const int n = SRC ? *sd.dn : 0;
+  // the faces of each half crossing the lane's column
+  unsigned long long mle = 0, mlh = 0;
+  if (SRC) {
+    for (int q = 0; q < np0; ++q)
+      if (face_column(lf[q], i, j)) mle |= 1ull << q;
+    for (int q = 0; q < np1; ++q)
+      if (face_column(lf[np0 + q], i, j)) mlh |= 1ull << q;
+  }
+  const bool ndxy_e = SRC && step_nd_xy(sd, 0, x0, j), ndxy_h = SRC && step_nd_xy(sd, 1, x0, j);
   const bool ade_e = ADE && gi.ade[0].has, ade_h = ADE && gi.ade[1].has;
+  unsigned axe[3] = {~0u, ~0u, ~0u}, axh[3] = {~0u, ~0u, ~0u};
+  if (ade_e) pole_xy(gi.ade[0], i, j, axe);
+  if (ade_h) pole_xy(gi.ade[1], i, j, axh);
+  const float *Exi = gi.f[0], *Eyi = gi.f[1], *Ezi = gi.f[2];
+  const float *Hxi = gi.f[3], *Hyi = gi.f[4], *Hzi = gi.f[5];
+  float *Exo = go.f[0], *Eyo = go.f[1], *Ezo = go.f[2];
+  float *Hxo = go.f[3], *Hyo = go.f[4], *Hzo = go.f[5];
+  const int16_t *Mex = gi.mi[0], *Mey = gi.mi[1], *Mez = gi.mi[2];
+  const int16_t *Mhx = gi.mi[3], *Mhy = gi.mi[4], *Mhz = gi.mi[5];
 
   // H of step n on plane k0 - 1: the E update's z - 1 neighbours
-  float hx_km = gi.f[3][p - pxy], hy_km = gi.f[4][p - pxy];
+  float hx_km = ld_at(plane_base(Hxi, (long long)k0 * pxy), ob);
+  float hy_km = ld_at(plane_base(Hyi, (long long)k0 * pxy), ob);
+  // the lane's constants arrive before the march: a load still in flight at
+  // the loop head would make every plane's first use of it wait for all
+  // memory operations issued since, stores included
+  __builtin_amdgcn_s_waitcnt(0);
   // carried from plane k - 1 to the H update of that plane
   float hx_c = 0.0f, hy_c = 0.0f, hz_c = 0.0f;          // H of step n
   float ex_c = 0.0f, ey_c = 0.0f, ez_c = 0.0f;          // E of step n+1
   float ey_ip_c = 0.0f, ez_ip_c = 0.0f, ex_jp_c = 0.0f, ez_jp_c = 0.0f;
-  int mhx_c = 0, mhy_c = 0, mhz_c = 0;
+  // H coefficients {GM1, GM2} and dispersive slots of plane k - 1, looked up
+  // while the plane's E half runs (the media loads never outlive the plane)
+  float2 chx = make_float2(0.f, 0.f), chy = chx, chz = chx;
+  int shx = -1, shy = -1, shz = -1;
 
-  for (int k = k0; k <= k1; ++k, p += pxy) {
+  // One wait per plane: every load of the plane (fields, media, and the psi
+  // and pole state of both halves) is issued first; every lane then computes
+  // all components, lanes outside a component's sweep producing values that
+  // no owned cell reads, and only the stores are masked.  A result that a
+  // masked lane left as a loaded register would make the H half wait for the
+  // E stores to drain (loads and stores share one counter).
+  for (int k = k0; k <= k1; ++k) {
+    const long long P = (long long)(k + 1) * pxy;     // plane k
+    const int kh = k - 1;
+    const bool hk = k > k0;
+    const long long Ph = P - pxy;
+    const bool eplane = k <= nz;
+    CStepAux *a = per_plane(ax);
+    // ---- loads of plane k
+    float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f, hz_jm = 0.0f, hx_jm = 0.0f, hz_im = 0.0f,
+          hy_im = 0.0f, ex0 = 0.0f, ey0 = 0.0f, ez0 = 0.0f;
+    int mx = 0, my = 0, mz = 0, mhx = 0, mhy = 0, mhz = 0;
+    float2 nhx = make_float2(0.f, 0.f), nhy = nhx, nhz = nhx;
+    int nsx = -1, nsy = -1, nsz = -1;
+    float idzh = 0.0f;
+    if (eplane) {
+      const SFD_GLOBAL float *hxk = plane_base(Hxi, P), *hyk = plane_base(Hyi, P),
+                             *hzk = plane_base(Hzi, P);
+      hx0 = ld_at(hxk, ob); hy0 = ld_at(hyk, ob); hz0 = ld_at(hzk, ob);
+      hz_jm = ld_at(hzk, oby); hx_jm = ld_at(hxk, oby);
+      hz_im = ld_at(hzk, obx); hy_im = ld_at(hyk, obx);
+      ex0 = ld_at(plane_base(Exi, P), ob); ey0 = ld_at(plane_base(Eyi, P), ob);
+      ez0 = ld_at(plane_base(Ezi, P), ob);
+      mx = ld_at(plane_base(Mex, P), om); my = ld_at(plane_base(Mey, P), om);
+      mz = ld_at(plane_base(Mez, P), om);
+      mhx = ld_at(plane_base(Mhx, P), om); mhy = ld_at(plane_base(Mhy, P), om);
+      mhz = ld_at(plane_base(Mhz, P), om);
+      idzh = ro_uniform(idzh_t, k + 1);
+    }
+    const bool ez_on = in && k < nz;
+    const bool wr = own && k < k1;
+    // CPML psi of the E corrections, in use order: Ex (y, z), Ey (z, x), Ez (y, x)
+    bool f0 = false, f1 = false, f2 = false, f3 = false, f4 = false, f5 = false;
+    float pv0 = 0.f, pv1 = 0.f, pv2 = 0.f, pv3 = 0.f, pv4 = 0.f, pv5 = 0.f;
+    float bez = 0.f, cez = 0.f;
+    int qzn = 0;
+    if (CPML && eplane) {
+      const bool szn = slab_node(k, nz, Lz0, Lz1, qzn);
+      f0 = ex_on && syn; f1 = ex_on && szn;
+      f2 = ey_on && szn; f3 = ey_on && sxn;
+      f4 = ez_on && syn; f5 = ez_on && sxn;
+      if (szn) { bez = ro_uniform(a->pbe[2], k + 1); cez = ro_uniform(a->pce[2], k + 1); }
+      if (f0) pv0 = (a->psi_i[PSI_EXY] + k * Ly * nx)[(unsigned)(qyn * nx + i)];
+      if (f1) pv1 = (a->psi_i[PSI_EXZ] + (long long)qzn * (ny + 1) * nx)[(unsigned)(j * nx + i)];
+      if (f2) pv2 = (a->psi_i[PSI_EYZ] + (long long)qzn * ny * (nx + 1))[(unsigned)(j * (nx + 1) + i)];
+      if (f3) pv3 = (a->psi_i[PSI_EYX] + k * ny * Lx)[(unsigned)(j * Lx + qxn)];
+      if (f4) pv4 = (a->psi_i[PSI_EZY] + k * Ly * (nx + 1))[(unsigned)(qyn * (nx + 1) + i)];
+      if (f5) pv5 = (a->psi_i[PSI_EZX] + k * (ny + 1) * Lx)[(unsigned)(j * Lx + qxn)];
+    }
+    // pole state of this plane's dispersive E edges
+    bool g0 = false, g1 = false, g2 = false;
+    long long z0 = 0, z1 = 0, z2 = 0;
+    float ep0 = 0.f, ep1 = 0.f, ep2 = 0.f;
+    jstate_t aj0[kPre], aj1[kPre], aj2[kPre];
+    if (ade_e && eplane) {
+      const auto &A = a->ai[0];
+      g0 = pole_plane(A, 0, k, z0) && ex_on && axe[0] != ~0u;
+      g1 = pole_plane(A, 1, k, z1) && ey_on && axe[1] != ~0u;
+      g2 = pole_plane(A, 2, k, z2) && ez_on && axe[2] != ~0u;
+      if (g0) pole_load(A, 0, z0, axe[0], ep0, aj0);
+      if (g1) pole_load(A, 1, z1, axe[1], ep1, aj1);
+      if (g2) pole_load(A, 2, z2, axe[2], ep2, aj2);
+    }
+    // the same for the H cells of plane k - 1: Hx (y, z), Hy (z, x), Hz (y, x)
+    const bool hxk = hk && hx_on && kh < nz, hyk = hk && hy_on && kh < nz, hzk = hk && hz_on;
+    bool r0 = false, r1 = false, r2 = false, r3 = false, r4 = false, r5 = false;
+    float qv0 = 0.f, qv1 = 0.f, qv2 = 0.f, qv3 = 0.f, qv4 = 0.f, qv5 = 0.f;
+    float bmz = 0.f, cmz = 0.f;
+    int qzc = 0;
+    if (CPML && hk) {
+      const bool szc = slab_cell(kh, nz, Lz0, Lz1, qzc);
+      r0 = hxk && syc; r1 = hxk && szc;
+      r2 = hyk && szc; r3 = hyk && sxc;
+      r4 = hzk && syc; r5 = hzk && sxc;
+      if (szc) { bmz = ro_uniform(a->pbm[2], kh + 1); cmz = ro_uniform(a->pcm[2], kh + 1); }
+      if (r0) qv0 = (a->psi_i[PSI_HXY] + kh * Ly * (nx + 1))[(unsigned)(qyc * (nx + 1) + i)];
+      if (r1) qv1 = (a->psi_i[PSI_HXZ] + (long long)qzc * ny * (nx + 1))[(unsigned)(j * (nx + 1) + i)];
+      if (r2) qv2 = (a->psi_i[PSI_HYZ] + (long long)qzc * (ny + 1) * nx)[(unsigned)(j * nx + i)];
+      if (r3) qv3 = (a->psi_i[PSI_HYX] + kh * (ny + 1) * Lx)[(unsigned)(j * Lx + qxc)];
+      if (r4) qv4 = (a->psi_i[PSI_HZY] + kh * Ly * nx)[(unsigned)(qyc * nx + i)];
+      if (r5) qv5 = (a->psi_i[PSI_HZX] + kh * ny * Lx)[(unsigned)(j * Lx + qxc)];
+    }
+    bool u0 = false, u1 = false, u2 = false;
+    long long w0 = 0, w1 = 0, w2 = 0;
+    float hp0 = 0.f, hp1 = 0.f, hp2 = 0.f;
+    jstate_t bj0[kPre], bj1[kPre], bj2[kPre];
+    if (ade_h && hk) {
+      const auto &A = a->ai[1];
+      u0 = pole_plane(A, 0, kh, w0) && hxk && axh[0] != ~0u;
+      u1 = pole_plane(A, 1, kh, w1) && hyk && axh[1] != ~0u;
+      u2 = pole_plane(A, 2, kh, w2) && hzk && axh[2] != ~0u;
+      if (u0) pole_load(A, 0, w0, axh[0], hp0, bj0);
+      if (u1) pole_load(A, 1, w1, axh[1], hp1, bj1);
+      if (u2) pole_load(A, 2, w2, axh[2], hp2, bj2);
+    }
+    const float idze = hk ? ro_uniform(idze_t, kh + 1) : 0.0f;
+    // TF/SF faces: the faces acting on each component of this lane's E cell
+    // (plane k) and H cell (plane k - 1), and the first one's incident value
+    unsigned long long be0 = 0, be1 = 0, be2 = 0, bh0 = 0, bh1 = 0, bh2 = 0;
+    float ie0 = 0.f, ie1 = 0.f, ie2 = 0.f, ih0 = 0.f, ih1 = 0.f, ih2 = 0.f;
+    if (SRC && lzm) {
+      if (eplane) {
+        const unsigned long long z = lzm[k - k0] & mle;
+        if (z) {
+          be0 = ex_on ? z & lcm[0] : 0; be1 = ey_on ? z & lcm[1] : 0;
+          be2 = ez_on ? z & lcm[2] : 0;
+          if (be0) ie0 = face_inc(sd.inc, lf[__builtin_ctzll(be0)], i, j, k);
+          if (be1) ie1 = face_inc(sd.inc, lf[__builtin_ctzll(be1)], i, j, k);
+          if (be2) ie2 = face_inc(sd.inc, lf[__builtin_ctzll(be2)], i, j, k);
+        }
+      }
+      if (hk) {
+        const unsigned long long z = lzm[kc1 + (kh - k0)] & mlh;
+        if (z) {
+          bh0 = hxk ? z & lcm[3] : 0; bh1 = hyk ? z & lcm[4] : 0; bh2 = hzk ? z & lcm[5] : 0;
+          if (bh0) ih0 = face_inc(sd.inc, lf[np0 + __builtin_ctzll(bh0)], i, j, kh);
+          if (bh1) ih1 = face_inc(sd.inc, lf[np0 + __builtin_ctzll(bh1)], i, j, kh);
+          if (bh2) ih2 = face_inc(sd.inc, lf[np0 + __builtin_ctzll(bh2)], i, j, kh);
+        }
+      }
+    }
+
+    // ---- E of step n+1 on plane k (Advance_Ex/Ey/Ez, AdvanceelectricCPML,
+    // AdvanceEDispersiveE, AdvancePlaneWaveE, AdvanceNodalE)
     float ex_n = 0.0f, ey_n = 0.0f, ez_n = 0.0f;
-    float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f;
-    int mhx = 0, mhy = 0, mhz = 0;
-    if (k <= nz) {
-      // ---- issue the loads of plane k
-      hx0 = gi.f[3][p]; hy0 = gi.f[4][p]; hz0 = gi.f[5][p];
-      const float hz_jm = gi.f[5][p - px], hx_jm = gi.f[3][p - px];
-      const float hz_im = gi.f[5][p - 1], hy_im = gi.f[4][p - 1];
-      ex_n = gi.f[0][p]; ey_n = gi.f[1][p]; ez_n = gi.f[2][p];
-      const int mx = gi.mi[0][p], my = gi.mi[1][p], mz = gi.mi[2][p];
-      mhx = gi.mi[3][p]; mhy = gi.mi[4][p]; mhz = gi.mi[5][p];
-      const float idzh = gi.id[5][k + 1];
-      int qzn = 0;
-      const bool szn = slab_node(k, nz, gi.L[2][0], gi.L[2][1], qzn);
-      const bool wr = own && k < k1;
-      const bool pw = SRC && (pwxy_e || step_pw_z(sd, 0, k));
-      const bool nd = SRC && step_nd(sd, 0, x0, j, k);
-      // ---- E of step n+1 (Advance_Ex/Ey/Ez, AdvanceelectricCPML,
-      // AdvanceEDispersiveE, AdvancePlaneWaveE, AdvanceNodalE)
-      if (in && i < nx) {
+    if (eplane) {
+      nhx = th.gg[mhx]; nhy = th.gg[mhy]; nhz = th.gg[mhz];
+      if (ade_h) { nsx = th.slot[mhx]; nsy = th.slot[mhy]; nsz = th.slot[mhz]; }
+      const bool nd = SRC && ndxy_e && k >= sd.cellbox[0][2] && k <= sd.cellbox[0][5];
+      {
         const float2 c = te.gg[mx];
-        float e = yee_upd(c.x, ex_n, c.y, curl2(hz0 - hz_jm, idyh, hy0 - hy_km, idzh));
-        if (syn)
-          e = madd(e, c.y, psi_next(gi.psi[PSI_EXY], go.psi[PSI_EXY], wr,
-                                    ((long long)k * Ly + qyn) * nx + i, gi.pbe[1][j + 1],
-                                    hz0 - hz_jm, gi.pce[1][j + 1]));
-        if (szn)
-          e = madd(e, -c.y, psi_next(gi.psi[PSI_EXZ], go.psi[PSI_EXZ], wr,
-                                     ((long long)qzn * (ny + 1) + j) * nx + i, gi.pbe[2][k + 1],
-                                     hy0 - hy_km, gi.pce[2][k + 1]));
-        if (ade_e) {
-          const int sl = te.slot[mx];
-          if (sl >= 0) e = step_poles<0>(gi.ade[0], go.ade[0], wr, dt, te, sl, i, j, k, e);
+        float e = yee_upd(c.x, ex0, c.y, curl2(hz0 - hz_jm, idyh, hy0 - hy_km, idzh));
+        if (CPML) {
+          if (f0)
+            e = madd(e, c.y, psi_new(pv0, (a->psi_o[PSI_EXY] + k * Ly * nx) + (unsigned)(qyn * nx + i),
+                                     wr, bey, hz0 - hz_jm, cey));
+          if (f1)
+            e = madd(e, -c.y, psi_new(pv1, (a->psi_o[PSI_EXZ] + (long long)qzn * (ny + 1) * nx) +
+                                               (unsigned)(j * nx + i),
+                                      wr, bez, hy0 - hy_km, cez));
         }
-        if (pw || nd) e = step_sources(sd, 0, 0, p, i, j, k, e, pw, nd, n);
+        if (g0) {
+          const int s = te.slot[mx];
+          if (s >= 0) e = pole_new(a->ai[0], a->ao[0], 0, z0, axe[0], wr, a->dt, te, s, ep0, aj0, e);
+        }
+        if (SRC) {
+          if (be0) e = faces_apply(lf, be0, ie0, sd.inc, sd.pwG[0], i, j, k, e);
+          if (nd && ex_on)
+            e = step_nodal(sd.src, 0, 0, P + (jc + 1) * (long long)gi.px + kXOff + ic, e, n);
+        }
         ex_n = e;
-        if (wr) go.f[0][p] = e;
+        if (wr && ex_on) st_at(plane_base(Exo, P), ob, e);
       }
-      if (in && j < ny) {
+      {
         const float2 c = te.gg[my];
-        float e = yee_upd(c.x, ey_n, c.y, curl2(hx0 - hx_km, idzh, hz0 - hz_im, idxh));
-        if (szn)
-          e = madd(e, c.y, psi_next(gi.psi[PSI_EYZ], go.psi[PSI_EYZ], wr,
-                                    ((long long)qzn * ny + j) * (nx + 1) + i, gi.pbe[2][k + 1],
-                                    hx0 - hx_km, gi.pce[2][k + 1]));
-        if (sxn)
-          e = madd(e, -c.y, psi_next(gi.psi[PSI_EYX], go.psi[PSI_EYX], wr,
-                                     ((long long)k * ny + j) * Lx + qxn, gi.pbe[0][i + 1],
-                                     hz0 - hz_im, gi.pce[0][i + 1]));
-        if (ade_e) {
-          const int sl = te.slot[my];
-          if (sl >= 0) e = step_poles<1>(gi.ade[0], go.ade[0], wr, dt, te, sl, i, j, k, e);
+        float e = yee_upd(c.x, ey0, c.y, curl2(hx0 - hx_km, idzh, hz0 - hz_im, idxh));
+        if (CPML) {
+          if (f2)
+            e = madd(e, c.y, psi_new(pv2, (a->psi_o[PSI_EYZ] + (long long)qzn * ny * (nx + 1)) +
+                                              (unsigned)(j * (nx + 1) + i),
+                                     wr, bez, hx0 - hx_km, cez));
+          if (f3)
+            e = madd(e, -c.y, psi_new(pv3, (a->psi_o[PSI_EYX] + k * ny * Lx) + (unsigned)(j * Lx + qxn),
+                                      wr, bex, hz0 - hz_im, cex));
         }
-        if (pw || nd) e = step_sources(sd, 0, 1, p, i, j, k, e, pw, nd, n);
+        if (g1) {
+          const int s = te.slot[my];
+          if (s >= 0) e = pole_new(a->ai[0], a->ao[0], 1, z1, axe[1], wr, a->dt, te, s, ep1, aj1, e);
+        }
+        if (SRC) {
+          if (be1) e = faces_apply(lf, be1, ie1, sd.inc, sd.pwG[0], i, j, k, e);
+          if (nd && ey_on)
+            e = step_nodal(sd.src, 0, 1, P + (jc + 1) * (long long)gi.px + kXOff + ic, e, n);
+        }
         ey_n = e;
-        if (wr) go.f[1][p] = e;
+        if (wr && ey_on) st_at(plane_base(Eyo, P), ob, e);
       }
-      if (in && k < nz) {
+      {
         const float2 c = te.gg[mz];
-        float e = yee_upd(c.x, ez_n, c.y, curl2(hy0 - hy_im, idxh, hx0 - hx_jm, idyh));
-        if (syn)
-          e = madd(e, -c.y, psi_next(gi.psi[PSI_EZY], go.psi[PSI_EZY], wr,
-                                     ((long long)k * Ly + qyn) * (nx + 1) + i, gi.pbe[1][j + 1],
-                                     hx0 - hx_jm, gi.pce[1][j + 1]));
-        if (sxn)
-          e = madd(e, c.y, psi_next(gi.psi[PSI_EZX], go.psi[PSI_EZX], wr,
-                                    ((long long)k * (ny + 1) + j) * Lx + qxn, gi.pbe[0][i + 1],
-                                    hy0 - hy_im, gi.pce[0][i + 1]));
-        if (ade_e) {
-          const int sl = te.slot[mz];
-          if (sl >= 0) e = step_poles<2>(gi.ade[0], go.ade[0], wr, dt, te, sl, i, j, k, e);
+        float e = yee_upd(c.x, ez0, c.y, curl2(hy0 - hy_im, idxh, hx0 - hx_jm, idyh));
+        if (CPML) {
+          if (f4)
+            e = madd(e, -c.y, psi_new(pv4, (a->psi_o[PSI_EZY] + k * Ly * (nx + 1)) +
+                                               (unsigned)(qyn * (nx + 1) + i),
+                                      wr, bey, hx0 - hx_jm, cey));
+          if (f5)
+            e = madd(e, c.y, psi_new(pv5, (a->psi_o[PSI_EZX] + k * (ny + 1) * Lx) + (unsigned)(j * Lx + qxn),
+                                     wr, bex, hy0 - hy_im, cex));
         }
-        if (pw || nd) e = step_sources(sd, 0, 2, p, i, j, k, e, pw, nd, n);
+        if (g2) {
+          const int s = te.slot[mz];
+          if (s >= 0) e = pole_new(a->ai[0], a->ao[0], 2, z2, axe[2], wr, a->dt, te, s, ep2, aj2, e);
+        }
+        if (SRC) {
+          if (be2) e = faces_apply(lf, be2, ie2, sd.inc, sd.pwG[0], i, j, k, e);
+          if (nd && ez_on)
+            e = step_nodal(sd.src, 0, 2, P + (jc + 1) * (long long)gi.px + kXOff + ic, e, n);
+        }
         ez_n = e;
-        if (wr) go.f[2][p] = e;
+        if (wr && ez_on) st_at(plane_base(Ezo, P), ob, e);
       }
     }
     // ---- E of plane k at x + 1 (next lane) and y + 1 (next row)
@@ -1207,67 +1530,77 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
     // ---- H of step n+1/2 on plane k - 1 (Advance_Hx/Hy/Hz,
     // AdvanceMagneticCPML, AdvanceMdispersiveH, AdvancePlaneWaveH,
     // AdvanceNodalH)
-    if (k > k0 && own) {
-      const int kh = k - 1;
-      const long long ph = p - pxy;
-      const float idze = gi.id[2][kh + 1];
-      int qzc = 0;
-      const bool szc = slab_cell(kh, nz, gi.L[2][0], gi.L[2][1], qzc);
-      const bool pw = SRC && (pwxy_h || step_pw_z(sd, 1, kh));
-      const bool nd = SRC && step_nd(sd, 1, x0, j, kh);
-      if (j < ny && kh < nz) {
-        const float2 c = th.gg[mhx_c];
+    if (hk) {
+      const bool nd = SRC && ndxy_h && kh >= sd.cellbox[1][2] && kh <= sd.cellbox[1][5];
+      {
+        const float2 c = chx;
         float h = yee_upd(c.x, hx_c, c.y, curl2(ey_n - ey_c, idze, ez_jp_c - ez_c, idye));
-        if (syc)
-          h = madd(h, -c.y, psi_next(gi.psi[PSI_HXY], go.psi[PSI_HXY], true,
-                                     ((long long)kh * Ly + qyc) * (nx + 1) + i,
-                                     gi.pbm[1][j + 1], ez_jp_c - ez_c, gi.pcm[1][j + 1]));
-        if (szc)
-          h = madd(h, c.y, psi_next(gi.psi[PSI_HXZ], go.psi[PSI_HXZ], true,
-                                    ((long long)qzc * ny + j) * (nx + 1) + i, gi.pbm[2][kh + 1],
-                                    ey_n - ey_c, gi.pcm[2][kh + 1]));
-        if (ade_h) {
-          const int sl = th.slot[mhx_c];
-          if (sl >= 0) h = step_poles<0>(gi.ade[1], go.ade[1], true, dt, th, sl, i, j, kh, h);
+        if (CPML) {
+          if (r0)
+            h = madd(h, -c.y, psi_new(qv0, (a->psi_o[PSI_HXY] + kh * Ly * (nx + 1)) +
+                                               (unsigned)(qyc * (nx + 1) + i),
+                                      true, bmy, ez_jp_c - ez_c, cmy));
+          if (r1)
+            h = madd(h, c.y, psi_new(qv1, (a->psi_o[PSI_HXZ] + (long long)qzc * ny * (nx + 1)) +
+                                              (unsigned)(j * (nx + 1) + i),
+                                     true, bmz, ey_n - ey_c, cmz));
         }
-        if (pw || nd) h = step_sources(sd, 1, 3, ph, i, j, kh, h, pw, nd, n);
-        go.f[3][ph] = h;
+        if (u0) {
+          const int s = shx;
+          if (s >= 0) h = pole_new(a->ai[1], a->ao[1], 0, w0, axh[0], true, a->dt, th, s, hp0, bj0, h);
+        }
+        if (SRC) {
+          if (bh0) h = faces_apply(lf + np0, bh0, ih0, sd.inc, sd.pwG[1], i, j, kh, h);
+          if (nd && hxk)
+            h = step_nodal(sd.src, 1, 3, Ph + (jc + 1) * (long long)gi.px + kXOff + ic, h, n);
+        }
+        if (hxk) st_at(plane_base(Hxo, Ph), ob, h);
       }
-      if (i < nx && kh < nz) {
-        const float2 c = th.gg[mhy_c];
+      {
+        const float2 c = chy;
         float h = yee_upd(c.x, hy_c, c.y, curl2(ez_ip_c - ez_c, idxe, ex_n - ex_c, idze));
-        if (szc)
-          h = madd(h, -c.y, psi_next(gi.psi[PSI_HYZ], go.psi[PSI_HYZ], true,
-                                     ((long long)qzc * (ny + 1) + j) * nx + i,
-                                     gi.pbm[2][kh + 1], ex_n - ex_c, gi.pcm[2][kh + 1]));
-        if (sxc)
-          h = madd(h, c.y, psi_next(gi.psi[PSI_HYX], go.psi[PSI_HYX], true,
-                                    ((long long)kh * (ny + 1) + j) * Lx + qxc, gi.pbm[0][i + 1],
-                                    ez_ip_c - ez_c, gi.pcm[0][i + 1]));
-        if (ade_h) {
-          const int sl = th.slot[mhy_c];
-          if (sl >= 0) h = step_poles<1>(gi.ade[1], go.ade[1], true, dt, th, sl, i, j, kh, h);
+        if (CPML) {
+          if (r2)
+            h = madd(h, -c.y, psi_new(qv2, (a->psi_o[PSI_HYZ] + (long long)qzc * (ny + 1) * nx) +
+                                               (unsigned)(j * nx + i),
+                                      true, bmz, ex_n - ex_c, cmz));
+          if (r3)
+            h = madd(h, c.y, psi_new(qv3, (a->psi_o[PSI_HYX] + kh * (ny + 1) * Lx) +
+                                              (unsigned)(j * Lx + qxc),
+                                     true, bmx, ez_ip_c - ez_c, cmx));
         }
-        if (pw || nd) h = step_sources(sd, 1, 4, ph, i, j, kh, h, pw, nd, n);
-        go.f[4][ph] = h;
+        if (u1) {
+          const int s = shy;
+          if (s >= 0) h = pole_new(a->ai[1], a->ao[1], 1, w1, axh[1], true, a->dt, th, s, hp1, bj1, h);
+        }
+        if (SRC) {
+          if (bh1) h = faces_apply(lf + np0, bh1, ih1, sd.inc, sd.pwG[1], i, j, kh, h);
+          if (nd && hyk)
+            h = step_nodal(sd.src, 1, 4, Ph + (jc + 1) * (long long)gi.px + kXOff + ic, h, n);
+        }
+        if (hyk) st_at(plane_base(Hyo, Ph), ob, h);
       }
-      if (i < nx && j < ny) {
-        const float2 c = th.gg[mhz_c];
+      {
+        const float2 c = chz;
         float h = yee_upd(c.x, hz_c, c.y, curl2(ex_jp_c - ex_c, idye, ey_ip_c - ey_c, idxe));
-        if (syc)
-          h = madd(h, c.y, psi_next(gi.psi[PSI_HZY], go.psi[PSI_HZY], true,
-                                    ((long long)kh * Ly + qyc) * nx + i, gi.pbm[1][j + 1],
-                                    ex_jp_c - ex_c, gi.pcm[1][j + 1]));
-        if (sxc)
-          h = madd(h, -c.y, psi_next(gi.psi[PSI_HZX], go.psi[PSI_HZX], true,
-                                     ((long long)kh * ny + j) * Lx + qxc, gi.pbm[0][i + 1],
-                                     ey_ip_c - ey_c, gi.pcm[0][i + 1]));
-        if (ade_h) {
-          const int sl = th.slot[mhz_c];
-          if (sl >= 0) h = step_poles<2>(gi.ade[1], go.ade[1], true, dt, th, sl, i, j, kh, h);
+        if (CPML) {
+          if (r4)
+            h = madd(h, c.y, psi_new(qv4, (a->psi_o[PSI_HZY] + kh * Ly * nx) + (unsigned)(qyc * nx + i),
+                                     true, bmy, ex_jp_c - ex_c, cmy));
+          if (r5)
+            h = madd(h, -c.y, psi_new(qv5, (a->psi_o[PSI_HZX] + kh * ny * Lx) + (unsigned)(j * Lx + qxc),
+                                      true, bmx, ey_ip_c - ey_c, cmx));
         }
-        if (pw || nd) h = step_sources(sd, 1, 5, ph, i, j, kh, h, pw, nd, n);
-        go.f[5][ph] = h;
+        if (u2) {
+          const int s = shz;
+          if (s >= 0) h = pole_new(a->ai[1], a->ao[1], 2, w2, axh[2], true, a->dt, th, s, hp2, bj2, h);
+        }
+        if (SRC) {
+          if (bh2) h = faces_apply(lf + np0, bh2, ih2, sd.inc, sd.pwG[1], i, j, kh, h);
+          if (nd && hzk)
+            h = step_nodal(sd.src, 1, 5, Ph + (jc + 1) * (long long)gi.px + kXOff + ic, h, n);
+        }
+        if (hzk) st_at(plane_base(Hzo, Ph), ob, h);
       }
     }
     // ---- carry plane k to the next iteration
@@ -1275,33 +1608,45 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
     hx_c = hx0; hy_c = hy0; hz_c = hz0;
     ex_c = ex_n; ey_c = ey_n; ez_c = ez_n;
     ey_ip_c = ey_ip_n; ez_ip_c = ez_ip_n; ex_jp_c = ex_jp_n; ez_jp_c = ez_jp_n;
-    mhx_c = mhx; mhy_c = mhy; mhz_c = mhz;
+    chx = nhx; chy = nhy; chz = nhz;
+    shx = nsx; shy = nsy; shz = nsz;
   }
 }
 
-size_t step_lds_bytes(const DevGrid &g, int rows) {
-  return step_tab_bytes(g.nmedia, g.ade[0].ndisp) + step_tab_bytes(g.nmedia, g.ade[1].ndisp) +
-         (size_t)2 * rows * kWave * sizeof(float2);
+size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc) {
+  const size_t x = step_tab_bytes(g.nmedia, g.ade[0].ndisp) +
+                   step_tab_bytes(g.nmedia, g.ade[1].ndisp) +
+                   (size_t)2 * rows * kWave * sizeof(float2);
+  return npw ? step_lds_faces(x, npw, kc, nullptr, nullptr) : x;
 }
 
-int launch_step(const StepDev &sd, int rows, void *stream) {
-  const size_t lds = step_lds_bytes(sd.gi, rows);
+int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
+  const size_t lds = step_lds_bytes(sd.gi, rows, sd.npw[0] + sd.npw[1], sd.tm.kc);
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)sd.tm.nblk);
-  const bool ade = sd.gi.ade[0].has || sd.gi.ade[1].has;
-  const bool src = sd.npw[0] || sd.npw[1] || sd.ncell[0] || sd.ncell[1];
-#define SFD_STEP(RR, A, S) \
-  hipLaunchKernelGGL((k_step<RR, A, S>), grid, dim3(kWave, RR), lds, s, sd)
-#define SFD_STEP_R(RR)                     \
-  do {                                     \
-    if (ade && src) SFD_STEP(RR, true, true);   \
-    else if (ade) SFD_STEP(RR, true, false);    \
-    else if (src) SFD_STEP(RR, false, true);    \
-    else SFD_STEP(RR, false, false);            \
-  } while (0)
-  if (rows == 4) SFD_STEP_R(4);
-  else if (rows == 8) SFD_STEP_R(8);
-  else return (int)hipErrorInvalidValue;
+  if (sd.tm.nblk <= 0) return 0;
+#define SFD_STEP(RR, C, A, S) \
+  hipLaunchKernelGGL((k_step<RR, C, A, S>), grid, dim3(kWave, RR), lds, s, sd)
+#define SFD_STEP_R(RR)                                         \
+  switch (cls & 7) {                                           \
+    case 0: SFD_STEP(RR, false, false, false); break;          \
+    case 1: SFD_STEP(RR, true, false, false); break;           \
+    case 2: SFD_STEP(RR, false, true, false); break;           \
+    case 3: SFD_STEP(RR, true, true, false); break;            \
+    case 4: SFD_STEP(RR, false, false, true); break;           \
+    case 5: SFD_STEP(RR, true, false, true); break;            \
+    case 6: SFD_STEP(RR, false, true, true); break;            \
+    default: SFD_STEP(RR, true, true, true); break;            \
+  }
+  if (rows == 4) {
+    SFD_STEP_R(4)
+  } else if (rows == 8) {
+    SFD_STEP_R(8)
+  } else if (rows == 16) {
+    SFD_STEP_R(16)
+  } else {
+    return (int)hipErrorInvalidValue;
+  }
 #undef SFD_STEP_R
 #undef SFD_STEP
   return (int)hipGetLastError();

@@ -409,7 +409,7 @@ class Solver:
         return r.value, k.value, x.value
 
     def set_step_mode(self, rows: int, zchunk: int = 32):
-        """One launch per step in run() (rows 4 or 8 per block, 0: two
+        """One launch per step in run() (rows 4, 8 or 16 per block, 0: two
         passes per step).  Needs fp32 storage; holds a second copy of the
         time-stepping state."""
         self._ck(self.lib.fdtd_set_step_mode(self.h, rows, zchunk), "fdtd_set_step_mode")

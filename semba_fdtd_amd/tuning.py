@@ -30,7 +30,7 @@ StepShape = Tuple[int, int]            # rows, zchunk (rows 0: two passes)
 
 DEFAULT_CANDIDATES: Tuple[Tiling, ...] = tuple(
     (r, k, x) for x in (1, 4) for r in (1, 2, 4) for k in (16, 32, 64))
-STEP_CANDIDATES: Tuple[StepShape, ...] = ((4, 16), (4, 32), (8, 16), (8, 32), (8, 64))
+STEP_CANDIDATES: Tuple[StepShape, ...] = ((4, 16), (8, 16), (8, 32), (16, 16), (16, 32))
 
 _cache: Dict[str, tuple] = {}
 

@@ -29,7 +29,7 @@ CASES = {
     "mur_walls": lambda: P.scatterer_problem(nsteps=90, L=((0, 0), (0, 0), (0, 0)),
                                              mur=(1, 1, 1, 1, 1, 1)),
 }
-SHAPES = [(4, 32), (8, 7), (4, 3), (8, 64)]
+SHAPES = [(4, 32), (8, 7), (4, 3), (8, 64), (16, 5)]
 
 
 def _bits(a):
