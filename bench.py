@@ -72,11 +72,25 @@ def step_algorithmic_bytes(pb, ndisp_edges, mode, stored_poles):
     return 84 * pb.cells + 8 * (pe + ph) + ndisp_edges * (8 + per_state * stored_poles)
 
 
+def onelaunch_step_bytes(pb, ndisp_edges, mode, stored_poles):
+    """Compulsory HBM bytes of one whole step when E and H of a plane are
+    updated in the same launch: each field read once and written once, each
+    media index read once (60 B per cell), 8 B per psi value and per
+    dispersive edge 8 B of Eprev plus the pole state."""
+    pe, ph = psi_counts(pb)
+    per_state = 16 if mode == "fast" else 32
+    return 60 * pb.cells + 8 * (pe + ph) + ndisp_edges * (8 + per_state * stored_poles)
+
+
+CLASS_NAMES = {0: "plain", 1: "cpml", 2: "poles", 3: "cpml+poles", 4: "sources",
+               5: "cpml+sources", 6: "poles+sources", 7: "cpml+poles+sources"}
+
 KERNELS = {
     "e_face": "k_epass (Yee E + CPML E, fused face pass)",
     "ade": "k_ade_box (pole-residue ADE over the dispersive edges)",
     "h": "k_hpass (Yee H + CPML H, fused face pass)",
-    "step": "k_step (one launch per step: Yee E and H, CPML, poles, TF/SF faces inline)",
+    "step_plain": "k_step<R, false, false, false> (one launch per step, interior tiles: "
+                  "Yee E and H of each plane; 60 B per owned cell)",
 }
 
 
@@ -279,14 +293,27 @@ def main():
     ms_k["e_face"] = ms_e - ms_k["ade"]
     stored = 1 if args.mode == "fast" else 2
     nbytes = algorithmic_bytes(pb, ndisp, args.mode, stored)
-    if s.get_step_mode()[0]:
-        # one launch per step: the step kernel alone, algorithmic bytes of the
-        # whole step
-        ms_k["step"] = s.time_step(reps)
-        nbytes["step"] = step_algorithmic_bytes(pb, ndisp, args.mode, stored)
+    onelaunch = bool(s.get_step_mode()[0])
+    classes = None
+    if onelaunch:
+        # one launch per step: each tile class's kernel timed alone; the
+        # interior (plain) class moves exactly 60 B per owned cell
+        ms_k = {}
+        classes = []
+        for cls, cells, ms_c in s.step_classes(reps):
+            key = "step_" + CLASS_NAMES[cls].replace("+", "_")
+            ms_k[key] = ms_c
+            nbytes[key] = int(60 * cells)
+            classes.append({"class": CLASS_NAMES[cls], "cells": int(cells), "ms": round(ms_c, 4)})
+        dom = max(ms_k, key=ms_k.get)
+        if dom != "step_plain":
+            KERNELS[dom] = (f"k_step tile class {dom[5:]} (60 B per owned cell counted: a lower "
+                            f"bound of its compulsory bytes)")
+        step_bytes = onelaunch_step_bytes(pb, ndisp, args.mode, stored)
+    else:
+        dom = max(ms_k, key=ms_k.get)
+        step_bytes = step_algorithmic_bytes(pb, ndisp, args.mode, stored)
     gbs = {k: nbytes[k] / (ms_k[k] / 1e3) / 1e9 if ms_k[k] > 0 else 0.0 for k in ms_k}
-    dom = "step" if "step" in ms_k else max(ms_k, key=ms_k.get)
-    step_bytes = step_algorithmic_bytes(pb, ndisp, args.mode, stored)
     step_gbs = step_bytes / (ms_per_step / 1e3) / 1e9
     probes = s.probes()
     finite = bool(np.isfinite(probes).all())
@@ -335,6 +362,8 @@ def main():
             "cells": pb.cells,
             "dispersive_edges": ndisp,
             "tiling": tiling,
+            "one_launch_step": onelaunch,
+            "step_classes": classes,
             "step_algorithmic_bytes": step_bytes,
             "step_algorithmic_gbs": round(step_gbs, 1),
             "step_hbm_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),

@@ -179,9 +179,20 @@ int fdtd_time_halves(fdtd_handle *h, int32_t reps, float ms[3]);
  * only.  No counterpart in the reference. */
 int fdtd_set_step_mode(fdtd_handle *h, int32_t rows, int32_t zchunk);
 int fdtd_get_step_mode(const fdtd_handle *h, int32_t *rows, int32_t *zchunk);
-/* Average ms of one one-launch step (the step kernel alone), over reps pairs;
- * the fields advance. */
+/* Average ms of one one-launch step (the step kernels alone), over reps
+ * pairs; the fields advance. */
 int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms);
+/* The tile classes of the one-launch step: cls[q] has bit 0 set for tiles
+ * that touch a CPML slab, bit 1 for tiles that meet a pole-state box, bit 2
+ * for tiles that touch a TF/SF face or a nodal cell; each class runs its own
+ * kernel.  cells[q]: the Yee cells (nx*ny*nz grid) the class's tiles own.
+ * With reps > 0, ms[q] is the average duration of one launch of the class's
+ * kernel over reps launches of the first half of a step pair (the state of
+ * step n is read and the second copy rewritten; the grid does not advance).
+ * Returns the number of classes (<= 8) or a negative code.  No counterpart in
+ * the reference; used by the benchmark. */
+int fdtd_step_classes(fdtd_handle *h, int32_t reps, int32_t cls[8], double cells[8],
+                      float ms[8]);
 
 /* ---- the hot path ----------------------------------------------------- */
 /* Fused E half step: Advance_Ex/Ey/Ez (timestepping.F90:1286-1297) +

@@ -145,6 +145,7 @@ struct fdtd_handle {
   // tiles grouped by class (bit 0 CPML, 1 poles, 2 sources): one launch per
   // class with tiles, each over its slice of the device tile list
   std::vector<int> step_cls, step_cnt, step_off;
+  std::vector<double> step_cells;        // Yee cells owned per class
   int *dtiles = nullptr;
   // TF/SF faces of both halves for the step kernels and the buffer of their
   // incident field, refreshed before every step

@@ -507,6 +507,7 @@ static int build_step(fdtd_handle *h) {
     }
     const int nn[3] = {h->g.nx, h->g.ny, h->g.nz};
     std::vector<std::vector<int>> lists(8);
+    std::vector<double> owned(8, 0.0);
     for (int bz = 0; bz < tm.tz; ++bz)
       for (int by = 0; by < tm.ty; ++by)
         for (int bx = 0; bx < tm.tx; ++bx) {
@@ -529,12 +530,18 @@ static int build_step(fdtd_handle *h) {
           if (meets(ade)) cls |= 2;
           if (meets(srcb)) cls |= 4;
           lists[cls].push_back((bz * tm.ty + by) * tm.tx + bx);
+          // Yee cells the tile owns: 63 columns, R - 1 rows, its planes
+          const long long ox = std::min(lo[0] + kWave - 2, nn[0] - 1) - lo[0] + 1;
+          const long long oy = std::min(lo[1] + R - 2, nn[1] - 1) - lo[1] + 1;
+          const long long oz = std::min(hi[2] - 1, nn[2] - 1) - lo[2] + 1;
+          if (ox > 0 && oy > 0 && oz > 0) owned[cls] += (double)(ox * oy * oz);
         }
     std::vector<int> all;
-    h->step_cls.clear(); h->step_cnt.clear(); h->step_off.clear();
+    h->step_cls.clear(); h->step_cnt.clear(); h->step_off.clear(); h->step_cells.clear();
     for (int c = 0; c < 8; ++c) {
       if (lists[c].empty()) continue;
       h->step_cls.push_back(c);
+      h->step_cells.push_back(owned[c]);
       h->step_cnt.push_back((int)lists[c].size());
       h->step_off.push_back((int)all.size());
       all.insert(all.end(), lists[c].begin(), lists[c].end());
@@ -1677,6 +1684,58 @@ int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms) {
   hipEventDestroy(ev[0]);
   hipEventDestroy(ev[1]);
   return rc;
+}
+
+int fdtd_step_classes(fdtd_handle *h, int32_t reps, int32_t cls[8], double cells[8],
+                      float ms[8]) {
+  if (!h || reps < 0 || !cls || !cells || (reps > 0 && !ms)) return FDTD_ERR_ARG;
+  if (h->step_rows == 0) return fail(h, FDTD_ERR_STATE, "one-launch steps are off");
+  hipSetDevice(h->device);
+  int rc;
+  if ((rc = prepare(h)) || (rc = build_step(h)) || (rc = sync_b(h))) return rc;
+  const int nc = (int)h->step_cls.size();
+  for (int q = 0; q < nc; ++q) {
+    cls[q] = h->step_cls[q];
+    cells[q] = h->step_cells[q];
+  }
+  if (reps == 0) return nc;
+  hipEvent_t ev[2];
+  if (hipEventCreate(&ev[0]) != hipSuccess) return fail(h, FDTD_ERR_HIP, "hipEventCreate failed");
+  if (hipEventCreate(&ev[1]) != hipSuccess) {
+    hipEventDestroy(ev[0]);
+    return fail(h, FDTD_ERR_HIP, "hipEventCreate failed");
+  }
+  // the incident-field buffer of the faces is filled once for step n
+  if (h->step_nf > 0 &&
+      launch_pw_incident(h->dsfaces, h->step_nf0, h->step_nf, h->step_maxcells, h->dstepsrc,
+                         h->dstep, h->dinc, h->stream))
+    rc = fail(h, FDTD_ERR_HIP, "incident-field launch failed");
+  for (int q = 0; q < nc && !rc; ++q) {
+    StepDev sd = h->sd[0];
+    sd.tiles = h->dtiles + h->step_off[q];
+    sd.tm.nblk = h->step_cnt[q];
+    auto one = [&]() {
+      return h->mode == FDTD_MODE_EXACT ? exact::launch_step(sd, h->step_rows, h->step_cls[q], h->stream)
+                                        : fast::launch_step(sd, h->step_rows, h->step_cls[q], h->stream);
+    };
+    if (one()) { rc = fail(h, FDTD_ERR_HIP, "step launch failed"); break; }
+    hipEventRecord(ev[0], h->stream);
+    for (int r = 0; r < reps && !rc; ++r)
+      if (one()) rc = fail(h, FDTD_ERR_HIP, "step launch failed");
+    hipEventRecord(ev[1], h->stream);
+    if (rc) break;
+    if (hipEventSynchronize(ev[1]) != hipSuccess) {
+      rc = fail(h, FDTD_ERR_HIP, "timing events failed");
+      break;
+    }
+    float t = 0.f;
+    hipEventElapsedTime(&t, ev[0], ev[1]);
+    ms[q] = t / reps;
+  }
+  hipEventDestroy(ev[0]);
+  hipEventDestroy(ev[1]);
+  // only the second copy was written: the grid's state (A) is unchanged
+  return rc ? rc : nc;
 }
 
 int fdtd_set_probes(fdtd_handle *h, int32_t nprobe, const int32_t *comp, const int32_t *i,

@@ -44,7 +44,7 @@ module sembafdtd_gpu
              fdtd_batch_create, fdtd_batch_destroy, fdtd_batch_admit, fdtd_batch_retire, &
              fdtd_batch_step_of, fdtd_batch_set_tiling, fdtd_batch_step, &
              fdtd_batch_synchronize, fdtd_set_step_mode, fdtd_get_step_mode, fdtd_time_step, &
-             fdtd_error_message
+             fdtd_step_classes, fdtd_error_message
 
    interface
       integer(c_int) function fdtd_create(desc, h) bind(C, name="fdtd_create")
@@ -392,6 +392,15 @@ module sembafdtd_gpu
          type(c_ptr), value :: h
          integer(c_int32_t), value :: reps
          real(c_float), intent(out) :: ms
+      end function
+      integer(c_int) function fdtd_step_classes(h, reps, cls, cells, ms) &
+            bind(C, name="fdtd_step_classes")
+         import :: c_int, c_ptr, c_int32_t, c_float, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: reps
+         integer(c_int32_t), intent(out) :: cls(8)
+         real(c_double), intent(out) :: cells(8)
+         real(c_float), intent(out) :: ms(8)
       end function
       ! ---- frequency-domain observation (observation.F90:3527-3600)
       integer(c_int) function fdtd_add_dft(h, kind, box, stride, nfreq, freq, id) &

@@ -414,6 +414,16 @@ class Solver:
         time-stepping state."""
         self._ck(self.lib.fdtd_set_step_mode(self.h, rows, zchunk), "fdtd_set_step_mode")
 
+    def step_classes(self, reps: int = 0):
+        """The tile classes of the one-launch step: [(class bits, owned Yee
+        cells, ms per launch or None)], bits 1 CPML, 2 poles, 4 sources."""
+        cls = (C.c_int32 * 8)()
+        cells = (C.c_double * 8)()
+        ms = (C.c_float * 8)()
+        n = self.lib.fdtd_step_classes(self.h, reps, cls, cells, ms)
+        self._ck(min(n, 0), "fdtd_step_classes")
+        return [(int(cls[q]), float(cells[q]), float(ms[q]) if reps else None) for q in range(n)]
+
     def get_step_mode(self):
         r, k = C.c_int32(), C.c_int32()
         self._ck(self.lib.fdtd_get_step_mode(self.h, C.byref(r), C.byref(k)), "fdtd_get_step_mode")

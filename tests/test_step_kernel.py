@@ -136,3 +136,23 @@ def test_step_checkpoint_resume():
     for c in range(6):
         assert np.array_equal(_bits(c3.field(c)), _bits(a.field(c))), c
     assert np.array_equal(_bits(np.concatenate([head, c3.probes()])), _bits(a.probes()))
+
+
+@pytest.mark.parametrize("name", ["graded_one_sided", "planewave_oblique", "mur_walls"])
+def test_step_classes_cover_the_grid(name):
+    """The tile classes own every Yee cell once, timing them leaves the grid's
+    state alone, and the run after timing equals the run without it."""
+    pb = CASES[name]()
+    a = sf.Solver(pb, mode="fast")
+    a.set_step_mode(8, 7)
+    a.run(20)
+    cl = a.step_classes(reps=3)
+    assert sum(c[1] for c in cl) == pb.nx * pb.ny * pb.nz
+    assert len({c[0] for c in cl}) == len(cl) and all(c[2] > 0 for c in cl)
+    a.run(21)
+    b = sf.Solver(pb, mode="fast")
+    b.set_step_mode(8, 7)
+    b.run(41)
+    for c in range(6):
+        assert np.array_equal(_bits(a.field(c)), _bits(b.field(c))), c
+    assert np.array_equal(_bits(a.probes()), _bits(b.probes()))
