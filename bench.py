@@ -199,8 +199,13 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 1 GiB device copies (the known-size kernel that "
                          "scales the FETCH_SIZE/WRITE_SIZE counters) and exit")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="keep the default tiling instead of timing the candidates")
+    ap.add_argument("--tiling", default="1,16,1,16,16",
+                    help="work decomposition: rows,zchunk,xcells of the two passes and "
+                         "rows,zchunk of the one-launch step (step rows 0: two passes per "
+                         "step); the default is the fastest measured on this workload")
+    ap.add_argument("--tune", action="store_true",
+                    help="time the candidate decompositions on the grid and keep the fastest "
+                         "instead of --tiling")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1500)
@@ -244,15 +249,18 @@ def main():
     stream = torch.cuda.Stream()
     t0 = time.perf_counter()
     s = sf.Solver(pb, mode=args.mode, stream=stream.cuda_stream, probe_buffer=total)
-    tiling = None
-    if not args.no_tune:
+    if args.tune:
         # the grid is at rest: the source-free passes leave it at zero, so the
         # sweep needs no checkpoint
         from semba_fdtd_amd import tuning
         tuned = tuning.autotune(s, reps=3, keep_state=False)
-        tiling = {"passes": list(tuned["tiling"]), "step": list(tuned["step"])}
-        print(f"[bench] tiling {tiling} ({tuned['ms']:.2f} ms per step)", file=sys.stderr,
-              flush=True)
+        tiling = {"passes": list(tuned["tiling"]), "step": list(tuned["step"]), "tuned": True}
+    else:
+        t = [int(v) for v in args.tiling.split(",")]
+        s.set_tiling(t[0], t[1], t[2])
+        s.set_step_mode(t[3], t[4])
+        tiling = {"passes": t[:3], "step": t[3:5], "tuned": False}
+    print(f"[bench] tiling {tiling}", file=sys.stderr, flush=True)
     s.run(args.warmup)  # builds pole state, captures the graph
     s.synchronize()
     setup_s = time.perf_counter() - t0
