@@ -508,6 +508,8 @@ static int build_step(fdtd_handle *h) {
     const int nn[3] = {h->g.nx, h->g.ny, h->g.nz};
     std::vector<std::vector<int>> lists(8);
     std::vector<double> owned(8, 0.0);
+    const long long ntiles = (long long)tm.tx * tm.ty * tm.tz;
+    std::vector<unsigned char> tcls((size_t)ntiles);
     for (int bz = 0; bz < tm.tz; ++bz)
       for (int by = 0; by < tm.ty; ++by)
         for (int bx = 0; bx < tm.tx; ++bx) {
@@ -529,12 +531,35 @@ static int build_step(fdtd_handle *h) {
           };
           if (meets(ade)) cls |= 2;
           if (meets(srcb)) cls |= 4;
-          lists[cls].push_back((bz * tm.ty + by) * tm.tx + bx);
+          tcls[(size_t)((bz * (long long)tm.ty + by) * tm.tx + bx)] = (unsigned char)cls;
           // Yee cells the tile owns: 63 columns, R - 1 rows, its planes
           const long long ox = std::min(lo[0] + kWave - 2, nn[0] - 1) - lo[0] + 1;
           const long long oy = std::min(lo[1] + R - 2, nn[1] - 1) - lo[1] + 1;
           const long long oz = std::min(hi[2] - 1, nn[2] - 1) - lo[2] + 1;
           if (ox > 0 && oy > 0 && oz > 0) owned[cls] += (double)(ox * oy * oz);
+        }
+    // runs of plain chunks above each other merge into one tile of up to 64
+    // planes (fewer recomputed planes and block starts); the other classes
+    // are small and keep one chunk per block for parallelism (and tiles with
+    // sources must: their per-plane face masks span one chunk).
+    // Entry: base tile | (chunks << 24).
+    const int M = ntiles < (1LL << 24) ? std::max(1, std::min(64 / tm.kc, 127)) : 1;
+    sd.merged = M > 1 ? 1 : 0;
+    std::vector<unsigned char> covered((size_t)ntiles, 0);
+    for (int bz = 0; bz < tm.tz; ++bz)
+      for (int by = 0; by < tm.ty; ++by)
+        for (int bx = 0; bx < tm.tx; ++bx) {
+          const long long t = (bz * (long long)tm.ty + by) * tm.tx + bx;
+          if (covered[(size_t)t]) continue;
+          const int c = tcls[(size_t)t];
+          int len = 1;
+          if (c == 0)
+            while (len < M && bz + len < tm.tz &&
+                   tcls[(size_t)(t + (long long)len * tm.tx * tm.ty)] == c) {
+              covered[(size_t)(t + (long long)len * tm.tx * tm.ty)] = 1;
+              ++len;
+            }
+          lists[c].push_back(sd.merged ? (int)(t | ((long long)len << 24)) : (int)t);
         }
     std::vector<int> all;
     h->step_cls.clear(); h->step_cnt.clear(); h->step_off.clear(); h->step_cells.clear();
@@ -572,6 +597,7 @@ static int build_step(fdtd_handle *h) {
     const DevGrid &gi = dir ? h->gB : h->g;
     const DevGrid &go = dir ? h->g : h->gB;
     StepAux a{};
+    for (int c = 0; c < 6; ++c) { a.fi[c] = gi.f[c]; a.fo[c] = go.f[c]; a.mi[c] = gi.mi[c]; }
     for (int s = 0; s < 12; ++s) { a.psi_i[s] = gi.psi[s]; a.psi_o[s] = go.psi[s]; }
     for (int ax = 0; ax < 3; ++ax) {
       a.pbe[ax] = gi.pbe[ax]; a.pce[ax] = gi.pce[ax];

@@ -203,9 +203,14 @@ struct StepSrc {
   int nsrc;
 };
 
-// the CPML psi and pole state of one step direction, in device memory (read
-// by the slab and dispersive cells only)
+// the array bases of one step direction, in device memory: the march reads
+// them once per plane through a constant-space pointer instead of holding
+// them in scalar registers across the loop (fields, media, CPML psi, pole
+// state)
 struct StepAux {
+  const float *fi[6];         // fields of step n
+  float *fo[6];               // fields of step n+1
+  const int16_t *mi[6];       // media indices
   const float *psi_i[12];     // read copy
   float *psi_o[12];           // written copy
   const float *pbe[3], *pce[3], *pbm[3], *pcm[3];
@@ -221,6 +226,7 @@ struct StepDev {
   TileMap tm;                 // tx, ty, tz over strides 63 x (R-1) x kc;
                               // nblk: the tiles of this launch
   const int *tiles;           // tile numbers of this launch's class
+  int merged;                 // entries are tile | (z chunks << 24)
   // TF/SF faces per half (staged in LDS by the blocks that meet them), and
   // the bounding box of each half's nodal cells
   int npw[2];

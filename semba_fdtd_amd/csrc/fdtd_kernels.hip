@@ -1046,6 +1046,36 @@ __device__ __forceinline__ CStepAux *per_plane(const StepAux *p) {
   return q;
 }
 
+// the base of output field c: from the descriptor (CPML, ADE, SRC classes) or
+// the kernel argument (plain tiles)
+#define fo_at(a, go, c) ((CPML || ADE || SRC) ? (a)->fo[c] : (go).f[c])
+
+// the values one plane of the march loads for the E half of the plane
+struct PlaneLoad {
+  float hx, hy, hz, hz_jm, hx_jm, hz_im, hy_im, ex, ey, ez;
+  int mx, my, mz, mhx, mhy, mhz;
+};
+// FI, MI: the field and media array bases, either held in scalar registers
+// across the march (the plain tiles, which have registers to spare) or read
+// per plane from the step descriptor (the other classes, whose own state
+// would otherwise push them into spills)
+template <class FI, class MI>
+__device__ __forceinline__ void load_plane(PlaneLoad &L, const FI &fi, const MI &mi, long long P,
+                                           unsigned ob, unsigned obx, unsigned oby,
+                                           unsigned om) {
+  const SFD_GLOBAL float *hxk = plane_base(fi[3], P), *hyk = plane_base(fi[4], P),
+                         *hzk = plane_base(fi[5], P);
+  L.hx = ld_at(hxk, ob); L.hy = ld_at(hyk, ob); L.hz = ld_at(hzk, ob);
+  L.hz_jm = ld_at(hzk, oby); L.hx_jm = ld_at(hxk, oby);
+  L.hz_im = ld_at(hzk, obx); L.hy_im = ld_at(hyk, obx);
+  L.ex = ld_at(plane_base(fi[0], P), ob); L.ey = ld_at(plane_base(fi[1], P), ob);
+  L.ez = ld_at(plane_base(fi[2], P), ob);
+  L.mx = ld_at(plane_base(mi[0], P), om); L.my = ld_at(plane_base(mi[1], P), om);
+  L.mz = ld_at(plane_base(mi[2], P), om);
+  L.mhx = ld_at(plane_base(mi[3], P), om); L.mhy = ld_at(plane_base(mi[4], P), om);
+  L.mhz = ld_at(plane_base(mi[5], P), om);
+}
+
 // CPML psi of one correction: the new value from the step-n value old,
 // written to the other copy when wr
 __device__ __forceinline__ float psi_new(float old, float *po, bool wr, float b, float d,
@@ -1175,27 +1205,6 @@ static __host__ __device__ __forceinline__ size_t step_tab_bytes(int nmedia, int
          (size_t)nd * kPMax * 6 * sizeof(coef_t);
 }
 
-// the values one plane of the march loads for the E half of the plane
-struct PlaneLoad {
-  float hx, hy, hz, hz_jm, hx_jm, hz_im, hy_im, ex, ey, ez;
-  int mx, my, mz, mhx, mhy, mhz;
-};
-__device__ __forceinline__ void load_plane(PlaneLoad &L, const float *const *fi,
-                                           const int16_t *const *mi, long long P, unsigned ob,
-                                           unsigned obx, unsigned oby, unsigned om) {
-  const SFD_GLOBAL float *hxk = plane_base(fi[3], P), *hyk = plane_base(fi[4], P),
-                         *hzk = plane_base(fi[5], P);
-  L.hx = ld_at(hxk, ob); L.hy = ld_at(hyk, ob); L.hz = ld_at(hzk, ob);
-  L.hz_jm = ld_at(hzk, oby); L.hx_jm = ld_at(hxk, oby);
-  L.hz_im = ld_at(hzk, obx); L.hy_im = ld_at(hyk, obx);
-  L.ex = ld_at(plane_base(fi[0], P), ob); L.ey = ld_at(plane_base(fi[1], P), ob);
-  L.ez = ld_at(plane_base(fi[2], P), ob);
-  L.mx = ld_at(plane_base(mi[0], P), om); L.my = ld_at(plane_base(mi[1], P), om);
-  L.mz = ld_at(plane_base(mi[2], P), om);
-  L.mhx = ld_at(plane_base(mi[3], P), om); L.mhy = ld_at(plane_base(mi[4], P), om);
-  L.mhz = ld_at(plane_base(mi[5], P), om);
-}
-
 // Tiles are launched in classes: CPML, tiles that touch a CPML slab; ADE,
 // tiles that meet the pole-state box of some component; SRC, tiles that
 // touch a TF/SF face or a nodal cell.  A class's kernel carries only the code
@@ -1212,7 +1221,9 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
       gi, gi.gm1, gi.gm2, reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offh),
       &gi.ade[1]);
   float2 *xch = reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offx);
-  const int t = sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)];
+  const int tw = sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)];
+  const int t = sd.merged ? (tw & 0xffffff) : tw;
+  const int nch = sd.merged ? max(tw >> 24, 1) : 1;    // z chunks of this tile
   const int bx = t % sd.tm.tx, by = (t / sd.tm.tx) % sd.tm.ty, bz = t / (sd.tm.tx * sd.tm.ty);
   const int k0 = bz * sd.tm.kc;
   // TF/SF faces of both halves in LDS (E faces, then H faces), with the mask
@@ -1261,7 +1272,7 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
   // lanes past the grid's edge address the edge cell (their results are
   // never used): every load and table index stays inside its array
   const int ic = min(i, nx), jc = min(j, ny);
-  const int k1 = min(k0 + sd.tm.kc, nz + 1);
+  const int k1 = min(k0 + nch * sd.tm.kc, nz + 1);
   const long long pxy = gi.pxy;
   const unsigned pxb = (unsigned)gi.px * 4u;
   // byte offsets inside a plane: the cell and its x - 1 and y - 1 neighbours;
@@ -1307,16 +1318,10 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
   unsigned axe[3] = {~0u, ~0u, ~0u}, axh[3] = {~0u, ~0u, ~0u};
   if (ade_e) pole_xy(gi.ade[0], i, j, axe);
   if (ade_h) pole_xy(gi.ade[1], i, j, axh);
-  const float *Exi = gi.f[0], *Eyi = gi.f[1], *Ezi = gi.f[2];
-  const float *Hxi = gi.f[3], *Hyi = gi.f[4], *Hzi = gi.f[5];
-  float *Exo = go.f[0], *Eyo = go.f[1], *Ezo = go.f[2];
-  float *Hxo = go.f[3], *Hyo = go.f[4], *Hzo = go.f[5];
-  const int16_t *Mex = gi.mi[0], *Mey = gi.mi[1], *Mez = gi.mi[2];
-  const int16_t *Mhx = gi.mi[3], *Mhy = gi.mi[4], *Mhz = gi.mi[5];
 
   // H of step n on plane k0 - 1: the E update's z - 1 neighbours
-  float hx_km = ld_at(plane_base(Hxi, (long long)k0 * pxy), ob);
-  float hy_km = ld_at(plane_base(Hyi, (long long)k0 * pxy), ob);
+  float hx_km = ld_at(plane_base(gi.f[3], (long long)k0 * pxy), ob);
+  float hy_km = ld_at(plane_base(gi.f[4], (long long)k0 * pxy), ob);
   // the lane's constants arrive before the march: a load still in flight at
   // the loop head would make every plane's first use of it wait for all
   // memory operations issued since, stores included
@@ -1346,7 +1351,12 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
     // ---- loads of plane k.  (Issuing plane k + 1's loads before plane k's
     // arithmetic was measured: no faster, 16 more registers.)
     PlaneLoad cur{};
-    if (eplane) load_plane(cur, gi.f, gi.mi, P, ob, obx, oby, om);
+    if (eplane) {
+      if constexpr (CPML || ADE || SRC)
+        load_plane(cur, a->fi, a->mi, P, ob, obx, oby, om);
+      else
+        load_plane(cur, gi.f, gi.mi, P, ob, obx, oby, om);
+    }
     const float hx0 = cur.hx, hy0 = cur.hy, hz0 = cur.hz, hz_jm = cur.hz_jm, hx_jm = cur.hx_jm,
                 hz_im = cur.hz_im, hy_im = cur.hy_im, ex0 = cur.ex, ey0 = cur.ey, ez0 = cur.ez;
     const int mx = cur.mx, my = cur.my, mz = cur.mz, mhx = cur.mhx, mhy = cur.mhy,
@@ -1476,7 +1486,7 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
             e = step_nodal(sd.src, 0, 0, P + (jc + 1) * (long long)gi.px + kXOff + ic, e, n);
         }
         ex_n = e;
-        if (wr && ex_on) st_at(plane_base(Exo, P), ob, e);
+        if (wr && ex_on) st_at(plane_base(fo_at(a, go, 0), P), ob, e);
       }
       {
         const float2 c = te.gg[my];
@@ -1500,7 +1510,7 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
             e = step_nodal(sd.src, 0, 1, P + (jc + 1) * (long long)gi.px + kXOff + ic, e, n);
         }
         ey_n = e;
-        if (wr && ey_on) st_at(plane_base(Eyo, P), ob, e);
+        if (wr && ey_on) st_at(plane_base(fo_at(a, go, 1), P), ob, e);
       }
       {
         const float2 c = te.gg[mz];
@@ -1524,7 +1534,7 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
             e = step_nodal(sd.src, 0, 2, P + (jc + 1) * (long long)gi.px + kXOff + ic, e, n);
         }
         ez_n = e;
-        if (wr && ez_on) st_at(plane_base(Ezo, P), ob, e);
+        if (wr && ez_on) st_at(plane_base(fo_at(a, go, 2), P), ob, e);
       }
     }
     // ---- E of plane k at x + 1 (next lane) and y + 1 (next row)
@@ -1565,7 +1575,7 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
           if (nd && hxk)
             h = step_nodal(sd.src, 1, 3, Ph + (jc + 1) * (long long)gi.px + kXOff + ic, h, n);
         }
-        if (hxk) st_at(plane_base(Hxo, Ph), ob, h);
+        if (hxk) st_at(plane_base(fo_at(a, go, 3), Ph), ob, h);
       }
       {
         const float2 c = chy;
@@ -1589,7 +1599,7 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
           if (nd && hyk)
             h = step_nodal(sd.src, 1, 4, Ph + (jc + 1) * (long long)gi.px + kXOff + ic, h, n);
         }
-        if (hyk) st_at(plane_base(Hyo, Ph), ob, h);
+        if (hyk) st_at(plane_base(fo_at(a, go, 4), Ph), ob, h);
       }
       {
         const float2 c = chz;
@@ -1611,7 +1621,7 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
           if (nd && hzk)
             h = step_nodal(sd.src, 1, 5, Ph + (jc + 1) * (long long)gi.px + kXOff + ic, h, n);
         }
-        if (hzk) st_at(plane_base(Hzo, Ph), ob, h);
+        if (hzk) st_at(plane_base(fo_at(a, go, 5), Ph), ob, h);
       }
     }
     // ---- carry plane k to the next iteration
