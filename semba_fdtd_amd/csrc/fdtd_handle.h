@@ -146,6 +146,9 @@ struct fdtd_handle {
   // class with tiles, each over its slice of the device tile list
   std::vector<int> step_cls, step_cnt, step_off;
   std::vector<double> step_cells;        // Yee cells owned per class
+  // the stored previous field of the pole updates may differ from the field
+  // (fields set from outside, granular calls): the next pair reads it
+  bool prev_exact = true;
   int *dtiles = nullptr;
   // TF/SF faces of both halves for the step kernels and the buffer of their
   // incident field, refreshed before every step

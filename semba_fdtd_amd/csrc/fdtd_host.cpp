@@ -633,7 +633,7 @@ static int sync_b(fdtd_handle *h) {
 // one step of the pair, from sd[dir].gi to sd[dir].go, then the H-side
 // boundary, observation and step counter on the written copy
 // the step kernels of one direction: one launch per tile class
-static int launch_step_kernels(fdtd_handle *h, int dir) {
+static int launch_step_kernels(fdtd_handle *h, int dir, bool exact_prev = false) {
   if (h->step_nf > 0 &&
       launch_pw_incident(h->dsfaces, h->step_nf0, h->step_nf, h->step_maxcells, h->dstepsrc,
                          h->dstep, h->dinc, h->stream))
@@ -642,6 +642,7 @@ static int launch_step_kernels(fdtd_handle *h, int dir) {
     StepDev sd = h->sd[dir];
     sd.tiles = h->dtiles + h->step_off[q];
     sd.tm.nblk = h->step_cnt[q];
+    sd.prev_from_field = exact_prev ? 0 : 1;
     const int e = h->mode == FDTD_MODE_EXACT
                       ? exact::launch_step(sd, h->step_rows, h->step_cls[q], h->stream)
                       : fast::launch_step(sd, h->step_rows, h->step_cls[q], h->stream);
@@ -650,9 +651,9 @@ static int launch_step_kernels(fdtd_handle *h, int dir) {
   return FDTD_OK;
 }
 
-static int run_step(fdtd_handle *h, int dir) {
+static int run_step(fdtd_handle *h, int dir, bool exact_prev = false) {
   const StepDev &sd = h->sd[dir];
-  int e = launch_step_kernels(h, dir);
+  int e = launch_step_kernels(h, dir, exact_prev);
   if (e) return e;
   if (!h->mur_runs.empty() &&
       launch_mur(sd.go, h->dmur, (int)h->mur_runs.size(), h->mur_total,
@@ -1549,6 +1550,7 @@ int fdtd_add_planewave(fdtd_handle *h, const int32_t box[6], int32_t nmodes, con
 
 int fdtd_advance_e(fdtd_handle *h, int32_t n) {
   if (!h) return FDTD_ERR_ARG;
+  h->prev_exact = true;
   hipSetDevice(h->device);
   int rc;
   if ((rc = prepare(h)) || (rc = run_e(h, 1, 1, 1)) || (rc = run_pw(h, 0, n, nullptr)) ||
@@ -1559,6 +1561,7 @@ int fdtd_advance_e(fdtd_handle *h, int32_t n) {
 
 int fdtd_advance_h(fdtd_handle *h, int32_t n) {
   if (!h) return FDTD_ERR_ARG;
+  h->prev_exact = true;
   hipSetDevice(h->device);
   int rc;
   if ((rc = prepare(h)) || (rc = run_h(h, 1, 1, 1)) || (rc = run_pw(h, 1, n, nullptr)) ||
@@ -1570,6 +1573,7 @@ int fdtd_advance_h(fdtd_handle *h, int32_t n) {
 #define SFD_GRANULAR(name, body)                                               \
   int name {                                                                   \
     if (!h) return FDTD_ERR_ARG;                                               \
+    h->prev_exact = true;                                                      \
     hipSetDevice(h->device);                                                   \
     int rc = prepare(h);                                                       \
     if (rc) return rc;                                                         \
@@ -1619,7 +1623,15 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
     HIPCHK(h, hipGraphInstantiate(&h->pexec, graph, nullptr, nullptr, 0));
   }
   if (pairs > 0 && (rc = sync_b(h))) return rc;
-  for (int32_t s = 0; s < pairs; ++s) HIPCHK(h, hipGraphLaunch(h->pexec, h->stream));
+  int32_t s0 = 0;
+  if (pairs > 0 && h->prev_exact) {
+    // the first pair after fields were set from outside reads the stored
+    // previous fields of the pole updates
+    if ((rc = run_step(h, 0, true)) || (rc = run_step(h, 1, true))) return rc;
+    s0 = 1;
+  }
+  if (pairs > 0) h->prev_exact = false;
+  for (int32_t s = s0; s < pairs; ++s) HIPCHK(h, hipGraphLaunch(h->pexec, h->stream));
   if (singles == 0) return FDTD_OK;
   if (!h->graph_ok) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
@@ -1825,7 +1837,7 @@ int fdtd_get_field(fdtd_handle *h, int32_t comp, float *host) {
 }
 
 int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host) {
-  if (h) h->step_stale = true;
+  if (h) h->step_stale = h->prev_exact = true;
   if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field write");
   hipSetDevice(h->device);
   return field_io(h, comp, host, nullptr);
@@ -2025,7 +2037,7 @@ int fdtd_save_state(fdtd_handle *h, int32_t n, void *buf, int64_t size) {
 }
 
 int fdtd_load_state(fdtd_handle *h, const void *buf, int64_t size, int32_t *n) {
-  if (h) h->step_stale = true;
+  if (h) h->step_stale = h->prev_exact = true;
   if (!h || !buf || !n) return fail(h, FDTD_ERR_ARG, "null state buffer");
   const int64_t need = fdtd_state_size(h);
   if (need < 0) return FDTD_ERR_STATE;

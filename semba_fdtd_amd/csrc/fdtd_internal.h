@@ -227,6 +227,8 @@ struct StepDev {
                               // nblk: the tiles of this launch
   const int *tiles;           // tile numbers of this launch's class
   int merged;                 // entries are tile | (z chunks << 24)
+  int prev_from_field;        // tiles without sources take the pole update's
+                              // previous field from the field of step n
   // TF/SF faces per half (staged in LDS by the blocks that meet them), and
   // the bounding box of each half's nodal cells
   int npw[2];

@@ -1106,8 +1106,8 @@ __device__ __forceinline__ bool pole_plane(const AS &A, int c, int k, long long 
 // load the previous field and the first kPre pole states of edge zo + axy
 template <class AS>
 __device__ __forceinline__ void pole_load(const AS &A, int c, long long zo, unsigned axy,
-                                          float &ep, jstate_t *pj) {
-  ep = (A.prev[c] + zo)[axy];
+                                          float &ep, jstate_t *pj, bool with_prev = true) {
+  if (with_prev) ep = (A.prev[c] + zo)[axy];
   const jstate_t *J = reinterpret_cast<const jstate_t *>(A.jst[c]) + zo;
 #pragma unroll
   for (int q = 0; q < kPre; ++q)
@@ -1394,9 +1394,13 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
       g0 = pole_plane(A, 0, k, z0) && ex_on && axe[0] != ~0u;
       g1 = pole_plane(A, 1, k, z1) && ey_on && axe[1] != ~0u;
       g2 = pole_plane(A, 2, k, z2) && ez_on && axe[2] != ~0u;
-      if (g0) pole_load(A, 0, z0, axe[0], ep0, aj0);
-      if (g1) pole_load(A, 1, z1, axe[1], ep1, aj1);
-      if (g2) pole_load(A, 2, z2, axe[2], ep2, aj2);
+      // the field of the previous step is the stored one unless nothing acted
+      // on the edge after last step's pole update (no sources in the tile and
+      // no fields set from outside since): then it is the field just loaded
+      const bool pf = !SRC && sd.prev_from_field;
+      if (g0) { pole_load(A, 0, z0, axe[0], ep0, aj0, !pf); if (pf) ep0 = ex0; }
+      if (g1) { pole_load(A, 1, z1, axe[1], ep1, aj1, !pf); if (pf) ep1 = ey0; }
+      if (g2) { pole_load(A, 2, z2, axe[2], ep2, aj2, !pf); if (pf) ep2 = ez0; }
     }
     // the same for the H cells of plane k - 1: Hx (y, z), Hy (z, x), Hz (y, x)
     const bool hxk = hk && hx_on && kh < nz, hyk = hk && hy_on && kh < nz, hzk = hk && hz_on;
@@ -1426,9 +1430,10 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
       u0 = pole_plane(A, 0, kh, w0) && hxk && axh[0] != ~0u;
       u1 = pole_plane(A, 1, kh, w1) && hyk && axh[1] != ~0u;
       u2 = pole_plane(A, 2, kh, w2) && hzk && axh[2] != ~0u;
-      if (u0) pole_load(A, 0, w0, axh[0], hp0, bj0);
-      if (u1) pole_load(A, 1, w1, axh[1], hp1, bj1);
-      if (u2) pole_load(A, 2, w2, axh[2], hp2, bj2);
+      const bool pf = !SRC && sd.prev_from_field;
+      if (u0) { pole_load(A, 0, w0, axh[0], hp0, bj0, !pf); if (pf) hp0 = hx_c; }
+      if (u1) { pole_load(A, 1, w1, axh[1], hp1, bj1, !pf); if (pf) hp1 = hy_c; }
+      if (u2) { pole_load(A, 2, w2, axh[2], hp2, bj2, !pf); if (pf) hp2 = hz_c; }
     }
     const float idze = hk ? ro_uniform(idze_t, kh + 1) : 0.0f;
     // TF/SF faces: the faces acting on each component of this lane's E cell
