@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+SERVE_MAX_CELLS = 12_000_000  # cells admitted at once by the batched server
 METRIC = "Mcell-updates/s + %HBM roofline; batched-serve p50/p99 latency vs plain"
 
 
@@ -174,12 +175,15 @@ def serve_leg(count, rate, mode):
     reqs = P.serving_requests(count=count, rate=rate)
     print(f"[bench] serve: {count} requests at {rate}/s", file=sys.stderr, flush=True)
     plain = serve.serve_plain(reqs, mode=mode)
-    batched = serve.serve_batched(reqs, mode=mode)
+    # at most SERVE_MAX_CELLS cells admitted at once: enough to fill the
+    # device, few enough that early requests finish early
+    batched = serve.serve_batched(reqs, mode=mode, max_cells=SERVE_MAX_CELLS)
     same = True
     pm = {r.name: r for r in plain}
     for r in batched:
         same = same and np.array_equal(r.probes.view(np.uint32), pm[r.name].probes.view(np.uint32))
     return {"requests": count, "arrival_rate_per_s": rate, "mode": mode,
+            "batched_max_cells": SERVE_MAX_CELLS,
             "batched": serve.summarize(batched), "plain": serve.summarize(plain),
             "results_identical": bool(same)}
 

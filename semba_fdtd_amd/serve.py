@@ -70,15 +70,21 @@ def serve_plain(requests, mode: str = "fast") -> List[Result]:
 
 def serve_batched(requests, mode: str = "fast", mem_budget: int = 200 << 30,
                   max_chunk: int = 250, rows: int = 4, zchunk: int = 32,
-                  blocks: int = 0, xcells: int = 1) -> List[Result]:
-    """Continuous batching: admit arrived requests while the memory budget
-    allows, advance the batch in chunks that end when the next admitted grid
-    finishes (or after max_chunk steps), retire finished grids."""
+                  blocks: int = 0, xcells: int = 1,
+                  max_cells: int | None = None) -> List[Result]:
+    """Continuous batching: admit arrived requests, in arrival order, while
+    the memory budget allows and (max_cells) the admitted grids hold fewer
+    cells than max_cells; advance the batch in chunks that end when the next
+    admitted grid finishes (or after max_chunk steps), retire finished grids.
+    max_cells trades latency for throughput: once the batch holds enough
+    cells to fill the device, admitting more only stretches every admitted
+    request's time."""
     t0 = time.perf_counter()
     batch = Batch(rows=rows, zchunk=zchunk, blocks=blocks, xcells=xcells)
     waiting = list(requests)
-    active = {}          # slot -> (solver, arrival, steps)
+    active = {}          # slot -> (solver, arrival, steps, bytes)
     used = 0
+    cells = 0
     out = []
     while waiting or active:
         now = time.perf_counter() - t0
@@ -88,11 +94,14 @@ def serve_batched(requests, mode: str = "fast", mem_budget: int = 200 << 30,
             need = device_bytes_estimate(pb)
             if active and used + need > mem_budget:
                 break
+            if active and max_cells is not None and cells >= max_cells:
+                break
             waiting.pop(0)
             s = Solver(pb, mode=mode, probe_buffer=max(min(ns, max_chunk), 1))
             slot = batch.admit(s)
             active[slot] = (s, arrival, ns, need)
             used += need
+            cells += pb.cells
         if not active:
             if waiting:
                 time.sleep(max(0.0, waiting[0][0] - (time.perf_counter() - t0)))
@@ -104,6 +113,7 @@ def serve_batched(requests, mode: str = "fast", mem_budget: int = 200 << 30,
             s, arrival, ns, need = active.pop(slot)
             batch.retire(slot)
             out.append(_finish(s, arrival, t0, ns))
+            cells -= s.pb.cells
             s.close()
             used -= need
     batch.close()
