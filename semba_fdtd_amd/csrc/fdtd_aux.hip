@@ -91,6 +91,41 @@ int launch_tfsf_list(const PwRun *runs, const long long *start, int nruns, long 
   return (int)hipGetLastError();
 }
 
+// A dense x-fastest box of ex*ey*ez elements moved between a packed buffer
+// and a grid array (rows of px elements, planes of pxy, the box's first
+// element at x offset kXOff - 1 of row 0, plane 0): host transfers then take
+// one contiguous copy instead of one pitched copy per plane.
+template <class T>
+__global__ void __launch_bounds__(256) k_box_move(T *grid, T *packed, long long ex, long long ey,
+                                                 long long ez, long long px, long long pxy,
+                                                 int to_grid) {
+  const long long n = ex * ey * ez;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long i = t % ex, j = (t / ex) % ey, k = t / (ex * ey);
+    const long long p = k * pxy + j * px + (kXOff - 1) + i;
+    if (to_grid)
+      grid[p] = packed[t];
+    else
+      packed[t] = grid[p];
+  }
+}
+
+int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3], long long px,
+                    long long pxy, int to_grid, void *stream) {
+  const long long n = ext[0] * ext[1] * ext[2];
+  if (n <= 0) return 0;
+  const unsigned b = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  if (es == 2)
+    hipLaunchKernelGGL(k_box_move<int16_t>, dim3(b), dim3(256), 0, (hipStream_t)stream,
+                       (int16_t *)grid, (int16_t *)packed, ext[0], ext[1], ext[2], px, pxy,
+                       to_grid);
+  else
+    hipLaunchKernelGGL(k_box_move<float>, dim3(b), dim3(256), 0, (hipStream_t)stream,
+                       (float *)grid, (float *)packed, ext[0], ext[1], ext[2], px, pxy, to_grid);
+  return (int)hipGetLastError();
+}
+
 // Incid of every TF/SF face cell for the one-launch step (the same
 // arithmetic as k_tfsf_list, which applies it in place)
 __global__ void __launch_bounds__(256) k_pw_incident(const PwFace *faces, int nf0,

@@ -85,21 +85,42 @@ void alloc_extent(const fdtd_handle *h, int comp, long long ext[3]) {
 
 // host Alloc-box array <-> device grid array (element size es), one 2-D copy
 // per z plane
+// A component's Alloc box between the host (dense, x fastest) and its grid
+// array: contiguous copies of runs of z planes through a packed device
+// staging buffer (at most 64 MiB, or one plane), and a kernel that moves each
+// run into (or out of) the padded layout.
 int copy_box(fdtd_handle *h, int comp, void *dev, const void *host_src, void *host_dst,
              size_t es) {
   long long ext[3];
   alloc_extent(h, comp, ext);
-  const size_t hrow = (size_t)ext[0] * es, hplane = hrow * (size_t)ext[1];
-  for (long long k = 0; k < ext[2]; ++k) {
-    char *d = (char *)dev + ((size_t)k * h->g.pxy + (kXOff - 1)) * es;
-    if (host_src)
-      HIPCHK(h, hipMemcpy2DAsync(d, (size_t)h->g.px * es, (const char *)host_src + k * hplane, hrow,
-                                 hrow, (size_t)ext[1], hipMemcpyHostToDevice, h->stream));
-    else
-      HIPCHK(h, hipMemcpy2DAsync((char *)host_dst + k * hplane, hrow, d, (size_t)h->g.px * es,
-                                 hrow, (size_t)ext[1], hipMemcpyDeviceToHost, h->stream));
+  const size_t plane = (size_t)(ext[0] * ext[1]) * es;
+  const long long per = std::max<long long>(1, (long long)((64u << 20) / std::max<size_t>(plane, 1)));
+  const size_t want = plane * (size_t)std::min(per, ext[2]);
+  if (want > h->stage_bytes) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->stage) hipFree(h->stage);
+    h->stage = nullptr;
+    h->stage_bytes = 0;
+    HIPCHK(h, hipMalloc(&h->stage, want));
+    h->stage_bytes = want;
   }
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (long long k0 = 0; k0 < ext[2]; k0 += per) {
+    const long long nk = std::min(per, ext[2] - k0);
+    const long long sub[3] = {ext[0], ext[1], nk};
+    char *g = (char *)dev + (size_t)k0 * (size_t)h->g.pxy * es;
+    const size_t bytes = plane * (size_t)nk;
+    if (host_src) {
+      HIPCHK(h, hipMemcpyAsync(h->stage, (const char *)host_src + (size_t)k0 * plane, bytes,
+                               hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, (hipError_t)launch_box_move(g, h->stage, es, sub, h->g.px, h->g.pxy, 1, h->stream));
+    } else {
+      HIPCHK(h, (hipError_t)launch_box_move(g, h->stage, es, sub, h->g.px, h->g.pxy, 0, h->stream));
+      HIPCHK(h, hipMemcpyAsync((char *)host_dst + (size_t)k0 * plane, h->stage, bytes,
+                               hipMemcpyDeviceToHost, h->stream));
+    }
+    // the staging buffer is reused by the next run
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
   return FDTD_OK;
 }
 
@@ -1156,6 +1177,7 @@ int fdtd_destroy(fdtd_handle *h) {
   for (void *p : h->dft_allocs) hipFree(p);
   for (void *p : h->dft_sums) hipFree(p);
   for (void *p : h->allocs) hipFree(p);
+  if (h->stage) hipFree(h->stage);
   if (h->solo_stream) h->stream = h->solo_stream;
   if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
   delete h;

@@ -336,6 +336,10 @@ int launch_dft(const DevGrid &g, const DftDev &d, int n, const int *dn, void *st
 // tiempo(n) + dt/2); runs[r] covers cells start[r] .. start[r+1]-1 of total
 int launch_tfsf_list(const PwRun *runs, const long long *start, int nruns, long long total,
                      int n, void *stream);
+// a dense box of es-byte elements (es 2 or 4), ext = its x, y, z extents,
+// moved between a packed device buffer and a grid array
+int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3], long long px,
+                    long long pxy, int to_grid, void *stream);
 // the incident field of every TF/SF face cell at step *dn for the one-launch
 // step: faces[0..nf0) act on E (time tiempo(n)), faces[nf0..nf) on H
 // (tiempo(n) + dt/2); face q's cells go to inc[faces[q].inc ...], the face's

@@ -79,13 +79,16 @@ def test_retired_grid_continues_alone():
     batch.close()
 
 
-def test_serving_results_equal_plain():
+@pytest.mark.parametrize("max_cells", [None, 60_000])
+def test_serving_results_equal_plain(max_cells):
     """Continuous batching: requests arriving while others run give the same
-    probe series and frequency-domain sums as the one-at-a-time path."""
+    probe series and frequency-domain sums as the one-at-a-time path, also
+    when admission is capped (60 000 cells: one or two grids at a time)."""
     from semba_fdtd_amd import serve
     reqs = P.serving_requests(count=5, rate=40.0, nmin=24, nmax=36, steps=(40, 120), nfreq=8)
     plain = {r.name: r for r in serve.serve_plain(reqs, mode="fast")}
-    batched = {r.name: r for r in serve.serve_batched(reqs, mode="fast", max_chunk=25)}
+    batched = {r.name: r for r in serve.serve_batched(reqs, mode="fast", max_chunk=25,
+                                                      max_cells=max_cells)}
     assert set(plain) == set(batched)
     for name in plain:
         a, b = plain[name], batched[name]
