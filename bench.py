@@ -30,7 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
-SERVE_MAX_CELLS = 12_000_000  # cells admitted at once by the batched server
+SERVE_MAX_CELLS = 24_000_000  # cells admitted at once by the batched server
 METRIC = "Mcell-updates/s + %HBM roofline; batched-serve p50/p99 latency vs plain"
 
 
