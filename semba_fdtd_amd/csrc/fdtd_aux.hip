@@ -220,6 +220,28 @@ __global__ void k_probes(DevGrid g, const ProbeDev *pr, int np, float *buf,
   if (threadIdx.x == 0 && row < cap) *count = row + 1;
 }
 
+// one block per probe set, as k_probes
+__global__ void k_probes_many(const DevGrid *grids, const ProbeSet *sets) {
+  const ProbeSet &ps = sets[blockIdx.x];
+  const DevGrid &g = grids[ps.grid];
+  __shared__ int row;
+  if (threadIdx.x == 0) row = *ps.count;
+  __syncthreads();
+  if (row < ps.cap) {
+    for (int t = threadIdx.x; t < ps.np; t += blockDim.x)
+      ps.buf[(long long)row * ps.np + t] = field_ld(g, ps.pr[t].comp, ps.pr[t].p);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && row < ps.cap) *ps.count = row + 1;
+}
+
+int launch_probes_many(const DevGrid *grids, const ProbeSet *sets, int nsets, void *stream) {
+  if (nsets <= 0) return 0;
+  hipLaunchKernelGGL(k_probes_many, dim3(nsets), dim3(256), 0, (hipStream_t)stream, grids,
+                     sets);
+  return (int)hipGetLastError();
+}
+
 int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
                   int *count, int cap, void *stream) {
   if (np <= 0) return 0;

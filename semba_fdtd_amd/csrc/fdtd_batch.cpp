@@ -4,12 +4,13 @@
 // set-up calls); admitting one hands its device state to the batch without
 // copying.  One step of the batch runs, for every grid at its own step:
 //   E face pass of all grids          one persistent launch (k_pass_batch)
-//   pole stage                        one launch per dispersive grid
+//   pole stage                        one launch for the grids with poles
 //   plane-wave E faces                one launch per (wave, face axis) group
 //   nodal E                           one launch per grid with E sources
 //   H face pass of all grids          one persistent launch
 //   plane-wave H faces, nodal H       as above
-//   probes, frequency-domain sums     per grid
+//   probes                            one launch for all grids
+//   frequency-domain sums             per grid
 //   step counters                     one launch for all grids
 // captured once per membership in a HIP graph and replayed.  Each grid keeps
 // its own step counter (its dstep), so grids admitted at different times
@@ -59,6 +60,15 @@ struct fdtd_batch {
   std::vector<BatchPwGroup> pw;
   int **dcounters = nullptr;
   int ncounters = 0;
+  // grids with pole state per field (indices into bd.grids) for the one
+  // pole-stage launch per half, and the probe sets of all grids
+  int *dade[2] = {nullptr, nullptr};
+  int nade[2] = {0, 0};
+  size_t ade_lds[2] = {0, 0};
+  int ade_tbl[2] = {1, 1};
+  long long ade_nmax[2] = {0, 0};
+  ProbeSet *dprobes = nullptr;
+  int nprobesets = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 };
@@ -166,6 +176,55 @@ int build_descriptors(fdtd_batch *b) {
   b->bd.total = start.back();
   b->ncounters = (int)counters.size();
   b->lds = lds;
+  // pole stage and probes: one launch each for the whole batch
+  {
+    const bool exact = b->mode == FDTD_MODE_EXACT;
+    std::vector<ProbeSet> sets;
+    std::vector<int> al[2];
+    for (int side = 0; side < 2; ++side) {
+      b->ade_lds[side] = 0;
+      b->ade_tbl[side] = 1;
+      b->ade_nmax[side] = 0;
+    }
+    int gi = 0;
+    for (size_t s = 0; s < b->slots.size(); ++s) {
+      fdtd_handle *h = b->slots[s];
+      if (!h) continue;
+      for (int side = 0; side < 2; ++side) {
+        const AdeSet &ad = h->g.ade[side];
+        if (!ad.has) continue;
+        long long nmax = 0;
+        for (int c = 0; c < 3; ++c) nmax = std::max(nmax, ad.nstate[c]);
+        if (nmax <= 0) continue;
+        al[side].push_back(gi);
+        b->ade_nmax[side] = std::max(b->ade_nmax[side], nmax);
+        const size_t la = exact ? exact::ade_lds_of(h->g, side) : fast::ade_lds_of(h->g, side);
+        b->ade_lds[side] = std::max(b->ade_lds[side], la);
+        if (h->g.nmedia + 1 > 4096) b->ade_tbl[side] = 0;
+      }
+      if (h->nprobe) {
+        ProbeSet ps{};
+        ps.pr = h->dprobes;
+        ps.buf = h->dpbuf;
+        ps.count = h->dpcount;
+        ps.np = h->nprobe;
+        ps.cap = h->pcap;
+        ps.grid = gi;
+        sets.push_back(ps);
+      }
+      ++gi;
+    }
+    for (int side = 0; side < 2; ++side) {
+      if (b->ade_lds[side] > 48 * 1024) b->ade_tbl[side] = 0;
+      b->nade[side] = (int)al[side].size();
+      b->dade[side] = nullptr;
+      if (!al[side].empty() && (rc = bupload(b, &b->dade[side], al[side].data(), al[side].size())))
+        return rc;
+    }
+    b->nprobesets = (int)sets.size();
+    b->dprobes = nullptr;
+    if (!sets.empty() && (rc = bupload(b, &b->dprobes, sets.data(), sets.size()))) return rc;
+  }
   // plane-wave groups across the batch, in each grid's face order: per half,
   // per wave index, per face axis (the runs read each grid's own counter)
   for (int half = 0; half < 2; ++half)
@@ -209,10 +268,17 @@ int launch_step(fdtd_batch *b) {
               : fast::launch_pass_batch(b->bd, half, blocks, b->rows, vec, b->lds, b->fp16,
                                         b->stream);
     if (e) return bfail(b, FDTD_ERR_HIP, "batch face pass failed: %s", hipGetErrorString((hipError_t)e));
-    // each grid's pole stage follows the face pass of its field
-    for (fdtd_handle *h : b->slots)
-      if (h && h->g.ade[half].has && (rc = run_ade(h, half)))
-        return bfail(b, rc, "grid: %s", h->err.c_str());
+    // the pole stage of every grid with poles on this field follows the face
+    // pass, in one launch
+    if (b->nade[half]) {
+      e = exact ? exact::launch_ade_many(b->bd.grids, b->dade[half], b->nade[half], half,
+                                         b->ade_tbl[half] ? b->ade_lds[half] : 0,
+                                         b->ade_tbl[half], b->ade_nmax[half], b->stream)
+                : fast::launch_ade_many(b->bd.grids, b->dade[half], b->nade[half], half,
+                                        b->ade_tbl[half] ? b->ade_lds[half] : 0,
+                                        b->ade_tbl[half], b->ade_nmax[half], b->stream);
+      if (e) return bfail(b, FDTD_ERR_HIP, "batch pole stage failed: %s", hipGetErrorString((hipError_t)e));
+    }
     for (const BatchPwGroup &g : b->pw) {
       if (g.half != half) continue;
       e = launch_tfsf_list(g.druns, g.dstart, (int)g.runs.size(), g.start.back(), 0, b->stream);
@@ -224,13 +290,10 @@ int launch_step(fdtd_batch *b) {
   }
   for (fdtd_handle *h : b->slots)
     if (h && (rc = run_mur(h))) return bfail(b, rc, "grid: %s", h->err.c_str());
-  for (fdtd_handle *h : b->slots) {
-    if (!h) continue;
-    if (h->nprobe && launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap,
-                                   b->stream))
-      return bfail(b, FDTD_ERR_HIP, "batch probe launch failed");
-    if ((rc = run_dft(h, 0, h->dstep))) return bfail(b, rc, "grid: %s", h->err.c_str());
-  }
+  if (launch_probes_many(b->bd.grids, b->dprobes, b->nprobesets, b->stream))
+    return bfail(b, FDTD_ERR_HIP, "batch probe launch failed");
+  for (fdtd_handle *h : b->slots)
+    if (h && (rc = run_dft(h, 0, h->dstep))) return bfail(b, rc, "grid: %s", h->err.c_str());
   if (launch_tick_many(b->dcounters, b->ncounters, b->stream))
     return bfail(b, FDTD_ERR_HIP, "batch step counter launch failed");
   return FDTD_OK;

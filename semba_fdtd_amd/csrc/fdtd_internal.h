@@ -319,6 +319,9 @@ __device__ __forceinline__ float nodal_apply(float v, const NodalCell &c, float 
                int yee, int cpml, int ade, void *stream);                    \
   size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc);       \
   int launch_step(const StepDev &sd, int rows, int cls, void *stream);       \
+  size_t ade_lds_of(const DevGrid &g, int side);                            \
+  int launch_ade_many(const DevGrid *grids, const int *list, int n, int side, \
+                      size_t lds, int tbl, long long nmax, void *stream);     \
   int launch_pass_batch(const BatchDev &bd, int half, int blocks, int rows,  \
                         int vec, size_t lds, int fp16, void *stream);        \
   }
@@ -350,6 +353,14 @@ int launch_pw_incident(const PwFace *faces, int nf0, int nf, long long maxcells,
 int launch_tick_many(int *const *counters, int n, void *stream);
 int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,
                   int *count, int cap, void *stream);
+// the probes of several grids of a batch in one launch
+struct ProbeSet {
+  const ProbeDev *pr;
+  float *buf;
+  int *count;
+  int np, cap, grid;          // grid: index into the batch's descriptors
+};
+int launch_probes_many(const DevGrid *grids, const ProbeSet *sets, int nsets, void *stream);
 // bounding box {xmin,ymin,zmin,xmax,ymax,zmax} of the dispersive edges of
 // component comp; out must hold {INT_MAX x3, INT_MIN x3} on entry
 int launch_disp_bbox(const DevGrid &g, int side, int c, int32_t *out, void *stream);

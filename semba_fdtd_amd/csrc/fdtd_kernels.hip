@@ -279,7 +279,7 @@ __device__ __forceinline__ float ade_apply(const AdeSet &ai, const AdeSet &ao, b
 // loads of a wave are each one contiguous run; non-dispersive edges cost only
 // their int16 media index.
 template <bool TBL, int SIDE>
-__global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
+__device__ __forceinline__ void ade_box_run(const DevGrid &g) {
   extern __shared__ float2 lds_tab[];
   const AdeSet &ad = g.ade[SIDE];
   const MedTab tab = SIDE ? stage_tables<TBL, true>(g, g.gm1, g.gm2, lds_tab, &ad)
@@ -313,6 +313,18 @@ __global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
     else r = ade_apply<2>(ad, ad, true, g.dt, tab, a, e);
     field_st(g, f, p, r);
   }
+}
+
+template <bool TBL, int SIDE>
+__global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
+  ade_box_run<TBL, SIDE>(g);
+}
+
+// the pole stage of several grids of a batch in one launch: blockIdx.z picks
+// the grid (list[z], an index into grids), the rest as k_ade_box
+template <bool TBL, int SIDE>
+__global__ void __launch_bounds__(256) k_ade_many(const DevGrid *grids, const int *list) {
+  ade_box_run<TBL, SIDE>(grids[list[blockIdx.z]]);
 }
 
 // one tile (64 x blockDim.y lanes, one z chunk) of the E face pass
@@ -925,6 +937,24 @@ static void launch_ade(const DevGrid &g, int side, hipStream_t s) {
     if (ta) hipLaunchKernelGGL((k_ade_box<true, 1>), grid, dim3(256), la, s, g);
     else hipLaunchKernelGGL((k_ade_box<false, 1>), grid, dim3(256), 0, s, g);
   }
+}
+
+size_t ade_lds_of(const DevGrid &g, int side) { return ade_lds_bytes(g, side); }
+
+int launch_ade_many(const DevGrid *grids, const int *list, int n, int side, size_t lds,
+                    int tbl, long long nmax, void *stream) {
+  if (n <= 0 || nmax <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const long long want = (nmax + 255) / 256;
+  const dim3 grid((unsigned)(want < 1024 ? want : 1024), 3, (unsigned)n);
+  if (side == 0) {
+    if (tbl) hipLaunchKernelGGL((k_ade_many<true, 0>), grid, dim3(256), lds, s, grids, list);
+    else hipLaunchKernelGGL((k_ade_many<false, 0>), grid, dim3(256), 0, s, grids, list);
+  } else {
+    if (tbl) hipLaunchKernelGGL((k_ade_many<true, 1>), grid, dim3(256), lds, s, grids, list);
+    else hipLaunchKernelGGL((k_ade_many<false, 1>), grid, dim3(256), 0, s, grids, list);
+  }
+  return (int)hipGetLastError();
 }
 
 int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, int cpml,
