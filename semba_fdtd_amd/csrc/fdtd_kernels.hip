@@ -1018,30 +1018,21 @@ int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
 // result is the same bit for bit; only the traffic changes: each field value
 // is read once and written once per step instead of twice.
 //
-// A block of 64 x R threads marches one z chunk of a tile.  Lanes 0..62 and
+// A block of 64 x R threads marches the z chunks of a tile.  Lanes 0..62 and
 // rows 0..R-2 own their cells; lane 63 and row R-1 overlap the neighbouring
-// tiles and compute only the E values the owned H cells need, and the plane
-// above the chunk is computed the same way.  E values travel to the x - 1
-// neighbour by a cross-lane shuffle and to the y - 1 neighbour through LDS;
-// H values of step n are read from memory (gi is never written during the
-// step).
+// tiles: their E values feed the owned H cells and are not stored, and the
+// plane above the chunk is computed the same way.  E values travel to the
+// x - 1 neighbour by a cross-lane shuffle and to the y - 1 neighbour through
+// LDS; H values of step n are read from memory (gi is never written during
+// the step).
+
 
 // Loads and stores of the march address a plane through a base pointer that
 // is the same for every lane (scalar registers) plus the lane's byte offset
-// inside the plane (one 32-bit register), instead of one 64-bit address per
-// array and lane.
-template <class T>
-__device__ __forceinline__ T ld_at(const T *base, unsigned ob) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + ob);
-}
-template <class T>
-__device__ __forceinline__ void st_at(T *base, unsigned ob, T v) {
-  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + ob) = v;
-}
-
-// the base of one plane of an array, made opaque to the compiler: the lane's
-// address is then this scalar base plus its 32-bit offset, rather than a
-// 64-bit (array base + offset) pair per array held across the march
+// inside the plane (one 32-bit register).  The plane base is made opaque to
+// the compiler, which otherwise hoists a 64-bit (array base + offset) pair per
+// array and lane out of the march; it stays in the global address space so
+// that the accesses are global, not flat, loads and stores.
 #define SFD_GLOBAL __attribute__((address_space(1)))
 template <class T>
 __device__ __forceinline__ SFD_GLOBAL T *plane_base(T *base, long long P) {
