@@ -654,6 +654,10 @@ static int sync_b(fdtd_handle *h) {
 // one step of the pair, from sd[dir].gi to sd[dir].go, then the H-side
 // boundary, observation and step counter on the written copy
 // the step kernels of one direction: one launch per tile class
+// The class kernels of one step direction, one after another on the
+// handle's stream.  (They read the same state and write disjoint cells, but
+// running the small classes on side streams beside the interior one was
+// measured slower: they compete for the same bandwidth and caches.)
 static int launch_step_kernels(fdtd_handle *h, int dir, bool exact_prev = false) {
   if (h->step_nf > 0 &&
       launch_pw_incident(h->dsfaces, h->step_nf0, h->step_nf, h->step_maxcells, h->dstepsrc,

@@ -278,6 +278,7 @@ __device__ __forceinline__ float ade_apply(const AdeSet &ai, const AdeSet &ao, b
 // visited x-fastest, so the media index, field, previous-field and state
 // loads of a wave are each one contiguous run; non-dispersive edges cost only
 // their int16 media index.
+// the pole stage of one grid whose descriptor lives in device memory
 template <bool TBL, int SIDE>
 __device__ __forceinline__ void ade_box_run(const DevGrid &g) {
   extern __shared__ float2 lds_tab[];
@@ -315,9 +316,43 @@ __device__ __forceinline__ void ade_box_run(const DevGrid &g) {
   }
 }
 
+// (the same body as ade_box_run, written out: a reference to the kernel's
+// by-value argument would copy the whole descriptor to scratch memory)
 template <bool TBL, int SIDE>
 __global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
-  ade_box_run<TBL, SIDE>(g);
+  extern __shared__ float2 lds_tab[];
+  const AdeSet &ad = g.ade[SIDE];
+  const MedTab tab = SIDE ? stage_tables<TBL, true>(g, g.gm1, g.gm2, lds_tab, &ad)
+                          : stage_tables<TBL, true>(g, g.g1, g.g2, lds_tab, &ad);
+  const int c = blockIdx.y;
+  const int f = 3 * SIDE + c;   // field component
+  const int *b = ad.abox[c];
+  const long long bx = b[3], bxy = (long long)b[3] * b[4];
+  const long long n = bxy * b[5];
+  for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n;
+       s += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(s % bx), y = (int)((s / bx) % b[4]), z = (int)(s / bxy);
+    const long long p = (long long)(b[2] + z + 1) * g.pxy + (long long)(b[1] + y + 1) * g.px +
+                        kXOff + b[0] + x;
+    const int m = g.mi[f][p];
+    const int slot = tab.slot[m];
+    if (slot < 0) continue;
+    AdePre a;
+    a.s = s;
+    a.slot = slot;
+    a.np = tab.anp[slot];
+    a.ep = ad.prev[c][s];
+    const jstate_t *J = reinterpret_cast<const jstate_t *>(ad.jst[c]);
+#pragma unroll
+    for (int q = 0; q < kPre; ++q)
+      if (q < a.np) a.j[q] = J[q * ad.nstate[c] + s];
+    const float e = field_ld(g, f, p);
+    float r;
+    if (c == 0) r = ade_apply<0>(ad, ad, true, g.dt, tab, a, e);
+    else if (c == 1) r = ade_apply<1>(ad, ad, true, g.dt, tab, a, e);
+    else r = ade_apply<2>(ad, ad, true, g.dt, tab, a, e);
+    field_st(g, f, p, r);
+  }
 }
 
 // the pole stage of several grids of a batch in one launch: blockIdx.z picks
@@ -1442,19 +1477,16 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
       if (r4) qv4 = (a->psi_i[PSI_HZY] + kh * Ly * nx)[(unsigned)(qyc * nx + i)];
       if (r5) qv5 = (a->psi_i[PSI_HZX] + kh * ny * Lx)[(unsigned)(j * Lx + qxc)];
     }
+    // (the pole state of H cells, rare, is read where it is used: prefetched
+    // here it would cost the pole classes 15 registers and spills)
     bool u0 = false, u1 = false, u2 = false;
     long long w0 = 0, w1 = 0, w2 = 0;
-    float hp0 = 0.f, hp1 = 0.f, hp2 = 0.f;
-    jstate_t bj0[kPre], bj1[kPre], bj2[kPre];
+    const bool hpf = !SRC && sd.prev_from_field;
     if (ade_h && hk) {
       const auto &A = a->ai[1];
       u0 = pole_plane(A, 0, kh, w0) && hxk && axh[0] != ~0u;
       u1 = pole_plane(A, 1, kh, w1) && hyk && axh[1] != ~0u;
       u2 = pole_plane(A, 2, kh, w2) && hzk && axh[2] != ~0u;
-      const bool pf = !SRC && sd.prev_from_field;
-      if (u0) { pole_load(A, 0, w0, axh[0], hp0, bj0, !pf); if (pf) hp0 = hx_c; }
-      if (u1) { pole_load(A, 1, w1, axh[1], hp1, bj1, !pf); if (pf) hp1 = hy_c; }
-      if (u2) { pole_load(A, 2, w2, axh[2], hp2, bj2, !pf); if (pf) hp2 = hz_c; }
     }
     const float idze = hk ? ro_uniform(idze_t, kh + 1) : 0.0f;
     // TF/SF faces: the faces acting on each component of this lane's E cell
@@ -1594,7 +1626,12 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
         }
         if (u0) {
           const int s = shx;
-          if (s >= 0) h = pole_new(a->ai[1], a->ao[1], 0, w0, axh[0], true, a->dt, th, s, hp0, bj0, h);
+          if (s >= 0) {
+            float hp = hx_c;
+            jstate_t bj[kPre];
+            pole_load(a->ai[1], 0, w0, axh[0], hp, bj, !hpf);
+            h = pole_new(a->ai[1], a->ao[1], 0, w0, axh[0], true, a->dt, th, s, hp, bj, h);
+          }
         }
         if (SRC) {
           if (bh0) h = faces_apply(lf + np0, bh0, ih0, sd.inc, sd.pwG[1], i, j, kh, h);
@@ -1618,7 +1655,12 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
         }
         if (u1) {
           const int s = shy;
-          if (s >= 0) h = pole_new(a->ai[1], a->ao[1], 1, w1, axh[1], true, a->dt, th, s, hp1, bj1, h);
+          if (s >= 0) {
+            float hp = hy_c;
+            jstate_t bj[kPre];
+            pole_load(a->ai[1], 1, w1, axh[1], hp, bj, !hpf);
+            h = pole_new(a->ai[1], a->ao[1], 1, w1, axh[1], true, a->dt, th, s, hp, bj, h);
+          }
         }
         if (SRC) {
           if (bh1) h = faces_apply(lf + np0, bh1, ih1, sd.inc, sd.pwG[1], i, j, kh, h);
@@ -1640,7 +1682,12 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
         }
         if (u2) {
           const int s = shz;
-          if (s >= 0) h = pole_new(a->ai[1], a->ao[1], 2, w2, axh[2], true, a->dt, th, s, hp2, bj2, h);
+          if (s >= 0) {
+            float hp = hz_c;
+            jstate_t bj[kPre];
+            pole_load(a->ai[1], 2, w2, axh[2], hp, bj, !hpf);
+            h = pole_new(a->ai[1], a->ao[1], 2, w2, axh[2], true, a->dt, th, s, hp, bj, h);
+          }
         }
         if (SRC) {
           if (bh2) h = faces_apply(lf + np0, bh2, ih2, sd.inc, sd.pwG[1], i, j, kh, h);
