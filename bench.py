@@ -73,14 +73,26 @@ def step_algorithmic_bytes(pb, ndisp_edges, mode, stored_poles):
     return 84 * pb.cells + 8 * (pe + ph) + ndisp_edges * (8 + per_state * stored_poles)
 
 
+def media_bytes(pb):
+    """Bytes per media index the one-launch step reads: the library keeps
+    8-bit copies of the media arrays for it when there are at most 256 media."""
+    return 1 if pb.nmedia + 1 <= 256 else 2
+
+
+def onelaunch_cell_bytes(pb):
+    """Compulsory bytes per cell of the one-launch step: each of the 6 fields
+    read once and written once, each of the 6 media indices read once."""
+    return 48 + 6 * media_bytes(pb)
+
+
 def onelaunch_step_bytes(pb, ndisp_edges, mode, stored_poles):
     """Compulsory HBM bytes of one whole step when E and H of a plane are
-    updated in the same launch: each field read once and written once, each
-    media index read once (60 B per cell), 8 B per psi value and per
-    dispersive edge 8 B of Eprev plus the pole state."""
+    updated in the same launch: onelaunch_cell_bytes per cell, 8 B per psi
+    value and per dispersive edge 8 B of Eprev plus the pole state."""
     pe, ph = psi_counts(pb)
     per_state = 16 if mode == "fast" else 32
-    return 60 * pb.cells + 8 * (pe + ph) + ndisp_edges * (8 + per_state * stored_poles)
+    return (onelaunch_cell_bytes(pb) * pb.cells + 8 * (pe + ph)
+            + ndisp_edges * (8 + per_state * stored_poles))
 
 
 CLASS_NAMES = {0: "plain", 1: "cpml", 2: "poles", 3: "cpml+poles", 4: "sources",
@@ -90,8 +102,8 @@ KERNELS = {
     "e_face": "k_epass (Yee E + CPML E, fused face pass)",
     "ade": "k_ade_box (pole-residue ADE over the dispersive edges)",
     "h": "k_hpass (Yee H + CPML H, fused face pass)",
-    "step_plain": "k_step<R, false, false, false> (one launch per step, interior tiles: "
-                  "Yee E and H of each plane; 60 B per owned cell)",
+    "step_plain": "k_step<R, false, false, false, media type> (one launch per step, interior "
+                  "tiles: Yee E and H of each plane)",
 }
 
 
@@ -312,14 +324,16 @@ def main():
         # interior (plain) class moves exactly 60 B per owned cell
         ms_k = {}
         classes = []
+        cb = onelaunch_cell_bytes(pb)
         for cls, cells, ms_c in s.step_classes(reps):
             key = "step_" + CLASS_NAMES[cls].replace("+", "_")
             ms_k[key] = ms_c
-            nbytes[key] = int(60 * cells)
+            nbytes[key] = int(cb * cells)
             classes.append({"class": CLASS_NAMES[cls], "cells": int(cells), "ms": round(ms_c, 4)})
         dom = max(ms_k, key=ms_k.get)
+        KERNELS["step_plain"] += f", {cb} B per owned cell"
         if dom != "step_plain":
-            KERNELS[dom] = (f"k_step tile class {dom[5:]} (60 B per owned cell counted: a lower "
+            KERNELS[dom] = (f"k_step tile class {dom[5:]} ({cb} B per owned cell counted: a lower "
                             f"bound of its compulsory bytes)")
         step_bytes = onelaunch_step_bytes(pb, ndisp, args.mode, stored)
     else:

@@ -126,6 +126,21 @@ int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3],
   return (int)hipGetLastError();
 }
 
+// 8-bit copy of a media index array (every index below 256)
+__global__ void __launch_bounds__(256) k_media_narrow(const int16_t *src, uint8_t *dst,
+                                                     long long n) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x)
+    dst[t] = (uint8_t)src[t];
+}
+
+int launch_media_narrow(const int16_t *src, uint8_t *dst, long long n, void *stream) {
+  if (n <= 0) return 0;
+  const unsigned b = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_media_narrow, dim3(b), dim3(256), 0, (hipStream_t)stream, src, dst, n);
+  return (int)hipGetLastError();
+}
+
 // Incid of every TF/SF face cell for the one-launch step (the same
 // arithmetic as k_tfsf_list, which applies it in place)
 __global__ void __launch_bounds__(256) k_pw_incident(const PwFace *faces, int nf0,

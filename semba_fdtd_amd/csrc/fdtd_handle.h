@@ -146,6 +146,7 @@ struct fdtd_handle {
   // class with tiles, each over its slice of the device tile list
   std::vector<int> step_cls, step_cnt, step_off;
   std::vector<double> step_cells;        // Yee cells owned per class
+  uint8_t *mi8[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // step mode
   void *stage = nullptr;                 // packed staging buffer of host transfers
   size_t stage_bytes = 0;
   // the stored previous field of the pole updates may differ from the field

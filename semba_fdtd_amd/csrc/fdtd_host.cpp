@@ -407,6 +407,9 @@ static int build_step(fdtd_handle *h) {
       sizes.push_back(a.prev[c] ? (size_t)std::max(a.nstate[c], 1LL) * 4 : 0);
       sizes.push_back(a.jst[c] ? h->adein[side].jbytes[c] : 0);
     }
+  // 8-bit copies of the media indices: a sixth less traffic per cell
+  const bool narrow = h->nmedia + 1 <= 256;
+  for (int c = 0; c < 6; ++c) sizes.push_back(narrow ? ne : 0);
   int rc;
   std::vector<void *> ptr(sizes.size(), nullptr);
   if (sizes != h->step_sizes) {
@@ -420,6 +423,7 @@ static int build_step(fdtd_handle *h) {
     }
     h->step_sizes = sizes;
     h->step_stale = true;
+    for (int c = 0; c < 6; ++c) h->mi8[c] = (uint8_t *)ptr[30 + c];
     h->gB = h->g;
     for (int c = 0; c < 6; ++c) h->gB.f[c] = (float *)ptr[c];
     for (int s = 0; s < 12; ++s) h->gB.psi[s] = (float *)ptr[6 + s];
@@ -429,6 +433,14 @@ static int build_step(fdtd_handle *h) {
         h->gB.ade[side].jst[c] = ptr[18 + side * 6 + c * 2 + 1];
       }
   }
+  // the 8-bit media copies are refreshed from the int16 arrays at every
+  // set-up (the media may have changed since)
+  if (narrow)
+    for (int c = 0; c < 6; ++c) {
+      if (!h->mi8[c]) return fail(h, FDTD_ERR_STATE, "8-bit media copy missing");
+      if (launch_media_narrow(h->g.mi[c], h->mi8[c], (long long)ne, h->stream))
+        return fail(h, FDTD_ERR_HIP, "media narrowing failed");
+    }
   // everything but the state pointers follows A
   DevGrid gb = h->g;
   for (int c = 0; c < 6; ++c) gb.f[c] = h->gB.f[c];
@@ -609,6 +621,8 @@ static int build_step(fdtd_handle *h) {
   if ((rc = upload(h, &dsrc, &src, 1, &h->step_small))) return rc;
   sd.src = dsrc;
   h->dstepsrc = dsrc;
+  sd.narrow = narrow ? 1 : 0;
+  for (int c = 0; c < 6; ++c) sd.mi8[c] = narrow ? h->mi8[c] : nullptr;
   sd.pwG[0] = src.pwG[0];
   sd.pwG[1] = src.pwG[1];
   sd.dn = h->dstep;
@@ -618,7 +632,10 @@ static int build_step(fdtd_handle *h) {
     const DevGrid &gi = dir ? h->gB : h->g;
     const DevGrid &go = dir ? h->g : h->gB;
     StepAux a{};
-    for (int c = 0; c < 6; ++c) { a.fi[c] = gi.f[c]; a.fo[c] = go.f[c]; a.mi[c] = gi.mi[c]; }
+    for (int c = 0; c < 6; ++c) {
+      a.fi[c] = gi.f[c]; a.fo[c] = go.f[c]; a.mi[c] = gi.mi[c];
+      a.mi8[c] = narrow ? h->mi8[c] : nullptr;
+    }
     for (int s = 0; s < 12; ++s) { a.psi_i[s] = gi.psi[s]; a.psi_o[s] = go.psi[s]; }
     for (int ax = 0; ax < 3; ++ax) {
       a.pbe[ax] = gi.pbe[ax]; a.pce[ax] = gi.pce[ax];

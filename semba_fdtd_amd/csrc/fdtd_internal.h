@@ -211,6 +211,7 @@ struct StepAux {
   const float *fi[6];         // fields of step n
   float *fo[6];               // fields of step n+1
   const int16_t *mi[6];       // media indices
+  const uint8_t *mi8[6];      // the same in 8 bits (grids with at most 256 media)
   const float *psi_i[12];     // read copy
   float *psi_o[12];           // written copy
   const float *pbe[3], *pce[3], *pbm[3], *pcm[3];
@@ -229,6 +230,8 @@ struct StepDev {
   int merged;                 // entries are tile | (z chunks << 24)
   int prev_from_field;        // tiles without sources take the pole update's
                               // previous field from the field of step n
+  int narrow;                 // media read from the 8-bit copies mi8
+  const uint8_t *mi8[6];
   // TF/SF faces per half (staged in LDS by the blocks that meet them), and
   // the bounding box of each half's nodal cells
   int npw[2];
@@ -339,6 +342,8 @@ int launch_dft(const DevGrid &g, const DftDev &d, int n, const int *dn, void *st
 // tiempo(n) + dt/2); runs[r] covers cells start[r] .. start[r+1]-1 of total
 int launch_tfsf_list(const PwRun *runs, const long long *start, int nruns, long long total,
                      int n, void *stream);
+// an 8-bit copy of n media indices, all below 256
+int launch_media_narrow(const int16_t *src, uint8_t *dst, long long n, void *stream);
 // a dense box of es-byte elements (es 2 or 4), ext = its x, y, z extents,
 // moved between a packed device buffer and a grid array
 int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3], long long px,

@@ -1265,7 +1265,7 @@ static __host__ __device__ __forceinline__ size_t step_tab_bytes(int nmedia, int
 // tiles that meet the pole-state box of some component; SRC, tiles that
 // touch a TF/SF face or a nodal cell.  A class's kernel carries only the code
 // its tiles need, so interior tiles run the plain Yee march.
-template <int R, bool CPML, bool ADE, bool SRC>
+template <int R, bool CPML, bool ADE, bool SRC, class MT>
 __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
   extern __shared__ float2 lds[];
   const DevGrid &gi = sd.gi;
@@ -1408,10 +1408,19 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
     // arithmetic was measured: no faster, 16 more registers.)
     PlaneLoad cur{};
     if (eplane) {
-      if constexpr (CPML || ADE || SRC)
-        load_plane(cur, a->fi, a->mi, P, ob, obx, oby, om);
-      else
-        load_plane(cur, gi.f, gi.mi, P, ob, obx, oby, om);
+      // media from the int16 arrays or their 8-bit copies (MT)
+      if constexpr (sizeof(MT) == 1) {
+        const unsigned om8 = ob >> 2;
+        if constexpr (CPML || ADE || SRC)
+          load_plane(cur, a->fi, a->mi8, P, ob, obx, oby, om8);
+        else
+          load_plane(cur, gi.f, sd.mi8, P, ob, obx, oby, om8);
+      } else {
+        if constexpr (CPML || ADE || SRC)
+          load_plane(cur, a->fi, a->mi, P, ob, obx, oby, om);
+        else
+          load_plane(cur, gi.f, gi.mi, P, ob, obx, oby, om);
+      }
     }
     const float hx0 = cur.hx, hy0 = cur.hy, hz0 = cur.hz, hz_jm = cur.hz_jm, hx_jm = cur.hx_jm,
                 hz_im = cur.hz_im, hy_im = cur.hy_im, ex0 = cur.ex, ey0 = cur.ey, ez0 = cur.ez;
@@ -1719,8 +1728,13 @@ int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)sd.tm.nblk);
   if (sd.tm.nblk <= 0) return 0;
-#define SFD_STEP(RR, C, A, S) \
-  hipLaunchKernelGGL((k_step<RR, C, A, S>), grid, dim3(kWave, RR), lds, s, sd)
+#define SFD_STEP(RR, C, A, S)                                                        \
+  do {                                                                               \
+    if (sd.narrow)                                                                   \
+      hipLaunchKernelGGL((k_step<RR, C, A, S, uint8_t>), grid, dim3(kWave, RR), lds, s, sd); \
+    else                                                                             \
+      hipLaunchKernelGGL((k_step<RR, C, A, S, int16_t>), grid, dim3(kWave, RR), lds, s, sd); \
+  } while (0)
 #define SFD_STEP_R(RR)                                         \
   switch (cls & 7) {                                           \
     case 0: SFD_STEP(RR, false, false, false); break;          \

@@ -181,3 +181,34 @@ def test_step_after_set_field_with_poles(mode):
     for c in range(6):
         assert np.array_equal(_bits(b.field(c)), _bits(a.field(c))), c
     assert np.array_equal(_bits(b.probes()), _bits(a.probes()))
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_step_many_media(mode):
+    """More media than an 8-bit index holds (300): the step kernels read the
+    int16 media arrays and equal the two-pass run bit for bit (with at most
+    256 media they read 8-bit copies, which every other case here covers)."""
+    n, nm = 24, 300
+    base = P.cavity_problem(n=n, nsteps=80)
+    rng = np.random.default_rng(9)
+    mi = base.mi.copy()
+    inner = (slice(4, n - 3),) * 3
+    for c in range(6):
+        mi[c][inner] = rng.integers(2, nm + 1, size=mi[c][inner].shape).astype(mi.dtype)
+    pb = P.Problem(n, n, n, epr=[1.0, 1.0] + list(rng.uniform(1.0, 6.0, nm - 1)),
+                   sigma=[0.0, 0.0] + list(rng.uniform(0.0, 0.2, nm - 1)),
+                   mur=[1.0] * (nm + 1), sigmam=[0.0] * (nm + 1),
+                   kind=[P.KIND_PEC] + [0] * nm, mi=mi, D=base.D, dt=base.dt,
+                   nodal=base.nodal, probes=base.probes, nsteps_max=80, name="many_media")
+    assert pb.nmedia + 1 > 256
+    runs = []
+    for step_mode in (False, True):
+        s = sf.Solver(pb, mode=mode)
+        if step_mode:
+            s.set_step_mode(8, 7)
+        s.run(60)
+        runs.append(s)
+    a, b = runs
+    for c in range(6):
+        assert np.array_equal(_bits(b.field(c)), _bits(a.field(c))), c
+    assert np.array_equal(_bits(b.probes()), _bits(a.probes()))
