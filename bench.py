@@ -271,11 +271,15 @@ def main():
         from semba_fdtd_amd import tuning
         tuned = tuning.autotune(s, reps=3, keep_state=False)
         tiling = {"passes": list(tuned["tiling"]), "step": list(tuned["step"]), "tuned": True}
+        if tuned["step"][0]:
+            tiling["class_rows"] = s.get_step_class_rows()
     else:
         t = [int(v) for v in args.tiling.split(",")]
         s.set_tiling(t[0], t[1], t[2])
         s.set_step_mode(t[3], t[4])
         tiling = {"passes": t[:3], "step": t[3:5], "tuned": False}
+        if t[3]:
+            tiling["class_rows"] = s.get_step_class_rows()
     print(f"[bench] tiling {tiling}", file=sys.stderr, flush=True)
     s.run(args.warmup)  # builds pole state, captures the graph
     s.synchronize()

@@ -179,6 +179,18 @@ int fdtd_time_halves(fdtd_handle *h, int32_t reps, float ms[3]);
  * only.  No counterpart in the reference. */
 int fdtd_set_step_mode(fdtd_handle *h, int32_t rows, int32_t zchunk);
 int fdtd_get_step_mode(const fdtd_handle *h, int32_t *rows, int32_t *zchunk);
+/* Rows per block (4, 6, 8 or 16; 0: the default) of the step kernels of the
+ * tile classes in the bit mask classes (bit c: class c as fdtd_step_classes
+ * numbers them).  A class on fewer rows than the step runs the step's tiles
+ * cut into bands of rows - 1 rows, so rows - 1 must divide the step's
+ * rows - 1 (16-row steps: 4 or 6); other values leave the class on the
+ * step's rows.  Smaller blocks give a kernel that needs many registers per
+ * lane more of them; by default the class with CPML, poles and sources runs
+ * 4-row blocks under 16-row steps, every other class the step's rows.
+ * fdtd_get_step_class_rows gives the rows each class runs with.  The results
+ * do not change.  No counterpart in the reference; used by the autotuner. */
+int fdtd_set_step_class_rows(fdtd_handle *h, int32_t classes, int32_t rows);
+int fdtd_get_step_class_rows(const fdtd_handle *h, int32_t rows[8]);
 /* Average ms of one one-launch step (the step kernels alone), over reps
  * pairs; the fields advance. */
 int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms);

@@ -73,6 +73,8 @@ SIGNATURES = {
     "fdtd_time_halves": (C.c_int, [vp, C.c_int32, f32p]),
     "fdtd_set_step_mode": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "fdtd_get_step_mode": (C.c_int, [vp, i32p, i32p]),
+    "fdtd_set_step_class_rows": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "fdtd_get_step_class_rows": (C.c_int, [vp, i32p]),
     "fdtd_time_step": (C.c_int, [vp, C.c_int32, f32p]),
     "fdtd_step_classes": (C.c_int, [vp, C.c_int32, i32p, f64p, f32p]),
     "fdtd_add_planewave": (C.c_int, [vp, i32p, C.c_int32, f32p, f32p, f32p, C.c_float,

@@ -133,8 +133,12 @@ struct fdtd_handle {
   bool graph_ok = false;
   // one-launch steps (k_step): a second copy of the time-stepping state (B),
   // the A->B and B->A descriptors and a captured pair of steps
-  int step_rows = 0;              // 0: two passes per step; 4 or 8 rows per block
+  int step_rows = 0;              // 0: two passes per step; else rows per block (4, 8, 16)
   int step_kc = 32;
+  int step_creq[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // rows per block asked for each tile
+                                  // class (0: the default); a class on fewer rows
+                                  // than step_rows runs the step's tiles cut into
+                                  // row bands
   bool step_ready = false;
   bool step_stale = true;         // B's fields must be refreshed from A
   std::vector<void *> step_allocs;       // the second state
@@ -145,6 +149,8 @@ struct fdtd_handle {
   // tiles grouped by class (bit 0 CPML, 1 poles, 2 sources): one launch per
   // class with tiles, each over its slice of the device tile list
   std::vector<int> step_cls, step_cnt, step_off;
+  std::vector<int> step_crows, step_cty, step_cmerged;  // rows, tiles along y, merged
+                                                        // entries of each class launch
   std::vector<double> step_cells;        // Yee cells owned per class
   uint8_t *mi8[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // step mode
   void *stage = nullptr;                 // packed staging buffer of host transfers

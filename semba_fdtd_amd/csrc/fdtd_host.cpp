@@ -392,6 +392,17 @@ bool step_supported(const fdtd_handle *h, int rows) {
   return lds <= 48 * 1024;
 }
 
+// rows per block of tile class c: the rows asked for, if they cut the step's
+// tiles into whole bands, else the step's.  By default the class with CPML,
+// poles and sources runs 4-row blocks under 16-row tiles: its 16-row kernel
+// needs more registers than 1024-thread blocks leave a lane and spills.
+static int class_rows(const fdtd_handle *h, int c) {
+  const int R = h->step_rows;
+  if (R <= 0) return 0;
+  const int q = h->step_creq[c] ? h->step_creq[c] : (c == 7 && R == 16 ? 4 : R);
+  return q > 1 && q <= R && (R - 1) % (q - 1) == 0 ? q : R;
+}
+
 static int build_step(fdtd_handle *h) {
   if (h->step_ready) return FDTD_OK;
   const int R = h->step_rows;
@@ -577,6 +588,13 @@ static int build_step(fdtd_handle *h) {
     // sources must: their per-plane face masks span one chunk).
     // Entry: base tile | (chunks << 24).
     const int M = ntiles < (1LL << 24) ? std::max(1, std::min(64 / tm.kc, 127)) : 1;
+    int E[8], S[8];
+    for (int c = 0; c < 8; ++c) {
+      E[c] = class_rows(h, c);
+      S[c] = (R - 1) / (E[c] - 1);
+      if (ntiles * S[c] > 0x7fffffffLL)
+        return fail(h, FDTD_ERR_ARG, "step tiling gives too many blocks");
+    }
     sd.merged = M > 1 ? 1 : 0;
     std::vector<unsigned char> covered((size_t)ntiles, 0);
     for (int bz = 0; bz < tm.tz; ++bz)
@@ -586,22 +604,37 @@ static int build_step(fdtd_handle *h) {
           if (covered[(size_t)t]) continue;
           const int c = tcls[(size_t)t];
           int len = 1;
-          if (c == 0)
+          if (c == 0 && S[c] == 1)
             while (len < M && bz + len < tm.tz &&
                    tcls[(size_t)(t + (long long)len * tm.tx * tm.ty)] == c) {
               covered[(size_t)(t + (long long)len * tm.tx * tm.ty)] = 1;
               ++len;
             }
-          lists[c].push_back(sd.merged ? (int)(t | ((long long)len << 24)) : (int)t);
+          if (c == 0 && S[c] == 1) {
+            lists[c].push_back(sd.merged ? (int)(t | ((long long)len << 24)) : (int)t);
+            continue;
+          }
+          // a class on smaller blocks: the tile's rows are cut into S bands of
+          // E - 1 rows, numbered on a tile grid S times finer along y (bands
+          // past the last row are left out)
+          for (int q = 0; q < S[c]; ++q) {
+            const long long bys = (long long)by * S[c] + q;
+            if (bys * (E[c] - 1) > h->g.ny) break;
+            lists[c].push_back((int)((bz * (long long)tm.ty * S[c] + bys) * tm.tx + bx));
+          }
         }
     std::vector<int> all;
     h->step_cls.clear(); h->step_cnt.clear(); h->step_off.clear(); h->step_cells.clear();
+    h->step_crows.clear(); h->step_cty.clear(); h->step_cmerged.clear();
     for (int c = 0; c < 8; ++c) {
       if (lists[c].empty()) continue;
       h->step_cls.push_back(c);
       h->step_cells.push_back(owned[c]);
       h->step_cnt.push_back((int)lists[c].size());
       h->step_off.push_back((int)all.size());
+      h->step_crows.push_back(E[c]);
+      h->step_cty.push_back(tm.ty * S[c]);
+      h->step_cmerged.push_back(c == 0 && S[c] == 1 ? sd.merged : 0);
       all.insert(all.end(), lists[c].begin(), lists[c].end());
     }
     h->dtiles = nullptr;
@@ -675,19 +708,30 @@ static int sync_b(fdtd_handle *h) {
 // handle's stream.  (They read the same state and write disjoint cells, but
 // running the small classes on side streams beside the interior one was
 // measured slower: they compete for the same bandwidth and caches.)
+// the descriptor of class launch q in direction dir
+static StepDev class_sd(const fdtd_handle *h, int dir, size_t q) {
+  StepDev sd = h->sd[dir];
+  sd.tiles = h->dtiles + h->step_off[q];
+  sd.tm.nblk = h->step_cnt[q];
+  sd.tm.ty = h->step_cty[q];
+  sd.merged = h->step_cmerged[q];
+  return sd;
+}
+
+static int launch_class(const fdtd_handle *h, const StepDev &sd, size_t q) {
+  return h->mode == FDTD_MODE_EXACT ? exact::launch_step(sd, h->step_crows[q], h->step_cls[q], h->stream)
+                                    : fast::launch_step(sd, h->step_crows[q], h->step_cls[q], h->stream);
+}
+
 static int launch_step_kernels(fdtd_handle *h, int dir, bool exact_prev = false) {
   if (h->step_nf > 0 &&
       launch_pw_incident(h->dsfaces, h->step_nf0, h->step_nf, h->step_maxcells, h->dstepsrc,
                          h->dstep, h->dinc, h->stream))
     return fail(h, FDTD_ERR_HIP, "incident-field launch failed");
   for (size_t q = 0; q < h->step_cls.size(); ++q) {
-    StepDev sd = h->sd[dir];
-    sd.tiles = h->dtiles + h->step_off[q];
-    sd.tm.nblk = h->step_cnt[q];
+    StepDev sd = class_sd(h, dir, q);
     sd.prev_from_field = exact_prev ? 0 : 1;
-    const int e = h->mode == FDTD_MODE_EXACT
-                      ? exact::launch_step(sd, h->step_rows, h->step_cls[q], h->stream)
-                      : fast::launch_step(sd, h->step_rows, h->step_cls[q], h->stream);
+    const int e = launch_class(h, sd, q);
     if (e) return fail(h, FDTD_ERR_HIP, "step launch failed: %s", hipGetErrorString((hipError_t)e));
   }
   return FDTD_OK;
@@ -1720,6 +1764,22 @@ int fdtd_set_step_mode(fdtd_handle *h, int32_t rows, int32_t zchunk) {
   return FDTD_OK;
 }
 
+int fdtd_set_step_class_rows(fdtd_handle *h, int32_t classes, int32_t rows) {
+  if (!h || classes < 0 || classes > 0xff ||
+      !(rows == 0 || rows == 4 || rows == 6 || rows == 8 || rows == 16))
+    return fail(h, FDTD_ERR_ARG, "class rows %d (classes 0x%x) not supported", rows, classes);
+  for (int c = 0; c < 8; ++c)
+    if (classes >> c & 1) h->step_creq[c] = rows;
+  invalidate_graph(h);
+  return FDTD_OK;
+}
+
+int fdtd_get_step_class_rows(const fdtd_handle *h, int32_t rows[8]) {
+  if (!h || !rows) return FDTD_ERR_ARG;
+  for (int c = 0; c < 8; ++c) rows[c] = class_rows(h, c);
+  return FDTD_OK;
+}
+
 int fdtd_get_step_mode(const fdtd_handle *h, int32_t *rows, int32_t *zchunk) {
   if (!h || !rows || !zchunk) return FDTD_ERR_ARG;
   *rows = h->step_rows;
@@ -1792,13 +1852,8 @@ int fdtd_step_classes(fdtd_handle *h, int32_t reps, int32_t cls[8], double cells
                          h->dstep, h->dinc, h->stream))
     rc = fail(h, FDTD_ERR_HIP, "incident-field launch failed");
   for (int q = 0; q < nc && !rc; ++q) {
-    StepDev sd = h->sd[0];
-    sd.tiles = h->dtiles + h->step_off[q];
-    sd.tm.nblk = h->step_cnt[q];
-    auto one = [&]() {
-      return h->mode == FDTD_MODE_EXACT ? exact::launch_step(sd, h->step_rows, h->step_cls[q], h->stream)
-                                        : fast::launch_step(sd, h->step_rows, h->step_cls[q], h->stream);
-    };
+    const StepDev sd = class_sd(h, 0, (size_t)q);
+    auto one = [&]() { return launch_class(h, sd, (size_t)q); };
     if (one()) { rc = fail(h, FDTD_ERR_HIP, "step launch failed"); break; }
     hipEventRecord(ev[0], h->stream);
     for (int r = 0; r < reps && !rc; ++r)

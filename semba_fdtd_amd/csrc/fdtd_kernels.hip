@@ -1748,6 +1748,8 @@ int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
   }
   if (rows == 4) {
     SFD_STEP_R(4)
+  } else if (rows == 6) {
+    SFD_STEP_R(6)
   } else if (rows == 8) {
     SFD_STEP_R(8)
   } else if (rows == 16) {

@@ -44,7 +44,8 @@ module sembafdtd_gpu
              fdtd_batch_create, fdtd_batch_destroy, fdtd_batch_admit, fdtd_batch_retire, &
              fdtd_batch_step_of, fdtd_batch_set_tiling, fdtd_batch_step, &
              fdtd_batch_synchronize, fdtd_set_step_mode, fdtd_get_step_mode, fdtd_time_step, &
-             fdtd_step_classes, fdtd_error_message
+             fdtd_step_classes, fdtd_set_step_class_rows, fdtd_get_step_class_rows, &
+             fdtd_error_message
 
    interface
       integer(c_int) function fdtd_create(desc, h) bind(C, name="fdtd_create")
@@ -386,6 +387,17 @@ module sembafdtd_gpu
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: h
          integer(c_int32_t), intent(out) :: rows, zchunk
+      end function
+      integer(c_int) function fdtd_set_step_class_rows(h, classes, rows) &
+            bind(C, name="fdtd_set_step_class_rows")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: classes, rows
+      end function
+      integer(c_int) function fdtd_get_step_class_rows(h, rows) bind(C, name="fdtd_get_step_class_rows")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(out) :: rows(8)
       end function
       integer(c_int) function fdtd_time_step(h, reps, ms) bind(C, name="fdtd_time_step")
          import :: c_int, c_ptr, c_int32_t, c_float

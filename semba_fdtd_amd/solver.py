@@ -414,6 +414,20 @@ class Solver:
         time-stepping state."""
         self._ck(self.lib.fdtd_set_step_mode(self.h, rows, zchunk), "fdtd_set_step_mode")
 
+    def set_step_class_rows(self, rows: int, classes: int = 0xFF):
+        """Rows per block (4, 6, 8 or 16; 0: the default) of the step kernels
+        of the tile classes in the bit mask ``classes`` (bit c: class c, bits
+        1 CPML, 2 poles, 4 sources).  A class on fewer rows than the step runs
+        the step's tiles cut into row bands.  The results do not change."""
+        self._ck(self.lib.fdtd_set_step_class_rows(self.h, classes, rows),
+                 "fdtd_set_step_class_rows")
+
+    def get_step_class_rows(self):
+        """Rows per block each of the 8 tile classes runs with."""
+        r = (C.c_int32 * 8)()
+        self._ck(self.lib.fdtd_get_step_class_rows(self.h, r), "fdtd_get_step_class_rows")
+        return [int(v) for v in r]
+
     def step_classes(self, reps: int = 0):
         """The tile classes of the one-launch step: [(class bits, owned Yee
         cells, ms per launch or None)], bits 1 CPML, 2 poles, 4 sources."""

@@ -7,7 +7,10 @@ Two ways to advance a step are timed on the grid itself with device events:
   (``fdtd_time_halves``);
 * one launch per step (``fdtd_set_step_mode``, ``fdtd_time_step``), blocks of
   64 x rows threads marching zchunk planes, which reads every field once and
-  writes it once per step but keeps a second copy of the state.
+  writes it once per step but keeps a second copy of the state.  Its tile
+  classes (interior, CPML, poles, sources) run kernels of their own; each
+  class then gets the block rows that run its tiles fastest
+  (``fdtd_set_step_class_rows``, timed per class with ``fdtd_step_classes``).
 
 The best choice depends on the grid's shape (ragged x edges, thin z), on
 binary16 storage and on the share of dispersive cells, so ``autotune`` keeps
@@ -56,9 +59,31 @@ def _save(path: Optional[str]) -> None:
 
 
 def _apply(solver, choice: tuple) -> None:
-    rows, zchunk, xcells, srows, skc = choice
+    rows, zchunk, xcells, srows, skc = choice[:5]
     solver.set_tiling(rows, zchunk, xcells)
     solver.set_step_mode(srows, skc)
+    solver.set_step_class_rows(0)
+    for c, r in enumerate(choice[5:13]):
+        solver.set_step_class_rows(r, 1 << c)
+
+
+def _class_rows(solver, reps: int, verbose: bool) -> list:
+    """Per tile class, the block rows (among the step's and the row bands
+    that divide its tiles) whose kernel runs the class fastest."""
+    srows = solver.get_step_mode()[0]
+    options = [r for r in (srows, 4, 6, 8) if r <= srows and (srows - 1) % (r - 1) == 0]
+    best = {}
+    for r in dict.fromkeys(options):
+        solver.set_step_class_rows(r)
+        for c, _, ms in solver.step_classes(reps):
+            if verbose:
+                print(f"class {c} rows {r}: {ms:.3f} ms", flush=True)
+            if c not in best or ms < best[c][1]:
+                best[c] = (r, ms)
+    rows = [srows] * 8
+    for c, (r, _) in best.items():
+        rows[c] = r
+    return rows
 
 
 def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: int = 4,
@@ -66,9 +91,10 @@ def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: in
              steps: Iterable[StepShape] = STEP_CANDIDATES) -> dict:
     """Time every candidate on ``solver``'s grid and set the fastest.
 
-    Returns {"tiling": (rows, zchunk, xcells), "step": (rows, zchunk), "ms":
-    ms per step of the choice, "table": [...], "step_table": [...]}; step
-    rows 0 means the two passes won.  Binary16 storage always runs four cells
+    Returns {"tiling": (rows, zchunk, xcells), "step": (rows, zchunk),
+    "class_rows": [rows of each tile class] or None, "ms": ms per step of the
+    choice, "table": [...], "step_table": [...]}; step rows 0 means the two
+    passes won.  Binary16 storage always runs four cells
     per lane and has no one-launch step.  ``keep_state=False`` skips the
     checkpoint around the sweep: right for a grid at rest (all fields and pole
     state zero, which the source-free passes leave zero), where copying the
@@ -78,12 +104,13 @@ def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: in
     if key in _cache:
         _apply(solver, _cache[key])
         c = _cache[key]
-        return {"tiling": c[:3], "step": c[3:], "ms": None, "table": [], "step_table": [],
-                "cached": True}
+        return {"tiling": c[:3], "step": c[3:5], "class_rows": list(c[5:13]) or None,
+                "ms": None, "table": [], "step_table": [], "cached": True}
     fp16 = getattr(solver, "storage", "fp32") == "fp16"
     state = solver.save_state() if keep_state else None
     before = solver.get_tiling() + solver.get_step_mode()
     table, step_table = [], []
+    class_rows = None
     try:
         solver.set_step_mode(0, 32)
         for rows, zchunk, xcells in candidates:
@@ -101,28 +128,44 @@ def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: in
                     solver.set_step_mode(rows, zchunk)
                 except FdtdError:
                     continue
+                solver.set_step_class_rows(0)
                 ms = solver.time_step(reps)
                 step_table.append(((rows, zchunk), ms))
                 if verbose:
                     print(f"one launch per step rows={rows} zchunk={zchunk}: {ms:.3f} ms",
                           flush=True)
+        if step_table:
+            bs = min(step_table, key=lambda r: r[1])
+            best_passes = min((r[1] + r[2] for r in table), default=float("inf"))
+            if bs[1] < best_passes:
+                solver.set_step_mode(*bs[0])
+                class_rows = _class_rows(solver, reps, verbose)
+                solver.set_step_class_rows(0)
+                for c, r in enumerate(class_rows):
+                    solver.set_step_class_rows(r, 1 << c)
+                ms = solver.time_step(reps)
+                step_table.append((tuple(bs[0]) + ("class rows",), ms))
+                if ms >= bs[1]:
+                    class_rows = None
     finally:
         if state is not None:
             solver.load_state(state)
     if not table:
         _apply(solver, before)
-        return {"tiling": before[:3], "step": before[3:], "ms": None, "table": [],
-                "step_table": step_table, "cached": False}
+        return {"tiling": before[:3], "step": before[3:], "class_rows": None, "ms": None,
+                "table": [], "step_table": step_table, "cached": False}
     best = min(table, key=lambda r: r[1] + r[2])
     ms = best[1] + best[2]
     step = (0, 32)
     if step_table:
         bs = min(step_table, key=lambda r: r[1])
         if bs[1] < ms:
-            step, ms = bs[0], bs[1]
-    choice = tuple(best[0]) + tuple(step)
+            step, ms = tuple(bs[0][:2]), bs[1]
+    if step[0] == 0 or len(bs[0]) == 2:
+        class_rows = None
+    choice = tuple(best[0]) + tuple(step) + tuple(class_rows or ())
     _apply(solver, choice)
     _cache[key] = choice
     _save(cache)
-    return {"tiling": best[0], "step": step, "ms": ms, "table": table,
-            "step_table": step_table, "cached": False}
+    return {"tiling": best[0], "step": step, "class_rows": class_rows, "ms": ms,
+            "table": table, "step_table": step_table, "cached": False}

@@ -137,10 +137,12 @@ def test_autotune_keeps_state():
     s = sf.Solver(pb, mode="exact")
     s.run(40)
     out = tuning.autotune(s, candidates=[(4, 32, 1), (2, 16, 4), (1, 8, 4)], reps=2,
-                          steps=[(4, 16), (8, 7)])
+                          steps=[(4, 16), (8, 7), (16, 5)])
     assert out["tiling"] in [(4, 32, 1), (2, 16, 4), (1, 8, 4)]
-    assert out["step"] in [(0, 32), (4, 16), (8, 7)]
+    assert out["step"] in [(0, 32), (4, 16), (8, 7), (16, 5)]
     assert s.get_tiling() == out["tiling"] and s.get_step_mode() == out["step"]
+    if out["class_rows"] is not None:
+        assert s.get_step_class_rows() == out["class_rows"]
     assert s.n == 40
     s.run(ns - 40)
     assert np.array_equal(s.probes().view(np.uint32), ref.probes().view(np.uint32))

@@ -212,3 +212,36 @@ def test_step_many_media(mode):
     for c in range(6):
         assert np.array_equal(_bits(b.field(c)), _bits(a.field(c))), c
     assert np.array_equal(_bits(b.probes()), _bits(a.probes()))
+
+
+@pytest.mark.parametrize("name", ["graded_one_sided", "wide_x", "planewave_oblique",
+                                  "magnetic_mur"])
+def test_step_class_rows_equal_oracle(name):
+    """Tile classes run in smaller blocks over row bands of the step's tiles
+    (4 and 6 rows under 16-row tiles, every class or only the interior one):
+    EXACT still equals the oracle bit for bit, FAST the two-pass FAST run."""
+    pb = CASES[name]()
+    ns = pb.nsteps_max
+    ref = oracle.Oracle(pb)
+    want = ref.run(ns)
+    a = sf.Solver(pb, mode="fast")
+    a.run(ns)
+    for mode in ("exact", "fast"):
+        for rows, classes in ((4, 0xFF), (6, 0xFF), (6, 0x01)):
+            print(f"step class rows {name} {mode} 16 x 5, rows {rows} classes {classes:#x}",
+                  flush=True)
+            s = sf.Solver(pb, mode=mode)
+            s.set_step_mode(16, 5)
+            assert s.get_step_class_rows() == [16] * 7 + [4]
+            s.set_step_class_rows(rows, classes)
+            assert s.get_step_class_rows() == [rows if classes >> c & 1 else (4 if c == 7 else 16)
+                                               for c in range(8)]
+            s.run(ns)
+            base = ref.fields if mode == "exact" else [a.field(c) for c in range(6)]
+            assert np.array_equal(_bits(s.probes()), _bits(want if mode == "exact" else a.probes()))
+            for c in range(6):
+                assert np.array_equal(_bits(s.field(c)), _bits(base[c])), (mode, rows, c)
+            cl = s.step_classes()
+            assert sum(c[1] for c in cl) == pb.nx * pb.ny * pb.nz
+            s.close()
+    a.close()
