@@ -325,7 +325,8 @@ def main():
     classes = None
     if onelaunch:
         # one launch per step: each tile class's kernel timed alone; the
-        # interior (plain) class moves exactly 60 B per owned cell
+        # interior (plain) class moves exactly onelaunch_cell_bytes per owned
+        # cell
         ms_k = {}
         classes = []
         cb = onelaunch_cell_bytes(pb)
