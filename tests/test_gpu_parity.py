@@ -2,7 +2,7 @@
 
 EXACT mode must be bit-identical to the oracle (which is bit-identical to the
 reference's own modules, tests/test_oracle_reference.py).  FAST mode (FMA
-contraction, folded fp32 poles) is judged against the FP64 oracle run: its
+contraction, folded fp32 poles) is checked against the FP64 oracle run: its
 probe relative L2 error must be <= 1e-5 (the FP32 target) or, where the
 reference's own FP32 arithmetic is already further than that from FP64 (the
 lossless resonant cavities), no more than 1.5x the reference's error.
