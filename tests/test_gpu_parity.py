@@ -287,6 +287,34 @@ def test_full_size_linearity(mode):
     assert np.abs(2.0 * pa - pbb).max() <= 1e-35
 
 
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+def test_full_size_step_equals_two_passes(mode):
+    """The benchmark's path at its full size: the 1024^3 heterogeneous grid
+    advanced with one launch per step (blocks of 64 x 16, 16-plane chunks,
+    the default rows per tile class) gives the same probe samples, bit for
+    bit, as the two passes; probes sit in the interior, the spheres, the
+    pole medium, the CPML and next to the TF/SF faces."""
+    print(f"full size step mode {mode}", flush=True)
+    ns = 60
+    pb = P.heterogeneous_problem(n=1024, nsteps=ns)
+    w = pb.planewaves[0]
+    z0 = w.box[4]
+    c = pb.nx // 2
+    pb.probes = list(pb.probes) + [(P.EX, c, c, z0), (P.EX, c, c, z0 + 1), (P.HY, c, c, z0),
+                                   (P.HY, c, c, z0 - 1), (P.EZ, c, c, z0), (P.EX, 5, c, z0 + 1),
+                                   (P.HY, c, 3, z0), (P.EY, c, c, 1020)]
+    out = []
+    for step in (False, True):
+        s = sf.Solver(pb, mode=mode)
+        if step:
+            s.set_step_mode(16, 16)
+        s.run(ns)
+        out.append(s.probes())
+        s.close()
+    assert np.isfinite(out[0]).all() and np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+
+
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 def test_dft_sums(mode):
     """Frequency-domain boxes (E and H, strided, 4 frequencies) through the
