@@ -96,7 +96,7 @@ def onelaunch_step_bytes(pb, ndisp_edges, mode, stored_poles):
 
 
 CLASS_NAMES = {0: "plain", 1: "cpml", 2: "poles", 3: "cpml+poles", 4: "sources",
-               5: "cpml+sources", 6: "poles+sources", 7: "cpml+poles+sources"}
+               5: "cpml+sources", 6: "poles+sources", 7: "cpml+poles+sources", 8: "plain4"}
 
 KERNELS = {
     "e_face": "k_epass (Yee E + CPML E, fused face pass)",
@@ -104,44 +104,103 @@ KERNELS = {
     "h": "k_hpass (Yee H + CPML H, fused face pass)",
     "step_plain": "k_step<R, false, false, false, media type> (one launch per step, interior "
                   "tiles: Yee E and H of each plane)",
+    "step_plain4": "k_step4<R, media type> (one launch per step, interior tiles four x cells "
+                   "per lane: Yee E and H of each plane)",
 }
 
 
-def cpu_baseline(n_sample, steps):
-    """The oracle (C restatement, OpenMP over k/j like the reference loops)
-    on the host cores, on the same heterogeneous workload at n_sample^3."""
+def _cpu_child(n_sample, steps, variant):
+    """Child process of cpu_baseline: on the heterogeneous workload at
+    n_sample^3, the oracle (variant "nofma" or "fma") or the reference's own
+    modules built with OpenMP (variant "reference"); prints one JSON line."""
     import oracle
     from semba_fdtd_amd import problem as P
-    pb = P.heterogeneous_problem(n=n_sample, L=max(4, 12 * n_sample // 1024), nsteps=steps,
+    pb = P.heterogeneous_problem(n=n_sample, L=max(4, 12 * n_sample // 1024), nsteps=steps + 2,
                                  host_media=True)
-    o = oracle.Oracle(pb)
+    if variant == "reference":
+        o = oracle.Reference(pb, omp=True)
+    else:
+        o = oracle.Oracle(pb, fma=variant == "fma")
     o.run(2)
     t0 = time.perf_counter()
     o.run(steps)
     dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(pb.cells * steps / dt / 1e6, 2), "unit": "Mcell-updates/s",
-            "cores": threads, "kind": "port",
-            "sample": f"oracle (fdtd_oracle.c, -O2 OpenMP {threads} threads) on the same "
-                      f"heterogeneous workload at {n_sample}^3 (CPML {pb.cpml_layers[0][0]}, "
-                      f"spheres, 20% 2-pole medium, complex128 poles), {steps} steps, "
-                      f"{dt:.1f} s"}
+    print(json.dumps({"cells": pb.cells, "steps": steps, "seconds": dt,
+                      "L": pb.cpml_layers[0][0]}), flush=True)
 
 
-def batch_leg(count, steps, plain_sample, mode):
+def physical_cores():
+    """Physical cores of the host (lscpu: cores per socket x sockets)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        kv = {k.strip(): v.strip() for k, _, v in (ln.partition(":") for ln in out.splitlines())}
+        return int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(n_sample, steps):
+    """CPU throughput on the host's physical cores -- at most the CPU share
+    the process is given (OMP_NUM_THREADS) -- bound close, one thread per
+    core, on the same heterogeneous workload at n_sample^3: the reference's
+    own compiled modules (oracle/_ref_omp: calc_constants, borderscpml,
+    electricdispersive, planewaves ... with their OpenMP directives, driven
+    by the harness that restates the Yee loops of timestepping.F90) where
+    that build travelled with the tree, and the oracle (C restatement, OpenMP
+    over k/j) with FMA contraction off (its parity build) and on (an
+    x86-64-v3 build).  Each runs in a child process so that the OpenMP
+    placement applies."""
+    import subprocess
+    import oracle
+    cores = physical_cores()
+    threads = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    variants = ["nofma", "fma"] + (["reference"] if os.path.exists(oracle.REF_OMP_SO) else [])
+    res = {}
+    for v in variants:
+        # the reference's serial pole loop makes it about 5x slower: a
+        # quarter of the steps keeps its sample near the others' length
+        ns = max(8, steps // 4) if v == "reference" else steps
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child",
+                              str(n_sample), str(ns), v], env=env, capture_output=True,
+                             text=True, timeout=900, check=True)
+        res[v] = json.loads(out.stdout.strip().splitlines()[-1])
+    rate = {v: r["cells"] * r["steps"] / r["seconds"] / 1e6 for v, r in res.items()}
+    r = res["fma"]
+    kind = "reference" if "reference" in rate else "port"
+    value = rate["reference"] if kind == "reference" else rate["fma"]
+    ref_txt = (f"reference modules (OpenMP build) {rate['reference']:.1f}; "
+               if kind == "reference" else "")
+    return {"value": round(value, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": kind,
+            "port_fma_on": round(rate["fma"], 2), "port_fma_off": round(rate["nofma"], 2),
+            "physical_cores": cores,
+            "sample": f"{threads} threads of {cores} physical cores, OMP_PROC_BIND=close "
+                      f"OMP_PLACES=cores, heterogeneous workload at {n_sample}^3 (CPML "
+                      f"{r['L']}, spheres, 20% 2-pole medium, complex128 poles), {steps} steps "
+                      f"(reference: {max(8, steps // 4)}): "
+                      f"{ref_txt}oracle fdtd_oracle.c -O2 FMA on (x86-64-v3) "
+                      f"{rate['fma']:.1f}, FMA off {rate['nofma']:.1f} Mcell-updates/s"}
+
+
+def batch_leg(count, steps, plain_sample, mode, storage="fp32"):
     """Batched uneven instances (log-uniform 32-128 cells per axis, 8-cell
     CPML, 1-6 blobs, plane wave): all instances advanced together, against a
-    sample of them solved one at a time on their own graphs."""
+    seeded random sample of the same instances solved one at a time, each on
+    its own captured graph.  Both sides are set up and warmed (2 steps)
+    before their timers start: the timed work is the same steps of the same
+    kind of grid."""
     import torch
     import semba_fdtd_amd as sf
     from semba_fdtd_amd import problem as P
     from semba_fdtd_amd.batch import Batch
     insts = P.batch_instances(count=count, nsteps=steps)
     cells = sum(p.cells for p in insts)
-    print(f"[bench] batch: {count} instances, {cells / 1e6:.1f} M cells, {steps} steps",
+    print(f"[bench] batch {storage}: {count} instances, {cells / 1e6:.1f} M cells, {steps} steps",
           file=sys.stderr, flush=True)
-    sol = [sf.Solver(p, mode=mode, probe_buffer=steps) for p in insts]
-    b = Batch()
+    sol = [sf.Solver(p, mode=mode, probe_buffer=steps, storage=storage) for p in insts]
+    b = Batch(xcells=4 if storage == "fp16" else 1)
     for s in sol:
         b.admit(s)
     b.step(2)
@@ -156,47 +215,57 @@ def batch_leg(count, steps, plain_sample, mode):
     for s in sol:
         s.close()
     del sol
-    # plain path on a sample: each instance alone, one after the other
-    sample = insts[:plain_sample]
-    scells = sum(p.cells for p in sample)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for p in sample:
-        s = sf.Solver(p, mode=mode, probe_buffer=steps)
-        s.run(steps)
+    # plain path on a seeded random sample: each instance alone, set up and
+    # warmed before its timer
+    rng = np.random.default_rng(1004)
+    pick = sorted(rng.choice(count, size=min(plain_sample, count), replace=False).tolist())
+    scells, tp = 0, 0.0
+    for q in pick:
+        s = sf.Solver(insts[q], mode=mode, probe_buffer=steps, storage=storage)
+        s.run(2)
         s.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(steps - 2)
+        s.synchronize()
+        tp += time.perf_counter() - t0
+        scells += insts[q].cells
         s.close()
-    tp = time.perf_counter() - t0
-    plain = scells * steps / tp / 1e6
+    plain = scells * (steps - 2) / tp / 1e6
     return {"instances": count, "cells": cells, "steps": steps, "mode": mode,
+            "storage": storage,
             "batched_mcell_updates_per_s": round(batched, 1),
             "plain_mcell_updates_per_s": round(plain, 1),
-            "plain_sample_instances": len(sample),
+            "plain_sample_instances": len(pick),
             "speedup": round(batched / plain, 2), "probes_finite": finite,
             "note": "batched: one persistent face-pass launch per half step for all instances, "
-                    "graph-replayed; plain: each instance alone on its own graph (set-up "
-                    "included), throughput measured on the first instances"}
+                    "graph-replayed; plain: a seeded random sample (numpy default_rng(1004)) "
+                    "of the same instances, each alone on its own graph; both timed after "
+                    "set-up and 2 warm-up steps"}
 
 
-def serve_leg(count, rate, mode):
+def serve_leg(count, rate, mode, storage="fp32"):
     """Serving stream (Poisson arrivals, cubic 48-192, 500-4000 steps, 30%
     with a 1-pole region, 64-frequency FP64 DFT): continuous batching against
     the plain one-solve-at-a-time path on the same stream."""
     from semba_fdtd_amd import problem as P
     from semba_fdtd_amd import serve
     reqs = P.serving_requests(count=count, rate=rate)
-    print(f"[bench] serve: {count} requests at {rate}/s", file=sys.stderr, flush=True)
-    plain = serve.serve_plain(reqs, mode=mode)
+    print(f"[bench] serve {storage}: {count} requests at {rate}/s", file=sys.stderr, flush=True)
+    plain = serve.serve_plain(reqs, mode=mode, storage=storage)
     # at most SERVE_MAX_CELLS cells admitted at once: enough to fill the
     # device, few enough that early requests finish early
-    batched = serve.serve_batched(reqs, mode=mode, max_cells=SERVE_MAX_CELLS)
+    batched = serve.serve_batched(reqs, mode=mode, storage=storage, max_cells=SERVE_MAX_CELLS,
+                                  xcells=4 if storage == "fp16" else 1)
     same = True
     pm = {r.name: r for r in plain}
     for r in batched:
         same = same and np.array_equal(r.probes.view(np.uint32), pm[r.name].probes.view(np.uint32))
-    return {"requests": count, "arrival_rate_per_s": rate, "mode": mode,
+    bs, ps = serve.summarize(batched), serve.summarize(plain)
+    return {"requests": count, "arrival_rate_per_s": rate, "mode": mode, "storage": storage,
             "batched_max_cells": SERVE_MAX_CELLS,
-            "batched": serve.summarize(batched), "plain": serve.summarize(plain),
+            "batched": bs, "plain": ps,
+            "throughput_ratio": round(bs["mcell_updates_per_s"] / ps["mcell_updates_per_s"], 2),
             "results_identical": bool(same)}
 
 
@@ -210,7 +279,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--disp-fraction", type=float, default=0.2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 1 GiB device copies (the known-size kernel that "
@@ -225,18 +294,14 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1500)
-    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=8.0)
-    ap.add_argument("--hbm-probe", action="store_true",
-                    help="only run semba_fdtd_amd/hbm_probe (achievable HBM bandwidth per access "
-                         "shape) and exit")
+    if len(sys.argv) == 5 and sys.argv[1] == "--cpu-child":
+        _cpu_child(int(sys.argv[2]), int(sys.argv[3]), sys.argv[4])
+        return
     args = ap.parse_args()
-    if args.hbm_probe:
-        import subprocess
-        exe = os.path.join(ROOT, "semba_fdtd_amd", "hbm_probe")
-        sys.exit(subprocess.run([exe]).returncode)
     if args.copy_calibration:
         import torch
         x = torch.ones(2**28, dtype=torch.float32, device="cuda")
@@ -324,28 +389,45 @@ def main():
     onelaunch = bool(s.get_step_mode()[0])
     classes = None
     if onelaunch:
-        # one launch per step: each tile class's kernel timed alone; the
-        # interior (plain) class moves exactly onelaunch_cell_bytes per owned
-        # cell
+        # one launch per step: each tile class's kernel timed alone; its
+        # compulsory bytes are onelaunch_cell_bytes per owned cell plus the
+        # CPML psi and pole state of those cells (the library counts them from
+        # its tile plan)
         ms_k = {}
         classes = []
         cb = onelaunch_cell_bytes(pb)
-        for cls, cells, ms_c in s.step_classes(reps):
+        timed_classes = s.step_classes(reps)
+        cbytes = s.step_class_bytes()
+        for cls, cells, ms_c in timed_classes:
             key = "step_" + CLASS_NAMES[cls].replace("+", "_")
             ms_k[key] = ms_c
-            nbytes[key] = int(cb * cells)
-            classes.append({"class": CLASS_NAMES[cls], "cells": int(cells), "ms": round(ms_c, 4)})
+            nbytes[key] = int(cbytes[cls])
+            classes.append({"class": CLASS_NAMES[cls], "cells": int(cells), "ms": round(ms_c, 4),
+                            "algorithmic_bytes": int(cbytes[cls]),
+                            "gbs": round(cbytes[cls] / (ms_c / 1e3) / 1e9, 1)})
         dom = max(ms_k, key=ms_k.get)
         KERNELS["step_plain"] += f", {cb} B per owned cell"
-        if dom != "step_plain":
-            KERNELS[dom] = (f"k_step tile class {dom[5:]} ({cb} B per owned cell counted: a lower "
-                            f"bound of its compulsory bytes)")
+        KERNELS["step_plain4"] += f", {cb} B per owned cell"
+        if dom not in ("step_plain", "step_plain4"):
+            KERNELS[dom] = (f"k_step tile class {dom[5:]} ({cb} B per owned cell, 8 B per psi "
+                            f"value, 8 + 16 B per pole-state cell)")
         step_bytes = onelaunch_step_bytes(pb, ndisp, args.mode, stored)
     else:
         dom = max(ms_k, key=ms_k.get)
         step_bytes = step_algorithmic_bytes(pb, ndisp, args.mode, stored)
     gbs = {k: nbytes[k] / (ms_k[k] / 1e3) / 1e9 if ms_k[k] > 0 else 0.0 for k in ms_k}
     step_gbs = step_bytes / (ms_per_step / 1e3) / 1e9
+    # the whole step against the half-step yardstick (84 B per cell, 8 B per
+    # psi value, 8 + 16 B per stored pole per dispersive edge) and against the
+    # one-launch step's own compulsory bytes
+    bytes84 = step_algorithmic_bytes(pb, ndisp, args.mode, stored)
+    step_roofline = {
+        "yardstick_84B": {"bytes": bytes84,
+                          "gbs": round(bytes84 / (ms_per_step / 1e3) / 1e9, 1),
+                          "frac": round(bytes84 / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "onelaunch": {"bytes": step_bytes, "gbs": round(step_gbs, 1),
+                      "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+    }
     probes = s.probes()
     finite = bool(np.isfinite(probes).all())
     s.close()
@@ -360,12 +442,17 @@ def main():
         if t.get("workload") == pb.name and t.get("mode") == args.mode:
             traffic = t.get("bytes_per_launch", {}).get(dom)
 
-    batch = None
+    batch = batch16 = None
     if not args.no_batch:
         batch = batch_leg(args.batch_instances, args.batch_steps, args.plain_sample, args.mode)
-    served = None
+        if args.mode == "fast":
+            batch16 = batch_leg(args.batch_instances, args.batch_steps, args.plain_sample,
+                                args.mode, storage="fp16")
+    served = served16 = None
     if not args.no_serve:
         served = serve_leg(args.serve_requests, args.serve_rate, args.mode)
+        if args.mode == "fast":
+            served16 = serve_leg(args.serve_requests, args.serve_rate, args.mode, storage="fp16")
 
     cpu = None
     if not args.no_cpu:
@@ -398,6 +485,7 @@ def main():
             "step_algorithmic_bytes": step_bytes,
             "step_algorithmic_gbs": round(step_gbs, 1),
             "step_hbm_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+            "step_roofline": step_roofline,
             "kernels": {k: {"ms": round(ms_k[k], 4), "algorithmic_bytes": nbytes[k],
                             "gbs": round(gbs[k], 1)} for k in ms_k},
             "probes_finite": finite,
@@ -413,7 +501,9 @@ def main():
         },
         "cpu_baseline": cpu,
         "batch": batch,
+        "batch_fp16": batch16,
         "serve": served,
+        "serve_fp16": served16,
     }
     print(json.dumps(out), flush=True)
 

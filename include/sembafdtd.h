@@ -81,7 +81,8 @@ typedef struct {
 /* Allocates the six fields and media matrices (launch_simulation's field
  * allocation, timestepping.F90:375-402); fields start at zero. */
 int fdtd_create(const fdtd_grid_desc *desc, fdtd_handle **out);
-/* Frees everything (Destroy_All_exceptSGGMxx, timestepping.F90:3177-3227). */
+/* Frees everything (Destroy_All_exceptSGGMxx, timestepping.F90:3177-3227).
+ * FDTD_ERR_STATE, with nothing freed, while h is admitted to a batch. */
 int fdtd_destroy(fdtd_handle *h);
 /* NUL-terminated message of the last failure on h (or of fdtd_create when
  * h is NULL). */
@@ -179,32 +180,44 @@ int fdtd_time_halves(fdtd_handle *h, int32_t reps, float ms[3]);
  * only.  No counterpart in the reference. */
 int fdtd_set_step_mode(fdtd_handle *h, int32_t rows, int32_t zchunk);
 int fdtd_get_step_mode(const fdtd_handle *h, int32_t *rows, int32_t *zchunk);
+/* Tile classes of the one-launch step: 0-7 carry bit 0 for tiles that touch a
+ * CPML slab, bit 1 for tiles that meet a pole-state box, bit 2 for tiles that
+ * touch a TF/SF face or a nodal cell (one lane per x cell); class 8 holds the
+ * plain tiles run four x cells per lane. */
+#define FDTD_STEP_CLASSES 9
 /* Rows per block (4, 6, 8 or 16; 0: the default) of the step kernels of the
- * tile classes in the bit mask classes (bit c: class c as fdtd_step_classes
- * numbers them).  A class on fewer rows than the step runs the step's tiles
- * cut into bands of rows - 1 rows, so rows - 1 must divide the step's
- * rows - 1 (16-row steps: 4 or 6); other values leave the class on the
- * step's rows.  Smaller blocks give a kernel that needs many registers per
- * lane more of them; by default the class with CPML, poles and sources runs
- * 4-row blocks under 16-row steps, every other class the step's rows.
- * fdtd_get_step_class_rows gives the rows each class runs with.  The results
- * do not change.  No counterpart in the reference; used by the autotuner. */
+ * tile classes in the bit mask classes (bit c: class c).  A class on fewer
+ * rows than the step runs the step's tiles cut into bands of rows - 1 rows;
+ * for classes 0-7 rows - 1 must divide the step's rows - 1 (16-row steps: 4
+ * or 6), class 8 runs 4, 8 or 16 rows; other values leave the class on its
+ * default.  Smaller blocks give a kernel that needs many registers per lane
+ * more of them; by default the class with CPML, poles and sources runs 4-row
+ * blocks under 16-row steps, every other class the step's rows.  fdtd_get_step_class_rows gives the rows each class runs with.
+ * The results do not change.  No counterpart in the reference; used by the
+ * autotuner. */
 int fdtd_set_step_class_rows(fdtd_handle *h, int32_t classes, int32_t rows);
-int fdtd_get_step_class_rows(const fdtd_handle *h, int32_t rows[8]);
+int fdtd_get_step_class_rows(const fdtd_handle *h, int32_t rows[FDTD_STEP_CLASSES]);
+/* Plain tiles of the one-launch step run four x cells per lane (on = 1, the
+ * default) or one (on = 0).  The results do not change.  No counterpart in
+ * the reference. */
+int fdtd_set_step_vector(fdtd_handle *h, int32_t on);
 /* Average ms of one one-launch step (the step kernels alone), over reps
  * pairs; the fields advance. */
 int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms);
-/* The tile classes of the one-launch step: cls[q] has bit 0 set for tiles
- * that touch a CPML slab, bit 1 for tiles that meet a pole-state box, bit 2
- * for tiles that touch a TF/SF face or a nodal cell; each class runs its own
- * kernel.  cells[q]: the Yee cells (nx*ny*nz grid) the class's tiles own.
+/* The tile classes of the one-launch step (FDTD_STEP_CLASSES), one kernel
+ * launch each: cls[q] the class, cells[q] the Yee cells (nx*ny*nz grid) its
+ * tiles own, bytes[q] the compulsory HBM bytes of one launch (every field
+ * read and written once and every media index read once per owned cell, 8 B
+ * per CPML psi value, per pole-state cell 8 B of previous field and 16 B per
+ * stored complex64 pole state in FAST, 32 B per complex128 one in EXACT).
  * With reps > 0, ms[q] is the average duration of one launch of the class's
  * kernel over reps launches of the first half of a step pair (the state of
  * step n is read and the second copy rewritten; the grid does not advance).
- * Returns the number of classes (<= 8) or a negative code.  No counterpart in
+ * Returns the number of classes (<= FDTD_STEP_CLASSES) or a negative code.  No counterpart in
  * the reference; used by the benchmark. */
-int fdtd_step_classes(fdtd_handle *h, int32_t reps, int32_t cls[8], double cells[8],
-                      float ms[8]);
+int fdtd_step_classes(fdtd_handle *h, int32_t reps, int32_t cls[FDTD_STEP_CLASSES],
+                      double cells[FDTD_STEP_CLASSES], double bytes[FDTD_STEP_CLASSES],
+                      float ms[FDTD_STEP_CLASSES]);
 
 /* ---- the hot path ----------------------------------------------------- */
 /* Fused E half step: Advance_Ex/Ey/Ez (timestepping.F90:1286-1297) +
@@ -231,7 +244,13 @@ int fdtd_advance_nodal_h(fdtd_handle *h, int32_t n); /* AdvanceNodalH */
 int fdtd_advance_mur(fdtd_handle *h);          /* AdvanceMagneticMUR */
 /* nsteps full steps n0 .. n0+nsteps-1 (E half then H half, probes sampled and
  * frequency-domain sums advanced after each step as UpdateObservation does,
- * timestepping.F90:1725-1742), replayed from a captured HIP graph. */
+ * timestepping.F90:1725-1742), replayed from a captured HIP graph.
+ * n0 >= 0.  Every step samples the point probes once: a run that would take
+ * more samples than the probe buffer has room for since the last
+ * fdtd_read_probes is refused (FDTD_ERR_ARG) before any launch.
+ * Stepping calls (fdtd_run, fdtd_advance_*, fdtd_update_observation,
+ * fdtd_sample_probes, the timing calls) return FDTD_ERR_STATE while h is
+ * admitted to a batch: the batch advances it. */
 int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps);
 
 /* ---- observation ------------------------------------------------------ */
@@ -241,7 +260,10 @@ int fdtd_set_probes(fdtd_handle *h, int32_t nprobe, const int32_t *comp,
                     const int32_t *i, const int32_t *j, const int32_t *k,
                     int32_t bufsteps);
 /* Copies the buffered samples [nsamples][nprobe] to out (capacity
- * max_samples rows), clears the buffer, returns the count in *nsamples. */
+ * max_samples rows), clears the buffer, returns the count in *nsamples.
+ * Calls that sample the probes (fdtd_run, fdtd_update_observation,
+ * fdtd_sample_probes, fdtd_batch_step) refuse to take more than bufsteps
+ * samples between two reads. */
 int fdtd_read_probes(fdtd_handle *h, float *out, int32_t max_samples,
                      int32_t *nsamples);
 /* Sample probes once now (used by drivers that step with fdtd_advance_*). */
@@ -279,6 +301,10 @@ int fdtd_save_state(fdtd_handle *h, int32_t n, void *buf, int64_t size);
 int fdtd_load_state(fdtd_handle *h, const void *buf, int64_t size, int32_t *n);
 /* Device bytes held by the handle. */
 int64_t fdtd_device_bytes(const fdtd_handle *h);
+/* Free and total bytes of device memory (hipMemGetInfo on `device`), for
+ * callers that admit grids by the memory the device has left.  No
+ * counterpart in the reference (it runs one grid per process). */
+int fdtd_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes);
 /* Number of dispersive edges of component comp (0..2 E, 3..5 H) holding pole
  * state (the node lists of InitEDispersives, electricdispersive.F90:148-244,
  * and InitMdispersives, magneticdispersive.F90:139-265); valid after the
@@ -337,7 +363,10 @@ int fdtd_batch_create(int32_t device, void *stream, fdtd_batch **out);
 int fdtd_batch_destroy(fdtd_batch *b);
 const char *fdtd_batch_last_error(const fdtd_batch *b);
 /* Admit grid h at its step n0; *slot names it in the batch.  While admitted
- * its own calls (reads, set-up) run on the batch's stream. */
+ * its own calls (reads, set-up) run on the batch's stream, and its stepping
+ * calls and fdtd_destroy are refused (FDTD_ERR_STATE) until it is retired
+ * (or the batch destroyed).  A grid belongs to at most one batch: admitting
+ * it again, to this batch or another, is refused (FDTD_ERR_STATE). */
 int fdtd_batch_admit(fdtd_batch *b, fdtd_handle *h, int32_t n0, int32_t *slot);
 int fdtd_batch_retire(fdtd_batch *b, int32_t slot);
 /* Step the grid of slot will run next (-1 for an empty slot). */

@@ -29,7 +29,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = {32: os.path.join(HERE, "liboracle_fdtd.so"),
              64: os.path.join(HERE, "liboracle_fdtd64.so")}
+# the real(4) build with FMA contraction allowed (x86-64-v3): only for timing
+# the CPU baseline with FMA on; parity uses the builds above
+ORACLE_FMA_SO = os.path.join(HERE, "liboracle_fdtd_fma.so")
 REF_SO = os.path.join(HERE, "_ref", "libsemba_ref.so")
+# the same modules built with OpenMP: only for timing the reference on the
+# host's cores
+REF_OMP_SO = os.path.join(HERE, "_ref_omp", "libsemba_ref.so")
 PMAX = 8
 PW_MODES = 4
 # vacuum constants as written in the reference (fdetypes.F90:219-222), kept
@@ -96,10 +102,13 @@ def _p(a, t):
 _LIBS = {}
 
 
-def load_oracle(precision: int = 32):
-    if precision in _LIBS:
-        return _LIBS[precision]
-    path = ORACLE_SO[precision]
+def load_oracle(precision: int = 32, fma: bool = False):
+    key = (precision, fma)
+    if key in _LIBS:
+        return _LIBS[key]
+    if fma and precision != 32:
+        raise ValueError("the FMA build is real(4) only")
+    path = ORACLE_FMA_SO if fma else ORACLE_SO[precision]
     if not os.path.exists(path):
         raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
     rt = _REAL[precision][0]
@@ -133,7 +142,7 @@ def load_oracle(precision: int = 32):
                                   _f64p]
     lib.orc_run.argtypes = [C.POINTER(Grid), C.c_int32, C.c_int32, C.c_int32,
                             _i32p, _i32p, _i32p, _i32p, rp]
-    _LIBS[precision] = lib
+    _LIBS[key] = lib
     return lib
 
 
@@ -260,9 +269,9 @@ def lines(lib, pb, precision: int = 32):
 class Oracle:
     """Runs a Problem through the C restatement (precision 32 or 64)."""
 
-    def __init__(self, pb, tables: Tables | None = None, precision: int = 32):
+    def __init__(self, pb, tables: Tables | None = None, precision: int = 32, fma: bool = False):
         self.precision = precision
-        self.lib = load_oracle(precision)
+        self.lib = load_oracle(precision, fma)
         rt, dt_ = _REAL[precision]
         rp = C.POINTER(rt)
         Nod, Grid, PW = _STRUCTS[precision]
@@ -433,10 +442,11 @@ class Oracle:
         return out[:, :npb]
 
 
-def load_reference():
-    if not os.path.exists(REF_SO):
+def load_reference(omp: bool = False):
+    path = REF_OMP_SO if omp else REF_SO
+    if not os.path.exists(path):
         return None
-    lib = C.CDLL(REF_SO)
+    lib = C.CDLL(path)
     lib.ref_init.argtypes = ([C.c_int32] * 4 + [_f32p] * 4 + [_i32p, _i32p, _f64p, _f64p,
                              _f32p, _f32p, _i32p, _f32p, _f32p] + [C.c_float] * 3 +
                              [C.c_double, _f32p, _f32p, _f32p, _i16p, C.c_float, C.c_float])
@@ -459,8 +469,8 @@ class Reference:
     The Fortran modules keep their state in module variables, so one
     Reference can exist per process; tests run it in a subprocess."""
 
-    def __init__(self, pb):
-        lib = load_reference()
+    def __init__(self, pb, omp: bool = False):
+        lib = load_reference(omp)
         if lib is None:
             raise RuntimeError("reference harness not built (make -C oracle ref)")
         self.lib, self.pb = lib, pb

@@ -373,6 +373,7 @@ contains
    ! 2316-2325, 2356-2364, 2396-2404 (global indices instead of offsets)
    subroutine yee_e()
       integer :: i, j, k, m
+!$OMP PARALLEL DO DEFAULT(SHARED) COLLAPSE(2) PRIVATE(i, j, k, m)
       do k = sgg%Sweep(iEx)%ZI, sgg%Sweep(iEx)%ZE
          do j = sgg%Sweep(iEx)%YI, sgg%Sweep(iEx)%YE
             do i = sgg%Sweep(iEx)%XI, sgg%Sweep(iEx)%XE
@@ -381,6 +382,8 @@ contains
             end do
          end do
       end do
+!$OMP END PARALLEL DO
+!$OMP PARALLEL DO DEFAULT(SHARED) COLLAPSE(2) PRIVATE(i, j, k, m)
       do k = sgg%Sweep(iEy)%ZI, sgg%Sweep(iEy)%ZE
          do j = sgg%Sweep(iEy)%YI, sgg%Sweep(iEy)%YE
             do i = sgg%Sweep(iEy)%XI, sgg%Sweep(iEy)%XE
@@ -389,6 +392,8 @@ contains
             end do
          end do
       end do
+!$OMP END PARALLEL DO
+!$OMP PARALLEL DO DEFAULT(SHARED) COLLAPSE(2) PRIVATE(i, j, k, m)
       do k = sgg%Sweep(iEz)%ZI, sgg%Sweep(iEz)%ZE
          do j = sgg%Sweep(iEz)%YI, sgg%Sweep(iEz)%YE
             do i = sgg%Sweep(iEz)%XI, sgg%Sweep(iEz)%XE
@@ -397,10 +402,12 @@ contains
             end do
          end do
       end do
+!$OMP END PARALLEL DO
    end subroutine
 
    subroutine yee_h()
       integer :: i, j, k, m
+!$OMP PARALLEL DO DEFAULT(SHARED) COLLAPSE(2) PRIVATE(i, j, k, m)
       do k = sgg%Sweep(iHx)%ZI, sgg%Sweep(iHx)%ZE
          do j = sgg%Sweep(iHx)%YI, sgg%Sweep(iHx)%YE
             do i = sgg%Sweep(iHx)%XI, sgg%Sweep(iHx)%XE
@@ -409,6 +416,8 @@ contains
             end do
          end do
       end do
+!$OMP END PARALLEL DO
+!$OMP PARALLEL DO DEFAULT(SHARED) COLLAPSE(2) PRIVATE(i, j, k, m)
       do k = sgg%Sweep(iHy)%ZI, sgg%Sweep(iHy)%ZE
          do j = sgg%Sweep(iHy)%YI, sgg%Sweep(iHy)%YE
             do i = sgg%Sweep(iHy)%XI, sgg%Sweep(iHy)%XE
@@ -417,6 +426,8 @@ contains
             end do
          end do
       end do
+!$OMP END PARALLEL DO
+!$OMP PARALLEL DO DEFAULT(SHARED) COLLAPSE(2) PRIVATE(i, j, k, m)
       do k = sgg%Sweep(iHz)%ZI, sgg%Sweep(iHz)%ZE
          do j = sgg%Sweep(iHz)%YI, sgg%Sweep(iHz)%YE
             do i = sgg%Sweep(iHz)%XI, sgg%Sweep(iHz)%XE
@@ -425,6 +436,7 @@ contains
             end do
          end do
       end do
+!$OMP END PARALLEL DO
    end subroutine
 
    ! One step in the order of ciclo_temporal (timestepping.F90:1248-1981)

@@ -75,8 +75,9 @@ SIGNATURES = {
     "fdtd_get_step_mode": (C.c_int, [vp, i32p, i32p]),
     "fdtd_set_step_class_rows": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "fdtd_get_step_class_rows": (C.c_int, [vp, i32p]),
+    "fdtd_set_step_vector": (C.c_int, [vp, C.c_int32]),
     "fdtd_time_step": (C.c_int, [vp, C.c_int32, f32p]),
-    "fdtd_step_classes": (C.c_int, [vp, C.c_int32, i32p, f64p, f32p]),
+    "fdtd_step_classes": (C.c_int, [vp, C.c_int32, i32p, f64p, f64p, f32p]),
     "fdtd_add_planewave": (C.c_int, [vp, i32p, C.c_int32, f32p, f32p, f32p, C.c_float,
                                      C.c_int32, f32p, C.c_float, C.c_float]),
     "fdtd_build_lines": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, f32p, f32p]),
@@ -95,6 +96,7 @@ SIGNATURES = {
     "fdtd_set_field": (C.c_int, [vp, C.c_int32, f32p]),
     "fdtd_synchronize": (C.c_int, [vp]),
     "fdtd_device_bytes": (C.c_int64, [vp]),
+    "fdtd_device_memory": (C.c_int, [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "fdtd_dispersive_edges": (C.c_int64, [vp, C.c_int32]),
     "fdtd_courant_dt": (C.c_double, [C.c_float] * 6 + [C.c_int32, C.c_double]),
     "fdtd_build_g_tables": (C.c_int, [C.c_int32] + [f32p] * 4 + [i32p, C.c_double,

@@ -336,9 +336,12 @@ int fdtd_batch_destroy(fdtd_batch *b) {
   hipSetDevice(b->device);
   hipStreamSynchronize(b->stream);
   for (fdtd_handle *h : b->slots)
-    if (h && h->solo_stream) {
-      h->stream = h->solo_stream;
-      h->solo_stream = nullptr;
+    if (h) {
+      if (h->solo_stream) {
+        h->stream = h->solo_stream;
+        h->solo_stream = nullptr;
+      }
+      h->batch = nullptr;
       invalidate_graph(h);
     }
   drop_descriptors(b);
@@ -350,8 +353,8 @@ int fdtd_batch_destroy(fdtd_batch *b) {
 int fdtd_batch_admit(fdtd_batch *b, fdtd_handle *h, int32_t n0, int32_t *slot) {
   if (!b || !h || !slot || n0 < 0) return bfail(b, FDTD_ERR_ARG, "bad admit arguments");
   if (h->device != b->device) return bfail(b, FDTD_ERR_ARG, "grid lives on another device");
-  for (fdtd_handle *o : b->slots)
-    if (o == h) return bfail(b, FDTD_ERR_ARG, "grid already admitted");
+  // a grid belongs to one batch at a time (this one or another)
+  if (h->batch) return bfail(b, FDTD_ERR_STATE, "grid already admitted to a batch");
   if (b->mode >= 0 && (h->mode != b->mode || h->g.fp16 != b->fp16))
     return bfail(b, FDTD_ERR_ARG, "grids of one batch share the arithmetic mode and storage");
   hipSetDevice(b->device);
@@ -379,6 +382,7 @@ int fdtd_batch_admit(fdtd_batch *b, fdtd_handle *h, int32_t n0, int32_t *slot) {
     b->version.push_back(0);
   }
   b->slots[s] = h;
+  h->batch = b;
   b->step[s] = n0;
   b->ready = false;
   *slot = s;
@@ -396,6 +400,7 @@ int fdtd_batch_retire(fdtd_batch *b, int32_t slot) {
     h->solo_stream = nullptr;
   }
   invalidate_graph(h);
+  h->batch = nullptr;
   b->slots[slot] = nullptr;
   b->ready = false;
   bool any = false;
@@ -436,9 +441,10 @@ int fdtd_batch_step(fdtd_batch *b, int32_t nsteps) {
         ((!h->pwin.empty() || !h->dftin.empty()) && last > (long long)h->tiempo.size()))
       return bfail(b, FDTD_ERR_ARG, "slot %zu: steps up to %lld beyond its time vector", s,
                    last - 1);
-    if (h->nprobe && h->pcap < nsteps)
-      return bfail(b, FDTD_ERR_ARG, "slot %zu: probe buffer holds %d steps, %d asked", s,
-                   h->pcap, nsteps);
+    if (h->nprobe && h->pbuffered + nsteps > h->pcap)
+      return bfail(b, FDTD_ERR_ARG,
+                   "slot %zu: probe buffer holds %d steps, %lld buffered, %d more asked", s,
+                   h->pcap, h->pbuffered, nsteps);
   }
   if (!any || nsteps == 0) return FDTD_OK;
   for (size_t s = 0; s < b->slots.size(); ++s)
@@ -457,7 +463,10 @@ int fdtd_batch_step(fdtd_batch *b, int32_t nsteps) {
   }
   for (int32_t s = 0; s < nsteps; ++s) BCHK(b, hipGraphLaunch(b->gexec, b->stream));
   for (size_t s = 0; s < b->slots.size(); ++s)
-    if (b->slots[s]) b->step[s] += nsteps;
+    if (b->slots[s]) {
+      b->step[s] += nsteps;
+      b->slots[s]->pbuffered += nsteps;
+    }
   return FDTD_OK;
 }
 

@@ -135,10 +135,12 @@ struct fdtd_handle {
   // the A->B and B->A descriptors and a captured pair of steps
   int step_rows = 0;              // 0: two passes per step; else rows per block (4, 8, 16)
   int step_kc = 32;
-  int step_creq[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // rows per block asked for each tile
-                                  // class (0: the default); a class on fewer rows
-                                  // than step_rows runs the step's tiles cut into
-                                  // row bands
+  int step_creq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // rows per block asked for each
+                                  // tile class (0: the default); a class on fewer
+                                  // rows than step_rows runs the step's tiles cut
+                                  // into row bands
+  bool step_vec = true;           // plain tiles run four x cells per lane (k_step4)
+  int lds_max = 65536;            // LDS bytes a block may use on the device
   bool step_ready = false;
   bool step_stale = true;         // B's fields must be refreshed from A
   std::vector<void *> step_allocs;       // the second state
@@ -146,19 +148,20 @@ struct fdtd_handle {
   std::vector<size_t> step_sizes;        // layout of the second state
   sfd::DevGrid gB;
   sfd::StepDev sd[2];             // [0] A -> B, [1] B -> A
-  // tiles grouped by class (bit 0 CPML, 1 poles, 2 sources): one launch per
-  // class with tiles, each over its slice of the device tile list
+  // tiles grouped by class (bit 0 CPML, 1 poles, 2 sources; 8 plain tiles
+  // four cells per lane): one launch per class with tiles, each over its
+  // slice of the device tile list
   std::vector<int> step_cls, step_cnt, step_off;
-  std::vector<int> step_crows, step_cty, step_cmerged;  // rows, tiles along y, merged
-                                                        // entries of each class launch
+  std::vector<int> step_crows;           // rows per block of each class launch
   std::vector<double> step_cells;        // Yee cells owned per class
+  std::vector<double> step_bytes;        // compulsory HBM bytes of one launch per class
   uint8_t *mi8[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // step mode
   void *stage = nullptr;                 // packed staging buffer of host transfers
   size_t stage_bytes = 0;
   // the stored previous field of the pole updates may differ from the field
   // (fields set from outside, granular calls): the next pair reads it
   bool prev_exact = true;
-  int *dtiles = nullptr;
+  sfd::StepTile *dtiles = nullptr;
   // TF/SF faces of both halves for the step kernels and the buffer of their
   // incident field, refreshed before every step
   sfd::PwFace *dsfaces = nullptr;
@@ -170,6 +173,13 @@ struct fdtd_handle {
   hipGraph_t pgraph = nullptr;
   std::vector<sfd::NodalCell> hcells[2];   // host copies of the nodal cells
   unsigned version = 0;           // bumped by every change that drops the graph
+  // the batch the grid is admitted to (null: none).  While it is set the
+  // batch's captured step replays over the grid's buffers and shares its step
+  // counter, so the grid's own stepping calls and fdtd_destroy are refused.
+  fdtd_batch *batch = nullptr;
+  // probe samples taken since the last fdtd_read_probes (or fdtd_set_probes):
+  // a call that would take more than the buffer holds is refused
+  long long pbuffered = 0;
 };
 
 
@@ -177,6 +187,10 @@ struct fdtd_handle {
 namespace sfd_host {
 int fail(fdtd_handle *h, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
 int dalloc(fdtd_handle *h, void **p, size_t bytes, std::vector<void *> *list = nullptr);
+int refuse_batched(fdtd_handle *h);
+// extents of component comp's Alloc box per axis
+void alloc_extent(const fdtd_handle *h, int comp, long long ext[3]);
+int probe_room(fdtd_handle *h, long long more);
 int prepare(fdtd_handle *h);
 int run_pw(fdtd_handle *h, int half, int n, const int *dn);
 int run_nodal(fdtd_handle *h, int half, int n, const int *dn);
@@ -185,3 +199,12 @@ int run_mur(fdtd_handle *h);
 int run_ade(fdtd_handle *h, int side);
 void invalidate_graph(fdtd_handle *h);
 }  // namespace sfd_host
+
+// a failed HIP call on grid h: its message, and FDTD_ERR_HIP from the caller
+#define HIPCHK(h, call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return sfd_host::fail(h, FDTD_ERR_HIP, "%s: %s (%s:%d)", #call,          \
+                            hipGetErrorString(e_), __FILE__, __LINE__);        \
+  } while (0)

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "fdtd_handle.h"
+#include "fdtd_plan.h"
 
 using namespace sfd;
 using namespace sfd_host;
@@ -32,13 +33,6 @@ int fail(fdtd_handle *h, int code, const char *fmt, ...) {
   return code;
 }
 
-#define HIPCHK(h, call)                                                        \
-  do {                                                                         \
-    hipError_t e_ = (call);                                                    \
-    if (e_ != hipSuccess)                                                      \
-      return fail(h, FDTD_ERR_HIP, "%s: %s (%s:%d)", #call,                    \
-                  hipGetErrorString(e_), __FILE__, __LINE__);                  \
-  } while (0)
 
 int dalloc(fdtd_handle *h, void **p, size_t bytes, std::vector<void *> *list) {
   if (bytes == 0) bytes = 16;
@@ -141,6 +135,23 @@ int field_io(fdtd_handle *h, int comp, const float *src, float *dst) {
   if (rc) return rc;
   for (size_t t = 0; t < n; ++t) dst[t] = __half2float(tmp[t]) * s;
   return FDTD_OK;
+}
+
+// stepping calls on a grid that belongs to a batch are refused: the batch's
+// captured step replays over the grid's buffers and reads its step counter
+int refuse_batched(fdtd_handle *h) {
+  return h->batch ? fail(h, FDTD_ERR_STATE, "grid is admitted to a batch: retire it first")
+                  : FDTD_OK;
+}
+
+// room in the probe buffer for `more` samples beyond those buffered since the
+// last read; the probe kernels stop writing at the buffer's end, so a call
+// that would overflow it is refused instead of dropping samples
+int probe_room(fdtd_handle *h, long long more) {
+  if (!h->nprobe || h->pbuffered + more <= h->pcap) return FDTD_OK;
+  return fail(h, FDTD_ERR_ARG,
+              "probe buffer holds %d steps, %lld buffered since the last read, %lld more asked",
+              h->pcap, h->pbuffered, more);
 }
 
 int check_ready(fdtd_handle *h) {
@@ -399,6 +410,11 @@ bool step_supported(const fdtd_handle *h, int rows) {
 static int class_rows(const fdtd_handle *h, int c) {
   const int R = h->step_rows;
   if (R <= 0) return 0;
+  if (c == kPlanVec) {
+    // the four-cell kernel: 4, 8 or 16 rows (the step's by default)
+    const int q = h->step_creq[c] ? h->step_creq[c] : R;
+    return (q == 4 || q == 8 || q == 16) && q <= R ? q : R;
+  }
   const int q = h->step_creq[c] ? h->step_creq[c] : (c == 7 && R == 16 ? 4 : R);
   return q > 1 && q <= R && (R - 1) % (q - 1) == 0 ? q : R;
 }
@@ -468,13 +484,7 @@ static int build_step(fdtd_handle *h) {
   TileMap &tm = sd.tm;
   tm.bdy = R;
   tm.vec = 1;
-  tm.tx = (h->g.nx + 1 + kWave - 2) / (kWave - 1);
-  tm.ty = (h->g.ny + 1 + R - 2) / (R - 1);
   tm.kc = std::min(h->step_kc, h->g.nz + 1);
-  tm.tz = (h->g.nz + 1 + tm.kc - 1) / tm.kc;
-  const long long nblk = (long long)tm.tx * tm.ty * tm.tz;
-  if (nblk > 0x7fffffffLL) return fail(h, FDTD_ERR_ARG, "step tiling gives too many blocks");
-  tm.nblk = (int)nblk;
   tm.remap = 1;
   StepSrc src{};
   // the faces of both halves in one list, each with its slice of the
@@ -523,119 +533,76 @@ static int build_step(fdtd_handle *h) {
       }
     }
   }
-  // tile classes: a tile covers lanes x0..x0+63, rows y0..y0+R-1 and planes
-  // k0..k1 (E is recomputed on k1), halo lanes and rows included
+  // the tile plan (fdtd_plan.cpp): tiles cut at the CPML slabs, the TF/SF
+  // faces, the nodal cells and the pole-state boxes, one launch per class
   {
-    struct Box { int lo[3], hi[3]; };
-    std::vector<Box> ade, srcb;
+    PlanGeom pg;
+    pg.n[0] = h->g.nx; pg.n[1] = h->g.ny; pg.n[2] = h->g.nz;
+    for (int a = 0; a < 3; ++a)
+      for (int s2 = 0; s2 < 2; ++s2) pg.L[a][s2] = h->g.has_cpml ? h->g.L[a][s2] : 0;
+    pg.R = R;
+    pg.kc = tm.kc;
+    for (int c = 0; c < 8; ++c) pg.rows[c] = class_rows(h, c);
+    pg.vrows = class_rows(h, kPlanVec);
+    // the four-cell kernel's LDS (tables and its row exchange) must fit
+    {
+      DevGrid g = h->g;
+      const size_t l4 = h->mode == FDTD_MODE_EXACT ? exact::step4_lds_bytes(g, class_rows(h, kPlanVec))
+                                                   : fast::step4_lds_bytes(g, class_rows(h, kPlanVec));
+      pg.vec = h->step_vec && l4 <= (size_t)h->lds_max;
+    }
     for (int side = 0; side < 2; ++side) {
       const AdeSet &A = h->g.ade[side];
       if (!A.has) continue;
       for (int c = 0; c < 3; ++c) {
         const int *b = A.abox[c];
         if (b[3] <= 0 || b[4] <= 0 || b[5] <= 0) continue;
-        ade.push_back({{b[0], b[1], b[2]}, {b[0] + b[3] - 1, b[1] + b[4] - 1, b[2] + b[5] - 1}});
+        pg.boxes.push_back({{b[0], b[1], b[2]}, {b[0] + b[3] - 1, b[1] + b[4] - 1, b[2] + b[5] - 1}, 2,
+                            3 * side + c});
       }
     }
     for (int half = 0; half < 2; ++half) {
       for (const PwFace &f : h->pw_faces[half]) {
-        Box b;
+        PlanBox b;
         for (int a = 0; a < 3; ++a) { b.lo[a] = f.lo[a]; b.hi[a] = f.hi[a]; }
         b.lo[f.axis] = b.hi[f.axis] = f.fixed;
-        srcb.push_back(b);
+        b.kind = 4;
+        pg.boxes.push_back(b);
       }
       if (sd.ncell[half] > 0) {
         const int *c = sd.cellbox[half];
-        srcb.push_back({{c[0], c[1], c[2]}, {c[3], c[4], c[5]}});
+        pg.boxes.push_back({{c[0], c[1], c[2]}, {c[3], c[4], c[5]}, 4});
       }
     }
-    const int nn[3] = {h->g.nx, h->g.ny, h->g.nz};
-    std::vector<std::vector<int>> lists(8);
-    std::vector<double> owned(8, 0.0);
-    const long long ntiles = (long long)tm.tx * tm.ty * tm.tz;
-    std::vector<unsigned char> tcls((size_t)ntiles);
-    for (int bz = 0; bz < tm.tz; ++bz)
-      for (int by = 0; by < tm.ty; ++by)
-        for (int bx = 0; bx < tm.tx; ++bx) {
-          const int lo[3] = {bx * (kWave - 1), by * (R - 1), bz * tm.kc};
-          const int hi[3] = {lo[0] + kWave - 1, lo[1] + R - 1,
-                             std::min(lo[2] + tm.kc, h->g.nz + 1)};
-          int cls = 0;
-          if (h->g.has_cpml)
-            for (int a = 0; a < 3; ++a)
-              if ((h->g.L[a][0] > 0 && lo[a] < h->g.L[a][0]) ||
-                  (h->g.L[a][1] > 0 && hi[a] >= nn[a] - h->g.L[a][1]))
-                cls |= 1;
-          auto meets = [&](const std::vector<Box> &v) {
-            for (const Box &b : v)
-              if (lo[0] <= b.hi[0] && b.lo[0] <= hi[0] && lo[1] <= b.hi[1] &&
-                  b.lo[1] <= hi[1] && lo[2] <= b.hi[2] && b.lo[2] <= hi[2])
-                return true;
-            return false;
-          };
-          if (meets(ade)) cls |= 2;
-          if (meets(srcb)) cls |= 4;
-          tcls[(size_t)((bz * (long long)tm.ty + by) * tm.tx + bx)] = (unsigned char)cls;
-          // Yee cells the tile owns: 63 columns, R - 1 rows, its planes
-          const long long ox = std::min(lo[0] + kWave - 2, nn[0] - 1) - lo[0] + 1;
-          const long long oy = std::min(lo[1] + R - 2, nn[1] - 1) - lo[1] + 1;
-          const long long oz = std::min(hi[2] - 1, nn[2] - 1) - lo[2] + 1;
-          if (ox > 0 && oy > 0 && oz > 0) owned[cls] += (double)(ox * oy * oz);
-        }
-    // runs of plain chunks above each other merge into one tile of up to 64
-    // planes (fewer recomputed planes and block starts); the other classes
-    // are small and keep one chunk per block for parallelism (and tiles with
-    // sources must: their per-plane face masks span one chunk).
-    // Entry: base tile | (chunks << 24).
-    const int M = ntiles < (1LL << 24) ? std::max(1, std::min(64 / tm.kc, 127)) : 1;
-    int E[8], S[8];
-    for (int c = 0; c < 8; ++c) {
-      E[c] = class_rows(h, c);
-      S[c] = (R - 1) / (E[c] - 1);
-      if (ntiles * S[c] > 0x7fffffffLL)
-        return fail(h, FDTD_ERR_ARG, "step tiling gives too many blocks");
-    }
-    sd.merged = M > 1 ? 1 : 0;
-    std::vector<unsigned char> covered((size_t)ntiles, 0);
-    for (int bz = 0; bz < tm.tz; ++bz)
-      for (int by = 0; by < tm.ty; ++by)
-        for (int bx = 0; bx < tm.tx; ++bx) {
-          const long long t = (bz * (long long)tm.ty + by) * tm.tx + bx;
-          if (covered[(size_t)t]) continue;
-          const int c = tcls[(size_t)t];
-          int len = 1;
-          if (c == 0 && S[c] == 1)
-            while (len < M && bz + len < tm.tz &&
-                   tcls[(size_t)(t + (long long)len * tm.tx * tm.ty)] == c) {
-              covered[(size_t)(t + (long long)len * tm.tx * tm.ty)] = 1;
-              ++len;
-            }
-          if (c == 0 && S[c] == 1) {
-            lists[c].push_back(sd.merged ? (int)(t | ((long long)len << 24)) : (int)t);
-            continue;
-          }
-          // a class on smaller blocks: the tile's rows are cut into S bands of
-          // E - 1 rows, numbered on a tile grid S times finer along y (bands
-          // past the last row are left out)
-          for (int q = 0; q < S[c]; ++q) {
-            const long long bys = (long long)by * S[c] + q;
-            if (bys * (E[c] - 1) > h->g.ny) break;
-            lists[c].push_back((int)((bz * (long long)tm.ty * S[c] + bys) * tm.tx + bx));
-          }
-        }
-    std::vector<int> all;
+    StepPlan plan;
+    std::string perr;
+    if (plan_step(pg, &plan, &perr)) return fail(h, FDTD_ERR_ARG, "%s", perr.c_str());
+    std::vector<StepTile> all;
     h->step_cls.clear(); h->step_cnt.clear(); h->step_off.clear(); h->step_cells.clear();
-    h->step_crows.clear(); h->step_cty.clear(); h->step_cmerged.clear();
-    for (int c = 0; c < 8; ++c) {
-      if (lists[c].empty()) continue;
+    h->step_bytes.clear();
+    // compulsory bytes of a class launch: each field read and written once
+    // and each media index read once per owned cell, 8 B per psi value (read
+    // and write), per pole-state cell the previous field (8 B) and the stored
+    // poles (complex64 folded pairs in FAST, complex128 in EXACT; read and
+    // write)
+    const double cb = 48.0 + 6.0 * (narrow ? 1.0 : 2.0);
+    const double per = h->mode == FDTD_MODE_EXACT ? 32.0 : 16.0;
+    h->step_crows.clear();
+    for (int c = 0; c < kPlanClasses; ++c) {
+      if (plan.tiles[c].empty()) continue;
+      if (plan.tiles[c].size() > 0x7fffffffULL)
+        return fail(h, FDTD_ERR_ARG, "step tiling gives too many blocks");
       h->step_cls.push_back(c);
-      h->step_cells.push_back(owned[c]);
-      h->step_cnt.push_back((int)lists[c].size());
+      h->step_cells.push_back(plan.owned[c]);
+      double by = cb * plan.owned[c] + 8.0 * plan.psi[c];
+      for (int side = 0; side < 2; ++side)
+        if (h->g.ade[side].has)
+          by += plan.disp[c][side] * (8.0 + per * std::max(h->g.ade[side].maxp, 1));
+      h->step_bytes.push_back(by);
+      h->step_cnt.push_back((int)plan.tiles[c].size());
       h->step_off.push_back((int)all.size());
-      h->step_crows.push_back(E[c]);
-      h->step_cty.push_back(tm.ty * S[c]);
-      h->step_cmerged.push_back(c == 0 && S[c] == 1 ? sd.merged : 0);
-      all.insert(all.end(), lists[c].begin(), lists[c].end());
+      h->step_crows.push_back(class_rows(h, c));
+      all.insert(all.end(), plan.tiles[c].begin(), plan.tiles[c].end());
     }
     h->dtiles = nullptr;
     if (!all.empty() && (rc = upload(h, &h->dtiles, all.data(), all.size(), &h->step_small)))
@@ -713,8 +680,6 @@ static StepDev class_sd(const fdtd_handle *h, int dir, size_t q) {
   StepDev sd = h->sd[dir];
   sd.tiles = h->dtiles + h->step_off[q];
   sd.tm.nblk = h->step_cnt[q];
-  sd.tm.ty = h->step_cty[q];
-  sd.merged = h->step_cmerged[q];
   return sd;
 }
 
@@ -1181,6 +1146,7 @@ int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
   h->device = d->device;
   h->mode = d->mode;
   h->storage = d->storage;
+  h->lds_max = (int)std::min<size_t>(prop.sharedMemPerBlock, 1u << 30);
   if (hipSetDevice(d->device) != hipSuccess) { delete h; return fail(nullptr, FDTD_ERR_DEVICE, "hipSetDevice failed"); }
   if (d->stream) {
     h->stream = (hipStream_t)d->stream;
@@ -1230,6 +1196,7 @@ int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
 
 int fdtd_destroy(fdtd_handle *h) {
   if (!h) return FDTD_OK;
+  if (int rc = refuse_batched(h)) return rc;
   hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
   invalidate_graph(h);
@@ -1492,6 +1459,7 @@ int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk, int32_
 
 int fdtd_time_halves(fdtd_handle *h, int32_t reps, float ms[3]) {
   if (!h || reps < 1 || !ms) return FDTD_ERR_ARG;
+  if (int rb = refuse_batched(h)) return rb;
   hipSetDevice(h->device);
   int rc = prepare(h);
   if (rc) return rc;
@@ -1586,12 +1554,15 @@ int fdtd_read_dft(fdtd_handle *h, int32_t id, double *out, int64_t capacity) {
 
 int fdtd_update_observation(fdtd_handle *h, int32_t n) {
   if (!h) return FDTD_ERR_ARG;
+  if (int rb = refuse_batched(h)) return rb;
   hipSetDevice(h->device);
   int rc = prepare(h);
   if (rc) return rc;
+  if ((rc = probe_room(h, 1))) return rc;
   if (h->nprobe && launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap,
                                  h->stream))
     return fail(h, FDTD_ERR_HIP, "probe launch failed");
+  h->pbuffered += h->nprobe ? 1 : 0;
   return run_dft(h, n, nullptr);
 }
 
@@ -1637,6 +1608,7 @@ int fdtd_add_planewave(fdtd_handle *h, const int32_t box[6], int32_t nmodes, con
 
 int fdtd_advance_e(fdtd_handle *h, int32_t n) {
   if (!h) return FDTD_ERR_ARG;
+  if (int rc = refuse_batched(h)) return rc;
   h->prev_exact = true;
   hipSetDevice(h->device);
   int rc;
@@ -1648,6 +1620,7 @@ int fdtd_advance_e(fdtd_handle *h, int32_t n) {
 
 int fdtd_advance_h(fdtd_handle *h, int32_t n) {
   if (!h) return FDTD_ERR_ARG;
+  if (int rc = refuse_batched(h)) return rc;
   h->prev_exact = true;
   hipSetDevice(h->device);
   int rc;
@@ -1660,6 +1633,7 @@ int fdtd_advance_h(fdtd_handle *h, int32_t n) {
 #define SFD_GRANULAR(name, body)                                               \
   int name {                                                                   \
     if (!h) return FDTD_ERR_ARG;                                               \
+    if (int rb = refuse_batched(h)) return rb;                                 \
     h->prev_exact = true;                                                      \
     hipSetDevice(h->device);                                                   \
     int rc = prepare(h);                                                       \
@@ -1680,11 +1654,14 @@ SFD_GRANULAR(fdtd_advance_nodal_h(fdtd_handle *h, int32_t n), run_nodal(h, 1, n,
 #undef SFD_GRANULAR
 
 int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
-  if (!h || nsteps < 0) return fail(h, FDTD_ERR_ARG, "bad run arguments");
+  if (!h || n0 < 0 || nsteps < 0) return fail(h, FDTD_ERR_ARG, "bad run arguments");
+  if (int rb = refuse_batched(h)) return rb;
   hipSetDevice(h->device);
   int rc = prepare(h);
   if (rc) return rc;
   if (nsteps == 0) return FDTD_OK;
+  // every step samples the probes once
+  if ((rc = probe_room(h, nsteps))) return rc;
   if (!h->srcs.empty() && n0 + nsteps > h->nvalues_steps)
     return fail(h, FDTD_ERR_ARG, "steps %d..%d beyond the time vector", n0, n0 + nsteps - 1);
   if ((!h->pwin.empty() || !h->dftin.empty()) && (n0 < 0 || n0 + nsteps > (int)h->tiempo.size()))
@@ -1719,6 +1696,7 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
   }
   if (pairs > 0) h->prev_exact = false;
   for (int32_t s = s0; s < pairs; ++s) HIPCHK(h, hipGraphLaunch(h->pexec, h->stream));
+  h->pbuffered += 2LL * pairs;
   if (singles == 0) return FDTD_OK;
   if (!h->graph_ok) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
@@ -1747,6 +1725,7 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
     h->graph_ok = true;
   }
   for (int32_t s = 0; s < singles; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+  h->pbuffered += singles;
   return FDTD_OK;
 }
 
@@ -1765,18 +1744,25 @@ int fdtd_set_step_mode(fdtd_handle *h, int32_t rows, int32_t zchunk) {
 }
 
 int fdtd_set_step_class_rows(fdtd_handle *h, int32_t classes, int32_t rows) {
-  if (!h || classes < 0 || classes > 0xff ||
+  if (!h || classes < 0 || classes >= (1 << FDTD_STEP_CLASSES) ||
       !(rows == 0 || rows == 4 || rows == 6 || rows == 8 || rows == 16))
     return fail(h, FDTD_ERR_ARG, "class rows %d (classes 0x%x) not supported", rows, classes);
-  for (int c = 0; c < 8; ++c)
+  for (int c = 0; c < FDTD_STEP_CLASSES; ++c)
     if (classes >> c & 1) h->step_creq[c] = rows;
   invalidate_graph(h);
   return FDTD_OK;
 }
 
-int fdtd_get_step_class_rows(const fdtd_handle *h, int32_t rows[8]) {
+int fdtd_get_step_class_rows(const fdtd_handle *h, int32_t rows[FDTD_STEP_CLASSES]) {
   if (!h || !rows) return FDTD_ERR_ARG;
-  for (int c = 0; c < 8; ++c) rows[c] = class_rows(h, c);
+  for (int c = 0; c < FDTD_STEP_CLASSES; ++c) rows[c] = class_rows(h, c);
+  return FDTD_OK;
+}
+
+int fdtd_set_step_vector(fdtd_handle *h, int32_t on) {
+  if (!h || (on != 0 && on != 1)) return fail(h, FDTD_ERR_ARG, "step vector flag %d", on);
+  h->step_vec = on != 0;
+  invalidate_graph(h);
   return FDTD_OK;
 }
 
@@ -1789,6 +1775,7 @@ int fdtd_get_step_mode(const fdtd_handle *h, int32_t *rows, int32_t *zchunk) {
 
 int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms) {
   if (!h || reps < 1 || !ms) return FDTD_ERR_ARG;
+  if (int rb = refuse_batched(h)) return rb;
   if (h->step_rows == 0) return fail(h, FDTD_ERR_STATE, "one-launch steps are off");
   hipSetDevice(h->device);
   int rc;
@@ -1827,9 +1814,11 @@ int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms) {
   return rc;
 }
 
-int fdtd_step_classes(fdtd_handle *h, int32_t reps, int32_t cls[8], double cells[8],
-                      float ms[8]) {
-  if (!h || reps < 0 || !cls || !cells || (reps > 0 && !ms)) return FDTD_ERR_ARG;
+int fdtd_step_classes(fdtd_handle *h, int32_t reps, int32_t cls[FDTD_STEP_CLASSES],
+                      double cells[FDTD_STEP_CLASSES], double bytes[FDTD_STEP_CLASSES],
+                      float ms[FDTD_STEP_CLASSES]) {
+  if (!h || reps < 0 || !cls || !cells || !bytes || (reps > 0 && !ms)) return FDTD_ERR_ARG;
+  if (int rb = refuse_batched(h)) return rb;
   if (h->step_rows == 0) return fail(h, FDTD_ERR_STATE, "one-launch steps are off");
   hipSetDevice(h->device);
   int rc;
@@ -1838,6 +1827,7 @@ int fdtd_step_classes(fdtd_handle *h, int32_t reps, int32_t cls[8], double cells
   for (int q = 0; q < nc; ++q) {
     cls[q] = h->step_cls[q];
     cells[q] = h->step_cells[q];
+    bytes[q] = h->step_bytes[q];
   }
   if (reps == 0) return nc;
   hipEvent_t ev[2];
@@ -1898,6 +1888,7 @@ int fdtd_set_probes(fdtd_handle *h, int32_t nprobe, const int32_t *comp, const i
   HIPCHK(h, hipMemset(h->dpcount, 0, 16));
   h->nprobe = nprobe;
   h->pcap = bufsteps;
+  h->pbuffered = 0;
   invalidate_graph(h);
   return FDTD_OK;
 }
@@ -1905,9 +1896,12 @@ int fdtd_set_probes(fdtd_handle *h, int32_t nprobe, const int32_t *comp, const i
 int fdtd_sample_probes(fdtd_handle *h) {
   if (!h) return FDTD_ERR_ARG;
   if (!h->nprobe) return FDTD_OK;
+  if (int rc = refuse_batched(h)) return rc;
+  if (int rc = probe_room(h, 1)) return rc;
   hipSetDevice(h->device);
   if (launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap, h->stream))
     return fail(h, FDTD_ERR_HIP, "probe launch failed");
+  h->pbuffered += 1;
   return FDTD_OK;
 }
 
@@ -1924,6 +1918,7 @@ int fdtd_read_probes(fdtd_handle *h, float *out, int32_t max_samples, int32_t *n
   const int zero = 0;
   HIPCHK(h, hipMemcpyAsync(h->dpcount, &zero, 4, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->pbuffered = 0;
   *nsamples = cnt;
   return FDTD_OK;
 }
@@ -1969,6 +1964,23 @@ int fdtd_synchronize(fdtd_handle *h) {
 }
 
 int64_t fdtd_device_bytes(const fdtd_handle *h) { return h ? h->bytes : 0; }
+
+int fdtd_device_memory(int32_t device, int64_t *free_bytes, int64_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(nullptr, FDTD_ERR_ARG, "null memory outputs");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return fail(nullptr, FDTD_ERR_DEVICE, "no HIP device %d", device);
+  int cur = 0;
+  hipGetDevice(&cur);
+  size_t f = 0, t = 0;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipMemGetInfo(&f, &t);
+  hipSetDevice(cur);
+  if (e != hipSuccess) return fail(nullptr, FDTD_ERR_HIP, "hipMemGetInfo: %s", hipGetErrorString(e));
+  *free_bytes = (int64_t)f;
+  *total_bytes = (int64_t)t;
+  return FDTD_OK;
+}
 
 int64_t fdtd_dispersive_edges(const fdtd_handle *h, int32_t comp) {
   if (!h || comp < 0 || comp > 5 || !h->ade_ready) return -1;

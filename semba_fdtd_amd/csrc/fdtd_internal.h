@@ -221,13 +221,28 @@ struct StepAux {
   double dt;
 };
 
+// One tile of a one-launch step: owned columns [x0, x0 + wx), rows
+// [y0, y0 + wy) and planes [k0, k1).  The block also computes E on column
+// x0 + wx, row y0 + wy and plane k1 (the halo: the E neighbours of its last H
+// column, row and plane), and stores nothing there.
+//   x: x0 | wx << 20    y: y0 | wy << 20    z: k0    w: k1
+struct StepTile {
+  int x, y, k0, k1;
+};
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline StepTile step_tile(int x0, int wx, int y0, int wy, int k0, int k1) {
+  return StepTile{x0 | (wx << 20), y0 | (wy << 20), k0, k1};
+}
+
 struct StepDev {
   DevGrid gi, go;             // read (step n) and write (step n+1) views
   const StepAux *aux;
-  TileMap tm;                 // tx, ty, tz over strides 63 x (R-1) x kc;
+  TileMap tm;                 // kc: planes per chunk of the tiles with sources
+                              // (their LDS face masks span kc + 1 planes);
                               // nblk: the tiles of this launch
-  const int *tiles;           // tile numbers of this launch's class
-  int merged;                 // entries are tile | (z chunks << 24)
+  const StepTile *tiles;      // the tiles of this launch's class
   int prev_from_field;        // tiles without sources take the pole update's
                               // previous field from the field of step n
   int narrow;                 // media read from the 8-bit copies mi8
@@ -322,6 +337,8 @@ __device__ __forceinline__ float nodal_apply(float v, const NodalCell &c, float 
                int yee, int cpml, int ade, void *stream);                    \
   size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc);       \
   int launch_step(const StepDev &sd, int rows, int cls, void *stream);       \
+  size_t step4_lds_bytes(const DevGrid &g, int rows);                       \
+  int launch_step4(const StepDev &sd, int rows, void *stream);              \
   size_t ade_lds_of(const DevGrid &g, int side);                            \
   int launch_ade_many(const DevGrid *grids, const int *list, int n, int side, \
                       size_t lds, int tbl, long long nmax, void *stream);     \

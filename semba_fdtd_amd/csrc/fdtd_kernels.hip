@@ -1277,11 +1277,10 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
       gi, gi.gm1, gi.gm2, reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offh),
       &gi.ade[1]);
   float2 *xch = reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offx);
-  const int tw = sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)];
-  const int t = sd.merged ? (tw & 0xffffff) : tw;
-  const int nch = sd.merged ? max(tw >> 24, 1) : 1;    // z chunks of this tile
-  const int bx = t % sd.tm.tx, by = (t / sd.tm.tx) % sd.tm.ty, bz = t / (sd.tm.tx * sd.tm.ty);
-  const int k0 = bz * sd.tm.kc;
+  const StepTile tl = sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)];
+  const int x0 = tl.x & 0xfffff, wx = tl.x >> 20;
+  const int y0 = tl.y & 0xfffff, wy = tl.y >> 20;
+  const int k0 = tl.k0;
   // TF/SF faces of both halves in LDS (E faces, then H faces), with the mask
   // of the faces crossing each plane of the chunk (E: k0..k0+kc, H: the same
   // numbering) and the mask of each component's faces
@@ -1320,15 +1319,16 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
   __syncthreads();
 
   const int lane = threadIdx.x, row = threadIdx.y;
-  const int x0 = bx * (kWave - 1);
-  const int i = x0 + lane, j = by * (R - 1) + row;
+  const int i = x0 + lane, j = y0 + row;
   const int nx = gi.nx, ny = gi.ny, nz = gi.nz;
   const bool in = i <= nx && j <= ny;
-  const bool own = in && lane < kWave - 1 && row < R - 1;
+  // the tile owns lanes [0, wx) (wx <= 63) and rows [0, wy) (wy <= R - 1);
+  // lane wx and row wy are its halo
+  const bool own = in && lane < wx && row < wy;
   // lanes past the grid's edge address the edge cell (their results are
   // never used): every load and table index stays inside its array
   const int ic = min(i, nx), jc = min(j, ny);
-  const int k1 = min(k0 + nch * sd.tm.kc, nz + 1);
+  const int k1 = min(tl.k1, nz + 1);
   const long long pxy = gi.pxy;
   const unsigned pxb = (unsigned)gi.px * 4u;
   // byte offsets inside a plane: the cell and its x - 1 and y - 1 neighbours;
@@ -1716,6 +1716,257 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
   }
 }
 
+// ---- plain interior tiles, four x cells per lane ----------------------------
+// k_step4 is k_step's plain path (no CPML slab, pole state or source in the
+// tile) with each lane owning four consecutive x cells: one 16-byte load or
+// store per array, plane and lane, a quarter of the memory instructions per
+// cell.  A tile starts on a 16-byte boundary (x0 % 4 == 0) and owns a
+// multiple of four columns (wx <= 252), so lane wx / 4 holds the halo column;
+// every owned column and the halo lie inside the x sweep of all six
+// components (x0 + wx + 1 <= nx), so a lane's four cells are stored together
+// or not at all.  Inside a lane the x neighbours are the lane's own cells;
+// across lanes the E values at x + 1 and the H values at x - 1 move by DPP
+// wave shifts, lane 0's H at x0 - 1 is a scalar load.  Rows exchange E
+// through LDS as in k_step.  Per cell the arithmetic and its order are
+// k_step's, so the results are the same bits.
+
+// 16-byte and 8-byte vectors as the compiler's own vector types (these load
+// and store through global-address-space pointers)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ f32x4 v4(float a, float b, float c, float d) {
+  f32x4 r;
+  r.x = a; r.y = b; r.z = c; r.w = d;
+  return r;
+}
+
+// lane l receives lane l - 1's v (wave_shr:1); lane 0 receives first
+__device__ __forceinline__ float from_left(float v, float first) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, first),
+                                                               __builtin_bit_cast(int, v), 0x138,
+                                                               0xf, 0xf, false));
+}
+// lane l receives lane l + 1's v (wave_shl:1); lane 63 receives 0
+__device__ __forceinline__ float from_right(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               0x130, 0xf, 0xf, false));
+}
+
+// the four media indices of a lane's cells, from the 8-bit copies (one
+// dword) or the int16 arrays (two dwords)
+template <class MT> struct Media4;
+template <> struct Media4<uint8_t> {
+  typedef uint32_t raw;
+  static __device__ __forceinline__ raw ld(const uint8_t *base, long long P, unsigned ob) {
+    return ld_at(reinterpret_cast<const SFD_GLOBAL uint32_t *>(plane_base(base, P)), ob >> 2);
+  }
+  static __device__ __forceinline__ int at(raw r, int c) { return (int)((r >> (8 * c)) & 0xffu); }
+};
+template <> struct Media4<int16_t> {
+  typedef u32x2 raw;
+  static __device__ __forceinline__ raw ld(const int16_t *base, long long P, unsigned ob) {
+    return ld_at(reinterpret_cast<const SFD_GLOBAL u32x2 *>(plane_base(base, P)), ob >> 1);
+  }
+  static __device__ __forceinline__ int at(raw r, int c) {
+    return (int)(((c < 2 ? r.x : r.y) >> (16 * (c & 1))) & 0xffffu);
+  }
+};
+
+// a wave-uniform field value (scalar load: read-only data within the launch)
+__device__ __forceinline__ float ld_uniform(const float *base, long long P, unsigned ob) {
+  return ((const __attribute__((address_space(4))) float *)(base + P))[ob >> 2];
+}
+
+static __host__ __device__ __forceinline__ size_t step4_xch_offset(int nmedia, int nd0, int nd1) {
+  return (step_tab_bytes(nmedia, nd0) + step_tab_bytes(nmedia, nd1) + 15) & ~(size_t)15;
+}
+
+template <int R, class MT>
+__global__ void __launch_bounds__(64 * R) k_step4(StepDev sd) {
+  typedef Media4<MT> M4;
+  typedef typename M4::raw raw;
+  extern __shared__ float2 lds[];
+  const DevGrid &gi = sd.gi;
+  const DevGrid &go = sd.go;
+  const size_t offh = step_tab_bytes(gi.nmedia, gi.ade[0].ndisp);
+  const size_t offx = step4_xch_offset(gi.nmedia, gi.ade[0].ndisp, gi.ade[1].ndisp);
+  const MedTab te = stage_tables<true, false>(gi, gi.g1, gi.g2, lds);
+  const MedTab th = stage_tables<true, false>(
+      gi, gi.gm1, gi.gm2, reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offh));
+  // E exchange between rows: [2 parities][R rows][64 lanes] of Ex, then of Ez
+  f32x4 *xe = reinterpret_cast<f32x4 *>(reinterpret_cast<char *>(lds) + offx);
+  f32x4 *xz = xe + 2 * R * kWave;
+  const StepTile tl = sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)];
+  const int x0 = tl.x & 0xfffff, wx = tl.x >> 20;
+  const int y0 = tl.y & 0xfffff, wy = tl.y >> 20;
+  const int k0 = tl.k0;
+  const int lane = threadIdx.x;
+  const int row = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const int nx = gi.nx, ny = gi.ny, nz = gi.nz;
+  const int j = y0 + row;
+  const int i = x0 + 4 * lane;          // the lane's first cell
+  const bool own = 4 * lane < wx && row < wy && j <= ny;
+  const int k1 = min(tl.k1, nz + 1);
+  // lanes past the grid's edge address the last whole vector of the row
+  // (their results are never used): every access stays inside its array
+  const int ic = min(i, (nx + 3) & ~3), jc = min(j, ny);
+  const long long pxy = gi.pxy;
+  const unsigned pxb = (unsigned)gi.px * 4u;
+  const unsigned ob = (unsigned)(jc + 1) * pxb + (unsigned)(kXOff + ic) * 4u;
+  const unsigned oby = ob - pxb;
+  const unsigned oe = (unsigned)(jc + 1) * pxb + (unsigned)(kXOff + x0 - 1) * 4u;  // x0 - 1
+  float idxh[4], idxe[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int q = min(ic + c, nx) + 1;
+    idxh[c] = gi.id[3][q];
+    idxe[c] = gi.id[0][q];
+  }
+  const float idyh = ro_uniform(gi.id[4], jc + 1), idye = ro_uniform(gi.id[1], jc + 1);
+  const float *idzh_t = gi.id[5], *idze_t = gi.id[2];
+  // H of step n on plane k0 - 1: the E update's z - 1 neighbours
+  f32x4 hx_c = ld_at(reinterpret_cast<const SFD_GLOBAL f32x4 *>(
+                          plane_base(gi.f[3], (long long)k0 * pxy)), ob);
+  f32x4 hy_c = ld_at(reinterpret_cast<const SFD_GLOBAL f32x4 *>(
+                          plane_base(gi.f[4], (long long)k0 * pxy)), ob);
+  __builtin_amdgcn_s_waitcnt(0);
+  f32x4 hz_c = v4(0.f, 0.f, 0.f, 0.f);
+  f32x4 ex_c = hz_c, ey_c = hz_c, ez_c = hz_c, exj_c = hz_c, ezj_c = hz_c;
+  float ey3_c = 0.f, ez3_c = 0.f;      // Ey, Ez at the next lane's first cell
+  raw mhx_c{}, mhy_c{}, mhz_c{};       // H media of plane k - 1
+  for (int k = k0; k <= k1; ++k) {
+    const long long P = (long long)(k + 1) * pxy;
+    const bool eplane = k <= nz;
+    const bool hk = k > k0;
+    const int kh = k - 1;
+    f32x4 hx = v4(0.f, 0.f, 0.f, 0.f), hy = hx, hz = hx, hzj = hx, hxj = hx;
+    f32x4 ex = hx, ey = hx, ez = hx;
+    raw mx{}, my{}, mz{}, mhx{}, mhy{}, mhz{};
+    float hze = 0.f, hye = 0.f;
+    if (eplane) {
+      const SFD_GLOBAL f32x4 *Hx = reinterpret_cast<const SFD_GLOBAL f32x4 *>(plane_base(gi.f[3], P));
+      const SFD_GLOBAL f32x4 *Hy = reinterpret_cast<const SFD_GLOBAL f32x4 *>(plane_base(gi.f[4], P));
+      const SFD_GLOBAL f32x4 *Hz = reinterpret_cast<const SFD_GLOBAL f32x4 *>(plane_base(gi.f[5], P));
+      hx = ld_at(Hx, ob); hy = ld_at(Hy, ob); hz = ld_at(Hz, ob);
+      hzj = ld_at(Hz, oby); hxj = ld_at(Hx, oby);
+      ex = ld_at(reinterpret_cast<const SFD_GLOBAL f32x4 *>(plane_base(gi.f[0], P)), ob);
+      ey = ld_at(reinterpret_cast<const SFD_GLOBAL f32x4 *>(plane_base(gi.f[1], P)), ob);
+      ez = ld_at(reinterpret_cast<const SFD_GLOBAL f32x4 *>(plane_base(gi.f[2], P)), ob);
+      const MT *mi[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        if constexpr (sizeof(MT) == 1) mi[c] = sd.mi8[c];
+        else mi[c] = gi.mi[c];
+      }
+      mx = M4::ld(mi[0], P, ob); my = M4::ld(mi[1], P, ob); mz = M4::ld(mi[2], P, ob);
+      mhx = M4::ld(mi[3], P, ob); mhy = M4::ld(mi[4], P, ob); mhz = M4::ld(mi[5], P, ob);
+      hze = ld_uniform(gi.f[5], P, oe);
+      hye = ld_uniform(gi.f[4], P, oe);
+    }
+    const float idzh = eplane ? ro_uniform(idzh_t, k + 1) : 0.0f;
+    const float idze = hk ? ro_uniform(idze_t, kh + 1) : 0.0f;
+    const bool wr = own && k < k1;
+    // ---- E of step n+1 on plane k
+    f32x4 exn = v4(0.f, 0.f, 0.f, 0.f), eyn = exn, ezn = exn;
+    if (eplane) {
+      const float hzm[4] = {from_left(hz.w, hze), hz.x, hz.y, hz.z};   // Hz at x - 1
+      const float hym[4] = {from_left(hy.w, hye), hy.x, hy.y, hy.z};   // Hy at x - 1
+      const float HX[4] = {hx.x, hx.y, hx.z, hx.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w},
+                  HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+      const float HZJ[4] = {hzj.x, hzj.y, hzj.z, hzj.w}, HXJ[4] = {hxj.x, hxj.y, hxj.z, hxj.w};
+      const float HXK[4] = {hx_c.x, hx_c.y, hx_c.z, hx_c.w}, HYK[4] = {hy_c.x, hy_c.y, hy_c.z, hy_c.w};
+      const float EX[4] = {ex.x, ex.y, ex.z, ex.w}, EY[4] = {ey.x, ey.y, ey.z, ey.w},
+                  EZ[4] = {ez.x, ez.y, ez.z, ez.w};
+      float rx[4], ry[4], rz[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float2 cx = te.gg[M4::at(mx, c)], cy = te.gg[M4::at(my, c)],
+                     cz = te.gg[M4::at(mz, c)];
+        rx[c] = yee_upd(cx.x, EX[c], cx.y, curl2(HZ[c] - HZJ[c], idyh, HY[c] - HYK[c], idzh));
+        ry[c] = yee_upd(cy.x, EY[c], cy.y, curl2(HX[c] - HXK[c], idzh, HZ[c] - hzm[c], idxh[c]));
+        rz[c] = yee_upd(cz.x, EZ[c], cz.y, curl2(HY[c] - hym[c], idxh[c], HX[c] - HXJ[c], idyh));
+      }
+      exn = v4(rx[0], rx[1], rx[2], rx[3]);
+      eyn = v4(ry[0], ry[1], ry[2], ry[3]);
+      ezn = v4(rz[0], rz[1], rz[2], rz[3]);
+      if (wr) st_at(reinterpret_cast<SFD_GLOBAL f32x4 *>(plane_base(go.f[0], P)), ob, exn);
+      if (wr && j < ny) st_at(reinterpret_cast<SFD_GLOBAL f32x4 *>(plane_base(go.f[1], P)), ob, eyn);
+      if (wr && k < nz) st_at(reinterpret_cast<SFD_GLOBAL f32x4 *>(plane_base(go.f[2], P)), ob, ezn);
+    }
+    // ---- E of plane k at x + 1 (next lane) and y + 1 (next row)
+    const float ey3 = from_right(eyn.x), ez3 = from_right(ezn.x);
+    const int par = (k - k0) & 1;
+    xe[(par * R + row) * kWave + lane] = exn;
+    xz[(par * R + row) * kWave + lane] = ezn;
+    __syncthreads();
+    f32x4 exj = v4(0.f, 0.f, 0.f, 0.f), ezj = exj;
+    if (row + 1 < R) {
+      exj = xe[(par * R + row + 1) * kWave + lane];
+      ezj = xz[(par * R + row + 1) * kWave + lane];
+    }
+    // ---- H of step n+1/2 on plane k - 1
+    if (hk) {
+      const float EXC[4] = {ex_c.x, ex_c.y, ex_c.z, ex_c.w}, EYC[4] = {ey_c.x, ey_c.y, ey_c.z, ey_c.w},
+                  EZC[4] = {ez_c.x, ez_c.y, ez_c.z, ez_c.w};
+      const float EXN[4] = {exn.x, exn.y, exn.z, exn.w}, EYN[4] = {eyn.x, eyn.y, eyn.z, eyn.w};
+      const float EXJ[4] = {exj_c.x, exj_c.y, exj_c.z, exj_c.w}, EZJ[4] = {ezj_c.x, ezj_c.y, ezj_c.z, ezj_c.w};
+      const float EYP[4] = {ey_c.y, ey_c.z, ey_c.w, ey3_c}, EZP[4] = {ez_c.y, ez_c.z, ez_c.w, ez3_c};
+      const float HXC[4] = {hx_c.x, hx_c.y, hx_c.z, hx_c.w}, HYC[4] = {hy_c.x, hy_c.y, hy_c.z, hy_c.w},
+                  HZC[4] = {hz_c.x, hz_c.y, hz_c.z, hz_c.w};
+      float rx[4], ry[4], rz[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float2 cx = th.gg[M4::at(mhx_c, c)], cy = th.gg[M4::at(mhy_c, c)],
+                     cz = th.gg[M4::at(mhz_c, c)];
+        rx[c] = yee_upd(cx.x, HXC[c], cx.y, curl2(EYN[c] - EYC[c], idze, EZJ[c] - EZC[c], idye));
+        ry[c] = yee_upd(cy.x, HYC[c], cy.y, curl2(EZP[c] - EZC[c], idxe[c], EXN[c] - EXC[c], idze));
+        rz[c] = yee_upd(cz.x, HZC[c], cz.y, curl2(EXJ[c] - EXC[c], idye, EYP[c] - EYC[c], idxe[c]));
+      }
+      const long long Ph = P - pxy;
+      if (own && kh < nz && j < ny)
+        st_at(reinterpret_cast<SFD_GLOBAL f32x4 *>(plane_base(go.f[3], Ph)), ob,
+              v4(rx[0], rx[1], rx[2], rx[3]));
+      if (own && kh < nz)
+        st_at(reinterpret_cast<SFD_GLOBAL f32x4 *>(plane_base(go.f[4], Ph)), ob,
+              v4(ry[0], ry[1], ry[2], ry[3]));
+      if (own && j < ny)
+        st_at(reinterpret_cast<SFD_GLOBAL f32x4 *>(plane_base(go.f[5], Ph)), ob,
+              v4(rz[0], rz[1], rz[2], rz[3]));
+    }
+    // ---- carry plane k to the next iteration
+    hx_c = hx; hy_c = hy; hz_c = hz;
+    ex_c = exn; ey_c = eyn; ez_c = ezn;
+    exj_c = exj; ezj_c = ezj;
+    ey3_c = ey3; ez3_c = ez3;
+    mhx_c = mhx; mhy_c = mhy; mhz_c = mhz;
+  }
+}
+
+size_t step4_lds_bytes(const DevGrid &g, int rows) {
+  return step4_xch_offset(g.nmedia, g.ade[0].ndisp, g.ade[1].ndisp) +
+         (size_t)2 * 2 * rows * kWave * 16;
+}
+
+int launch_step4(const StepDev &sd, int rows, void *stream) {
+  const size_t lds = step4_lds_bytes(sd.gi, rows);
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)sd.tm.nblk);
+  if (sd.tm.nblk <= 0) return 0;
+#define SFD_STEP4(RR)                                                                       \
+  do {                                                                                      \
+    if (sd.narrow)                                                                          \
+      hipLaunchKernelGGL((k_step4<RR, uint8_t>), grid, dim3(kWave, RR), lds, s, sd);        \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_step4<RR, int16_t>), grid, dim3(kWave, RR), lds, s, sd);        \
+  } while (0)
+  if (rows == 4) SFD_STEP4(4);
+  else if (rows == 8) SFD_STEP4(8);
+  else if (rows == 16) SFD_STEP4(16);
+  else return (int)hipErrorInvalidValue;
+#undef SFD_STEP4
+  return (int)hipGetLastError();
+}
+
 size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc) {
   const size_t x = step_tab_bytes(g.nmedia, g.ade[0].ndisp) +
                    step_tab_bytes(g.nmedia, g.ade[1].ndisp) +
@@ -1724,6 +1975,7 @@ size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc) {
 }
 
 int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
+  if (cls == 8) return launch_step4(sd, rows, stream);
   const size_t lds = step_lds_bytes(sd.gi, rows, sd.npw[0] + sd.npw[1], sd.tm.kc);
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)sd.tm.nblk);

@@ -35,7 +35,8 @@ module sembafdtd_gpu
              fdtd_set_probes, fdtd_read_probes, fdtd_sample_probes, fdtd_get_field, &
              fdtd_set_field, fdtd_synchronize, fdtd_courant_dt, fdtd_build_g_tables, &
              fdtd_build_spacing, fdtd_build_cpml_axis, fdtd_build_ade, fdtd_evolucion, &
-             fdtd_abi_version, fdtd_device_bytes, fdtd_dispersive_edges, fdtd_media_fill, &
+             fdtd_abi_version, fdtd_device_bytes, fdtd_device_memory, fdtd_dispersive_edges, &
+             fdtd_media_fill, &
              fdtd_media_pec_walls, fdtd_media_paint_sphere, fdtd_media_paint_box, &
              fdtd_set_mdispersive, fdtd_advance_h_dispersive, fdtd_set_mur, fdtd_build_mur, &
              fdtd_advance_mur, fdtd_set_tiling, fdtd_get_tiling, fdtd_time_halves, &
@@ -45,6 +46,7 @@ module sembafdtd_gpu
              fdtd_batch_step_of, fdtd_batch_set_tiling, fdtd_batch_step, &
              fdtd_batch_synchronize, fdtd_set_step_mode, fdtd_get_step_mode, fdtd_time_step, &
              fdtd_step_classes, fdtd_set_step_class_rows, fdtd_get_step_class_rows, &
+             fdtd_set_step_vector, &
              fdtd_error_message
 
    interface
@@ -292,6 +294,12 @@ module sembafdtd_gpu
          import :: c_int64_t, c_ptr
          type(c_ptr), value :: h
       end function
+      integer(c_int) function fdtd_device_memory(device, free_bytes, total_bytes) &
+            bind(C, name="fdtd_device_memory")
+         import :: c_int, c_int32_t, c_int64_t
+         integer(c_int32_t), value :: device
+         integer(c_int64_t), intent(out) :: free_bytes, total_bytes
+      end function
       integer(c_int64_t) function fdtd_dispersive_edges(h, comp) bind(C, name="fdtd_dispersive_edges")
          import :: c_int64_t, c_ptr, c_int32_t
          type(c_ptr), value :: h
@@ -397,7 +405,12 @@ module sembafdtd_gpu
       integer(c_int) function fdtd_get_step_class_rows(h, rows) bind(C, name="fdtd_get_step_class_rows")
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: h
-         integer(c_int32_t), intent(out) :: rows(8)
+         integer(c_int32_t), intent(out) :: rows(9)
+      end function
+      integer(c_int) function fdtd_set_step_vector(h, on) bind(C, name="fdtd_set_step_vector")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: on
       end function
       integer(c_int) function fdtd_time_step(h, reps, ms) bind(C, name="fdtd_time_step")
          import :: c_int, c_ptr, c_int32_t, c_float
@@ -405,14 +418,15 @@ module sembafdtd_gpu
          integer(c_int32_t), value :: reps
          real(c_float), intent(out) :: ms
       end function
-      integer(c_int) function fdtd_step_classes(h, reps, cls, cells, ms) &
+      integer(c_int) function fdtd_step_classes(h, reps, cls, cells, bytes, ms) &
             bind(C, name="fdtd_step_classes")
          import :: c_int, c_ptr, c_int32_t, c_float, c_double
          type(c_ptr), value :: h
          integer(c_int32_t), value :: reps
-         integer(c_int32_t), intent(out) :: cls(8)
-         real(c_double), intent(out) :: cells(8)
-         real(c_float), intent(out) :: ms(8)
+         integer(c_int32_t), intent(out) :: cls(9)
+         real(c_double), intent(out) :: cells(9)
+         real(c_double), intent(out) :: bytes(9)
+         real(c_float), intent(out) :: ms(9)
       end function
       ! ---- frequency-domain observation (observation.F90:3527-3600)
       integer(c_int) function fdtd_add_dft(h, kind, box, stride, nfreq, freq, id) &

@@ -2,22 +2,31 @@
 plain one-solve-at-a-time path.
 
 A request is (arrival time, Problem, steps).  ``serve_batched`` admits
-requests as they arrive while the device-memory budget allows, keeps their
-fields and CPML/pole state resident, and advances all admitted grids together
+requests as they arrive while the device's free memory allows (hipMemGetInfo
+through ``fdtd_device_memory``, less a reserve), keeps their fields and
+CPML/pole state resident, and advances all admitted grids together
 (``Batch``); a grid leaves the batch as soon as its steps are done, its probe
 series and frequency-domain sums are copied out, and waiting requests take
 its place.  ``serve_plain`` solves the same stream one request at a time.
 Both report per-request latency (arrival to results on the host) and the
 aggregate Mcell-updates/s.
+
+Per-request failure handling, at every chunk boundary: a request whose probe
+samples hold a non-finite value leaves the batch with status "diverged" (the
+reference's own per-step NaN check is commented out, errorreport.F90:
+1433-1465), and a request the caller cancels (``cancel(name)`` returns True;
+the reference polls stop files, errorreport.F90:803-870) leaves with status
+"cancelled".  The other requests run on unchanged.
 """
 from __future__ import annotations
 
 import time
 from dataclasses import dataclass
-from typing import List
+from typing import Callable, List, Optional
 
 import numpy as np
 
+from . import _lib
 from .batch import Batch
 from .solver import Solver
 
@@ -31,6 +40,7 @@ class Result:
     done: float
     probes: np.ndarray
     dft: list
+    status: str = "done"       # "done", "diverged" or "cancelled"
 
     @property
     def latency(self) -> float:
@@ -46,14 +56,25 @@ def device_bytes_estimate(pb) -> int:
     return int(base * 1.25) + (1 << 20)
 
 
-def _finish(s: Solver, arrival: float, t0: float, steps: int) -> Result:
+def device_memory(device: int = 0):
+    """(free, total) bytes of the device's memory (hipMemGetInfo)."""
+    import ctypes as C
+    lib = _lib.load()
+    free, total = C.c_int64(), C.c_int64()
+    rc = lib.fdtd_device_memory(device, C.byref(free), C.byref(total))
+    if rc:
+        raise _lib.FdtdError(f"fdtd_device_memory failed ({rc})")
+    return free.value, total.value
+
+
+def _finish(s: Solver, arrival: float, t0: float, steps: int, status: str = "done") -> Result:
     probes = s.probes()
     dft = s.dft_sums()
     pb = s.pb
-    return Result(pb.name, pb.cells, steps, arrival, time.perf_counter() - t0, probes, dft)
+    return Result(pb.name, pb.cells, steps, arrival, time.perf_counter() - t0, probes, dft, status)
 
 
-def serve_plain(requests, mode: str = "fast") -> List[Result]:
+def serve_plain(requests, mode: str = "fast", storage: str = "fp32") -> List[Result]:
     """One request at a time, in arrival order, each on its own graph."""
     t0 = time.perf_counter()
     out = []
@@ -61,61 +82,78 @@ def serve_plain(requests, mode: str = "fast") -> List[Result]:
         now = time.perf_counter() - t0
         if now < arrival:
             time.sleep(arrival - now)
-        s = Solver(pb, mode=mode, probe_buffer=max(ns, 1))
+        s = Solver(pb, mode=mode, probe_buffer=max(ns, 1), storage=storage)
         s.run(ns)
         out.append(_finish(s, arrival, t0, ns))
         s.close()
     return out
 
 
-def serve_batched(requests, mode: str = "fast", mem_budget: int = 200 << 30,
-                  max_chunk: int = 250, rows: int = 4, zchunk: int = 32,
-                  blocks: int = 0, xcells: int = 1,
-                  max_cells: int | None = None) -> List[Result]:
+def serve_batched(requests, mode: str = "fast", storage: str = "fp32",
+                  reserve: int = 8 << 30, max_chunk: int = 250, rows: int = 4,
+                  zchunk: int = 32, blocks: int = 0, xcells: int = 1,
+                  max_cells: Optional[int] = None, device: int = 0,
+                  cancel: Optional[Callable[[str], bool]] = None,
+                  check_finite: bool = True) -> List[Result]:
     """Continuous batching: admit arrived requests, in arrival order, while
-    the memory budget allows and (max_cells) the admitted grids hold fewer
-    cells than max_cells; advance the batch in chunks that end when the next
+    the device's free memory stays above ``reserve`` after the request's
+    estimated footprint and (max_cells) the admitted grids hold fewer cells
+    than max_cells; advance the batch in chunks that end when the next
     admitted grid finishes (or after max_chunk steps), retire finished grids.
     max_cells trades latency for throughput: once the batch holds enough
     cells to fill the device, admitting more only stretches every admitted
-    request's time."""
+    request's time.  At each chunk boundary, requests with non-finite probe
+    samples (check_finite) or cancelled by the caller leave the batch early
+    with status "diverged" / "cancelled"."""
     t0 = time.perf_counter()
     batch = Batch(rows=rows, zchunk=zchunk, blocks=blocks, xcells=xcells)
     waiting = list(requests)
-    active = {}          # slot -> (solver, arrival, steps, bytes)
-    used = 0
+    active = {}          # slot -> (solver, arrival, steps)
     cells = 0
     out = []
+
+    def leave(slot, status):
+        nonlocal cells
+        s, arrival, ns = active.pop(slot)
+        batch.retire(slot)
+        out.append(_finish(s, arrival, t0, ns if status == "done" else s.n, status))
+        cells -= s.pb.cells
+        s.close()
+
     while waiting or active:
         now = time.perf_counter() - t0
-        # admission by device memory, in arrival order
+        # admission by the device's free memory, in arrival order
         while waiting and waiting[0][0] <= now:
             arrival, pb, ns = waiting[0]
-            need = device_bytes_estimate(pb)
-            if active and used + need > mem_budget:
+            free, _ = device_memory(device)
+            if active and free - device_bytes_estimate(pb) < reserve:
                 break
             if active and max_cells is not None and cells >= max_cells:
                 break
             waiting.pop(0)
-            s = Solver(pb, mode=mode, probe_buffer=max(min(ns, max_chunk), 1))
+            s = Solver(pb, mode=mode, probe_buffer=max(min(ns, max_chunk), 1), storage=storage,
+                       device=device)
             slot = batch.admit(s)
-            active[slot] = (s, arrival, ns, need)
-            used += need
+            active[slot] = (s, arrival, ns)
             cells += pb.cells
         if not active:
             if waiting:
                 time.sleep(max(0.0, waiting[0][0] - (time.perf_counter() - t0)))
             continue
-        left = min(ns - s.n for s, _, ns, _ in active.values())
+        left = min(ns - s.n for s, _, ns in active.values())
         k = max(1, min(left, max_chunk))
         batch.step(k)
-        for slot in [sl for sl, (s, _, ns, _) in active.items() if s.n >= ns]:
-            s, arrival, ns, need = active.pop(slot)
-            batch.retire(slot)
-            out.append(_finish(s, arrival, t0, ns))
-            cells -= s.pb.cells
-            s.close()
-            used -= need
+        for slot in [sl for sl, (s, _, ns) in active.items() if s.n >= ns]:
+            leave(slot, "done")
+        for slot in list(active):
+            s = active[slot][0]
+            if cancel is not None and cancel(s.pb.name):
+                leave(slot, "cancelled")
+            elif check_finite and s.nprobe:
+                batch.synchronize()
+                s._drain()
+                if s._samples and not np.isfinite(s._samples[-1]).all():
+                    leave(slot, "diverged")
     batch.close()
     return out
 

@@ -414,29 +414,47 @@ class Solver:
         time-stepping state."""
         self._ck(self.lib.fdtd_set_step_mode(self.h, rows, zchunk), "fdtd_set_step_mode")
 
-    def set_step_class_rows(self, rows: int, classes: int = 0xFF):
+    STEP_CLASSES = 9
+
+    def set_step_class_rows(self, rows: int, classes: int = 0x1FF):
         """Rows per block (4, 6, 8 or 16; 0: the default) of the step kernels
-        of the tile classes in the bit mask ``classes`` (bit c: class c, bits
-        1 CPML, 2 poles, 4 sources).  A class on fewer rows than the step runs
-        the step's tiles cut into row bands.  The results do not change."""
+        of the tile classes in the bit mask ``classes`` (bit c: class c; for
+        c < 8 bits 1 CPML, 2 poles, 4 sources; class 8 the plain tiles run
+        four x cells per lane, 4 or 8 rows).  A class on fewer rows than the
+        step runs the step's tiles cut into row bands.  The results do not
+        change."""
         self._ck(self.lib.fdtd_set_step_class_rows(self.h, classes, rows),
                  "fdtd_set_step_class_rows")
 
+    def set_step_vector(self, on: bool):
+        """Plain tiles of the one-launch step four x cells per lane (the
+        default) or one."""
+        self._ck(self.lib.fdtd_set_step_vector(self.h, 1 if on else 0), "fdtd_set_step_vector")
+
     def get_step_class_rows(self):
-        """Rows per block each of the 8 tile classes runs with."""
-        r = (C.c_int32 * 8)()
+        """Rows per block each of the tile classes runs with."""
+        r = (C.c_int32 * self.STEP_CLASSES)()
         self._ck(self.lib.fdtd_get_step_class_rows(self.h, r), "fdtd_get_step_class_rows")
         return [int(v) for v in r]
 
     def step_classes(self, reps: int = 0):
         """The tile classes of the one-launch step: [(class bits, owned Yee
-        cells, ms per launch or None)], bits 1 CPML, 2 poles, 4 sources."""
-        cls = (C.c_int32 * 8)()
-        cells = (C.c_double * 8)()
-        ms = (C.c_float * 8)()
-        n = self.lib.fdtd_step_classes(self.h, reps, cls, cells, ms)
+        cells, ms per launch or None)], bits 1 CPML, 2 poles, 4 sources; 8 the
+        plain tiles four x cells per lane."""
+        cls = (C.c_int32 * self.STEP_CLASSES)()
+        cells = (C.c_double * self.STEP_CLASSES)()
+        nbytes = (C.c_double * self.STEP_CLASSES)()
+        ms = (C.c_float * self.STEP_CLASSES)()
+        n = self.lib.fdtd_step_classes(self.h, reps, cls, cells, nbytes, ms)
         self._ck(min(n, 0), "fdtd_step_classes")
+        self._class_bytes = {int(cls[q]): float(nbytes[q]) for q in range(n)}
         return [(int(cls[q]), float(cells[q]), float(ms[q]) if reps else None) for q in range(n)]
+
+    def step_class_bytes(self):
+        """{class: compulsory HBM bytes of one launch of its kernel} (fields,
+        media, CPML psi and pole state of the cells its tiles own)."""
+        self.step_classes()
+        return dict(self._class_bytes)
 
     def get_step_mode(self):
         r, k = C.c_int32(), C.c_int32()
@@ -461,8 +479,9 @@ class Solver:
         return int(self.lib.fdtd_device_bytes(self.h))
 
     def close(self):
+        """Free the grid (refused while it is admitted to a batch)."""
         if getattr(self, "h", None):
-            self.lib.fdtd_destroy(self.h)
+            self._ck(self.lib.fdtd_destroy(self.h), "fdtd_destroy")
             self.h = None
 
     def __del__(self):

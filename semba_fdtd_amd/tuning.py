@@ -35,6 +35,8 @@ DEFAULT_CANDIDATES: Tuple[Tiling, ...] = tuple(
     (r, k, x) for x in (1, 4) for r in (1, 2, 4) for k in (16, 32, 64))
 STEP_CANDIDATES: Tuple[StepShape, ...] = ((4, 16), (8, 16), (8, 32), (16, 16), (16, 32))
 
+NCLASS = 9                             # tile classes of the one-launch step
+
 _cache: Dict[str, tuple] = {}
 
 
@@ -63,7 +65,7 @@ def _apply(solver, choice: tuple) -> None:
     solver.set_tiling(rows, zchunk, xcells)
     solver.set_step_mode(srows, skc)
     solver.set_step_class_rows(0)
-    for c, r in enumerate(choice[5:13]):
+    for c, r in enumerate(choice[5:5 + NCLASS]):
         solver.set_step_class_rows(r, 1 << c)
 
 
@@ -72,6 +74,7 @@ def _class_rows(solver, reps: int, verbose: bool) -> list:
     that divide its tiles) whose kernel runs the class fastest."""
     srows = solver.get_step_mode()[0]
     options = [r for r in (srows, 4, 6, 8) if r <= srows and (srows - 1) % (r - 1) == 0]
+    default = solver.get_step_class_rows()
     best = {}
     for r in dict.fromkeys(options):
         solver.set_step_class_rows(r)
@@ -80,7 +83,7 @@ def _class_rows(solver, reps: int, verbose: bool) -> list:
                 print(f"class {c} rows {r}: {ms:.3f} ms", flush=True)
             if c not in best or ms < best[c][1]:
                 best[c] = (r, ms)
-    rows = [srows] * 8
+    rows = list(default)
     for c, (r, _) in best.items():
         rows[c] = r
     return rows
@@ -104,7 +107,7 @@ def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: in
     if key in _cache:
         _apply(solver, _cache[key])
         c = _cache[key]
-        return {"tiling": c[:3], "step": c[3:5], "class_rows": list(c[5:13]) or None,
+        return {"tiling": c[:3], "step": c[3:5], "class_rows": list(c[5:5 + NCLASS]) or None,
                 "ms": None, "table": [], "step_table": [], "cached": True}
     fp16 = getattr(solver, "storage", "fp32") == "fp16"
     state = solver.save_state() if keep_state else None

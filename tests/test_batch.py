@@ -98,3 +98,61 @@ def test_serving_results_equal_plain(max_cells):
             assert np.array_equal(x.view(np.uint64), y.view(np.uint64)), name
     s = serve.summarize(list(batched.values()))
     assert s["requests"] == 5 and s["p99_latency_s"] >= s["p50_latency_s"] > 0
+
+
+def test_admitted_grid_refuses_stepping_and_destroy():
+    """While a grid belongs to a batch its own stepping calls and
+    fdtd_destroy are refused (FDTD_ERR_STATE) before any launch, and it cannot
+    join a second batch; once retired it steps alone again.  Each refused call
+    returns before touching the device, so nothing here can fault."""
+    from semba_fdtd_amd._lib import FdtdError
+    pb = P.scatterer_problem(nsteps=40)
+    s = sf.Solver(pb, mode="fast")
+    lib, h = s.lib, s.h
+    batch = Batch()
+    slot = batch.admit(s)
+    ERR_STATE = -4
+    assert lib.fdtd_run(h, s.n, 2) == ERR_STATE
+    assert lib.fdtd_advance_e(h, 0) == ERR_STATE
+    assert lib.fdtd_advance_h(h, 0) == ERR_STATE
+    assert lib.fdtd_advance_e_yee(h) == ERR_STATE
+    assert lib.fdtd_advance_h_cpml(h) == ERR_STATE
+    assert lib.fdtd_advance_mur(h) == ERR_STATE
+    assert lib.fdtd_update_observation(h, 0) == ERR_STATE
+    assert lib.fdtd_sample_probes(h) == ERR_STATE
+    assert lib.fdtd_destroy(h) == ERR_STATE
+    assert b"batch" in lib.fdtd_last_error(h)
+    other = Batch()
+    with pytest.raises(FdtdError):
+        other.admit(s)
+    with pytest.raises(FdtdError):
+        batch.admit(s)
+    batch.step(10)
+    batch.retire(slot)
+    s.run(10)                                      # alone again
+    assert s.n == 20 and s.probes().shape[0] == 20
+    other.close()
+    batch.close()
+    s.close()
+
+
+def test_run_checks_step_and_probe_room():
+    """fdtd_run refuses a negative first step whatever the grid holds, and a
+    run that would overflow the probe buffer between two reads (the samples
+    would be lost), before any launch."""
+    pb = P.cavity_problem(n=12, nsteps=40)        # nodal source only: no plane wave
+    s = sf.Solver(pb, mode="fast", probe_buffer=10)
+    lib, h = s.lib, s.h
+    ERR_ARG = -1
+    assert lib.fdtd_run(h, -1, 2) == ERR_ARG
+    assert lib.fdtd_run(h, 0, 11) == ERR_ARG
+    assert lib.fdtd_run(h, 0, 6) == 0
+    assert lib.fdtd_run(h, 6, 6) == ERR_ARG      # 6 buffered + 6 > 10
+    assert lib.fdtd_run(h, 6, 4) == 0
+    assert lib.fdtd_sample_probes(h) == ERR_ARG  # full
+    s.n = 10
+    s._pending = 10
+    got = s.probes()                               # reads and clears the buffer
+    assert got.shape == (10, s.nprobe)
+    assert lib.fdtd_run(h, 10, 10) == 0
+    s.close()

@@ -1,0 +1,171 @@
+"""Parity on the benchmark configurations' own workloads.
+
+* Batched uneven instances (8 of the 256 seeded instances of the batched
+  configuration, at their real 32-128 cell edges, 8-cell CPML, blobs, plane
+  wave, 1500 steps), all admitted to one batch: EXACT equals the oracle bit
+  for bit; FAST is within 1e-5 (probe relative L2) of the FP32 oracle and of
+  the FP64 oracle (or 1.5x the FP32 oracle's own distance to FP64).
+* Serving stream (4 of its requests at their real sizes, 62-172 cells per
+  edge, 527-3307 steps, 64-frequency DFT, poles in some): the batched server
+  gives every request the plain path's results bit for bit, in FP32 and
+  binary16 storage; one request equals the FP64 oracle to the FP32
+  tolerance.
+* Open scatterer at its full size (256^3, 10-cell CPML with the L = 10 sigma
+  rule of borderscpml.F90:1494-1497, z graded x1.03, TF/SF plane wave,
+  dielectric sphere), 200 steps: EXACT equals the oracle bit for bit.
+
+Binary16 storage against FP64 is covered in test_fp16.py.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import semba_fdtd_amd as sf
+from semba_fdtd_amd import problem as P
+from semba_fdtd_amd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-5
+REF_FACTOR = 1.5
+BATCH_PICK = [2, 129, 182, 222, 237, 239, 248, 249]   # smallest, largest, 6 seeded
+SERVE_PICK = [7, 11, 14, 6]                            # 62^3 .. 172^3
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def _batch(insts, mode, steps, storage="fp32"):
+    sol = [sf.Solver(p, mode=mode, probe_buffer=steps, storage=storage) for p in insts]
+    b = Batch(xcells=4 if storage == "fp16" else 1)
+    for s in sol:
+        b.admit(s)
+    b.step(steps)
+    b.close()
+    out = [s.probes() for s in sol]
+    for s in sol:
+        s.close()
+    return out
+
+
+def test_batch_instances_exact_equals_oracle():
+    insts = P.batch_instances()
+    pick = [insts[q] for q in BATCH_PICK]
+    steps = pick[0].nsteps_max
+    got = _batch(pick, "exact", steps)
+    for q, pb in enumerate(pick):
+        print(f"batch instance {BATCH_PICK[q]} {pb.nx}x{pb.ny}x{pb.nz} exact", flush=True)
+        want = oracle.Oracle(pb).run(steps)
+        assert got[q].shape == want.shape
+        assert np.array_equal(_bits(got[q]), _bits(want)), BATCH_PICK[q]
+
+
+def test_batch_instances_fast_within_fp32_tolerance():
+    insts = P.batch_instances()
+    pick = [insts[q] for q in BATCH_PICK]
+    steps = pick[0].nsteps_max
+    got = _batch(pick, "fast", steps)
+    for q, pb in enumerate(pick):
+        r32 = oracle.Oracle(pb).run(steps).astype(np.float64)
+        r64 = oracle.Oracle(pb, precision=64).run(steps)
+        e32, e64, eref = rel_l2(got[q], r32), rel_l2(got[q], r64), rel_l2(r32, r64)
+        print(f"batch instance {BATCH_PICK[q]} {pb.nx}x{pb.ny}x{pb.nz} fast: rel-L2 vs FP32 "
+              f"oracle {e32:.3e}, vs FP64 {e64:.3e} (FP32 oracle vs FP64 {eref:.3e})", flush=True)
+        assert np.isfinite(got[q]).all()
+        assert e32 <= FP32_TOL, BATCH_PICK[q]
+        assert e64 <= max(FP32_TOL, REF_FACTOR * eref), BATCH_PICK[q]
+
+
+def _serve_requests():
+    reqs = P.serving_requests()
+    sub = [reqs[q] for q in SERVE_PICK]
+    t0 = min(t for t, _, _ in sub)
+    return [(t - t0, pb, ns) for t, pb, ns in sorted(sub, key=lambda r: r[0])]
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_serving_requests_batched_equals_plain(storage):
+    from semba_fdtd_amd import serve
+    reqs = _serve_requests()
+    print(f"serving {storage}: " + ", ".join(f"{pb.name} {pb.nx}^3 x {ns}" for _, pb, ns in reqs),
+          flush=True)
+    plain = {r.name: r for r in serve.serve_plain(reqs, mode="fast", storage=storage)}
+    batched = {r.name: r for r in serve.serve_batched(reqs, mode="fast", storage=storage,
+                                                      xcells=4 if storage == "fp16" else 1)}
+    assert set(plain) == set(batched)
+    for name in plain:
+        a, b = plain[name], batched[name]
+        assert b.status == "done" and b.probes.shape[0] == b.steps
+        assert np.isfinite(a.probes).all()
+        assert np.array_equal(_bits(a.probes), _bits(b.probes)), name
+        for x, y in zip(a.dft, b.dft):
+            assert np.array_equal(x.view(np.uint64), y.view(np.uint64)), name
+    s = serve.summarize(list(batched.values()))
+    assert s["p99_latency_s"] >= s["p50_latency_s"] > 0
+
+
+def test_serving_request_fast_within_fp32_tolerance():
+    reqs = P.serving_requests()
+    _, pb, ns = reqs[SERVE_PICK[0]]
+    s = sf.Solver(pb, mode="fast", probe_buffer=ns)
+    s.run(ns)
+    got = s.probes()
+    dft = s.dft_sums()
+    s.close()
+    o32, o64 = oracle.Oracle(pb), oracle.Oracle(pb, precision=64)
+    r32, r64 = o32.run(ns).astype(np.float64), o64.run(ns)
+    e32, e64, eref = rel_l2(got, r32), rel_l2(got, r64), rel_l2(r32, r64)
+    d64 = o64.dft_sums()[0]
+    edft = float(np.linalg.norm(dft[0] - d64) / np.linalg.norm(d64))
+    print(f"{pb.name} {pb.nx}^3 x {ns}: rel-L2 vs FP32 oracle {e32:.3e}, vs FP64 {e64:.3e} "
+          f"(FP32 oracle vs FP64 {eref:.3e}); 64-frequency DFT vs FP64 {edft:.3e}", flush=True)
+    assert e32 <= FP32_TOL
+    assert e64 <= max(FP32_TOL, REF_FACTOR * eref)
+    assert edft <= FP32_TOL
+
+
+def test_open_scatterer_full_size_exact_equals_oracle():
+    ns = 200
+    pb = P.open_scatterer_problem(n=256, nsteps=ns)
+    assert pb.cpml_layers == [[10, 10]] * 3
+    print(f"{pb.name} 256^3 exact, {ns} steps", flush=True)
+    want = oracle.Oracle(pb).run(ns)
+    for step_mode in (False, True):
+        s = sf.Solver(pb, mode="exact")
+        if step_mode:
+            s.set_step_mode(16, 16)
+        s.run(ns)
+        got = s.probes()
+        s.close()
+        assert np.abs(want).max() > 0
+        assert np.array_equal(_bits(got), _bits(want)), step_mode
+
+
+def test_heterogeneous1024_2000_steps_bounded():
+    """The large heterogeneous grid (1024^3, 12-cell CPML, 8 lossy spheres,
+    2-pole medium in 20% of the volume, TF/SF plane wave) for its full 2000
+    steps in the benchmark's decomposition: the pulse crosses the grid (about
+    1970 steps at 0.52 cells per step) and reaches the far CPML; every probe
+    sample stays finite and bounded (the incident amplitude is 1 V/m), and
+    the probes saw the pulse."""
+    ns = 2000
+    pb = P.heterogeneous_problem(n=1024, nsteps=ns)
+    s = sf.Solver(pb, mode="fast", probe_buffer=500)
+    s.set_tiling(1, 16, 1)
+    s.set_step_mode(16, 16)
+    for done in range(0, ns, 500):
+        s.run(500)
+        got = s.probes()
+        print(f"heterogeneous1024 step {done + 500}: max |probe| {np.abs(got).max():.3e}",
+              flush=True)
+        assert np.isfinite(got).all()
+    got = s.probes()
+    s.close()
+    assert got.shape == (ns, len(pb.probes))
+    peak = np.abs(got).max()
+    assert 1e-3 < peak < 10.0, peak

@@ -232,10 +232,10 @@ def test_step_class_rows_equal_oracle(name):
                   flush=True)
             s = sf.Solver(pb, mode=mode)
             s.set_step_mode(16, 5)
-            assert s.get_step_class_rows() == [16] * 7 + [4]
+            assert s.get_step_class_rows() == [16] * 7 + [4, 16]
             s.set_step_class_rows(rows, classes)
-            assert s.get_step_class_rows() == [rows if classes >> c & 1 else (4 if c == 7 else 16)
-                                               for c in range(8)]
+            assert s.get_step_class_rows()[:8] == [rows if classes >> c & 1 else (4 if c == 7 else 16)
+                                                   for c in range(8)]
             s.run(ns)
             base = ref.fields if mode == "exact" else [a.field(c) for c in range(6)]
             assert np.array_equal(_bits(s.probes()), _bits(want if mode == "exact" else a.probes()))
@@ -245,3 +245,77 @@ def test_step_class_rows_equal_oracle(name):
             assert sum(c[1] for c in cl) == pb.nx * pb.ny * pb.nz
             s.close()
     a.close()
+
+
+# grids wide enough in x for the four-cell kernel (k_step4): interior runs of
+# more than 126 columns, ending in a ragged tile; with CPML (the tiles stop at
+# the slabs), without CPML (they reach the grid's last columns), with Mur
+# faces, and with more media than 8 bits hold (int16 media reads)
+def _wide_many_media():
+    nx, ny, nz, nm = 270, 12, 10, 300
+    base = P.scatterer_problem(nsteps=60, n=(nx, ny, nz), L=((4, 5), (2, 2), (2, 3)))
+    rng = np.random.default_rng(11)
+    mi = base.materialize() if base.mi is None else base.mi.copy()
+    inner = (slice(3, nz - 2), slice(3, ny - 2), slice(6, nx - 6))
+    for c in range(6):
+        mi[c][inner] = rng.integers(2, nm + 1, size=mi[c][inner].shape).astype(mi.dtype)
+    pb = P.Problem(nx, ny, nz, epr=[1.0, 1.0] + list(rng.uniform(1.0, 6.0, nm - 1)),
+                   sigma=[0.0, 0.0] + list(rng.uniform(0.0, 0.2, nm - 1)),
+                   mur=[1.0] * (nm + 1), sigmam=[0.0] * (nm + 1),
+                   kind=[P.KIND_PEC] + [0] * nm, mi=mi, D=base.D, dt=base.dt,
+                   cpml_layers=base.cpml_layers, cpml_order=base.cpml_order,
+                   cpml_refl=base.cpml_refl, nodal=base.nodal, probes=base.probes,
+                   nsteps_max=60, name="wide_many_media")
+    assert pb.nmedia + 1 > 256
+    return pb
+
+
+VEC_CASES = {
+    "vec_scatterer": lambda: P.scatterer_problem(nsteps=60, n=(300, 20, 14),
+                                                 L=((5, 6), (2, 3), (3, 2))),
+    "vec_mur_walls": lambda: P.scatterer_problem(nsteps=50, n=(270, 10, 9),
+                                                 L=((0, 0), (0, 0), (0, 0)), mur=(1,) * 6),
+    "vec_cavity": lambda: P.cavity_problem(n=140, nsteps=160),
+    "vec_many_media": _wide_many_media,
+}
+VEC_SHAPES = [(16, 16), (8, 7), (4, 32)]
+
+
+@pytest.mark.parametrize("name", list(VEC_CASES))
+def test_step4_exact_equals_oracle(name):
+    """The plain tiles run four x cells per lane (class 8) and EXACT still
+    equals the oracle bit for bit, for every block shape; with the four-cell
+    kernel off the same step equals it too."""
+    pb = VEC_CASES[name]()
+    ns = pb.nsteps_max
+    ref = oracle.Oracle(pb)
+    want = ref.run(ns)
+    for (rows, kc), vec in [(s, True) for s in VEC_SHAPES] + [(VEC_SHAPES[0], False)]:
+        print(f"step4 case {name} rows {rows} zchunk {kc} vec {vec}", flush=True)
+        s = sf.Solver(pb, mode="exact")
+        s.set_step_mode(rows, kc)
+        s.set_step_vector(vec)
+        classes = [c for c, _, _ in s.step_classes()]
+        assert (8 in classes) == vec, classes
+        s.run(ns)
+        assert np.array_equal(_bits(s.probes()), _bits(want)), (rows, kc, vec)
+        for c in range(6):
+            assert np.array_equal(_bits(s.field(c)), _bits(ref.fields[c])), (rows, kc, vec, c)
+        assert sum(c[1] for c in s.step_classes()) == pb.nx * pb.ny * pb.nz
+        s.close()
+
+
+def test_step4_fast_equals_two_pass():
+    """FAST: the four-cell kernel gives the two-pass FAST run's bits."""
+    pb = VEC_CASES["vec_scatterer"]()
+    ns = pb.nsteps_max
+    a = sf.Solver(pb, mode="fast")
+    a.run(ns)
+    for rows in (16, 8):
+        b = sf.Solver(pb, mode="fast")
+        b.set_step_mode(rows, 16)
+        b.run(ns)
+        assert np.array_equal(_bits(b.probes()), _bits(a.probes())), rows
+        for c in range(6):
+            assert np.array_equal(_bits(b.field(c)), _bits(a.field(c))), (rows, c)
+        b.close()
