@@ -3,12 +3,15 @@ pole state in fp32) against the FP64 oracle, at the 1e-3 probe relative L2
 goal.
 
 Every stored field value is rounded to 11 significant bits once per half
-step, and the rounding errors travel with the waves; their sum grows roughly
-as the square root of the step count (rounding to nearest; stochastic
-rounding adds variance and was measured worse).  The cases below hold the
-goal where binary16 storage meets it.  The longer runs that do not are marked
-xfail (strict: a pass would be reported) with the error measured on the
-MI355X, and the bound is never raised.
+step (unit roundoff 2^-11 = 4.9e-4), and the rounding errors travel with the
+waves; their sum grows roughly as the square root of the step count
+(rounding to nearest; stochastic rounding adds variance and was measured
+worse).  Measured on the MI355X, no case meets the goal: 1.3e-3 after 100
+steps of a plane wave, 2.3e-3 after the batched configuration's 1500 steps,
+3.4e-3 for a serving request with a pole medium, while fp32 storage of the
+same runs stays at 3e-7 to 9e-7.  Each case is marked xfail (strict: a pass
+would be reported) with its measured error; the bound is the goal and is
+never raised.
 """
 import numpy as np
 import pytest
@@ -55,7 +58,15 @@ CASES = {
 }
 
 # (case, steps) -> measured rel-L2 where binary16 storage misses the goal
-OVER = {}
+# (MI355X, probe rel-L2 against the FP64 oracle)
+OVER = {
+    ("planewave_z", 100): 1.280e-3,
+    ("planewave_z", 200): 1.972e-3,
+    ("planewave_oblique", 150): 1.416e-3,
+    ("instance129", 300): 1.336e-3,
+    ("instance129", 1500): 2.276e-3,
+    ("request7", 527): 3.359e-3,
+}
 
 RUNS = [("planewave_z", 100), ("planewave_z", 200), ("planewave_oblique", 150),
         ("instance129", 300), ("instance129", 1500), ("request7", 527)]

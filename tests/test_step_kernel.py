@@ -248,12 +248,14 @@ def test_step_class_rows_equal_oracle(name):
 
 
 # grids wide enough in x for the four-cell kernel (k_step4): interior runs of
-# more than 126 columns, ending in a ragged tile; with CPML (the tiles stop at
+# more than 126 columns, ending in a ragged tile, and CPML slabs thick enough
+# in y and z (8-9 layers) that the planner keeps them as pieces of their own,
+# so interior rows and planes hold plain tiles; with CPML (the tiles stop at
 # the slabs), without CPML (they reach the grid's last columns), with Mur
 # faces, and with more media than 8 bits hold (int16 media reads)
 def _wide_many_media():
-    nx, ny, nz, nm = 270, 12, 10, 300
-    base = P.scatterer_problem(nsteps=60, n=(nx, ny, nz), L=((4, 5), (2, 2), (2, 3)))
+    nx, ny, nz, nm = 270, 40, 36, 300
+    base = P.scatterer_problem(nsteps=60, n=(nx, ny, nz), L=((4, 5), (8, 8), (8, 9)))
     rng = np.random.default_rng(11)
     mi = base.materialize() if base.mi is None else base.mi.copy()
     inner = (slice(3, nz - 2), slice(3, ny - 2), slice(6, nx - 6))
@@ -271,11 +273,12 @@ def _wide_many_media():
 
 
 VEC_CASES = {
-    "vec_scatterer": lambda: P.scatterer_problem(nsteps=60, n=(300, 20, 14),
-                                                 L=((5, 6), (2, 3), (3, 2))),
+    "vec_scatterer": lambda: P.scatterer_problem(nsteps=60, n=(300, 48, 44),
+                                                 L=((5, 6), (8, 9), (9, 8))),
     "vec_mur_walls": lambda: P.scatterer_problem(nsteps=50, n=(270, 10, 9),
                                                  L=((0, 0), (0, 0), (0, 0)), mur=(1,) * 6),
-    "vec_cavity": lambda: P.cavity_problem(n=140, nsteps=160),
+    "vec_no_cpml": lambda: P.scatterer_problem(nsteps=80, n=(300, 40, 36),
+                                               L=((0, 0), (0, 0), (0, 0)), disp=False),
     "vec_many_media": _wide_many_media,
 }
 VEC_SHAPES = [(16, 16), (8, 7), (4, 32)]
