@@ -130,38 +130,48 @@ int plan_step(const PlanGeom &g, StepPlan *out, std::string *err) {
   // whole number of their bands, so that no sliver band is left
   int Wy = R - 1;
   if (g.vec && g.vrows < R) Wy = std::max(g.vrows - 1, (R - 1) / (g.vrows - 1) * (g.vrows - 1));
+  // the columns of [a, b) four-cell tiles may own: 16-byte aligned bounds,
+  // the halo column of the last tile inside every x sweep (x0 + wx + 1 <=
+  // nx), and off the x CPML slabs, which a piece holds when a slab narrower
+  // than a sliver was folded into it (those columns stay in one-cell tiles)
+  auto vec_span = [&](int a, int b, int *a4, int *b4) {
+    if (g.L[0][0] > 0) a = std::max(a, g.L[0][0]);
+    if (g.L[0][1] > 0) b = std::min(b, g.n[0] - g.L[0][1] - 1);
+    *a4 = (a + 3) & ~3;
+    *b4 = std::min(b, g.n[0] - 1) & ~3;
+    return *b4 - *a4 >= 2 * Wx;
+  };
   // a piece wide enough for four-cell tiles moves its bounds onto 16-byte
-  // boundaries, its neighbours taking the columns in between (the halo column
-  // of its last tile stays inside every x sweep: x0 + wx + 1 <= nx)
+  // boundaries, its neighbours taking the columns in between
   if (g.vec)
     for (size_t q = 0; q < sx.size(); ++q) {
-      const int a4 = (sx[q].lo + 3) & ~3, b4 = std::min(sx[q].hi, g.n[0] - 1) & ~3;
-      if (b4 - a4 < 2 * Wx) continue;
-      if (q > 0 && a4 > sx[q].lo) { sx[q - 1].hi = a4; sx[q].lo = a4; }
-      if (q + 1 < sx.size() && b4 < sx[q].hi) { sx[q + 1].lo = b4; sx[q].hi = b4; }
+      int a4, b4;
+      if (!vec_span(sx[q].lo, sx[q].hi, &a4, &b4)) continue;
+      // (a bound more than 3 columns in is a CPML clip: the piece keeps
+      // those columns for its own one-cell tiles)
+      if (q > 0 && a4 > sx[q].lo && a4 - sx[q].lo <= 3) { sx[q - 1].hi = a4; sx[q].lo = a4; }
+      if (q + 1 < sx.size() && b4 < sx[q].hi && sx[q].hi - b4 <= 3) {
+        sx[q + 1].lo = b4;
+        sx[q].hi = b4;
+      }
     }
   // x tiles: a piece wide enough becomes four-cell tiles between 16-byte
   // aligned bounds; what the rounding leaves runs as 63-column tiles
   std::vector<XTile> xt;
   for (const Seg &s : sx) {
     int a = s.lo, b = s.hi;
-    if (g.vec) {
-      const int a4 = (a + 3) & ~3;
-      // every owned column of a four-cell tile and its halo lie inside the
-      // x sweep of every component: x0 + wx + 1 <= nx
-      const int b4 = std::min(b, g.n[0] - 1) & ~3;
-      if (b4 - a4 >= 2 * Wx) {
-        std::vector<Seg> p;
-        if (a4 > a) split(a, a4, Wx, 1, &p);
-        for (const Seg &q : p) xt.push_back({q.lo, q.hi, false});
-        p.clear();
-        split(a4, b4, 4 * Wx, 4, &p);
-        for (const Seg &q : p) xt.push_back({q.lo, q.hi, true});
-        p.clear();
-        if (b > b4) split(b4, b, Wx, 1, &p);
-        for (const Seg &q : p) xt.push_back({q.lo, q.hi, false});
-        continue;
-      }
+    int a4, b4;
+    if (g.vec && vec_span(a, b, &a4, &b4)) {
+      std::vector<Seg> p;
+      if (a4 > a) split(a, a4, Wx, 1, &p);
+      for (const Seg &q : p) xt.push_back({q.lo, q.hi, false});
+      p.clear();
+      split(a4, b4, 4 * Wx, 4, &p);
+      for (const Seg &q : p) xt.push_back({q.lo, q.hi, true});
+      p.clear();
+      if (b > b4) split(b4, b, Wx, 1, &p);
+      for (const Seg &q : p) xt.push_back({q.lo, q.hi, false});
+      continue;
     }
     std::vector<Seg> p;
     split(a, b, Wx, 1, &p);

@@ -151,3 +151,17 @@ def test_plan_ragged(R, kc, vrows):
 def test_plan_cavity_no_cpml():
     check((64, 64, 64), [[0, 0]] * 3, 16, 16, [[30, 30, 30, 30, 30, 30, 4]])
     check((5, 3, 2), [[0, 0]] * 3, 4, 16, [])
+
+
+def test_plan_thin_x_slab_keeps_four_cell_tiles():
+    """A 4-cell x CPML slab is narrower than a sliver and folds into the
+    interior piece next to it; the four-cell tiles of that piece start past
+    the slab (one-cell CPML tiles take its columns) instead of the whole
+    piece falling back to one-cell tiles."""
+    n, L = (270, 40, 36), [[4, 5], [8, 8], [8, 9]]
+    boxes = [[248, 22, 20, 248, 22, 20, 4], [249, 22, 19, 249, 22, 19, 4]]
+    for R, kc in ((16, 16), (8, 7), (4, 32)):
+        tiles = check(n, L, R, kc, boxes, vrows=R)
+        vec = tiles[tiles[:, 0] == 8]
+        assert len(vec) > 0, (R, kc)
+        assert vec[:, 1].min() >= 4
