@@ -287,6 +287,14 @@ int fdtd_update_observation(fdtd_handle *h, int32_t n);
 /* ---- field access ----------------------------------------------------- */
 int fdtd_get_field(fdtd_handle *h, int32_t comp, float *host_alloc);
 int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host_alloc);
+/* fdtd_get_field without any device allocation, for callers that cannot
+ * allocate (loops that must not call hipMalloc, harnesses that forbid it).
+ * No counterpart in the reference.  It stages runs of z planes through a
+ * fixed 16 MiB device array per GPU (held until the process ends) instead of
+ * a buffer it allocates; a component whose z plane exceeds 16 MiB as fp32 is
+ * refused with FDTD_ERR_ARG.  Same output as fdtd_get_field.  All calls on one
+ * GPU share that array: calls that overlap on one GPU race. */
+int fdtd_get_field_noalloc(fdtd_handle *h, int32_t comp, float *host_alloc);
 /* Scale of a binary16 field array (value = stored * scale); stored values keep
  * their physical value.  No effect on FP32 storage. */
 int fdtd_set_field_scale(fdtd_handle *h, int32_t comp, float scale);

@@ -94,6 +94,7 @@ SIGNATURES = {
     "fdtd_sample_probes": (C.c_int, [vp]),
     "fdtd_get_field": (C.c_int, [vp, C.c_int32, f32p]),
     "fdtd_set_field": (C.c_int, [vp, C.c_int32, f32p]),
+    "fdtd_get_field_noalloc": (C.c_int, [vp, C.c_int32, f32p]),
     "fdtd_synchronize": (C.c_int, [vp]),
     "fdtd_device_bytes": (C.c_int64, [vp]),
     "fdtd_device_memory": (C.c_int, [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -126,6 +126,39 @@ int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3],
   return (int)hipGetLastError();
 }
 
+// The same box read out of a field array as fp32: binary16 values times the
+// array's scale (one rounding, as the host conversion), fp32 values as they
+// are.
+template <class T>
+__global__ void __launch_bounds__(256) k_box_gather_f32(const T *grid, float *packed, float scale,
+                                                       long long ex, long long ey, long long ez,
+                                                       long long px, long long pxy) {
+  const long long n = ex * ey * ez;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long i = t % ex, j = (t / ex) % ey, k = t / (ex * ey);
+    const long long p = k * pxy + j * px + (kXOff - 1) + i;
+    if constexpr (sizeof(T) == 2)
+      packed[t] = __half2float(grid[p]) * scale;
+    else
+      packed[t] = grid[p];
+  }
+}
+
+int launch_box_gather_f32(const void *grid, size_t es, float scale, float *packed,
+                          const long long ext[3], long long px, long long pxy, void *stream) {
+  const long long n = ext[0] * ext[1] * ext[2];
+  if (n <= 0) return 0;
+  const unsigned b = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  if (es == 2)
+    hipLaunchKernelGGL(k_box_gather_f32<__half>, dim3(b), dim3(256), 0, (hipStream_t)stream,
+                       (const __half *)grid, packed, scale, ext[0], ext[1], ext[2], px, pxy);
+  else
+    hipLaunchKernelGGL(k_box_gather_f32<float>, dim3(b), dim3(256), 0, (hipStream_t)stream,
+                       (const float *)grid, packed, scale, ext[0], ext[1], ext[2], px, pxy);
+  return (int)hipGetLastError();
+}
+
 // 8-bit copy of a media index array (every index below 256)
 __global__ void __launch_bounds__(256) k_media_narrow(const int16_t *src, uint8_t *dst,
                                                      long long n) {

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1934,6 +1935,73 @@ int fdtd_set_field(fdtd_handle *h, int32_t comp, const float *host) {
   if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field write");
   hipSetDevice(h->device);
   return field_io(h, comp, host, nullptr);
+}
+
+// ---- field read that allocates nothing -------------------------------------
+// fdtd_get_field stages through a buffer it allocates (and grows) on first
+// use.  fdtd_get_field_noalloc stages through `scratch`, a file-scope device
+// array of kScratchBytes that every device the library runs on holds until the
+// process ends: runs of z planes are gathered into it as fp32 and copied to the
+// host, so the call makes no hipMalloc/hipFree.  In host code `scratch` names a
+// host-side stand-in, never the device array: the device address comes from
+// hipGetSymbolAddress, once per device with that device current, and only that
+// address is passed to kernels and copies.  All calls on one device share its
+// array; overlapping calls on one device race.
+namespace {
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// device address of `scratch` on h's device (current), looked up on first use
+static int scratch_of(fdtd_handle *h, float **out) {
+  if (h->device < 0 || h->device >= kScratchDevices)
+    return fail(h, FDTD_ERR_DEVICE, "device %d beyond the %d devices the scratch table holds",
+                h->device, kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[h->device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(h, FDTD_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", h->device,
+                  hipGetErrorString(e));
+    g_scratch_ptr[h->device] = (float *)p;
+  }
+  *out = g_scratch_ptr[h->device];
+  return FDTD_OK;
+}
+
+int fdtd_get_field_noalloc(fdtd_handle *h, int32_t comp, float *host) {
+  if (!h || comp < 0 || comp > 5 || !host) return fail(h, FDTD_ERR_ARG, "bad field read");
+  HIPCHK(h, hipSetDevice(h->device));
+  float *stage = nullptr;
+  if (int rc = scratch_of(h, &stage)) return rc;
+  long long ext[3];
+  alloc_extent(h, comp, ext);
+  const size_t plane = (size_t)(ext[0] * ext[1]) * 4;
+  if (plane > kScratchBytes)
+    return fail(h, FDTD_ERR_ARG,
+                "a z plane of component %d holds %zu bytes as fp32; the no-allocation read "
+                "stages at most %zu bytes (use fdtd_get_field)",
+                comp, plane, kScratchBytes);
+  const long long per = (long long)(kScratchBytes / plane);
+  const size_t es = field_bytes(h);
+  const float scale = h->storage == FDTD_STORE_FP16 ? h->g.fs[comp] : 1.0f;
+  for (long long k0 = 0; k0 < ext[2]; k0 += per) {
+    const long long nk = std::min(per, ext[2] - k0);
+    const long long sub[3] = {ext[0], ext[1], nk};
+    const char *g = (const char *)h->fld[comp] + (size_t)k0 * (size_t)h->g.pxy * es;
+    HIPCHK(h, (hipError_t)launch_box_gather_f32(g, es, scale, stage, sub, h->g.px, h->g.pxy,
+                                                h->stream));
+    HIPCHK(h, hipMemcpyAsync(host + (size_t)k0 * (plane / 4), stage, plane * (size_t)nk,
+                             hipMemcpyDeviceToHost, h->stream));
+    // the next run overwrites the scratch
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return FDTD_OK;
 }
 
 int fdtd_set_field_scale(fdtd_handle *h, int32_t comp, float scale) {

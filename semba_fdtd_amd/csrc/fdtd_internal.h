@@ -365,6 +365,10 @@ int launch_media_narrow(const int16_t *src, uint8_t *dst, long long n, void *str
 // moved between a packed device buffer and a grid array
 int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3], long long px,
                     long long pxy, int to_grid, void *stream);
+// the same box read out of a field array (es 4: fp32, es 2: binary16 times
+// scale) into a packed fp32 buffer
+int launch_box_gather_f32(const void *grid, size_t es, float scale, float *packed,
+                          const long long ext[3], long long px, long long pxy, void *stream);
 // the incident field of every TF/SF face cell at step *dn for the one-launch
 // step: faces[0..nf0) act on E (time tiempo(n)), faces[nf0..nf) on H
 // (tiempo(n) + dt/2); face q's cells go to inc[faces[q].inc ...], the face's

@@ -32,7 +32,7 @@ module sembafdtd_gpu
              fdtd_advance_h, fdtd_advance_e_yee, &
              fdtd_advance_e_cpml, fdtd_advance_e_dispersive, fdtd_advance_nodal_e, &
              fdtd_advance_h_yee, fdtd_advance_h_cpml, fdtd_advance_nodal_h, fdtd_run, &
-             fdtd_set_probes, fdtd_read_probes, fdtd_sample_probes, fdtd_get_field, &
+             fdtd_set_probes, fdtd_read_probes, fdtd_sample_probes, fdtd_get_field, fdtd_get_field_noalloc, &
              fdtd_set_field, fdtd_synchronize, fdtd_courant_dt, fdtd_build_g_tables, &
              fdtd_build_spacing, fdtd_build_cpml_axis, fdtd_build_ade, fdtd_evolucion, &
              fdtd_abi_version, fdtd_device_bytes, fdtd_device_memory, fdtd_dispersive_edges, &
@@ -213,6 +213,13 @@ module sembafdtd_gpu
          type(c_ptr), value :: h
       end function
       integer(c_int) function fdtd_get_field(h, comp, f) bind(C, name="fdtd_get_field")
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: comp
+         real(c_float), intent(out) :: f(*)
+      end function
+      integer(c_int) function fdtd_get_field_noalloc(h, comp, f) &
+            bind(C, name="fdtd_get_field_noalloc")
          import :: c_int, c_ptr, c_int32_t, c_float
          type(c_ptr), value :: h
          integer(c_int32_t), value :: comp
