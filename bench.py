@@ -95,8 +95,13 @@ def onelaunch_step_bytes(pb, ndisp_edges, mode, stored_poles):
             + ndisp_edges * (8 + per_state * stored_poles))
 
 
-CLASS_NAMES = {0: "plain", 1: "cpml", 2: "poles", 3: "cpml+poles", 4: "sources",
-               5: "cpml+sources", 6: "poles+sources", 7: "cpml+poles+sources", 8: "plain4"}
+_BITS = {0: "plain", 1: "cpml", 2: "poles", 3: "cpml+poles", 4: "sources",
+         5: "cpml+sources", 6: "poles+sources", 7: "cpml+poles+sources"}
+# tile classes of the one-launch step: 0-7 the k_step variants, 8 the
+# four-cell interior kernel, 9-16 the half-width (32 lanes per row) variants
+CLASS_NAMES = dict(_BITS)
+CLASS_NAMES[8] = "plain4"
+CLASS_NAMES.update({9 + c: "half:" + n for c, n in _BITS.items()})
 
 KERNELS = {
     "e_face": "k_epass (Yee E + CPML E, fused face pass)",
@@ -399,7 +404,7 @@ def main():
         timed_classes = s.step_classes(reps)
         cbytes = s.step_class_bytes()
         for cls, cells, ms_c in timed_classes:
-            key = "step_" + CLASS_NAMES[cls].replace("+", "_")
+            key = "step_" + CLASS_NAMES[cls].replace("+", "_").replace(":", "_")
             ms_k[key] = ms_c
             nbytes[key] = int(cbytes[cls])
             classes.append({"class": CLASS_NAMES[cls], "cells": int(cells), "ms": round(ms_c, 4),

@@ -183,16 +183,21 @@ int fdtd_get_step_mode(const fdtd_handle *h, int32_t *rows, int32_t *zchunk);
 /* Tile classes of the one-launch step: 0-7 carry bit 0 for tiles that touch a
  * CPML slab, bit 1 for tiles that meet a pole-state box, bit 2 for tiles that
  * touch a TF/SF face or a nodal cell (one lane per x cell); class 8 holds the
- * plain tiles run four x cells per lane. */
-#define FDTD_STEP_CLASSES 9
+ * plain tiles run four x cells per lane; classes 9-16 are classes 0-7 on
+ * half-width tiles (32 lanes per row, two rows per wave) for x pieces of at
+ * most 31 columns, such as the x CPML slabs. */
+#define FDTD_STEP_CLASSES 17
 /* Rows per block (4, 6, 8 or 16; 0: the default) of the step kernels of the
  * tile classes in the bit mask classes (bit c: class c).  A class on fewer
  * rows than the step runs the step's tiles cut into bands of rows - 1 rows;
  * for classes 0-7 rows - 1 must divide the step's rows - 1 (16-row steps: 4
  * or 6), class 8 runs 4, 8 or 16 rows; other values leave the class on its
- * default.  Smaller blocks give a kernel that needs many registers per lane
- * more of them; by default the class with CPML, poles and sources runs 4-row
- * blocks under 16-row steps, every other class the step's rows.  fdtd_get_step_class_rows gives the rows each class runs with.
+ * default.  Classes 9-16 run blocks of that many waves (any of the four
+ * values up to the step's rows), each tile owning up to 2 rows - 1 rows.
+ * Smaller blocks give a kernel that needs many registers per lane more of
+ * them; by default the class with CPML, poles and sources runs 4-row blocks
+ * under 16-row steps (8 waves on half-width tiles), every other class the
+ * step's rows.  fdtd_get_step_class_rows gives the rows each class runs with.
  * The results do not change.  No counterpart in the reference; used by the
  * autotuner. */
 int fdtd_set_step_class_rows(fdtd_handle *h, int32_t classes, int32_t rows);

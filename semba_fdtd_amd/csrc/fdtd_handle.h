@@ -135,7 +135,7 @@ struct fdtd_handle {
   // the A->B and B->A descriptors and a captured pair of steps
   int step_rows = 0;              // 0: two passes per step; else rows per block (4, 8, 16)
   int step_kc = 32;
-  int step_creq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // rows per block asked for each
+  int step_creq[17] = {};  // rows per block asked for each
                                   // tile class (0: the default); a class on fewer
                                   // rows than step_rows runs the step's tiles cut
                                   // into row bands

@@ -416,6 +416,13 @@ static int class_rows(const fdtd_handle *h, int c) {
     const int q = h->step_creq[c] ? h->step_creq[c] : R;
     return (q == 4 || q == 8 || q == 16) && q <= R ? q : R;
   }
+  if (c >= kPlanHalf) {
+    // half-width tiles are banded on their own (2E - 1 rows for E waves), so
+    // any supported block fits; the CPML + poles + sources kernel needs more
+    // than the 128 registers a lane has in 16-wave blocks, 8 waves by default
+    const int q = h->step_creq[c] ? h->step_creq[c] : (c - kPlanHalf == 7 && R > 8 ? 8 : R);
+    return q <= R ? q : R;
+  }
   const int q = h->step_creq[c] ? h->step_creq[c] : (c == 7 && R == 16 ? 4 : R);
   return q > 1 && q <= R && (R - 1) % (q - 1) == 0 ? q : R;
 }
@@ -543,7 +550,10 @@ static int build_step(fdtd_handle *h) {
       for (int s2 = 0; s2 < 2; ++s2) pg.L[a][s2] = h->g.has_cpml ? h->g.L[a][s2] : 0;
     pg.R = R;
     pg.kc = tm.kc;
-    for (int c = 0; c < 8; ++c) pg.rows[c] = class_rows(h, c);
+    for (int c = 0; c < 8; ++c) {
+      pg.rows[c] = class_rows(h, c);
+      pg.hrows[c] = class_rows(h, kPlanHalf + c);
+    }
     pg.vrows = class_rows(h, kPlanVec);
     // the four-cell kernel's LDS (tables and its row exchange) must fit
     {

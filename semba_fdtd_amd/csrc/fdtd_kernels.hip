@@ -1265,8 +1265,14 @@ static __host__ __device__ __forceinline__ size_t step_tab_bytes(int nmedia, int
 // tiles that meet the pole-state box of some component; SRC, tiles that
 // touch a TF/SF face or a nodal cell.  A class's kernel carries only the code
 // its tiles need, so interior tiles run the plain Yee march.
-template <int R, bool CPML, bool ADE, bool SRC, class MT>
+// XW: lanes per grid row.  64: a wave holds one row of 64 columns; 32: a wave
+// holds two rows of 32 columns (half-width tiles, for pieces of at most 31
+// columns such as the x CPML slabs, which would leave most of a 64-lane row
+// idle); the block's R waves then hold 2R rows.
+template <int R, bool CPML, bool ADE, bool SRC, class MT, int XW>
 __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
+  static_assert(XW == 64 || XW == 32, "a row is a whole or a half wave");
+  constexpr int RR = R * (kWave / XW);   // grid rows per block
   extern __shared__ float2 lds[];
   const DevGrid &gi = sd.gi;
   const DevGrid &go = sd.go;
@@ -1318,12 +1324,14 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
   }
   __syncthreads();
 
-  const int lane = threadIdx.x, row = threadIdx.y;
+  // lane: the column in the tile's row; row: the tile's row (a wave holds
+  // kWave / XW consecutive rows)
+  const int lane = (int)threadIdx.x % XW, row = (int)threadIdx.y * (kWave / XW) + (int)threadIdx.x / XW;
   const int i = x0 + lane, j = y0 + row;
   const int nx = gi.nx, ny = gi.ny, nz = gi.nz;
   const bool in = i <= nx && j <= ny;
-  // the tile owns lanes [0, wx) (wx <= 63) and rows [0, wy) (wy <= R - 1);
-  // lane wx and row wy are its halo
+  // the tile owns lanes [0, wx) (wx <= XW - 1) and rows [0, wy) (wy <= RR -
+  // 1); lane wx and row wy are its halo
   const bool own = in && lane < wx && row < wy;
   // lanes past the grid's edge address the edge cell (their results are
   // never used): every load and table index stays inside its array
@@ -1604,14 +1612,16 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
         if (wr && ez_on) st_at(plane_base(fo_at(a, go, 2), P), ob, e);
       }
     }
-    // ---- E of plane k at x + 1 (next lane) and y + 1 (next row)
+    // ---- E of plane k at x + 1 (next lane) and y + 1 (next row).  (With
+    // half-width rows the last lane of a row receives the next row's first
+    // value: that lane is at most the halo column, whose H is not stored.)
     const float ey_ip_n = __shfl_down(ey_n, 1), ez_ip_n = __shfl_down(ez_n, 1);
     const int par = (k - k0) & 1;
-    xch[(par * R + row) * kWave + lane] = make_float2(ex_n, ez_n);
+    xch[(par * RR + row) * XW + lane] = make_float2(ex_n, ez_n);
     __syncthreads();
     float ex_jp_n = 0.0f, ez_jp_n = 0.0f;
-    if (row + 1 < R) {
-      const float2 v = xch[(par * R + row + 1) * kWave + lane];
+    if (row + 1 < RR) {
+      const float2 v = xch[(par * RR + row + 1) * XW + lane];
       ex_jp_n = v.x;
       ez_jp_n = v.y;
     }
@@ -1974,18 +1984,27 @@ size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc) {
   return npw ? step_lds_faces(x, npw, kc, nullptr, nullptr) : x;
 }
 
+// class cls: 0-7 the k_step variant (bits 1 CPML, 2 poles, 4 sources), 8 the
+// four-cell kernel, 9-16 the half-width variant of class cls - 9
 int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
   if (cls == 8) return launch_step4(sd, rows, stream);
   const size_t lds = step_lds_bytes(sd.gi, rows, sd.npw[0] + sd.npw[1], sd.tm.kc);
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)sd.tm.nblk);
   if (sd.tm.nblk <= 0) return 0;
-#define SFD_STEP(RR, C, A, S)                                                        \
-  do {                                                                               \
-    if (sd.narrow)                                                                   \
-      hipLaunchKernelGGL((k_step<RR, C, A, S, uint8_t>), grid, dim3(kWave, RR), lds, s, sd); \
-    else                                                                             \
-      hipLaunchKernelGGL((k_step<RR, C, A, S, int16_t>), grid, dim3(kWave, RR), lds, s, sd); \
+  const bool half = cls >= 9;
+  if (half) cls -= 9;
+#define SFD_STEP_W(RR, C, A, S, W)                                                           \
+  do {                                                                                       \
+    if (sd.narrow)                                                                           \
+      hipLaunchKernelGGL((k_step<RR, C, A, S, uint8_t, W>), grid, dim3(kWave, RR), lds, s, sd); \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_step<RR, C, A, S, int16_t, W>), grid, dim3(kWave, RR), lds, s, sd); \
+  } while (0)
+#define SFD_STEP(RR, C, A, S)                  \
+  do {                                         \
+    if (half) SFD_STEP_W(RR, C, A, S, 32);     \
+    else SFD_STEP_W(RR, C, A, S, 64);          \
   } while (0)
 #define SFD_STEP_R(RR)                                         \
   switch (cls & 7) {                                           \
@@ -2011,6 +2030,7 @@ int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
   }
 #undef SFD_STEP_R
 #undef SFD_STEP
+#undef SFD_STEP_W
   return (int)hipGetLastError();
 }
 

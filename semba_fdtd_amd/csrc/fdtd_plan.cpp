@@ -18,7 +18,10 @@
 // the pieces are split into tiles no larger than a block owns: 63 columns
 // (or 252 in four-cell tiles, which start on a 16-byte boundary and own a
 // multiple of four columns), R - 1 rows, kc planes.  Plain tiles stacked in z
-// merge into one tile of up to 64 planes.
+// merge into one tile of up to 64 planes.  A piece of at most 31 columns (the
+// x CPML slabs with the faces next to them) runs half-width tiles: 32 lanes
+// per row, two rows per wave, up to 2E - 1 rows per tile for a class on E
+// waves per block.
 #include <algorithm>
 #include <cstdint>
 #include <string>
@@ -82,7 +85,8 @@ void split(int lo, int hi, int W, int unit, std::vector<Seg> *out) {
 
 struct XTile {
   int lo, hi;
-  bool vec;  // a four-cell tile (plain) or its split into 63-column tiles
+  bool vec;   // a four-cell tile (plain) or its split into 63-column tiles
+  bool half;  // at most 31 columns: half-width tiles
 };
 
 }  // namespace
@@ -107,6 +111,11 @@ int plan_step(const PlanGeom &g, StepPlan *out, std::string *err) {
     if (err) *err = "step plan: bad geometry";
     return -1;
   }
+  for (int c = 0; c < 8; ++c)
+    if (g.rows[c] < 2 || g.rows[c] > R || g.hrows[c] < 2 || g.hrows[c] > R) {
+      if (err) *err = "step plan: bad rows per block";
+      return -1;
+    }
   // cut points per axis: CPML slab edges (a tile is outside the low slab if
   // it starts at L0, outside the high one if its halo stays below n - L1),
   // and the planes just outside each box
@@ -120,7 +129,7 @@ int plan_step(const PlanGeom &g, StepPlan *out, std::string *err) {
     }
   }
   const int N[3] = {g.n[0] + 1, g.n[1] + 1, g.n[2] + 1};
-  const int Wx = kWave - 1;
+  const int Wx = kWave - 1, Wh = kWave / 2 - 1;
   // slivers narrower than these fold into a neighbour; wider pieces (a CPML
   // slab with the faces next to it) keep tiles of their own
   std::vector<Seg> sx = segments(cuts[0], N[0], 8);
@@ -164,18 +173,18 @@ int plan_step(const PlanGeom &g, StepPlan *out, std::string *err) {
     if (g.vec && vec_span(a, b, &a4, &b4)) {
       std::vector<Seg> p;
       if (a4 > a) split(a, a4, Wx, 1, &p);
-      for (const Seg &q : p) xt.push_back({q.lo, q.hi, false});
+      for (const Seg &q : p) xt.push_back({q.lo, q.hi, false, q.hi - q.lo <= Wh});
       p.clear();
       split(a4, b4, 4 * Wx, 4, &p);
-      for (const Seg &q : p) xt.push_back({q.lo, q.hi, true});
+      for (const Seg &q : p) xt.push_back({q.lo, q.hi, true, false});
       p.clear();
       if (b > b4) split(b4, b, Wx, 1, &p);
-      for (const Seg &q : p) xt.push_back({q.lo, q.hi, false});
+      for (const Seg &q : p) xt.push_back({q.lo, q.hi, false, q.hi - q.lo <= Wh});
       continue;
     }
     std::vector<Seg> p;
     split(a, b, Wx, 1, &p);
-    for (const Seg &q : p) xt.push_back({q.lo, q.hi, false});
+    for (const Seg &q : p) xt.push_back({q.lo, q.hi, false, q.hi - q.lo <= Wh});
   }
   std::vector<Seg> yt, zt;
   for (const Seg &s : sy) split(s.lo, s.hi, Wy, 1, &yt);
@@ -250,11 +259,31 @@ int plan_step(const PlanGeom &g, StepPlan *out, std::string *err) {
     if (p.open) emit(p.cls, p.x0, p.wx, p.y0, p.wy, p.k0, p.k1);
     p.open = false;
   };
+  // half-width tiles: per z tile and y piece, the class of the x piece's
+  // columns over the whole y piece (halo included), then bands of at most
+  // 2E - 1 rows for the class's E waves per block
+  for (const Seg &z : zt)
+    for (const Seg &y : sy)
+      for (const XTile &x : xt) {
+        if (!x.half) continue;
+        const int lo[3] = {x.lo, y.lo, z.lo}, hi[3] = {x.hi, y.hi, z.hi};
+        const int c = plan_classify(g, lo, hi);
+        const int cls = kPlanHalf + c;
+        std::vector<Seg> bands;
+        split(y.lo, y.hi, 2 * g.hrows[c] - 1, 1, &bands);
+        for (const Seg &b : bands) {
+          const int wy = b.hi - b.lo;
+          out->owned[cls] += owned_cells(x.lo, x.hi - x.lo, b.lo, wy, z.lo, z.hi);
+          state_counts(cls, x.lo, x.hi - x.lo, b.lo, wy, z.lo, z.hi);
+          out->tiles[cls].push_back(step_tile(x.lo, x.hi - x.lo, b.lo, wy, z.lo, z.hi));
+        }
+      }
   for (const Seg &z : zt) {
     for (size_t by = 0; by < yt.size(); ++by) {
       const Seg &y = yt[by];
       for (size_t bx = 0; bx < xt.size(); ++bx) {
         const XTile &x = xt[bx];
+        if (x.half) continue;
         // the pieces of this x tile: itself, or its 63-column split when a
         // four-cell tile is not plain
         Seg pieces[4];
@@ -316,7 +345,8 @@ int plan_step(const PlanGeom &g, StepPlan *out, std::string *err) {
 }  // namespace sfd_host
 
 // The CPU tests' view of the plan: geom = {nx, ny, nz, Lx0, Lx1, Ly0, Ly1,
-// Lz0, Lz1, R, kc, vec, vrows}; rows[8] rows per block of each class; boxes
+// Lz0, Lz1, R, kc, vec, vrows}; rows[16] rows per block of classes 0-7, then
+// waves per block of their half-width variants (classes 9-16); boxes
 // [nbox][7] = lo xyz, hi xyz (inclusive), kind (2 poles, 4 sources).  Writes
 // up to cap tiles as [cls, x0, wx, y0, wy, k0, k1] and the count to *ntiles
 // (all of them, also when more than cap).
@@ -335,7 +365,10 @@ extern "C" int sfd_plan_step(const int32_t *geom, const int32_t *rows, int32_t n
   g.kc = geom[10];
   g.vec = geom[11] != 0;
   g.vrows = geom[12];
-  for (int c = 0; c < 8; ++c) g.rows[c] = rows[c];
+  for (int c = 0; c < 8; ++c) {
+    g.rows[c] = rows[c];
+    g.hrows[c] = rows[8 + c];
+  }
   for (int b = 0; b < nbox; ++b) {
     PlanBox pb;
     for (int a = 0; a < 3; ++a) {

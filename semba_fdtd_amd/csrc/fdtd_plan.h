@@ -8,9 +8,12 @@
 namespace sfd_host {
 
 // tile classes: bits 0 CPML, 1 poles, 2 sources (0..7, one k_step variant
-// each), and 8: plain tiles run four x cells per lane (k_step4)
-constexpr int kPlanClasses = 9;
+// each); 8: plain tiles run four x cells per lane (k_step4); 9..16: the
+// half-width variant of class c - 9 (k_step with 32 lanes per row, for x
+// pieces of at most 31 columns)
+constexpr int kPlanClasses = 17;
 constexpr int kPlanVec = 8;
+constexpr int kPlanHalf = 9;
 
 // a box of cells [lo, hi] (inclusive) that tiles meeting it must handle:
 // kind 2 a pole-state box (of field component comp, 0-5), 4 a TF/SF face or
@@ -27,6 +30,8 @@ struct PlanGeom {
   int R;             // rows per block of the step (tiles own up to R - 1 rows)
   int kc;            // planes per z chunk (tiles with sources own at most kc)
   int rows[8];       // rows per block of each class (<= R)
+  int hrows[8];      // waves per block of each half-width class (<= R; a tile
+                     // owns up to 2 hrows - 1 rows)
   int vrows;         // rows per block of the four-cell kernel (<= R)
   bool vec;          // plain tiles may run four x cells per lane
   std::vector<PlanBox> boxes;

@@ -414,15 +414,16 @@ class Solver:
         time-stepping state."""
         self._ck(self.lib.fdtd_set_step_mode(self.h, rows, zchunk), "fdtd_set_step_mode")
 
-    STEP_CLASSES = 9
+    STEP_CLASSES = 17
 
-    def set_step_class_rows(self, rows: int, classes: int = 0x1FF):
+    def set_step_class_rows(self, rows: int, classes: int = 0x1FFFF):
         """Rows per block (4, 6, 8 or 16; 0: the default) of the step kernels
         of the tile classes in the bit mask ``classes`` (bit c: class c; for
         c < 8 bits 1 CPML, 2 poles, 4 sources; class 8 the plain tiles run
-        four x cells per lane, 4 or 8 rows).  A class on fewer rows than the
-        step runs the step's tiles cut into row bands.  The results do not
-        change."""
+        four x cells per lane, 4 or 8 rows; classes 9-16 the half-width
+        variants of classes 0-7, in waves per block).  A class on fewer rows
+        than the step runs the step's tiles cut into row bands.  The results
+        do not change."""
         self._ck(self.lib.fdtd_set_step_class_rows(self.h, classes, rows),
                  "fdtd_set_step_class_rows")
 
@@ -438,9 +439,9 @@ class Solver:
         return [int(v) for v in r]
 
     def step_classes(self, reps: int = 0):
-        """The tile classes of the one-launch step: [(class bits, owned Yee
-        cells, ms per launch or None)], bits 1 CPML, 2 poles, 4 sources; 8 the
-        plain tiles four x cells per lane."""
+        """The tile classes of the one-launch step: [(class, owned Yee cells,
+        ms per launch or None)]: bits 1 CPML, 2 poles, 4 sources; 8 the plain
+        tiles four x cells per lane; 9 + bits the half-width tiles."""
         cls = (C.c_int32 * self.STEP_CLASSES)()
         cells = (C.c_double * self.STEP_CLASSES)()
         nbytes = (C.c_double * self.STEP_CLASSES)()

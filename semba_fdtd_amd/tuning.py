@@ -35,7 +35,7 @@ DEFAULT_CANDIDATES: Tuple[Tiling, ...] = tuple(
     (r, k, x) for x in (1, 4) for r in (1, 2, 4) for k in (16, 32, 64))
 STEP_CANDIDATES: Tuple[StepShape, ...] = ((4, 16), (8, 16), (8, 32), (16, 16), (16, 32))
 
-NCLASS = 9                             # tile classes of the one-launch step
+NCLASS = 17                            # tile classes of the one-launch step
 
 _cache: Dict[str, tuple] = {}
 
@@ -70,10 +70,13 @@ def _apply(solver, choice: tuple) -> None:
 
 
 def _class_rows(solver, reps: int, verbose: bool) -> list:
-    """Per tile class, the block rows (among the step's and the row bands
-    that divide its tiles) whose kernel runs the class fastest."""
+    """Per tile class, the block rows whose kernel runs the class fastest:
+    classes 0-7 among the step's rows and the row bands that divide its
+    tiles, the four-cell and half-width classes among every supported block
+    up to the step's rows (a request a class cannot take leaves it on its
+    default, which is then timed again)."""
     srows = solver.get_step_mode()[0]
-    options = [r for r in (srows, 4, 6, 8) if r <= srows and (srows - 1) % (r - 1) == 0]
+    options = [r for r in (srows, 4, 6, 8) if r <= srows]
     default = solver.get_step_class_rows()
     best = {}
     for r in dict.fromkeys(options):

@@ -218,8 +218,9 @@ def test_step_many_media(mode):
                                   "magnetic_mur"])
 def test_step_class_rows_equal_oracle(name):
     """Tile classes run in smaller blocks over row bands of the step's tiles
-    (4 and 6 rows under 16-row tiles, every class or only the interior one):
-    EXACT still equals the oracle bit for bit, FAST the two-pass FAST run."""
+    (4 and 6 rows under 16-row tiles, every class or only the interior one),
+    and the half-width classes (9-16) on 4 and 8 waves: EXACT still equals the
+    oracle bit for bit, FAST the two-pass FAST run."""
     pb = CASES[name]()
     ns = pb.nsteps_max
     ref = oracle.Oracle(pb)
@@ -227,15 +228,16 @@ def test_step_class_rows_equal_oracle(name):
     a = sf.Solver(pb, mode="fast")
     a.run(ns)
     for mode in ("exact", "fast"):
-        for rows, classes in ((4, 0xFF), (6, 0xFF), (6, 0x01)):
+        for rows, classes in ((4, 0xFF), (6, 0xFF), (6, 0x01), (4, 0x1FE00), (8, 0x1FE00)):
             print(f"step class rows {name} {mode} 16 x 5, rows {rows} classes {classes:#x}",
                   flush=True)
             s = sf.Solver(pb, mode=mode)
             s.set_step_mode(16, 5)
-            assert s.get_step_class_rows() == [16] * 7 + [4, 16]
+            default = [16] * 7 + [4, 16] + [16] * 7 + [8]
+            assert s.get_step_class_rows() == default
             s.set_step_class_rows(rows, classes)
-            assert s.get_step_class_rows()[:8] == [rows if classes >> c & 1 else (4 if c == 7 else 16)
-                                                   for c in range(8)]
+            assert s.get_step_class_rows() == [rows if classes >> c & 1 else default[c]
+                                               for c in range(17)]
             s.run(ns)
             base = ref.fields if mode == "exact" else [a.field(c) for c in range(6)]
             assert np.array_equal(_bits(s.probes()), _bits(want if mode == "exact" else a.probes()))

@@ -6,8 +6,10 @@ library's planner (sfd_plan_step, the function build_step uses) is run on the
 * fit their blocks: at most 63 owned columns (one cell per lane) or 252 in
   four-cell tiles, which start on a 16-byte boundary, own a multiple of four
   columns and keep their halo column inside every component's x sweep; at
-  most rows - 1 owned rows; tiles with sources at most kc planes (their face
-  masks in LDS span kc + 1 planes), every tile at most 64;
+  most rows - 1 owned rows; half-width tiles (32 lanes per row, two rows per
+  wave) at most 31 columns and 2 waves - 1 rows; tiles with sources at most
+  kc planes (their face masks in LDS span kc + 1 planes), every tile at most
+  64;
 * have every E neighbour of an owned H cell computed in the block (lane + 1,
   row + 1 and plane k1 stay inside the block and the march);
 * address only memory inside the padded arrays (the four-cell kernel's
@@ -26,7 +28,11 @@ from semba_fdtd_amd import _lib
 KWAVE = 64
 
 
-def plan(n, L, R, kc, boxes, rows=None, vec=1, vrows=8):
+def plan(n, L, R, kc, boxes, rows=None, vec=1, vrows=8, hrows=None):
+    """Tiles [cls, x0, wx, y0, wy, k0, k1] of the planner; rows: rows per
+    block of classes 0-7 (default R), hrows: waves per block of their
+    half-width variants (default R, the CPML + poles + sources one 8 when R
+    is 16, as the library does)."""
     lib = C.CDLL(_lib.LIB_PATH)
     f = lib.sfd_plan_step
     f.restype = C.c_int
@@ -34,7 +40,10 @@ def plan(n, L, R, kc, boxes, rows=None, vec=1, vrows=8):
     f.argtypes = [i32p, i32p, C.c_int32, i32p, i32p, C.c_int64, C.POINTER(C.c_int64)]
     geom = np.array(list(n) + [L[0][0], L[0][1], L[1][0], L[1][1], L[2][0], L[2][1], R, kc, vec,
                                min(vrows, R)], np.int32)
-    rows = np.array(rows or [R] * 8, np.int32)
+    rows = list(rows or [R] * 8)
+    if hrows is None:
+        hrows = [R] * 7 + [8 if R > 8 else R]
+    rows = np.array(rows + list(hrows), np.int32)
     bx = np.array(boxes, np.int32).reshape(-1, 7) if boxes else np.zeros((0, 7), np.int32)
     cnt = C.c_int64()
     p = lambda a: a.ctypes.data_as(i32p)
@@ -70,7 +79,10 @@ def check(n, L, R, kc, boxes, rows=None, vec=1, vrows=8):
         assert k1 - k0 <= 64
         if cls & 4 and cls < 8:
             assert k1 - k0 <= kc
-        if cls == 8:
+        if cls >= 9:
+            assert wx <= KWAVE // 2 - 1              # lane wx of the 32 is the halo lane
+            assert wy <= 2 * rows[cls - 1] - 1       # two rows per wave; row wy is the halo
+        elif cls == 8:
             e = min(vrows, R)
             assert x0 % 4 == 0 and wx % 4 == 0 and wx <= 4 * (KWAVE - 1)
             assert x0 + wx + 1 <= nx               # halo column inside every x sweep
@@ -88,7 +100,8 @@ def check(n, L, R, kc, boxes, rows=None, vec=1, vrows=8):
         if cls == 8:
             assert need == 0, (cls, lo, hi, need)
         else:
-            assert need & ~cls == 0, (cls, lo, hi, need)
+            bits = cls - 9 if cls >= 9 else cls
+            assert need & ~bits == 0, (cls, lo, hi, need)
         own[k0:k1, y0:y0 + wy, x0:x0 + wx] += 1
         yee += max(0, min(x0 + wx, nx) - x0) * max(0, min(y0 + wy, ny) - y0) * max(0, min(k1, nz) - k0)
     assert own.min() == 1 and own.max() == 1
