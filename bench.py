@@ -205,7 +205,7 @@ def batch_leg(count, steps, plain_sample, mode, storage="fp32"):
     print(f"[bench] batch {storage}: {count} instances, {cells / 1e6:.1f} M cells, {steps} steps",
           file=sys.stderr, flush=True)
     sol = [sf.Solver(p, mode=mode, probe_buffer=steps, storage=storage) for p in insts]
-    b = Batch(xcells=4 if storage == "fp16" else 1)
+    b = Batch()
     for s in sol:
         b.admit(s)
     b.step(2)
@@ -260,8 +260,7 @@ def serve_leg(count, rate, mode, storage="fp32"):
     plain = serve.serve_plain(reqs, mode=mode, storage=storage)
     # at most SERVE_MAX_CELLS cells admitted at once: enough to fill the
     # device, few enough that early requests finish early
-    batched = serve.serve_batched(reqs, mode=mode, storage=storage, max_cells=SERVE_MAX_CELLS,
-                                  xcells=4 if storage == "fp16" else 1)
+    batched = serve.serve_batched(reqs, mode=mode, storage=storage, max_cells=SERVE_MAX_CELLS)
     same = True
     pm = {r.name: r for r in plain}
     for r in batched:
@@ -272,6 +271,81 @@ def serve_leg(count, rate, mode, storage="fp32"):
             "batched": bs, "plain": ps,
             "throughput_ratio": round(bs["mcell_updates_per_s"] / ps["mcell_updates_per_s"], 2),
             "results_identical": bool(same)}
+
+
+def config_rows(cpu_seconds=20.0):
+    """Per-configuration throughput beside the 1024^3 line (``--configs``):
+    the plain-method cavity (64^3, PEC walls, Hertzian dipole, 2000 steps) and
+    the open scatterer (256^3, 10-cell CPML, graded z, TF/SF wave, sphere,
+    4000 steps), each run for its full step count on the GPU (FAST, set up and
+    warmed before the timer; the autotuner picks the decomposition), and the
+    oracle (OpenMP over the host's physical cores, bound close, FMA on and
+    off) timed on a bounded number of steps of the same grid on the host.
+    Algorithmic bytes: 84 B per cell plus 8 B per psi value per step."""
+    import subprocess
+    import torch
+    import semba_fdtd_amd as sf
+    from semba_fdtd_amd import problem as P
+    from semba_fdtd_amd import tuning
+    cores = physical_cores()
+    threads = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    out = {}
+    for name, pb in (("config1_cavity64", P.cavity_problem(n=64, nsteps=2000)),
+                     ("config2_open_scatterer256", P.open_scatterer_problem(n=256, nsteps=4000))):
+        ns = pb.nsteps_max
+        print(f"[bench] {name}: {pb.nx}x{pb.ny}x{pb.nz}, {ns} steps", file=sys.stderr, flush=True)
+        s = sf.Solver(pb, mode="fast", probe_buffer=ns)
+        # the grid is at rest and the timed passes run no sources: it stays
+        # at rest, so the sweep needs no checkpoint
+        tuned = tuning.autotune(s, reps=3, keep_state=False)
+        s.run(2)
+        s.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(ns - 2)
+        s.synchronize()
+        dt = time.perf_counter() - t0
+        finite = bool(np.isfinite(s.probes()).all())
+        s.close()
+        rate = pb.cells * (ns - 2) / dt / 1e6
+        pe, ph = psi_counts(pb)
+        bpc = 84.0 + 8.0 * (pe + ph) / pb.cells
+        row = {"cells": pb.cells, "steps": ns, "gpu_mcell_updates_per_s": round(rate, 1),
+               "gpu_seconds": round(dt, 3), "algorithmic_bytes_per_cell": round(bpc, 2),
+               "hbm_roofline_frac_84B": round(rate * 1e6 * bpc / (HBM_PEAK_GBS * 1e9), 4),
+               "decomposition": {"passes": list(tuned["tiling"]), "step": list(tuned["step"])},
+               "probes_finite": finite}
+        # the oracle on a bounded sample of the same grid's steps
+        sample = max(4, int(cpu_seconds * 100e6 / pb.cells))
+        for v in ("fma", "nofma"):
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-config",
+                                name, str(min(sample, ns)), v], env=env, capture_output=True,
+                               text=True, timeout=900, check=True)
+            res = json.loads(r.stdout.strip().splitlines()[-1])
+            row[f"cpu_oracle_{v}_mcell_updates_per_s"] = round(
+                res["cells"] * res["steps"] / res["seconds"] / 1e6, 2)
+        row["cpu_threads"] = threads
+        row["cpu_sample_steps"] = min(sample, ns)
+        out[name] = row
+        print(f"[bench] {name}: {json.dumps(row)}", file=sys.stderr, flush=True)
+    return out
+
+
+def _cpu_config(name, steps, variant):
+    """Child process of config_rows: the oracle on the named configuration
+    for `steps` steps; prints one JSON line."""
+    import oracle
+    from semba_fdtd_amd import problem as P
+    pb = (P.cavity_problem(n=64, nsteps=steps + 2) if name.startswith("config1")
+          else P.open_scatterer_problem(n=256, nsteps=max(steps + 2, 200)))
+    o = oracle.Oracle(pb, fma=variant == "fma")
+    o.run(2)
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    print(json.dumps({"cells": pb.cells, "steps": steps, "seconds": dt}), flush=True)
 
 
 def main():
@@ -303,8 +377,14 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=8.0)
+    ap.add_argument("--configs", action="store_true",
+                    help="also time the plain-method cavity (64^3) and the open scatterer "
+                         "(256^3) at their full step counts, with the oracle on the host")
     if len(sys.argv) == 5 and sys.argv[1] == "--cpu-child":
         _cpu_child(int(sys.argv[2]), int(sys.argv[3]), sys.argv[4])
+        return
+    if len(sys.argv) == 5 and sys.argv[1] == "--cpu-config":
+        _cpu_config(sys.argv[2], int(sys.argv[3]), sys.argv[4])
         return
     args = ap.parse_args()
     if args.copy_calibration:
@@ -459,6 +539,8 @@ def main():
         if args.mode == "fast":
             served16 = serve_leg(args.serve_requests, args.serve_rate, args.mode, storage="fp16")
 
+    configs = config_rows() if args.configs else None
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -509,6 +591,7 @@ def main():
         "batch_fp16": batch16,
         "serve": served,
         "serve_fp16": served16,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
 

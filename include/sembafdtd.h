@@ -384,9 +384,11 @@ int fdtd_batch_admit(fdtd_batch *b, fdtd_handle *h, int32_t n0, int32_t *slot);
 int fdtd_batch_retire(fdtd_batch *b, int32_t slot);
 /* Step the grid of slot will run next (-1 for an empty slot). */
 int64_t fdtd_batch_step_of(const fdtd_batch *b, int32_t slot);
-/* Tiling of the batched face passes (rows 1, 2 or 4; zchunk planes; xcells 1
- * or 4 x cells per lane, always 4 with binary16 storage) and the persistent
- * launch's block count (0: 2048). */
+/* Tiling of the batched face passes (rows 1, 2 or 4; zchunk planes; xcells 1:
+ * one cell per lane, each block taking a run of 64 * rows consecutive columns
+ * of a grid's x-y plane, x fastest, so that narrow grids leave no lane idle;
+ * xcells 4: four x cells per lane over 256-cell x segments) and the
+ * persistent launch's block count (0: 8192).  Either storage runs either. */
 int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t xcells,
                           int32_t blocks);
 /* nsteps steps of every admitted grid. */

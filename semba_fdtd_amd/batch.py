@@ -35,9 +35,10 @@ class Batch:
             raise FdtdError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
     def set_tiling(self, rows: int, zchunk: int, blocks: int = 0, xcells: int = 1):
-        """Blocks of 64 x rows threads marching zchunk z planes, xcells (1 or
-        4) x cells per lane, and the persistent launch's block count (0:
-        2048)."""
+        """Blocks of 64 x rows threads marching zchunk z planes, xcells x
+        cells per lane (1: each block takes a run of 64 * rows consecutive
+        columns of a grid's x-y plane; 4: 256-cell x segments), and the
+        persistent launch's block count (0: 8192)."""
         self._ck(self.lib.fdtd_batch_set_tiling(self.b, rows, zchunk, xcells, blocks),
                  "fdtd_batch_set_tiling")
 

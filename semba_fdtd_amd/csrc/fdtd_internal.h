@@ -85,6 +85,8 @@ struct TileMap {
   int remap;
   int bdy;          // rows per block (block = 64 x bdy threads, <= 256)
   int vec;          // x cells per lane of the fused passes (1 or 4)
+  int flat;         // 1: tiles are runs of 64 * bdy columns of the x-y plane
+                    // (one cell per lane), tx per plane, ty = 1
 };
 
 // First-order Mur (bordersmur.F90): one tangential H component on one

@@ -362,14 +362,86 @@ __global__ void __launch_bounds__(256) k_ade_many(const DevGrid *grids, const in
   ade_box_run<TBL, SIDE>(grids[list[blockIdx.z]]);
 }
 
-// one tile (64 x blockDim.y lanes, one z chunk) of the E face pass
-template <bool YEE, bool CPML, bool TBL>
+// Field storage of the face passes: each field array holds fp32 values, or
+// binary16 values with a per-array scale.
+struct F4 {
+  float v[4];
+};
+
+template <class T> struct Store;
+// fp32 fields: plain 16-byte vectors (the scale arguments are unused)
+template <> struct Store<float> {
+  static __device__ __forceinline__ F4 ld4(const float *b, long long p, float) {
+    const float4 t = *reinterpret_cast<const float4 *>(b + p);
+    return {{t.x, t.y, t.z, t.w}};
+  }
+  static __device__ __forceinline__ float ld1(const float *b, long long p, float) { return b[p]; }
+  static __device__ __forceinline__ void st1(float *b, long long p, float, float v) { b[p] = v; }
+  static __device__ __forceinline__ void st4(float *b, long long p, float, const F4 &v) {
+    *reinterpret_cast<float4 *>(b + p) = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+  }
+};
+// binary16 fields with a per-array scale: value = half * s, stored as
+// round(value / s); 8-byte vectors, fp32 arithmetic.  Every scale product is
+// rounded to fp32 on its own (an empty asm barrier keeps the compiler from
+// fusing it into the arithmetic that uses it, or into the conversion to
+// binary16 as a mixed-precision FMA, which would round once instead of
+// twice), so every kernel shape rounds alike.
+__device__ __forceinline__ float rounded(float x) {
+  asm("" : "+v"(x));
+  return x;
+}
+template <> struct Store<__half> {
+  static __device__ __forceinline__ F4 ld4(const float *b, long long p, float s) {
+    const uint2 t = *reinterpret_cast<const uint2 *>(reinterpret_cast<const __half *>(b) + p);
+    const float2 lo = __half22float2(*reinterpret_cast<const __half2 *>(&t.x));
+    const float2 hi = __half22float2(*reinterpret_cast<const __half2 *>(&t.y));
+    return {{rounded(lo.x * s), rounded(lo.y * s), rounded(hi.x * s), rounded(hi.y * s)}};
+  }
+  static __device__ __forceinline__ float ld1(const float *b, long long p, float s) {
+    return rounded(__half2float(reinterpret_cast<const __half *>(b)[p]) * s);
+  }
+  static __device__ __forceinline__ void st1(float *b, long long p, float is, float v) {
+    reinterpret_cast<__half *>(b)[p] = __float2half_rn(rounded(v * is));
+  }
+  static __device__ __forceinline__ void st4(float *b, long long p, float is, const F4 &v) {
+    const __half2 lo = __floats2half2_rn(rounded(v.v[0] * is), rounded(v.v[1] * is));
+    const __half2 hi = __floats2half2_rn(rounded(v.v[2] * is), rounded(v.v[3] * is));
+    uint2 t;
+    t.x = *reinterpret_cast<const unsigned *>(&lo);
+    t.y = *reinterpret_cast<const unsigned *>(&hi);
+    *reinterpret_cast<uint2 *>(reinterpret_cast<__half *>(b) + p) = t;
+  }
+};
+
+// The column (i, j) of the tile's lane and the tile's z chunk bz.  Tiles are
+// 64-wide x segments of blockDim.y rows, or, in flat tile maps, runs of
+// 64 * blockDim.y consecutive columns of the x-y plane in x-fastest order
+// (grids whose nx + 1 is far from a multiple of 64 then leave no lane idle;
+// every neighbour of the face passes comes from memory, so a run may wrap
+// from one row to the next).  j > ny marks a lane past the plane.
+__device__ __forceinline__ void tile_column(const TileMap &tm, int blk, int nx, int &i, int &j,
+                                            int &bz) {
+  int bx, by;
+  decode_tile(tm, blk, bx, by, bz);
+  if (tm.flat) {
+    const int f = bx * kWave * (int)blockDim.y + (int)threadIdx.y * kWave + (int)threadIdx.x;
+    i = f % (nx + 1);
+    j = f / (nx + 1);
+  } else {
+    i = bx * kWave + threadIdx.x;
+    j = by * blockDim.y + threadIdx.y;
+  }
+}
+
+// one tile (64 x blockDim.y lanes, one z chunk) of the E face pass; T the
+// field storage (fp32, or binary16 with the arrays' scales)
+template <bool YEE, bool CPML, bool TBL, class T = float>
 __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
                                            const MedTab &tab, int blk) {
-  int bx, by, bz;
-  decode_tile(tm, blk, bx, by, bz);
-  const int i = bx * kWave + threadIdx.x;
-  const int j = by * blockDim.y + threadIdx.y;
+  typedef Store<T> S;
+  int i, j, bz;
+  tile_column(tm, blk, g.nx, i, j, bz);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny) return;
   const int k0 = bz * tm.kc;
@@ -385,8 +457,8 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
   constexpr bool NEEDH = YEE || CPML;
   float hx_km = 0.0f, hy_km = 0.0f;
   if (NEEDH) {
-    hx_km = Hx[p - pxy];
-    hy_km = Hy[p - pxy];
+    hx_km = S::ld1(Hx, p - pxy, g.fs[3]);
+    hy_km = S::ld1(Hy, p - pxy, g.fs[4]);
   }
   const float idyh = g.id[4][j + 1];
   const float idxh = g.id[3][i + 1];
@@ -405,15 +477,15 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
     float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f, hz_jm = 0.0f, hx_jm = 0.0f, hz_im = 0.0f,
           hy_im = 0.0f;
     if (NEEDH) {
-      hx0 = Hx[p]; hy0 = Hy[p]; hz0 = Hz[p];
-      hz_jm = Hz[p - px]; hx_jm = Hx[p - px];
-      hz_im = Hz[p - 1]; hy_im = Hy[p - 1];
+      hx0 = S::ld1(Hx, p, g.fs[3]); hy0 = S::ld1(Hy, p, g.fs[4]); hz0 = S::ld1(Hz, p, g.fs[5]);
+      hz_jm = S::ld1(Hz, p - px, g.fs[5]); hx_jm = S::ld1(Hx, p - px, g.fs[3]);
+      hz_im = S::ld1(Hz, p - 1, g.fs[5]); hy_im = S::ld1(Hy, p - 1, g.fs[4]);
     }
     int mx = 0, my = 0, mz = 0;
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    if (cx) { mx = g.mi[0][p]; ex = Ex[p]; }
-    if (cy) { my = g.mi[1][p]; ey = Ey[p]; }
-    if (cz) { mz = g.mi[2][p]; ez = Ez[p]; }
+    if (cx) { mx = g.mi[0][p]; ex = S::ld1(Ex, p, g.fs[0]); }
+    if (cy) { my = g.mi[1][p]; ey = S::ld1(Ey, p, g.fs[1]); }
+    if (cz) { mz = g.mi[2][p]; ez = S::ld1(Ez, p, g.fs[2]); }
     int qzn = 0;
     const bool szn = CPML && slab_node(k, nz, g.L[2][0], g.L[2][1], qzn);
     // ---- compute
@@ -431,7 +503,7 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
           e = madd(e, -c.y, psi_step(g.psi[PSI_EXZ], s, g.pbe[2][k + 1], hy0 - hy_km, g.pce[2][k + 1]));
         }
       }
-      Ex[p] = e;
+      S::st1(Ex, p, g.fi[0], e);
     }
     if (cy) {
       float e = ey;
@@ -447,7 +519,7 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
           e = madd(e, -c.y, psi_step(g.psi[PSI_EYX], s, g.pbe[0][i + 1], hz0 - hz_im, g.pce[0][i + 1]));
         }
       }
-      Ey[p] = e;
+      S::st1(Ey, p, g.fi[1], e);
     }
     if (cz) {
       float e = ez;
@@ -463,20 +535,19 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
           e = madd(e, c.y, psi_step(g.psi[PSI_EZX], s, g.pbe[0][i + 1], hy0 - hy_im, g.pce[0][i + 1]));
         }
       }
-      Ez[p] = e;
+      S::st1(Ez, p, g.fi[2], e);
     }
     if (NEEDH) { hx_km = hx0; hy_km = hy0; }
   }
 }
 
 // one tile of the H face pass
-template <bool YEE, bool CPML, bool TBL>
+template <bool YEE, bool CPML, bool TBL, class T = float>
 __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
                                            const MedTab &tab, int blk) {
-  int bx, by, bz;
-  decode_tile(tm, blk, bx, by, bz);
-  const int i = bx * kWave + threadIdx.x;
-  const int j = by * blockDim.y + threadIdx.y;
+  typedef Store<T> S;
+  int i, j, bz;
+  tile_column(tm, blk, g.nx, i, j, bz);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny) return;
   const int k0 = bz * tm.kc;
@@ -491,7 +562,7 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
   float *__restrict__ Hz = g.f[5];
   const float idye = g.id[1][j + 1];
   const float idxe = g.id[0][i + 1];
-  float ex_c = Ex[p], ey_c = Ey[p];
+  float ex_c = S::ld1(Ex, p, g.fs[0]), ey_c = S::ld1(Ey, p, g.fs[1]);
   int qxc = 0, qyc = 0;
   bool sxc = false, syc = false;
   if (CPML) {
@@ -500,15 +571,15 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
   }
   for (int k = k0; k < k1; ++k, p += pxy) {
     const float idze = g.id[2][k + 1];
-    const float ex_kp = Ex[p + pxy], ey_kp = Ey[p + pxy];
-    const float ez_c = Ez[p];
-    const float ez_jp = Ez[p + px], ex_jp = Ex[p + px];
-    const float ez_ip = Ez[p + 1], ey_ip = Ey[p + 1];
+    const float ex_kp = S::ld1(Ex, p + pxy, g.fs[0]), ey_kp = S::ld1(Ey, p + pxy, g.fs[1]);
+    const float ez_c = S::ld1(Ez, p, g.fs[2]);
+    const float ez_jp = S::ld1(Ez, p + px, g.fs[2]), ex_jp = S::ld1(Ex, p + px, g.fs[0]);
+    const float ez_ip = S::ld1(Ez, p + 1, g.fs[2]), ey_ip = S::ld1(Ey, p + 1, g.fs[1]);
     int qzc = 0;
     const bool szc = CPML && slab_cell(k, nz, g.L[2][0], g.L[2][1], qzc);
     if (j < ny && k < nz) {  // Hx sweep: x[0,nx] y[0,ny-1] z[0,nz-1]
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[3][p]);
-      float h = Hx[p];
+      float h = S::ld1(Hx, p, g.fs[3]);
       if (YEE) h = yee_upd(c.x, h, c.y, curl2(ey_kp - ey_c, idze, ez_jp - ez_c, idye));
       if (CPML) {
         if (syc) {  // psi_Hxy (borderscpml.F90:911-921)
@@ -520,11 +591,11 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
           h = madd(h, c.y, psi_step(g.psi[PSI_HXZ], s, g.pbm[2][k + 1], ey_kp - ey_c, g.pcm[2][k + 1]));
         }
       }
-      Hx[p] = h;
+      S::st1(Hx, p, g.fi[3], h);
     }
     if (i < nx && k < nz) {  // Hy sweep: x[0,nx-1] y[0,ny] z[0,nz-1]
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[4][p]);
-      float h = Hy[p];
+      float h = S::ld1(Hy, p, g.fs[4]);
       if (YEE) h = yee_upd(c.x, h, c.y, curl2(ez_ip - ez_c, idxe, ex_kp - ex_c, idze));
       if (CPML) {
         if (szc) {  // psi_Hyz (:1001-1011)
@@ -536,11 +607,11 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
           h = madd(h, c.y, psi_step(g.psi[PSI_HYX], s, g.pbm[0][i + 1], ez_ip - ez_c, g.pcm[0][i + 1]));
         }
       }
-      Hy[p] = h;
+      S::st1(Hy, p, g.fi[4], h);
     }
     if (i < nx && j < ny) {  // Hz sweep: x[0,nx-1] y[0,ny-1] z[0,nz]
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[5][p]);
-      float h = Hz[p];
+      float h = S::ld1(Hz, p, g.fs[5]);
       if (YEE) h = yee_upd(c.x, h, c.y, curl2(ex_jp - ex_c, idye, ey_ip - ey_c, idxe));
       if (CPML) {
         if (syc) {  // psi_Hzy (:931-941)
@@ -552,7 +623,7 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
           h = madd(h, -c.y, psi_step(g.psi[PSI_HZX], s, g.pbm[0][i + 1], ey_ip - ey_c, g.pcm[0][i + 1]));
         }
       }
-      Hz[p] = h;
+      S::st1(Hz, p, g.fi[5], h);
     }
     ex_c = ex_kp;
     ey_c = ey_kp;
@@ -566,44 +637,6 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
 // shuffle, and from one extra load at the wave's edge.  The row pitch leaves
 // room for the last vector of a row, whose cells beyond the sweep are written
 // back unchanged.
-struct F4 {
-  float v[4];
-};
-
-template <class T> struct Store;
-// fp32 fields: plain 16-byte vectors (the scale arguments are unused)
-template <> struct Store<float> {
-  static __device__ __forceinline__ F4 ld4(const float *b, long long p, float) {
-    const float4 t = *reinterpret_cast<const float4 *>(b + p);
-    return {{t.x, t.y, t.z, t.w}};
-  }
-  static __device__ __forceinline__ float ld1(const float *b, long long p, float) { return b[p]; }
-  static __device__ __forceinline__ void st4(float *b, long long p, float, const F4 &v) {
-    *reinterpret_cast<float4 *>(b + p) = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
-  }
-};
-// binary16 fields with a per-array scale: value = half * s, stored as
-// round(value / s); 8-byte vectors, fp32 arithmetic
-template <> struct Store<__half> {
-  static __device__ __forceinline__ F4 ld4(const float *b, long long p, float s) {
-    const uint2 t = *reinterpret_cast<const uint2 *>(reinterpret_cast<const __half *>(b) + p);
-    const float2 lo = __half22float2(*reinterpret_cast<const __half2 *>(&t.x));
-    const float2 hi = __half22float2(*reinterpret_cast<const __half2 *>(&t.y));
-    return {{lo.x * s, lo.y * s, hi.x * s, hi.y * s}};
-  }
-  static __device__ __forceinline__ float ld1(const float *b, long long p, float s) {
-    return __half2float(reinterpret_cast<const __half *>(b)[p]) * s;
-  }
-  static __device__ __forceinline__ void st4(float *b, long long p, float is, const F4 &v) {
-    const __half2 lo = __floats2half2_rn(v.v[0] * is, v.v[1] * is);
-    const __half2 hi = __floats2half2_rn(v.v[2] * is, v.v[3] * is);
-    uint2 t;
-    t.x = *reinterpret_cast<const unsigned *>(&lo);
-    t.y = *reinterpret_cast<const unsigned *>(&hi);
-    *reinterpret_cast<uint2 *>(reinterpret_cast<__half *>(b) + p) = t;
-  }
-};
-
 __device__ __forceinline__ void ld_mi4(const int16_t *m, long long p, int (&o)[4]) {
   const short4 t = *reinterpret_cast<const short4 *>(m + p);
   o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
@@ -918,8 +951,8 @@ __global__ void __launch_bounds__(256) k_pass_batch(BatchDev bd) {
       if (H) hpass4_tile<true, T>(g, tm, tab, blk);
       else epass4_tile<true, T>(g, tm, tab, blk);
     } else {
-      if (H) hpass_tile<true, true, true>(g, tm, tab, blk);
-      else epass_tile<true, true, true>(g, tm, tab, blk);
+      if (H) hpass_tile<true, true, true, T>(g, tm, tab, blk);
+      else epass_tile<true, true, true, T>(g, tm, tab, blk);
     }
   }
 }
@@ -929,9 +962,12 @@ int launch_pass_batch(const BatchDev &bd, int half, int blocks, int rows, int ve
   if (bd.total <= 0) return 0;
   const dim3 blk(kWave, (unsigned)rows);
   hipStream_t s = (hipStream_t)stream;
-  if (fp16) {
+  if (fp16 && vec == 4) {
     if (half) hipLaunchKernelGGL((k_pass_batch<true, __half, 4>), dim3(blocks), blk, lds, s, bd);
     else hipLaunchKernelGGL((k_pass_batch<false, __half, 4>), dim3(blocks), blk, lds, s, bd);
+  } else if (fp16) {
+    if (half) hipLaunchKernelGGL((k_pass_batch<true, __half, 1>), dim3(blocks), blk, lds, s, bd);
+    else hipLaunchKernelGGL((k_pass_batch<false, __half, 1>), dim3(blocks), blk, lds, s, bd);
   } else if (vec == 4) {
     if (half) hipLaunchKernelGGL((k_pass_batch<true, float, 4>), dim3(blocks), blk, lds, s, bd);
     else hipLaunchKernelGGL((k_pass_batch<false, float, 4>), dim3(blocks), blk, lds, s, bd);

@@ -42,7 +42,7 @@ def _bits(a):
 
 def _batch(insts, mode, steps, storage="fp32"):
     sol = [sf.Solver(p, mode=mode, probe_buffer=steps, storage=storage) for p in insts]
-    b = Batch(xcells=4 if storage == "fp16" else 1)
+    b = Batch()
     for s in sol:
         b.admit(s)
     b.step(steps)
@@ -95,8 +95,7 @@ def test_serving_requests_batched_equals_plain(storage):
     print(f"serving {storage}: " + ", ".join(f"{pb.name} {pb.nx}^3 x {ns}" for _, pb, ns in reqs),
           flush=True)
     plain = {r.name: r for r in serve.serve_plain(reqs, mode="fast", storage=storage)}
-    batched = {r.name: r for r in serve.serve_batched(reqs, mode="fast", storage=storage,
-                                                      xcells=4 if storage == "fp16" else 1)}
+    batched = {r.name: r for r in serve.serve_batched(reqs, mode="fast", storage=storage)}
     assert set(plain) == set(batched)
     for name in plain:
         a, b = plain[name], batched[name]

@@ -103,9 +103,11 @@ def test_fp16_storage_against_fp64(name, ns):
     assert err <= FP16_TOL
 
 
-def test_fp16_batch_equals_alone():
-    """Binary16 grids in one batch (persistent four-cell launch) give the same
-    bits as each grid advanced alone."""
+@pytest.mark.parametrize("xcells", [1, 4])
+def test_fp16_batch_equals_alone(xcells):
+    """Binary16 grids in one batch (persistent launch, flat one-cell tiles or
+    four-cell tiles) give the same bits as each grid advanced alone (the
+    four-cell passes)."""
     insts = [P.batch_instances()[q] for q in (2, 129, 248)]
     ns = 200
     alone = []
@@ -115,7 +117,7 @@ def test_fp16_batch_equals_alone():
         alone.append(s.probes())
         s.close()
     sol = [sf.Solver(pb, mode="fast", storage="fp16", probe_buffer=ns) for pb in insts]
-    b = Batch(xcells=4)
+    b = Batch(xcells=xcells)
     for s in sol:
         b.admit(s)
     b.step(ns)
