@@ -391,6 +391,14 @@ int64_t fdtd_batch_step_of(const fdtd_batch *b, int32_t slot);
  * persistent launch's block count (0: 8192).  Either storage runs either. */
 int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t xcells,
                           int32_t blocks);
+/* One launch per step for the batch (rows 4, 8 or 16; 0: the face passes
+ * above): every grid holds a second copy of its state and its own tile plan
+ * (as fdtd_set_step_mode, which this sets on each admitted grid), the batch
+ * advances pairs of steps with one launch per tile class over the tiles of
+ * all grids, and an odd last step runs the face passes.  fp32 storage only.
+ * The results equal the grids' own steps bit for bit.  No counterpart in the
+ * reference. */
+int fdtd_batch_set_step_mode(fdtd_batch *b, int32_t rows, int32_t zchunk);
 /* nsteps steps of every admitted grid. */
 int fdtd_batch_step(fdtd_batch *b, int32_t nsteps);
 int fdtd_batch_synchronize(fdtd_batch *b);

@@ -63,6 +63,7 @@ SIGNATURES = {
     "fdtd_batch_retire": (C.c_int, [vp, C.c_int32]),
     "fdtd_batch_step_of": (C.c_int64, [vp, C.c_int32]),
     "fdtd_batch_set_tiling": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "fdtd_batch_set_step_mode": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "fdtd_batch_step": (C.c_int, [vp, C.c_int32]),
     "fdtd_batch_synchronize": (C.c_int, [vp]),
     "fdtd_add_dft": (C.c_int, [vp, C.c_int32, i32p, i32p, C.c_int32, f32p, i32p]),

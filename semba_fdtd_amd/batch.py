@@ -18,7 +18,7 @@ from ._lib import FdtdError
 
 class Batch:
     def __init__(self, device: int = 0, stream=None, rows: int = 4, zchunk: int = 32,
-                 blocks: int = 0, xcells: int = 1):
+                 blocks: int = 0, xcells: int = 1, step_rows: int = 0, step_zchunk: int = 16):
         self.lib = lib = _lib.load()
         b = C.c_void_p()
         rc = lib.fdtd_batch_create(device, stream, C.byref(b))
@@ -28,6 +28,7 @@ class Batch:
         self.b = b
         self.members = {}          # slot -> Solver
         self.set_tiling(rows, zchunk, blocks, xcells)
+        self.set_step_mode(step_rows, step_zchunk)
 
     def _ck(self, rc, what):
         if rc:
@@ -41,6 +42,14 @@ class Batch:
         persistent launch's block count (0: 8192)."""
         self._ck(self.lib.fdtd_batch_set_tiling(self.b, rows, zchunk, xcells, blocks),
                  "fdtd_batch_set_tiling")
+
+    def set_step_mode(self, rows: int, zchunk: int = 16):
+        """One launch per step for the whole batch (rows 4, 8 or 16 per
+        block, zchunk planes per chunk; 0: the face passes): every grid's
+        own tile plan, one launch per tile class over all grids, steps in
+        pairs.  fp32 storage only; the results do not change."""
+        self._ck(self.lib.fdtd_batch_set_step_mode(self.b, rows, zchunk),
+                 "fdtd_batch_set_step_mode")
 
     def admit(self, solver) -> int:
         """Admit a set-up Solver at its current step; returns its slot."""

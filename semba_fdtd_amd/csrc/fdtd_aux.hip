@@ -176,10 +176,8 @@ int launch_media_narrow(const int16_t *src, uint8_t *dst, long long n, void *str
 
 // Incid of every TF/SF face cell for the one-launch step (the same
 // arithmetic as k_tfsf_list, which applies it in place)
-__global__ void __launch_bounds__(256) k_pw_incident(const PwFace *faces, int nf0,
-                                                     const StepSrc *sp, const int *dn,
-                                                     float *inc) {
-  const int q = blockIdx.y;
+__device__ __forceinline__ void pw_incident_face(const PwFace *faces, int nf0, const StepSrc *sp,
+                                                 const int *dn, float *inc, int q) {
   const PwFace &f = faces[q];
   const int half = q >= nf0 ? 1 : 0;
   const int ua = f.axis == 0 ? 1 : 0;
@@ -198,12 +196,34 @@ __global__ void __launch_bounds__(256) k_pw_incident(const PwFace *faces, int nf
   }
 }
 
+__global__ void __launch_bounds__(256) k_pw_incident(const PwFace *faces, int nf0,
+                                                     const StepSrc *sp, const int *dn,
+                                                     float *inc) {
+  pw_incident_face(faces, nf0, sp, dn, inc, blockIdx.y);
+}
+
+// the same for the grids of a batch: blockIdx.z the grid, blockIdx.y its face
+__global__ void __launch_bounds__(256) k_pw_incident_many(const PwIncJob *jobs) {
+  const PwIncJob &J = jobs[blockIdx.z];
+  if ((int)blockIdx.y >= J.nf) return;
+  pw_incident_face(J.faces, J.nf0, J.src, J.dn, J.inc, blockIdx.y);
+}
+
 int launch_pw_incident(const PwFace *faces, int nf0, int nf, long long maxcells,
                        const StepSrc *src, const int *dn, float *inc, void *stream) {
   if (nf <= 0 || maxcells <= 0) return 0;
   const long long b = std::min<long long>((maxcells + 255) / 256, 4096);
   hipLaunchKernelGGL(k_pw_incident, dim3((unsigned)b, (unsigned)nf), dim3(256), 0,
                      (hipStream_t)stream, faces, nf0, src, dn, inc);
+  return (int)hipGetLastError();
+}
+
+int launch_pw_incident_many(const PwIncJob *jobs, int njobs, int maxnf, long long maxcells,
+                            void *stream) {
+  if (njobs <= 0 || maxnf <= 0 || maxcells <= 0) return 0;
+  const long long b = std::min<long long>((maxcells + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_pw_incident_many, dim3((unsigned)b, (unsigned)maxnf, (unsigned)njobs),
+                     dim3(256), 0, (hipStream_t)stream, jobs);
   return (int)hipGetLastError();
 }
 

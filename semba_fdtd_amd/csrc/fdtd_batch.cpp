@@ -15,6 +15,18 @@
 // captured once per membership in a HIP graph and replayed.  Each grid keeps
 // its own step counter (its dstep), so grids admitted at different times
 // advance in lock-step from different n.
+//
+// With one launch per step (fdtd_batch_set_step_mode) the batch advances
+// pairs of steps (A -> B -> A, every grid holding its second state copy), and
+// each step of a pair runs:
+//   incident field of the TF/SF faces  one launch for all grids with faces
+//   step kernels                       one launch per tile class (and block
+//                                      shape and media width) over the tiles
+//                                      of every grid's own tile plan
+//   Mur faces                          per grid with Mur faces
+//   probes, step counters              one launch each for all grids
+//   frequency-domain sums              per grid
+// An odd last step runs the two-pass step above.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -71,6 +83,24 @@ struct fdtd_batch {
   int nprobesets = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // one launch per step: rows per block (0: two passes) and planes per chunk
+  int step_rows = 0, step_kc = 16;
+  bool sready = false;
+  std::vector<void *> sallocs;
+  StepDev *dsd[2][2] = {};           // [direction][previous field exact]: per grid
+  DevGrid *dgo[2] = {};              // per direction: the written copy of each grid
+  struct ClassLaunch {
+    int cls, narrow, rows;
+    size_t lds;
+    GridTile *tiles;
+    int n;
+  };
+  std::vector<ClassLaunch> slaunch;
+  PwIncJob *djobs = nullptr;
+  int njobs = 0, maxnf = 0;
+  long long maxcells = 0;
+  hipGraph_t pgraph = nullptr;
+  hipGraphExec_t pexec = nullptr;
 };
 
 namespace {
@@ -102,8 +132,20 @@ void drop_graph(fdtd_batch *b) {
   b->graph = nullptr;
 }
 
+void drop_step(fdtd_batch *b) {
+  if (b->pexec) hipGraphExecDestroy(b->pexec);
+  if (b->pgraph) hipGraphDestroy(b->pgraph);
+  b->pexec = nullptr;
+  b->pgraph = nullptr;
+  for (void *p : b->sallocs) hipFree(p);
+  b->sallocs.clear();
+  b->slaunch.clear();
+  b->sready = false;
+}
+
 void drop_descriptors(fdtd_batch *b) {
   drop_graph(b);
+  drop_step(b);
   for (void *p : b->allocs) hipFree(p);
   b->allocs.clear();
   b->pw.clear();
@@ -111,10 +153,10 @@ void drop_descriptors(fdtd_batch *b) {
 }
 
 template <class T>
-int bupload(fdtd_batch *b, T **dst, const T *src, size_t n) {
+int bupload(fdtd_batch *b, T **dst, const T *src, size_t n, std::vector<void *> *list = nullptr) {
   void *p = nullptr;
   BCHK(b, hipMalloc(&p, std::max<size_t>(n * sizeof(T), 16)));
-  b->allocs.push_back(p);
+  (list ? list : &b->allocs)->push_back(p);
   if (n) BCHK(b, hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, b->stream));
   *dst = (T *)p;
   return FDTD_OK;
@@ -265,6 +307,137 @@ int build_descriptors(fdtd_batch *b) {
       }
   BCHK(b, hipStreamSynchronize(b->stream));
   b->ready = true;
+  return FDTD_OK;
+}
+
+// The one-launch step of every grid: each grid's own tile plan and
+// descriptors (build_step, under the batch's rows and chunk), gathered into
+// one launch per (class, block rows, media width).
+int build_step_descriptors(fdtd_batch *b) {
+  if (b->sready) return FDTD_OK;
+  drop_step(b);
+  const bool exact = b->mode == FDTD_MODE_EXACT;
+  int rc;
+  std::vector<StepDev> sds[2][2];
+  std::vector<DevGrid> go[2];
+  std::vector<PwIncJob> jobs;
+  struct Key {
+    int cls, narrow, rows;
+    bool operator<(const Key &o) const {
+      return cls != o.cls ? cls < o.cls : narrow != o.narrow ? narrow < o.narrow : rows < o.rows;
+    }
+  };
+  std::vector<std::pair<Key, std::vector<GridTile>>> groups;
+  std::vector<size_t> glds;
+  b->maxnf = 0;
+  b->maxcells = 0;
+  int gi = 0;
+  for (size_t s = 0; s < b->slots.size(); ++s) {
+    fdtd_handle *h = b->slots[s];
+    if (!h) continue;
+    if (!step_supported(h, b->step_rows))
+      return bfail(b, FDTD_ERR_STATE,
+                   "slot %zu: one-launch steps need fp32 storage, small media tables and at "
+                   "most 64 TF/SF faces per half", s);
+    if (h->step_rows != b->step_rows || h->step_kc != b->step_kc) {
+      h->step_rows = b->step_rows;
+      h->step_kc = b->step_kc;
+      invalidate_graph(h);
+    }
+    if ((rc = build_step(h)) || (rc = sync_b(h))) return bfail(b, rc, "slot %zu: %s", s, h->err.c_str());
+    for (int d = 0; d < 2; ++d) {
+      for (int e = 0; e < 2; ++e) {
+        StepDev sd = h->sd[d];
+        sd.prev_from_field = e ? 0 : 1;
+        sd.tiles = nullptr;
+        sd.tm.nblk = 0;
+        sds[d][e].push_back(sd);
+      }
+      go[d].push_back(h->sd[d].go);
+    }
+    if (h->step_nf > 0) {
+      jobs.push_back(PwIncJob{h->dsfaces, h->step_nf0, h->step_nf, h->dstepsrc, h->dstep, h->dinc});
+      b->maxnf = std::max(b->maxnf, h->step_nf);
+      b->maxcells = std::max(b->maxcells, h->step_maxcells);
+    }
+    const DevGrid &g = h->g;
+    for (size_t q = 0; q < h->step_cls.size(); ++q) {
+      const Key key{h->step_cls[q], h->sd[0].narrow, h->step_crows[q]};
+      size_t gq = 0;
+      while (gq < groups.size() && (groups[gq].first < key || key < groups[gq].first)) ++gq;
+      if (gq == groups.size()) {
+        groups.push_back({key, {}});
+        glds.push_back(0);
+      }
+      const size_t lds =
+          key.cls == 8 ? (exact ? exact::step4_lds_bytes(g, key.rows) : fast::step4_lds_bytes(g, key.rows))
+                       : (exact ? exact::step_lds_bytes(g, key.rows, h->step_nf, h->sd[0].tm.kc)
+                                : fast::step_lds_bytes(g, key.rows, h->step_nf, h->sd[0].tm.kc));
+      glds[gq] = std::max(glds[gq], lds);
+      for (int t = 0; t < h->step_cnt[q]; ++t) {
+        GridTile gt{};
+        gt.t = h->step_tiles[(size_t)h->step_off[q] + (size_t)t];
+        gt.grid = gi;
+        groups[gq].second.push_back(gt);
+      }
+    }
+    ++gi;
+  }
+  for (int d = 0; d < 2; ++d) {
+    for (int e = 0; e < 2; ++e)
+      if ((rc = bupload(b, &b->dsd[d][e], sds[d][e].data(), sds[d][e].size(), &b->sallocs)))
+        return rc;
+    if ((rc = bupload(b, &b->dgo[d], go[d].data(), go[d].size(), &b->sallocs))) return rc;
+  }
+  b->njobs = (int)jobs.size();
+  b->djobs = nullptr;
+  if (!jobs.empty() && (rc = bupload(b, &b->djobs, jobs.data(), jobs.size(), &b->sallocs)))
+    return rc;
+  for (size_t gq = 0; gq < groups.size(); ++gq) {
+    const std::vector<GridTile> &v = groups[gq].second;
+    if (v.empty()) continue;
+    if (v.size() > 0x7fffffffULL) return bfail(b, FDTD_ERR_ARG, "batch holds too many tiles");
+    fdtd_batch::ClassLaunch cl{groups[gq].first.cls, groups[gq].first.narrow,
+                               groups[gq].first.rows, glds[gq], nullptr, (int)v.size()};
+    if ((rc = bupload(b, &cl.tiles, v.data(), v.size(), &b->sallocs))) return rc;
+    b->slaunch.push_back(cl);
+  }
+  BCHK(b, hipStreamSynchronize(b->stream));
+  b->sready = true;
+  return FDTD_OK;
+}
+
+// one step of a pair for every grid, direction d (0: A -> B, 1: B -> A)
+int launch_step_dir(fdtd_batch *b, int d, bool exact_prev) {
+  const bool exact = b->mode == FDTD_MODE_EXACT;
+  int e, rc;
+  if (b->njobs &&
+      launch_pw_incident_many(b->djobs, b->njobs, b->maxnf, b->maxcells, b->stream))
+    return bfail(b, FDTD_ERR_HIP, "batch incident-field launch failed");
+  const StepDev *sds = b->dsd[d][exact_prev ? 1 : 0];
+  for (const fdtd_batch::ClassLaunch &cl : b->slaunch) {
+    e = exact ? exact::launch_step_many(sds, cl.tiles, cl.n, cl.narrow, cl.rows, cl.cls, cl.lds,
+                                        b->stream)
+              : fast::launch_step_many(sds, cl.tiles, cl.n, cl.narrow, cl.rows, cl.cls, cl.lds,
+                                       b->stream);
+    if (e) return bfail(b, FDTD_ERR_HIP, "batch step launch failed: %s", hipGetErrorString((hipError_t)e));
+  }
+  for (fdtd_handle *h : b->slots) {
+    if (!h || h->mur_runs.empty()) continue;
+    if (launch_mur(h->sd[d].go, h->dmur, (int)h->mur_runs.size(), h->mur_total, exact, b->stream))
+      return bfail(b, FDTD_ERR_HIP, "batch Mur launch failed");
+  }
+  if (launch_probes_many(b->dgo[d], b->dprobes, b->nprobesets, b->stream))
+    return bfail(b, FDTD_ERR_HIP, "batch probe launch failed");
+  for (fdtd_handle *h : b->slots) {
+    if (!h) continue;
+    for (const DftDev &df : h->dfts)
+      if (launch_dft(h->sd[d].go, df, 0, h->dstep, b->stream))
+        return bfail(b, FDTD_ERR_HIP, "batch DFT launch failed");
+  }
+  (void)rc;
+  if (launch_tick_many(b->dcounters, b->ncounters, b->stream))
+    return bfail(b, FDTD_ERR_HIP, "batch step counter launch failed");
   return FDTD_OK;
 }
 
@@ -440,6 +613,16 @@ int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t x
   return FDTD_OK;
 }
 
+int fdtd_batch_set_step_mode(fdtd_batch *b, int32_t rows, int32_t zchunk) {
+  if (!b || !(rows == 0 || rows == 4 || rows == 8 || rows == 16) || zchunk < 1)
+    return bfail(b, FDTD_ERR_ARG, "batch step mode %d rows x %d planes not supported", rows,
+                 zchunk);
+  b->step_rows = rows;
+  b->step_kc = zchunk;
+  drop_step(b);
+  return FDTD_OK;
+}
+
 int fdtd_batch_step(fdtd_batch *b, int32_t nsteps) {
   if (!b || nsteps < 0) return bfail(b, FDTD_ERR_ARG, "bad step count");
   hipSetDevice(b->device);
@@ -463,7 +646,44 @@ int fdtd_batch_step(fdtd_batch *b, int32_t nsteps) {
     if (b->slots[s] && b->slots[s]->version != b->version[s]) b->ready = false;
   int rc = build_descriptors(b);
   if (rc) return rc;
-  if (!b->gexec) {
+  // one launch per step: pairs A -> B -> A, an odd last step on the two passes
+  const int pairs = b->step_rows > 0 ? nsteps / 2 : 0;
+  const int singles = nsteps - 2 * pairs;
+  if (pairs > 0) {
+    if ((rc = build_step_descriptors(b))) return rc;
+    // (building the members' step state bumped their versions)
+    for (size_t s = 0; s < b->slots.size(); ++s)
+      if (b->slots[s]) b->version[s] = b->slots[s]->version;
+    bool exact_prev = false;
+    for (fdtd_handle *h : b->slots)
+      if (h) {
+        // fields written from outside since the last pair: B follows A
+        if ((rc = sync_b(h))) return bfail(b, rc, "grid: %s", h->err.c_str());
+        exact_prev = exact_prev || h->prev_exact;
+      }
+    int p0 = 0;
+    if (exact_prev) {
+      // the first pair after fields were set from outside reads the stored
+      // previous fields of the pole updates
+      if ((rc = launch_step_dir(b, 0, true)) || (rc = launch_step_dir(b, 1, true))) return rc;
+      p0 = 1;
+      for (fdtd_handle *h : b->slots)
+        if (h) h->prev_exact = false;
+    }
+    if (p0 < pairs && !b->pexec) {
+      BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int r2 = launch_step_dir(b, 0, false);
+      if (!r2) r2 = launch_step_dir(b, 1, false);
+      hipGraph_t graph = nullptr;
+      hipError_t ec = hipStreamEndCapture(b->stream, &graph);
+      if (r2) { if (graph) hipGraphDestroy(graph); return r2; }
+      if (ec != hipSuccess) return bfail(b, FDTD_ERR_HIP, "batch capture failed: %s", hipGetErrorString(ec));
+      b->pgraph = graph;
+      BCHK(b, hipGraphInstantiate(&b->pexec, graph, nullptr, nullptr, 0));
+    }
+    for (int q = p0; q < pairs; ++q) BCHK(b, hipGraphLaunch(b->pexec, b->stream));
+  }
+  if (singles > 0 && !b->gexec) {
     BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int r2 = launch_step(b);
     hipGraph_t graph = nullptr;
@@ -473,7 +693,7 @@ int fdtd_batch_step(fdtd_batch *b, int32_t nsteps) {
     b->graph = graph;
     BCHK(b, hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
   }
-  for (int32_t s = 0; s < nsteps; ++s) BCHK(b, hipGraphLaunch(b->gexec, b->stream));
+  for (int32_t s = 0; s < singles; ++s) BCHK(b, hipGraphLaunch(b->gexec, b->stream));
   for (size_t s = 0; s < b->slots.size(); ++s)
     if (b->slots[s]) {
       b->step[s] += nsteps;

@@ -162,6 +162,7 @@ struct fdtd_handle {
   // (fields set from outside, granular calls): the next pair reads it
   bool prev_exact = true;
   sfd::StepTile *dtiles = nullptr;
+  std::vector<sfd::StepTile> step_tiles;  // host copy of the tile list (batches)
   // TF/SF faces of both halves for the step kernels and the buffer of their
   // incident field, refreshed before every step
   sfd::PwFace *dsfaces = nullptr;
@@ -198,6 +199,12 @@ int run_dft(fdtd_handle *h, int n, const int *dn);
 int run_mur(fdtd_handle *h);
 int run_ade(fdtd_handle *h, int side);
 void invalidate_graph(fdtd_handle *h);
+// the one-launch step: its descriptors and tile plan (step_rows > 0), the
+// second state refreshed from the first when stale, a class's rows per block
+int build_step(fdtd_handle *h);
+int sync_b(fdtd_handle *h);
+int class_rows(const fdtd_handle *h, int c);
+bool step_supported(const fdtd_handle *h, int rows);
 }  // namespace sfd_host
 
 // a failed HIP call on grid h: its message, and FDTD_ERR_HIP from the caller

@@ -408,7 +408,7 @@ bool step_supported(const fdtd_handle *h, int rows) {
 // tiles into whole bands, else the step's.  By default the class with CPML,
 // poles and sources runs 4-row blocks under 16-row tiles: its 16-row kernel
 // needs more registers than 1024-thread blocks leave a lane and spills.
-static int class_rows(const fdtd_handle *h, int c) {
+int class_rows(const fdtd_handle *h, int c) {
   const int R = h->step_rows;
   if (R <= 0) return 0;
   if (c == kPlanVec) {
@@ -427,7 +427,7 @@ static int class_rows(const fdtd_handle *h, int c) {
   return q > 1 && q <= R && (R - 1) % (q - 1) == 0 ? q : R;
 }
 
-static int build_step(fdtd_handle *h) {
+int build_step(fdtd_handle *h) {
   if (h->step_ready) return FDTD_OK;
   const int R = h->step_rows;
   if (!step_supported(h, R))
@@ -618,6 +618,7 @@ static int build_step(fdtd_handle *h) {
     h->dtiles = nullptr;
     if (!all.empty() && (rc = upload(h, &h->dtiles, all.data(), all.size(), &h->step_small)))
       return rc;
+    h->step_tiles = all;
   }
   src.waves = h->dwaves;
   src.phys = h->dphys;
@@ -670,7 +671,7 @@ static int build_step(fdtd_handle *h) {
 
 // cells outside the sweeps are never written by a step, so B's must equal
 // A's: refreshed after the fields were set from outside
-static int sync_b(fdtd_handle *h) {
+int sync_b(fdtd_handle *h) {
   if (!h->step_stale) return FDTD_OK;
   const size_t ne = field_elems(h);
   for (int c = 0; c < 6; ++c)

@@ -238,6 +238,23 @@ inline StepTile step_tile(int x0, int wx, int y0, int wy, int k0, int k1) {
   return StepTile{x0 | (wx << 20), y0 | (wy << 20), k0, k1};
 }
 
+// the incident-field pre-pass of one grid of a batch (launch_pw_incident's
+// arguments)
+struct PwIncJob {
+  const PwFace *faces;
+  int nf0, nf;
+  const StepSrc *src;
+  const int *dn;
+  float *inc;
+};
+
+// a tile of one grid of a batch stepped in one launch with other grids' tiles
+struct GridTile {
+  StepTile t;
+  int grid;                   // index into the launch's StepDev array
+  int pad[3];
+};
+
 struct StepDev {
   DevGrid gi, go;             // read (step n) and write (step n+1) views
   const StepAux *aux;
@@ -339,6 +356,8 @@ __device__ __forceinline__ float nodal_apply(float v, const NodalCell &c, float 
                int yee, int cpml, int ade, void *stream);                    \
   size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc);       \
   int launch_step(const StepDev &sd, int rows, int cls, void *stream);       \
+  int launch_step_many(const StepDev *sds, const GridTile *tiles, int n, int narrow, int rows, \
+                       int cls, size_t lds, void *stream);                   \
   size_t step4_lds_bytes(const DevGrid &g, int rows);                       \
   int launch_step4(const StepDev &sd, int rows, void *stream);              \
   size_t ade_lds_of(const DevGrid &g, int side);                            \
@@ -377,6 +396,10 @@ int launch_box_gather_f32(const void *grid, size_t es, float scale, float *packe
 // first free axis fastest
 int launch_pw_incident(const PwFace *faces, int nf0, int nf, long long maxcells,
                        const StepSrc *src, const int *dn, float *inc, void *stream);
+// the pre-pass of many grids in one launch (maxnf: the most faces of a grid,
+// maxcells: the most cells of a face)
+int launch_pw_incident_many(const PwIncJob *jobs, int njobs, int maxnf, long long maxcells,
+                            void *stream);
 // increments every counter of the list (per-grid step counters of a batch)
 int launch_tick_many(int *const *counters, int n, void *stream);
 int launch_probes(const DevGrid &g, const ProbeDev *pr, int np, float *buf,

@@ -1306,7 +1306,7 @@ static __host__ __device__ __forceinline__ size_t step_tab_bytes(int nmedia, int
 // columns such as the x CPML slabs, which would leave most of a 64-lane row
 // idle); the block's R waves then hold 2R rows.
 template <int R, bool CPML, bool ADE, bool SRC, class MT, int XW>
-__global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
+__device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) {
   static_assert(XW == 64 || XW == 32, "a row is a whole or a half wave");
   constexpr int RR = R * (kWave / XW);   // grid rows per block
   extern __shared__ float2 lds[];
@@ -1319,7 +1319,6 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
       gi, gi.gm1, gi.gm2, reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offh),
       &gi.ade[1]);
   float2 *xch = reinterpret_cast<float2 *>(reinterpret_cast<char *>(lds) + offx);
-  const StepTile tl = sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)];
   const int x0 = tl.x & 0xfffff, wx = tl.x >> 20;
   const int y0 = tl.y & 0xfffff, wy = tl.y >> 20;
   const int k0 = tl.k0;
@@ -1762,6 +1761,20 @@ __global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
   }
 }
 
+template <int R, bool CPML, bool ADE, bool SRC, class MT, int XW>
+__global__ void __launch_bounds__(64 * R) k_step(StepDev sd) {
+  step_tile<R, CPML, ADE, SRC, MT, XW>(sd, sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)]);
+}
+
+// The same tiles of many grids in one launch (batches of grids): block b runs
+// tile bt[b].t of grid bt[b].grid, whose descriptor is sds[bt[b].grid].
+template <int R, bool CPML, bool ADE, bool SRC, class MT, int XW>
+__global__ void __launch_bounds__(64 * R) k_step_many(const StepDev *__restrict__ sds,
+                                                      const GridTile *__restrict__ bt, int n) {
+  const GridTile g = bt[xcd_remap(blockIdx.x, n)];
+  step_tile<R, CPML, ADE, SRC, MT, XW>(sds[g.grid], g.t);
+}
+
 // ---- plain interior tiles, four x cells per lane ----------------------------
 // k_step4 is k_step's plain path (no CPML slab, pole state or source in the
 // tile) with each lane owning four consecutive x cells: one 16-byte load or
@@ -1828,7 +1841,7 @@ static __host__ __device__ __forceinline__ size_t step4_xch_offset(int nmedia, i
 }
 
 template <int R, class MT>
-__global__ void __launch_bounds__(64 * R) k_step4(StepDev sd) {
+__device__ __forceinline__ void step4_tile(const StepDev &sd, const StepTile tl) {
   typedef Media4<MT> M4;
   typedef typename M4::raw raw;
   extern __shared__ float2 lds[];
@@ -1842,7 +1855,6 @@ __global__ void __launch_bounds__(64 * R) k_step4(StepDev sd) {
   // E exchange between rows: [2 parities][R rows][64 lanes] of Ex, then of Ez
   f32x4 *xe = reinterpret_cast<f32x4 *>(reinterpret_cast<char *>(lds) + offx);
   f32x4 *xz = xe + 2 * R * kWave;
-  const StepTile tl = sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)];
   const int x0 = tl.x & 0xfffff, wx = tl.x >> 20;
   const int y0 = tl.y & 0xfffff, wy = tl.y >> 20;
   const int k0 = tl.k0;
@@ -1988,29 +2000,54 @@ __global__ void __launch_bounds__(64 * R) k_step4(StepDev sd) {
   }
 }
 
+template <int R, class MT>
+__global__ void __launch_bounds__(64 * R) k_step4(StepDev sd) {
+  step4_tile<R, MT>(sd, sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)]);
+}
+
+template <int R, class MT>
+__global__ void __launch_bounds__(64 * R) k_step4_many(const StepDev *__restrict__ sds,
+                                                       const GridTile *__restrict__ bt, int n) {
+  const GridTile g = bt[xcd_remap(blockIdx.x, n)];
+  step4_tile<R, MT>(sds[g.grid], g.t);
+}
+
 size_t step4_lds_bytes(const DevGrid &g, int rows) {
   return step4_xch_offset(g.nmedia, g.ade[0].ndisp, g.ade[1].ndisp) +
          (size_t)2 * 2 * rows * kWave * 16;
 }
 
-int launch_step4(const StepDev &sd, int rows, void *stream) {
-  const size_t lds = step4_lds_bytes(sd.gi, rows);
-  hipStream_t s = (hipStream_t)stream;
-  const dim3 grid((unsigned)sd.tm.nblk);
-  if (sd.tm.nblk <= 0) return 0;
-#define SFD_STEP4(RR)                                                                       \
-  do {                                                                                      \
-    if (sd.narrow)                                                                          \
-      hipLaunchKernelGGL((k_step4<RR, uint8_t>), grid, dim3(kWave, RR), lds, s, sd);        \
-    else                                                                                    \
-      hipLaunchKernelGGL((k_step4<RR, int16_t>), grid, dim3(kWave, RR), lds, s, sd);        \
+// The step kernels of one class over n blocks: the tiles of one grid
+// (sds == nullptr: k_step / k_step4 on sd) or tiles of many grids (k_step_many
+// / k_step4_many on sds and bt).  narrow: media read from the 8-bit copies.
+static int launch_step4_any(const StepDev &sd, const StepDev *sds, const GridTile *bt, int n,
+                            int narrow, int rows, size_t lds, hipStream_t s) {
+  const dim3 grid((unsigned)n);
+  if (n <= 0) return 0;
+#define SFD_STEP4_M(RR, M)                                                                    \
+  do {                                                                                        \
+    if (sds)                                                                                  \
+      hipLaunchKernelGGL((k_step4_many<RR, M>), grid, dim3(kWave, RR), lds, s, sds, bt, n);  \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_step4<RR, M>), grid, dim3(kWave, RR), lds, s, sd);               \
+  } while (0)
+#define SFD_STEP4(RR)                    \
+  do {                                   \
+    if (narrow) SFD_STEP4_M(RR, uint8_t); \
+    else SFD_STEP4_M(RR, int16_t);       \
   } while (0)
   if (rows == 4) SFD_STEP4(4);
   else if (rows == 8) SFD_STEP4(8);
   else if (rows == 16) SFD_STEP4(16);
   else return (int)hipErrorInvalidValue;
 #undef SFD_STEP4
+#undef SFD_STEP4_M
   return (int)hipGetLastError();
+}
+
+int launch_step4(const StepDev &sd, int rows, void *stream) {
+  return launch_step4_any(sd, nullptr, nullptr, sd.tm.nblk, sd.narrow, rows,
+                          step4_lds_bytes(sd.gi, rows), (hipStream_t)stream);
 }
 
 size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc) {
@@ -2022,20 +2059,25 @@ size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc) {
 
 // class cls: 0-7 the k_step variant (bits 1 CPML, 2 poles, 4 sources), 8 the
 // four-cell kernel, 9-16 the half-width variant of class cls - 9
-int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
-  if (cls == 8) return launch_step4(sd, rows, stream);
-  const size_t lds = step_lds_bytes(sd.gi, rows, sd.npw[0] + sd.npw[1], sd.tm.kc);
-  hipStream_t s = (hipStream_t)stream;
-  const dim3 grid((unsigned)sd.tm.nblk);
-  if (sd.tm.nblk <= 0) return 0;
+static int launch_step_any(const StepDev &sd, const StepDev *sds, const GridTile *bt, int n,
+                           int narrow, int rows, int cls, size_t lds, hipStream_t s) {
+  if (cls == 8) return launch_step4_any(sd, sds, bt, n, narrow, rows, lds, s);
+  const dim3 grid((unsigned)n);
+  if (n <= 0) return 0;
   const bool half = cls >= 9;
   if (half) cls -= 9;
-#define SFD_STEP_W(RR, C, A, S, W)                                                           \
-  do {                                                                                       \
-    if (sd.narrow)                                                                           \
-      hipLaunchKernelGGL((k_step<RR, C, A, S, uint8_t, W>), grid, dim3(kWave, RR), lds, s, sd); \
-    else                                                                                     \
-      hipLaunchKernelGGL((k_step<RR, C, A, S, int16_t, W>), grid, dim3(kWave, RR), lds, s, sd); \
+#define SFD_STEP_K(RR, C, A, S, M, W)                                                         \
+  do {                                                                                        \
+    if (sds)                                                                                  \
+      hipLaunchKernelGGL((k_step_many<RR, C, A, S, M, W>), grid, dim3(kWave, RR), lds, s, sds, \
+                         bt, n);                                                              \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_step<RR, C, A, S, M, W>), grid, dim3(kWave, RR), lds, s, sd);    \
+  } while (0)
+#define SFD_STEP_W(RR, C, A, S, W)                        \
+  do {                                                    \
+    if (narrow) SFD_STEP_K(RR, C, A, S, uint8_t, W);      \
+    else SFD_STEP_K(RR, C, A, S, int16_t, W);             \
   } while (0)
 #define SFD_STEP(RR, C, A, S)                  \
   do {                                         \
@@ -2067,7 +2109,21 @@ int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
 #undef SFD_STEP_R
 #undef SFD_STEP
 #undef SFD_STEP_W
+#undef SFD_STEP_K
   return (int)hipGetLastError();
+}
+
+int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
+  const size_t lds = cls == 8 ? step4_lds_bytes(sd.gi, rows)
+                              : step_lds_bytes(sd.gi, rows, sd.npw[0] + sd.npw[1], sd.tm.kc);
+  return launch_step_any(sd, nullptr, nullptr, sd.tm.nblk, sd.narrow, rows, cls, lds,
+                         (hipStream_t)stream);
+}
+
+int launch_step_many(const StepDev *sds, const GridTile *tiles, int n, int narrow, int rows,
+                     int cls, size_t lds, void *stream) {
+  const StepDev none{};
+  return launch_step_any(none, sds, tiles, n, narrow, rows, cls, lds, (hipStream_t)stream);
 }
 
 }  // namespace SFD_NS

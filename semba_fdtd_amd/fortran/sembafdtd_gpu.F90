@@ -43,7 +43,8 @@ module sembafdtd_gpu
              fdtd_add_dft, fdtd_dft_size, fdtd_read_dft, fdtd_update_observation, &
              fdtd_set_field_scale, fdtd_state_size, fdtd_save_state, fdtd_load_state, &
              fdtd_batch_create, fdtd_batch_destroy, fdtd_batch_admit, fdtd_batch_retire, &
-             fdtd_batch_step_of, fdtd_batch_set_tiling, fdtd_batch_step, &
+             fdtd_batch_step_of, fdtd_batch_set_tiling, fdtd_batch_set_step_mode, &
+             fdtd_batch_step, &
              fdtd_batch_synchronize, fdtd_set_step_mode, fdtd_get_step_mode, fdtd_time_step, &
              fdtd_step_classes, fdtd_set_step_class_rows, fdtd_get_step_class_rows, &
              fdtd_set_step_vector, &
@@ -517,6 +518,12 @@ module sembafdtd_gpu
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: b
          integer(c_int32_t), value :: rows, zchunk, xcells, blocks
+      end function
+      integer(c_int) function fdtd_batch_set_step_mode(b, rows, zchunk) &
+            bind(C, name="fdtd_batch_set_step_mode")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: b
+         integer(c_int32_t), value :: rows, zchunk
       end function
       integer(c_int) function fdtd_batch_step(b, nsteps) bind(C, name="fdtd_batch_step")
          import :: c_int, c_ptr, c_int32_t

@@ -156,3 +156,65 @@ def test_run_checks_step_and_probe_room():
     assert got.shape == (10, s.nprobe)
     assert lib.fdtd_run(h, 10, 10) == 0
     s.close()
+
+
+@pytest.mark.parametrize("rows", [16, 8])
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_batch_step_mode_equals_alone(mode, rows):
+    """One launch per step for the batch (every grid's own tile plan, one
+    launch per tile class over all grids, pairs of steps, an odd last step on
+    the face passes), grids joining and leaving between calls of odd and even
+    length: each grid equals itself advanced alone, bit for bit."""
+    print(f"batch step mode {mode} rows {rows}", flush=True)
+    probs = _problems()
+    counts = [21 + 69 + 10 + 30, 21 + 69 + 10, 69 + 10, 69]
+    alone = []
+    for (pb, _), ns in zip(probs, counts):
+        s = sf.Solver(pb, mode=mode)
+        s.run(ns)
+        alone.append(_state(s))
+        s.close()
+    sol = [sf.Solver(pb, mode=mode) for pb, _ in probs]
+    batch = Batch(step_rows=rows, step_zchunk=8)
+    sa = batch.admit(sol[0])
+    sb = batch.admit(sol[1])
+    batch.step(21)
+    sc = batch.admit(sol[2])
+    sd = batch.admit(sol[3])
+    batch.step(69)
+    batch.retire(sd)
+    batch.step(10)
+    batch.retire(sb)
+    batch.retire(sc)
+    batch.step(30)
+    batch.retire(sa)
+    assert [s.n for s in sol] == counts
+    for q, s in enumerate(sol):
+        got = _state(s)
+        want = alone[q]
+        assert np.array_equal(got[0].view(np.uint32), want[0].view(np.uint32)), q
+        for k in range(6):
+            assert np.array_equal(got[1][k].view(np.uint32), want[1][k].view(np.uint32)), (q, k)
+        for g, w in zip(got[2], want[2]):
+            assert np.array_equal(g.view(np.uint64), w.view(np.uint64)), q
+    batch.close()
+    for s in sol:
+        s.close()
+
+
+def test_batch_step_mode_refuses_binary16():
+    """One launch per step needs fp32 storage: a batch of binary16 grids in
+    step mode refuses to step (before any launch) and steps on the face
+    passes once step mode is off."""
+    from semba_fdtd_amd._lib import FdtdError
+    pb = P.scatterer_problem(nsteps=20, disp=False)
+    s = sf.Solver(pb, mode="fast", storage="fp16")
+    batch = Batch(step_rows=16)
+    batch.admit(s)
+    with pytest.raises(FdtdError):
+        batch.step(4)
+    batch.set_step_mode(0)
+    batch.step(4)
+    batch.close()
+    assert s.n == 4
+    s.close()
