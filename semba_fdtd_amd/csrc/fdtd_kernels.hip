@@ -1362,7 +1362,10 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
   // lane: the column in the tile's row; row: the tile's row (a wave holds
   // kWave / XW consecutive rows)
   const int lane = (int)threadIdx.x % XW, row = (int)threadIdx.y * (kWave / XW) + (int)threadIdx.x / XW;
-  const int i = x0 + lane, j = y0 + row;
+  // lanes and rows past the halo work on the halo column and row again: their
+  // results are never used, and their loads hit the halo's lines instead of
+  // fetching columns and rows the tile does not own
+  const int i = x0 + min(lane, wx), j = y0 + min(row, wy);
   const int nx = gi.nx, ny = gi.ny, nz = gi.nz;
   const bool in = i <= nx && j <= ny;
   // the tile owns lanes [0, wx) (wx <= XW - 1) and rows [0, wy) (wy <= RR -
@@ -1861,8 +1864,10 @@ __device__ __forceinline__ void step4_tile(const StepDev &sd, const StepTile tl)
   const int lane = threadIdx.x;
   const int row = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   const int nx = gi.nx, ny = gi.ny, nz = gi.nz;
-  const int j = y0 + row;
-  const int i = x0 + 4 * lane;          // the lane's first cell
+  // the lane's first cell; lanes and rows past the halo repeat the halo's
+  // (their results are never used, their loads hit the halo's lines)
+  const int j = y0 + min(row, wy);
+  const int i = x0 + 4 * min(lane, wx >> 2);
   const bool own = 4 * lane < wx && row < wy && j <= ny;
   const int k1 = min(tl.k1, nz + 1);
   // lanes past the grid's edge address the last whole vector of the row

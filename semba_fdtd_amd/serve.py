@@ -91,7 +91,7 @@ def serve_plain(requests, mode: str = "fast", storage: str = "fp32") -> List[Res
 
 def serve_batched(requests, mode: str = "fast", storage: str = "fp32",
                   reserve: int = 8 << 30, max_chunk: int = 250, rows: int = 4,
-                  zchunk: int = 32, blocks: int = 0, xcells: int = 1,
+                  zchunk: int = 32, blocks: int = 0, xcells: int = 1, step_rows: int = 0,
                   max_cells: Optional[int] = None, device: int = 0,
                   cancel: Optional[Callable[[str], bool]] = None,
                   check_finite: bool = True) -> List[Result]:
@@ -102,11 +102,12 @@ def serve_batched(requests, mode: str = "fast", storage: str = "fp32",
     admitted grid finishes (or after max_chunk steps), retire finished grids.
     max_cells trades latency for throughput: once the batch holds enough
     cells to fill the device, admitting more only stretches every admitted
-    request's time.  At each chunk boundary, requests with non-finite probe
+    request's time.  step_rows > 0 runs the batch one launch per step
+    (``Batch.set_step_mode``).  At each chunk boundary, requests with non-finite probe
     samples (check_finite) or cancelled by the caller leave the batch early
     with status "diverged" / "cancelled"."""
     t0 = time.perf_counter()
-    batch = Batch(rows=rows, zchunk=zchunk, blocks=blocks, xcells=xcells)
+    batch = Batch(rows=rows, zchunk=zchunk, blocks=blocks, xcells=xcells, step_rows=step_rows)
     waiting = list(requests)
     active = {}          # slot -> (solver, arrival, steps)
     cells = 0
@@ -126,7 +127,9 @@ def serve_batched(requests, mode: str = "fast", storage: str = "fp32",
         while waiting and waiting[0][0] <= now:
             arrival, pb, ns = waiting[0]
             free, _ = device_memory(device)
-            if active and free - device_bytes_estimate(pb) < reserve:
+            # one launch per step holds a second copy of the state
+            need = device_bytes_estimate(pb) * (2 if step_rows else 1)
+            if active and free - need < reserve:
                 break
             if active and max_cells is not None and cells >= max_cells:
                 break
