@@ -333,19 +333,86 @@ def config_rows(cpu_seconds=20.0):
     return out
 
 
-def _cpu_config(name, steps, variant):
-    """Child process of config_rows: the oracle on the named configuration
-    for `steps` steps; prints one JSON line."""
-    import oracle
+def _config_problem(name, steps):
+    """The configuration's grid for a CPU sample of `steps` steps."""
     from semba_fdtd_amd import problem as P
-    pb = (P.cavity_problem(n=64, nsteps=steps + 2) if name.startswith("config1")
-          else P.open_scatterer_problem(n=256, nsteps=max(steps + 2, 200)))
-    o = oracle.Oracle(pb, fma=variant == "fma")
+    if name.startswith("config1"):
+        return P.cavity_problem(n=64, nsteps=steps + 2)
+    if name.startswith("config2"):
+        return P.open_scatterer_problem(n=256, nsteps=max(steps + 2, 200))
+    if name.startswith("config3"):
+        return P.heterogeneous_problem(n=256, L=4, nsteps=steps + 2, host_media=True)
+    raise ValueError(name)
+
+
+def _cpu_config(name, steps, variant):
+    """Child process of config_rows and cpu_table: on the named configuration
+    for `steps` steps, the oracle with FMA contraction on ("fma") or off
+    ("nofma"), or the reference's own modules built with OpenMP
+    ("reference"; one instance per process: their module state is global);
+    prints one JSON line."""
+    import oracle
+    if name.startswith("config4"):
+        # a seeded sample of 8 of the 256 batched instances, one after another
+        from semba_fdtd_amd import problem as P
+        insts = P.batch_instances(count=256, nsteps=1500)
+        pick = sorted(np.random.default_rng(1004).choice(256, size=8, replace=False).tolist())
+        cells, dt = 0, 0.0
+        for q in pick:
+            pb = insts[q]
+            pb.nsteps_max = steps + 2
+            o = oracle.Oracle(pb, fma=variant == "fma")
+            o.run(2)
+            t0 = time.perf_counter()
+            o.run(steps)
+            dt += time.perf_counter() - t0
+            cells += pb.cells
+        print(json.dumps({"cells": cells, "steps": steps, "seconds": dt, "instances": pick}),
+              flush=True)
+        return
+    pb = _config_problem(name, steps)
+    if variant == "reference":
+        o = oracle.Reference(pb, omp=True)
+    else:
+        o = oracle.Oracle(pb, fma=variant == "fma")
     o.run(2)
     t0 = time.perf_counter()
     o.run(steps)
     dt = time.perf_counter() - t0
     print(json.dumps({"cells": pb.cells, "steps": steps, "seconds": dt}), flush=True)
+
+
+def cpu_table(threads):
+    """CPU throughput per configuration (``--cpu-table``, no GPU needed): the
+    oracle with FMA contraction on and off and, where oracle/_ref_omp was
+    built, the reference's own modules with their OpenMP directives, each in
+    a child process on `threads` threads bound close to cores, on bounded
+    samples: 64^3 cavity 2000 steps (reference 500), 256^3 open scatterer 40
+    steps (reference 10), the heterogeneous workload at 256^3 with a 4-cell
+    CPML 40 steps (reference 10), 8 seeded batched instances 40 steps each
+    (oracle only)."""
+    import subprocess
+    import oracle
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    plan = [("config1_cavity64", 2000, 500), ("config2_open_scatterer256", 40, 10),
+            ("config3_heterogeneous256_sample", 40, 10), ("config4_batch_sample8", 40, 0)]
+    out = {}
+    for name, s1, s2 in plan:
+        row = {}
+        variants = [("fma", s1), ("nofma", s1)]
+        if s2 and os.path.exists(oracle.REF_OMP_SO):
+            variants.append(("reference", s2))
+        for v, ns in variants:
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-config", name,
+                                str(ns), v], env=env, capture_output=True, text=True,
+                               timeout=1800, check=True)
+            res = json.loads(r.stdout.strip().splitlines()[-1])
+            row[v] = round(res["cells"] * res["steps"] / res["seconds"] / 1e6, 2)
+            row[f"{v}_steps"] = ns
+        out[name] = row
+        print(f"[bench] {name}: {json.dumps(row)}", file=sys.stderr, flush=True)
+    return {"threads": threads, "unit": "Mcell-updates/s", "rows": out}
 
 
 def main():
@@ -385,6 +452,11 @@ def main():
         return
     if len(sys.argv) == 5 and sys.argv[1] == "--cpu-config":
         _cpu_config(sys.argv[2], int(sys.argv[3]), sys.argv[4])
+        return
+    if len(sys.argv) >= 2 and sys.argv[1] == "--cpu-table":
+        # CPU only: no GPU is touched
+        threads = int(sys.argv[2]) if len(sys.argv) > 2 else physical_cores()
+        print(json.dumps(cpu_table(threads)), flush=True)
         return
     args = ap.parse_args()
     if args.copy_calibration:
