@@ -58,6 +58,10 @@ constexpr int kPre = SFD_PREFETCH_POLES;
 
 // hardware dispatches block b to XCD b % 8; give each XCD one contiguous
 // range of tiles so that x/y neighbours share that XCD's L2
+// global memory, spelled out for the pointers the compiler cannot infer it
+// for (those read out of memory rather than passed as kernel arguments)
+#define SFD_GLOBAL __attribute__((address_space(1)))
+
 __device__ __forceinline__ int xcd_remap(int b, int nblk) {
   const int q = nblk >> 3, r = nblk & 7;
   const int xcd = b & 7, loc = b >> 3;
@@ -120,6 +124,12 @@ __device__ __forceinline__ float madd(float x, float g, float v) {
 
 // psi <- b*psi + d*c ; returns the new psi
 __device__ __forceinline__ float psi_step(float *psi, long long s, float b,
+                                          float d, float c) {
+  const float v = madd(d * c, b, psi[s]);
+  psi[s] = v;
+  return v;
+}
+__device__ __forceinline__ float psi_step(SFD_GLOBAL float *psi, long long s, float b,
                                           float d, float c) {
   const float v = madd(d * c, b, psi[s]);
   psi[s] = v;
@@ -377,6 +387,12 @@ template <> struct Store<float> {
   }
   static __device__ __forceinline__ float ld1(const float *b, long long p, float) { return b[p]; }
   static __device__ __forceinline__ void st1(float *b, long long p, float, float v) { b[p] = v; }
+  static __device__ __forceinline__ float ld1(const SFD_GLOBAL float *b, long long p, float) {
+    return b[p];
+  }
+  static __device__ __forceinline__ void st1(SFD_GLOBAL float *b, long long p, float, float v) {
+    b[p] = v;
+  }
   static __device__ __forceinline__ void st4(float *b, long long p, float, const F4 &v) {
     *reinterpret_cast<float4 *>(b + p) = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
   }
@@ -403,6 +419,13 @@ template <> struct Store<__half> {
   }
   static __device__ __forceinline__ void st1(float *b, long long p, float is, float v) {
     reinterpret_cast<__half *>(b)[p] = __float2half_rn(rounded(v * is));
+  }
+  static __device__ __forceinline__ float ld1(const SFD_GLOBAL float *b, long long p, float s) {
+    const unsigned short h = ((const SFD_GLOBAL unsigned short *)b)[p];
+    return rounded(__half2float(__ushort_as_half(h)) * s);
+  }
+  static __device__ __forceinline__ void st1(SFD_GLOBAL float *b, long long p, float is, float v) {
+    ((SFD_GLOBAL unsigned short *)b)[p] = __half_as_ushort(__float2half_rn(rounded(v * is)));
   }
   static __device__ __forceinline__ void st4(float *b, long long p, float is, const F4 &v) {
     const __half2 lo = __floats2half2_rn(rounded(v.v[0] * is), rounded(v.v[1] * is));
@@ -436,8 +459,8 @@ __device__ __forceinline__ void tile_column(const TileMap &tm, int blk, int nx, 
 
 // one tile (64 x blockDim.y lanes, one z chunk) of the E face pass; T the
 // field storage (fp32, or binary16 with the arrays' scales)
-template <bool YEE, bool CPML, bool TBL, class T = float>
-__device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
+template <bool YEE, bool CPML, bool TBL, class T = float, class G = DevGrid>
+__device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
                                            const MedTab &tab, int blk) {
   typedef Store<T> S;
   int i, j, bz;
@@ -448,20 +471,20 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
   const int k1 = min(k0 + tm.kc, nz + 1);
   const long long px = g.px, pxy = g.pxy;
   long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + kXOff + i;
-  float *__restrict__ Ex = g.f[0];
-  float *__restrict__ Ey = g.f[1];
-  float *__restrict__ Ez = g.f[2];
-  const float *__restrict__ Hx = g.f[3];
-  const float *__restrict__ Hy = g.f[4];
-  const float *__restrict__ Hz = g.f[5];
+  SFD_GLOBAL float *__restrict__ Ex = (SFD_GLOBAL float *)g.f[0];
+  SFD_GLOBAL float *__restrict__ Ey = (SFD_GLOBAL float *)g.f[1];
+  SFD_GLOBAL float *__restrict__ Ez = (SFD_GLOBAL float *)g.f[2];
+  const SFD_GLOBAL float *__restrict__ Hx = (SFD_GLOBAL float *)g.f[3];
+  const SFD_GLOBAL float *__restrict__ Hy = (SFD_GLOBAL float *)g.f[4];
+  const SFD_GLOBAL float *__restrict__ Hz = (SFD_GLOBAL float *)g.f[5];
   constexpr bool NEEDH = YEE || CPML;
   float hx_km = 0.0f, hy_km = 0.0f;
   if (NEEDH) {
     hx_km = S::ld1(Hx, p - pxy, g.fs[3]);
     hy_km = S::ld1(Hy, p - pxy, g.fs[4]);
   }
-  const float idyh = g.id[4][j + 1];
-  const float idxh = g.id[3][i + 1];
+  const float idyh = ((const SFD_GLOBAL float *)g.id[4])[j + 1];
+  const float idxh = ((const SFD_GLOBAL float *)g.id[3])[i + 1];
   const bool cx = i < nx, cy = j < ny;   // Ex needs i<nx, Ey j<ny, Ez k<nz
   // CPML slab membership along x and y is fixed for the thread
   int qxn = 0, qyn = 0;
@@ -473,7 +496,7 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
   for (int k = k0; k < k1; ++k, p += pxy) {
     const bool cz = k < nz;
     // ---- issue the independent loads of this plane
-    const float idzh = g.id[5][k + 1];
+    const float idzh = ((const SFD_GLOBAL float *)g.id[5])[k + 1];
     float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f, hz_jm = 0.0f, hx_jm = 0.0f, hz_im = 0.0f,
           hy_im = 0.0f;
     if (NEEDH) {
@@ -483,9 +506,9 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
     }
     int mx = 0, my = 0, mz = 0;
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    if (cx) { mx = g.mi[0][p]; ex = S::ld1(Ex, p, g.fs[0]); }
-    if (cy) { my = g.mi[1][p]; ey = S::ld1(Ey, p, g.fs[1]); }
-    if (cz) { mz = g.mi[2][p]; ez = S::ld1(Ez, p, g.fs[2]); }
+    if (cx) { mx = ((const SFD_GLOBAL int16_t *)g.mi[0])[p]; ex = S::ld1(Ex, p, g.fs[0]); }
+    if (cy) { my = ((const SFD_GLOBAL int16_t *)g.mi[1])[p]; ey = S::ld1(Ey, p, g.fs[1]); }
+    if (cz) { mz = ((const SFD_GLOBAL int16_t *)g.mi[2])[p]; ez = S::ld1(Ez, p, g.fs[2]); }
     int qzn = 0;
     const bool szn = CPML && slab_node(k, nz, g.L[2][0], g.L[2][1], qzn);
     // ---- compute
@@ -496,11 +519,11 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
       if (CPML) {
         if (syn) {  // psi_Exy, left/right (borderscpml.F90:609-623)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * nx + i;
-          e = madd(e, c.y, psi_step(g.psi[PSI_EXY], s, g.pbe[1][j + 1], hz0 - hz_jm, g.pce[1][j + 1]));
+          e = madd(e, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EXY], s, ((const SFD_GLOBAL float *)g.pbe[1])[j + 1], hz0 - hz_jm, ((const SFD_GLOBAL float *)g.pce[1])[j + 1]));
         }
         if (szn) {  // psi_Exz, down/up (:699-712)
           const long long s = ((long long)qzn * (ny + 1) + j) * nx + i;
-          e = madd(e, -c.y, psi_step(g.psi[PSI_EXZ], s, g.pbe[2][k + 1], hy0 - hy_km, g.pce[2][k + 1]));
+          e = madd(e, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EXZ], s, ((const SFD_GLOBAL float *)g.pbe[2])[k + 1], hy0 - hy_km, ((const SFD_GLOBAL float *)g.pce[2])[k + 1]));
         }
       }
       S::st1(Ex, p, g.fi[0], e);
@@ -512,11 +535,11 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
       if (CPML) {
         if (szn) {  // psi_Eyz, down/up (:680-693)
           const long long s = ((long long)qzn * ny + j) * (nx + 1) + i;
-          e = madd(e, c.y, psi_step(g.psi[PSI_EYZ], s, g.pbe[2][k + 1], hx0 - hx_km, g.pce[2][k + 1]));
+          e = madd(e, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EYZ], s, ((const SFD_GLOBAL float *)g.pbe[2])[k + 1], hx0 - hx_km, ((const SFD_GLOBAL float *)g.pce[2])[k + 1]));
         }
         if (sxn) {  // psi_Eyx, back/front (:790-802)
           const long long s = ((long long)k * ny + j) * (g.L[0][0] + g.L[0][1]) + qxn;
-          e = madd(e, -c.y, psi_step(g.psi[PSI_EYX], s, g.pbe[0][i + 1], hz0 - hz_im, g.pce[0][i + 1]));
+          e = madd(e, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EYX], s, ((const SFD_GLOBAL float *)g.pbe[0])[i + 1], hz0 - hz_im, ((const SFD_GLOBAL float *)g.pce[0])[i + 1]));
         }
       }
       S::st1(Ey, p, g.fi[1], e);
@@ -528,11 +551,11 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
       if (CPML) {
         if (syn) {  // psi_Ezy, left/right (:630-642)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * (nx + 1) + i;
-          e = madd(e, -c.y, psi_step(g.psi[PSI_EZY], s, g.pbe[1][j + 1], hx0 - hx_jm, g.pce[1][j + 1]));
+          e = madd(e, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EZY], s, ((const SFD_GLOBAL float *)g.pbe[1])[j + 1], hx0 - hx_jm, ((const SFD_GLOBAL float *)g.pce[1])[j + 1]));
         }
         if (sxn) {  // psi_Ezx, back/front (:771-783)
           const long long s = ((long long)k * (ny + 1) + j) * (g.L[0][0] + g.L[0][1]) + qxn;
-          e = madd(e, c.y, psi_step(g.psi[PSI_EZX], s, g.pbe[0][i + 1], hy0 - hy_im, g.pce[0][i + 1]));
+          e = madd(e, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EZX], s, ((const SFD_GLOBAL float *)g.pbe[0])[i + 1], hy0 - hy_im, ((const SFD_GLOBAL float *)g.pce[0])[i + 1]));
         }
       }
       S::st1(Ez, p, g.fi[2], e);
@@ -542,8 +565,8 @@ __device__ __forceinline__ void epass_tile(const DevGrid &g, const TileMap &tm,
 }
 
 // one tile of the H face pass
-template <bool YEE, bool CPML, bool TBL, class T = float>
-__device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
+template <bool YEE, bool CPML, bool TBL, class T = float, class G = DevGrid>
+__device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
                                            const MedTab &tab, int blk) {
   typedef Store<T> S;
   int i, j, bz;
@@ -554,14 +577,14 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
   const int k1 = min(k0 + tm.kc, nz + 1);
   const long long px = g.px, pxy = g.pxy;
   long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + kXOff + i;
-  const float *__restrict__ Ex = g.f[0];
-  const float *__restrict__ Ey = g.f[1];
-  const float *__restrict__ Ez = g.f[2];
-  float *__restrict__ Hx = g.f[3];
-  float *__restrict__ Hy = g.f[4];
-  float *__restrict__ Hz = g.f[5];
-  const float idye = g.id[1][j + 1];
-  const float idxe = g.id[0][i + 1];
+  const SFD_GLOBAL float *__restrict__ Ex = (SFD_GLOBAL float *)g.f[0];
+  const SFD_GLOBAL float *__restrict__ Ey = (SFD_GLOBAL float *)g.f[1];
+  const SFD_GLOBAL float *__restrict__ Ez = (SFD_GLOBAL float *)g.f[2];
+  SFD_GLOBAL float *__restrict__ Hx = (SFD_GLOBAL float *)g.f[3];
+  SFD_GLOBAL float *__restrict__ Hy = (SFD_GLOBAL float *)g.f[4];
+  SFD_GLOBAL float *__restrict__ Hz = (SFD_GLOBAL float *)g.f[5];
+  const float idye = ((const SFD_GLOBAL float *)g.id[1])[j + 1];
+  const float idxe = ((const SFD_GLOBAL float *)g.id[0])[i + 1];
   float ex_c = S::ld1(Ex, p, g.fs[0]), ey_c = S::ld1(Ey, p, g.fs[1]);
   int qxc = 0, qyc = 0;
   bool sxc = false, syc = false;
@@ -570,7 +593,7 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
     syc = slab_cell(j, ny, g.L[1][0], g.L[1][1], qyc);
   }
   for (int k = k0; k < k1; ++k, p += pxy) {
-    const float idze = g.id[2][k + 1];
+    const float idze = ((const SFD_GLOBAL float *)g.id[2])[k + 1];
     const float ex_kp = S::ld1(Ex, p + pxy, g.fs[0]), ey_kp = S::ld1(Ey, p + pxy, g.fs[1]);
     const float ez_c = S::ld1(Ez, p, g.fs[2]);
     const float ez_jp = S::ld1(Ez, p + px, g.fs[2]), ex_jp = S::ld1(Ex, p + px, g.fs[0]);
@@ -578,49 +601,49 @@ __device__ __forceinline__ void hpass_tile(const DevGrid &g, const TileMap &tm,
     int qzc = 0;
     const bool szc = CPML && slab_cell(k, nz, g.L[2][0], g.L[2][1], qzc);
     if (j < ny && k < nz) {  // Hx sweep: x[0,nx] y[0,ny-1] z[0,nz-1]
-      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[3][p]);
+      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, ((const SFD_GLOBAL int16_t *)g.mi[3])[p]);
       float h = S::ld1(Hx, p, g.fs[3]);
       if (YEE) h = yee_upd(c.x, h, c.y, curl2(ey_kp - ey_c, idze, ez_jp - ez_c, idye));
       if (CPML) {
         if (syc) {  // psi_Hxy (borderscpml.F90:911-921)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyc) * (nx + 1) + i;
-          h = madd(h, -c.y, psi_step(g.psi[PSI_HXY], s, g.pbm[1][j + 1], ez_jp - ez_c, g.pcm[1][j + 1]));
+          h = madd(h, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HXY], s, ((const SFD_GLOBAL float *)g.pbm[1])[j + 1], ez_jp - ez_c, ((const SFD_GLOBAL float *)g.pcm[1])[j + 1]));
         }
         if (szc) {  // psi_Hxz (:1021-1031)
           const long long s = ((long long)qzc * ny + j) * (nx + 1) + i;
-          h = madd(h, c.y, psi_step(g.psi[PSI_HXZ], s, g.pbm[2][k + 1], ey_kp - ey_c, g.pcm[2][k + 1]));
+          h = madd(h, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HXZ], s, ((const SFD_GLOBAL float *)g.pbm[2])[k + 1], ey_kp - ey_c, ((const SFD_GLOBAL float *)g.pcm[2])[k + 1]));
         }
       }
       S::st1(Hx, p, g.fi[3], h);
     }
     if (i < nx && k < nz) {  // Hy sweep: x[0,nx-1] y[0,ny] z[0,nz-1]
-      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[4][p]);
+      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, ((const SFD_GLOBAL int16_t *)g.mi[4])[p]);
       float h = S::ld1(Hy, p, g.fs[4]);
       if (YEE) h = yee_upd(c.x, h, c.y, curl2(ez_ip - ez_c, idxe, ex_kp - ex_c, idze));
       if (CPML) {
         if (szc) {  // psi_Hyz (:1001-1011)
           const long long s = ((long long)qzc * (ny + 1) + j) * nx + i;
-          h = madd(h, -c.y, psi_step(g.psi[PSI_HYZ], s, g.pbm[2][k + 1], ex_kp - ex_c, g.pcm[2][k + 1]));
+          h = madd(h, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HYZ], s, ((const SFD_GLOBAL float *)g.pbm[2])[k + 1], ex_kp - ex_c, ((const SFD_GLOBAL float *)g.pcm[2])[k + 1]));
         }
         if (sxc) {  // psi_Hyx (:1111-1121)
           const long long s = ((long long)k * (ny + 1) + j) * (g.L[0][0] + g.L[0][1]) + qxc;
-          h = madd(h, c.y, psi_step(g.psi[PSI_HYX], s, g.pbm[0][i + 1], ez_ip - ez_c, g.pcm[0][i + 1]));
+          h = madd(h, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HYX], s, ((const SFD_GLOBAL float *)g.pbm[0])[i + 1], ez_ip - ez_c, ((const SFD_GLOBAL float *)g.pcm[0])[i + 1]));
         }
       }
       S::st1(Hy, p, g.fi[4], h);
     }
     if (i < nx && j < ny) {  // Hz sweep: x[0,nx-1] y[0,ny-1] z[0,nz]
-      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, g.mi[5][p]);
+      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, ((const SFD_GLOBAL int16_t *)g.mi[5])[p]);
       float h = S::ld1(Hz, p, g.fs[5]);
       if (YEE) h = yee_upd(c.x, h, c.y, curl2(ex_jp - ex_c, idye, ey_ip - ey_c, idxe));
       if (CPML) {
         if (syc) {  // psi_Hzy (:931-941)
           const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyc) * nx + i;
-          h = madd(h, c.y, psi_step(g.psi[PSI_HZY], s, g.pbm[1][j + 1], ex_jp - ex_c, g.pcm[1][j + 1]));
+          h = madd(h, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HZY], s, ((const SFD_GLOBAL float *)g.pbm[1])[j + 1], ex_jp - ex_c, ((const SFD_GLOBAL float *)g.pcm[1])[j + 1]));
         }
         if (sxc) {  // psi_Hzx (:1091-1101)
           const long long s = ((long long)k * ny + j) * (g.L[0][0] + g.L[0][1]) + qxc;
-          h = madd(h, -c.y, psi_step(g.psi[PSI_HZX], s, g.pbm[0][i + 1], ey_ip - ey_c, g.pcm[0][i + 1]));
+          h = madd(h, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HZX], s, ((const SFD_GLOBAL float *)g.pbm[0])[i + 1], ey_ip - ey_c, ((const SFD_GLOBAL float *)g.pcm[0])[i + 1]));
         }
       }
       S::st1(Hz, p, g.fi[5], h);
@@ -938,9 +961,14 @@ template <bool H, class T, int VEC>
 __global__ void __launch_bounds__(256) k_pass_batch(BatchDev bd) {
   extern __shared__ float2 lds_tab[];
   for (int t = blockIdx.x; t < bd.total; t += gridDim.x) {
-    const int b = batch_grid(bd.start, bd.ngrid, t);
+    const int b = __builtin_amdgcn_readfirstlane(batch_grid(bd.start, bd.ngrid, t));
     const DevGrid &g = bd.grids[b];
-    const TileMap &tm = bd.tms[b];
+    // the one-cell passes read the grid's descriptor through the constant
+    // address space: scalar loads into scalar registers (it never changes
+    // during the launch), not vector loads into some 40 vector registers
+    typedef __attribute__((address_space(4))) DevGrid CDevGrid;
+    const CDevGrid &cg = *(const CDevGrid *)(bd.grids + b);
+    const TileMap tm = *(const __attribute__((address_space(4))) TileMap *)(bd.tms + b);
     __syncthreads();  // the previous tile is done with the LDS tables
     const MedTab tab = H ? stage_tables<true, false>(g, g.gm1, g.gm2, lds_tab)
                          : stage_tables<true, false>(g, g.g1, g.g2, lds_tab);
@@ -951,8 +979,8 @@ __global__ void __launch_bounds__(256) k_pass_batch(BatchDev bd) {
       if (H) hpass4_tile<true, T>(g, tm, tab, blk);
       else epass4_tile<true, T>(g, tm, tab, blk);
     } else {
-      if (H) hpass_tile<true, true, true, T>(g, tm, tab, blk);
-      else epass_tile<true, true, true, T>(g, tm, tab, blk);
+      if (H) hpass_tile<true, true, true, T, CDevGrid>(cg, tm, tab, blk);
+      else epass_tile<true, true, true, T, CDevGrid>(cg, tm, tab, blk);
     }
   }
 }
@@ -1104,7 +1132,6 @@ int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
 // the compiler, which otherwise hoists a 64-bit (array base + offset) pair per
 // array and lane out of the march; it stays in the global address space so
 // that the accesses are global, not flat, loads and stores.
-#define SFD_GLOBAL __attribute__((address_space(1)))
 template <class T>
 __device__ __forceinline__ SFD_GLOBAL T *plane_base(T *base, long long P) {
   SFD_GLOBAL T *q = (SFD_GLOBAL T *)(base + P);
