@@ -63,6 +63,7 @@ struct fdtd_handle {
   sfd::DevGrid g{};
   sfd::TileMap tm{};           // 64-cell x tiles (granular passes)
   sfd::TileMap tm4{};          // 256-cell x tiles (fused passes, 4 cells per lane)
+  int tile_zchunk = 32;        // planes per z chunk as set (tm.kc: clipped to the grid)
   std::vector<void *> allocs;
   int64_t bytes = 0;
   float *fld[6] = {};

@@ -167,6 +167,7 @@ int check_ready(fdtd_handle *h) {
 // blocks of 64 x rows threads march z in chunks of zchunk planes
 void set_tiles(fdtd_handle *h, int rows, int zchunk, int xcells) {
   const DevGrid &g = h->g;
+  h->tile_zchunk = zchunk;
   for (int v = 0; v < 2; ++v) {
     TileMap &tm = v ? h->tm4 : h->tm;
     const int w = v ? 4 * kWave : kWave;
@@ -1464,7 +1465,7 @@ int fdtd_set_tiling(fdtd_handle *h, int32_t rows, int32_t zchunk, int32_t xcells
 int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk, int32_t *xcells) {
   if (!h || !rows || !zchunk || !xcells) return FDTD_ERR_ARG;
   *rows = h->tm.bdy;
-  *zchunk = h->tm.kc;
+  *zchunk = h->tile_zchunk;
   *xcells = h->g.fp16 ? 4 : h->tm4.vec;
   return FDTD_OK;
 }

@@ -458,11 +458,15 @@ __device__ __forceinline__ void tile_column(const TileMap &tm, int blk, int nx, 
 }
 
 // one tile (64 x blockDim.y lanes, one z chunk) of the E face pass; T the
-// field storage (fp32, or binary16 with the arrays' scales)
+// field storage (fp32, or binary16 with the arrays' scales).  Each plane
+// issues every load it needs (fields, media, the psi values and z profile of
+// the CPML slabs) before any arithmetic, so a plane costs one memory round
+// trip; the x and y profile values are fixed for the thread.
 template <bool YEE, bool CPML, bool TBL, class T = float, class G = DevGrid>
 __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
                                            const MedTab &tab, int blk) {
   typedef Store<T> S;
+  typedef const SFD_GLOBAL float *cgf;
   int i, j, bz;
   tile_column(tm, blk, g.nx, i, j, bz);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -483,20 +487,30 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
     hx_km = S::ld1(Hx, p - pxy, g.fs[3]);
     hy_km = S::ld1(Hy, p - pxy, g.fs[4]);
   }
-  const float idyh = ((const SFD_GLOBAL float *)g.id[4])[j + 1];
-  const float idxh = ((const SFD_GLOBAL float *)g.id[3])[i + 1];
+  const float idyh = ((cgf)g.id[4])[j + 1];
+  const float idxh = ((cgf)g.id[3])[i + 1];
   const bool cx = i < nx, cy = j < ny;   // Ex needs i<nx, Ey j<ny, Ez k<nz
-  // CPML slab membership along x and y is fixed for the thread
+  // CPML slab membership and profiles along x and y are fixed for the thread
   int qxn = 0, qyn = 0;
   bool sxn = false, syn = false;
+  float bx = 0.0f, ccx = 0.0f, by = 0.0f, ccy = 0.0f;
   if (CPML) {
     sxn = slab_node(i, nx, g.L[0][0], g.L[0][1], qxn);
     syn = slab_node(j, ny, g.L[1][0], g.L[1][1], qyn);
+    if (sxn) { bx = ((cgf)g.pbe[0])[i + 1]; ccx = ((cgf)g.pce[0])[i + 1]; }
+    if (syn) { by = ((cgf)g.pbe[1])[j + 1]; ccy = ((cgf)g.pce[1])[j + 1]; }
   }
+  const int Lx = g.L[0][0] + g.L[0][1], Ly = g.L[1][0] + g.L[1][1];
+  SFD_GLOBAL float *const pExy = (SFD_GLOBAL float *)g.psi[PSI_EXY];
+  SFD_GLOBAL float *const pExz = (SFD_GLOBAL float *)g.psi[PSI_EXZ];
+  SFD_GLOBAL float *const pEyz = (SFD_GLOBAL float *)g.psi[PSI_EYZ];
+  SFD_GLOBAL float *const pEyx = (SFD_GLOBAL float *)g.psi[PSI_EYX];
+  SFD_GLOBAL float *const pEzy = (SFD_GLOBAL float *)g.psi[PSI_EZY];
+  SFD_GLOBAL float *const pEzx = (SFD_GLOBAL float *)g.psi[PSI_EZX];
   for (int k = k0; k < k1; ++k, p += pxy) {
     const bool cz = k < nz;
     // ---- issue the independent loads of this plane
-    const float idzh = ((const SFD_GLOBAL float *)g.id[5])[k + 1];
+    const float idzh = ((cgf)g.id[5])[k + 1];
     float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f, hz_jm = 0.0f, hx_jm = 0.0f, hz_im = 0.0f,
           hy_im = 0.0f;
     if (NEEDH) {
@@ -511,19 +525,39 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
     if (cz) { mz = ((const SFD_GLOBAL int16_t *)g.mi[2])[p]; ez = S::ld1(Ez, p, g.fs[2]); }
     int qzn = 0;
     const bool szn = CPML && slab_node(k, nz, g.L[2][0], g.L[2][1], qzn);
-    // ---- compute
+    // psi cells of this plane (borderscpml.F90 slab layouts) and their state
+    const long long sxy = ((long long)k * Ly + qyn) * nx + i;          // psi_Exy
+    const long long sxz = ((long long)qzn * (ny + 1) + j) * nx + i;    // psi_Exz
+    const long long syz = ((long long)qzn * ny + j) * (nx + 1) + i;    // psi_Eyz
+    const long long syx = ((long long)k * ny + j) * Lx + qxn;          // psi_Eyx
+    const long long szy = ((long long)k * Ly + qyn) * (nx + 1) + i;    // psi_Ezy
+    const long long szx = ((long long)k * (ny + 1) + j) * Lx + qxn;    // psi_Ezx
+    float bzp = 0.0f, ccz = 0.0f, vxy = 0.0f, vxz = 0.0f, vyz = 0.0f, vyx = 0.0f, vzy = 0.0f,
+          vzx = 0.0f;
+    if (CPML) {
+      if (szn) { bzp = ((cgf)g.pbe[2])[k + 1]; ccz = ((cgf)g.pce[2])[k + 1]; }
+      if (cx && syn) vxy = pExy[sxy];
+      if (cx && szn) vxz = pExz[sxz];
+      if (cy && szn) vyz = pEyz[syz];
+      if (cy && sxn) vyx = pEyx[syx];
+      if (cz && syn) vzy = pEzy[szy];
+      if (cz && sxn) vzx = pEzx[szx];
+    }
+    // ---- compute; psi <- b*psi + c*curl, then E += +-G2*psi
     if (cx) {
       float e = ex;
       const float2 c = coef<TBL>(tab, g.g1, g.g2, mx);
       if (YEE) e = yee_upd(c.x, e, c.y, curl2(hz0 - hz_jm, idyh, hy0 - hy_km, idzh));
       if (CPML) {
         if (syn) {  // psi_Exy, left/right (borderscpml.F90:609-623)
-          const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * nx + i;
-          e = madd(e, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EXY], s, ((const SFD_GLOBAL float *)g.pbe[1])[j + 1], hz0 - hz_jm, ((const SFD_GLOBAL float *)g.pce[1])[j + 1]));
+          const float v = madd((hz0 - hz_jm) * ccy, by, vxy);
+          pExy[sxy] = v;
+          e = madd(e, c.y, v);
         }
         if (szn) {  // psi_Exz, down/up (:699-712)
-          const long long s = ((long long)qzn * (ny + 1) + j) * nx + i;
-          e = madd(e, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EXZ], s, ((const SFD_GLOBAL float *)g.pbe[2])[k + 1], hy0 - hy_km, ((const SFD_GLOBAL float *)g.pce[2])[k + 1]));
+          const float v = madd((hy0 - hy_km) * ccz, bzp, vxz);
+          pExz[sxz] = v;
+          e = madd(e, -c.y, v);
         }
       }
       S::st1(Ex, p, g.fi[0], e);
@@ -534,12 +568,14 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
       if (YEE) e = yee_upd(c.x, e, c.y, curl2(hx0 - hx_km, idzh, hz0 - hz_im, idxh));
       if (CPML) {
         if (szn) {  // psi_Eyz, down/up (:680-693)
-          const long long s = ((long long)qzn * ny + j) * (nx + 1) + i;
-          e = madd(e, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EYZ], s, ((const SFD_GLOBAL float *)g.pbe[2])[k + 1], hx0 - hx_km, ((const SFD_GLOBAL float *)g.pce[2])[k + 1]));
+          const float v = madd((hx0 - hx_km) * ccz, bzp, vyz);
+          pEyz[syz] = v;
+          e = madd(e, c.y, v);
         }
         if (sxn) {  // psi_Eyx, back/front (:790-802)
-          const long long s = ((long long)k * ny + j) * (g.L[0][0] + g.L[0][1]) + qxn;
-          e = madd(e, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EYX], s, ((const SFD_GLOBAL float *)g.pbe[0])[i + 1], hz0 - hz_im, ((const SFD_GLOBAL float *)g.pce[0])[i + 1]));
+          const float v = madd((hz0 - hz_im) * ccx, bx, vyx);
+          pEyx[syx] = v;
+          e = madd(e, -c.y, v);
         }
       }
       S::st1(Ey, p, g.fi[1], e);
@@ -550,12 +586,14 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
       if (YEE) e = yee_upd(c.x, e, c.y, curl2(hy0 - hy_im, idxh, hx0 - hx_jm, idyh));
       if (CPML) {
         if (syn) {  // psi_Ezy, left/right (:630-642)
-          const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyn) * (nx + 1) + i;
-          e = madd(e, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EZY], s, ((const SFD_GLOBAL float *)g.pbe[1])[j + 1], hx0 - hx_jm, ((const SFD_GLOBAL float *)g.pce[1])[j + 1]));
+          const float v = madd((hx0 - hx_jm) * ccy, by, vzy);
+          pEzy[szy] = v;
+          e = madd(e, -c.y, v);
         }
         if (sxn) {  // psi_Ezx, back/front (:771-783)
-          const long long s = ((long long)k * (ny + 1) + j) * (g.L[0][0] + g.L[0][1]) + qxn;
-          e = madd(e, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_EZX], s, ((const SFD_GLOBAL float *)g.pbe[0])[i + 1], hy0 - hy_im, ((const SFD_GLOBAL float *)g.pce[0])[i + 1]));
+          const float v = madd((hy0 - hy_im) * ccx, bx, vzx);
+          pEzx[szx] = v;
+          e = madd(e, c.y, v);
         }
       }
       S::st1(Ez, p, g.fi[2], e);
@@ -564,11 +602,12 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
   }
 }
 
-// one tile of the H face pass
+// one tile of the H face pass (loads issued per plane as in the E pass)
 template <bool YEE, bool CPML, bool TBL, class T = float, class G = DevGrid>
 __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
                                            const MedTab &tab, int blk) {
   typedef Store<T> S;
+  typedef const SFD_GLOBAL float *cgf;
   int i, j, bz;
   tile_column(tm, blk, g.nx, i, j, bz);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -583,67 +622,111 @@ __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
   SFD_GLOBAL float *__restrict__ Hx = (SFD_GLOBAL float *)g.f[3];
   SFD_GLOBAL float *__restrict__ Hy = (SFD_GLOBAL float *)g.f[4];
   SFD_GLOBAL float *__restrict__ Hz = (SFD_GLOBAL float *)g.f[5];
-  const float idye = ((const SFD_GLOBAL float *)g.id[1])[j + 1];
-  const float idxe = ((const SFD_GLOBAL float *)g.id[0])[i + 1];
+  const float idye = ((cgf)g.id[1])[j + 1];
+  const float idxe = ((cgf)g.id[0])[i + 1];
   float ex_c = S::ld1(Ex, p, g.fs[0]), ey_c = S::ld1(Ey, p, g.fs[1]);
   int qxc = 0, qyc = 0;
   bool sxc = false, syc = false;
+  float bx = 0.0f, ccx = 0.0f, by = 0.0f, ccy = 0.0f;
   if (CPML) {
-    sxc = slab_cell(i, nx, g.L[0][0], g.L[0][1], qxc);
-    syc = slab_cell(j, ny, g.L[1][0], g.L[1][1], qyc);
+    // slab_cell also answers true at c = n on an axis without layers (whose
+    // profiles are then absent): the x cells count for Hy and Hz only (i < nx),
+    // the y cells for Hx and Hz (j < ny), the z cells for Hx and Hy (k < nz)
+    sxc = i < nx && slab_cell(i, nx, g.L[0][0], g.L[0][1], qxc);
+    syc = j < ny && slab_cell(j, ny, g.L[1][0], g.L[1][1], qyc);
+    if (sxc) { bx = ((cgf)g.pbm[0])[i + 1]; ccx = ((cgf)g.pcm[0])[i + 1]; }
+    if (syc) { by = ((cgf)g.pbm[1])[j + 1]; ccy = ((cgf)g.pcm[1])[j + 1]; }
   }
+  const int Lx = g.L[0][0] + g.L[0][1], Ly = g.L[1][0] + g.L[1][1];
+  SFD_GLOBAL float *const pHxy = (SFD_GLOBAL float *)g.psi[PSI_HXY];
+  SFD_GLOBAL float *const pHxz = (SFD_GLOBAL float *)g.psi[PSI_HXZ];
+  SFD_GLOBAL float *const pHyz = (SFD_GLOBAL float *)g.psi[PSI_HYZ];
+  SFD_GLOBAL float *const pHyx = (SFD_GLOBAL float *)g.psi[PSI_HYX];
+  SFD_GLOBAL float *const pHzy = (SFD_GLOBAL float *)g.psi[PSI_HZY];
+  SFD_GLOBAL float *const pHzx = (SFD_GLOBAL float *)g.psi[PSI_HZX];
   for (int k = k0; k < k1; ++k, p += pxy) {
-    const float idze = ((const SFD_GLOBAL float *)g.id[2])[k + 1];
+    const bool ax = j < ny && k < nz;   // Hx sweep: x[0,nx] y[0,ny-1] z[0,nz-1]
+    const bool ay = i < nx && k < nz;   // Hy sweep: x[0,nx-1] y[0,ny] z[0,nz-1]
+    const bool az = i < nx && j < ny;   // Hz sweep: x[0,nx-1] y[0,ny-1] z[0,nz]
+    const float idze = ((cgf)g.id[2])[k + 1];
     const float ex_kp = S::ld1(Ex, p + pxy, g.fs[0]), ey_kp = S::ld1(Ey, p + pxy, g.fs[1]);
     const float ez_c = S::ld1(Ez, p, g.fs[2]);
     const float ez_jp = S::ld1(Ez, p + px, g.fs[2]), ex_jp = S::ld1(Ex, p + px, g.fs[0]);
     const float ez_ip = S::ld1(Ez, p + 1, g.fs[2]), ey_ip = S::ld1(Ey, p + 1, g.fs[1]);
+    int mx = 0, my = 0, mz = 0;
+    float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+    if (ax) { mx = ((const SFD_GLOBAL int16_t *)g.mi[3])[p]; hx = S::ld1(Hx, p, g.fs[3]); }
+    if (ay) { my = ((const SFD_GLOBAL int16_t *)g.mi[4])[p]; hy = S::ld1(Hy, p, g.fs[4]); }
+    if (az) { mz = ((const SFD_GLOBAL int16_t *)g.mi[5])[p]; hz = S::ld1(Hz, p, g.fs[5]); }
     int qzc = 0;
-    const bool szc = CPML && slab_cell(k, nz, g.L[2][0], g.L[2][1], qzc);
-    if (j < ny && k < nz) {  // Hx sweep: x[0,nx] y[0,ny-1] z[0,nz-1]
-      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, ((const SFD_GLOBAL int16_t *)g.mi[3])[p]);
-      float h = S::ld1(Hx, p, g.fs[3]);
+    const bool szc = CPML && k < nz && slab_cell(k, nz, g.L[2][0], g.L[2][1], qzc);
+    const long long sxy = ((long long)k * Ly + qyc) * (nx + 1) + i;    // psi_Hxy
+    const long long sxz = ((long long)qzc * ny + j) * (nx + 1) + i;    // psi_Hxz
+    const long long syz = ((long long)qzc * (ny + 1) + j) * nx + i;    // psi_Hyz
+    const long long syx = ((long long)k * (ny + 1) + j) * Lx + qxc;    // psi_Hyx
+    const long long szy = ((long long)k * Ly + qyc) * nx + i;          // psi_Hzy
+    const long long szx = ((long long)k * ny + j) * Lx + qxc;          // psi_Hzx
+    float bzp = 0.0f, ccz = 0.0f, vxy = 0.0f, vxz = 0.0f, vyz = 0.0f, vyx = 0.0f, vzy = 0.0f,
+          vzx = 0.0f;
+    if (CPML) {
+      if (szc) { bzp = ((cgf)g.pbm[2])[k + 1]; ccz = ((cgf)g.pcm[2])[k + 1]; }
+      if (ax && syc) vxy = pHxy[sxy];
+      if (ax && szc) vxz = pHxz[sxz];
+      if (ay && szc) vyz = pHyz[syz];
+      if (ay && sxc) vyx = pHyx[syx];
+      if (az && syc) vzy = pHzy[szy];
+      if (az && sxc) vzx = pHzx[szx];
+    }
+    if (ax) {
+      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, mx);
+      float h = hx;
       if (YEE) h = yee_upd(c.x, h, c.y, curl2(ey_kp - ey_c, idze, ez_jp - ez_c, idye));
       if (CPML) {
         if (syc) {  // psi_Hxy (borderscpml.F90:911-921)
-          const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyc) * (nx + 1) + i;
-          h = madd(h, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HXY], s, ((const SFD_GLOBAL float *)g.pbm[1])[j + 1], ez_jp - ez_c, ((const SFD_GLOBAL float *)g.pcm[1])[j + 1]));
+          const float v = madd((ez_jp - ez_c) * ccy, by, vxy);
+          pHxy[sxy] = v;
+          h = madd(h, -c.y, v);
         }
         if (szc) {  // psi_Hxz (:1021-1031)
-          const long long s = ((long long)qzc * ny + j) * (nx + 1) + i;
-          h = madd(h, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HXZ], s, ((const SFD_GLOBAL float *)g.pbm[2])[k + 1], ey_kp - ey_c, ((const SFD_GLOBAL float *)g.pcm[2])[k + 1]));
+          const float v = madd((ey_kp - ey_c) * ccz, bzp, vxz);
+          pHxz[sxz] = v;
+          h = madd(h, c.y, v);
         }
       }
       S::st1(Hx, p, g.fi[3], h);
     }
-    if (i < nx && k < nz) {  // Hy sweep: x[0,nx-1] y[0,ny] z[0,nz-1]
-      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, ((const SFD_GLOBAL int16_t *)g.mi[4])[p]);
-      float h = S::ld1(Hy, p, g.fs[4]);
+    if (ay) {
+      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, my);
+      float h = hy;
       if (YEE) h = yee_upd(c.x, h, c.y, curl2(ez_ip - ez_c, idxe, ex_kp - ex_c, idze));
       if (CPML) {
         if (szc) {  // psi_Hyz (:1001-1011)
-          const long long s = ((long long)qzc * (ny + 1) + j) * nx + i;
-          h = madd(h, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HYZ], s, ((const SFD_GLOBAL float *)g.pbm[2])[k + 1], ex_kp - ex_c, ((const SFD_GLOBAL float *)g.pcm[2])[k + 1]));
+          const float v = madd((ex_kp - ex_c) * ccz, bzp, vyz);
+          pHyz[syz] = v;
+          h = madd(h, -c.y, v);
         }
         if (sxc) {  // psi_Hyx (:1111-1121)
-          const long long s = ((long long)k * (ny + 1) + j) * (g.L[0][0] + g.L[0][1]) + qxc;
-          h = madd(h, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HYX], s, ((const SFD_GLOBAL float *)g.pbm[0])[i + 1], ez_ip - ez_c, ((const SFD_GLOBAL float *)g.pcm[0])[i + 1]));
+          const float v = madd((ez_ip - ez_c) * ccx, bx, vyx);
+          pHyx[syx] = v;
+          h = madd(h, c.y, v);
         }
       }
       S::st1(Hy, p, g.fi[4], h);
     }
-    if (i < nx && j < ny) {  // Hz sweep: x[0,nx-1] y[0,ny-1] z[0,nz]
-      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, ((const SFD_GLOBAL int16_t *)g.mi[5])[p]);
-      float h = S::ld1(Hz, p, g.fs[5]);
+    if (az) {
+      const float2 c = coef<TBL>(tab, g.gm1, g.gm2, mz);
+      float h = hz;
       if (YEE) h = yee_upd(c.x, h, c.y, curl2(ex_jp - ex_c, idye, ey_ip - ey_c, idxe));
       if (CPML) {
         if (syc) {  // psi_Hzy (:931-941)
-          const long long s = ((long long)k * (g.L[1][0] + g.L[1][1]) + qyc) * nx + i;
-          h = madd(h, c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HZY], s, ((const SFD_GLOBAL float *)g.pbm[1])[j + 1], ex_jp - ex_c, ((const SFD_GLOBAL float *)g.pcm[1])[j + 1]));
+          const float v = madd((ex_jp - ex_c) * ccy, by, vzy);
+          pHzy[szy] = v;
+          h = madd(h, c.y, v);
         }
         if (sxc) {  // psi_Hzx (:1091-1101)
-          const long long s = ((long long)k * ny + j) * (g.L[0][0] + g.L[0][1]) + qxc;
-          h = madd(h, -c.y, psi_step((SFD_GLOBAL float *)g.psi[PSI_HZX], s, ((const SFD_GLOBAL float *)g.pbm[0])[i + 1], ey_ip - ey_c, ((const SFD_GLOBAL float *)g.pcm[0])[i + 1]));
+          const float v = madd((ey_ip - ey_c) * ccx, bx, vzx);
+          pHzx[szx] = v;
+          h = madd(h, -c.y, v);
         }
       }
       S::st1(Hz, p, g.fi[5], h);
@@ -948,6 +1031,17 @@ __global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm) {
 // batch: block b takes tiles b, b + gridDim.x, ...; the grid of a tile is
 // found by binary search in the prefix of tile counts, its media tables are
 // staged in LDS before the tile runs.
+// a tile map read word by word through the constant address space (scalar
+// loads; the maps never change during a launch)
+__device__ __forceinline__ TileMap const_tilemap(const TileMap *t) {
+  static_assert(sizeof(TileMap) % sizeof(int) == 0, "TileMap holds ints only");
+  typedef const __attribute__((address_space(4))) int CInt;
+  TileMap m;
+  int *o = reinterpret_cast<int *>(&m);
+  for (int q = 0; q < (int)(sizeof(TileMap) / sizeof(int)); ++q) o[q] = ((CInt *)t)[q];
+  return m;
+}
+
 __device__ __forceinline__ int batch_grid(const int *start, int ngrid, int t) {
   int lo = 0, hi = ngrid - 1;
   while (lo < hi) {
@@ -968,7 +1062,7 @@ __global__ void __launch_bounds__(256) k_pass_batch(BatchDev bd) {
     // during the launch), not vector loads into some 40 vector registers
     typedef __attribute__((address_space(4))) DevGrid CDevGrid;
     const CDevGrid &cg = *(const CDevGrid *)(bd.grids + b);
-    const TileMap tm = *(const __attribute__((address_space(4))) TileMap *)(bd.tms + b);
+    const TileMap tm = const_tilemap(bd.tms + b);
     __syncthreads();  // the previous tile is done with the LDS tables
     const MedTab tab = H ? stage_tables<true, false>(g, g.gm1, g.gm2, lds_tab)
                          : stage_tables<true, false>(g, g.g1, g.g2, lds_tab);
