@@ -372,6 +372,35 @@ __global__ void __launch_bounds__(256) k_ade_many(const DevGrid *grids, const in
   ade_box_run<TBL, SIDE>(grids[list[blockIdx.z]]);
 }
 
+// Loads and stores of the march address a plane through a base pointer that
+// is the same for every lane (scalar registers) plus the lane's byte offset
+// inside the plane (one 32-bit register).  The plane base is made opaque to
+// the compiler, which otherwise hoists a 64-bit (array base + offset) pair per
+// array and lane out of the march; it stays in the global address space so
+// that the accesses are global, not flat, loads and stores.
+template <class T>
+__device__ __forceinline__ SFD_GLOBAL T *plane_base(T *base, long long P) {
+  SFD_GLOBAL T *q = (SFD_GLOBAL T *)(base + P);
+  asm volatile("" : "+s"(q));
+  return q;
+}
+template <class T>
+__device__ __forceinline__ T ld_at(const SFD_GLOBAL T *base, unsigned ob) {
+  return *reinterpret_cast<const SFD_GLOBAL T *>(reinterpret_cast<const SFD_GLOBAL char *>(base) +
+                                                 ob);
+}
+template <class T>
+__device__ __forceinline__ void st_at(SFD_GLOBAL T *base, unsigned ob, T v) {
+  *reinterpret_cast<SFD_GLOBAL T *>(reinterpret_cast<SFD_GLOBAL char *>(base) + ob) = v;
+}
+
+// an element of a table the kernel never writes, at an index the same for
+// every lane: read through the constant address space, i.e. by a scalar load
+// that does not join the vector-memory counter shared with the stores
+__device__ __forceinline__ float ro_uniform(const float *t, int idx) {
+  return ((const __attribute__((address_space(4))) float *)t)[idx];
+}
+
 // Field storage of the face passes: each field array holds fp32 values, or
 // binary16 values with a per-array scale.
 struct F4 {
@@ -395,6 +424,13 @@ template <> struct Store<float> {
   }
   static __device__ __forceinline__ void st4(float *b, long long p, float, const F4 &v) {
     *reinterpret_cast<float4 *>(b + p) = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+  }
+  // a plane base and the lane's byte offset in the plane
+  static __device__ __forceinline__ float ldo(const SFD_GLOBAL char *b, unsigned ob, float) {
+    return ld_at((const SFD_GLOBAL float *)b, ob);
+  }
+  static __device__ __forceinline__ void sto(SFD_GLOBAL char *b, unsigned ob, float, float v) {
+    st_at((SFD_GLOBAL float *)b, ob, v);
   }
 };
 // binary16 fields with a per-array scale: value = half * s, stored as
@@ -435,6 +471,13 @@ template <> struct Store<__half> {
     t.y = *reinterpret_cast<const unsigned *>(&hi);
     *reinterpret_cast<uint2 *>(reinterpret_cast<__half *>(b) + p) = t;
   }
+  static __device__ __forceinline__ float ldo(const SFD_GLOBAL char *b, unsigned ob, float s) {
+    const unsigned short h = ld_at((const SFD_GLOBAL unsigned short *)b, ob);
+    return rounded(__half2float(__ushort_as_half(h)) * s);
+  }
+  static __device__ __forceinline__ void sto(SFD_GLOBAL char *b, unsigned ob, float is, float v) {
+    st_at((SFD_GLOBAL unsigned short *)b, ob, __half_as_ushort(__float2half_rn(rounded(v * is))));
+  }
 };
 
 // The column (i, j) of the tile's lane and the tile's z chunk bz.  Tiles are
@@ -458,39 +501,42 @@ __device__ __forceinline__ void tile_column(const TileMap &tm, int blk, int nx, 
 }
 
 // one tile (64 x blockDim.y lanes, one z chunk) of the E face pass; T the
-// field storage (fp32, or binary16 with the arrays' scales).  Each plane
-// issues every load it needs (fields, media, the psi values and z profile of
-// the CPML slabs) before any arithmetic, so a plane costs one memory round
-// trip; the x and y profile values are fixed for the thread.
+// field storage (fp32, or binary16 with the arrays' scales).  A lane
+// addresses each z plane by a base the same for every lane (scalar
+// registers) plus its byte offset in the plane, fixed over the march; the psi
+// slabs likewise.  Each plane issues every load it needs (fields, media, the
+// psi values of the CPML slabs) before any arithmetic, so a plane costs one
+// memory round trip; the per-plane spacing and z profile are scalar loads,
+// the x and y profiles fixed for the lane.
 template <bool YEE, bool CPML, bool TBL, class T = float, class G = DevGrid>
 __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
                                            const MedTab &tab, int blk) {
   typedef Store<T> S;
   typedef const SFD_GLOBAL float *cgf;
+  constexpr unsigned ES = sizeof(T);
   int i, j, bz;
   tile_column(tm, blk, g.nx, i, j, bz);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny) return;
   const int k0 = bz * tm.kc;
   const int k1 = min(k0 + tm.kc, nz + 1);
-  const long long px = g.px, pxy = g.pxy;
-  long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + kXOff + i;
-  SFD_GLOBAL float *__restrict__ Ex = (SFD_GLOBAL float *)g.f[0];
-  SFD_GLOBAL float *__restrict__ Ey = (SFD_GLOBAL float *)g.f[1];
-  SFD_GLOBAL float *__restrict__ Ez = (SFD_GLOBAL float *)g.f[2];
-  const SFD_GLOBAL float *__restrict__ Hx = (SFD_GLOBAL float *)g.f[3];
-  const SFD_GLOBAL float *__restrict__ Hy = (SFD_GLOBAL float *)g.f[4];
-  const SFD_GLOBAL float *__restrict__ Hz = (SFD_GLOBAL float *)g.f[5];
+  const long long fpb = g.pxy * ES, mpb = g.pxy * 2;   // plane strides, bytes
+  const unsigned lo = (unsigned)((j + 1) * (int)g.px + kXOff + i);
+  const unsigned of = lo * ES, om = lo * 2u, fy = (unsigned)g.px * ES;
+  char *const bE[3] = {(char *)g.f[0], (char *)g.f[1], (char *)g.f[2]};
+  char *const bH[3] = {(char *)g.f[3], (char *)g.f[4], (char *)g.f[5]};
+  char *const bM[3] = {(char *)g.mi[0], (char *)g.mi[1], (char *)g.mi[2]};
   constexpr bool NEEDH = YEE || CPML;
   float hx_km = 0.0f, hy_km = 0.0f;
-  if (NEEDH) {
-    hx_km = S::ld1(Hx, p - pxy, g.fs[3]);
-    hy_km = S::ld1(Hy, p - pxy, g.fs[4]);
+  if (NEEDH) {   // plane k0 - 1 of Hx, Hy
+    const long long kb = (long long)k0 * fpb;
+    hx_km = S::ldo(plane_base(bH[0], kb), of, g.fs[3]);
+    hy_km = S::ldo(plane_base(bH[1], kb), of, g.fs[4]);
   }
   const float idyh = ((cgf)g.id[4])[j + 1];
   const float idxh = ((cgf)g.id[3])[i + 1];
   const bool cx = i < nx, cy = j < ny;   // Ex needs i<nx, Ey j<ny, Ez k<nz
-  // CPML slab membership and profiles along x and y are fixed for the thread
+  // CPML slab membership and profiles along x and y are fixed for the lane
   int qxn = 0, qyn = 0;
   bool sxn = false, syn = false;
   float bx = 0.0f, ccx = 0.0f, by = 0.0f, ccy = 0.0f;
@@ -501,47 +547,67 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
     if (syn) { by = ((cgf)g.pbe[1])[j + 1]; ccy = ((cgf)g.pce[1])[j + 1]; }
   }
   const int Lx = g.L[0][0] + g.L[0][1], Ly = g.L[1][0] + g.L[1][1];
-  SFD_GLOBAL float *const pExy = (SFD_GLOBAL float *)g.psi[PSI_EXY];
-  SFD_GLOBAL float *const pExz = (SFD_GLOBAL float *)g.psi[PSI_EXZ];
-  SFD_GLOBAL float *const pEyz = (SFD_GLOBAL float *)g.psi[PSI_EYZ];
-  SFD_GLOBAL float *const pEyx = (SFD_GLOBAL float *)g.psi[PSI_EYX];
-  SFD_GLOBAL float *const pEzy = (SFD_GLOBAL float *)g.psi[PSI_EZY];
-  SFD_GLOBAL float *const pEzx = (SFD_GLOBAL float *)g.psi[PSI_EZX];
-  for (int k = k0; k < k1; ++k, p += pxy) {
+  // psi slab layouts (borderscpml.F90): byte offsets of the lane in a slab
+  // plane; the plane (k, or the z slab index) adds a uniform base
+  const unsigned oA = (unsigned)(qyn * nx + i) * 4u;         // psi_Exy
+  const unsigned oB = (unsigned)(j * nx + i) * 4u;           // psi_Exz
+  const unsigned oC = (unsigned)(j * (nx + 1) + i) * 4u;     // psi_Eyz
+  const unsigned oD = (unsigned)(j * Lx + qxn) * 4u;         // psi_Eyx, psi_Ezx
+  const unsigned oE = (unsigned)(qyn * (nx + 1) + i) * 4u;   // psi_Ezy
+  float *const pExy = (float *)g.psi[PSI_EXY], *const pExz = (float *)g.psi[PSI_EXZ];
+  float *const pEyz = (float *)g.psi[PSI_EYZ], *const pEyx = (float *)g.psi[PSI_EYX];
+  float *const pEzy = (float *)g.psi[PSI_EZY], *const pEzx = (float *)g.psi[PSI_EZX];
+  for (int k = k0; k < k1; ++k) {
     const bool cz = k < nz;
+    const long long kb = (long long)(k + 1) * fpb, kmb = (long long)(k + 1) * mpb;
     // ---- issue the independent loads of this plane
-    const float idzh = ((cgf)g.id[5])[k + 1];
+    const float idzh = ro_uniform(g.id[5], k + 1);
     float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f, hz_jm = 0.0f, hx_jm = 0.0f, hz_im = 0.0f,
           hy_im = 0.0f;
     if (NEEDH) {
-      hx0 = S::ld1(Hx, p, g.fs[3]); hy0 = S::ld1(Hy, p, g.fs[4]); hz0 = S::ld1(Hz, p, g.fs[5]);
-      hz_jm = S::ld1(Hz, p - px, g.fs[5]); hx_jm = S::ld1(Hx, p - px, g.fs[3]);
-      hz_im = S::ld1(Hz, p - 1, g.fs[5]); hy_im = S::ld1(Hy, p - 1, g.fs[4]);
+      const SFD_GLOBAL char *hxk = plane_base(bH[0], kb), *hyk = plane_base(bH[1], kb),
+                            *hzk = plane_base(bH[2], kb);
+      hx0 = S::ldo(hxk, of, g.fs[3]); hy0 = S::ldo(hyk, of, g.fs[4]);
+      hz0 = S::ldo(hzk, of, g.fs[5]);
+      hz_jm = S::ldo(hzk, of - fy, g.fs[5]); hx_jm = S::ldo(hxk, of - fy, g.fs[3]);
+      hz_im = S::ldo(hzk, of - ES, g.fs[5]); hy_im = S::ldo(hyk, of - ES, g.fs[4]);
     }
+    SFD_GLOBAL char *const exk = plane_base(bE[0], kb), *const eyk = plane_base(bE[1], kb),
+                         *const ezk = plane_base(bE[2], kb);
     int mx = 0, my = 0, mz = 0;
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    if (cx) { mx = ((const SFD_GLOBAL int16_t *)g.mi[0])[p]; ex = S::ld1(Ex, p, g.fs[0]); }
-    if (cy) { my = ((const SFD_GLOBAL int16_t *)g.mi[1])[p]; ey = S::ld1(Ey, p, g.fs[1]); }
-    if (cz) { mz = ((const SFD_GLOBAL int16_t *)g.mi[2])[p]; ez = S::ld1(Ez, p, g.fs[2]); }
+    if (cx) { mx = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); ex = S::ldo(exk, of, g.fs[0]); }
+    if (cy) { my = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); ey = S::ldo(eyk, of, g.fs[1]); }
+    if (cz) { mz = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); ez = S::ldo(ezk, of, g.fs[2]); }
     int qzn = 0;
     const bool szn = CPML && slab_node(k, nz, g.L[2][0], g.L[2][1], qzn);
-    // psi cells of this plane (borderscpml.F90 slab layouts) and their state
-    const long long sxy = ((long long)k * Ly + qyn) * nx + i;          // psi_Exy
-    const long long sxz = ((long long)qzn * (ny + 1) + j) * nx + i;    // psi_Exz
-    const long long syz = ((long long)qzn * ny + j) * (nx + 1) + i;    // psi_Eyz
-    const long long syx = ((long long)k * ny + j) * Lx + qxn;          // psi_Eyx
-    const long long szy = ((long long)k * Ly + qyn) * (nx + 1) + i;    // psi_Ezy
-    const long long szx = ((long long)k * (ny + 1) + j) * Lx + qxn;    // psi_Ezx
     float bzp = 0.0f, ccz = 0.0f, vxy = 0.0f, vxz = 0.0f, vyz = 0.0f, vyx = 0.0f, vzy = 0.0f,
           vzx = 0.0f;
+    SFD_GLOBAL float *sExy = nullptr, *sExz = nullptr, *sEyz = nullptr, *sEyx = nullptr,
+                     *sEzy = nullptr, *sEzx = nullptr;
     if (CPML) {
-      if (szn) { bzp = ((cgf)g.pbe[2])[k + 1]; ccz = ((cgf)g.pce[2])[k + 1]; }
-      if (cx && syn) vxy = pExy[sxy];
-      if (cx && szn) vxz = pExz[sxz];
-      if (cy && szn) vyz = pEyz[syz];
-      if (cy && sxn) vyx = pEyx[syx];
-      if (cz && syn) vzy = pEzy[szy];
-      if (cz && sxn) vzx = pEzx[szx];
+      if (szn) { bzp = ro_uniform(g.pbe[2], k + 1); ccz = ro_uniform(g.pce[2], k + 1); }
+      // slab plane bases: psi_Exy (k*Ly + q)*nx + i, psi_Exz (q*(ny+1) + j)*nx + i,
+      // psi_Eyz (q*ny + j)*(nx+1) + i, psi_Eyx (k*ny + j)*Lx + q,
+      // psi_Ezy (k*Ly + q)*(nx+1) + i, psi_Ezx (k*(ny+1) + j)*Lx + q
+      if (Ly) {   // uniform: the y slabs exist
+        sExy = plane_base(pExy, (long long)k * Ly * nx);
+        sEzy = plane_base(pEzy, (long long)k * Ly * (nx + 1));
+      }
+      if (szn) {
+        sExz = plane_base(pExz, (long long)qzn * (ny + 1) * nx);
+        sEyz = plane_base(pEyz, (long long)qzn * ny * (nx + 1));
+      }
+      if (Lx) {
+        sEyx = plane_base(pEyx, (long long)k * ny * Lx);
+        sEzx = plane_base(pEzx, (long long)k * (ny + 1) * Lx);
+      }
+      if (cx && syn) vxy = ld_at(sExy, oA);
+      if (cx && szn) vxz = ld_at(sExz, oB);
+      if (cy && szn) vyz = ld_at(sEyz, oC);
+      if (cy && sxn) vyx = ld_at(sEyx, oD);
+      if (cz && syn) vzy = ld_at(sEzy, oE);
+      if (cz && sxn) vzx = ld_at(sEzx, oD);
     }
     // ---- compute; psi <- b*psi + c*curl, then E += +-G2*psi
     if (cx) {
@@ -551,16 +617,16 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
       if (CPML) {
         if (syn) {  // psi_Exy, left/right (borderscpml.F90:609-623)
           const float v = madd((hz0 - hz_jm) * ccy, by, vxy);
-          pExy[sxy] = v;
+          st_at(sExy, oA, v);
           e = madd(e, c.y, v);
         }
         if (szn) {  // psi_Exz, down/up (:699-712)
           const float v = madd((hy0 - hy_km) * ccz, bzp, vxz);
-          pExz[sxz] = v;
+          st_at(sExz, oB, v);
           e = madd(e, -c.y, v);
         }
       }
-      S::st1(Ex, p, g.fi[0], e);
+      S::sto(exk, of, g.fi[0], e);
     }
     if (cy) {
       float e = ey;
@@ -569,16 +635,16 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
       if (CPML) {
         if (szn) {  // psi_Eyz, down/up (:680-693)
           const float v = madd((hx0 - hx_km) * ccz, bzp, vyz);
-          pEyz[syz] = v;
+          st_at(sEyz, oC, v);
           e = madd(e, c.y, v);
         }
         if (sxn) {  // psi_Eyx, back/front (:790-802)
           const float v = madd((hz0 - hz_im) * ccx, bx, vyx);
-          pEyx[syx] = v;
+          st_at(sEyx, oD, v);
           e = madd(e, -c.y, v);
         }
       }
-      S::st1(Ey, p, g.fi[1], e);
+      S::sto(eyk, of, g.fi[1], e);
     }
     if (cz) {
       float e = ez;
@@ -587,44 +653,48 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
       if (CPML) {
         if (syn) {  // psi_Ezy, left/right (:630-642)
           const float v = madd((hx0 - hx_jm) * ccy, by, vzy);
-          pEzy[szy] = v;
+          st_at(sEzy, oE, v);
           e = madd(e, -c.y, v);
         }
         if (sxn) {  // psi_Ezx, back/front (:771-783)
           const float v = madd((hy0 - hy_im) * ccx, bx, vzx);
-          pEzx[szx] = v;
+          st_at(sEzx, oD, v);
           e = madd(e, c.y, v);
         }
       }
-      S::st1(Ez, p, g.fi[2], e);
+      S::sto(ezk, of, g.fi[2], e);
     }
     if (NEEDH) { hx_km = hx0; hy_km = hy0; }
   }
 }
 
-// one tile of the H face pass (loads issued per plane as in the E pass)
+// one tile of the H face pass (addressing and loads as in the E pass)
 template <bool YEE, bool CPML, bool TBL, class T = float, class G = DevGrid>
 __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
                                            const MedTab &tab, int blk) {
   typedef Store<T> S;
   typedef const SFD_GLOBAL float *cgf;
+  constexpr unsigned ES = sizeof(T);
   int i, j, bz;
   tile_column(tm, blk, g.nx, i, j, bz);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny) return;
   const int k0 = bz * tm.kc;
   const int k1 = min(k0 + tm.kc, nz + 1);
-  const long long px = g.px, pxy = g.pxy;
-  long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + kXOff + i;
-  const SFD_GLOBAL float *__restrict__ Ex = (SFD_GLOBAL float *)g.f[0];
-  const SFD_GLOBAL float *__restrict__ Ey = (SFD_GLOBAL float *)g.f[1];
-  const SFD_GLOBAL float *__restrict__ Ez = (SFD_GLOBAL float *)g.f[2];
-  SFD_GLOBAL float *__restrict__ Hx = (SFD_GLOBAL float *)g.f[3];
-  SFD_GLOBAL float *__restrict__ Hy = (SFD_GLOBAL float *)g.f[4];
-  SFD_GLOBAL float *__restrict__ Hz = (SFD_GLOBAL float *)g.f[5];
+  const long long fpb = g.pxy * ES, mpb = g.pxy * 2;
+  const unsigned lo = (unsigned)((j + 1) * (int)g.px + kXOff + i);
+  const unsigned of = lo * ES, om = lo * 2u, fy = (unsigned)g.px * ES;
+  char *const bE[3] = {(char *)g.f[0], (char *)g.f[1], (char *)g.f[2]};
+  char *const bH[3] = {(char *)g.f[3], (char *)g.f[4], (char *)g.f[5]};
+  char *const bM[3] = {(char *)g.mi[3], (char *)g.mi[4], (char *)g.mi[5]};
   const float idye = ((cgf)g.id[1])[j + 1];
   const float idxe = ((cgf)g.id[0])[i + 1];
-  float ex_c = S::ld1(Ex, p, g.fs[0]), ey_c = S::ld1(Ey, p, g.fs[1]);
+  float ex_c, ey_c;
+  {  // plane k0 of Ex, Ey
+    const long long kb = (long long)(k0 + 1) * fpb;
+    ex_c = S::ldo(plane_base(bE[0], kb), of, g.fs[0]);
+    ey_c = S::ldo(plane_base(bE[1], kb), of, g.fs[1]);
+  }
   int qxc = 0, qyc = 0;
   bool sxc = false, syc = false;
   float bx = 0.0f, ccx = 0.0f, by = 0.0f, ccy = 0.0f;
@@ -638,44 +708,63 @@ __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
     if (syc) { by = ((cgf)g.pbm[1])[j + 1]; ccy = ((cgf)g.pcm[1])[j + 1]; }
   }
   const int Lx = g.L[0][0] + g.L[0][1], Ly = g.L[1][0] + g.L[1][1];
-  SFD_GLOBAL float *const pHxy = (SFD_GLOBAL float *)g.psi[PSI_HXY];
-  SFD_GLOBAL float *const pHxz = (SFD_GLOBAL float *)g.psi[PSI_HXZ];
-  SFD_GLOBAL float *const pHyz = (SFD_GLOBAL float *)g.psi[PSI_HYZ];
-  SFD_GLOBAL float *const pHyx = (SFD_GLOBAL float *)g.psi[PSI_HYX];
-  SFD_GLOBAL float *const pHzy = (SFD_GLOBAL float *)g.psi[PSI_HZY];
-  SFD_GLOBAL float *const pHzx = (SFD_GLOBAL float *)g.psi[PSI_HZX];
-  for (int k = k0; k < k1; ++k, p += pxy) {
+  const unsigned oA = (unsigned)(qyc * (nx + 1) + i) * 4u;   // psi_Hxy
+  const unsigned oB = (unsigned)(j * (nx + 1) + i) * 4u;     // psi_Hxz
+  const unsigned oC = (unsigned)(j * nx + i) * 4u;           // psi_Hyz
+  const unsigned oD = (unsigned)(j * Lx + qxc) * 4u;         // psi_Hyx, psi_Hzx
+  const unsigned oE = (unsigned)(qyc * nx + i) * 4u;         // psi_Hzy
+  float *const pHxy = (float *)g.psi[PSI_HXY], *const pHxz = (float *)g.psi[PSI_HXZ];
+  float *const pHyz = (float *)g.psi[PSI_HYZ], *const pHyx = (float *)g.psi[PSI_HYX];
+  float *const pHzy = (float *)g.psi[PSI_HZY], *const pHzx = (float *)g.psi[PSI_HZX];
+  for (int k = k0; k < k1; ++k) {
     const bool ax = j < ny && k < nz;   // Hx sweep: x[0,nx] y[0,ny-1] z[0,nz-1]
     const bool ay = i < nx && k < nz;   // Hy sweep: x[0,nx-1] y[0,ny] z[0,nz-1]
     const bool az = i < nx && j < ny;   // Hz sweep: x[0,nx-1] y[0,ny-1] z[0,nz]
-    const float idze = ((cgf)g.id[2])[k + 1];
-    const float ex_kp = S::ld1(Ex, p + pxy, g.fs[0]), ey_kp = S::ld1(Ey, p + pxy, g.fs[1]);
-    const float ez_c = S::ld1(Ez, p, g.fs[2]);
-    const float ez_jp = S::ld1(Ez, p + px, g.fs[2]), ex_jp = S::ld1(Ex, p + px, g.fs[0]);
-    const float ez_ip = S::ld1(Ez, p + 1, g.fs[2]), ey_ip = S::ld1(Ey, p + 1, g.fs[1]);
+    const long long kb = (long long)(k + 1) * fpb, kmb = (long long)(k + 1) * mpb;
+    const float idze = ro_uniform(g.id[2], k + 1);
+    const SFD_GLOBAL char *exn = plane_base(bE[0], kb + fpb), *eyn = plane_base(bE[1], kb + fpb);
+    const SFD_GLOBAL char *exk = plane_base(bE[0], kb), *eyk = plane_base(bE[1], kb),
+                          *ezk = plane_base(bE[2], kb);
+    const float ex_kp = S::ldo(exn, of, g.fs[0]), ey_kp = S::ldo(eyn, of, g.fs[1]);
+    const float ez_c = S::ldo(ezk, of, g.fs[2]);
+    const float ez_jp = S::ldo(ezk, of + fy, g.fs[2]), ex_jp = S::ldo(exk, of + fy, g.fs[0]);
+    const float ez_ip = S::ldo(ezk, of + ES, g.fs[2]), ey_ip = S::ldo(eyk, of + ES, g.fs[1]);
+    SFD_GLOBAL char *const hxk = plane_base(bH[0], kb), *const hyk = plane_base(bH[1], kb),
+                         *const hzk = plane_base(bH[2], kb);
     int mx = 0, my = 0, mz = 0;
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-    if (ax) { mx = ((const SFD_GLOBAL int16_t *)g.mi[3])[p]; hx = S::ld1(Hx, p, g.fs[3]); }
-    if (ay) { my = ((const SFD_GLOBAL int16_t *)g.mi[4])[p]; hy = S::ld1(Hy, p, g.fs[4]); }
-    if (az) { mz = ((const SFD_GLOBAL int16_t *)g.mi[5])[p]; hz = S::ld1(Hz, p, g.fs[5]); }
+    if (ax) { mx = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); hx = S::ldo(hxk, of, g.fs[3]); }
+    if (ay) { my = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); hy = S::ldo(hyk, of, g.fs[4]); }
+    if (az) { mz = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); hz = S::ldo(hzk, of, g.fs[5]); }
     int qzc = 0;
     const bool szc = CPML && k < nz && slab_cell(k, nz, g.L[2][0], g.L[2][1], qzc);
-    const long long sxy = ((long long)k * Ly + qyc) * (nx + 1) + i;    // psi_Hxy
-    const long long sxz = ((long long)qzc * ny + j) * (nx + 1) + i;    // psi_Hxz
-    const long long syz = ((long long)qzc * (ny + 1) + j) * nx + i;    // psi_Hyz
-    const long long syx = ((long long)k * (ny + 1) + j) * Lx + qxc;    // psi_Hyx
-    const long long szy = ((long long)k * Ly + qyc) * nx + i;          // psi_Hzy
-    const long long szx = ((long long)k * ny + j) * Lx + qxc;          // psi_Hzx
     float bzp = 0.0f, ccz = 0.0f, vxy = 0.0f, vxz = 0.0f, vyz = 0.0f, vyx = 0.0f, vzy = 0.0f,
           vzx = 0.0f;
+    SFD_GLOBAL float *sHxy = nullptr, *sHxz = nullptr, *sHyz = nullptr, *sHyx = nullptr,
+                     *sHzy = nullptr, *sHzx = nullptr;
     if (CPML) {
-      if (szc) { bzp = ((cgf)g.pbm[2])[k + 1]; ccz = ((cgf)g.pcm[2])[k + 1]; }
-      if (ax && syc) vxy = pHxy[sxy];
-      if (ax && szc) vxz = pHxz[sxz];
-      if (ay && szc) vyz = pHyz[syz];
-      if (ay && sxc) vyx = pHyx[syx];
-      if (az && syc) vzy = pHzy[szy];
-      if (az && sxc) vzx = pHzx[szx];
+      if (szc) { bzp = ro_uniform(g.pbm[2], k + 1); ccz = ro_uniform(g.pcm[2], k + 1); }
+      // slab plane bases: psi_Hxy (k*Ly + q)*(nx+1) + i, psi_Hxz (q*ny + j)*(nx+1) + i,
+      // psi_Hyz (q*(ny+1) + j)*nx + i, psi_Hyx (k*(ny+1) + j)*Lx + q,
+      // psi_Hzy (k*Ly + q)*nx + i, psi_Hzx (k*ny + j)*Lx + q
+      if (Ly) {   // uniform: the y slabs exist
+        sHxy = plane_base(pHxy, (long long)k * Ly * (nx + 1));
+        sHzy = plane_base(pHzy, (long long)k * Ly * nx);
+      }
+      if (szc) {
+        sHxz = plane_base(pHxz, (long long)qzc * ny * (nx + 1));
+        sHyz = plane_base(pHyz, (long long)qzc * (ny + 1) * nx);
+      }
+      if (Lx) {
+        sHyx = plane_base(pHyx, (long long)k * (ny + 1) * Lx);
+        sHzx = plane_base(pHzx, (long long)k * ny * Lx);
+      }
+      if (ax && syc) vxy = ld_at(sHxy, oA);
+      if (ax && szc) vxz = ld_at(sHxz, oB);
+      if (ay && szc) vyz = ld_at(sHyz, oC);
+      if (ay && sxc) vyx = ld_at(sHyx, oD);
+      if (az && syc) vzy = ld_at(sHzy, oE);
+      if (az && sxc) vzx = ld_at(sHzx, oD);
     }
     if (ax) {
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, mx);
@@ -684,16 +773,16 @@ __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
       if (CPML) {
         if (syc) {  // psi_Hxy (borderscpml.F90:911-921)
           const float v = madd((ez_jp - ez_c) * ccy, by, vxy);
-          pHxy[sxy] = v;
+          st_at(sHxy, oA, v);
           h = madd(h, -c.y, v);
         }
         if (szc) {  // psi_Hxz (:1021-1031)
           const float v = madd((ey_kp - ey_c) * ccz, bzp, vxz);
-          pHxz[sxz] = v;
+          st_at(sHxz, oB, v);
           h = madd(h, c.y, v);
         }
       }
-      S::st1(Hx, p, g.fi[3], h);
+      S::sto(hxk, of, g.fi[3], h);
     }
     if (ay) {
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, my);
@@ -702,16 +791,16 @@ __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
       if (CPML) {
         if (szc) {  // psi_Hyz (:1001-1011)
           const float v = madd((ex_kp - ex_c) * ccz, bzp, vyz);
-          pHyz[syz] = v;
+          st_at(sHyz, oC, v);
           h = madd(h, -c.y, v);
         }
         if (sxc) {  // psi_Hyx (:1111-1121)
           const float v = madd((ez_ip - ez_c) * ccx, bx, vyx);
-          pHyx[syx] = v;
+          st_at(sHyx, oD, v);
           h = madd(h, c.y, v);
         }
       }
-      S::st1(Hy, p, g.fi[4], h);
+      S::sto(hyk, of, g.fi[4], h);
     }
     if (az) {
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, mz);
@@ -720,16 +809,16 @@ __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
       if (CPML) {
         if (syc) {  // psi_Hzy (:931-941)
           const float v = madd((ex_jp - ex_c) * ccy, by, vzy);
-          pHzy[szy] = v;
+          st_at(sHzy, oE, v);
           h = madd(h, c.y, v);
         }
         if (sxc) {  // psi_Hzx (:1091-1101)
           const float v = madd((ey_ip - ey_c) * ccx, bx, vzx);
-          pHzx[szx] = v;
+          st_at(sHzx, oD, v);
           h = madd(h, -c.y, v);
         }
       }
-      S::st1(Hz, p, g.fi[5], h);
+      S::sto(hzk, of, g.fi[5], h);
     }
     ex_c = ex_kp;
     ey_c = ey_kp;
@@ -1219,35 +1308,6 @@ int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
 // LDS; H values of step n are read from memory (gi is never written during
 // the step).
 
-
-// Loads and stores of the march address a plane through a base pointer that
-// is the same for every lane (scalar registers) plus the lane's byte offset
-// inside the plane (one 32-bit register).  The plane base is made opaque to
-// the compiler, which otherwise hoists a 64-bit (array base + offset) pair per
-// array and lane out of the march; it stays in the global address space so
-// that the accesses are global, not flat, loads and stores.
-template <class T>
-__device__ __forceinline__ SFD_GLOBAL T *plane_base(T *base, long long P) {
-  SFD_GLOBAL T *q = (SFD_GLOBAL T *)(base + P);
-  asm volatile("" : "+s"(q));
-  return q;
-}
-template <class T>
-__device__ __forceinline__ T ld_at(const SFD_GLOBAL T *base, unsigned ob) {
-  return *reinterpret_cast<const SFD_GLOBAL T *>(reinterpret_cast<const SFD_GLOBAL char *>(base) +
-                                                 ob);
-}
-template <class T>
-__device__ __forceinline__ void st_at(SFD_GLOBAL T *base, unsigned ob, T v) {
-  *reinterpret_cast<SFD_GLOBAL T *>(reinterpret_cast<SFD_GLOBAL char *>(base) + ob) = v;
-}
-
-// an element of a table the kernel never writes, at an index the same for
-// every lane: read through the constant address space, i.e. by a scalar load
-// that does not join the vector-memory counter shared with the stores
-__device__ __forceinline__ float ro_uniform(const float *t, int idx) {
-  return ((const __attribute__((address_space(4))) float *)t)[idx];
-}
 
 // the descriptor pointer made opaque to the compiler once per plane: what is
 // read through it is read where it is used (slab and dispersive cells), not
