@@ -389,7 +389,7 @@ int64_t fdtd_batch_step_of(const fdtd_batch *b, int32_t slot);
  * one cell per lane, each block taking a run of 64 * rows consecutive columns
  * of a grid's x-y plane, x fastest, so that narrow grids leave no lane idle;
  * xcells 4: four x cells per lane over 256-cell x segments) and the
- * persistent launch's block count (0: 8192).  Either storage runs either. */
+ * persistent launch's block count (0: 16384).  Either storage runs either. */
 int fdtd_batch_set_tiling(fdtd_batch *b, int32_t rows, int32_t zchunk, int32_t xcells,
                           int32_t blocks);
 /* One launch per step for the batch (rows 4, 8 or 16; 0: the face passes

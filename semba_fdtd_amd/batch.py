@@ -17,7 +17,7 @@ from ._lib import FdtdError
 
 
 class Batch:
-    def __init__(self, device: int = 0, stream=None, rows: int = 4, zchunk: int = 32,
+    def __init__(self, device: int = 0, stream=None, rows: int = 4, zchunk: int = 16,
                  blocks: int = 0, xcells: int = 1, step_rows: int = 0, step_zchunk: int = 16):
         self.lib = lib = _lib.load()
         b = C.c_void_p()
@@ -39,7 +39,7 @@ class Batch:
         """Blocks of 64 x rows threads marching zchunk z planes, xcells x
         cells per lane (1: each block takes a run of 64 * rows consecutive
         columns of a grid's x-y plane; 4: 256-cell x segments), and the
-        persistent launch's block count (0: 8192)."""
+        persistent launch's block count (0: 16384)."""
         self._ck(self.lib.fdtd_batch_set_tiling(self.b, rows, zchunk, xcells, blocks),
                  "fdtd_batch_set_tiling")
 

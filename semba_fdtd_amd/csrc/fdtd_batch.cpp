@@ -63,7 +63,7 @@ struct fdtd_batch {
   std::vector<fdtd_handle *> slots;   // admitted grids; retired slots are null
   std::vector<long long> step;        // host copy of each slot's step counter
   std::vector<unsigned> version;      // each grid's state version at the last build
-  int rows = 4, zchunk = 32, xcells = 1, blocks = 0;
+  int rows = 4, zchunk = 16, xcells = 1, blocks = 0;
   // device descriptors, rebuilt when the membership changes
   bool ready = false;
   std::vector<void *> allocs;
@@ -443,9 +443,9 @@ int launch_step_dir(fdtd_batch *b, int d, bool exact_prev) {
 
 int launch_step(fdtd_batch *b) {
   const bool exact = b->mode == FDTD_MODE_EXACT;
-  // persistent blocks: 8192 measured fastest on the 256 uneven instances
-  // (flat tiles of 64 x 4 threads, 32-plane chunks)
-  const int blocks = b->blocks > 0 ? b->blocks : 8192;
+  // persistent blocks: 16384 measured fastest on the 256 uneven instances
+  // (flat tiles of 64 x 4 threads, 16-plane chunks)
+  const int blocks = b->blocks > 0 ? b->blocks : 16384;
   int e, rc;
   for (int half = 0; half < 2; ++half) {
     const int vec = b->xcells == 4 ? 4 : 1;
