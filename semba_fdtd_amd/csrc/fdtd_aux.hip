@@ -11,7 +11,7 @@
 namespace sfd {
 
 __device__ __forceinline__ long long fidx(const DevGrid &g, int i, int j, int k) {
-  return (long long)(k + 1) * g.pxy + (long long)(j + 1) * g.px + kXOff + i;
+  return (long long)(k + 1) * g.pxy + (long long)(j + 1) * g.px + g.xoff + i;
 }
 
 // AdvanceNodalE/H (nodalsources.F90:396-777).  cells are in the reference's
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) k_tfsf_list(const PwRun *runs, const long
   const int step = r.dn ? *r.dn : n;
   const float tm = r.half ? (float)(r.tiempo[step] + (double)0.5f * r.dt) : (float)r.tiempo[step];
   const float ehi = pw_incid(f, r.wave, r.phys, r.pstride, tm, cc);
-  const long long p = (long long)(cc[2] + 1) * r.pxy + (long long)(cc[1] + 1) * r.px + kXOff + cc[0];
+  const long long p = (long long)(cc[2] + 1) * r.pxy + (long long)(cc[1] + 1) * r.px + r.xoff + cc[0];
   if (r.fp16) {
     __half *F = reinterpret_cast<__half *>(r.F);
     const float old = __half2float(F[p]) * r.fs;
@@ -93,17 +93,17 @@ int launch_tfsf_list(const PwRun *runs, const long long *start, int nruns, long 
 
 // A dense x-fastest box of ex*ey*ez elements moved between a packed buffer
 // and a grid array (rows of px elements, planes of pxy, the box's first
-// element at x offset kXOff - 1 of row 0, plane 0): host transfers then take
+// element at x offset xoff - 1 of row 0, plane 0): host transfers then take
 // one contiguous copy instead of one pitched copy per plane.
 template <class T>
 __global__ void __launch_bounds__(256) k_box_move(T *grid, T *packed, long long ex, long long ey,
                                                  long long ez, long long px, long long pxy,
-                                                 int to_grid) {
+                                                 int xoff, int to_grid) {
   const long long n = ex * ey * ez;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
     const long long i = t % ex, j = (t / ex) % ey, k = t / (ex * ey);
-    const long long p = k * pxy + j * px + (kXOff - 1) + i;
+    const long long p = k * pxy + j * px + (xoff - 1) + i;
     if (to_grid)
       grid[p] = packed[t];
     else
@@ -112,17 +112,18 @@ __global__ void __launch_bounds__(256) k_box_move(T *grid, T *packed, long long 
 }
 
 int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3], long long px,
-                    long long pxy, int to_grid, void *stream) {
+                    long long pxy, int xoff, int to_grid, void *stream) {
   const long long n = ext[0] * ext[1] * ext[2];
   if (n <= 0) return 0;
   const unsigned b = (unsigned)std::min<long long>((n + 255) / 256, 8192);
   if (es == 2)
     hipLaunchKernelGGL(k_box_move<int16_t>, dim3(b), dim3(256), 0, (hipStream_t)stream,
                        (int16_t *)grid, (int16_t *)packed, ext[0], ext[1], ext[2], px, pxy,
-                       to_grid);
+                       xoff, to_grid);
   else
     hipLaunchKernelGGL(k_box_move<float>, dim3(b), dim3(256), 0, (hipStream_t)stream,
-                       (float *)grid, (float *)packed, ext[0], ext[1], ext[2], px, pxy, to_grid);
+                       (float *)grid, (float *)packed, ext[0], ext[1], ext[2], px, pxy, xoff,
+                       to_grid);
   return (int)hipGetLastError();
 }
 
@@ -132,12 +133,12 @@ int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3],
 template <class T>
 __global__ void __launch_bounds__(256) k_box_gather_f32(const T *grid, float *packed, float scale,
                                                        long long ex, long long ey, long long ez,
-                                                       long long px, long long pxy) {
+                                                       long long px, long long pxy, int xoff) {
   const long long n = ex * ey * ez;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
     const long long i = t % ex, j = (t / ex) % ey, k = t / (ex * ey);
-    const long long p = k * pxy + j * px + (kXOff - 1) + i;
+    const long long p = k * pxy + j * px + (xoff - 1) + i;
     if constexpr (sizeof(T) == 2)
       packed[t] = __half2float(grid[p]) * scale;
     else
@@ -146,16 +147,19 @@ __global__ void __launch_bounds__(256) k_box_gather_f32(const T *grid, float *pa
 }
 
 int launch_box_gather_f32(const void *grid, size_t es, float scale, float *packed,
-                          const long long ext[3], long long px, long long pxy, void *stream) {
+                          const long long ext[3], long long px, long long pxy, int xoff,
+                          void *stream) {
   const long long n = ext[0] * ext[1] * ext[2];
   if (n <= 0) return 0;
   const unsigned b = (unsigned)std::min<long long>((n + 255) / 256, 8192);
   if (es == 2)
     hipLaunchKernelGGL(k_box_gather_f32<__half>, dim3(b), dim3(256), 0, (hipStream_t)stream,
-                       (const __half *)grid, packed, scale, ext[0], ext[1], ext[2], px, pxy);
+                       (const __half *)grid, packed, scale, ext[0], ext[1], ext[2], px, pxy,
+                       xoff);
   else
     hipLaunchKernelGGL(k_box_gather_f32<float>, dim3(b), dim3(256), 0, (hipStream_t)stream,
-                       (const float *)grid, packed, scale, ext[0], ext[1], ext[2], px, pxy);
+                       (const float *)grid, packed, scale, ext[0], ext[1], ext[2], px, pxy,
+                       xoff);
   return (int)hipGetLastError();
 }
 

@@ -107,9 +107,9 @@ int copy_box(fdtd_handle *h, int comp, void *dev, const void *host_src, void *ho
     if (host_src) {
       HIPCHK(h, hipMemcpyAsync(h->stage, (const char *)host_src + (size_t)k0 * plane, bytes,
                                hipMemcpyHostToDevice, h->stream));
-      HIPCHK(h, (hipError_t)launch_box_move(g, h->stage, es, sub, h->g.px, h->g.pxy, 1, h->stream));
+      HIPCHK(h, (hipError_t)launch_box_move(g, h->stage, es, sub, h->g.px, h->g.pxy, h->g.xoff, 1, h->stream));
     } else {
-      HIPCHK(h, (hipError_t)launch_box_move(g, h->stage, es, sub, h->g.px, h->g.pxy, 0, h->stream));
+      HIPCHK(h, (hipError_t)launch_box_move(g, h->stage, es, sub, h->g.px, h->g.pxy, h->g.xoff, 0, h->stream));
       HIPCHK(h, hipMemcpyAsync((char *)host_dst + (size_t)k0 * plane, h->stage, bytes,
                                hipMemcpyDeviceToHost, h->stream));
     }
@@ -534,7 +534,7 @@ int build_step(fdtd_handle *h) {
     b[3] = b[4] = b[5] = -0x7fffffff;
     for (const NodalCell &c : C) {
       const long long k = c.p / h->g.pxy - 1, r = c.p % h->g.pxy;
-      const long long j = r / h->g.px - 1, i = r % h->g.px - kXOff;
+      const long long j = r / h->g.px - 1, i = r % h->g.px - h->g.xoff;
       const long long q[3] = {i, j, k};
       for (int a = 0; a < 3; ++a) {
         b[a] = (int)std::min<long long>(b[a], q[a]);
@@ -749,7 +749,7 @@ int build_nodal(fdtd_handle *h) {
             for (int j = S.box[2]; j <= S.box[3]; ++j)
               for (int i = S.box[0]; i <= S.box[1]; ++i) {
                 NodalCell nc{};
-                nc.p = (long long)(k + 1) * h->g.pxy + (long long)(j + 1) * h->g.px + kXOff + i;
+                nc.p = (long long)(k + 1) * h->g.pxy + (long long)(j + 1) * h->g.px + h->g.xoff + i;
                 nc.comp = c;
                 nc.hard = hard;
                 nc.src = (int)s;
@@ -972,6 +972,7 @@ int build_planewaves(fdtd_handle *h) {
           r.fs = h->g.fs[f.comp];
           r.fi = h->g.fi[f.comp];
           r.px = h->g.px;
+          r.xoff = h->g.xoff;
           r.pxy = h->g.pxy;
           r.wave = h->dwaves;
           r.phys = h->dphys;
@@ -1172,7 +1173,16 @@ int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
   }
   DevGrid &g = h->g;
   g.nx = d->nx; g.ny = d->ny; g.nz = d->nz;
-  g.px = ((long long)kXOff + d->nx + 8 + 31) / 32 * 32;
+  // rows of grids at least 256 cells wide start on a 128-byte line; narrower
+  // rows are packed to 16 bytes, so a row of a few tens of cells does not
+  // fetch whole lines of padding at both ends
+  if (d->nx >= 256) {
+    g.xoff = kXOff;
+    g.px = ((long long)kXOff + d->nx + 8 + 31) / 32 * 32;
+  } else {
+    g.xoff = kXOffNarrow;
+    g.px = ((long long)kXOffNarrow + d->nx + 8 + 3) / 4 * 4;
+  }
   g.pxy = g.px * (long long)(d->ny + 3);
   const size_t ne = field_elems(h);
   // binary16 storage: E scaled by 1 (V/m), H by 1/eta0 so both sit near 1
@@ -1890,7 +1900,7 @@ int fdtd_set_probes(fdtd_handle *h, int32_t nprobe, const int32_t *comp, const i
     for (int a = 0; a < 3; ++a)
       if (crd[a] < -1 || crd[a] > n[a] + 1) return fail(h, FDTD_ERR_ARG, "probe %d outside the grid", p);
     pr[p].comp = comp[p];
-    pr[p].p = (long long)(k[p] + 1) * h->g.pxy + (long long)(j[p] + 1) * h->g.px + kXOff + i[p];
+    pr[p].p = (long long)(k[p] + 1) * h->g.pxy + (long long)(j[p] + 1) * h->g.px + h->g.xoff + i[p];
   }
   int rc;
   h->dprobes = nullptr; h->dpbuf = nullptr; h->dpcount = nullptr;
@@ -2006,7 +2016,7 @@ int fdtd_get_field_noalloc(fdtd_handle *h, int32_t comp, float *host) {
     const long long nk = std::min(per, ext[2] - k0);
     const long long sub[3] = {ext[0], ext[1], nk};
     const char *g = (const char *)h->fld[comp] + (size_t)k0 * (size_t)h->g.pxy * es;
-    HIPCHK(h, (hipError_t)launch_box_gather_f32(g, es, scale, stage, sub, h->g.px, h->g.pxy,
+    HIPCHK(h, (hipError_t)launch_box_gather_f32(g, es, scale, stage, sub, h->g.px, h->g.pxy, h->g.xoff,
                                                 h->stream));
     HIPCHK(h, hipMemcpyAsync(host + (size_t)k0 * (plane / 4), stage, plane * (size_t)nk,
                              hipMemcpyDeviceToHost, h->stream));

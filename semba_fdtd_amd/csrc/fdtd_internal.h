@@ -4,11 +4,14 @@
 // HBM layout of one grid
 //   fields      6 arrays (Ex Ey Ez Hx Hy Hz), each (nz+3) planes x (ny+3) rows
 //               x px elements; cell (i,j,k) at ((k+1)*(ny+3)+(j+1))*px + xoff+i.
-//               xoff = 32 puts x = 0 on a 128-byte line boundary in every row
-//               (px is a multiple of 32 with room for x up to n+5, so the last
+//               Grids at least 256 cells wide: xoff = 32 puts x = 0 on a
+//               128-byte line boundary in every row (px a multiple of 32).
+//               Narrower grids: xoff = 4 and px a multiple of 4 (16 bytes),
+//               so that short rows do not drag whole padding lines along.
+//               Either way px leaves room for x up to n+7, so the last
 //               16-byte vector of a row and its right neighbour stay in the
-//               row), the Alloc padding planes/rows/columns (-1 and n+1) stay
-//               zero forever.
+//               row, and the Alloc padding planes/rows/columns (-1 and n+1)
+//               stay zero forever.
 //   media idx   6 int16 arrays with the same geometry as the fields.
 //   tables      G1/G2/GM1/GM2 and kind bits per media index; inverse spacings
 //               (length n+3, index i+1); CPML profiles per axis.
@@ -33,7 +36,8 @@ namespace sfd {
 
 constexpr int kPMax = 8;     // poles per dispersive medium
 constexpr int kWave = 64;    // CDNA wavefront
-constexpr int kXOff = 32;    // x = 0 offset inside a row (elements)
+constexpr int kXOff = 32;    // x = 0 offset inside a row of a wide grid (elements)
+constexpr int kXOffNarrow = 4;   // the same for grids under 256 cells wide
 
 enum { PSI_EXY = 0, PSI_EXZ, PSI_EYX, PSI_EYZ, PSI_EZX, PSI_EZY,
        PSI_HXY, PSI_HXZ, PSI_HYX, PSI_HYZ, PSI_HZX, PSI_HZY };
@@ -60,6 +64,7 @@ struct DevGrid {
   int nx, ny, nz;
   int nmedia;
   long long px, pxy;            // row pitch and plane stride in elements
+  int xoff;                     // x = 0 offset inside a row (kXOff or kXOffNarrow)
   float *f[6];                  // fp32 fields, or binary16 ones when fp16
   int fp16;                     // fields stored as binary16 * fs[c]
   float fs[6], fi[6];           // per-array scale and its inverse
@@ -148,6 +153,7 @@ struct PwRun {
   int fp16;                   // F holds binary16 values * fs
   float fs, fi;
   long long px, pxy;          // its row pitch and plane stride
+  int xoff;                   // x = 0 offset inside a row
   const PwWave *wave;
   const float *phys;
   int pstride;
@@ -385,11 +391,12 @@ int launch_media_narrow(const int16_t *src, uint8_t *dst, long long n, void *str
 // a dense box of es-byte elements (es 2 or 4), ext = its x, y, z extents,
 // moved between a packed device buffer and a grid array
 int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3], long long px,
-                    long long pxy, int to_grid, void *stream);
+                    long long pxy, int xoff, int to_grid, void *stream);
 // the same box read out of a field array (es 4: fp32, es 2: binary16 times
 // scale) into a packed fp32 buffer
 int launch_box_gather_f32(const void *grid, size_t es, float scale, float *packed,
-                          const long long ext[3], long long px, long long pxy, void *stream);
+                          const long long ext[3], long long px, long long pxy, int xoff,
+                          void *stream);
 // the incident field of every TF/SF face cell at step *dn for the one-launch
 // step: faces[0..nf0) act on E (time tiempo(n)), faces[nf0..nf) on H
 // (tiempo(n) + dt/2); face q's cells go to inc[faces[q].inc ...], the face's

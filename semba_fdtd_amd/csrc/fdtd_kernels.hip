@@ -304,7 +304,7 @@ __device__ __forceinline__ void ade_box_run(const DevGrid &g) {
        s += (long long)gridDim.x * blockDim.x) {
     const int x = (int)(s % bx), y = (int)((s / bx) % b[4]), z = (int)(s / bxy);
     const long long p = (long long)(b[2] + z + 1) * g.pxy + (long long)(b[1] + y + 1) * g.px +
-                        kXOff + b[0] + x;
+                        g.xoff + b[0] + x;
     const int m = g.mi[f][p];
     const int slot = tab.slot[m];
     if (slot < 0) continue;
@@ -343,7 +343,7 @@ __global__ void __launch_bounds__(256) k_ade_box(DevGrid g) {
        s += (long long)gridDim.x * blockDim.x) {
     const int x = (int)(s % bx), y = (int)((s / bx) % b[4]), z = (int)(s / bxy);
     const long long p = (long long)(b[2] + z + 1) * g.pxy + (long long)(b[1] + y + 1) * g.px +
-                        kXOff + b[0] + x;
+                        g.xoff + b[0] + x;
     const int m = g.mi[f][p];
     const int slot = tab.slot[m];
     if (slot < 0) continue;
@@ -521,7 +521,7 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
   const int k0 = bz * tm.kc;
   const int k1 = min(k0 + tm.kc, nz + 1);
   const long long fpb = g.pxy * ES, mpb = g.pxy * 2;   // plane strides, bytes
-  const unsigned lo = (unsigned)((j + 1) * (int)g.px + kXOff + i);
+  const unsigned lo = (unsigned)((j + 1) * (int)g.px + g.xoff + i);
   const unsigned of = lo * ES, om = lo * 2u, fy = (unsigned)g.px * ES;
   char *const bE[3] = {(char *)g.f[0], (char *)g.f[1], (char *)g.f[2]};
   char *const bH[3] = {(char *)g.f[3], (char *)g.f[4], (char *)g.f[5]};
@@ -682,7 +682,7 @@ __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
   const int k0 = bz * tm.kc;
   const int k1 = min(k0 + tm.kc, nz + 1);
   const long long fpb = g.pxy * ES, mpb = g.pxy * 2;
-  const unsigned lo = (unsigned)((j + 1) * (int)g.px + kXOff + i);
+  const unsigned lo = (unsigned)((j + 1) * (int)g.px + g.xoff + i);
   const unsigned of = lo * ES, om = lo * 2u, fy = (unsigned)g.px * ES;
   char *const bE[3] = {(char *)g.f[0], (char *)g.f[1], (char *)g.f[2]};
   char *const bH[3] = {(char *)g.f[3], (char *)g.f[4], (char *)g.f[5]};
@@ -852,7 +852,7 @@ __device__ __forceinline__ void epass4_tile(const DevGrid &g, const TileMap &tm,
   const int k0 = bz * tm.kc;
   const int k1 = min(k0 + tm.kc, nz + 1);
   const long long px = g.px, pxy = g.pxy;
-  long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + kXOff + i0;
+  long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + g.xoff + i0;
   float *__restrict__ Ex = g.f[0];
   float *__restrict__ Ey = g.f[1];
   float *__restrict__ Ez = g.f[2];
@@ -978,7 +978,7 @@ __device__ __forceinline__ void hpass4_tile(const DevGrid &g, const TileMap &tm,
   const int k0 = bz * tm.kc;
   const int k1 = min(k0 + tm.kc, nz + 1);
   const long long px = g.px, pxy = g.pxy;
-  long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + kXOff + i0;
+  long long p = (long long)(k0 + 1) * pxy + (long long)(j + 1) * px + g.xoff + i0;
   const float *__restrict__ Ex = g.f[0];
   const float *__restrict__ Ey = g.f[1];
   const float *__restrict__ Ez = g.f[2];
@@ -1560,7 +1560,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
   const unsigned pxb = (unsigned)gi.px * 4u;
   // byte offsets inside a plane: the cell and its x - 1 and y - 1 neighbours;
   // om: the cell in the int16 media arrays
-  const unsigned ob = (unsigned)(jc + 1) * pxb + (unsigned)(kXOff + ic) * 4u;
+  const unsigned ob = (unsigned)(jc + 1) * pxb + (unsigned)(gi.xoff + ic) * 4u;
   const unsigned obx = ob - 4u, oby = ob - pxb, om = ob >> 1;
   const bool ex_on = in && i < nx, ey_on = in && j < ny;
   const bool hx_on = own && j < ny, hy_on = own && i < nx, hz_on = own && i < nx && j < ny;
@@ -1777,7 +1777,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
         if (SRC) {
           if (be0) e = faces_apply(lf, be0, ie0, sd.inc, sd.pwG[0], i, j, k, e);
           if (nd && ex_on)
-            e = step_nodal(sd.src, 0, 0, P + (jc + 1) * (long long)gi.px + kXOff + ic, e, n);
+            e = step_nodal(sd.src, 0, 0, P + (jc + 1) * (long long)gi.px + gi.xoff + ic, e, n);
         }
         ex_n = e;
         if (wr && ex_on) st_at(plane_base(fo_at(a, go, 0), P), ob, e);
@@ -1801,7 +1801,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
         if (SRC) {
           if (be1) e = faces_apply(lf, be1, ie1, sd.inc, sd.pwG[0], i, j, k, e);
           if (nd && ey_on)
-            e = step_nodal(sd.src, 0, 1, P + (jc + 1) * (long long)gi.px + kXOff + ic, e, n);
+            e = step_nodal(sd.src, 0, 1, P + (jc + 1) * (long long)gi.px + gi.xoff + ic, e, n);
         }
         ey_n = e;
         if (wr && ey_on) st_at(plane_base(fo_at(a, go, 1), P), ob, e);
@@ -1825,7 +1825,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
         if (SRC) {
           if (be2) e = faces_apply(lf, be2, ie2, sd.inc, sd.pwG[0], i, j, k, e);
           if (nd && ez_on)
-            e = step_nodal(sd.src, 0, 2, P + (jc + 1) * (long long)gi.px + kXOff + ic, e, n);
+            e = step_nodal(sd.src, 0, 2, P + (jc + 1) * (long long)gi.px + gi.xoff + ic, e, n);
         }
         ez_n = e;
         if (wr && ez_on) st_at(plane_base(fo_at(a, go, 2), P), ob, e);
@@ -1874,7 +1874,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
         if (SRC) {
           if (bh0) h = faces_apply(lf + np0, bh0, ih0, sd.inc, sd.pwG[1], i, j, kh, h);
           if (nd && hxk)
-            h = step_nodal(sd.src, 1, 3, Ph + (jc + 1) * (long long)gi.px + kXOff + ic, h, n);
+            h = step_nodal(sd.src, 1, 3, Ph + (jc + 1) * (long long)gi.px + gi.xoff + ic, h, n);
         }
         if (hxk) st_at(plane_base(fo_at(a, go, 3), Ph), ob, h);
       }
@@ -1903,7 +1903,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
         if (SRC) {
           if (bh1) h = faces_apply(lf + np0, bh1, ih1, sd.inc, sd.pwG[1], i, j, kh, h);
           if (nd && hyk)
-            h = step_nodal(sd.src, 1, 4, Ph + (jc + 1) * (long long)gi.px + kXOff + ic, h, n);
+            h = step_nodal(sd.src, 1, 4, Ph + (jc + 1) * (long long)gi.px + gi.xoff + ic, h, n);
         }
         if (hyk) st_at(plane_base(fo_at(a, go, 4), Ph), ob, h);
       }
@@ -1930,7 +1930,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
         if (SRC) {
           if (bh2) h = faces_apply(lf + np0, bh2, ih2, sd.inc, sd.pwG[1], i, j, kh, h);
           if (nd && hzk)
-            h = step_nodal(sd.src, 1, 5, Ph + (jc + 1) * (long long)gi.px + kXOff + ic, h, n);
+            h = step_nodal(sd.src, 1, 5, Ph + (jc + 1) * (long long)gi.px + gi.xoff + ic, h, n);
         }
         if (hzk) st_at(plane_base(fo_at(a, go, 5), Ph), ob, h);
       }
@@ -2056,9 +2056,9 @@ __device__ __forceinline__ void step4_tile(const StepDev &sd, const StepTile tl)
   const int ic = min(i, (nx + 3) & ~3), jc = min(j, ny);
   const long long pxy = gi.pxy;
   const unsigned pxb = (unsigned)gi.px * 4u;
-  const unsigned ob = (unsigned)(jc + 1) * pxb + (unsigned)(kXOff + ic) * 4u;
+  const unsigned ob = (unsigned)(jc + 1) * pxb + (unsigned)(gi.xoff + ic) * 4u;
   const unsigned oby = ob - pxb;
-  const unsigned oe = (unsigned)(jc + 1) * pxb + (unsigned)(kXOff + x0 - 1) * 4u;  // x0 - 1
+  const unsigned oe = (unsigned)(jc + 1) * pxb + (unsigned)(gi.xoff + x0 - 1) * 4u;  // x0 - 1
   float idxh[4], idxe[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {

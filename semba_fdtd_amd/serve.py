@@ -50,7 +50,7 @@ class Result:
 def device_bytes_estimate(pb) -> int:
     """Device memory of one resident grid before it is created: fields and
     media indices over the padded grid, plus CPML and pole state."""
-    px = (32 + pb.nx + 2 + 31) // 32 * 32
+    px = (32 + pb.nx + 8 + 31) // 32 * 32 if pb.nx >= 256 else (4 + pb.nx + 8 + 3) // 4 * 4
     plane = px * (pb.ny + 3)
     base = plane * (pb.nz + 3) * (6 * 4 + 6 * 2)
     return int(base * 1.25) + (1 << 20)

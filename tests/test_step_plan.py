@@ -70,7 +70,9 @@ def needs(n, L, boxes, lo, hi):
 def check(n, L, R, kc, boxes, rows=None, vec=1, vrows=8):
     tiles, rows = plan(n, L, R, kc, boxes, rows, vec, vrows)
     nx, ny, nz = n
-    px = (32 + nx + 8 + 31) // 32 * 32
+    # row layout of fdtd_create: 32-element lead-in, 128-byte pitch from 256
+    # cells wide; below that a 4-element lead-in and a 16-byte pitch
+    xoff, px = (32, (32 + nx + 8 + 31) // 32 * 32) if nx >= 256 else (4, (4 + nx + 8 + 3) // 4 * 4)
     own = np.zeros((nz + 1, ny + 1, nx + 1), np.uint8)
     yee = 0
     for cls, x0, wx, y0, wy, k0, k1 in tiles:
@@ -89,7 +91,7 @@ def check(n, L, R, kc, boxes, rows=None, vec=1, vrows=8):
             assert wy <= e - 1
             # lane l loads 16 bytes at min(x0 + 4l, (nx + 3) & ~3): inside the row
             ic = np.minimum(x0 + 4 * np.arange(KWAVE), (nx + 3) & ~3)
-            assert (ic % 4 == 0).all() and (32 + ic + 3 < px).all()
+            assert (ic % 4 == 0).all() and (xoff + ic + 3 < px).all()
             assert wx // 4 < KWAVE                   # the halo lane exists
         else:
             assert wx <= KWAVE - 1                   # lane wx is the halo lane
