@@ -37,6 +37,12 @@ CASES = {
     "one_sided_cpml": ("scatterer_problem", dict(nsteps=200, n=(70, 24, 22),
                                                  L=((5, 0), (0, 6), (4, 4)), refl=1e-3)),
     "wide_x": ("scatterer_problem", dict(nsteps=120, n=(131, 9, 11), L=((6, 7), (2, 2), (3, 2)))),
+    # the two row layouts: under 256 cells wide rows start 4 elements in with
+    # a 16-byte pitch, from 256 cells 32 in with a 128-byte pitch
+    "rows_narrow_255": ("scatterer_problem", dict(nsteps=120, n=(255, 14, 13),
+                                                  L=((6, 5), (3, 3), (3, 3)))),
+    "rows_wide_256": ("scatterer_problem", dict(nsteps=120, n=(256, 14, 13),
+                                                L=((6, 5), (3, 3), (3, 3)))),
     "planewave_z": ("open_scatterer_problem", dict(n=32, L=6, radius=5, nsteps=200, spread=10,
                                                    delay=40)),
     "planewave_oblique_graded": ("open_scatterer_problem",

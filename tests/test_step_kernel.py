@@ -21,6 +21,11 @@ CASES = {
                                                     L=((5, 0), (0, 6), (4, 4)), graded_z=1.03),
     "wide_x": lambda: P.scatterer_problem(nsteps=80, n=(131, 9, 11),
                                           L=((6, 7), (2, 2), (3, 2))),
+    # either side of the switch between the narrow and the wide row layout
+    "rows_narrow_255": lambda: P.scatterer_problem(nsteps=61, n=(255, 14, 13),
+                                                   L=((6, 5), (3, 3), (3, 3))),
+    "rows_wide_256": lambda: P.scatterer_problem(nsteps=61, n=(256, 14, 13),
+                                                 L=((6, 5), (3, 3), (3, 3))),
     "planewave_oblique": lambda: P.open_scatterer_problem(
         n=30, L=5, radius=4, nsteps=101, spread=8, delay=30, grading=1.05,
         direction=(0.3, -0.5, 0.8), polar=(0.0, 0.848, 0.53)),
