@@ -88,7 +88,8 @@ def onelaunch_cell_bytes(pb):
 def onelaunch_step_bytes(pb, ndisp_edges, mode, stored_poles):
     """Compulsory HBM bytes of one whole step when E and H of a plane are
     updated in the same launch: onelaunch_cell_bytes per cell, 8 B per psi
-    value and per dispersive edge 8 B of Eprev plus the pole state."""
+    value and per dispersive edge 8 B of Eprev plus the pole state (an upper
+    bound: tiles that lie in one medium read no media indices)."""
     pe, ph = psi_counts(pb)
     per_state = 16 if mode == "fast" else 32
     return (onelaunch_cell_bytes(pb) * pb.cells + 8 * (pe + ph)
@@ -564,11 +565,13 @@ def main():
                             "gbs": round(cbytes[cls] / (ms_c / 1e3) / 1e9, 1)})
         dom = max(ms_k, key=ms_k.get)
         KERNELS["step_plain"] += f", {cb} B per owned cell"
-        KERNELS["step_plain4"] += f", {cb} B per owned cell"
+        KERNELS["step_plain4"] += (f", {cb} B per owned cell, 48 B in single-medium tiles "
+                                   "(no media indices read)")
         if dom not in ("step_plain", "step_plain4"):
             KERNELS[dom] = (f"k_step tile class {dom[5:]} ({cb} B per owned cell, 8 B per psi "
                             f"value, 8 + 16 B per pole-state cell)")
-        step_bytes = onelaunch_step_bytes(pb, ndisp, args.mode, stored)
+        # the classes' own compulsory bytes (single-medium tiles read no media)
+        step_bytes = int(sum(cbytes[c] for c, _, _ in timed_classes))
     else:
         dom = max(ms_k, key=ms_k.get)
         step_bytes = step_algorithmic_bytes(pb, ndisp, args.mode, stored)

@@ -127,6 +127,39 @@ int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3],
   return (int)hipGetLastError();
 }
 
+// One block per four-cell step tile: does every cell it reads (owned, the
+// halo column x0 + wx and row y0 + wy, planes k0..k1, inside the grid) hold
+// the same medium in all six media arrays?  out: that medium, else -1.
+__global__ void __launch_bounds__(256) k_tile_media(DevGrid g, const StepTile *tiles, int *out) {
+  const StepTile t = tiles[blockIdx.x];
+  const int x0 = t.x & 0xfffff, wx = t.x >> 20, y0 = t.y & 0xfffff, wy = t.y >> 20;
+  const int k0 = t.k0, k1 = (int)((unsigned)t.k1 & kTileK1Mask);
+  const int xe = min(x0 + wx, g.nx), ye = min(y0 + wy, g.ny), ke = min(k1, g.nz);
+  const long long X = xe - x0 + 1, Y = ye - y0 + 1, Z = ke - k0 + 1;
+  __shared__ int bad;
+  const int m0 = g.mi[0][fidx(g, x0, y0, k0)];
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const long long n = X * Y * Z;
+  int b = 0;
+  for (long long q = threadIdx.x; q < n && !b; q += blockDim.x) {
+    const int x = x0 + (int)(q % X), y = y0 + (int)((q / X) % Y), z = k0 + (int)(q / (X * Y));
+    const long long p = fidx(g, x, y, z);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) b |= g.mi[c][p] != m0;
+  }
+  if (b) atomicOr(&bad, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = bad ? -1 : m0;
+}
+
+int launch_tile_media(const DevGrid &g, const StepTile *tiles, int n, int *out, void *stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_tile_media, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, g,
+                     tiles, out);
+  return (int)hipGetLastError();
+}
+
 // The same box read out of a field array as fp32: binary16 values times the
 // array's scale (one rounding, as the host conversion), fp32 values as they
 // are.

@@ -154,6 +154,7 @@ struct fdtd_handle {
   // slice of the device tile list
   std::vector<int> step_cls, step_cnt, step_off;
   std::vector<int> step_crows;           // rows per block of each class launch
+  std::vector<int> step_uni;             // leading single-medium tiles of each class launch
   std::vector<double> step_cells;        // Yee cells owned per class
   std::vector<double> step_bytes;        // compulsory HBM bytes of one launch per class
   uint8_t *mi8[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // step mode

@@ -619,6 +619,46 @@ int build_step(fdtd_handle *h) {
     h->dtiles = nullptr;
     if (!all.empty() && (rc = upload(h, &h->dtiles, all.data(), all.size(), &h->step_small)))
       return rc;
+    // four-cell tiles whose cells all hold one medium (every component, the
+    // halo included) skip the media loads: they lead their class's tiles,
+    // the medium packed above k1's plane bits, and their compulsory bytes
+    // lose the media indices
+    h->step_uni.assign(h->step_cls.size(), 0);
+    bool packed = false;
+    for (size_t q = 0; q < h->step_cls.size(); ++q) {
+      if (h->step_cls[q] != kPlanVec || h->nmedia + 1 > (1 << (32 - kTileMediumShift)))
+        continue;
+      const int n = h->step_cnt[q], off = h->step_off[q];
+      int *dm = nullptr;
+      if ((rc = dalloc(h, (void **)&dm, (size_t)n * 4, &h->step_small))) return rc;
+      if (launch_tile_media(h->g, h->dtiles + off, n, dm, h->stream))
+        return fail(h, FDTD_ERR_HIP, "tile media launch failed");
+      std::vector<int> m((size_t)n);
+      HIPCHK(h, hipMemcpyAsync(m.data(), dm, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      std::vector<StepTile> uni, mixed;
+      double uowned = 0.0;
+      for (int t = 0; t < n; ++t) {
+        StepTile s = all[(size_t)off + (size_t)t];
+        if (m[(size_t)t] < 0) { mixed.push_back(s); continue; }
+        const int x0 = s.x & 0xfffff, wx = s.x >> 20, y0 = s.y & 0xfffff, wy = s.y >> 20;
+        uowned += (double)std::max(0, std::min(x0 + wx, h->g.nx) - x0) *
+                  std::max(0, std::min(y0 + wy, h->g.ny) - y0) *
+                  std::max(0, std::min(s.k1, h->g.nz) - s.k0);
+        s.k1 = (int)((unsigned)s.k1 | ((unsigned)m[(size_t)t] << kTileMediumShift));
+        uni.push_back(s);
+      }
+      std::copy(uni.begin(), uni.end(), all.begin() + off);
+      std::copy(mixed.begin(), mixed.end(), all.begin() + off + (long)uni.size());
+      h->step_uni[q] = (int)uni.size();
+      h->step_bytes[q] -= 6.0 * (narrow ? 1.0 : 2.0) * uowned;
+      packed = packed || !uni.empty();
+    }
+    if (packed) {
+      HIPCHK(h, hipMemcpyAsync(h->dtiles, all.data(), all.size() * sizeof(StepTile),
+                               hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
     h->step_tiles = all;
   }
   src.waves = h->dwaves;
@@ -697,8 +737,10 @@ static StepDev class_sd(const fdtd_handle *h, int dir, size_t q) {
 }
 
 static int launch_class(const fdtd_handle *h, const StepDev &sd, size_t q) {
-  return h->mode == FDTD_MODE_EXACT ? exact::launch_step(sd, h->step_crows[q], h->step_cls[q], h->stream)
-                                    : fast::launch_step(sd, h->step_crows[q], h->step_cls[q], h->stream);
+  const int nuni = h->step_uni[q];
+  return h->mode == FDTD_MODE_EXACT
+             ? exact::launch_step(sd, h->step_crows[q], h->step_cls[q], nuni, h->stream)
+             : fast::launch_step(sd, h->step_crows[q], h->step_cls[q], nuni, h->stream);
 }
 
 static int launch_step_kernels(fdtd_handle *h, int dir, bool exact_prev = false) {

@@ -237,6 +237,11 @@ struct StepAux {
 struct StepTile {
   int x, y, k0, k1;
 };
+// k1 holds the end plane in its low 20 bits; a four-cell tile whose cells
+// all have one medium (every component, halo included) carries that medium
+// index above them
+constexpr unsigned kTileK1Mask = 0xfffffu;
+constexpr int kTileMediumShift = 20;
 #ifdef __HIPCC__
 __host__ __device__
 #endif
@@ -361,7 +366,7 @@ __device__ __forceinline__ float nodal_apply(float v, const NodalCell &c, float 
   int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4,      \
                int yee, int cpml, int ade, void *stream);                    \
   size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc);       \
-  int launch_step(const StepDev &sd, int rows, int cls, void *stream);       \
+  int launch_step(const StepDev &sd, int rows, int cls, int nuni, void *stream); \
   int launch_step_many(const StepDev *sds, const GridTile *tiles, int n, int narrow, int rows, \
                        int cls, size_t lds, void *stream);                   \
   size_t step4_lds_bytes(const DevGrid &g, int rows);                       \
@@ -394,6 +399,9 @@ int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3],
                     long long pxy, int xoff, int to_grid, void *stream);
 // the same box read out of a field array (es 4: fp32, es 2: binary16 times
 // scale) into a packed fp32 buffer
+// the medium of each four-cell step tile whose cells (owned, halo row and
+// column, planes k0..k1) hold one medium in all six media arrays, else -1
+int launch_tile_media(const DevGrid &g, const StepTile *tiles, int n, int *out, void *stream);
 int launch_box_gather_f32(const void *grid, size_t es, float scale, float *packed,
                           const long long ext[3], long long px, long long pxy, int xoff,
                           void *stream);

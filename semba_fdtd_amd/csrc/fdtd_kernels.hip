@@ -1555,7 +1555,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
   // lanes past the grid's edge address the edge cell (their results are
   // never used): every load and table index stays inside its array
   const int ic = min(i, nx), jc = min(j, ny);
-  const int k1 = min(tl.k1, nz + 1);
+  const int k1 = min((int)((unsigned)tl.k1 & kTileK1Mask), nz + 1);
   const long long pxy = gi.pxy;
   const unsigned pxb = (unsigned)gi.px * 4u;
   // byte offsets inside a plane: the cell and its x - 1 and y - 1 neighbours;
@@ -2024,7 +2024,9 @@ static __host__ __device__ __forceinline__ size_t step4_xch_offset(int nmedia, i
   return (step_tab_bytes(nmedia, nd0) + step_tab_bytes(nmedia, nd1) + 15) & ~(size_t)15;
 }
 
-template <int R, class MT>
+// UNI: every cell the tile reads holds one medium (carried in k1's high
+// bits): no media loads, one coefficient pair per field for the whole tile
+template <int R, class MT, bool UNI = false>
 __device__ __forceinline__ void step4_tile(const StepDev &sd, const StepTile tl) {
   typedef Media4<MT> M4;
   typedef typename M4::raw raw;
@@ -2050,7 +2052,8 @@ __device__ __forceinline__ void step4_tile(const StepDev &sd, const StepTile tl)
   const int j = y0 + min(row, wy);
   const int i = x0 + 4 * min(lane, wx >> 2);
   const bool own = 4 * lane < wx && row < wy && j <= ny;
-  const int k1 = min(tl.k1, nz + 1);
+  const int k1 = min((int)((unsigned)tl.k1 & kTileK1Mask), nz + 1);
+  const int m0 = UNI ? (int)((unsigned)tl.k1 >> kTileMediumShift) : 0;
   // lanes past the grid's edge address the last whole vector of the row
   // (their results are never used): every access stays inside its array
   const int ic = min(i, (nx + 3) & ~3), jc = min(j, ny);
@@ -2096,14 +2099,16 @@ __device__ __forceinline__ void step4_tile(const StepDev &sd, const StepTile tl)
       ex = ld_at(reinterpret_cast<const SFD_GLOBAL f32x4 *>(plane_base(gi.f[0], P)), ob);
       ey = ld_at(reinterpret_cast<const SFD_GLOBAL f32x4 *>(plane_base(gi.f[1], P)), ob);
       ez = ld_at(reinterpret_cast<const SFD_GLOBAL f32x4 *>(plane_base(gi.f[2], P)), ob);
-      const MT *mi[6];
+      if constexpr (!UNI) {
+        const MT *mi[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        if constexpr (sizeof(MT) == 1) mi[c] = sd.mi8[c];
-        else mi[c] = gi.mi[c];
+        for (int c = 0; c < 6; ++c) {
+          if constexpr (sizeof(MT) == 1) mi[c] = sd.mi8[c];
+          else mi[c] = gi.mi[c];
+        }
+        mx = M4::ld(mi[0], P, ob); my = M4::ld(mi[1], P, ob); mz = M4::ld(mi[2], P, ob);
+        mhx = M4::ld(mi[3], P, ob); mhy = M4::ld(mi[4], P, ob); mhz = M4::ld(mi[5], P, ob);
       }
-      mx = M4::ld(mi[0], P, ob); my = M4::ld(mi[1], P, ob); mz = M4::ld(mi[2], P, ob);
-      mhx = M4::ld(mi[3], P, ob); mhy = M4::ld(mi[4], P, ob); mhz = M4::ld(mi[5], P, ob);
       hze = ld_uniform(gi.f[5], P, oe);
       hye = ld_uniform(gi.f[4], P, oe);
     }
@@ -2124,8 +2129,8 @@ __device__ __forceinline__ void step4_tile(const StepDev &sd, const StepTile tl)
       float rx[4], ry[4], rz[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const float2 cx = te.gg[M4::at(mx, c)], cy = te.gg[M4::at(my, c)],
-                     cz = te.gg[M4::at(mz, c)];
+        const float2 cx = te.gg[UNI ? m0 : M4::at(mx, c)], cy = te.gg[UNI ? m0 : M4::at(my, c)],
+                     cz = te.gg[UNI ? m0 : M4::at(mz, c)];
         rx[c] = yee_upd(cx.x, EX[c], cx.y, curl2(HZ[c] - HZJ[c], idyh, HY[c] - HYK[c], idzh));
         ry[c] = yee_upd(cy.x, EY[c], cy.y, curl2(HX[c] - HXK[c], idzh, HZ[c] - hzm[c], idxh[c]));
         rz[c] = yee_upd(cz.x, EZ[c], cz.y, curl2(HY[c] - hym[c], idxh[c], HX[c] - HXJ[c], idyh));
@@ -2160,8 +2165,9 @@ __device__ __forceinline__ void step4_tile(const StepDev &sd, const StepTile tl)
       float rx[4], ry[4], rz[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const float2 cx = th.gg[M4::at(mhx_c, c)], cy = th.gg[M4::at(mhy_c, c)],
-                     cz = th.gg[M4::at(mhz_c, c)];
+        const float2 cx = th.gg[UNI ? m0 : M4::at(mhx_c, c)],
+                     cy = th.gg[UNI ? m0 : M4::at(mhy_c, c)],
+                     cz = th.gg[UNI ? m0 : M4::at(mhz_c, c)];
         rx[c] = yee_upd(cx.x, HXC[c], cx.y, curl2(EYN[c] - EYC[c], idze, EZJ[c] - EZC[c], idye));
         ry[c] = yee_upd(cy.x, HYC[c], cy.y, curl2(EZP[c] - EZC[c], idxe[c], EXN[c] - EXC[c], idze));
         rz[c] = yee_upd(cz.x, HZC[c], cz.y, curl2(EXJ[c] - EXC[c], idye, EYP[c] - EYC[c], idxe[c]));
@@ -2186,9 +2192,9 @@ __device__ __forceinline__ void step4_tile(const StepDev &sd, const StepTile tl)
   }
 }
 
-template <int R, class MT>
+template <int R, class MT, bool UNI>
 __global__ void __launch_bounds__(64 * R) k_step4(StepDev sd) {
-  step4_tile<R, MT>(sd, sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)]);
+  step4_tile<R, MT, UNI>(sd, sd.tiles[xcd_remap(blockIdx.x, sd.tm.nblk)]);
 }
 
 template <int R, class MT>
@@ -2206,16 +2212,29 @@ size_t step4_lds_bytes(const DevGrid &g, int rows) {
 // The step kernels of one class over n blocks: the tiles of one grid
 // (sds == nullptr: k_step / k_step4 on sd) or tiles of many grids (k_step_many
 // / k_step4_many on sds and bt).  narrow: media read from the 8-bit copies.
+// one grid: the nuni single-medium tiles that lead the class's tiles, then
+// the rest; a batch's tiles all run the general kernel
 static int launch_step4_any(const StepDev &sd, const StepDev *sds, const GridTile *bt, int n,
-                            int narrow, int rows, size_t lds, hipStream_t s) {
+                            int narrow, int rows, size_t lds, hipStream_t s, int nuni = 0) {
   const dim3 grid((unsigned)n);
   if (n <= 0) return 0;
+  nuni = sds ? 0 : std::min(std::max(nuni, 0), n);
+  StepDev su = sd, sm = sd;
+  su.tm.nblk = nuni;
+  sm.tiles = sd.tiles + nuni;
+  sm.tm.nblk = n - nuni;
 #define SFD_STEP4_M(RR, M)                                                                    \
   do {                                                                                        \
-    if (sds)                                                                                  \
+    if (sds) {                                                                                \
       hipLaunchKernelGGL((k_step4_many<RR, M>), grid, dim3(kWave, RR), lds, s, sds, bt, n);  \
-    else                                                                                      \
-      hipLaunchKernelGGL((k_step4<RR, M>), grid, dim3(kWave, RR), lds, s, sd);               \
+    } else {                                                                                  \
+      if (nuni > 0)                                                                           \
+        hipLaunchKernelGGL((k_step4<RR, M, true>), dim3((unsigned)nuni), dim3(kWave, RR), lds, \
+                           s, su);                                                            \
+      if (n > nuni)                                                                           \
+        hipLaunchKernelGGL((k_step4<RR, M, false>), dim3((unsigned)(n - nuni)),              \
+                           dim3(kWave, RR), lds, s, sm);                                      \
+    }                                                                                         \
   } while (0)
 #define SFD_STEP4(RR)                    \
   do {                                   \
@@ -2246,8 +2265,9 @@ size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc) {
 // class cls: 0-7 the k_step variant (bits 1 CPML, 2 poles, 4 sources), 8 the
 // four-cell kernel, 9-16 the half-width variant of class cls - 9
 static int launch_step_any(const StepDev &sd, const StepDev *sds, const GridTile *bt, int n,
-                           int narrow, int rows, int cls, size_t lds, hipStream_t s) {
-  if (cls == 8) return launch_step4_any(sd, sds, bt, n, narrow, rows, lds, s);
+                           int narrow, int rows, int cls, size_t lds, hipStream_t s,
+                           int nuni = 0) {
+  if (cls == 8) return launch_step4_any(sd, sds, bt, n, narrow, rows, lds, s, nuni);
   const dim3 grid((unsigned)n);
   if (n <= 0) return 0;
   const bool half = cls >= 9;
@@ -2299,11 +2319,11 @@ static int launch_step_any(const StepDev &sd, const StepDev *sds, const GridTile
   return (int)hipGetLastError();
 }
 
-int launch_step(const StepDev &sd, int rows, int cls, void *stream) {
+int launch_step(const StepDev &sd, int rows, int cls, int nuni, void *stream) {
   const size_t lds = cls == 8 ? step4_lds_bytes(sd.gi, rows)
                               : step_lds_bytes(sd.gi, rows, sd.npw[0] + sd.npw[1], sd.tm.kc);
   return launch_step_any(sd, nullptr, nullptr, sd.tm.nblk, sd.narrow, rows, cls, lds,
-                         (hipStream_t)stream);
+                         (hipStream_t)stream, nuni);
 }
 
 int launch_step_many(const StepDev *sds, const GridTile *tiles, int n, int narrow, int rows,
