@@ -91,7 +91,7 @@ def serve_plain(requests, mode: str = "fast", storage: str = "fp32") -> List[Res
 
 def serve_batched(requests, mode: str = "fast", storage: str = "fp32",
                   reserve: int = 8 << 30, max_chunk: int = 250, rows: int = 4,
-                  zchunk: int = 32, blocks: int = 0, xcells: int = 1, step_rows: int = 0,
+                  zchunk: int = 16, blocks: int = 0, xcells: int = 1, step_rows: int = 0,
                   max_cells: Optional[int] = None, device: int = 0,
                   cancel: Optional[Callable[[str], bool]] = None,
                   check_finite: bool = True) -> List[Result]:
