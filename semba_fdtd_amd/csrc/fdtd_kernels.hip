@@ -425,7 +425,13 @@ template <> struct Store<float> {
   static __device__ __forceinline__ void st4(float *b, long long p, float, const F4 &v) {
     *reinterpret_cast<float4 *>(b + p) = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
   }
-  // a plane base and the lane's byte offset in the plane
+  // a plane base and the lane's byte offset in the plane; ldr/cvt split the
+  // load from the conversion so that a plane's loads all issue first
+  typedef float raw_t;
+  static __device__ __forceinline__ raw_t ldr(const SFD_GLOBAL char *b, unsigned ob) {
+    return ld_at((const SFD_GLOBAL float *)b, ob);
+  }
+  static __device__ __forceinline__ float cvt(raw_t r, float) { return r; }
   static __device__ __forceinline__ float ldo(const SFD_GLOBAL char *b, unsigned ob, float) {
     return ld_at((const SFD_GLOBAL float *)b, ob);
   }
@@ -471,9 +477,15 @@ template <> struct Store<__half> {
     t.y = *reinterpret_cast<const unsigned *>(&hi);
     *reinterpret_cast<uint2 *>(reinterpret_cast<__half *>(b) + p) = t;
   }
-  static __device__ __forceinline__ float ldo(const SFD_GLOBAL char *b, unsigned ob, float s) {
-    const unsigned short h = ld_at((const SFD_GLOBAL unsigned short *)b, ob);
+  typedef unsigned short raw_t;
+  static __device__ __forceinline__ raw_t ldr(const SFD_GLOBAL char *b, unsigned ob) {
+    return ld_at((const SFD_GLOBAL unsigned short *)b, ob);
+  }
+  static __device__ __forceinline__ float cvt(raw_t h, float s) {
     return rounded(__half2float(__ushort_as_half(h)) * s);
+  }
+  static __device__ __forceinline__ float ldo(const SFD_GLOBAL char *b, unsigned ob, float s) {
+    return cvt(ldr(b, ob), s);
   }
   static __device__ __forceinline__ void sto(SFD_GLOBAL char *b, unsigned ob, float is, float v) {
     st_at((SFD_GLOBAL unsigned short *)b, ob, __half_as_ushort(__float2half_rn(rounded(v * is))));
@@ -562,23 +574,27 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
     const long long kb = (long long)(k + 1) * fpb, kmb = (long long)(k + 1) * mpb;
     // ---- issue the independent loads of this plane
     const float idzh = ro_uniform(g.id[5], k + 1);
-    float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f, hz_jm = 0.0f, hx_jm = 0.0f, hz_im = 0.0f,
-          hy_im = 0.0f;
+    // raw loads first (binary16 storage converts after every load of the
+    // plane has issued, outside the branches)
+    typename S::raw_t rhx = 0, rhy = 0, rhz = 0, rhz_jm = 0, rhx_jm = 0, rhz_im = 0, rhy_im = 0;
     if (NEEDH) {
       const SFD_GLOBAL char *hxk = plane_base(bH[0], kb), *hyk = plane_base(bH[1], kb),
                             *hzk = plane_base(bH[2], kb);
-      hx0 = S::ldo(hxk, of, g.fs[3]); hy0 = S::ldo(hyk, of, g.fs[4]);
-      hz0 = S::ldo(hzk, of, g.fs[5]);
-      hz_jm = S::ldo(hzk, of - fy, g.fs[5]); hx_jm = S::ldo(hxk, of - fy, g.fs[3]);
-      hz_im = S::ldo(hzk, of - ES, g.fs[5]); hy_im = S::ldo(hyk, of - ES, g.fs[4]);
+      rhx = S::ldr(hxk, of); rhy = S::ldr(hyk, of); rhz = S::ldr(hzk, of);
+      rhz_jm = S::ldr(hzk, of - fy); rhx_jm = S::ldr(hxk, of - fy);
+      rhz_im = S::ldr(hzk, of - ES); rhy_im = S::ldr(hyk, of - ES);
     }
     SFD_GLOBAL char *const exk = plane_base(bE[0], kb), *const eyk = plane_base(bE[1], kb),
                          *const ezk = plane_base(bE[2], kb);
+    // binary16: every lane loads all three components (i <= nx, j <= ny,
+    // k <= nz lie inside each array's Alloc box), so that no conversion
+    // waits for its load inside a branch; fp32 loads what the sweeps use
+    constexpr bool ALL = sizeof(T) == 2;
     int mx = 0, my = 0, mz = 0;
-    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    if (cx) { mx = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); ex = S::ldo(exk, of, g.fs[0]); }
-    if (cy) { my = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); ey = S::ldo(eyk, of, g.fs[1]); }
-    if (cz) { mz = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); ez = S::ldo(ezk, of, g.fs[2]); }
+    typename S::raw_t rex = 0, rey = 0, rez = 0;
+    if (ALL || cx) { mx = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); rex = S::ldr(exk, of); }
+    if (ALL || cy) { my = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); rey = S::ldr(eyk, of); }
+    if (ALL || cz) { mz = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); rez = S::ldr(ezk, of); }
     int qzn = 0;
     const bool szn = CPML && slab_node(k, nz, g.L[2][0], g.L[2][1], qzn);
     float bzp = 0.0f, ccz = 0.0f, vxy = 0.0f, vxz = 0.0f, vyz = 0.0f, vyx = 0.0f, vzy = 0.0f,
@@ -609,6 +625,14 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
       if (cz && syn) vzy = ld_at(sEzy, oE);
       if (cz && sxn) vzx = ld_at(sEzx, oD);
     }
+    float hx0 = 0.0f, hy0 = 0.0f, hz0 = 0.0f, hz_jm = 0.0f, hx_jm = 0.0f, hz_im = 0.0f,
+          hy_im = 0.0f;
+    if (NEEDH) {
+      hx0 = S::cvt(rhx, g.fs[3]); hy0 = S::cvt(rhy, g.fs[4]); hz0 = S::cvt(rhz, g.fs[5]);
+      hz_jm = S::cvt(rhz_jm, g.fs[5]); hx_jm = S::cvt(rhx_jm, g.fs[3]);
+      hz_im = S::cvt(rhz_im, g.fs[5]); hy_im = S::cvt(rhy_im, g.fs[4]);
+    }
+    const float ex = S::cvt(rex, g.fs[0]), ey = S::cvt(rey, g.fs[1]), ez = S::cvt(rez, g.fs[2]);
     // ---- compute; psi <- b*psi + c*curl, then E += +-G2*psi
     if (cx) {
       float e = ex;
@@ -725,17 +749,20 @@ __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
     const SFD_GLOBAL char *exn = plane_base(bE[0], kb + fpb), *eyn = plane_base(bE[1], kb + fpb);
     const SFD_GLOBAL char *exk = plane_base(bE[0], kb), *eyk = plane_base(bE[1], kb),
                           *ezk = plane_base(bE[2], kb);
-    const float ex_kp = S::ldo(exn, of, g.fs[0]), ey_kp = S::ldo(eyn, of, g.fs[1]);
-    const float ez_c = S::ldo(ezk, of, g.fs[2]);
-    const float ez_jp = S::ldo(ezk, of + fy, g.fs[2]), ex_jp = S::ldo(exk, of + fy, g.fs[0]);
-    const float ez_ip = S::ldo(ezk, of + ES, g.fs[2]), ey_ip = S::ldo(eyk, of + ES, g.fs[1]);
+    // raw loads first (converted once every load of the plane has issued)
+    const typename S::raw_t rex_kp = S::ldr(exn, of), rey_kp = S::ldr(eyn, of);
+    const typename S::raw_t rez_c = S::ldr(ezk, of);
+    const typename S::raw_t rez_jp = S::ldr(ezk, of + fy), rex_jp = S::ldr(exk, of + fy);
+    const typename S::raw_t rez_ip = S::ldr(ezk, of + ES), rey_ip = S::ldr(eyk, of + ES);
     SFD_GLOBAL char *const hxk = plane_base(bH[0], kb), *const hyk = plane_base(bH[1], kb),
                          *const hzk = plane_base(bH[2], kb);
+    // binary16: all three components for every lane (inside each Alloc box)
+    constexpr bool ALL = sizeof(T) == 2;
     int mx = 0, my = 0, mz = 0;
-    float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-    if (ax) { mx = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); hx = S::ldo(hxk, of, g.fs[3]); }
-    if (ay) { my = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); hy = S::ldo(hyk, of, g.fs[4]); }
-    if (az) { mz = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); hz = S::ldo(hzk, of, g.fs[5]); }
+    typename S::raw_t rhx = 0, rhy = 0, rhz = 0;
+    if (ALL || ax) { mx = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); rhx = S::ldr(hxk, of); }
+    if (ALL || ay) { my = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); rhy = S::ldr(hyk, of); }
+    if (ALL || az) { mz = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); rhz = S::ldr(hzk, of); }
     int qzc = 0;
     const bool szc = CPML && k < nz && slab_cell(k, nz, g.L[2][0], g.L[2][1], qzc);
     float bzp = 0.0f, ccz = 0.0f, vxy = 0.0f, vxz = 0.0f, vyz = 0.0f, vyx = 0.0f, vzy = 0.0f,
@@ -766,6 +793,11 @@ __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
       if (az && syc) vzy = ld_at(sHzy, oE);
       if (az && sxc) vzx = ld_at(sHzx, oD);
     }
+    const float ex_kp = S::cvt(rex_kp, g.fs[0]), ey_kp = S::cvt(rey_kp, g.fs[1]);
+    const float ez_c = S::cvt(rez_c, g.fs[2]);
+    const float ez_jp = S::cvt(rez_jp, g.fs[2]), ex_jp = S::cvt(rex_jp, g.fs[0]);
+    const float ez_ip = S::cvt(rez_ip, g.fs[2]), ey_ip = S::cvt(rey_ip, g.fs[1]);
+    const float hx = S::cvt(rhx, g.fs[3]), hy = S::cvt(rhy, g.fs[4]), hz = S::cvt(rhz, g.fs[5]);
     if (ax) {
       const float2 c = coef<TBL>(tab, g.gm1, g.gm2, mx);
       float h = hx;
