@@ -159,7 +159,7 @@ int fdtd_add_planewave(fdtd_handle *h, const int32_t box[6], int32_t nmodes, con
  * 4) consecutive x cells of the fused passes.  No counterpart in the
  * reference (its loops are OpenMP over k); chosen by the autotuner
  * (semba_fdtd_amd.tuning) or left at the default 4 rows x 32 planes x 1 cell.
- * Binary16 storage always runs 4 cells per lane.  fdtd_get_tiling returns the
+ * Either storage runs either.  fdtd_get_tiling returns the
  * values as set (a chunk longer than the grid runs as the whole grid). */
 int fdtd_set_tiling(fdtd_handle *h, int32_t rows, int32_t zchunk, int32_t xcells);
 int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk, int32_t *xcells);

@@ -1518,7 +1518,7 @@ int fdtd_get_tiling(const fdtd_handle *h, int32_t *rows, int32_t *zchunk, int32_
   if (!h || !rows || !zchunk || !xcells) return FDTD_ERR_ARG;
   *rows = h->tm.bdy;
   *zchunk = h->tile_zchunk;
-  *xcells = h->g.fp16 ? 4 : h->tm4.vec;
+  *xcells = h->tm4.vec;
   return FDTD_OK;
 }
 

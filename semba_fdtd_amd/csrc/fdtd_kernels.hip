@@ -1133,18 +1133,18 @@ __global__ void __launch_bounds__(256) k_hpass4(DevGrid g, TileMap tm) {
   hpass4_tile<true, T>(g, tm, tab, blockIdx.x);
 }
 
-template <bool YEE, bool CPML, bool TBL>
+template <bool YEE, bool CPML, bool TBL, class T = float>
 __global__ void __launch_bounds__(256) k_epass(DevGrid g, TileMap tm) {
   extern __shared__ float2 lds_tab[];
   const MedTab tab = stage_tables<TBL, false>(g, g.g1, g.g2, lds_tab);
-  epass_tile<YEE, CPML, TBL>(g, tm, tab, blockIdx.x);
+  epass_tile<YEE, CPML, TBL, T>(g, tm, tab, blockIdx.x);
 }
 
-template <bool YEE, bool CPML, bool TBL>
+template <bool YEE, bool CPML, bool TBL, class T = float>
 __global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm) {
   extern __shared__ float2 lds_tab[];
   const MedTab tab = stage_tables<TBL, false>(g, g.gm1, g.gm2, lds_tab);
-  hpass_tile<YEE, CPML, TBL>(g, tm, tab, blockIdx.x);
+  hpass_tile<YEE, CPML, TBL, T>(g, tm, tab, blockIdx.x);
 }
 
 // ---- batches of independent grids ----------------------------------------
@@ -1285,11 +1285,15 @@ int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
     else                                                                       \
       hipLaunchKernelGGL((k_epass<Y, C, false>), dim3(nb), blk, 0, s, g, tm);   \
   } while (0)
-  if (yee && cpml && tbl && (tm4.vec == 4 || g.fp16)) {
+  if (yee && cpml && tbl && tm4.vec == 4) {
     // fused pass, four cells per lane (the CPML variant also serves grids
     // without CPML)
     if (g.fp16) hipLaunchKernelGGL(k_epass4<__half>, dim3(tm4.nblk), blk, lds, s, g, tm4);
     else hipLaunchKernelGGL(k_epass4<float>, dim3(tm4.nblk), blk, lds, s, g, tm4);
+  } else if (g.fp16 && yee && cpml && tbl) {
+    // binary16 storage, one cell per lane (the set-up admits binary16 only
+    // with the fused Yee + CPML pass and LDS tables)
+    hipLaunchKernelGGL((k_epass<true, true, true, __half>), dim3(nb), blk, lds, s, g, tm);
   } else if (yee && cpml) SFD_E(true, true);
   else if (yee) SFD_E(true, false);
   else if (cpml) SFD_E(false, true);
@@ -1312,9 +1316,11 @@ int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
     else                                                                       \
       hipLaunchKernelGGL((k_hpass<Y, C, false>), dim3(nb), blk, 0, s, g, tm);   \
   } while (0)
-  if (yee && cpml && tbl && (tm4.vec == 4 || g.fp16)) {
+  if (yee && cpml && tbl && tm4.vec == 4) {
     if (g.fp16) hipLaunchKernelGGL(k_hpass4<__half>, dim3(tm4.nblk), blk, lds, s, g, tm4);
     else hipLaunchKernelGGL(k_hpass4<float>, dim3(tm4.nblk), blk, lds, s, g, tm4);
+  } else if (g.fp16 && yee && cpml && tbl) {
+    hipLaunchKernelGGL((k_hpass<true, true, true, __half>), dim3(nb), blk, lds, s, g, tm);
   } else if (yee && cpml) SFD_H(true, true);
   else if (yee) SFD_H(true, false);
   else if (cpml) SFD_H(false, true);

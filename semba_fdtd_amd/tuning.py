@@ -100,8 +100,7 @@ def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: in
     Returns {"tiling": (rows, zchunk, xcells), "step": (rows, zchunk),
     "class_rows": [rows of each tile class] or None, "ms": ms per step of the
     choice, "table": [...], "step_table": [...]}; step rows 0 means the two
-    passes won.  Binary16 storage always runs four cells
-    per lane and has no one-launch step.  ``keep_state=False`` skips the
+    passes won.  Binary16 storage has no one-launch step.  ``keep_state=False`` skips the
     checkpoint around the sweep: right for a grid at rest (all fields and pole
     state zero, which the source-free passes leave zero), where copying the
     state out and back would cost more than the sweep."""
@@ -120,8 +119,6 @@ def autotune(solver, candidates: Iterable[Tiling] = DEFAULT_CANDIDATES, reps: in
     try:
         solver.set_step_mode(0, 32)
         for rows, zchunk, xcells in candidates:
-            if fp16 and xcells != 4:
-                continue
             solver.set_tiling(rows, zchunk, xcells)
             e, h, ade = solver.time_halves(reps)
             table.append(((rows, zchunk, xcells), e, h, ade))

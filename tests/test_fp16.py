@@ -106,16 +106,22 @@ def test_fp16_storage_against_fp64(name, ns):
 @pytest.mark.parametrize("xcells", [1, 4])
 def test_fp16_batch_equals_alone(xcells):
     """Binary16 grids in one batch (persistent launch, flat one-cell tiles or
-    four-cell tiles) give the same bits as each grid advanced alone (the
-    four-cell passes)."""
+    four-cell tiles) give the same bits as each grid advanced alone, in the
+    one-cell and in the four-cell passes."""
     insts = [P.batch_instances()[q] for q in (2, 129, 248)]
     ns = 200
     alone = []
     for pb in insts:
-        s = sf.Solver(pb, mode="fast", storage="fp16", probe_buffer=ns)
-        s.run(ns)
-        alone.append(s.probes())
-        s.close()
+        runs = []
+        for xc in (1, 4):
+            s = sf.Solver(pb, mode="fast", storage="fp16", probe_buffer=ns)
+            s.set_tiling(4, 32, xc)
+            assert s.get_tiling() == (4, 32, xc)
+            s.run(ns)
+            runs.append(s.probes())
+            s.close()
+        assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))
+        alone.append(runs[0])
     sol = [sf.Solver(pb, mode="fast", storage="fp16", probe_buffer=ns) for pb in insts]
     b = Batch(xcells=xcells)
     for s in sol:
