@@ -393,8 +393,11 @@ int run_mur(fdtd_handle *h) {
 // other; fdtd_run advances pairs of steps A -> B -> A, so between calls the
 // state is always in A and every other entry point sees it there.
 bool step_supported(const fdtd_handle *h, int rows) {
-  // lanes address a plane by a 32-bit byte offset
-  if (h->g.fp16 || h->nmedia + 1 > 4096 || h->g.pxy * 4 >= (1LL << 31)) return false;
+  // lanes address a plane by a 32-bit byte offset; a tile's end plane fits
+  // the low bits of its k1 word (a single-medium tile's medium sits above)
+  if (h->g.fp16 || h->nmedia + 1 > 4096 || h->g.pxy * 4 >= (1LL << 31) ||
+      h->g.nz + 1 > (int)kTileK1Mask)
+    return false;
   if ((int)h->pw_faces[0].size() > kStepFaces || (int)h->pw_faces[1].size() > kStepFaces)
     return false;
   DevGrid g = h->g;
