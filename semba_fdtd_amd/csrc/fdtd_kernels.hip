@@ -129,12 +129,6 @@ __device__ __forceinline__ float psi_step(float *psi, long long s, float b,
   psi[s] = v;
   return v;
 }
-__device__ __forceinline__ float psi_step(SFD_GLOBAL float *psi, long long s, float b,
-                                          float d, float c) {
-  const float v = madd(d * c, b, psi[s]);
-  psi[s] = v;
-  return v;
-}
 
 // per-medium tables staged in LDS (TBL) or read from global memory
 struct MedTab {
