@@ -127,6 +127,50 @@ int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3],
   return (int)hipGetLastError();
 }
 
+// One block per step tile of a pole class without sources: the stored
+// previous field of every dispersive cell the tile owns (E and H, inside the
+// component's sweep and pole box) set to the cell's field, which is what the
+// step's pole update left there when it skipped the store.
+__global__ void __launch_bounds__(256) k_prev_settle(DevGrid g, const StepTile *tiles) {
+  const StepTile t = tiles[blockIdx.x];
+  const int x0 = t.x & 0xfffff, wx = t.x >> 20, y0 = t.y & 0xfffff, wy = t.y >> 20;
+  const int k0 = t.k0, k1 = (int)((unsigned)t.k1 & kTileK1Mask);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long long X = min(x0 + wx, nx + 1) - x0, Y = min(y0 + wy, ny + 1) - y0,
+                  Z = min(k1, nz + 1) - k0;
+  if (X <= 0 || Y <= 0 || Z <= 0) return;
+  const long long n = X * Y * Z;
+  for (long long q = threadIdx.x; q < n; q += blockDim.x) {
+    const int i = x0 + (int)(q % X), j = y0 + (int)((q / X) % Y), k = k0 + (int)(q / (X * Y));
+    const long long p = fidx(g, i, j, k);
+    for (int side = 0; side < 2; ++side) {
+      const AdeSet &A = g.ade[side];
+      if (!A.has) continue;
+      for (int c = 0; c < 3; ++c) {
+        const bool on = side == 0 ? (c == 0 ? i < nx : c == 1 ? j < ny : k < nz)
+                                  : (c == 0 ? (j < ny && k < nz)
+                                            : c == 1 ? (i < nx && k < nz) : (i < nx && j < ny));
+        const int *b = A.abox[c];
+        if (!on || i < b[0] || i >= b[0] + b[3] || j < b[1] || j >= b[1] + b[4] || k < b[2] ||
+            k >= b[2] + b[5])
+          continue;
+        const int comp = 3 * side + c;
+        if (A.dslot[g.mi[comp][p]] < 0) continue;
+        const long long zo = (long long)(k - b[2]) * b[3] * b[4];
+        const long long axy = (long long)(j - b[1]) * b[3] + (i - b[0]);
+        A.prev[c][zo + axy] = g.f[comp][p];
+      }
+    }
+  }
+}
+
+int launch_prev_settle(const DevGrid &g, const StepTile *tiles, int n, void *stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_prev_settle, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, g,
+                     tiles);
+  return (int)hipGetLastError();
+}
+
 // One block per four-cell step tile: does every cell it reads (owned, the
 // halo column x0 + wx and row y0 + wy, planes k0..k1, inside the grid) hold
 // the same medium in all six media arrays?  out: that medium, else -1.

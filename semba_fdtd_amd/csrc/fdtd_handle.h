@@ -163,6 +163,7 @@ struct fdtd_handle {
   // the stored previous field of the pole updates may differ from the field
   // (fields set from outside, granular calls): the next pair reads it
   bool prev_exact = true;
+  bool prev_lazy = false;      // the stored previous fields await settle_prev
   sfd::StepTile *dtiles = nullptr;
   std::vector<sfd::StepTile> step_tiles;  // host copy of the tile list (batches)
   // TF/SF faces of both halves for the step kernels and the buffer of their

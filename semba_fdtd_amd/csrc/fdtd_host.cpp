@@ -754,9 +754,26 @@ static int launch_step_kernels(fdtd_handle *h, int dir, bool exact_prev = false)
   for (size_t q = 0; q < h->step_cls.size(); ++q) {
     StepDev sd = class_sd(h, dir, q);
     sd.prev_from_field = exact_prev ? 0 : 1;
+    sd.lazy_prev = 1;
     const int e = launch_class(h, sd, q);
     if (e) return fail(h, FDTD_ERR_HIP, "step launch failed: %s", hipGetErrorString((hipError_t)e));
   }
+  return FDTD_OK;
+}
+
+// After steps that left the stored previous field of the pole classes
+// without sources to this pass: set it from the fields of the state (A), so
+// that every other reader (the two passes, checkpoints, the next plan, a
+// batch) finds it exact.  Called before any entry point returns.
+static int settle_prev(fdtd_handle *h) {
+  if (!h->prev_lazy) return FDTD_OK;
+  for (size_t q = 0; q < h->step_cls.size(); ++q) {
+    const int c = h->step_cls[q] >= kPlanHalf ? h->step_cls[q] - kPlanHalf : h->step_cls[q];
+    if (h->step_cls[q] == kPlanVec || !(c & 2) || (c & 4)) continue;
+    if (launch_prev_settle(h->g, h->dtiles + h->step_off[q], h->step_cnt[q], h->stream))
+      return fail(h, FDTD_ERR_HIP, "previous-field settling launch failed");
+  }
+  h->prev_lazy = false;
   return FDTD_OK;
 }
 
@@ -1765,6 +1782,10 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
   if (pairs > 0) h->prev_exact = false;
   for (int32_t s = s0; s < pairs; ++s) HIPCHK(h, hipGraphLaunch(h->pexec, h->stream));
   h->pbuffered += 2LL * pairs;
+  if (pairs > 0) {
+    h->prev_lazy = true;
+    if ((rc = settle_prev(h))) return rc;
+  }
   if (singles == 0) return FDTD_OK;
   if (!h->graph_ok) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
@@ -1868,6 +1889,8 @@ int fdtd_time_step(fdtd_handle *h, int32_t reps, float *ms) {
     for (int r = 0; r < reps && !rc; ++r) rc = pair();
     hipEventRecord(ev[1], h->stream);
   }
+  h->prev_lazy = true;
+  if (!rc) rc = settle_prev(h);
   if (!rc) {
     if (hipEventSynchronize(ev[1]) != hipSuccess) {
       rc = fail(h, FDTD_ERR_HIP, "timing events failed");
@@ -1910,7 +1933,11 @@ int fdtd_step_classes(fdtd_handle *h, int32_t reps, int32_t cls[FDTD_STEP_CLASSE
                          h->dstep, h->dinc, h->stream))
     rc = fail(h, FDTD_ERR_HIP, "incident-field launch failed");
   for (int q = 0; q < nc && !rc; ++q) {
-    const StepDev sd = class_sd(h, 0, (size_t)q);
+    // as in the step: tiles without sources leave the stored previous field
+    // of B, which nothing reads before it is written again
+    StepDev sd = class_sd(h, 0, (size_t)q);
+    sd.prev_from_field = 1;
+    sd.lazy_prev = 1;
     auto one = [&]() { return launch_class(h, sd, (size_t)q); };
     if (one()) { rc = fail(h, FDTD_ERR_HIP, "step launch failed"); break; }
     hipEventRecord(ev[0], h->stream);

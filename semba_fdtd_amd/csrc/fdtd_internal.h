@@ -275,6 +275,8 @@ struct StepDev {
   const StepTile *tiles;      // the tiles of this launch's class
   int prev_from_field;        // tiles without sources take the pole update's
                               // previous field from the field of step n
+  int lazy_prev;              // ... and then leave the stored previous field
+                              // to a settling pass (launch_prev_settle)
   int narrow;                 // media read from the 8-bit copies mi8
   const uint8_t *mi8[6];
   // TF/SF faces per half (staged in LDS by the blocks that meet them), and
@@ -399,6 +401,10 @@ int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3],
                     long long pxy, int xoff, int to_grid, void *stream);
 // the same box read out of a field array (es 4: fp32, es 2: binary16 times
 // scale) into a packed fp32 buffer
+// the stored previous field of every dispersive cell a step tile owns set
+// to the cell's field (the tiles of the pole classes without sources, after
+// steps that left it to this pass)
+int launch_prev_settle(const DevGrid &g, const StepTile *tiles, int n, void *stream);
 // the medium of each four-cell step tile whose cells (owned, halo row and
 // column, planes k0..k1) hold one medium in all six media arrays, else -1
 int launch_tile_media(const DevGrid &g, const StepTile *tiles, int n, int *out, void *stream);

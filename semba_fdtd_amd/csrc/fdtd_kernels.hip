@@ -1419,11 +1419,12 @@ __device__ __forceinline__ void pole_load(const AS &A, int c, long long zo, unsi
     if (q < A.maxp) pj[q] = (J + q * A.nstate[c])[axy];
 }
 // the new field of that edge (slot: its dispersive medium); the rest of its
-// poles are read here, the new state is written to ao when wr
+// poles are read here, the new state is written to ao when wr, the new
+// previous field when wr and wrp
 template <class AS>
 __device__ __forceinline__ float pole_new(const AS &ai, const AS &ao, int c,
-                                          long long zo, unsigned axy, bool wr, double dt,
-                                          const MedTab &t, int slot, float ep,
+                                          long long zo, unsigned axy, bool wr, bool wrp,
+                                          double dt, const MedTab &t, int slot, float ep,
                                           const jstate_t *pj, float e) {
   const long long N = ai.nstate[c];
   const jstate_t *Ji = reinterpret_cast<const jstate_t *>(ai.jst[c]) + zo;
@@ -1433,7 +1434,7 @@ __device__ __forceinline__ float pole_new(const AS &ai, const AS &ao, int c,
                [&](int q, jstate_t v) {
                  if (wr) (Jo + q * N)[axy] = v;
                });
-  if (wr) (ao.prev[c] + zo)[axy] = e;
+  if (wr && wrp) (ao.prev[c] + zo)[axy] = e;
   return e;
 }
 
@@ -1713,6 +1714,9 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
     long long z0 = 0, z1 = 0, z2 = 0;
     float ep0 = 0.f, ep1 = 0.f, ep2 = 0.f;
     jstate_t aj0[kPre], aj1[kPre], aj2[kPre];
+    // the stored previous field is written unless it is left to the settling
+    // pass (tiles without sources, whose next step takes it from the field)
+    bool wrp_e = true;
     if (ade_e && eplane) {
       const auto &A = a->ai[0];
       g0 = pole_plane(A, 0, k, z0) && ex_on && axe[0] != ~0u;
@@ -1722,6 +1726,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
       // on the edge after last step's pole update (no sources in the tile and
       // no fields set from outside since): then it is the field just loaded
       const bool pf = !SRC && sd.prev_from_field;
+      wrp_e = !(pf && sd.lazy_prev);
       if (g0) { pole_load(A, 0, z0, axe[0], ep0, aj0, !pf); if (pf) ep0 = ex0; }
       if (g1) { pole_load(A, 1, z1, axe[1], ep1, aj1, !pf); if (pf) ep1 = ey0; }
       if (g2) { pole_load(A, 2, z2, axe[2], ep2, aj2, !pf); if (pf) ep2 = ez0; }
@@ -1750,6 +1755,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
     bool u0 = false, u1 = false, u2 = false;
     long long w0 = 0, w1 = 0, w2 = 0;
     const bool hpf = !SRC && sd.prev_from_field;
+    const bool wrp_h = !(hpf && sd.lazy_prev);
     if (ade_h && hk) {
       const auto &A = a->ai[1];
       u0 = pole_plane(A, 0, kh, w0) && hxk && axh[0] != ~0u;
@@ -1804,7 +1810,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
         }
         if (g0) {
           const int s = te.slot[mx];
-          if (s >= 0) e = pole_new(a->ai[0], a->ao[0], 0, z0, axe[0], wr, a->dt, te, s, ep0, aj0, e);
+          if (s >= 0) e = pole_new(a->ai[0], a->ao[0], 0, z0, axe[0], wr, wrp_e, a->dt, te, s, ep0, aj0, e);
         }
         if (SRC) {
           if (be0) e = faces_apply(lf, be0, ie0, sd.inc, sd.pwG[0], i, j, k, e);
@@ -1828,7 +1834,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
         }
         if (g1) {
           const int s = te.slot[my];
-          if (s >= 0) e = pole_new(a->ai[0], a->ao[0], 1, z1, axe[1], wr, a->dt, te, s, ep1, aj1, e);
+          if (s >= 0) e = pole_new(a->ai[0], a->ao[0], 1, z1, axe[1], wr, wrp_e, a->dt, te, s, ep1, aj1, e);
         }
         if (SRC) {
           if (be1) e = faces_apply(lf, be1, ie1, sd.inc, sd.pwG[0], i, j, k, e);
@@ -1852,7 +1858,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
         }
         if (g2) {
           const int s = te.slot[mz];
-          if (s >= 0) e = pole_new(a->ai[0], a->ao[0], 2, z2, axe[2], wr, a->dt, te, s, ep2, aj2, e);
+          if (s >= 0) e = pole_new(a->ai[0], a->ao[0], 2, z2, axe[2], wr, wrp_e, a->dt, te, s, ep2, aj2, e);
         }
         if (SRC) {
           if (be2) e = faces_apply(lf, be2, ie2, sd.inc, sd.pwG[0], i, j, k, e);
@@ -1900,7 +1906,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
             float hp = hx_c;
             jstate_t bj[kPre];
             pole_load(a->ai[1], 0, w0, axh[0], hp, bj, !hpf);
-            h = pole_new(a->ai[1], a->ao[1], 0, w0, axh[0], true, a->dt, th, s, hp, bj, h);
+            h = pole_new(a->ai[1], a->ao[1], 0, w0, axh[0], true, wrp_h, a->dt, th, s, hp, bj, h);
           }
         }
         if (SRC) {
@@ -1929,7 +1935,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
             float hp = hy_c;
             jstate_t bj[kPre];
             pole_load(a->ai[1], 1, w1, axh[1], hp, bj, !hpf);
-            h = pole_new(a->ai[1], a->ao[1], 1, w1, axh[1], true, a->dt, th, s, hp, bj, h);
+            h = pole_new(a->ai[1], a->ao[1], 1, w1, axh[1], true, wrp_h, a->dt, th, s, hp, bj, h);
           }
         }
         if (SRC) {
@@ -1956,7 +1962,7 @@ __device__ __forceinline__ void step_tile(const StepDev &sd, const StepTile tl) 
             float hp = hz_c;
             jstate_t bj[kPre];
             pole_load(a->ai[1], 2, w2, axh[2], hp, bj, !hpf);
-            h = pole_new(a->ai[1], a->ao[1], 2, w2, axh[2], true, a->dt, th, s, hp, bj, h);
+            h = pole_new(a->ai[1], a->ao[1], 2, w2, axh[2], true, wrp_h, a->dt, th, s, hp, bj, h);
           }
         }
         if (SRC) {
