@@ -569,7 +569,8 @@ def main():
                                    "(no media indices read)")
         if dom not in ("step_plain", "step_plain4"):
             KERNELS[dom] = (f"k_step tile class {dom[5:]} ({cb} B per owned cell, 8 B per psi "
-                            f"value, 8 + 16 B per pole-state cell)")
+                            f"value, 16 B per stored pole per pole-state cell, + 8 B of "
+                            f"previous field in tiles with sources)")
         # the classes' own compulsory bytes (single-medium tiles read no media)
         step_bytes = int(sum(cbytes[c] for c, _, _ in timed_classes))
     else:

@@ -597,9 +597,10 @@ int build_step(fdtd_handle *h) {
     h->step_bytes.clear();
     // compulsory bytes of a class launch: each field read and written once
     // and each media index read once per owned cell, 8 B per psi value (read
-    // and write), per pole-state cell the previous field (8 B) and the stored
-    // poles (complex64 folded pairs in FAST, complex128 in EXACT; read and
-    // write)
+    // and write), per pole-state cell the stored poles (complex64 folded
+    // pairs in FAST, complex128 in EXACT; read and write) and, in tiles with
+    // sources, the previous field (8 B; the other tiles take it from the
+    // field and leave its store to the settling pass)
     const double cb = 48.0 + 6.0 * (narrow ? 1.0 : 2.0);
     const double per = h->mode == FDTD_MODE_EXACT ? 32.0 : 16.0;
     h->step_crows.clear();
@@ -610,9 +611,11 @@ int build_step(fdtd_handle *h) {
       h->step_cls.push_back(c);
       h->step_cells.push_back(plan.owned[c]);
       double by = cb * plan.owned[c] + 8.0 * plan.psi[c];
+      const int bits = c >= kPlanHalf ? c - kPlanHalf : c;
+      const double prevb = c != kPlanVec && (bits & 4) ? 8.0 : 0.0;
       for (int side = 0; side < 2; ++side)
         if (h->g.ade[side].has)
-          by += plan.disp[c][side] * (8.0 + per * std::max(h->g.ade[side].maxp, 1));
+          by += plan.disp[c][side] * (prevb + per * std::max(h->g.ade[side].maxp, 1));
       h->step_bytes.push_back(by);
       h->step_cnt.push_back((int)plan.tiles[c].size());
       h->step_off.push_back((int)all.size());
