@@ -196,9 +196,10 @@ int fdtd_get_step_mode(const fdtd_handle *h, int32_t *rows, int32_t *zchunk);
  * default.  Classes 9-16 run blocks of that many waves (any of the four
  * values up to the step's rows), each tile owning up to 2 rows - 1 rows.
  * Smaller blocks give a kernel that needs many registers per lane more of
- * them; by default the class with CPML, poles and sources runs 4-row blocks
- * under 16-row steps (8 waves on half-width tiles), every other class the
- * step's rows.  fdtd_get_step_class_rows gives the rows each class runs with.
+ * them; by default, under 16-row steps, the CPML-only class and the class
+ * with CPML, poles and sources run 4-row blocks, the half-width CPML +
+ * sources tiles 8 waves and the half-width CPML + poles + sources tiles 4
+ * (at most the step's rows), every other class the step's rows.  fdtd_get_step_class_rows gives the rows each class runs with.
  * The results do not change.  No counterpart in the reference; used by the
  * autotuner. */
 int fdtd_set_step_class_rows(fdtd_handle *h, int32_t classes, int32_t rows);

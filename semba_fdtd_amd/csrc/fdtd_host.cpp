@@ -422,12 +422,19 @@ int class_rows(const fdtd_handle *h, int c) {
   }
   if (c >= kPlanHalf) {
     // half-width tiles are banded on their own (2E - 1 rows for E waves), so
-    // any supported block fits; the CPML + poles + sources kernel needs more
-    // than the 128 registers a lane has in 16-wave blocks, 8 waves by default
-    const int q = h->step_creq[c] ? h->step_creq[c] : (c - kPlanHalf == 7 && R > 8 ? 8 : R);
+    // any supported block fits.  Defaults measured on the 1024^3 grid under
+    // 16-row steps: CPML + poles + sources (more than the 128 registers a
+    // lane has in 16-wave blocks) on 4 waves, CPML + sources on 8
+    const int b = c - kPlanHalf;
+    const int q = h->step_creq[c] ? h->step_creq[c]
+                  : b == 7          ? std::min(R, 4)
+                  : b == 5 && R > 8 ? 8
+                                    : R;
     return q <= R ? q : R;
   }
-  const int q = h->step_creq[c] ? h->step_creq[c] : (c == 7 && R == 16 ? 4 : R);
+  // CPML-only and CPML + poles + sources tiles in 4-row blocks under 16-row
+  // steps (measured the faster on the 1024^3 grid)
+  const int q = h->step_creq[c] ? h->step_creq[c] : ((c == 7 || c == 1) && R == 16 ? 4 : R);
   return q > 1 && q <= R && (R - 1) % (q - 1) == 0 ? q : R;
 }
 

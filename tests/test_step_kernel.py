@@ -224,7 +224,8 @@ def test_step_many_media(mode):
 def test_step_class_rows_equal_oracle(name):
     """Tile classes run in smaller blocks over row bands of the step's tiles
     (4 and 6 rows under 16-row tiles, every class or only the interior one),
-    and the half-width classes (9-16) on 4 and 8 waves: EXACT still equals the
+    and the half-width classes (9-16) on 4 and 8 waves, over the defaults
+    (some classes on fewer rows than the step's): EXACT still equals the
     oracle bit for bit, FAST the two-pass FAST run."""
     pb = CASES[name]()
     ns = pb.nsteps_max
@@ -238,7 +239,7 @@ def test_step_class_rows_equal_oracle(name):
                   flush=True)
             s = sf.Solver(pb, mode=mode)
             s.set_step_mode(16, 5)
-            default = [16] * 7 + [4, 16] + [16] * 7 + [8]
+            default = [16, 4] + [16] * 5 + [4, 16] + [16] * 5 + [8, 16, 4]
             assert s.get_step_class_rows() == default
             s.set_step_class_rows(rows, classes)
             assert s.get_step_class_rows() == [rows if classes >> c & 1 else default[c]
