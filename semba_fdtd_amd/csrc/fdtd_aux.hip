@@ -127,6 +127,54 @@ int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3],
   return (int)hipGetLastError();
 }
 
+// One block per tile of a batch of flat one-cell tiles: does every cell the
+// tile owns (its run of columns of the x-y plane, its z chunk) hold the same
+// medium in all six media arrays?  out[t]: that medium, else -1.
+__global__ void __launch_bounds__(256) k_batch_tile_media(BatchDev bd, int *out) {
+  const int t = blockIdx.x;
+  int lo = 0, hi = bd.ngrid - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bd.start[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  const DevGrid &g = bd.grids[lo];
+  const TileMap &tm = bd.tms[lo];
+  if (!tm.flat) {
+    if (threadIdx.x == 0) out[t] = -1;
+    return;
+  }
+  const int blk = t - bd.start[lo];
+  const int bx = blk % tm.tx, bz = blk / (tm.tx * tm.ty);
+  const long long cols = (long long)(g.nx + 1) * (g.ny + 1);
+  const long long f0 = (long long)bx * 64 * tm.bdy;
+  const long long f1 = f0 + 64LL * tm.bdy < cols ? f0 + 64LL * tm.bdy : cols;
+  const int k0 = bz * tm.kc, k1 = min(k0 + tm.kc, g.nz + 1);
+  const long long X = f1 - f0, Z = k1 - k0;
+  __shared__ int bad;
+  const int i0 = (int)(f0 % (g.nx + 1)), j0 = (int)(f0 / (g.nx + 1));
+  const int m0 = g.mi[0][fidx(g, i0, j0, k0)];
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  int b = 0;
+  for (long long q = threadIdx.x; q < X * Z && !b; q += blockDim.x) {
+    const long long f = f0 + q % X;
+    const int i = (int)(f % (g.nx + 1)), j = (int)(f / (g.nx + 1)), k = k0 + (int)(q / X);
+    const long long p = fidx(g, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) b |= g.mi[c][p] != m0;
+  }
+  if (b) atomicOr(&bad, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) out[t] = (bad || X <= 0 || Z <= 0) ? -1 : m0;
+}
+
+int launch_batch_tile_media(const BatchDev &bd, int *out, void *stream) {
+  if (bd.total <= 0) return 0;
+  hipLaunchKernelGGL(k_batch_tile_media, dim3((unsigned)bd.total), dim3(256), 0,
+                     (hipStream_t)stream, bd, out);
+  return (int)hipGetLastError();
+}
+
 // One block per step tile of a pole class without sources: the stored
 // previous field of every dispersive cell the tile owns (E and H, inside the
 // component's sweep and pole box) set to the cell's field, which is what the

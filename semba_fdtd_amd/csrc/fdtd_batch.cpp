@@ -226,6 +226,15 @@ int build_descriptors(fdtd_batch *b) {
   b->bd.start = ds;
   b->bd.ngrid = (int)grids.size();
   b->bd.total = start.back();
+  // flat tiles whose cells all hold one medium skip the media loads
+  b->bd.tmed = nullptr;
+  if (b->xcells == 1 && b->bd.total > 0) {
+    int *dm = nullptr;
+    BCHK(b, hipMalloc((void **)&dm, (size_t)b->bd.total * sizeof(int)));
+    b->allocs.push_back(dm);
+    BCHK(b, (hipError_t)launch_batch_tile_media(b->bd, dm, b->stream));
+    b->bd.tmed = dm;
+  }
   b->ncounters = (int)counters.size();
   b->lds = lds;
   // pole stage and probes: one launch each for the whole batch

@@ -298,6 +298,8 @@ struct BatchDev {
   const int *start;           // [ngrid+1] prefix of tile counts
   int ngrid;
   int total;
+  const int *tmed;            // per tile: the medium all its cells hold, or -1
+                              // (null: every tile loads its media)
 };
 
 #ifdef __HIPCC__
@@ -405,6 +407,9 @@ int launch_box_move(void *grid, void *packed, size_t es, const long long ext[3],
 // to the cell's field (the tiles of the pole classes without sources, after
 // steps that left it to this pass)
 int launch_prev_settle(const DevGrid &g, const StepTile *tiles, int n, void *stream);
+// the medium of each flat one-cell tile of a batch whose owned cells hold one
+// medium in all six media arrays, else -1 (out[t] for the batch's tile t)
+int launch_batch_tile_media(const BatchDev &bd, int *out, void *stream);
 // the medium of each four-cell step tile whose cells (owned, halo row and
 // column, planes k0..k1) hold one medium in all six media arrays, else -1
 int launch_tile_media(const DevGrid &g, const StepTile *tiles, int n, int *out, void *stream);

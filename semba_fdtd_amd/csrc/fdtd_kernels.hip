@@ -516,7 +516,7 @@ __device__ __forceinline__ void tile_column(const TileMap &tm, int blk, int nx, 
 // the x and y profiles fixed for the lane.
 template <bool YEE, bool CPML, bool TBL, class T = float, class G = DevGrid>
 __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
-                                           const MedTab &tab, int blk) {
+                                           const MedTab &tab, int blk, int m0 = -1) {
   typedef Store<T> S;
   typedef const SFD_GLOBAL float *cgf;
   constexpr unsigned ES = sizeof(T);
@@ -586,9 +586,10 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
     constexpr bool ALL = sizeof(T) == 2;
     int mx = 0, my = 0, mz = 0;
     typename S::raw_t rex = 0, rey = 0, rez = 0;
-    if (ALL || cx) { mx = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); rex = S::ldr(exk, of); }
-    if (ALL || cy) { my = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); rey = S::ldr(eyk, of); }
-    if (ALL || cz) { mz = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); rez = S::ldr(ezk, of); }
+    // m0 >= 0: every cell of the tile holds that medium (no media loads)
+    if (ALL || cx) { mx = m0 >= 0 ? m0 : ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); rex = S::ldr(exk, of); }
+    if (ALL || cy) { my = m0 >= 0 ? m0 : ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); rey = S::ldr(eyk, of); }
+    if (ALL || cz) { mz = m0 >= 0 ? m0 : ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); rez = S::ldr(ezk, of); }
     int qzn = 0;
     const bool szn = CPML && slab_node(k, nz, g.L[2][0], g.L[2][1], qzn);
     float bzp = 0.0f, ccz = 0.0f, vxy = 0.0f, vxz = 0.0f, vyz = 0.0f, vyx = 0.0f, vzy = 0.0f,
@@ -689,7 +690,7 @@ __device__ __forceinline__ void epass_tile(const G &g, const TileMap &tm,
 // one tile of the H face pass (addressing and loads as in the E pass)
 template <bool YEE, bool CPML, bool TBL, class T = float, class G = DevGrid>
 __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
-                                           const MedTab &tab, int blk) {
+                                           const MedTab &tab, int blk, int m0 = -1) {
   typedef Store<T> S;
   typedef const SFD_GLOBAL float *cgf;
   constexpr unsigned ES = sizeof(T);
@@ -754,9 +755,9 @@ __device__ __forceinline__ void hpass_tile(const G &g, const TileMap &tm,
     constexpr bool ALL = sizeof(T) == 2;
     int mx = 0, my = 0, mz = 0;
     typename S::raw_t rhx = 0, rhy = 0, rhz = 0;
-    if (ALL || ax) { mx = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); rhx = S::ldr(hxk, of); }
-    if (ALL || ay) { my = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); rhy = S::ldr(hyk, of); }
-    if (ALL || az) { mz = ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); rhz = S::ldr(hzk, of); }
+    if (ALL || ax) { mx = m0 >= 0 ? m0 : ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[0], kmb), om); rhx = S::ldr(hxk, of); }
+    if (ALL || ay) { my = m0 >= 0 ? m0 : ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[1], kmb), om); rhy = S::ldr(hyk, of); }
+    if (ALL || az) { mz = m0 >= 0 ? m0 : ld_at((const SFD_GLOBAL int16_t *)plane_base(bM[2], kmb), om); rhz = S::ldr(hzk, of); }
     int qzc = 0;
     const bool szc = CPML && k < nz && slab_cell(k, nz, g.L[2][0], g.L[2][1], qzc);
     float bzp = 0.0f, ccz = 0.0f, vxy = 0.0f, vxz = 0.0f, vyz = 0.0f, vyx = 0.0f, vzy = 0.0f,
@@ -1188,8 +1189,10 @@ __global__ void __launch_bounds__(256) k_pass_batch(BatchDev bd) {
       if (H) hpass4_tile<true, T>(g, tm, tab, blk);
       else epass4_tile<true, T>(g, tm, tab, blk);
     } else {
-      if (H) hpass_tile<true, true, true, T, CDevGrid>(cg, tm, tab, blk);
-      else epass_tile<true, true, true, T, CDevGrid>(cg, tm, tab, blk);
+      // a tile whose cells all hold one medium skips the media loads
+      const int m0 = bd.tmed ? __builtin_amdgcn_readfirstlane(bd.tmed[t]) : -1;
+      if (H) hpass_tile<true, true, true, T, CDevGrid>(cg, tm, tab, blk, m0);
+      else epass_tile<true, true, true, T, CDevGrid>(cg, tm, tab, blk, m0);
     }
   }
 }
