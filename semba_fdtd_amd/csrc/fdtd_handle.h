@@ -155,6 +155,11 @@ struct fdtd_handle {
   std::vector<int> step_cls, step_cnt, step_off;
   std::vector<int> step_crows;           // rows per block of each class launch
   std::vector<int> step_uni;             // leading single-medium tiles of each class launch
+  // per tile of the one-cell pass map tm: the medium filling it, or -1
+  // (valid while tmed_ready; tiling and media changes clear it)
+  int *tmed = nullptr;
+  int tmed_cap = 0;
+  bool tmed_ready = false;
   std::vector<double> step_cells;        // Yee cells owned per class
   std::vector<double> step_bytes;        // compulsory HBM bytes of one launch per class
   uint8_t *mi8[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // step mode
@@ -196,6 +201,7 @@ int refuse_batched(fdtd_handle *h);
 void alloc_extent(const fdtd_handle *h, int comp, long long ext[3]);
 int probe_room(fdtd_handle *h, long long more);
 int prepare(fdtd_handle *h);
+int build_tile_media(fdtd_handle *h);
 int run_pw(fdtd_handle *h, int half, int n, const int *dn);
 int run_nodal(fdtd_handle *h, int half, int n, const int *dn);
 int run_dft(fdtd_handle *h, int n, const int *dn);

@@ -194,6 +194,26 @@ void invalidate_graph(fdtd_handle *h) {
   h->pgraph = nullptr;
   h->graph_ok = false;
   h->step_ready = false;
+  h->tmed_ready = false;
+}
+
+// the medium of each tile of the one-cell pass map (or -1): the passes then
+// skip the media loads of single-medium tiles.  Built outside any capture,
+// before the passes it serves are launched or recorded.
+int build_tile_media(fdtd_handle *h) {
+  if (h->tmed_ready) return FDTD_OK;
+  if (h->tm.nblk > h->tmed_cap) {
+    if (h->tmed) hipFree(h->tmed);
+    h->tmed = nullptr;
+    h->tmed_cap = 0;
+    HIPCHK(h, hipMalloc(&h->tmed, (size_t)h->tm.nblk * sizeof(int)));
+    h->tmed_cap = h->tm.nblk;
+  }
+  const int e = h->mode == FDTD_MODE_EXACT ? exact::launch_pass_tile_media(h->g, h->tm, h->tmed, h->stream)
+                                           : fast::launch_pass_tile_media(h->g, h->tm, h->tmed, h->stream);
+  if (e) return fail(h, FDTD_ERR_HIP, "tile media launch failed: %s", hipGetErrorString((hipError_t)e));
+  h->tmed_ready = true;
+  return FDTD_OK;
 }
 
 // ---- compact pole state --------------------------------------------------
@@ -1137,8 +1157,9 @@ int run_e(fdtd_handle *h, int yee, int cpml, int ade) {
   // layers it updates no psi
   const int c = cpml && (h->g.has_cpml || yee);
   const int a = ade && h->g.ade[0].has;
-  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_e(h->g, h->tm, h->tm4, yee, c, a, h->stream)
-                                     : fast::launch_e(h->g, h->tm, h->tm4, yee, c, a, h->stream);
+  const int *tmed = h->tmed_ready ? h->tmed : nullptr;
+  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_e(h->g, h->tm, h->tm4, tmed, yee, c, a, h->stream)
+                                     : fast::launch_e(h->g, h->tm, h->tm4, tmed, yee, c, a, h->stream);
   if (e) return fail(h, FDTD_ERR_HIP, "E pass launch failed: %s", hipGetErrorString((hipError_t)e));
   return FDTD_OK;
 }
@@ -1148,8 +1169,9 @@ int run_h(fdtd_handle *h, int yee, int cpml, int ade) {
     return fail(h, FDTD_ERR_STATE, "binary16 storage runs the fused passes only (<= 4095 media)");
   const int c = cpml && (h->g.has_cpml || yee);
   const int a = ade && h->g.ade[1].has;
-  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_h(h->g, h->tm, h->tm4, yee, c, a, h->stream)
-                                     : fast::launch_h(h->g, h->tm, h->tm4, yee, c, a, h->stream);
+  const int *tmed = h->tmed_ready ? h->tmed : nullptr;
+  int e = h->mode == FDTD_MODE_EXACT ? exact::launch_h(h->g, h->tm, h->tm4, tmed, yee, c, a, h->stream)
+                                     : fast::launch_h(h->g, h->tm, h->tm4, tmed, yee, c, a, h->stream);
   if (e) return fail(h, FDTD_ERR_HIP, "H pass launch failed: %s", hipGetErrorString((hipError_t)e));
   return FDTD_OK;
 }
@@ -1305,6 +1327,7 @@ int fdtd_destroy(fdtd_handle *h) {
   for (void *p : h->dft_sums) hipFree(p);
   for (void *p : h->allocs) hipFree(p);
   if (h->stage) hipFree(h->stage);
+  if (h->tmed) hipFree(h->tmed);
   if (h->solo_stream) h->stream = h->solo_stream;
   if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -1557,6 +1580,7 @@ int fdtd_time_halves(fdtd_handle *h, int32_t reps, float ms[3]) {
   if (int rb = refuse_batched(h)) return rb;
   hipSetDevice(h->device);
   int rc = prepare(h);
+  if (!rc) rc = build_tile_media(h);
   if (rc) return rc;
   hipEvent_t ev[4];
   for (int i = 0; i < 4; ++i)
@@ -1707,7 +1731,7 @@ int fdtd_advance_e(fdtd_handle *h, int32_t n) {
   h->prev_exact = true;
   hipSetDevice(h->device);
   int rc;
-  if ((rc = prepare(h)) || (rc = run_e(h, 1, 1, 1)) || (rc = run_pw(h, 0, n, nullptr)) ||
+  if ((rc = prepare(h)) || (rc = build_tile_media(h)) || (rc = run_e(h, 1, 1, 1)) || (rc = run_pw(h, 0, n, nullptr)) ||
       (rc = run_nodal(h, 0, n, nullptr)))
     return rc;
   return FDTD_OK;
@@ -1719,7 +1743,7 @@ int fdtd_advance_h(fdtd_handle *h, int32_t n) {
   h->prev_exact = true;
   hipSetDevice(h->device);
   int rc;
-  if ((rc = prepare(h)) || (rc = run_h(h, 1, 1, 1)) || (rc = run_pw(h, 1, n, nullptr)) ||
+  if ((rc = prepare(h)) || (rc = build_tile_media(h)) || (rc = run_h(h, 1, 1, 1)) || (rc = run_pw(h, 1, n, nullptr)) ||
       (rc = run_nodal(h, 1, n, nullptr)) || (rc = run_mur(h)))
     return rc;
   return FDTD_OK;
@@ -1732,6 +1756,7 @@ int fdtd_advance_h(fdtd_handle *h, int32_t n) {
     h->prev_exact = true;                                                      \
     hipSetDevice(h->device);                                                   \
     int rc = prepare(h);                                                       \
+    if (!rc) rc = build_tile_media(h);                                         \
     if (rc) return rc;                                                         \
     return body;                                                               \
   }
@@ -1797,6 +1822,7 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
     if ((rc = settle_prev(h))) return rc;
   }
   if (singles == 0) return FDTD_OK;
+  if ((rc = build_tile_media(h))) return rc;
   if (!h->graph_ok) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->graph) hipGraphDestroy(h->graph);

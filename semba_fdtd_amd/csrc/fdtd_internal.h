@@ -362,13 +362,18 @@ __device__ __forceinline__ float nodal_apply(float v, const NodalCell &c, float 
 }
 #endif
 
-// launchers, one set per arithmetic mode
+// launchers, one set per arithmetic mode.  tmed (null: none) holds, per
+// tile of the one-cell map tm, the medium of every cell of the tile or -1,
+// as launch_pass_tile_media writes it; the one-cell passes skip the media
+// loads of the tiles it names.
 #define SFD_DECLARE_LAUNCHERS(NS)                                              \
   namespace NS {                                                             \
   int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4,      \
-               int yee, int cpml, int ade, void *stream);                    \
+               const int *tmed, int yee, int cpml, int ade, void *stream);   \
   int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4,      \
-               int yee, int cpml, int ade, void *stream);                    \
+               const int *tmed, int yee, int cpml, int ade, void *stream);   \
+  int launch_pass_tile_media(const DevGrid &g, const TileMap &tm, int *out,  \
+                             void *stream);                                  \
   size_t step_lds_bytes(const DevGrid &g, int rows, int npw, int kc);       \
   int launch_step(const StepDev &sd, int rows, int cls, int nuni, void *stream); \
   int launch_step_many(const StepDev *sds, const GridTile *tiles, int n, int narrow, int rows, \

@@ -1129,17 +1129,78 @@ __global__ void __launch_bounds__(256) k_hpass4(DevGrid g, TileMap tm) {
 }
 
 template <bool YEE, bool CPML, bool TBL, class T = float>
-__global__ void __launch_bounds__(256) k_epass(DevGrid g, TileMap tm) {
+__global__ void __launch_bounds__(256) k_epass(DevGrid g, TileMap tm, const int *tmed) {
   extern __shared__ float2 lds_tab[];
   const MedTab tab = stage_tables<TBL, false>(g, g.g1, g.g2, lds_tab);
-  epass_tile<YEE, CPML, TBL, T>(g, tm, tab, blockIdx.x);
+  const int m0 = tmed ? __builtin_amdgcn_readfirstlane(tmed[blockIdx.x]) : -1;
+  epass_tile<YEE, CPML, TBL, T>(g, tm, tab, blockIdx.x, m0);
 }
 
 template <bool YEE, bool CPML, bool TBL, class T = float>
-__global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm) {
+__global__ void __launch_bounds__(256) k_hpass(DevGrid g, TileMap tm, const int *tmed) {
   extern __shared__ float2 lds_tab[];
   const MedTab tab = stage_tables<TBL, false>(g, g.gm1, g.gm2, lds_tab);
-  hpass_tile<YEE, CPML, TBL, T>(g, tm, tab, blockIdx.x);
+  const int m0 = tmed ? __builtin_amdgcn_readfirstlane(tmed[blockIdx.x]) : -1;
+  hpass_tile<YEE, CPML, TBL, T>(g, tm, tab, blockIdx.x, m0);
+}
+
+// One block per tile of a one-cell pass map: the medium index that all six
+// media arrays hold on every cell the tile's lanes own (columns i <= nx,
+// j <= ny, the planes of its z chunk), or -1 where they differ.  The tiles
+// are decoded as the passes decode them.
+__global__ void __launch_bounds__(256) k_pass_tile_media(DevGrid g, TileMap tm, int *out) {
+  int bx, by, bz;
+  decode_tile(tm, blockIdx.x, bx, by, bz);
+  const int nx1 = g.nx + 1, ny1 = g.ny + 1;
+  const int k0 = bz * tm.kc, k1 = min(k0 + tm.kc, g.nz + 1);
+  long long c0 = 0, nc;
+  int i0 = 0, j0 = 0, wi = 1;
+  if (tm.flat) {
+    c0 = (long long)bx * kWave * tm.bdy;
+    const long long ce = min(c0 + (long long)kWave * tm.bdy, (long long)nx1 * ny1);
+    nc = ce - c0;
+  } else {
+    i0 = bx * kWave;
+    j0 = by * tm.bdy;
+    wi = min(kWave, nx1 - i0);
+    const int wj = min(tm.bdy, ny1 - j0);
+    nc = wi > 0 && wj > 0 ? (long long)wi * wj : 0;
+  }
+  if (nc <= 0 || k1 <= k0) {
+    if (threadIdx.x == 0) out[blockIdx.x] = -1;
+    return;
+  }
+  __shared__ int bad;
+  auto at = [&](int i, int j, int k) {
+    return (long long)(k + 1) * g.pxy + (long long)(j + 1) * g.px + g.xoff + i;
+  };
+  int ia = i0, ja = j0;
+  if (tm.flat) { ia = (int)(c0 % nx1); ja = (int)(c0 / nx1); }
+  const int m0 = g.mi[0][at(ia, ja, k0)];
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const long long n = nc * (k1 - k0);
+  int b = 0;
+  for (long long q = threadIdx.x; q < n && !b; q += blockDim.x) {
+    const long long col = q % nc;
+    const int k = k0 + (int)(q / nc);
+    int i, j;
+    if (tm.flat) { i = (int)((c0 + col) % nx1); j = (int)((c0 + col) / nx1); }
+    else { i = i0 + (int)(col % wi); j = j0 + (int)(col / wi); }
+    const long long p = at(i, j, k);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) b |= g.mi[c][p] != m0;
+  }
+  if (b) atomicOr(&bad, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = bad ? -1 : m0;
+}
+
+int launch_pass_tile_media(const DevGrid &g, const TileMap &tm, int *out, void *stream) {
+  if (tm.nblk <= 0) return 0;
+  hipLaunchKernelGGL(k_pass_tile_media, dim3((unsigned)tm.nblk), dim3(256), 0,
+                     (hipStream_t)stream, g, tm, out);
+  return (int)hipGetLastError();
 }
 
 // ---- batches of independent grids ----------------------------------------
@@ -1268,8 +1329,8 @@ int launch_ade_many(const DevGrid *grids, const int *list, int n, int side, size
   return (int)hipGetLastError();
 }
 
-int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, int cpml,
-             int ade, void *stream) {
+int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4, const int *tmed, int yee,
+             int cpml, int ade, void *stream) {
   const dim3 blk(kWave, (unsigned)tm.bdy);
   hipStream_t s = (hipStream_t)stream;
   const int nb = tm.nblk;
@@ -1278,9 +1339,9 @@ int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
 #define SFD_E(Y, C)                                                            \
   do {                                                                         \
     if (tbl)                                                                   \
-      hipLaunchKernelGGL((k_epass<Y, C, true>), dim3(nb), blk, lds, s, g, tm);  \
+      hipLaunchKernelGGL((k_epass<Y, C, true>), dim3(nb), blk, lds, s, g, tm, tmed); \
     else                                                                       \
-      hipLaunchKernelGGL((k_epass<Y, C, false>), dim3(nb), blk, 0, s, g, tm);   \
+      hipLaunchKernelGGL((k_epass<Y, C, false>), dim3(nb), blk, 0, s, g, tm, tmed); \
   } while (0)
   if (yee && cpml && tbl && tm4.vec == 4) {
     // fused pass, four cells per lane (the CPML variant also serves grids
@@ -1290,7 +1351,7 @@ int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
   } else if (g.fp16 && yee && cpml && tbl) {
     // binary16 storage, one cell per lane (the set-up admits binary16 only
     // with the fused Yee + CPML pass and LDS tables)
-    hipLaunchKernelGGL((k_epass<true, true, true, __half>), dim3(nb), blk, lds, s, g, tm);
+    hipLaunchKernelGGL((k_epass<true, true, true, __half>), dim3(nb), blk, lds, s, g, tm, tmed);
   } else if (yee && cpml) SFD_E(true, true);
   else if (yee) SFD_E(true, false);
   else if (cpml) SFD_E(false, true);
@@ -1299,8 +1360,8 @@ int launch_e(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
   return (int)hipGetLastError();
 }
 
-int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, int cpml,
-             int ade, void *stream) {
+int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, const int *tmed, int yee,
+             int cpml, int ade, void *stream) {
   const dim3 blk(kWave, (unsigned)tm.bdy);
   hipStream_t s = (hipStream_t)stream;
   const int nb = tm.nblk;
@@ -1309,15 +1370,15 @@ int launch_h(const DevGrid &g, const TileMap &tm, const TileMap &tm4, int yee, i
 #define SFD_H(Y, C)                                                            \
   do {                                                                         \
     if (tbl)                                                                   \
-      hipLaunchKernelGGL((k_hpass<Y, C, true>), dim3(nb), blk, lds, s, g, tm);  \
+      hipLaunchKernelGGL((k_hpass<Y, C, true>), dim3(nb), blk, lds, s, g, tm, tmed); \
     else                                                                       \
-      hipLaunchKernelGGL((k_hpass<Y, C, false>), dim3(nb), blk, 0, s, g, tm);   \
+      hipLaunchKernelGGL((k_hpass<Y, C, false>), dim3(nb), blk, 0, s, g, tm, tmed); \
   } while (0)
   if (yee && cpml && tbl && tm4.vec == 4) {
     if (g.fp16) hipLaunchKernelGGL(k_hpass4<__half>, dim3(tm4.nblk), blk, lds, s, g, tm4);
     else hipLaunchKernelGGL(k_hpass4<float>, dim3(tm4.nblk), blk, lds, s, g, tm4);
   } else if (g.fp16 && yee && cpml && tbl) {
-    hipLaunchKernelGGL((k_hpass<true, true, true, __half>), dim3(nb), blk, lds, s, g, tm);
+    hipLaunchKernelGGL((k_hpass<true, true, true, __half>), dim3(nb), blk, lds, s, g, tm, tmed);
   } else if (yee && cpml) SFD_H(true, true);
   else if (yee) SFD_H(true, false);
   else if (cpml) SFD_H(false, true);
