@@ -259,6 +259,33 @@ def test_pec_walls_device_equals_host():
         assert np.array_equal(f[sw] == 0.0, host[c][sw] == 0), c
 
 
+def test_media_written_between_runs():
+    """The one-cell passes read no media indices in tiles whose cells all
+    hold one medium, classified once per tiling and media layout.  Media
+    written between two runs are followed: the lossy medium painted over a
+    slab that was all vacuum (one whole tile of 4 rows x 8 planes among
+    them) gives the oracle's fields with the same change, bit for bit."""
+    print("media written between runs: scatterer 24x22x20, tiling 4x8x1", flush=True)
+    pb = P.scatterer_problem(nsteps=120, disp=False)
+    gpu = sf.Solver(pb, mode="exact")
+    gpu.set_step_mode(0)
+    gpu.set_tiling(4, 8, 1)
+    ref = oracle.Oracle(pb)
+    gpu.run(50)
+    ref.run(50)
+    # cells k 8..15, j 0..3, every i (Alloc index = cell index + 1)
+    before = ref.mi[:, 9:17, 1:5, 1:pb.nx + 2]
+    assert np.all(before == 1)
+    ref.mi[:, 9:17, 1:5, 1:pb.nx + 2] = 2
+    for c in range(6):
+        m = P.to_alloc_box(ref.mi[c], c, gpu.n3)
+        assert gpu.lib.fdtd_set_media_index(gpu.h, c, m.ctypes.data) == 0
+    gpu.run(50)
+    ref.run(50)
+    for c in range(6):
+        assert np.array_equal(gpu.field(c).view(np.uint32), ref.fields[c].view(np.uint32)), c
+
+
 @pytest.mark.parametrize("mode", ["fast", "exact"])
 def test_full_size_linearity(mode):
     """Full-size heterogeneous grid (1024^3, 12-cell CPML, 8 lossy spheres,
