@@ -277,9 +277,10 @@ def test_media_written_between_runs():
     before = ref.mi[:, 9:17, 1:5, 1:pb.nx + 2]
     assert np.all(before == 1)
     ref.mi[:, 9:17, 1:5, 1:pb.nx + 2] = 2
+    from semba_fdtd_amd._lib import i16p, ptr
     for c in range(6):
         m = P.to_alloc_box(ref.mi[c], c, gpu.n3)
-        assert gpu.lib.fdtd_set_media_index(gpu.h, c, m.ctypes.data) == 0
+        assert gpu.lib.fdtd_set_media_index(gpu.h, c, ptr(m, i16p)) == 0
     gpu.run(50)
     ref.run(50)
     for c in range(6):
