@@ -73,6 +73,7 @@ SIGNATURES = {
     "fdtd_get_tiling": (C.c_int, [vp, i32p, i32p, i32p]),
     "fdtd_time_halves": (C.c_int, [vp, C.c_int32, f32p]),
     "fdtd_set_step_mode": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "fdtd_set_graph_steps": (C.c_int, [vp, C.c_int32]),
     "fdtd_get_step_mode": (C.c_int, [vp, i32p, i32p]),
     "fdtd_set_step_class_rows": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "fdtd_get_step_class_rows": (C.c_int, [vp, i32p]),

@@ -132,6 +132,10 @@ struct fdtd_handle {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   bool graph_ok = false;
+  // the same two-pass step recorded graph_steps times in one graph
+  int graph_steps = 16;
+  hipGraphExec_t mexec = nullptr;
+  hipGraph_t mgraph = nullptr;
   // one-launch steps (k_step): a second copy of the time-stepping state (B),
   // the A->B and B->A descriptors and a captured pair of steps
   int step_rows = 0;              // 0: two passes per step; else rows per block (4, 8, 16)

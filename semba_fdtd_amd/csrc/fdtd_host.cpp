@@ -182,8 +182,16 @@ void set_tiles(fdtd_handle *h, int rows, int zchunk, int xcells) {
   }
 }
 
+static void drop_multi_graph(fdtd_handle *h) {
+  if (h->mexec) hipGraphExecDestroy(h->mexec);
+  if (h->mgraph) hipGraphDestroy(h->mgraph);
+  h->mexec = nullptr;
+  h->mgraph = nullptr;
+}
+
 void invalidate_graph(fdtd_handle *h) {
   ++h->version;
+  drop_multi_graph(h);
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   if (h->graph) hipGraphDestroy(h->graph);
   if (h->pexec) hipGraphExecDestroy(h->pexec);
@@ -1823,34 +1831,62 @@ int fdtd_run(fdtd_handle *h, int32_t n0, int32_t nsteps) {
   }
   if (singles == 0) return FDTD_OK;
   if ((rc = build_tile_media(h))) return rc;
-  if (!h->graph_ok) {
-    if (h->gexec) hipGraphExecDestroy(h->gexec);
-    if (h->graph) hipGraphDestroy(h->graph);
-    h->gexec = nullptr;
-    h->graph = nullptr;
+  // one two-pass step, reading the step number from the device counter and
+  // advancing it, recorded nrep times into one graph
+  auto capture = [h](int nrep, hipGraph_t *gout, hipGraphExec_t *xout) -> int {
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int r2 = run_e(h, 1, 1, 1);
-    if (!r2) r2 = run_pw(h, 0, 0, h->dstep);
-    if (!r2) r2 = run_nodal(h, 0, 0, h->dstep);
-    if (!r2) r2 = run_h(h, 1, 1, 1);
-    if (!r2) r2 = run_pw(h, 1, 0, h->dstep);
-    if (!r2) r2 = run_nodal(h, 1, 0, h->dstep);
-    if (!r2) r2 = run_mur(h);
-    if (!r2 && h->nprobe)
-      r2 = launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap, h->stream)
-               ? fail(h, FDTD_ERR_HIP, "probe launch failed") : 0;
-    if (!r2) r2 = run_dft(h, 0, h->dstep);
-    if (!r2) r2 = launch_tick(h->dstep, h->stream) ? fail(h, FDTD_ERR_HIP, "tick failed") : 0;
+    int r2 = 0;
+    for (int r = 0; r < nrep && !r2; ++r) {
+      r2 = run_e(h, 1, 1, 1);
+      if (!r2) r2 = run_pw(h, 0, 0, h->dstep);
+      if (!r2) r2 = run_nodal(h, 0, 0, h->dstep);
+      if (!r2) r2 = run_h(h, 1, 1, 1);
+      if (!r2) r2 = run_pw(h, 1, 0, h->dstep);
+      if (!r2) r2 = run_nodal(h, 1, 0, h->dstep);
+      if (!r2) r2 = run_mur(h);
+      if (!r2 && h->nprobe)
+        r2 = launch_probes(h->g, h->dprobes, h->nprobe, h->dpbuf, h->dpcount, h->pcap, h->stream)
+                 ? fail(h, FDTD_ERR_HIP, "probe launch failed") : 0;
+      if (!r2) r2 = run_dft(h, 0, h->dstep);
+      if (!r2) r2 = launch_tick(h->dstep, h->stream) ? fail(h, FDTD_ERR_HIP, "tick failed") : 0;
+    }
     hipGraph_t graph = nullptr;
     hipError_t ec = hipStreamEndCapture(h->stream, &graph);
     if (r2) { if (graph) hipGraphDestroy(graph); return r2; }
     if (ec != hipSuccess) return fail(h, FDTD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
-    h->graph = graph;
-    HIPCHK(h, hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0));
+    *gout = graph;
+    HIPCHK(h, hipGraphInstantiate(xout, graph, nullptr, nullptr, 0));
+    return FDTD_OK;
+  };
+  const int K = h->graph_steps;
+  int32_t left = singles;
+  if (K > 1 && left >= K) {
+    if (!h->mexec && (rc = capture(K, &h->mgraph, &h->mexec))) return rc;
+    for (; left >= K; left -= K) HIPCHK(h, hipGraphLaunch(h->mexec, h->stream));
+  }
+  if (left > 0 && !h->graph_ok) {
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    if ((rc = capture(1, &h->graph, &h->gexec))) return rc;
     h->graph_ok = true;
   }
-  for (int32_t s = 0; s < singles; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+  for (int32_t s = 0; s < left; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
   h->pbuffered += singles;
+  return FDTD_OK;
+}
+
+int fdtd_set_graph_steps(fdtd_handle *h, int32_t steps) {
+  if (!h) return FDTD_ERR_ARG;
+  if (steps < 1 || steps > 64) return fail(h, FDTD_ERR_ARG, "graph steps %d outside 1..64", steps);
+  if (int rb = refuse_batched(h)) return rb;
+  if (steps != h->graph_steps) {
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    drop_multi_graph(h);
+    h->graph_steps = steps;
+  }
   return FDTD_OK;
 }
 

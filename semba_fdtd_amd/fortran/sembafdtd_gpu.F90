@@ -47,7 +47,7 @@ module sembafdtd_gpu
              fdtd_batch_step, &
              fdtd_batch_synchronize, fdtd_set_step_mode, fdtd_get_step_mode, fdtd_time_step, &
              fdtd_step_classes, fdtd_set_step_class_rows, fdtd_get_step_class_rows, &
-             fdtd_set_step_vector, &
+             fdtd_set_step_vector, fdtd_set_graph_steps, &
              fdtd_error_message
 
    interface
@@ -392,6 +392,12 @@ module sembafdtd_gpu
          type(c_ptr), value :: h
          integer(c_int32_t), value :: reps
          real(c_float), intent(out) :: ms(3)
+      end function
+      ! ---- steps per graph replay of fdtd_run
+      integer(c_int) function fdtd_set_graph_steps(h, steps) bind(C, name="fdtd_set_graph_steps")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: steps
       end function
       ! ---- one launch per step in fdtd_run
       integer(c_int) function fdtd_set_step_mode(h, rows, zchunk) bind(C, name="fdtd_set_step_mode")

@@ -408,6 +408,10 @@ class Solver:
                  "fdtd_get_tiling")
         return r.value, k.value, x.value
 
+    def set_graph_steps(self, steps: int):
+        """Two-pass steps recorded per graph replay in run() (1..64)."""
+        self._ck(self.lib.fdtd_set_graph_steps(self.h, steps), "fdtd_set_graph_steps")
+
     def set_step_mode(self, rows: int, zchunk: int = 32):
         """One launch per step in run() (rows 4, 8 or 16 per block, 0: two
         passes per step).  Needs fp32 storage; holds a second copy of the

@@ -287,6 +287,27 @@ def test_media_written_between_runs():
         assert np.array_equal(gpu.field(c).view(np.uint32), ref.fields[c].view(np.uint32)), c
 
 
+@pytest.mark.parametrize("gsteps", [1, 7])
+def test_graph_steps_bit_identical(gsteps):
+    """fdtd_run replaying one or several recorded two-pass steps per graph
+    launch (the remainder one step per launch, over runs of uneven length;
+    the default of 16 runs in every other test) gives the oracle's probes and
+    fields bit for bit."""
+    print(f"graph steps {gsteps}: scatterer", flush=True)
+    pb, ns = _problem("scatterer")
+    ref = oracle.Oracle(pb)
+    want = ref.run(ns)
+    gpu = sf.Solver(pb, mode="exact")
+    gpu.set_step_mode(0)
+    gpu.set_graph_steps(gsteps)
+    for k in (5, 3 * gsteps + 2, ns - 5 - (3 * gsteps + 2)):
+        gpu.run(k)
+    got = gpu.probes()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), rel_l2(got, want)
+    for c in range(6):
+        assert np.array_equal(gpu.field(c).view(np.uint32), ref.fields[c].view(np.uint32)), c
+
+
 @pytest.mark.parametrize("mode", ["fast", "exact"])
 def test_full_size_linearity(mode):
     """Full-size heterogeneous grid (1024^3, 12-cell CPML, 8 lossy spheres,
