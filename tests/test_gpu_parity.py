@@ -83,7 +83,7 @@ def test_exact_mode_bit_identical(name):
     assert np.abs(want).max() > 0
 
 
-TILINGS = [(4, 32, 4), (1, 16, 4), (2, 7, 4), (2, 7, 1), (1, 64, 1)]
+TILINGS = [(4, 32, 4), (1, 16, 4), (2, 7, 4), (2, 7, 1), (1, 64, 1), (1, 4, 1), (2, 2, 1)]
 
 
 @pytest.mark.parametrize("name", list(CASES))
