@@ -158,7 +158,8 @@ int fdtd_add_planewave(fdtd_handle *h, const int32_t box[6], int32_t nmodes, con
  * 1, 2 or 4) that march zchunk z planes each, every lane owning xcells (1 or
  * 4) consecutive x cells of the fused passes.  No counterpart in the
  * reference (its loops are OpenMP over k); chosen by the autotuner
- * (semba_fdtd_amd.tuning) or left at the default 4 rows x 32 planes x 1 cell.
+ * (semba_fdtd_amd.tuning) or left at the default by grid depth: 2 rows x 4
+ * planes up to 160 planes (nz + 1), 4 x 8 up to 640, 4 x 16 beyond; 1 cell.
  * Either storage runs either.  fdtd_get_tiling returns the
  * values as set (a chunk longer than the grid runs as the whole grid). */
 int fdtd_set_tiling(fdtd_handle *h, int32_t rows, int32_t zchunk, int32_t xcells);

@@ -1311,9 +1311,13 @@ int fdtd_create(const fdtd_grid_desc *d, fdtd_handle **out) {
     fdtd_destroy(h);
     return rc;
   }
-  // tiles: 64 x 4 threads, z chunks of 32 planes (or fewer for thin grids),
-  // one x cell per lane
-  set_tiles(h, 4, 32, 1);
+  // tiles of 64 x rows threads, one x cell per lane; short z chunks give
+  // small grids enough blocks (64^3: 22 us per step with 4-plane chunks
+  // against 41 us with 16 and 70 us with 32; 256^3: 8 planes best)
+  const int planes = g.nz + 1;
+  if (planes <= 160) set_tiles(h, 2, 4, 1);
+  else if (planes <= 640) set_tiles(h, 4, 8, 1);
+  else set_tiles(h, 4, 16, 1);
   hipStreamSynchronize(h->stream);
   *out = h;
   return FDTD_OK;
